@@ -1,0 +1,207 @@
+"""Benchmark of the MI355X xPU path on its headline workload.
+
+Default (N=1): GpuPreAgg over a 100M-row columnar table resident in HBM
+(key int4 Zipf(1.1) over 10,000 values, v int8, w float8, x int4;
+GROUP BY key computing count(*), sum(v), avg(w), min(x), max(x)).  One step is
+one XpuTaskExec of the whole table through the session (the full hot path:
+column loads, group-by into kds_final).  Secondary lines: the 1M-row scan and
+the 200M-row star join + group-by, measured the same way.
+
+Prints one JSON line on stdout.  Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pg-strom_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK = 8.0e12   # MI355X HBM3E bytes/s
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def time_steps(run_step, warmup, steps):
+    from pgstrom import xpuhip
+    for _ in range(warmup):
+        run_step()
+    xpuhip.lib().xpuhip_device_synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run_step()
+        kms.append(xpuhip.last_kernel_ms())
+    xpuhip.lib().xpuhip_device_synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, float(np.mean(kms)), kms
+
+
+def bench_preagg(n, warmup, steps):
+    from pgstrom import datagen, kds, workloads as W, xpuhip
+    log("generating %d-row pre-aggregation table" % n)
+    cols = datagen.preagg_table(n)
+    src = kds.build_arrow_kds(cols)
+    blob = W.preagg_query(cols)
+    d_src = xpuhip.DeviceBuffer(len(src), src)
+    del src
+    s = xpuhip.Session(blob)
+
+    def step():
+        rc, res, _ = s.exec_device(d_src, n)
+        assert rc == 0
+
+    wall, kms, _ = time_steps(step, warmup, steps)
+    fin = s.final()
+    groups = W.decode_groups(fin, [a for a, _, _ in W.preagg_tlist(None, cols[0], None, None, None)])
+    total = sum(g[0] for g in groups.values())
+    if total != (warmup + steps) * n:
+        raise RuntimeError("count(*) over all groups is %d, expected %d" % (total, (warmup + steps) * n))
+    tuning = s.get_tuning()
+    s.close()
+    d_src.free()
+    algo_bytes = n * (4 + 8 + 8 + 4)
+    return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo_bytes,
+                groups=len(groups), tuning=tuning, cols=cols)
+
+
+def bench_scan(n, warmup, steps):
+    from pgstrom import datagen, kds, workloads as W, xpuhip
+    log("generating %d-row scan table" % n)
+    cols = datagen.scan_table(n)
+    src = kds.build_arrow_kds(cols)
+    blob = W.scan_query(cols)
+    d_src = xpuhip.DeviceBuffer(len(src), src)
+    dst_len = 64 << 20
+    d_dst = xpuhip.DeviceBuffer(dst_len)
+    s = xpuhip.Session(blob)
+    out = {}
+
+    def step():
+        rc, res, _ = s.exec_device(d_src, n, d_dst, dst_len)
+        assert rc == 0
+        out["nout"] = res.nitems_out
+
+    wall, kms, _ = time_steps(step, warmup, steps)
+    s.close()
+    hd = d_dst.to_host(256)
+    usage = int(np.frombuffer(hd[8:16].tobytes(), dtype=np.uint64)[0])
+    d_src.free()
+    d_dst.free()
+    # referenced columns (a, b, c, d offsets) + validity maps + text bytes read, output written
+    nout = out["nout"]
+    text_bytes = int(cols[3].values.offsets[-1])
+    algo = n * (4 + 8 + 8 + 4) + 4 * (n // 8) + text_bytes + usage + 8 * nout
+    return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo, nout=nout)
+
+
+def bench_star(n, warmup, steps):
+    from pgstrom import datagen, kds, workloads as W, xpuhip
+    log("generating %d-row star-join fact table and dimensions" % n)
+    fact, dims = datagen.star_tables(n)
+    km = W.star_inner(dims)
+    src = kds.build_arrow_kds(fact)
+    blob = W.star_query(fact)
+    d_src = xpuhip.DeviceBuffer(len(src), src)
+    del src
+    s = xpuhip.Session(blob)
+    s.attach_inner(km)
+
+    def step():
+        rc, res, _ = s.exec_device(d_src, n)
+        assert rc == 0
+
+    wall, kms, _ = time_steps(step, warmup, steps)
+    s.close()
+    d_src.free()
+    algo = n * (4 * 3 + 8 + 8)
+    return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo)
+
+
+def cpu_baseline(cols_fn, sample_rows):
+    """The oracle (own CPU restatement, multi-threaded) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from pgstrom import datagen, kds, workloads as W
+    cols = datagen.preagg_table(sample_rows, fragment=1)
+    src = kds.build_arrow_kds(cols)
+    blob = W.preagg_query(cols)
+    ncores = min(os.cpu_count() or 1, 16)
+    o = oracle.OracleSession(blob)
+    t0 = time.perf_counter()
+    rc, _, _ = o.exec(src, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    o.close()
+    assert rc == 0
+    return dict(value=sample_rows / dt, unit="rows/s", cores=ncores, kind="port",
+                sample="%d rows of the same pre-aggregation table and query, %.2f s" % (sample_rows, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=100_000_000)
+    ap.add_argument("--only", default="all", choices=["all", "preagg"])
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not run)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from pgstrom import xpuhip
+    xpuhip.init(0)
+
+    pa = bench_preagg(args.rows, args.warmup, args.steps)
+    kms = pa["kernel_ms"]
+    achieved = pa["algo_bytes"] / (kms / 1e3)
+    others = {}
+    if args.only == "all":
+        sc = bench_scan(1_000_000, 3, 20)
+        others["scan_1M"] = dict(rows_per_s=sc["rows"] / (sc["kernel_ms"] / 1e3),
+                                 kernel_ms=sc["kernel_ms"], wall_ms=sc["wall_ms"],
+                                 hbm_gbs=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / 1e9,
+                                 frac=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / HBM_PEAK,
+                                 rows_out=sc["nout"])
+        st = bench_star(200_000_000, 1, 5)
+        others["star_join_200M"] = dict(rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3),
+                                        kernel_ms=st["kernel_ms"], wall_ms=st["wall_ms"],
+                                        hbm_gbs=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
+                                        frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK)
+    log("cpu baseline")
+    cpu = cpu_baseline(None, args.cpu_sample)
+
+    line = {
+        "metric": "rows/s & %HBM-peak per scan/join/preagg; p99 latency, batched vs plain",
+        "value": pa["rows"] / (pa["wall_ms"] / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": pa["wall_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64+f64",
+        "data": "synthetic (seeded numpy PCG64), resident in HBM",
+        "config": {"workload": "GpuPreAgg 100M rows: key int4 Zipf(1.1) over 10,000, v int8, "
+                               "w float8, x int4; GROUP BY key count(*), sum(v), avg(w), min(x), max(x)",
+                   "rows": pa["rows"], "groups": pa["groups"], "tuning": pa["tuning"]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel_ms": kms, "algo_bytes_per_launch": pa["algo_bytes"]},
+        "cpu_baseline": cpu,
+        "workloads": others,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

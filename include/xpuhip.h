@@ -1,0 +1,113 @@
+/*
+ * xpuhip.h - C ABI of the MI355X xPU execution library (libxpuhip.so).
+ *
+ * These entry points replace the device half of PG-Strom's GPU service: what
+ * gpu_service.c does between receiving an XpuCommand and writing the response
+ * back (session open with kernel-pointer resolution, inner-buffer import,
+ * per-chunk kernel launch with suspend/resume, final-buffer write-back).  The
+ * service loop (socket, command queue, worker threads) calls them; all
+ * arguments are plain pointers and sizes in the reference's own wire formats
+ * (include/pgstrom_xpu.h).  Return values are the reference's ERRCODE_*:
+ *   0    success
+ *   'd'  ERRCODE_SUSPEND_NO_SPACE: kds_dst is full; the rows written so far are
+ *        valid; call again with *resume_pos unchanged to continue the chunk
+ *        (the reference resumes the kernel, gpu_service.c:3896-3908)
+ *   'E'  ERRCODE_DEVICE_ERROR, with *errbuf filled
+ * The library never falls back to the CPU: without a usable GPU every entry
+ * point fails with 'E'.
+ */
+#ifndef XPUHIP_H
+#define XPUHIP_H
+#include "pgstrom_xpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xpuhip_session xpuhip_session;
+
+/* Tuning of the pipeline launch; zero fields keep the library's choice
+ * (chosen per task kind by the autotuner, see xpuhip_autotune). */
+typedef struct
+{
+    uint32_t    block_size;         /* threads per block (64..512, multiple of 64) */
+    uint32_t    rows_per_thread;    /* group-by rows per thread per tile */
+    uint32_t    lds_nents;          /* group-by LDS table entries (power of 2) */
+    uint32_t    blocks_per_cu;      /* persistent blocks per compute unit */
+} xpuhip_tuning;
+
+/* Device setup; replaces gpuservSetupGpuContext (gpu_service.c:6462-6523).
+ * device < 0 selects device 0. */
+int     xpuhip_init(int device, kern_errorbuf *errbuf);
+void    xpuhip_shutdown(void);
+
+/* OpenSession: validates and compiles the kern_session_info blob built by
+ * pgstromBuildSessionInfo (executor.c:517-801), and allocates the session's
+ * kds_final when it carries a group-by.  Replaces gpuservHandleOpenSession
+ * (gpu_service.c:3319-3410) and its device-pointer patching (:3156-3317). */
+int     xpuhip_session_open(const kern_session_info *session, size_t length,
+                            xpuhip_session **p_sess, kern_errorbuf *errbuf);
+
+/* Validation and compilation only, without a GPU (for planners and tests):
+ * returns 0 if the session would open. */
+int     xpuhip_session_check(const kern_session_info *session, size_t length,
+                             kern_errorbuf *errbuf);
+
+/* Imports the inner relations (kern_multirels with KDS_FORMAT_HASH chunks,
+ * gpu_join.c:2128-2164) into device memory for the session's hash joins;
+ * replaces the shm import of gpu_service.c:2197-2354. */
+int     xpuhip_session_attach_inner(xpuhip_session *sess, const kern_multirels *kmrels,
+                                    size_t length, kern_errorbuf *errbuf);
+
+/* XpuTaskExec on a host-resident chunk (KDS_FORMAT_ARROW): copies it to the
+ * device, runs the pipeline, and for a scan/join projection copies the rows
+ * into kds_dst (KDS_FORMAT_ROW, dst_length bytes, header from the session).
+ * *resume_pos must be 0 for a new chunk.  Replaces gpuservHandleGpuTaskExec
+ * (gpu_service.c:4005-4290) with __gpuservLaunchGpuTaskExecKernel (:3759-4003). */
+int     xpuhip_task_exec(xpuhip_session *sess,
+                         const kern_data_store *kds_src, size_t src_length,
+                         kern_data_store *kds_dst, size_t dst_length,
+                         uint64_t *resume_pos,
+                         kern_exec_results *results, kern_errorbuf *errbuf);
+
+/* The same on a chunk already in device memory: d_kds_src is a device
+ * address of a KDS_FORMAT_ARROW chunk of src_nitems rows; a projection writes
+ * into the device buffer d_kds_dst of dst_length bytes. */
+int     xpuhip_task_exec_device(xpuhip_session *sess,
+                                const kern_data_store *d_kds_src, uint32_t src_nitems,
+                                kern_data_store *d_kds_dst, size_t dst_length,
+                                uint64_t *resume_pos,
+                                kern_exec_results *results, kern_errorbuf *errbuf);
+
+/* XpuTaskFinal of a group-by session: writes the session's kds_final into
+ * kds_final (host memory) as the service writes it back, HASH converted to
+ * ROW (gpu_service.c:2780-2818).  With kds_final == NULL returns the number of
+ * bytes needed.  Returns the bytes written, or -1 with errbuf filled. */
+int64_t xpuhip_task_final(xpuhip_session *sess, kern_data_store *kds_final, size_t length,
+                          kern_exec_results *results, kern_errorbuf *errbuf);
+
+/* Releases the session and its device memory (gpuservCloseSession). */
+void    xpuhip_session_close(xpuhip_session *sess);
+
+/* Launch tuning of one session (see xpuhip_tuning). */
+int     xpuhip_session_set_tuning(xpuhip_session *sess, const xpuhip_tuning *tuning);
+int     xpuhip_session_get_tuning(xpuhip_session *sess, xpuhip_tuning *tuning);
+
+/* Device memory helpers for callers that keep chunks resident in HBM. */
+int     xpuhip_device_alloc(size_t length, void **p_dptr, kern_errorbuf *errbuf);
+int     xpuhip_device_free(void *dptr);
+int     xpuhip_memcpy_htod(void *dptr, const void *src, size_t length);
+int     xpuhip_memcpy_dtoh(void *dst, const void *dptr, size_t length);
+int     xpuhip_device_synchronize(void);
+
+/* Kernel time of the last task executed by the calling thread, in
+ * milliseconds, measured with HIP events on the stream it ran on. */
+double  xpuhip_last_kernel_ms(void);
+
+/* Layout of this library's ABI structs as JSON (for conformance tests). */
+const char *xpuhip_abi_layout_json(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XPUHIP_H */

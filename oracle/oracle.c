@@ -1,0 +1,1915 @@
+/*
+ * oracle.c - CPU restatement of PG-Strom's xPU scan / join / pre-aggregation
+ * semantics.  TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline);
+ * never linked into the product.
+ *
+ * It is a direct tree-walking interpreter of the kern_expression bytecode,
+ * deliberately shaped unlike the GPU path (which compiles the bytecode into
+ * flat register programs).  Each piece follows the reference:
+ *   expression VM        xpu_common.cu:390-551 (Const/Param/Var/Bool/NullTest),
+ *                        :1363-1398 (HashValue), :1400-1425 (SaveExpr),
+ *                        :1442-1481 (JoinQuals statuses)
+ *   datum operators      xpu_basetype.cu:325-772 (casts, NaN-aware compares,
+ *                        overflow-checked arithmetic), xpu_textlib.cu:321-380
+ *   hash                 xpu_common.cu:2920-3041 (lookup3, PG seed)
+ *   column loads         xpu_basetype.cu:242-306 (Arrow fixed width),
+ *                        xpu_textlib.cu:237-274 (Arrow utf8/binary),
+ *                        xpu_common.cu:60-130 (minimal-tuple deform)
+ *   hash join            cuda_gpujoin.cu:304-494 (chain walk, hash compare,
+ *                        join quals, left outer null-extension)
+ *   projection           xpu_common.cu:1096-1190 (kern_form_minimal_tuple)
+ *   group-by             cuda_gpupreagg.cu:18-223 (state init/layout),
+ *                        :1196-1605 (state updates), :1607-1731 (lookup)
+ *   final write-back     gpu_service.c:2780-2818 (HASH -> ROW)
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../pg-strom_amd/csrc/xpu_opcodes_table.h"
+
+#define OMAX_SLOTS  256
+#define OMAX_DEPTH  16
+#define OMAX_ATTRS  64
+
+typedef __int128 int128_t;
+
+/* ------------------------------------------------------------------
+ * lookup3 hash with PostgreSQL's seed (xpu_common.cu:2920-3041);
+ * Bob Jenkins' published mix/final functions.
+ * ------------------------------------------------------------------ */
+#define ROT(x, k) (((x) << (k)) | ((x) >> (32 - (k))))
+#define MIX(a, b, c) do { \
+    a -= c; a ^= ROT(c, 4);  c += b; \
+    b -= a; b ^= ROT(a, 6);  a += c; \
+    c -= b; c ^= ROT(b, 8);  b += a; \
+    a -= c; a ^= ROT(c, 16); c += b; \
+    b -= a; b ^= ROT(a, 19); a += c; \
+    c -= b; c ^= ROT(b, 4);  b += a; } while (0)
+#define FINAL(a, b, c) do { \
+    c ^= b; c -= ROT(b, 14); \
+    a ^= c; a -= ROT(c, 11); \
+    b ^= a; b -= ROT(a, 25); \
+    c ^= b; c -= ROT(b, 16); \
+    a ^= c; a -= ROT(c, 4);  \
+    b ^= a; b -= ROT(a, 14); \
+    c ^= b; c -= ROT(b, 24); } while (0)
+
+uint32_t
+oracle_hash_any(const void *ptr, int sz)
+{
+    const uint8_t *k = (const uint8_t *)ptr;
+    uint32_t a, b, c, len = (uint32_t)sz;
+
+    a = b = c = 0x9e3779b9U + len + 3923095U;
+    /* little-endian byte assembly is identical for aligned and unaligned
+     * inputs, so one loop covers both paths of the reference */
+    while (len >= 12)
+    {
+        a += (uint32_t)k[0] | ((uint32_t)k[1] << 8) | ((uint32_t)k[2] << 16) | ((uint32_t)k[3] << 24);
+        b += (uint32_t)k[4] | ((uint32_t)k[5] << 8) | ((uint32_t)k[6] << 16) | ((uint32_t)k[7] << 24);
+        c += (uint32_t)k[8] | ((uint32_t)k[9] << 8) | ((uint32_t)k[10] << 16) | ((uint32_t)k[11] << 24);
+        MIX(a, b, c);
+        k += 12;
+        len -= 12;
+    }
+    switch (len)
+    {
+        case 11: c += (uint32_t)k[10] << 24;    /* fall through */
+        case 10: c += (uint32_t)k[9] << 16;     /* fall through */
+        case 9:  c += (uint32_t)k[8] << 8;      /* fall through */
+        case 8:  b += (uint32_t)k[7] << 24;     /* fall through */
+        case 7:  b += (uint32_t)k[6] << 16;     /* fall through */
+        case 6:  b += (uint32_t)k[5] << 8;      /* fall through */
+        case 5:  b += k[4];                     /* fall through */
+        case 4:  a += (uint32_t)k[3] << 24;     /* fall through */
+        case 3:  a += (uint32_t)k[2] << 16;     /* fall through */
+        case 2:  a += (uint32_t)k[1] << 8;      /* fall through */
+        case 1:  a += k[0];
+    }
+    FINAL(a, b, c);
+    return c;
+}
+
+/* ------------------------------------------------------------------
+ * function table
+ * ------------------------------------------------------------------ */
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen };
+enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
+       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len };
+
+typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, res; } ofunc;
+
+#define T_Invalid   TypeOpCode__Invalid
+#define T_bool      TypeOpCode__bool
+#define T_int1      TypeOpCode__int1
+#define T_int2      TypeOpCode__int2
+#define T_int4      TypeOpCode__int4
+#define T_int8      TypeOpCode__int8
+#define T_float4    TypeOpCode__float4
+#define T_float8    TypeOpCode__float8
+#define T_text      TypeOpCode__text
+#define OFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) \
+    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##RES },
+static const ofunc ofunc_table[] = { XPU_FUNCTION_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0 } };
+
+static const ofunc *
+lookup_func(int opcode)
+{
+    for (const ofunc *f = ofunc_table; f->name; f++)
+        if (f->opcode == opcode)
+            return f;
+    return NULL;
+}
+
+static bool is_int_type(int t) { return t == T_int1 || t == T_int2 || t == T_int4 || t == T_int8 || t == TypeOpCode__date; }
+static bool is_flt_type(int t) { return t == T_float4 || t == T_float8; }
+static int  type_width(int t)
+{
+    switch (t) {
+        case T_bool: case T_int1: return 1;
+        case T_int2: return 2;
+        case T_int4: case T_float4: case TypeOpCode__date: return 4;
+        case T_int8: case T_float8: return 8;
+        default: return -1;
+    }
+}
+static void int_range(int t, int64_t *lo, int64_t *hi)
+{
+    switch (t) {
+        case T_int1: *lo = -128; *hi = 127; break;
+        case T_int2: *lo = -32768; *hi = 32767; break;
+        case T_int4: case TypeOpCode__date: *lo = INT32_MIN; *hi = INT32_MAX; break;
+        default: *lo = INT64_MIN; *hi = INT64_MAX; break;
+    }
+}
+
+/* ------------------------------------------------------------------
+ * datum and context
+ * ------------------------------------------------------------------ */
+typedef struct
+{
+    bool        isnull;
+    int64_t     i;          /* bool / ints / date */
+    double      d;          /* float8 */
+    float       f;          /* float4 */
+    const char *p;          /* text payload (no header) */
+    int32_t     len;
+} od;
+
+typedef struct
+{
+    const kern_session_info *session;
+    const kern_varslot_desc *vdesc;
+    int         nslots;
+    od          slots[OMAX_SLOTS];
+    bool        compare_nulls;
+    uint32_t    errcode;
+    char        errmsg[KERN_ERRORBUF_MESSAGE_LEN + 1];
+} octx;
+
+static bool
+oerror(octx *c, uint32_t code, const char *fmt, const char *arg)
+{
+    if (c->errcode == 0 || (ERRCODE_IS_SUSPEND(c->errcode) && !ERRCODE_IS_SUSPEND(code)))
+    {
+        c->errcode = code;
+        snprintf(c->errmsg, sizeof(c->errmsg), fmt, arg);
+    }
+    return false;
+}
+
+#define KEXP_FIRST_ARG(k) \
+    (((k)->nr_args > 0 && (k)->args_offset > 0) ? \
+     (const kern_expression *)((const char *)(k) + (k)->args_offset) : NULL)
+#define KEXP_NEXT_ARG(a) \
+    ((const kern_expression *)((const char *)(a) + XPU_MAXALIGN((a)->len)))
+
+static bool eval(octx *c, const kern_expression *k, od *r);
+
+/* datum from a fixed-width / varlena image in the PostgreSQL format */
+static bool
+datum_from_image(octx *c, int type, const char *addr, od *r)
+{
+    memset(r, 0, sizeof(od));
+    switch (type)
+    {
+        case T_bool: r->i = *(const uint8_t *)addr ? 1 : 0; break;
+        case T_int1: r->i = *(const int8_t *)addr; break;
+        case T_int2: { int16_t v; memcpy(&v, addr, 2); r->i = v; } break;
+        case T_int4: case TypeOpCode__date: { int32_t v; memcpy(&v, addr, 4); r->i = v; } break;
+        case T_int8: { int64_t v; memcpy(&v, addr, 8); r->i = v; } break;
+        case T_float4: memcpy(&r->f, addr, 4); break;
+        case T_float8: memcpy(&r->d, addr, 8); break;
+        case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+        {
+            uint8_t b0 = (uint8_t)addr[0];
+            if (b0 == 0x01 || (b0 & 0x03) == 0x02)
+                return oerror(c, ERRCODE_SUSPEND_FALLBACK,
+                              "%s: text datum is compressed or external", "oracle");
+            if (b0 & 0x01)
+            {
+                r->len = ((b0 >> 1) & 0x7f) - 1;
+                r->p = addr + 1;
+            }
+            else
+            {
+                uint32_t h; memcpy(&h, addr, 4);
+                r->len = (int32_t)((h >> 2) & 0x3fffffff) - 4;
+                r->p = addr + 4;
+            }
+            break;
+        }
+        default:
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: unsupported type", "oracle");
+    }
+    return true;
+}
+
+/* size and bytes of a datum written into a tuple (xpu_*_datum_write) */
+static int
+datum_write(int type, const od *v, char *buf)
+{
+    switch (type)
+    {
+        case T_bool: if (buf) *(uint8_t *)buf = (uint8_t)(v->i != 0); return 1;
+        case T_int1: if (buf) *(int8_t *)buf = (int8_t)v->i; return 1;
+        case T_int2: if (buf) { int16_t x = (int16_t)v->i; memcpy(buf, &x, 2); } return 2;
+        case T_int4: case TypeOpCode__date:
+            if (buf) { int32_t x = (int32_t)v->i; memcpy(buf, &x, 4); } return 4;
+        case T_int8: if (buf) memcpy(buf, &v->i, 8); return 8;
+        case T_float4: if (buf) memcpy(buf, &v->f, 4); return 4;
+        case T_float8: if (buf) memcpy(buf, &v->d, 8); return 8;
+        case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+            if (buf)
+            {
+                uint32_t h = (uint32_t)(v->len + 4) << 2;
+                memcpy(buf, &h, 4);
+                memcpy(buf + 4, v->p, v->len);
+            }
+            return 4 + v->len;
+    }
+    return -1;
+}
+
+static uint32_t
+datum_hash(int type, const od *v)
+{
+    switch (type)
+    {
+        case T_bool: { uint8_t b = v->i != 0; return oracle_hash_any(&b, 1); }
+        case T_int1: { int8_t x = (int8_t)v->i; return oracle_hash_any(&x, 1); }
+        case T_int2: { int16_t x = (int16_t)v->i; return oracle_hash_any(&x, 2); }
+        case T_int4: case TypeOpCode__date: { int32_t x = (int32_t)v->i; return oracle_hash_any(&x, 4); }
+        case T_int8: return oracle_hash_any(&v->i, 8);
+        case T_float4: return oracle_hash_any(&v->f, 4);
+        case T_float8: return oracle_hash_any(&v->d, 8);
+        case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+            return oracle_hash_any(v->p, v->len);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------
+ * operators
+ * ------------------------------------------------------------------ */
+static double as_double(int t, const od *v)
+{
+    if (t == T_float8) return v->d;
+    if (t == T_float4) return (double)v->f;
+    return (double)v->i;
+}
+
+static bool
+float_cmp(int op, double x, double y)
+{
+    /* NaN sorts above every other value (xpu_basetype.cu:467-478) */
+    switch (op)
+    {
+        case O_eq: return isnan(x) ? isnan(y) : (!isnan(y) && x == y);
+        case O_ne: return isnan(x) ? !isnan(y) : (isnan(y) || x != y);
+        case O_lt: return !isnan(x) && (isnan(y) || x < y);
+        case O_le: return isnan(y) || (!isnan(x) && x <= y);
+        case O_gt: return !isnan(y) && (isnan(x) || x > y);
+        case O_ge: return isnan(x) || (!isnan(y) && x >= y);
+    }
+    return false;
+}
+
+static bool
+cmp_result(int op, int comp)
+{
+    switch (op)
+    {
+        case O_eq: return comp == 0;
+        case O_ne: return comp != 0;
+        case O_lt: return comp < 0;
+        case O_le: return comp <= 0;
+        case O_gt: return comp > 0;
+        case O_ge: return comp >= 0;
+    }
+    return false;
+}
+
+static bool
+eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
+{
+    od a[2];
+    const kern_expression *karg = KEXP_FIRST_ARG(k);
+
+    memset(r, 0, sizeof(od));
+    if ((int)k->nr_args != f->nargs)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: wrong number of arguments", f->name);
+    for (int i = 0; i < f->nargs; i++, karg = KEXP_NEXT_ARG(karg))
+        if (!eval(c, karg, &a[i]))
+            return false;
+    if (a[0].isnull || (f->nargs > 1 && a[1].isnull))
+    {
+        /* NULL input: comparisons honour kmode_compare_nulls for eq/ne
+         * (xpu_basetype.h:30-60); everything else yields NULL */
+        if ((f->kind == K_cmp || f->kind == K_textcmp) && c->compare_nulls &&
+            (f->op == O_eq || f->op == O_ne))
+        {
+            bool both = a[0].isnull && a[1].isnull;
+            r->i = (f->op == O_eq) ? both : !both;
+            return true;
+        }
+        r->isnull = true;
+        return true;
+    }
+    switch (f->kind)
+    {
+        case K_cmp:
+            if (is_flt_type(f->t0) || is_flt_type(f->t1))
+            {
+                if (f->t0 == T_float4 && f->t1 == T_float4)
+                    r->i = float_cmp(f->op, (double)a[0].f, (double)a[1].f);
+                else
+                    r->i = float_cmp(f->op, as_double(f->t0, &a[0]), as_double(f->t1, &a[1]));
+            }
+            else
+                r->i = cmp_result(f->op, a[0].i < a[1].i ? -1 : a[0].i > a[1].i ? 1 : 0);
+            return true;
+        case K_textcmp:
+        {
+            int n = a[0].len < a[1].len ? a[0].len : a[1].len;
+            int comp = memcmp(a[0].p, a[1].p, n);
+            if (comp == 0)
+                comp = a[0].len < a[1].len ? -1 : a[0].len > a[1].len ? 1 : 0;
+            r->i = cmp_result(f->op, comp);
+            return true;
+        }
+        case K_textlen:
+            r->i = a[0].len;
+            return true;
+        case K_arith:
+            if (is_int_type(f->res))
+            {
+                int64_t lo, hi;
+                int128_t x = a[0].i, y = a[1].i, v = 0;
+                int_range(f->res, &lo, &hi);
+                switch (f->op)
+                {
+                    case O_pl: v = x + y; break;
+                    case O_mi: v = x - y; break;
+                    case O_mul: v = x * y; break;
+                    case O_div:
+                        if (y == 0)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: division by zero", f->name);
+                        v = x / y;
+                        break;
+                    case O_mod:
+                        if (y == 0)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: division by zero", f->name);
+                        v = (y == -1) ? 0 : x % y;
+                        break;
+                }
+                if (v < lo || v > hi)
+                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                r->i = (int64_t)v;
+                return true;
+            }
+            if (f->res == T_float4)
+            {
+                float x = a[0].f, y = a[1].f, v = 0;
+                switch (f->op)
+                {
+                    case O_pl: v = x + y; break;
+                    case O_mi: v = x - y; break;
+                    case O_mul: v = x * y; break;
+                    case O_div:
+                        if (y == 0.0f)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: division by zero", f->name);
+                        v = x / y;
+                        if (v == 0.0f && x != 0.0f)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                        break;
+                }
+                if (isinf(v) && !isinf(x) && !isinf(y))
+                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                r->f = v;
+                return true;
+            }
+            else
+            {
+                double x = as_double(f->t0, &a[0]), y = as_double(f->t1, &a[1]), v = 0;
+                switch (f->op)
+                {
+                    case O_pl: v = x + y; break;
+                    case O_mi: v = x - y; break;
+                    case O_mul: v = x * y; break;
+                    case O_div:
+                        if (y == 0.0)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: division by zero", f->name);
+                        v = x / y;
+                        if (v == 0.0 && x != 0.0)
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                        break;
+                }
+                if (isinf(v) && !isinf(x) && !isinf(y))
+                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                r->d = v;
+                return true;
+            }
+        case K_unary:
+            if (is_int_type(f->res))
+            {
+                int64_t lo, hi;
+                int_range(f->res, &lo, &hi);
+                switch (f->op)
+                {
+                    case O_up: r->i = a[0].i; break;
+                    case O_um:
+                    case O_abs:
+                        if (a[0].i == lo && (f->op == O_um || a[0].i < 0))
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: value out of range", f->name);
+                        r->i = (f->op == O_um) ? -a[0].i : (a[0].i < 0 ? -a[0].i : a[0].i);
+                        break;
+                    case O_not:
+                        r->i = ~a[0].i;
+                        break;
+                }
+                return true;
+            }
+            if (f->res == T_float4)
+                r->f = f->op == O_um ? -a[0].f : f->op == O_abs ? fabsf(a[0].f) : a[0].f;
+            else
+                r->d = f->op == O_um ? -a[0].d : f->op == O_abs ? fabs(a[0].d) : a[0].d;
+            return true;
+        case K_bitop:
+        {
+            int64_t x = a[0].i, y = a[1].i, v = 0;
+            int bits = type_width(f->res) * 8;
+            switch (f->op)
+            {
+                case O_and: v = x & y; break;
+                case O_or:  v = x | y; break;
+                case O_xor: v = x ^ y; break;
+                case O_shl: v = (int64_t)((uint64_t)x << (y & (bits - 1))); break;
+                case O_shr: v = x >> (y & (bits - 1)); break;
+            }
+            /* wrap into the result width */
+            if (bits == 8) v = (int8_t)v;
+            else if (bits == 16) v = (int16_t)v;
+            else if (bits == 32) v = (int32_t)v;
+            r->i = v;
+            return true;
+        }
+        case K_cast:
+        {
+            int s = f->t0, d = f->res;
+            if (d == T_bool)
+            {
+                r->i = a[0].i != 0;
+                return true;
+            }
+            if (is_int_type(d))
+            {
+                int64_t lo, hi;
+                int_range(d, &lo, &hi);
+                if (is_int_type(s))
+                {
+                    if (a[0].i < lo || a[0].i > hi)
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                    r->i = a[0].i;
+                    return true;
+                }
+                if (s == T_float4)
+                {
+                    float v = rintf(a[0].f);
+                    if (isnan(a[0].f) || v < (float)lo || v > (float)hi ||
+                        (d == T_int8 && v >= 9223372036854775808.0f))
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                    r->i = (int64_t)v;
+                    return true;
+                }
+                {
+                    double v = rint(a[0].d);
+                    if (isnan(a[0].d) || v < (double)lo || v > (double)hi ||
+                        (d == T_int8 && v >= 9223372036854775808.0))
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                    r->i = (int64_t)v;
+                    return true;
+                }
+            }
+            if (d == T_float4 && s == T_float8)
+            {
+                /* __FLOAT_FITS_IN_FLOAT4 (xpu_basetype.cu:427-435) */
+                double x = a[0].d;
+                if (isinf(x) || isnan(x) || x < -3.40282346638528859812e+38 ||
+                    x > 3.40282346638528859812e+38)
+                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                r->f = (float)x;
+            }
+            else if (d == T_float4)
+                r->f = (float)a[0].i;
+            else
+                r->d = (s == T_float4) ? (double)a[0].f : (double)a[0].i;
+            return true;
+        }
+    }
+    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: unsupported function", f->name);
+}
+
+static const char *
+session_param(const kern_session_info *s, uint32_t id)
+{
+    if (id >= s->nparams || s->poffset[id] == 0)
+        return NULL;
+    return (const char *)s + s->poffset[id];
+}
+
+static bool
+eval(octx *c, const kern_expression *k, od *r)
+{
+    const kern_expression *karg;
+    od tmp;
+
+    memset(r, 0, sizeof(od));
+    switch (k->opcode)
+    {
+        case FuncOpCode__ConstExpr:
+            if (k->u.c.const_isnull)
+            {
+                r->isnull = true;
+                return true;
+            }
+            return datum_from_image(c, k->exptype, k->u.c.const_value, r);
+        case FuncOpCode__ParamExpr:
+        {
+            const char *p = session_param(c->session, k->u.p.param_id);
+            if (!p)
+            {
+                r->isnull = true;
+                return true;
+            }
+            return datum_from_image(c, k->exptype, p, r);
+        }
+        case FuncOpCode__VarExpr:
+            if (k->u.v.var_slot_id >= c->nslots)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s: slot out of range", "VarExpr");
+            *r = c->slots[k->u.v.var_slot_id];
+            return true;
+        case FuncOpCode__SaveExpr:
+            karg = KEXP_FIRST_ARG(k);
+            if (!eval(c, karg, &tmp))
+                return false;
+            c->slots[k->u.save.sv_slot_id] = tmp;
+            *r = tmp;
+            return true;
+        case FuncOpCode__BoolExpr_And:
+        {
+            /* SQL three-valued AND, stopping at the first false
+             * (xpu_common.cu:472-503) */
+            bool anynull = false;
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                if (tmp.isnull)
+                    anynull = true;
+                else if (!tmp.i)
+                {
+                    r->i = 0;
+                    return true;
+                }
+            }
+            r->isnull = anynull;
+            r->i = !anynull;
+            return true;
+        }
+        case FuncOpCode__BoolExpr_Or:
+        {
+            bool anynull = false;
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                if (tmp.isnull)
+                    anynull = true;
+                else if (tmp.i)
+                {
+                    r->i = 1;
+                    return true;
+                }
+            }
+            r->isnull = anynull;
+            r->i = 0;
+            return true;
+        }
+        case FuncOpCode__BoolExpr_Not:
+            if (!eval(c, KEXP_FIRST_ARG(k), &tmp))
+                return false;
+            r->isnull = tmp.isnull;
+            r->i = tmp.isnull ? 0 : !tmp.i;
+            return true;
+        case FuncOpCode__NullTestExpr_IsNull:
+        case FuncOpCode__NullTestExpr_IsNotNull:
+            if (!eval(c, KEXP_FIRST_ARG(k), &tmp))
+                return false;
+            r->i = (k->opcode == FuncOpCode__NullTestExpr_IsNull) ? tmp.isnull : !tmp.isnull;
+            return true;
+        case FuncOpCode__BoolTestExpr_IsTrue:
+        case FuncOpCode__BoolTestExpr_IsNotTrue:
+        case FuncOpCode__BoolTestExpr_IsFalse:
+        case FuncOpCode__BoolTestExpr_IsNotFalse:
+        case FuncOpCode__BoolTestExpr_IsUnknown:
+        case FuncOpCode__BoolTestExpr_IsNotUnknown:
+            if (!eval(c, KEXP_FIRST_ARG(k), &tmp))
+                return false;
+            switch (k->opcode)
+            {
+                case FuncOpCode__BoolTestExpr_IsTrue:     r->i = !tmp.isnull && tmp.i; break;
+                case FuncOpCode__BoolTestExpr_IsNotTrue:  r->i = tmp.isnull || !tmp.i; break;
+                case FuncOpCode__BoolTestExpr_IsFalse:    r->i = !tmp.isnull && !tmp.i; break;
+                case FuncOpCode__BoolTestExpr_IsNotFalse: r->i = tmp.isnull || tmp.i; break;
+                case FuncOpCode__BoolTestExpr_IsUnknown:  r->i = tmp.isnull; break;
+                default:                                  r->i = !tmp.isnull; break;
+            }
+            return true;
+        case FuncOpCode__CoalesceExpr:
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                if (!tmp.isnull)
+                {
+                    *r = tmp;
+                    return true;
+                }
+            }
+            r->isnull = true;
+            return true;
+        case FuncOpCode__HashValue:
+        {
+            uint32_t h = 0xffffffffU;
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                if (!tmp.isnull)
+                    h = pg_hash_merge(h, datum_hash(karg->exptype, &tmp));
+            }
+            r->i = (int64_t)(h ^ 0xffffffffU);
+            return true;
+        }
+        default:
+        {
+            const ofunc *f = lookup_func(k->opcode);
+            if (!f)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s: unsupported opcode", "oracle");
+            return eval_func(c, k, f, r);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * loaders
+ * ------------------------------------------------------------------ */
+static const kern_colmeta *
+kds_cmeta(const kern_data_store *kds, int resno)
+{
+    return &kds->colmeta[resno - 1];
+}
+
+/* Arrow column (xpu_basetype.cu:242-306, xpu_textlib.cu:237-274) */
+static bool
+load_arrow(octx *c, const kern_data_store *kds, int resno, uint32_t row, int type, od *r)
+{
+    const kern_colmeta *cm = kds_cmeta(kds, resno);
+    const char *base = (const char *)kds;
+
+    memset(r, 0, sizeof(od));
+    if (cm->nullmap_offset)
+    {
+        uint32_t idx = row >> 3;
+        if (idx >= cm->nullmap_length ||
+            (((const uint8_t *)base + cm->nullmap_offset)[idx] & (1U << (row & 7))) == 0)
+        {
+            r->isnull = true;
+            return true;
+        }
+    }
+    if (type == T_text || type == TypeOpCode__bytea)
+    {
+        const uint32_t *off = (const uint32_t *)(base + cm->values_offset);
+        if (4 * ((uint64_t)row + 1) > cm->values_length ||
+            off[row] > off[row + 1] || off[row + 1] > cm->extra_length)
+        {
+            r->isnull = true;
+            return true;
+        }
+        r->p = base + cm->extra_offset + off[row];
+        r->len = (int32_t)(off[row + 1] - off[row]);
+        return true;
+    }
+    if (type == T_bool)
+    {
+        const uint8_t *bits = (const uint8_t *)base + cm->values_offset;
+        r->i = (bits[row >> 3] >> (row & 7)) & 1;
+        return true;
+    }
+    {
+        int w = type_width(type);
+        if (w < 0 || (uint64_t)w * (row + 1) > cm->values_length)
+        {
+            r->isnull = true;
+            return true;
+        }
+        return datum_from_image(c, type, base + cm->values_offset + (uint64_t)w * row, r);
+    }
+}
+
+/* attribute `resno` of a minimal tuple (xpu_common.cu:60-130 deform) */
+static bool
+load_tuple_attr(octx *c, const kern_data_store *kds, const kern_tupitem *t, int resno,
+                int type, od *r)
+{
+    int ncols = t->t_infomask2 & HEAP_NATTS_MASK;
+    bool hasnull = (t->t_infomask & HEAP_HASNULL) != 0;
+    const char *pos = (const char *)t + t->t_hoff - MINIMAL_TUPLE_OFFSET;
+
+    memset(r, 0, sizeof(od));
+    if (resno > ncols || resno > (int)kds->ncols)
+    {
+        r->isnull = true;
+        return true;
+    }
+    for (int j = 0; j < resno; j++)
+    {
+        const kern_colmeta *cm = &kds->colmeta[j];
+        if (hasnull && !(t->t_bits[j >> 3] & (1 << (j & 7))))
+        {
+            if (j == resno - 1)
+            {
+                r->isnull = true;
+                return true;
+            }
+            continue;
+        }
+        if (cm->attlen > 0 || *(const uint8_t *)pos == 0)
+            pos = (const char *)t + XPU_TYPEALIGN(cm->attalign, pos - (const char *)t);
+        if (j == resno - 1)
+            return datum_from_image(c, type, pos, r);
+        if (cm->attlen > 0)
+            pos += cm->attlen;
+        else
+        {
+            uint8_t b0 = *(const uint8_t *)pos;
+            if (b0 == 0x01)
+                return oerror(c, ERRCODE_SUSPEND_FALLBACK, "%s: external datum", "oracle");
+            if (b0 & 0x01)
+                pos += (b0 >> 1) & 0x7f;
+            else
+            {
+                uint32_t h; memcpy(&h, pos, 4);
+                pos += (h >> 2) & 0x3fffffff;
+            }
+        }
+    }
+    r->isnull = true;
+    return true;
+}
+
+static bool
+exec_loadvars_arrow(octx *c, const kern_expression *kload, const kern_data_store *kds, uint32_t row)
+{
+    if (!kload)
+        return true;
+    for (int i = 0; i < kload->u.load.nitems; i++)
+    {
+        int resno = kload->u.load.desc[i].vl_resno;
+        int slot = kload->u.load.desc[i].vl_slot_id;
+        if (resno <= 0 || resno > (int)kds->ncols)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: resno out of range", "LoadVars");
+        if (!load_arrow(c, kds, resno, row, c->vdesc[slot].vs_type_code, &c->slots[slot]))
+            return false;
+    }
+    return true;
+}
+
+static bool
+exec_loadvars_tuple(octx *c, const kern_expression *kload, const kern_data_store *kds,
+                    const kern_tupitem *t)
+{
+    if (!kload)
+        return true;
+    for (int i = 0; i < kload->u.load.nitems; i++)
+    {
+        int resno = kload->u.load.desc[i].vl_resno;
+        int slot = kload->u.load.desc[i].vl_slot_id;
+        if (!t)
+        {
+            memset(&c->slots[slot], 0, sizeof(od));
+            c->slots[slot].isnull = true;
+            continue;
+        }
+        if (!load_tuple_attr(c, kds, t, resno, c->vdesc[slot].vs_type_code, &c->slots[slot]))
+            return false;
+    }
+    return true;
+}
+
+/* ------------------------------------------------------------------
+ * session
+ * ------------------------------------------------------------------ */
+typedef struct
+{
+    uint32_t    hash;
+    uint32_t    nkeys;
+    od          keys[OMAX_ATTRS];
+    char       *keybuf;         /* owns copies of text key bytes */
+    /* per attribute state */
+    int64_t     i0[OMAX_ATTRS];     /* count / nitems / min/max int */
+    int64_t     i1[OMAX_ATTRS];     /* psum_int sum */
+    int128_t    s128[OMAX_ATTRS];
+    double      d0[OMAX_ATTRS];     /* fp sum or fp min/max */
+    double      d1[OMAX_ATTRS];
+    uint32_t    attrs[OMAX_ATTRS];
+} ogroup;
+
+typedef struct
+{
+    ogroup    **groups;
+    size_t      ngroups;
+    size_t      nrooms;
+    int64_t    *slots;          /* open addressing: group index + 1 */
+    size_t      nslots;
+} ogtable;
+
+struct oracle_session
+{
+    kern_session_info *session;
+    size_t      length;
+    const kern_varslot_desc *vdesc;
+    int         nslots;
+    int         nrels;
+    const kern_expression *load[OMAX_DEPTH];
+    const kern_expression *hashv[OMAX_DEPTH];
+    const kern_expression *joinq[OMAX_DEPTH];
+    const kern_expression *scan_quals;
+    const kern_expression *proj;
+    const kern_expression *kds_dst_head_dummy;
+    const kern_data_store *kds_dst_head;
+    const kern_data_store *kds_final_head;
+    const kern_expression *gb_keyhash;
+    const kern_expression *gb_keyload;
+    const kern_expression *gb_keycomp;
+    const kern_expression *gb_actions;
+    int         key_types[OMAX_ATTRS];
+    ogtable     gt;
+};
+
+static const kern_expression *
+session_kexp(const kern_session_info *s, uint32_t off)
+{
+    return off ? (const kern_expression *)((const char *)s + off) : NULL;
+}
+
+static const kern_expression *
+packed_sub(const kern_expression *kpack, int idx)
+{
+    if (!kpack || idx < 0 || (uint32_t)idx > kpack->u.pack.npacked)
+        return NULL;
+    uint32_t off = kpack->u.pack.offset[idx];
+    return off ? (const kern_expression *)((const char *)kpack + off) : NULL;
+}
+
+static void
+set_error(kern_errorbuf *eb, uint32_t code, const char *func, const char *msg)
+{
+    if (!eb)
+        return;
+    memset(eb, 0, sizeof(*eb));
+    eb->errcode = code;
+    snprintf(eb->filename, sizeof(eb->filename), "oracle.c");
+    snprintf(eb->funcname, sizeof(eb->funcname), "%s", func);
+    snprintf(eb->message, sizeof(eb->message), "%s", msg);
+}
+
+oracle_session *
+oracle_session_open(const kern_session_info *session, size_t length, kern_errorbuf *eb)
+{
+    oracle_session *os = calloc(1, sizeof(oracle_session));
+    kern_session_info *s = malloc(length);
+
+    memcpy(s, session, length);
+    os->session = s;
+    os->length = length;
+    os->nslots = s->kcxt_kvars_nslots;
+    if (os->nslots > OMAX_SLOTS)
+    {
+        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "too many slots");
+        free(s); free(os);
+        return NULL;
+    }
+    os->vdesc = s->kcxt_kvars_defs ? (const kern_varslot_desc *)((const char *)s + s->kcxt_kvars_defs) : NULL;
+    {
+        const kern_expression *kp = session_kexp(s, s->xpucode_load_vars_packed);
+        const kern_expression *kj = session_kexp(s, s->xpucode_join_quals_packed);
+        const kern_expression *kh = session_kexp(s, s->xpucode_hash_values_packed);
+        os->nrels = kj ? (int)kj->u.pack.npacked - 1 : 0;
+        if (os->nrels < 0) os->nrels = 0;
+        for (int d = 0; d <= os->nrels && d < OMAX_DEPTH; d++)
+        {
+            os->load[d] = packed_sub(kp, d);
+            os->joinq[d] = packed_sub(kj, d);
+            os->hashv[d] = packed_sub(kh, d);
+        }
+    }
+    os->scan_quals = session_kexp(s, s->xpucode_scan_quals);
+    os->proj = session_kexp(s, s->xpucode_projection);
+    os->kds_dst_head = s->projection_kds_dst ? (const kern_data_store *)((const char *)s + s->projection_kds_dst) : NULL;
+    os->kds_final_head = s->groupby_kds_final ? (const kern_data_store *)((const char *)s + s->groupby_kds_final) : NULL;
+    os->gb_keyhash = session_kexp(s, s->xpucode_groupby_keyhash);
+    os->gb_keyload = session_kexp(s, s->xpucode_groupby_keyload);
+    os->gb_keycomp = session_kexp(s, s->xpucode_groupby_keycomp);
+    os->gb_actions = session_kexp(s, s->xpucode_groupby_actions);
+    if (os->gb_actions)
+    {
+        if (os->gb_actions->u.pagg.nattrs > OMAX_ATTRS)
+        {
+            set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "too many group-by attributes");
+            free(s); free(os);
+            return NULL;
+        }
+        for (int j = 0; j < os->gb_actions->u.pagg.nattrs; j++)
+        {
+            const kern_aggregate_desc *d = &os->gb_actions->u.pagg.desc[j];
+            os->key_types[j] = (d->arg0_slot_id >= 0) ? os->vdesc[d->arg0_slot_id].vs_type_code : 0;
+        }
+    }
+    return os;
+}
+
+void
+oracle_session_close(oracle_session *os)
+{
+    if (!os)
+        return;
+    for (size_t i = 0; i < os->gt.ngroups; i++)
+    {
+        free(os->gt.groups[i]->keybuf);
+        free(os->gt.groups[i]);
+    }
+    free(os->gt.groups);
+    free(os->gt.slots);
+    free(os->session);
+    free(os);
+}
+
+static void
+octx_init(octx *c, const oracle_session *os)
+{
+    memset(c, 0, sizeof(octx));
+    c->session = os->session;
+    c->vdesc = os->vdesc;
+    c->nslots = os->nslots;
+}
+
+/* ------------------------------------------------------------------
+ * projection: kern_form_minimal_tuple (xpu_common.cu:1096-1190)
+ * ------------------------------------------------------------------ */
+static int
+form_tuple(const oracle_session *os, octx *c, char *buf, uint32_t rowid)
+{
+    const kern_expression *kp = os->proj;
+    const kern_data_store *kds_dst = os->kds_dst_head;
+    int nattrs = kp->u.proj.nattrs;
+    bool hasnull = false;
+    uint32_t t_hoff;
+    uint16_t infomask = 0;
+    kern_tupitem *t = (kern_tupitem *)buf;
+
+    if ((int)kds_dst->ncols < nattrs)
+        nattrs = kds_dst->ncols;
+    for (int j = 0; j < nattrs; j++)
+    {
+        int s = kp->u.proj.slot_id[j];
+        if (s >= c->nslots || c->slots[s].isnull)
+        {
+            hasnull = true;
+            infomask |= HEAP_HASNULL;
+            break;
+        }
+    }
+    t_hoff = KERN_TUPITEM_T_BITS_OFFSET;
+    if (hasnull)
+        t_hoff += XPU_BITMAPLEN(nattrs);
+    t_hoff = XPU_MAXALIGN(t_hoff);
+    if (buf)
+    {
+        memset(buf, 0, t_hoff);
+        t->t_infomask2 = nattrs & HEAP_NATTS_MASK;
+        t->t_hoff = t_hoff + MINIMAL_TUPLE_OFFSET;
+    }
+    for (int j = 0; j < nattrs; j++)
+    {
+        const kern_colmeta *cm = &kds_dst->colmeta[j];
+        int s = kp->u.proj.slot_id[j];
+        if (s >= c->nslots || c->slots[s].isnull)
+            continue;
+        uint32_t nxt = XPU_TYPEALIGN(cm->attalign, t_hoff);
+        if (buf && nxt > t_hoff)
+            memset(buf + t_hoff, 0, nxt - t_hoff);
+        int sz = datum_write(c->vdesc[s].vs_type_code, &c->slots[s], buf ? buf + nxt : NULL);
+        if (sz < 0)
+            return -1;
+        if (cm->attlen == -1)
+            infomask |= HEAP_HASVARWIDTH;
+        if (buf && hasnull)
+            t->t_bits[j >> 3] |= (1 << (j & 7));
+        t_hoff = nxt + sz;
+    }
+    t_hoff += ROWID_SIZE;
+    if (buf)
+    {
+        t->t_len = t_hoff;
+        t->t_infomask = infomask;
+        memcpy(buf + t_hoff - ROWID_SIZE, &rowid, ROWID_SIZE);
+    }
+    return t_hoff;
+}
+
+/* run the projection's SaveExprs to fill the slots */
+static bool
+prep_projection(const oracle_session *os, octx *c)
+{
+    const kern_expression *kp = os->proj;
+    const kern_expression *karg = KEXP_FIRST_ARG(kp);
+    od tmp;
+    for (uint32_t i = 0; i < kp->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+        if (!eval(c, karg, &tmp))
+            return false;
+    return true;
+}
+
+/* ------------------------------------------------------------------
+ * group-by
+ * ------------------------------------------------------------------ */
+static bool
+run_actions_saves(const oracle_session *os, octx *c)
+{
+    const kern_expression *ka = os->gb_actions;
+    const kern_expression *karg = KEXP_FIRST_ARG(ka);
+    od tmp;
+    for (uint32_t i = 0; i < ka->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+        if (!eval(c, karg, &tmp))
+            return false;
+    return true;
+}
+
+static bool
+keys_equal(const oracle_session *os, const ogroup *g, octx *c)
+{
+    const kern_expression *ka = os->gb_actions;
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        const kern_aggregate_desc *d = &ka->u.pagg.desc[j];
+        if (d->action != KAGG_ACTION__VREF)
+            continue;
+        const od *a = &c->slots[d->arg0_slot_id];
+        const od *b = &g->keys[j];
+        int t = os->key_types[j];
+        /* kmode_compare_nulls: NULL keys group together */
+        if (a->isnull || b->isnull)
+        {
+            if (a->isnull != b->isnull)
+                return false;
+            continue;
+        }
+        switch (t)
+        {
+            case T_float4: if (!float_cmp(O_eq, a->f, b->f)) return false; break;
+            case T_float8: if (!float_cmp(O_eq, a->d, b->d)) return false; break;
+            case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+                if (a->len != b->len || memcmp(a->p, b->p, a->len) != 0) return false;
+                break;
+            default: if (a->i != b->i) return false; break;
+        }
+    }
+    return true;
+}
+
+static void
+group_init(const oracle_session *os, ogroup *g)
+{
+    const kern_expression *ka = os->gb_actions;
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        switch (ka->u.pagg.desc[j].action)
+        {
+            case KAGG_ACTION__PMIN_INT32:
+            case KAGG_ACTION__PMIN_INT64: g->i0[j] = INT64_MAX; break;
+            case KAGG_ACTION__PMAX_INT32:
+            case KAGG_ACTION__PMAX_INT64: g->i0[j] = INT64_MIN; break;
+            case KAGG_ACTION__PMIN_FP64: g->d0[j] = 1.7976931348623157e308; break;
+            case KAGG_ACTION__PMAX_FP64: g->d0[j] = -1.7976931348623157e308; break;
+            default: break;
+        }
+    }
+}
+
+static ogroup *
+group_lookup(const oracle_session *os, ogtable *gt, octx *c, uint32_t hash)
+{
+    const kern_expression *ka = os->gb_actions;
+    if (gt->nslots == 0 || gt->ngroups * 2 >= gt->nslots)
+    {
+        size_t ns = gt->nslots ? gt->nslots * 2 : 1024;
+        int64_t *slots = calloc(ns, sizeof(int64_t));
+        for (size_t i = 0; i < gt->ngroups; i++)
+        {
+            size_t h = gt->groups[i]->hash & (ns - 1);
+            while (slots[h]) h = (h + 1) & (ns - 1);
+            slots[h] = i + 1;
+        }
+        free(gt->slots);
+        gt->slots = slots;
+        gt->nslots = ns;
+    }
+    size_t h = hash & (gt->nslots - 1);
+    while (gt->slots[h])
+    {
+        ogroup *g = gt->groups[gt->slots[h] - 1];
+        if (g->hash == hash && keys_equal(os, g, c))
+            return g;
+        h = (h + 1) & (gt->nslots - 1);
+    }
+    /* insert */
+    ogroup *g = calloc(1, sizeof(ogroup));
+    size_t klen = 0, kpos = 0;
+    g->hash = hash;
+    group_init(os, g);
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+        if (ka->u.pagg.desc[j].action == KAGG_ACTION__VREF)
+            klen += c->slots[ka->u.pagg.desc[j].arg0_slot_id].len;
+    g->keybuf = malloc(klen + 1);
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        const kern_aggregate_desc *d = &ka->u.pagg.desc[j];
+        if (d->action != KAGG_ACTION__VREF)
+            continue;
+        g->keys[j] = c->slots[d->arg0_slot_id];
+        if (g->keys[j].p && !g->keys[j].isnull)
+        {
+            memcpy(g->keybuf + kpos, g->keys[j].p, g->keys[j].len);
+            g->keys[j].p = g->keybuf + kpos;
+            kpos += g->keys[j].len;
+        }
+    }
+    if (gt->ngroups == gt->nrooms)
+    {
+        gt->nrooms = gt->nrooms ? gt->nrooms * 2 : 1024;
+        gt->groups = realloc(gt->groups, sizeof(ogroup *) * gt->nrooms);
+    }
+    gt->groups[gt->ngroups++] = g;
+    gt->slots[h] = gt->ngroups;
+    return g;
+}
+
+static bool
+fetch_int64(int t, const od *v, int64_t *out)
+{
+    if (v->isnull) return false;
+    if (t == T_float4) *out = (int64_t)v->f;
+    else if (t == T_float8) *out = (int64_t)v->d;
+    else *out = v->i;
+    return true;
+}
+
+static bool
+fetch_fp64(int t, const od *v, double *out)
+{
+    if (v->isnull) return false;
+    *out = as_double(t, v);
+    return true;
+}
+
+/* __updateOneTupleGroupBy (cuda_gpupreagg.cu:1499-1605) */
+static void
+group_update(const oracle_session *os, ogroup *g, octx *c)
+{
+    const kern_expression *ka = os->gb_actions;
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        const kern_aggregate_desc *d = &ka->u.pagg.desc[j];
+        int t = d->arg0_slot_id >= 0 ? c->vdesc[d->arg0_slot_id].vs_type_code : 0;
+        const od *v = d->arg0_slot_id >= 0 ? &c->slots[d->arg0_slot_id] : NULL;
+        int64_t iv;
+        double fv;
+
+        if (d->filter_slot_id >= 0)
+        {
+            const od *fl = &c->slots[d->filter_slot_id];
+            if (fl->isnull || !fl->i)
+                continue;
+        }
+        switch (d->action)
+        {
+            case KAGG_ACTION__NROWS_ANY:
+                g->i0[j]++;
+                break;
+            case KAGG_ACTION__NROWS_COND:
+                if (!v->isnull) g->i0[j]++;
+                break;
+            case KAGG_ACTION__PMIN_INT32:
+            case KAGG_ACTION__PMIN_INT64:
+                if (fetch_int64(t, v, &iv))
+                {
+                    g->attrs[j] |= __PAGG_MINMAX_ATTRS__VALID;
+                    if (iv < g->i0[j]) g->i0[j] = iv;
+                }
+                break;
+            case KAGG_ACTION__PMAX_INT32:
+            case KAGG_ACTION__PMAX_INT64:
+                if (fetch_int64(t, v, &iv))
+                {
+                    g->attrs[j] |= __PAGG_MINMAX_ATTRS__VALID;
+                    if (iv > g->i0[j]) g->i0[j] = iv;
+                }
+                break;
+            case KAGG_ACTION__PMIN_FP64:
+                if (fetch_fp64(t, v, &fv))
+                {
+                    g->attrs[j] |= __PAGG_MINMAX_ATTRS__VALID;
+                    if (fv < g->d0[j]) g->d0[j] = fv;
+                }
+                break;
+            case KAGG_ACTION__PMAX_FP64:
+                if (fetch_fp64(t, v, &fv))
+                {
+                    g->attrs[j] |= __PAGG_MINMAX_ATTRS__VALID;
+                    if (fv > g->d0[j]) g->d0[j] = fv;
+                }
+                break;
+            case KAGG_ACTION__PSUM_INT:
+            case KAGG_ACTION__PAVG_INT:
+                if (fetch_int64(t, v, &iv))
+                {
+                    g->i0[j]++;
+                    g->i1[j] = (int64_t)((uint64_t)g->i1[j] + (uint64_t)iv);
+                }
+                break;
+            case KAGG_ACTION__PSUM_INT64:
+            case KAGG_ACTION__PAVG_INT64:
+                if (fetch_int64(t, v, &iv))
+                {
+                    g->i0[j]++;
+                    g->s128[j] += iv;
+                }
+                break;
+            case KAGG_ACTION__PSUM_FP:
+            case KAGG_ACTION__PAVG_FP:
+                if (fetch_fp64(t, v, &fv))
+                {
+                    g->i0[j]++;
+                    g->d0[j] += fv;
+                }
+                break;
+            case KAGG_ACTION__STDDEV:
+                if (fetch_fp64(t, v, &fv))
+                {
+                    g->i0[j]++;
+                    g->d0[j] += fv;
+                    g->d1[j] += fv * fv;
+                }
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+static void
+group_merge(const oracle_session *os, ogroup *dst, const ogroup *src)
+{
+    const kern_expression *ka = os->gb_actions;
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        switch (ka->u.pagg.desc[j].action)
+        {
+            case KAGG_ACTION__NROWS_ANY:
+            case KAGG_ACTION__NROWS_COND:
+                dst->i0[j] += src->i0[j];
+                break;
+            case KAGG_ACTION__PMIN_INT32:
+            case KAGG_ACTION__PMIN_INT64:
+                dst->attrs[j] |= src->attrs[j];
+                if (src->i0[j] < dst->i0[j]) dst->i0[j] = src->i0[j];
+                break;
+            case KAGG_ACTION__PMAX_INT32:
+            case KAGG_ACTION__PMAX_INT64:
+                dst->attrs[j] |= src->attrs[j];
+                if (src->i0[j] > dst->i0[j]) dst->i0[j] = src->i0[j];
+                break;
+            case KAGG_ACTION__PMIN_FP64:
+                dst->attrs[j] |= src->attrs[j];
+                if (src->d0[j] < dst->d0[j]) dst->d0[j] = src->d0[j];
+                break;
+            case KAGG_ACTION__PMAX_FP64:
+                dst->attrs[j] |= src->attrs[j];
+                if (src->d0[j] > dst->d0[j]) dst->d0[j] = src->d0[j];
+                break;
+            case KAGG_ACTION__PSUM_INT:
+            case KAGG_ACTION__PAVG_INT:
+                dst->i0[j] += src->i0[j];
+                dst->i1[j] = (int64_t)((uint64_t)dst->i1[j] + (uint64_t)src->i1[j]);
+                break;
+            case KAGG_ACTION__PSUM_INT64:
+            case KAGG_ACTION__PAVG_INT64:
+                dst->i0[j] += src->i0[j];
+                dst->s128[j] += src->s128[j];
+                break;
+            case KAGG_ACTION__PSUM_FP:
+            case KAGG_ACTION__PAVG_FP:
+                dst->i0[j] += src->i0[j];
+                dst->d0[j] += src->d0[j];
+                break;
+            case KAGG_ACTION__STDDEV:
+                dst->i0[j] += src->i0[j];
+                dst->d0[j] += src->d0[j];
+                dst->d1[j] += src->d1[j];
+                break;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
+ * per-row pipeline: joins (depth-first) then projection or group-by
+ * ------------------------------------------------------------------ */
+typedef struct
+{
+    char       *buf;
+    size_t      len;
+    size_t      cap;
+    uint32_t    nrows;
+} obuf;
+
+static void
+obuf_append(obuf *b, const void *p, size_t n)
+{
+    if (b->len + n > b->cap)
+    {
+        b->cap = (b->cap + n) * 2;
+        b->buf = realloc(b->buf, b->cap);
+    }
+    memcpy(b->buf + b->len, p, n);
+    b->len += n;
+}
+
+typedef struct
+{
+    const oracle_session *os;
+    const kern_multirels *kmrels;
+    octx       *c;
+    obuf       *out;        /* projection: packed tuples (rowid patched later) */
+    ogtable    *gt;         /* group-by */
+    uint32_t    nitems_out;
+    uint32_t    depth_out[OMAX_DEPTH];
+} orow;
+
+static const kern_data_store *
+inner_kds(const kern_multirels *km, int depth)
+{
+    uint64_t off = km->chunks[depth - 1].kds_offset;
+    return off ? (const kern_data_store *)((const char *)km + off) : NULL;
+}
+
+static bool emit_row(orow *w);
+
+static bool
+join_depth(orow *w, int depth)
+{
+    const oracle_session *os = w->os;
+    octx *c = w->c;
+
+    if (depth > os->nrels)
+        return emit_row(w);
+
+    const kern_data_store *kds = inner_kds(w->kmrels, depth);
+    const kern_expression *kh = os->hashv[depth];
+    const kern_expression *kj = os->joinq[depth];
+    bool left_outer = w->kmrels->chunks[depth - 1].left_outer;
+    bool matched = false;
+    od hv;
+    /* slots are overwritten by deeper depths; save ours to restore */
+    od saved[OMAX_SLOTS];
+
+    if (!kh || !eval(c, kh, &hv))
+        return kh ? false : oerror(c, ERRCODE_DEVICE_ERROR, "%s: no hash value", "join");
+    uint32_t hash = (uint32_t)hv.i;
+    const uint64_t *hslots = (const uint64_t *)((const char *)kds + KDS_HEAD_LENGTH(kds));
+    uint64_t off = hslots[hash % kds->hash_nslots];
+    memcpy(saved, c->slots, sizeof(od) * c->nslots);
+    while (off != 0 && off != UINT64_MAX)
+    {
+        const kern_hashitem *hi = (const kern_hashitem *)((const char *)kds + kds->length - off);
+        off = hi->next;
+        if (hi->t.hash != hash)
+            continue;
+        if (!exec_loadvars_tuple(c, os->load[depth], kds, &hi->t))
+            return false;
+        int status = 1;
+        if (kj)
+        {
+            /* ExecGpuJoinQuals (xpu_common.cu:1442-1481) */
+            const kern_expression *karg = KEXP_FIRST_ARG(kj);
+            for (uint32_t i = 0; i < kj->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                od q;
+                if (status < 0 && (karg->expflags & KEXP_FLAG__IS_PUSHED_DOWN))
+                    continue;
+                if (!eval(c, karg, &q))
+                    return false;
+                if (q.isnull || !q.i)
+                {
+                    if (!(karg->expflags & KEXP_FLAG__IS_PUSHED_DOWN))
+                    {
+                        status = 0;
+                        break;
+                    }
+                    status = -1;
+                }
+            }
+        }
+        if (status != 0)
+            matched = true;
+        if (status > 0)
+        {
+            w->depth_out[depth]++;
+            if (!join_depth(w, depth + 1))
+                return false;
+            memcpy(c->slots, saved, sizeof(od) * c->nslots);
+        }
+    }
+    if (left_outer && !matched)
+    {
+        /* null-extended row, filtered by the pushed-down quals only */
+        bool ok = true;
+        if (!exec_loadvars_tuple(c, os->load[depth], kds, NULL))
+            return false;
+        if (kj)
+        {
+            const kern_expression *karg = KEXP_FIRST_ARG(kj);
+            for (uint32_t i = 0; i < kj->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                od q;
+                if (!(karg->expflags & KEXP_FLAG__IS_PUSHED_DOWN))
+                    continue;
+                if (!eval(c, karg, &q))
+                    return false;
+                if (q.isnull || !q.i)
+                {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+        if (ok)
+        {
+            w->depth_out[depth]++;
+            if (!join_depth(w, depth + 1))
+                return false;
+        }
+        memcpy(c->slots, saved, sizeof(od) * c->nslots);
+    }
+    return true;
+}
+
+static bool
+emit_row(orow *w)
+{
+    const oracle_session *os = w->os;
+    octx *c = w->c;
+
+    if (os->gb_actions)
+    {
+        od hv;
+        uint32_t hash = 0;
+        if (os->gb_keyhash)
+        {
+            if (!eval(c, os->gb_keyhash, &hv))
+                return false;
+            hash = (uint32_t)hv.i;
+        }
+        if (!run_actions_saves(os, c))
+            return false;
+        ogroup *g = group_lookup(os, w->gt, c, hash);
+        group_update(os, g, c);
+        w->nitems_out++;
+        return true;
+    }
+    if (os->proj)
+    {
+        if (!prep_projection(os, c))
+            return false;
+        int sz = form_tuple(os, c, NULL, 0);
+        if (sz < 0)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: unable to form tuple", "projection");
+        char tmp[XPU_MAXALIGN(sz) + 8];
+        memset(tmp, 0, sizeof(tmp));
+        form_tuple(os, c, tmp, 0);
+        uint32_t len32 = (uint32_t)XPU_MAXALIGN(sz);
+        obuf_append(w->out, &len32, 4);
+        obuf_append(w->out, tmp, len32);
+        w->out->nrows++;
+        w->nitems_out++;
+        return true;
+    }
+    w->nitems_out++;
+    return true;
+}
+
+typedef struct
+{
+    obuf        out;
+    ogtable     gt;
+    uint32_t    nitems_in;
+    uint32_t    nitems_out;
+    uint32_t    depth_out[OMAX_DEPTH];
+    uint32_t    errcode;
+    char        errmsg[KERN_ERRORBUF_MESSAGE_LEN + 1];
+} othread;
+
+static void
+run_rows(const oracle_session *os, const kern_data_store *kds_src, const kern_multirels *kmrels,
+         uint32_t row_begin, uint32_t row_end, othread *th)
+{
+    octx *c = malloc(sizeof(octx));
+    orow w;
+
+    octx_init(c, os);
+    memset(&w, 0, sizeof(w));
+    w.os = os;
+    w.kmrels = kmrels;
+    w.c = c;
+    w.out = &th->out;
+    w.gt = &th->gt;
+    for (uint32_t row = row_begin; row < row_end; row++)
+    {
+        od q;
+        for (int s = 0; s < c->nslots; s++)
+        {
+            memset(&c->slots[s], 0, sizeof(od));
+            c->slots[s].isnull = true;
+        }
+        if (!exec_loadvars_arrow(c, os->load[0], kds_src, row))
+            break;
+        if (os->scan_quals)
+        {
+            if (!eval(c, os->scan_quals, &q))
+                break;
+            if (q.isnull || !q.i)
+                continue;
+        }
+        th->nitems_in++;
+        if (!join_depth(&w, 1))
+            break;
+    }
+    th->nitems_out = w.nitems_out;
+    memcpy(th->depth_out, w.depth_out, sizeof(w.depth_out));
+    th->errcode = c->errcode;
+    memcpy(th->errmsg, c->errmsg, sizeof(th->errmsg));
+    free(c);
+}
+
+int
+oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
+                 const kern_multirels *kmrels,
+                 kern_data_store *kds_dst, size_t dst_length,
+                 int nthreads, kern_exec_results *results, kern_errorbuf *eb)
+{
+    uint32_t nitems = kds_src->nitems;
+    othread *th;
+    int rc = 0;
+
+    if (kds_src->format != KDS_FORMAT_ARROW)
+    {
+        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "oracle reads KDS_FORMAT_ARROW only");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (os->nrels > 0 && (!kmrels || (int)kmrels->num_inner_rels < os->nrels))
+    {
+        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "inner relations missing");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (nthreads < 1)
+        nthreads = 1;
+    th = calloc(nthreads, sizeof(othread));
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+    for (int t = 0; t < nthreads; t++)
+    {
+        uint64_t b = (uint64_t)nitems * t / nthreads;
+        uint64_t e = (uint64_t)nitems * (t + 1) / nthreads;
+        run_rows(os, kds_src, kmrels, (uint32_t)b, (uint32_t)e, &th[t]);
+    }
+    if (results)
+    {
+        memset(results, 0, sizeof(kern_exec_results));
+        results->num_rels = os->nrels;
+    }
+    for (int t = 0; t < nthreads; t++)
+    {
+        if (th[t].errcode && !rc)
+        {
+            rc = th[t].errcode;
+            set_error(eb, rc, "oracle_task_exec", th[t].errmsg);
+        }
+        if (results)
+        {
+            results->nitems_raw = nitems;
+            results->nitems_in += th[t].nitems_in;
+            results->nitems_out += th[t].nitems_out;
+            for (int d = 1; d <= os->nrels && d <= 8; d++)
+                results->stats[d - 1].nitems_out += th[t].depth_out[d];
+        }
+    }
+    /* projection: write kds_dst in input order */
+    if (!rc && os->proj && os->gb_actions == NULL)
+    {
+        const kern_data_store *head = os->kds_dst_head;
+        size_t head_sz = KDS_HEAD_LENGTH(head);
+        uint64_t total_rows = 0, total_bytes = 0;
+        for (int t = 0; t < nthreads; t++)
+        {
+            total_rows += th[t].out.nrows;
+            total_bytes += th[t].out.len - 4ULL * th[t].out.nrows;
+        }
+        if (!kds_dst || head_sz + 8 * total_rows + total_bytes > dst_length)
+        {
+            rc = ERRCODE_SUSPEND_NO_SPACE;
+            set_error(eb, rc, __func__, "kds_dst has no space");
+        }
+        else
+        {
+            uint64_t *rowidx = (uint64_t *)((char *)kds_dst + head_sz);
+            uint64_t usage = 0;
+            uint32_t rowid = 0;
+            memcpy(kds_dst, head, head_sz);
+            kds_dst->length = dst_length;
+            for (int t = 0; t < nthreads; t++)
+            {
+                size_t pos = 0;
+                while (pos < th[t].out.len)
+                {
+                    uint32_t len;
+                    memcpy(&len, th[t].out.buf + pos, 4);
+                    pos += 4;
+                    usage += len;
+                    char *dst = (char *)kds_dst + dst_length - usage;
+                    memcpy(dst, th[t].out.buf + pos, len);
+                    kern_tupitem *ti = (kern_tupitem *)dst;
+                    memcpy(dst + ti->t_len - ROWID_SIZE, &rowid, ROWID_SIZE);
+                    rowidx[rowid++] = usage;
+                    pos += len;
+                }
+            }
+            kds_dst->nitems = rowid;
+            kds_dst->usage = usage;
+        }
+    }
+    /* group-by: merge thread tables into the session table */
+    if (!rc && os->gb_actions)
+    {
+        for (int t = 0; t < nthreads; t++)
+        {
+            octx *c = malloc(sizeof(octx));
+            octx_init(c, os);
+            for (size_t i = 0; i < th[t].gt.ngroups; i++)
+            {
+                ogroup *src = th[t].gt.groups[i];
+                const kern_expression *ka = os->gb_actions;
+                for (int j = 0; j < ka->u.pagg.nattrs; j++)
+                    if (ka->u.pagg.desc[j].action == KAGG_ACTION__VREF)
+                        c->slots[ka->u.pagg.desc[j].arg0_slot_id] = src->keys[j];
+                ogroup *dst = group_lookup(os, &os->gt, c, src->hash);
+                group_merge(os, dst, src);
+            }
+            free(c);
+        }
+    }
+    for (int t = 0; t < nthreads; t++)
+    {
+        free(th[t].out.buf);
+        for (size_t i = 0; i < th[t].gt.ngroups; i++)
+        {
+            free(th[t].gt.groups[i]->keybuf);
+            free(th[t].gt.groups[i]);
+        }
+        free(th[t].gt.groups);
+        free(th[t].gt.slots);
+    }
+    free(th);
+    return rc;
+}
+
+/* ------------------------------------------------------------------
+ * final buffer: group tuples in the layout of __writeOutOneTuplePreAgg
+ * (cuda_gpupreagg.cu:18-223), returned as KDS_FORMAT_ROW the way the
+ * service writes kds_final back (gpu_service.c:2780-2818).
+ * ------------------------------------------------------------------ */
+static int
+write_group_tuple(const oracle_session *os, const ogroup *g, char *buf)
+{
+    const kern_expression *ka = os->gb_actions;
+    const kern_data_store *kf = os->kds_final_head;
+    int nattrs = (int)kf->ncols < ka->u.pagg.nattrs ? (int)kf->ncols : ka->u.pagg.nattrs;
+    uint32_t t_hoff = XPU_MAXALIGN(KERN_TUPITEM_T_BITS_OFFSET + XPU_BITMAPLEN(nattrs));
+    uint16_t infomask = HEAP_HASNULL;
+    kern_tupitem *t = (kern_tupitem *)buf;
+
+    if (buf)
+    {
+        memset(buf, 0, t_hoff);
+        t->t_infomask2 = nattrs & HEAP_NATTS_MASK;
+        t->t_hoff = t_hoff + MINIMAL_TUPLE_OFFSET;
+        t->hash = g->hash;
+    }
+    for (int j = 0; j < nattrs; j++)
+    {
+        const kern_aggregate_desc *d = &ka->u.pagg.desc[j];
+        const kern_colmeta *cm = &kf->colmeta[j];
+        uint32_t nxt = XPU_TYPEALIGN(cm->attalign, t_hoff);
+        char *p = buf ? buf + nxt : NULL;
+        int nbytes = 0;
+
+        if (buf && nxt > t_hoff)
+            memset(buf + t_hoff, 0, nxt - t_hoff);
+        switch (d->action)
+        {
+            case KAGG_ACTION__VREF:
+                /* keys never set HEAP_HASVARWIDTH (cuda_gpupreagg.cu:56-70) */
+                if (!g->keys[j].isnull)
+                    nbytes = datum_write(os->key_types[j], &g->keys[j], p);
+                break;
+            case KAGG_ACTION__NROWS_ANY:
+            case KAGG_ACTION__NROWS_COND:
+                nbytes = 8;
+                if (p) memcpy(p, &g->i0[j], 8);
+                break;
+            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
+            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
+            {
+                kagg_state__pminmax_int64_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.attrs = g->attrs[j];
+                r.value = g->i0[j];
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64:
+            {
+                kagg_state__pminmax_fp64_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.attrs = g->attrs[j];
+                r.value = g->d0[j];
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
+            {
+                kagg_state__psum_int_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.nitems = g->i0[j];
+                r.sum = g->i1[j];
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            {
+                kagg_state__psum_numeric_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.nitems = (uint64_t)g->i0[j];
+                r.sum.u64_lo = (uint64_t)g->s128[j];
+                r.sum.u64_hi = (uint64_t)(g->s128[j] >> 64);
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
+            {
+                kagg_state__psum_fp_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.nitems = g->i0[j];
+                r.sum = g->d0[j];
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            case KAGG_ACTION__STDDEV:
+            {
+                kagg_state__stddev_packed r;
+                memset(&r, 0, sizeof(r));
+                r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.nitems = g->i0[j];
+                r.sum_x = g->d0[j];
+                r.sum_x2 = g->d1[j];
+                nbytes = sizeof(r);
+                if (p) memcpy(p, &r, sizeof(r));
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
+            default:
+                return -1;
+        }
+        if (buf && nbytes > 0)
+            t->t_bits[j >> 3] |= (1 << (j & 7));
+        t_hoff = nxt + nbytes;
+    }
+    t_hoff += ROWID_SIZE;
+    if (buf)
+    {
+        t->t_len = t_hoff;
+        t->t_infomask = infomask;
+    }
+    return t_hoff;
+}
+
+int64_t
+oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_length,
+                  kern_errorbuf *eb)
+{
+    const kern_data_store *head = os->kds_final_head;
+    size_t head_sz;
+    uint64_t usage = 0, need = 0;
+
+    if (!head || !os->gb_actions)
+    {
+        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "session has no group-by");
+        return -1;
+    }
+    head_sz = KDS_HEAD_LENGTH(head);
+    for (size_t i = 0; i < os->gt.ngroups; i++)
+        need += XPU_TYPEALIGN(XPU_L1_CACHELINE_SZ,
+                              offsetof(kern_hashitem, t) + write_group_tuple(os, os->gt.groups[i], NULL));
+    need += head_sz + 8 * os->gt.ngroups;
+    if (!kds_final)
+        return (int64_t)need;
+    if (need > final_length)
+    {
+        set_error(eb, ERRCODE_SUSPEND_NO_SPACE, __func__, "kds_final has no space");
+        return -1;
+    }
+    memcpy(kds_final, head, head_sz);
+    kds_final->format = KDS_FORMAT_ROW;
+    kds_final->hash_nslots = 0;
+    kds_final->length = need;
+    uint64_t *rowidx = (uint64_t *)((char *)kds_final + head_sz);
+    for (size_t i = 0; i < os->gt.ngroups; i++)
+    {
+        int sz = write_group_tuple(os, os->gt.groups[i], NULL);
+        uint64_t unit = XPU_TYPEALIGN(XPU_L1_CACHELINE_SZ, offsetof(kern_hashitem, t) + sz);
+        usage += unit;
+        kern_hashitem *hi = (kern_hashitem *)((char *)kds_final + need - usage);
+        memset(hi, 0, unit);
+        write_group_tuple(os, os->gt.groups[i], (char *)&hi->t);
+        uint32_t rowid = (uint32_t)i;
+        memcpy((char *)&hi->t + hi->t.t_len - ROWID_SIZE, &rowid, ROWID_SIZE);
+        rowidx[i] = (char *)kds_final + need - (char *)&hi->t;
+    }
+    kds_final->nitems = (uint32_t)os->gt.ngroups;
+    kds_final->usage = usage;
+    return (int64_t)need;
+}

@@ -1,0 +1,40 @@
+/*
+ * oracle.h - CPU restatement of PG-Strom's xPU scan / join / pre-aggregation
+ * semantics.  TEST INFRASTRUCTURE ONLY: tests, smoke() and bench.py's
+ * cpu_baseline leg load it as the checker; the product never links it.
+ *
+ * It consumes the same kern_session_info blob and kern_data_store inputs as
+ * the HIP library (include/xpuhip.h) and produces the same output formats:
+ * a KDS_FORMAT_ROW kds_dst in input-row order for scans/joins, and a
+ * group table written as the reference's kds_final (HASH tuples converted to
+ * ROW, gpu_service.c:2780-2818) for pre-aggregation.
+ */
+#ifndef PGSTROM_ORACLE_H
+#define PGSTROM_ORACLE_H
+#include "pgstrom_xpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_session oracle_session;
+
+oracle_session *oracle_session_open(const kern_session_info *session, size_t length,
+                                    kern_errorbuf *errbuf);
+int     oracle_task_exec(oracle_session *os,
+                         const kern_data_store *kds_src,
+                         const kern_multirels *kmrels,
+                         kern_data_store *kds_dst, size_t dst_length,
+                         int nthreads,
+                         kern_exec_results *results,
+                         kern_errorbuf *errbuf);
+int64_t oracle_task_final(oracle_session *os, kern_data_store *kds_final,
+                          size_t final_length, kern_errorbuf *errbuf);
+void    oracle_session_close(oracle_session *os);
+
+uint32_t oracle_hash_any(const void *ptr, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+/*
+ * ref_driver.cc - test-only checker built against the reference sources.
+ *
+ * This file is compiled, together with the reference's device-agnostic VM
+ * sources (src/xpu_common.cu, xpu_basetype.cu, xpu_numeric.cu, xpu_textlib.cu,
+ * xpu_timelib.cu, xpu_misclib.cu, xpu_jsonlib.cu, compiled as host C++), into
+ * oracle/_ref/libpgstrom_ref.so by oracle/Makefile.  It is never part of the
+ * product: only tests and tests/golden generators load it, to
+ *   - dump the reference's struct layouts and opcode numbers (ref_abi_json),
+ *   - compute the reference's pg_hash_any (xpu_common.cu:2920-3041).
+ */
+#include "xpu_common.h"
+#include <stdlib.h>
+#include <string>
+
+static std::string g_json;
+
+static void emit(const char *key, long long val)
+{
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s\"%s\": %lld", g_json.size() > 1 ? ",\n " : "", key, val);
+    g_json += buf;
+}
+
+#define SZ(T)           emit("sizeof:" #T, (long long)sizeof(T))
+#define OFF(T, F)       emit("offsetof:" #T "." #F, (long long)offsetof(T, F))
+
+extern "C" const char *
+ref_abi_json(void)
+{
+    g_json = "{";
+    SZ(kern_errorbuf);
+    OFF(kern_errorbuf, lineno); OFF(kern_errorbuf, filename);
+    OFF(kern_errorbuf, funcname); OFF(kern_errorbuf, message);
+
+    SZ(ArrowTypeOptions);
+    OFF(ArrowTypeOptions, unitsz); OFF(ArrowTypeOptions, align);
+    OFF(ArrowTypeOptions, integer.bitWidth); OFF(ArrowTypeOptions, integer.is_signed);
+    OFF(ArrowTypeOptions, floating_point.precision);
+
+    SZ(kern_colmeta);
+    OFF(kern_colmeta, attbyval); OFF(kern_colmeta, attalign); OFF(kern_colmeta, attlen);
+    OFF(kern_colmeta, attnum); OFF(kern_colmeta, attcacheoff); OFF(kern_colmeta, atttypid);
+    OFF(kern_colmeta, atttypmod); OFF(kern_colmeta, atttypkind); OFF(kern_colmeta, kds_format);
+    OFF(kern_colmeta, dtype_sizeof); OFF(kern_colmeta, kds_offset);
+    OFF(kern_colmeta, idx_subattrs); OFF(kern_colmeta, num_subattrs);
+    OFF(kern_colmeta, attname); OFF(kern_colmeta, attopts); OFF(kern_colmeta, field_index);
+    OFF(kern_colmeta, virtual_offset); OFF(kern_colmeta, nullmap_offset);
+    OFF(kern_colmeta, nullmap_length); OFF(kern_colmeta, values_offset);
+    OFF(kern_colmeta, values_length); OFF(kern_colmeta, extra_offset);
+    OFF(kern_colmeta, extra_length);
+
+    SZ(kern_data_store);
+    OFF(kern_data_store, length); OFF(kern_data_store, usage); OFF(kern_data_store, nitems);
+    OFF(kern_data_store, ncols); OFF(kern_data_store, format); OFF(kern_data_store, has_varlena);
+    OFF(kern_data_store, tdhasoid); OFF(kern_data_store, tdtypeid); OFF(kern_data_store, tdtypmod);
+    OFF(kern_data_store, table_oid); OFF(kern_data_store, hash_nslots);
+    OFF(kern_data_store, block_offset); OFF(kern_data_store, block_nloaded);
+    OFF(kern_data_store, column_nrooms); OFF(kern_data_store, arrow_virtual_usage);
+    OFF(kern_data_store, parquet_row_group); OFF(kern_data_store, parquet_parallel_load);
+    OFF(kern_data_store, parquet_cache_enabled); OFF(kern_data_store, nr_colmeta);
+    OFF(kern_data_store, colmeta);
+
+    SZ(kern_tupitem);
+    OFF(kern_tupitem, t_len); OFF(kern_tupitem, hash); OFF(kern_tupitem, has_xact_attrs);
+    OFF(kern_tupitem, t_infomask2); OFF(kern_tupitem, t_infomask); OFF(kern_tupitem, t_hoff);
+    OFF(kern_tupitem, t_bits);
+    emit("const:MINIMAL_TUPLE_OFFSET", (long long)MINIMAL_TUPLE_OFFSET);
+    SZ(kern_hashitem); OFF(kern_hashitem, next); OFF(kern_hashitem, t);
+    SZ(kern_fallbackitem); OFF(kern_fallbackitem, depth); OFF(kern_fallbackitem, matched);
+    OFF(kern_fallbackitem, t);
+
+    SZ(kern_varload_desc); OFF(kern_varload_desc, vl_slot_id);
+    SZ(kern_varmove_desc); OFF(kern_varmove_desc, vm_slot_id); OFF(kern_varmove_desc, vm_from_xdatum);
+    SZ(kern_varslot_desc);
+    OFF(kern_varslot_desc, vs_typbyval); OFF(kern_varslot_desc, vs_typalign);
+    OFF(kern_varslot_desc, vs_typlen); OFF(kern_varslot_desc, vs_typmod);
+    OFF(kern_varslot_desc, vs_offset); OFF(kern_varslot_desc, idx_subfield);
+    OFF(kern_varslot_desc, num_subfield); OFF(kern_varslot_desc, vs_ops);
+    SZ(kern_fallback_desc); OFF(kern_fallback_desc, fb_slot_id); OFF(kern_fallback_desc, fb_kvec_offset);
+    SZ(kern_aggregate_desc);
+    OFF(kern_aggregate_desc, arg0_slot_id); OFF(kern_aggregate_desc, arg1_slot_id);
+    OFF(kern_aggregate_desc, filter_slot_id); OFF(kern_aggregate_desc, typmod);
+
+    OFF(kern_expression, len); OFF(kern_expression, exptype); OFF(kern_expression, expr_ops);
+    OFF(kern_expression, expflags); OFF(kern_expression, opcode); OFF(kern_expression, fn_dptr);
+    OFF(kern_expression, nr_args); OFF(kern_expression, args_offset); OFF(kern_expression, u.data);
+    OFF(kern_expression, u.c.const_type); OFF(kern_expression, u.c.const_isnull);
+    OFF(kern_expression, u.c.const_value);
+    OFF(kern_expression, u.p.param_id); OFF(kern_expression, u.p.__data);
+    OFF(kern_expression, u.v.var_offset); OFF(kern_expression, u.v.var_slot_id);
+    OFF(kern_expression, u.v.__data);
+    OFF(kern_expression, u.casewhen.case_comp); OFF(kern_expression, u.casewhen.case_else);
+    OFF(kern_expression, u.casewhen.data);
+    OFF(kern_expression, u.load.depth); OFF(kern_expression, u.load.nitems);
+    OFF(kern_expression, u.load.desc);
+    OFF(kern_expression, u.move.depth); OFF(kern_expression, u.move.nitems);
+    OFF(kern_expression, u.move.desc);
+    OFF(kern_expression, u.save.sv_slot_id); OFF(kern_expression, u.save.data);
+    OFF(kern_expression, u.pagg.nattrs); OFF(kern_expression, u.pagg.desc);
+    OFF(kern_expression, u.proj.hash); OFF(kern_expression, u.proj.nattrs);
+    OFF(kern_expression, u.proj.slot_id);
+    OFF(kern_expression, u.pack.npacked); OFF(kern_expression, u.pack.offset);
+
+    SZ(kagg_state__pminmax_int64_packed); OFF(kagg_state__pminmax_int64_packed, value);
+    SZ(kagg_state__pminmax_fp64_packed); OFF(kagg_state__pminmax_fp64_packed, value);
+    SZ(kagg_state__psum_int_packed); OFF(kagg_state__psum_int_packed, sum);
+    SZ(kagg_state__psum_fp_packed); OFF(kagg_state__psum_fp_packed, sum);
+    SZ(kagg_state__psum_numeric_packed); OFF(kagg_state__psum_numeric_packed, sum);
+    SZ(kagg_state__stddev_packed); SZ(kagg_state__covar_packed);
+
+    SZ(kern_session_info);
+    OFF(kern_session_info, query_plan_id); OFF(kern_session_info, kcxt_kvars_nrooms);
+    OFF(kern_session_info, kcxt_kvars_nslots); OFF(kern_session_info, kcxt_kvars_defs);
+    OFF(kern_session_info, kcxt_kvecs_bufsz); OFF(kern_session_info, kcxt_kvecs_ndims);
+    OFF(kern_session_info, kcxt_extra_bufsz); OFF(kern_session_info, cuda_stack_size);
+    OFF(kern_session_info, xpu_task_flags); OFF(kern_session_info, optimal_gpus);
+    OFF(kern_session_info, xpucode_load_vars_packed); OFF(kern_session_info, xpucode_move_vars_packed);
+    OFF(kern_session_info, xpucode_scan_quals); OFF(kern_session_info, xpucode_join_quals_packed);
+    OFF(kern_session_info, xpucode_hash_values_packed); OFF(kern_session_info, xpucode_gist_evals_packed);
+    OFF(kern_session_info, xpucode_projection); OFF(kern_session_info, xpucode_groupby_keyhash);
+    OFF(kern_session_info, xpucode_groupby_keyload); OFF(kern_session_info, xpucode_groupby_keycomp);
+    OFF(kern_session_info, xpucode_groupby_actions); OFF(kern_session_info, xpucode_gpusort_keydesc);
+    OFF(kern_session_info, hostEpochTimestamp); OFF(kern_session_info, session_xact_state);
+    OFF(kern_session_info, session_timezone); OFF(kern_session_info, session_encode);
+    OFF(kern_session_info, session_currency_frac_digits);
+    OFF(kern_session_info, pinned_inner_buffer_partition_size);
+    OFF(kern_session_info, projection_kds_dst); OFF(kern_session_info, select_into_pathname);
+    OFF(kern_session_info, select_into_kds_head); OFF(kern_session_info, select_into_projdesc);
+    OFF(kern_session_info, pgsql_port_number); OFF(kern_session_info, pgsql_plan_node_id);
+    OFF(kern_session_info, join_inner_handle); OFF(kern_session_info, groupby_kds_final);
+    OFF(kern_session_info, groupby_prepfn_bufsz); OFF(kern_session_info, groupby_ngroups_estimation);
+    OFF(kern_session_info, gpusort_htup_margin); OFF(kern_session_info, gpusort_limit_count);
+    OFF(kern_session_info, fallback_kds_head); OFF(kern_session_info, fallback_desc_defs);
+    OFF(kern_session_info, fallback_desc_nitems); OFF(kern_session_info, nparams);
+    OFF(kern_session_info, poffset);
+
+    SZ(kern_exec_task); OFF(kern_exec_task, kds_src_pathname); OFF(kern_exec_task, kds_src_iovec);
+    OFF(kern_exec_task, kds_src_offset); OFF(kern_exec_task, data);
+    SZ(kern_exec_results);
+    OFF(kern_exec_results, chunks_offset); OFF(kern_exec_results, chunks_nitems);
+    OFF(kern_exec_results, ojmap_offset); OFF(kern_exec_results, ojmap_length);
+    OFF(kern_exec_results, right_outer_join); OFF(kern_exec_results, final_plan_task);
+    OFF(kern_exec_results, final_nitems); OFF(kern_exec_results, final_usage);
+    OFF(kern_exec_results, final_total); OFF(kern_exec_results, final_sorting_msec);
+    OFF(kern_exec_results, select_into_nblocks); OFF(kern_exec_results, npages_direct_read);
+    OFF(kern_exec_results, nitems_raw); OFF(kern_exec_results, nitems_in);
+    OFF(kern_exec_results, nitems_out); OFF(kern_exec_results, num_rels);
+    OFF(kern_exec_results, stats);
+
+    OFF(XpuCommand, magic); OFF(XpuCommand, tag); OFF(XpuCommand, length);
+    OFF(XpuCommand, priv); OFF(XpuCommand, chain); OFF(XpuCommand, u);
+
+    SZ(kern_multirels);
+    OFF(kern_multirels, length); OFF(kern_multirels, ojmap_sz); OFF(kern_multirels, kbuf_part_offset);
+    OFF(kern_multirels, num_inner_rels); OFF(kern_multirels, chunks);
+    OFF(kern_multirels, chunks[0].kds_offset); OFF(kern_multirels, chunks[0].ojmap_offset);
+    OFF(kern_multirels, chunks[0].gist_offset); OFF(kern_multirels, chunks[0].gist_ctid_resno);
+    OFF(kern_multirels, chunks[0].is_nestloop); OFF(kern_multirels, chunks[0].left_outer);
+    OFF(kern_multirels, chunks[0].right_outer); OFF(kern_multirels, chunks[0].pinned_buffer);
+    OFF(kern_multirels, chunks[0].buffer_id);
+    emit("sizeof:kern_multirels.chunks[0]", (long long)sizeof(((kern_multirels *)0)->chunks[0]));
+
+    /* opcode numbers */
+#define TYPE_OPCODE(NAME,a,b)   emit("type:" #NAME, (long long)TypeOpCode__##NAME);
+#include "xpu_opcodes.h"
+#undef TYPE_OPCODE
+#define FUNC_OPCODE(a,b,c,NAME,d,e)         emit("func:" #NAME, (long long)FuncOpCode__##NAME);
+#define DEVONLY_FUNC_OPCODE(a,NAME,b,c,d)   emit("func:" #NAME, (long long)FuncOpCode__##NAME);
+#include "xpu_opcodes.h"
+#undef FUNC_OPCODE
+#undef DEVONLY_FUNC_OPCODE
+    emit("func:ConstExpr", FuncOpCode__ConstExpr);
+    emit("func:ParamExpr", FuncOpCode__ParamExpr);
+    emit("func:VarExpr", FuncOpCode__VarExpr);
+    emit("func:BoolExpr_And", FuncOpCode__BoolExpr_And);
+    emit("func:BoolExpr_Or", FuncOpCode__BoolExpr_Or);
+    emit("func:BoolExpr_Not", FuncOpCode__BoolExpr_Not);
+    emit("func:NullTestExpr_IsNull", FuncOpCode__NullTestExpr_IsNull);
+    emit("func:NullTestExpr_IsNotNull", FuncOpCode__NullTestExpr_IsNotNull);
+    emit("func:BoolTestExpr_IsTrue", FuncOpCode__BoolTestExpr_IsTrue);
+    emit("func:BoolTestExpr_IsNotTrue", FuncOpCode__BoolTestExpr_IsNotTrue);
+    emit("func:BoolTestExpr_IsFalse", FuncOpCode__BoolTestExpr_IsFalse);
+    emit("func:BoolTestExpr_IsNotFalse", FuncOpCode__BoolTestExpr_IsNotFalse);
+    emit("func:BoolTestExpr_IsUnknown", FuncOpCode__BoolTestExpr_IsUnknown);
+    emit("func:BoolTestExpr_IsNotUnknown", FuncOpCode__BoolTestExpr_IsNotUnknown);
+    emit("func:DistinctFrom", FuncOpCode__DistinctFrom);
+    emit("func:CoalesceExpr", FuncOpCode__CoalesceExpr);
+    emit("func:LeastExpr", FuncOpCode__LeastExpr);
+    emit("func:GreatestExpr", FuncOpCode__GreatestExpr);
+    emit("func:CaseWhenExpr", FuncOpCode__CaseWhenExpr);
+    emit("func:LoadVars", FuncOpCode__LoadVars);
+    emit("func:MoveVars", FuncOpCode__MoveVars);
+    emit("func:JoinQuals", FuncOpCode__JoinQuals);
+    emit("func:HashValue", FuncOpCode__HashValue);
+    emit("func:GiSTEval", FuncOpCode__GiSTEval);
+    emit("func:SaveExpr", FuncOpCode__SaveExpr);
+    emit("func:AggFuncs", FuncOpCode__AggFuncs);
+    emit("func:Projection", FuncOpCode__Projection);
+    emit("func:SortKeys", FuncOpCode__SortKeys);
+    emit("func:Packed", FuncOpCode__Packed);
+    emit("const:KERN_EXPRESSION_MAGIC", KERN_EXPRESSION_MAGIC);
+    g_json += "}\n";
+    return g_json.c_str();
+}
+
+/* The reference's own hash of a byte string (xpu_common.cu:2920-3041). */
+extern "C" uint32_t
+ref_pg_hash_any(const void *ptr, int sz)
+{
+    return pg_hash_any(ptr, sz);
+}

@@ -1,0 +1,1125 @@
+/*
+ * xpu_compile.cpp - translation of a kern_session_info blob into the flat
+ * register programs executed by the MI355X pipeline kernel.
+ *
+ * Input contract: the blob the reference backend builds in
+ * pgstromBuildSessionInfo (executor.c:517-801) from the bytecode emitted by
+ * codegen.c.  Each kern_expression is checked the way __KEXP_IS_VALID
+ * (xpu_common.h:2697-2723) checks it (bounds and the per-node magic) before it
+ * is translated; anything this build cannot execute is rejected here, at
+ * session open, with ERRCODE_DEVICE_ERROR instead of failing inside a kernel.
+ */
+#include "xpu_compile.h"
+
+#include <stdio.h>
+#include <string.h>
+
+#include "xpu_opcodes_table.h"
+
+namespace {
+
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen };
+enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
+       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len };
+
+struct devfunc { int opcode; const char *name; int kind, op, nargs, t0, t1, res; };
+
+#define T_Invalid   TypeOpCode__Invalid
+#define T_bool      TypeOpCode__bool
+#define T_int1      TypeOpCode__int1
+#define T_int2      TypeOpCode__int2
+#define T_int4      TypeOpCode__int4
+#define T_int8      TypeOpCode__int8
+#define T_float4    TypeOpCode__float4
+#define T_float8    TypeOpCode__float8
+#define T_text      TypeOpCode__text
+#define DEVFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) \
+    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##RES },
+const devfunc devfunc_table[] = { XPU_FUNCTION_TABLE(DEVFUNC) };
+
+const devfunc *
+lookup_devfunc(int opcode)
+{
+    for (const devfunc &f : devfunc_table)
+        if (f.opcode == opcode)
+            return &f;
+    return nullptr;
+}
+
+int
+type_domain(int t)
+{
+    switch (t)
+    {
+        case TypeOpCode__bool: case TypeOpCode__int1: case TypeOpCode__int2:
+        case TypeOpCode__int4: case TypeOpCode__int8: case TypeOpCode__date:
+            return XD_INT;
+        case TypeOpCode__float4: return XD_F32;
+        case TypeOpCode__float8: return XD_F64;
+        case TypeOpCode__text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+            return XD_TEXT;
+    }
+    return -1;
+}
+
+int
+type_width(int t)
+{
+    switch (t)
+    {
+        case TypeOpCode__bool: case TypeOpCode__int1: return 1;
+        case TypeOpCode__int2: return 2;
+        case TypeOpCode__int4: case TypeOpCode__float4: case TypeOpCode__date: return 4;
+        case TypeOpCode__int8: case TypeOpCode__float8: return 8;
+    }
+    return -1;
+}
+
+int
+int_class(int t)
+{
+    switch (t)
+    {
+        case TypeOpCode__bool: return XT_BOOL;
+        case TypeOpCode__int1: return XT_INT1;
+        case TypeOpCode__int2: return XT_INT2;
+        case TypeOpCode__int4: case TypeOpCode__date: return XT_INT4;
+        default: return XT_INT8;
+    }
+}
+
+struct Reg { int r; int dom; int type; };
+
+struct Compiler
+{
+    const kern_session_info *s;
+    size_t      length;
+    xpu_compiled *out;
+    xplan      *P;
+    std::vector<xop> ops;
+    int         slot_reg[256];      /* -1 until first use */
+    int         slot_type[256];
+    int         nslots = 0;
+    int         slot_top = 3;       /* R0 TRUE, R1 result, R2 tuple */
+    int         temp_next = XP_NREGS;   /* temporaries grow down per segment */
+    int         temp_min = XP_NREGS;
+    bool        compare_nulls = false;
+    std::string err;
+
+    bool fail(const std::string &m)
+    {
+        if (err.empty())
+            err = m;
+        return false;
+    }
+
+    /* temporaries: allocated downward from the top of the register file and
+     * released at the end of each segment */
+    int alloc_reg(int n = 1)
+    {
+        temp_next -= n;
+        if (temp_next < slot_top)
+        {
+            fail("expression needs more than the per-lane register file");
+            temp_next = XP_NREGS - n;
+        }
+        if (temp_next < temp_min)
+            temp_min = temp_next;
+        return temp_next;
+    }
+
+    /* slot registers: allocated upward on first use, live for the task */
+    int slot_register(int slot)
+    {
+        if (slot_reg[slot] < 0)
+        {
+            int dom = type_domain(slot_type[slot]);
+            slot_reg[slot] = slot_top;
+            slot_top += (dom == XD_TEXT) ? 2 : 1;
+            if (slot_top > temp_next)
+                fail("expression needs more than the per-lane register file");
+        }
+        return slot_reg[slot];
+    }
+
+    int msg(const std::string &m)
+    {
+        for (size_t i = 0; i < out->messages.size(); i++)
+            if (out->messages[i] == m)
+                return (int)i;
+        out->messages.push_back(m);
+        return (int)out->messages.size() - 1;
+    }
+
+    void emit(int op, int dst, int a = 0, int b = 0, int guard = XP_REG_TRUE,
+              int aux = 0, int64_t imm = 0)
+    {
+        xop o;
+        memset(&o, 0, sizeof(o));
+        o.op = op;
+        o.dst = dst;
+        o.a = a;
+        o.b = b;
+        o.guard = guard;
+        o.aux = aux;
+        o.imm = imm;
+        ops.push_back(o);
+    }
+
+    xseg begin_seg()
+    {
+        xseg g;
+        g.begin = (uint16_t)ops.size();
+        g.end = g.begin;
+        temp_next = XP_NREGS;
+        return g;
+    }
+
+    void end_seg(xseg &g)
+    {
+        g.end = (uint16_t)ops.size();
+    }
+
+    /* ------------------------------------------------------------------ */
+    const kern_expression *kexp_at(uint32_t off)
+    {
+        if (off == 0)
+            return nullptr;
+        if (off + sizeof(kern_expression) > length)
+        {
+            fail("xpucode offset out of the session blob");
+            return nullptr;
+        }
+        return (const kern_expression *)((const char *)s + off);
+    }
+
+    bool check_node(const kern_expression *parent, const kern_expression *k)
+    {
+        const char *base = (const char *)s;
+        if ((const char *)k < base || (const char *)k + offsetof(kern_expression, u) > base + length)
+            return fail("kern_expression out of the session blob");
+        if (k->len < offsetof(kern_expression, u) + 4 ||
+            (const char *)k + k->len > base + length)
+            return fail("kern_expression has a broken length");
+        if (parent && (const char *)k + k->len > (const char *)parent + parent->len)
+            return fail("kern_expression argument overruns its parent");
+        uint32_t magic;
+        memcpy(&magic, (const char *)k + k->len - 4, 4);
+        uint32_t expect = KERN_EXPRESSION_MAGIC ^ ((uint32_t)k->exptype << 6) ^
+                          ((uint32_t)k->opcode << 14);
+        if (magic != expect)
+            return fail("kern_expression magic mismatch (corrupted xpucode)");
+        return true;
+    }
+
+    const kern_expression *first_arg(const kern_expression *k)
+    {
+        if (k->nr_args == 0 || k->args_offset == 0)
+            return nullptr;
+        return (const kern_expression *)((const char *)k + k->args_offset);
+    }
+
+    const kern_expression *next_arg(const kern_expression *a)
+    {
+        return (const kern_expression *)((const char *)a + XPU_MAXALIGN(a->len));
+    }
+
+    const kern_expression *packed_sub(const kern_expression *kp, int idx)
+    {
+        if (!kp || kp->opcode != FuncOpCode__Packed || idx < 0 ||
+            (uint32_t)idx > kp->u.pack.npacked)
+            return nullptr;
+        uint32_t off = kp->u.pack.offset[idx];
+        if (off == 0)
+            return nullptr;
+        const kern_expression *k = (const kern_expression *)((const char *)kp + off);
+        return check_node(kp, k) ? k : nullptr;
+    }
+
+    /* ------------------------------------------------------------------ */
+    Reg reg_of_slot(int slot)
+    {
+        int dom = type_domain(slot_type[slot]);
+        if (dom < 0)
+        {
+            fail("slot of a type this build does not execute");
+            return Reg{0, 0, 0};
+        }
+        return Reg{slot_register(slot), dom, slot_type[slot]};
+    }
+
+    Reg const_value(int type, bool isnull, const char *val)
+    {
+        int dom = type_domain(type);
+        if (dom < 0)
+        {
+            fail("constant of unsupported type");
+            return Reg{0, 0, 0};
+        }
+        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        if (isnull)
+        {
+            emit(XO_CONST, r, 0, 0, XP_REG_TRUE, 1, 0);
+            return Reg{r, dom, type};
+        }
+        int64_t imm = 0;
+        switch (type)
+        {
+            case TypeOpCode__bool: imm = *(const uint8_t *)val ? 1 : 0; break;
+            case TypeOpCode__int1: imm = *(const int8_t *)val; break;
+            case TypeOpCode__int2: { int16_t v; memcpy(&v, val, 2); imm = v; } break;
+            case TypeOpCode__int4: case TypeOpCode__date: { int32_t v; memcpy(&v, val, 4); imm = v; } break;
+            case TypeOpCode__int8: memcpy(&imm, val, 8); break;
+            case TypeOpCode__float4: { uint32_t v; memcpy(&v, val, 4); imm = v; } break;
+            case TypeOpCode__float8: memcpy(&imm, val, 8); break;
+            default:
+            {
+                /* varlena: the payload goes to the constant pool */
+                uint8_t b0 = (uint8_t)val[0];
+                const char *p;
+                uint32_t len;
+                if (b0 == 0x01 || (b0 & 0x03) == 0x02)
+                {
+                    fail("compressed or external constant");
+                    return Reg{r, dom, type};
+                }
+                if (b0 & 0x01)
+                {
+                    len = ((b0 >> 1) & 0x7f) - 1;
+                    p = val + 1;
+                }
+                else
+                {
+                    uint32_t h;
+                    memcpy(&h, val, 4);
+                    len = ((h >> 2) & 0x3fffffff) - 4;
+                    p = val + 4;
+                }
+                uint64_t off = out->const_pool.size();
+                out->const_pool.insert(out->const_pool.end(), p, p + len);
+                while (out->const_pool.size() % 8)
+                    out->const_pool.push_back(0);
+                emit(XO_CONST_TEXT, r, 0, 0, XP_REG_TRUE, 0, (int64_t)(off | ((uint64_t)len << 32)));
+                return Reg{r, dom, type};
+            }
+        }
+        emit(XO_CONST, r, 0, 0, XP_REG_TRUE, 0, imm);
+        return Reg{r, dom, type};
+    }
+
+    Reg to_f64(Reg x)
+    {
+        if (x.dom == XD_F64)
+            return x;
+        int r = alloc_reg();
+        emit(x.dom == XD_F32 ? XO_F32_F64 : XO_I_F64, r, x.r);
+        return Reg{r, XD_F64, TypeOpCode__float8};
+    }
+
+    /* compile expression k; errors in it are raised only where R[guard] */
+    Reg expr(const kern_expression *parent, const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        if (!err.empty() || !check_node(parent, k))
+            return bad;
+        switch (k->opcode)
+        {
+            case FuncOpCode__ConstExpr:
+                return const_value(k->exptype, k->u.c.const_isnull, k->u.c.const_value);
+            case FuncOpCode__ParamExpr:
+            {
+                uint32_t id = k->u.p.param_id;
+                if (id >= s->nparams)
+                {
+                    fail("ParamExpr refers to an undefined parameter");
+                    return bad;
+                }
+                uint32_t off = s->poffset[id];
+                if (off && off >= length)
+                {
+                    fail("parameter offset out of the session blob");
+                    return bad;
+                }
+                return const_value(k->exptype, off == 0, (const char *)s + off);
+            }
+            case FuncOpCode__VarExpr:
+            {
+                int slot = k->u.v.var_slot_id;
+                if (slot >= nslots)
+                {
+                    fail("VarExpr refers to an undefined slot");
+                    return bad;
+                }
+                return reg_of_slot(slot);
+            }
+            case FuncOpCode__SaveExpr:
+            {
+                int slot = k->u.save.sv_slot_id;
+                if (slot >= nslots || k->nr_args != 1)
+                {
+                    fail("malformed SaveExpr");
+                    return bad;
+                }
+                Reg v = expr(k, first_arg(k), guard);
+                Reg sr = reg_of_slot(slot);
+                if (v.r != sr.r)
+                {
+                    if (v.dom != sr.dom)
+                    {
+                        fail("SaveExpr type mismatch");
+                        return bad;
+                    }
+                    emit(sr.dom == XD_TEXT ? XO_MOV2 : XO_MOV, sr.r, v.r);
+                }
+                return sr;
+            }
+            case FuncOpCode__BoolExpr_And:
+            case FuncOpCode__BoolExpr_Or:
+            {
+                bool is_and = (k->opcode == FuncOpCode__BoolExpr_And);
+                int acc = alloc_reg();
+                int gr = (k->nr_args > 1) ? alloc_reg() : XP_REG_TRUE;
+                int mark = temp_next;
+                int g = guard;
+                emit(XO_CONST, acc, 0, 0, XP_REG_TRUE, 0, is_and ? 1 : 0);
+                const kern_expression *a = first_arg(k);
+                for (uint32_t i = 0; i < k->nr_args; i++, a = next_arg(a))
+                {
+                    Reg v = expr(k, a, g);
+                    emit(is_and ? XO_AND3 : XO_OR3, acc, acc, v.r);
+                    temp_next = mark;
+                    if (i + 1 < k->nr_args)
+                    {
+                        /* later arguments run only where the result is still open */
+                        emit(is_and ? XO_G_NOTFALSE : XO_G_NOTTRUE, gr, acc, g);
+                        g = gr;
+                    }
+                }
+                temp_next = acc;
+                return Reg{acc, XD_INT, TypeOpCode__bool};
+            }
+            case FuncOpCode__BoolExpr_Not:
+            {
+                int r = alloc_reg();
+                int mark = temp_next;
+                Reg v = expr(k, first_arg(k), guard);
+                emit(XO_NOT3, r, v.r);
+                temp_next = mark;
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            }
+            case FuncOpCode__NullTestExpr_IsNull:
+            case FuncOpCode__NullTestExpr_IsNotNull:
+            {
+                int r = alloc_reg();
+                int mark = temp_next;
+                Reg v = expr(k, first_arg(k), guard);
+                emit(k->opcode == FuncOpCode__NullTestExpr_IsNull ? XO_ISNULL : XO_ISNOTNULL, r, v.r);
+                temp_next = mark;
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            }
+            case FuncOpCode__BoolTestExpr_IsTrue:
+            case FuncOpCode__BoolTestExpr_IsNotTrue:
+            case FuncOpCode__BoolTestExpr_IsFalse:
+            case FuncOpCode__BoolTestExpr_IsNotFalse:
+            case FuncOpCode__BoolTestExpr_IsUnknown:
+            case FuncOpCode__BoolTestExpr_IsNotUnknown:
+            {
+                int r = alloc_reg();
+                int mark = temp_next;
+                Reg v = expr(k, first_arg(k), guard);
+                emit(XO_BOOLTEST, r, v.r, 0, XP_REG_TRUE,
+                     k->opcode - FuncOpCode__BoolTestExpr_IsTrue);
+                temp_next = mark;
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            }
+            case FuncOpCode__CoalesceExpr:
+            {
+                int dom = type_domain(k->exptype);
+                if (dom < 0)
+                {
+                    fail("COALESCE of unsupported type");
+                    return bad;
+                }
+                int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+                int gr = (k->nr_args > 1) ? alloc_reg() : XP_REG_TRUE;
+                int mark = temp_next;
+                int g = guard;
+                emit(XO_SETNULL, r);
+                const kern_expression *a = first_arg(k);
+                for (uint32_t i = 0; i < k->nr_args; i++, a = next_arg(a))
+                {
+                    Reg v = expr(k, a, g);
+                    if (v.dom != dom)
+                    {
+                        fail("COALESCE argument type mismatch");
+                        return bad;
+                    }
+                    emit(XO_COALESCE, r, v.r, dom == XD_TEXT ? 1 : 0);
+                    temp_next = mark;
+                    if (i + 1 < k->nr_args)
+                    {
+                        emit(XO_G_ISNULL, gr, r, g);
+                        g = gr;
+                    }
+                }
+                temp_next = r;
+                return Reg{r, dom, (int)k->exptype};
+            }
+            case FuncOpCode__HashValue:
+            {
+                int h = alloc_reg();
+                int mark = temp_next;
+                emit(XO_HASH_INIT, h);
+                const kern_expression *a = first_arg(k);
+                for (uint32_t i = 0; i < k->nr_args; i++, a = next_arg(a), temp_next = mark)
+                {
+                    Reg v = expr(k, a, guard);
+                    int w = (v.dom == XD_TEXT) ? 0 : type_width(a->exptype);
+                    if (v.dom != XD_TEXT && w < 0)
+                    {
+                        fail("hash key of unsupported type");
+                        return bad;
+                    }
+                    emit(XO_HASH_ADD, h, v.r, v.dom, XP_REG_TRUE, w);
+                }
+                emit(XO_HASH_FIN, h);
+                return Reg{h, XD_INT, TypeOpCode__int4};
+            }
+            default:
+                return func(k, guard);
+        }
+    }
+
+    Reg func(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        const devfunc *f = lookup_devfunc(k->opcode);
+        if (!f)
+        {
+            char buf[96];
+            snprintf(buf, sizeof(buf), "device function opcode %d is not supported", (int)k->opcode);
+            fail(buf);
+            return bad;
+        }
+        if ((int)k->nr_args != f->nargs)
+        {
+            fail(std::string(f->name) + ": wrong number of arguments");
+            return bad;
+        }
+        /* the result register comes first, so the arguments' temporaries
+         * are released once the operator has consumed them */
+        int r = alloc_reg();
+        int mark = temp_next;
+        Reg a[2];
+        const kern_expression *ka = first_arg(k);
+        for (int i = 0; i < f->nargs; i++, ka = next_arg(ka))
+            a[i] = expr(k, ka, guard);
+        if (!err.empty())
+            return bad;
+        Reg res = func_op(f, a, r, guard);
+        temp_next = mark;
+        return res;
+    }
+
+    Reg func_op(const devfunc *f, Reg *a, int r, int guard)
+    {
+        Reg bad{0, 0, 0};
+        int cmpop = -1;
+        switch (f->op)
+        {
+            case O_eq: cmpop = XC_EQ; break;
+            case O_ne: cmpop = XC_NE; break;
+            case O_lt: cmpop = XC_LT; break;
+            case O_le: cmpop = XC_LE; break;
+            case O_gt: cmpop = XC_GT; break;
+            case O_ge: cmpop = XC_GE; break;
+        }
+        int cn = compare_nulls ? 0x100 : 0;
+        std::string fname(f->name);
+        switch (f->kind)
+        {
+            case K_cmp:
+                if (a[0].dom == XD_INT && a[1].dom == XD_INT)
+                    emit(XO_CMP_I, r, a[0].r, a[1].r, XP_REG_TRUE, cmpop | cn);
+                else if (a[0].dom == XD_F32 && a[1].dom == XD_F32)
+                    emit(XO_CMP_F32, r, a[0].r, a[1].r, XP_REG_TRUE, cmpop | cn);
+                else
+                {
+                    Reg x = to_f64(a[0]), y = to_f64(a[1]);
+                    emit(XO_CMP_F64, r, x.r, y.r, XP_REG_TRUE, cmpop | cn);
+                }
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            case K_textcmp:
+                emit(XO_CMP_TEXT, r, a[0].r, a[1].r, XP_REG_TRUE, cmpop | cn);
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            case K_textlen:
+                emit(XO_TEXTLEN, r, a[0].r);
+                return Reg{r, XD_INT, TypeOpCode__int4};
+            case K_arith:
+                if (type_domain(f->res) == XD_INT)
+                {
+                    static const int iops[] = {0, 0, 0, 0, 0, 0, 0, XO_ADD_I, XO_SUB_I, XO_MUL_I,
+                                               XO_DIV_I, XO_MOD_I};
+                    bool divlike = (f->op == O_div || f->op == O_mod);
+                    int m = msg(fname + (divlike ? ": division by zero" : ": value out of range"));
+                    int m2 = msg(fname + ": value out of range");
+                    emit(iops[f->op], r, a[0].r, a[1].r, guard, int_class(f->res),
+                         (int64_t)m | ((int64_t)m2 << 16));
+                    return Reg{r, XD_INT, f->res};
+                }
+                if (f->res == TypeOpCode__float4)
+                {
+                    static const int fops[] = {0, 0, 0, 0, 0, 0, 0, XO_ADD_F32, XO_SUB_F32,
+                                               XO_MUL_F32, XO_DIV_F32};
+                    int m = msg(fname + ": value out of range");
+                    int m2 = msg(fname + ": division by zero");
+                    emit(fops[f->op], r, a[0].r, a[1].r, guard, 0, (int64_t)m | ((int64_t)m2 << 16));
+                    return Reg{r, XD_F32, TypeOpCode__float4};
+                }
+                else
+                {
+                    static const int dops[] = {0, 0, 0, 0, 0, 0, 0, XO_ADD_F64, XO_SUB_F64,
+                                               XO_MUL_F64, XO_DIV_F64};
+                    Reg x = to_f64(a[0]), y = to_f64(a[1]);
+                    int m = msg(fname + ": value out of range");
+                    int m2 = msg(fname + ": division by zero");
+                    emit(dops[f->op], r, x.r, y.r, guard, 0, (int64_t)m | ((int64_t)m2 << 16));
+                    return Reg{r, XD_F64, TypeOpCode__float8};
+                }
+            case K_unary:
+                if (a[0].dom == XD_INT)
+                {
+                    int op = f->op == O_up ? XO_MOV : f->op == O_um ? XO_NEG_I :
+                             f->op == O_abs ? XO_ABS_I : XO_NOT_I;
+                    emit(op, r, a[0].r, 0, guard, int_class(f->res), msg(fname + ": value out of range"));
+                    return Reg{r, XD_INT, f->res};
+                }
+                if (a[0].dom == XD_F32)
+                    emit(f->op == O_up ? XO_MOV : f->op == O_um ? XO_NEG_F32 : XO_ABS_F32, r, a[0].r);
+                else
+                    emit(f->op == O_up ? XO_MOV : f->op == O_um ? XO_NEG_F64 : XO_ABS_F64, r, a[0].r);
+                return Reg{r, a[0].dom, f->res};
+            case K_bitop:
+            {
+                int op = f->op == O_and ? XO_AND_I : f->op == O_or ? XO_OR_I :
+                         f->op == O_xor ? XO_XOR_I : f->op == O_shl ? XO_SHL_I : XO_SHR_I;
+                emit(op, r, a[0].r, a[1].r, XP_REG_TRUE, int_class(f->res));
+                return Reg{r, XD_INT, f->res};
+            }
+            case K_cast:
+            {
+                int sd = a[0].dom, dd = type_domain(f->res);
+                int m = msg(fname + ": out of range");
+                if (f->res == TypeOpCode__bool)
+                    emit(XO_I_BOOL, r, a[0].r);
+                else if (sd == XD_INT && dd == XD_INT)
+                    emit(XO_I2I, r, a[0].r, 0, guard, int_class(f->res), m);
+                else if (sd == XD_F32 && dd == XD_INT)
+                    emit(XO_F32_I, r, a[0].r, 0, guard, int_class(f->res), m);
+                else if (sd == XD_F64 && dd == XD_INT)
+                    emit(XO_F64_I, r, a[0].r, 0, guard, int_class(f->res), m);
+                else if (sd == XD_INT && dd == XD_F32)
+                    emit(XO_I_F32, r, a[0].r);
+                else if (sd == XD_INT && dd == XD_F64)
+                    emit(XO_I_F64, r, a[0].r);
+                else if (sd == XD_F32 && dd == XD_F64)
+                    emit(XO_F32_F64, r, a[0].r);
+                else if (sd == XD_F64 && dd == XD_F32)
+                    emit(XO_F64_F32, r, a[0].r, 0, guard, 0, m);
+                else
+                {
+                    fail(fname + ": unsupported cast");
+                    return bad;
+                }
+                return Reg{r, dd, f->res};
+            }
+        }
+        fail(fname + ": unsupported function kind");
+        return bad;
+    }
+
+    /* LoadVars of one depth */
+    bool loadvars(const kern_expression *kload, int depth, int r_tuple, bool nulls)
+    {
+        if (!kload)
+            return true;
+        if (kload->opcode != FuncOpCode__LoadVars)
+            return fail("LoadVars expected");
+        for (int i = 0; i < kload->u.load.nitems; i++)
+        {
+            int resno = kload->u.load.desc[i].vl_resno;
+            int slot = kload->u.load.desc[i].vl_slot_id;
+            if (slot < 0 || slot >= nslots || resno <= 0)
+                return fail("LoadVars descriptor out of range");
+            Reg sr = reg_of_slot(slot);
+            if (sr.dom < 0)
+                return fail("LoadVars of an unsupported type");
+            if (nulls)
+            {
+                emit(XO_SETNULL, sr.r);
+                continue;
+            }
+            if (depth == 0)
+            {
+                if (sr.dom == XD_TEXT)
+                    emit(XO_LDCOL_VAR, sr.r, 0, 0, XP_REG_TRUE, resno - 1);
+                else if (sr.type == TypeOpCode__bool)
+                    emit(XO_LDCOL_BOOL, sr.r, 0, 0, XP_REG_TRUE, resno - 1);
+                else
+                    emit(XO_LDCOL, sr.r, 0, sr.dom, XP_REG_TRUE, resno - 1, type_width(sr.type));
+            }
+            else
+                emit(XO_LDTUP, sr.r, r_tuple, sr.type, XP_REG_TRUE, depth, resno);
+        }
+        return true;
+    }
+
+    /* ExecGpuJoinQuals (xpu_common.cu:1442-1481) as a status register */
+    bool joinquals(const kern_expression *kj, int r_status, bool other_only)
+    {
+        emit(XO_CONST, r_status, 0, 0, XP_REG_TRUE, 0, 1);
+        if (!kj)
+            return true;
+        if (kj->opcode != FuncOpCode__JoinQuals)
+            return fail("JoinQuals expected");
+        int g = XP_REG_TRUE;
+        const kern_expression *a = first_arg(kj);
+        for (uint32_t i = 0; i < kj->nr_args; i++, a = next_arg(a))
+        {
+            if (!check_node(kj, a))
+                return false;
+            bool pushed = (a->expflags & KEXP_FLAG__IS_PUSHED_DOWN) != 0;
+            if (other_only && !pushed)
+                continue;
+            int gq = g;
+            if (pushed && !other_only)
+            {
+                /* pushed-down quals are skipped once the status is -1 */
+                gq = alloc_reg();
+                emit(XO_G_POSITIVE, gq, r_status, g);
+            }
+            Reg v = expr(kj, a, gq);
+            if (v.dom != XD_INT)
+                return fail("join qual is not boolean");
+            emit(XO_JQ_STATUS, r_status, v.r, gq, XP_REG_TRUE, 0,
+                 (pushed && !other_only) ? -1 : 0);
+            if (!pushed || other_only)
+            {
+                /* an unmatched plain join qual ends the evaluation */
+                int ng = alloc_reg();
+                emit(XO_G_NONZERO, ng, r_status, g);
+                g = ng;
+            }
+        }
+        return true;
+    }
+};
+
+int
+layout_state_size(int action)
+{
+    switch (action)
+    {
+        case KAGG_ACTION__NROWS_ANY:
+        case KAGG_ACTION__NROWS_COND:   return 8;
+        case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
+        case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
+            return sizeof(kagg_state__pminmax_int64_packed);
+        case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64:
+            return sizeof(kagg_state__pminmax_fp64_packed);
+        case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
+            return sizeof(kagg_state__psum_int_packed);
+        case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            return sizeof(kagg_state__psum_numeric_packed);
+        case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
+            return sizeof(kagg_state__psum_fp_packed);
+        case KAGG_ACTION__STDDEV:
+            return sizeof(kagg_state__stddev_packed);
+    }
+    return -1;
+}
+
+/* 8-byte LDS arrays an action needs in the partial table */
+int
+lds_arrays(int action)
+{
+    switch (action)
+    {
+        case KAGG_ACTION__NROWS_ANY: case KAGG_ACTION__NROWS_COND: return 1;
+        case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
+        case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
+        case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64: return 1;
+        case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT: return 2;
+        case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64: return 3;
+        case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP: return 2;
+        case KAGG_ACTION__STDDEV: return 3;
+    }
+    return -1;
+}
+
+}   /* namespace */
+
+void
+xpu_set_error(kern_errorbuf *eb, uint32_t errcode, const char *filename, int lineno,
+              const char *funcname, const char *message)
+{
+    if (!eb)
+        return;
+    memset(eb, 0, sizeof(*eb));
+    eb->errcode = errcode;
+    eb->lineno = lineno;
+    snprintf(eb->filename, sizeof(eb->filename), "%s", filename);
+    snprintf(eb->funcname, sizeof(eb->funcname), "%s", funcname);
+    snprintf(eb->message, sizeof(eb->message), "%s", message);
+}
+
+int
+xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out,
+                    kern_errorbuf *eb)
+{
+    Compiler C;
+    xplan *P = &out->plan;
+
+    memset(P, 0, sizeof(xplan));
+    C.s = s;
+    C.length = length;
+    C.out = out;
+    C.P = P;
+#define FAIL(msg) do { xpu_set_error(eb, ERRCODE_DEVICE_ERROR, "xpu_compile.cpp", __LINE__, \
+                                     __func__, (msg)); return ERRCODE_DEVICE_ERROR; } while (0)
+    if (length < offsetof(kern_session_info, poffset) ||
+        length < offsetof(kern_session_info, poffset) + 4ULL * s->nparams)
+        FAIL("session blob is too short");
+    if (s->gpusort_htup_margin != 0 || s->xpucode_gpusort_keydesc != 0)
+        FAIL("GPU-Sort is not supported by this build");
+    if (s->xpucode_gist_evals_packed != 0)
+        FAIL("GiST joins are not supported by this build");
+    if (s->select_into_kds_head != 0)
+        FAIL("SELECT INTO direct is not supported by this build");
+    P->task_kind = s->xpu_task_flags & DEVTASK__MASK;
+
+    /* kvars slots -> registers */
+    C.nslots = s->kcxt_kvars_nslots;
+    if (C.nslots > 256)
+        FAIL("too many kvars slots");
+    out->nslots = C.nslots;
+    if (C.nslots > 0)
+    {
+        if (!s->kcxt_kvars_defs ||
+            s->kcxt_kvars_defs + sizeof(kern_varslot_desc) * (uint64_t)C.nslots > length)
+            FAIL("kvars slot definitions out of the session blob");
+        const kern_varslot_desc *vd = (const kern_varslot_desc *)((const char *)s + s->kcxt_kvars_defs);
+        for (int i = 0; i < C.nslots; i++)
+        {
+            /* registers are given on first use; a slot of a type this build
+             * does not execute is an error only if something uses it */
+            C.slot_type[i] = vd[i].vs_type_code;
+            C.slot_reg[i] = -1;
+        }
+    }
+
+    /* structure of the pipeline */
+    const kern_expression *kload = C.kexp_at(s->xpucode_load_vars_packed);
+    const kern_expression *kjoin = C.kexp_at(s->xpucode_join_quals_packed);
+    const kern_expression *khash = C.kexp_at(s->xpucode_hash_values_packed);
+    if (kload && !C.check_node(nullptr, kload))
+        FAIL(C.err.c_str());
+    if (kjoin && !C.check_node(nullptr, kjoin))
+        FAIL(C.err.c_str());
+    if (khash && !C.check_node(nullptr, khash))
+        FAIL(C.err.c_str());
+    int nrels = 0;
+    if (kjoin)
+        nrels = (int)kjoin->u.pack.npacked - 1;
+    else if (khash)
+        nrels = (int)khash->u.pack.npacked - 1;
+    if (nrels < 0)
+        nrels = 0;
+    if (nrels > XP_MAX_DEPTH)
+        FAIL("too many inner relations");
+    P->nrels = nrels;
+    /* results are read by the kernel right after their segment, so they all
+     * share R1; the inner tuple pointer is R2 (a depth's inner values are in
+     * their slot registers once seg_join has run, and the probe hash is the
+     * hash stored in the matched item) */
+    P->r_scan_ok = XP_REG_RESULT;
+    P->r_hash = XP_REG_RESULT;
+    P->r_status = XP_REG_RESULT;
+    P->r_keyhash = XP_REG_RESULT;
+    P->r_tuple = XP_REG_TUPLE;
+
+    /* depth 0: loads + scan quals */
+    P->seg_scan = C.begin_seg();
+    if (!C.loadvars(C.packed_sub(kload, 0), 0, 0, false))
+        FAIL(C.err.c_str());
+    {
+        const kern_expression *kq = C.kexp_at(s->xpucode_scan_quals);
+        if (kq)
+        {
+            Reg v = C.expr(nullptr, kq, XP_REG_TRUE);
+            if (C.err.empty() && v.dom != XD_INT)
+                C.fail("scan quals are not boolean");
+            if (C.err.empty())
+                C.emit(XO_MOV, P->r_scan_ok, v.r);
+        }
+        else
+            C.emit(XO_CONST, P->r_scan_ok, 0, 0, XP_REG_TRUE, 0, 1);
+    }
+    C.end_seg(P->seg_scan);
+    if (!C.err.empty())
+        FAIL(C.err.c_str());
+
+    /* joins */
+    for (int d = 1; d <= nrels; d++)
+    {
+        const kern_expression *kh = C.packed_sub(khash, d);
+        const kern_expression *kj = C.packed_sub(kjoin, d);
+        if (!kh)
+            FAIL("only hash joins are supported (no hash keys for a depth)");
+        P->depth[d].seg_hash = C.begin_seg();
+        {
+            Reg h = C.expr(nullptr, kh, XP_REG_TRUE);
+            if (C.err.empty())
+                C.emit(XO_MOV, P->r_hash, h.r);
+        }
+        C.end_seg(P->depth[d].seg_hash);
+        P->depth[d].seg_join = C.begin_seg();
+        C.loadvars(C.packed_sub(kload, d), d, P->r_tuple, false);
+        C.joinquals(kj, P->r_status, false);
+        C.end_seg(P->depth[d].seg_join);
+        P->depth[d].seg_oj = C.begin_seg();
+        C.loadvars(C.packed_sub(kload, d), d, P->r_tuple, true);
+        C.joinquals(kj, P->r_status, true);
+        C.end_seg(P->depth[d].seg_oj);
+        if (!C.err.empty())
+            FAIL(C.err.c_str());
+    }
+
+    /* final stage */
+    const kern_expression *kproj = C.kexp_at(s->xpucode_projection);
+    const kern_expression *kactions = C.kexp_at(s->xpucode_groupby_actions);
+    if (s->projection_kds_dst)
+    {
+        if (s->projection_kds_dst + sizeof(kern_data_store) > length)
+            FAIL("projection_kds_dst out of the session blob");
+        const kern_data_store *h = (const kern_data_store *)((const char *)s + s->projection_kds_dst);
+        size_t hl = KDS_HEAD_LENGTH(h);
+        if (s->projection_kds_dst + hl > length)
+            FAIL("projection_kds_dst out of the session blob");
+        out->kds_dst_head.assign((const char *)h, (const char *)h + hl);
+    }
+    if (kactions)
+    {
+        /* ---- group-by ---- */
+        if (!C.check_node(nullptr, kactions) || kactions->opcode != FuncOpCode__AggFuncs)
+            FAIL("malformed AggFuncs");
+        if (!s->groupby_kds_final || s->groupby_kds_final + sizeof(kern_data_store) > length)
+            FAIL("group-by without kds_final header");
+        const kern_data_store *kf = (const kern_data_store *)((const char *)s + s->groupby_kds_final);
+        size_t hl = KDS_HEAD_LENGTH(kf);
+        if (s->groupby_kds_final + hl > length)
+            FAIL("kds_final header out of the session blob");
+        out->kds_final_head.assign((const char *)kf, (const char *)kf + hl);
+        out->has_groupby = true;
+        int nattrs = kactions->u.pagg.nattrs;
+        if (nattrs > XP_MAX_PROJ || nattrs > (int)kf->ncols)
+            FAIL("too many group-by attributes");
+        P->gb_nattrs = nattrs;
+
+        P->seg_final = C.begin_seg();
+        const kern_expression *kkh = C.kexp_at(s->xpucode_groupby_keyhash);
+        std::vector<int> key_slots;
+        if (kkh)
+        {
+            if (!C.check_node(nullptr, kkh) || kkh->opcode != FuncOpCode__HashValue)
+                FAIL("malformed group-by key hash");
+            Reg h = C.expr(nullptr, kkh, XP_REG_TRUE);
+            if (C.err.empty())
+                C.emit(XO_MOV, P->r_keyhash, h.r);
+        }
+        else
+            C.emit(XO_CONST, P->r_keyhash, 0, 0, XP_REG_TRUE, 0, 0);
+        /* aggregate arguments (SaveExpr children of AggFuncs) */
+        {
+            const kern_expression *a = C.first_arg(kactions);
+            for (uint32_t i = 0; i < kactions->nr_args; i++, a = C.next_arg(a))
+                C.expr(kactions, a, XP_REG_TRUE);
+        }
+        C.end_seg(P->seg_final);
+        if (!C.err.empty())
+            FAIL(C.err.c_str());
+
+        /* state layout (__writeOutOneTuplePreAgg, cuda_gpupreagg.cu:18-223) */
+        uint32_t off = 0;
+        bool seen_key = false;
+        int nkeys = 0, key_bits = 0;
+        bool lds_ok = true;
+        int lds_arr = 0;
+        for (int j = 0; j < nattrs; j++)
+        {
+            const kern_aggregate_desc *d = &kactions->u.pagg.desc[j];
+            const kern_colmeta *cm = &kf->colmeta[j];
+            xagg *ag = &P->aggs[j];
+            ag->action = d->action;
+            ag->arg_reg = 0xff;
+            ag->filter_reg = 0xff;
+            ag->key_index = 0xff;
+            if (d->arg0_slot_id >= 0)
+            {
+                if (d->arg0_slot_id >= C.nslots)
+                    FAIL("aggregate argument slot out of range");
+                Reg r = C.reg_of_slot(d->arg0_slot_id);
+                if (r.dom < 0)
+                    FAIL("aggregate argument of unsupported type");
+                ag->arg_reg = r.r;
+                ag->arg_domain = r.dom;
+            }
+            if (d->filter_slot_id >= 0)
+            {
+                if (d->filter_slot_id >= C.nslots)
+                    FAIL("aggregate filter slot out of range");
+                ag->filter_reg = C.slot_reg[d->filter_slot_id];
+            }
+            if (d->action == KAGG_ACTION__VREF)
+            {
+                if (nkeys >= XP_MAX_KEYS)
+                    FAIL("too many grouping keys");
+                if (d->arg0_slot_id < 0)
+                    FAIL("grouping key without a slot");
+                int t = C.slot_type[d->arg0_slot_id];
+                xattr *ka = &P->keys[nkeys];
+                ka->reg = ag->arg_reg;
+                ka->domain = ag->arg_domain;
+                ka->attalign = cm->attalign;
+                ka->width = (ka->domain == XD_TEXT) ? -1 : type_width(t);
+                if (ka->width > 0 && cm->attlen != ka->width)
+                    FAIL("grouping key width differs from the kds_final column");
+                if (ka->domain == XD_TEXT)
+                    lds_ok = false;
+                else
+                {
+                    P->key_bits[nkeys] = 8 * ka->width;
+                    key_bits += 8 * ka->width + 1;
+                }
+                ag->key_index = nkeys++;
+                seen_key = true;
+                continue;
+            }
+            if (d->action == KAGG_ACTION__VREF_NOKEY)
+                FAIL("VREF_NOKEY is not supported");
+            int sz = layout_state_size(d->action);
+            if (sz < 0)
+                FAIL("aggregate action not supported by this build");
+            if (seen_key)
+                FAIL("partial aggregates must precede the grouping keys in kds_final");
+            off = XPU_TYPEALIGN(cm->attalign, off);
+            if (off % 8 != 0)
+                FAIL("misaligned partial aggregate state");
+            ag->state_off = off;
+            ag->state_size = sz;
+            off += sz;
+            int la = lds_arrays(d->action);
+            if (la < 0)
+                lds_ok = false;
+            ag->lds_off = lds_arr;
+            lds_arr += la > 0 ? la : 0;
+        }
+        P->nkeys = nkeys;
+        P->gb_states_end = off;
+        P->gb_t_hoff = XPU_MAXALIGN(KERN_TUPITEM_T_BITS_OFFSET + XPU_BITMAPLEN(nattrs));
+        P->gb_lds_arrays = lds_arr;
+        P->lds_fast = (lds_ok && key_bits <= 63) ? 1 : 0;
+        out->lds_entry_bytes = 8 * (1 + lds_arr) + 4;
+
+        /* hash of the grouping keys from their registers (for flushing the
+         * LDS table, whose entries hold only the key values) */
+        P->seg_keyhash_regs = C.begin_seg();
+        {
+            int h = C.alloc_reg();
+            C.emit(XO_HASH_INIT, h);
+            for (int k = 0; k < nkeys; k++)
+                C.emit(XO_HASH_ADD, h, P->keys[k].reg, P->keys[k].domain, XP_REG_TRUE,
+                       P->keys[k].domain == XD_TEXT ? 0 : P->keys[k].width);
+            C.emit(XO_HASH_FIN, h);
+            C.emit(XO_MOV, P->r_keyhash, h);
+        }
+        C.end_seg(P->seg_keyhash_regs);
+        if (kf->format == KDS_FORMAT_HASH && kf->hash_nslots == 0)
+            FAIL("kds_final in HASH format without hash slots");
+        if (nkeys > 0 && kf->format != KDS_FORMAT_HASH)
+            FAIL("GROUP BY needs a KDS_FORMAT_HASH kds_final");
+    }
+    else if (kproj)
+    {
+        /* ---- projection ---- */
+        if (!C.check_node(nullptr, kproj) || kproj->opcode != FuncOpCode__Projection)
+            FAIL("malformed Projection");
+        if (out->kds_dst_head.empty())
+            FAIL("projection without kds_dst header");
+        const kern_data_store *h = (const kern_data_store *)out->kds_dst_head.data();
+        out->has_projection = true;
+        P->seg_final = C.begin_seg();
+        const kern_expression *a = C.first_arg(kproj);
+        for (uint32_t i = 0; i < kproj->nr_args; i++, a = C.next_arg(a))
+            C.expr(kproj, a, XP_REG_TRUE);
+        C.end_seg(P->seg_final);
+        if (!C.err.empty())
+            FAIL(C.err.c_str());
+        int nattrs = kproj->u.proj.nattrs;
+        if (nattrs > (int)h->ncols)
+            nattrs = h->ncols;
+        if (nattrs > XP_MAX_PROJ)
+            FAIL("too many projected attributes");
+        P->proj_nattrs = nattrs;
+        for (int j = 0; j < nattrs; j++)
+        {
+            int slot = kproj->u.proj.slot_id[j];
+            const kern_colmeta *cm = &h->colmeta[j];
+            xattr *pa = &P->proj[j];
+            if (slot >= C.nslots)
+            {
+                pa->reg = 0xff;     /* considered NULL (xpu_common.cu:1117) */
+                continue;
+            }
+            Reg r = C.reg_of_slot(slot);
+            if (r.dom < 0)
+                FAIL("projection of an unsupported type");
+            pa->reg = r.r;
+            pa->domain = r.dom;
+            pa->attalign = cm->attalign;
+            pa->width = (r.dom == XD_TEXT) ? -1 : type_width(r.type);
+            if (pa->width > 0 && cm->attlen != pa->width)
+                FAIL("projected value width differs from the kds_dst column");
+        }
+    }
+    else
+        FAIL("session has neither a projection nor a group-by");
+
+    if (C.ops.size() > XP_MAX_OPS)
+        FAIL("compiled program is too long");
+    if (!C.err.empty())
+        FAIL(C.err.c_str());
+    /* pick the register file: temporaries were numbered down from
+     * XP_NREGS; move them right above the slots' registers */
+    {
+        int ntemps = XP_NREGS - C.temp_min;
+        int nr = (C.slot_top + ntemps <= XP_NREGS_SMALL) ? XP_NREGS_SMALL : XP_NREGS;
+        if (C.slot_top + ntemps > XP_NREGS)
+            FAIL("expression needs more than the per-lane register file");
+        int shift = XP_NREGS - nr;
+        for (xop &o : C.ops)
+        {
+            if (o.dst >= C.temp_min) o.dst -= shift;
+            if (o.a >= C.temp_min) o.a -= shift;
+            if (o.b >= C.temp_min && o.op != XO_HASH_ADD && o.op != XO_LDCOL &&
+                o.op != XO_LDTUP && o.op != XO_COALESCE)
+                o.b -= shift;
+            if (o.guard >= C.temp_min) o.guard -= shift;
+        }
+        P->nregs = nr;
+    }
+    P->nops = C.ops.size();
+    memcpy(P->ops, C.ops.data(), sizeof(xop) * C.ops.size());
+    return 0;
+#undef FAIL
+}

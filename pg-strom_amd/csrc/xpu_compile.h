@@ -1,0 +1,33 @@
+/*
+ * xpu_compile.h - session blob -> xplan translation (host side).
+ */
+#ifndef XPU_COMPILE_H
+#define XPU_COMPILE_H
+#include <string>
+#include <vector>
+
+#include "pgstrom_xpu.h"
+#include "xpu_program.h"
+
+struct xpu_compiled
+{
+    xplan               plan;
+    std::vector<char>   const_pool;         /* text constants (payload bytes) */
+    std::vector<std::string> messages;      /* error texts, indexed by xop.imm */
+    std::vector<char>   kds_dst_head;       /* KDS_FORMAT_ROW header of kds_dst */
+    std::vector<char>   kds_final_head;     /* header of kds_final (HASH or ROW) */
+    bool                has_projection = false;
+    bool                has_groupby = false;
+    uint32_t            nslots = 0;
+    uint32_t            lds_entry_bytes = 0;    /* bytes per LDS group entry */
+};
+
+/* Validates and translates the session blob; returns 0 or an ERRCODE_*
+ * with the error buffer filled. */
+int xpu_compile_session(const kern_session_info *session, size_t length,
+                        xpu_compiled *out, kern_errorbuf *errbuf);
+
+void xpu_set_error(kern_errorbuf *eb, uint32_t errcode, const char *filename,
+                   int lineno, const char *funcname, const char *message);
+
+#endif

@@ -1,0 +1,76 @@
+/*
+ * xpu_kernels.h - launch parameters and the per-task control block shared by
+ * the HIP kernels (xpu_kernels.hip) and the host runtime (xpuhip_runtime.cpp).
+ */
+#ifndef XPU_KERNELS_H
+#define XPU_KERNELS_H
+#include <stdint.h>
+
+#include "pgstrom_xpu.h"
+#include "xpu_program.h"
+
+/* look-back tile status words: flag(2) | rows(30) | bytes/8 (32) */
+#define XLB_FLAG_SHIFT      62
+#define XLB_AGG             1ULL    /* tile total published */
+#define XLB_PREFIX          2ULL    /* inclusive prefix published */
+#define XLB_STOP            3ULL    /* this or an earlier tile overflowed / aborted */
+#define XLB_ROWS_MASK       0x3fffffffULL
+#define XLB_SPIN_LIMIT      (1U << 26)
+
+#define XLDS_EMPTY_KEY      0xffffffffffffffffULL
+#define XGB_LOCKED_SLOT     0xffffffffffffffffULL
+
+/* per-task control block in device memory (cleared before each launch) */
+typedef struct
+{
+    uint32_t    tile_ticket;        /* dynamic tile counter */
+    uint32_t    resume_tile;        /* first tile that did not fit, plus one (0: none) */
+    uint32_t    errcode;            /* first error of the task */
+    uint32_t    errmsg;             /* message id in the session's table */
+    uint32_t    nitems_in;          /* rows that passed the scan quals */
+    uint32_t    nitems_out;         /* rows reaching the final stage */
+    uint32_t    depth_out[XP_MAX_DEPTH + 1];
+    uint32_t    __pad;
+    uint64_t    out_nitems;         /* projection rows written */
+    uint64_t    out_usage;          /* projection bytes written */
+    uint64_t    lds_miss;           /* group-by rows sent straight to kds_final */
+    uint64_t    global_merges;      /* group-by merges into kds_final */
+} xpu_task_ctl;
+
+/* error messages the kernels raise themselves (beyond the compiled table) */
+#define XMSG_LOOKBACK_TIMEOUT   0xfff0
+#define XMSG_FINAL_NO_SPACE     0xfff1
+#define XMSG_TEXT_EXTERNAL      0xfff2
+#define XMSG_GROUP_RETRY        0xfff3
+#define XMSG_TILE_TOO_LARGE     0xfff4
+
+typedef struct
+{
+    const xplan *plan;
+    const kern_data_store *kds_src;     /* KDS_FORMAT_ARROW in device memory */
+    kern_data_store *kds_out;           /* kds_dst (projection) / kds_final (group-by) */
+    xpu_task_ctl *ctl;
+    uint64_t   *tile_status;
+    const char *const_pool;
+    uint32_t    row_begin;              /* first row of this launch */
+    uint32_t    row_end;                /* one past the last row */
+    uint32_t    rows_per_tile;
+    uint32_t    ntiles;
+    uint32_t    lds_nents;              /* group-by LDS table entries (power of 2; 0: off) */
+    uint32_t    __pad;
+} xpu_kparams;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* host-side launchers (defined in xpu_kernels.hip) */
+int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
+                        size_t lds_bytes, bool preagg, void *stream);
+int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
+int xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head,
+                            uint32_t head_len, uint64_t length, const xpu_task_ctl *ctl,
+                            void *stream);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+/*
+ * xpu_program.h - the compiled form of a PG-Strom session for the MI355X
+ * pipeline kernel (shared by the host compiler and the HIP kernels).
+ *
+ * At session open the kern_expression trees of the session blob (scan quals,
+ * per-depth hash values and join quals, projection, group-by key hash and
+ * aggregate arguments) are translated into straight-line "xop" programs over
+ * a small file of 64-bit per-lane registers with a per-lane NULL mask.  The
+ * kernel walks an op list with a wave-uniform program counter, so dispatch is
+ * scalar and the register file stays in VGPRs (dynamic indices are uniform);
+ * the reference instead calls device function pointers recursively
+ * (EXEC_KERN_EXPRESSION, xpu_common.h:2694) with per-thread alloca stacks.
+ *
+ * Short-circuit semantics of AND/OR/CASE (an argument that is never evaluated
+ * by the reference cannot raise an error) are kept with guard registers:
+ * every op that may raise an error names a guard register, and only lanes
+ * whose guard is true report the error.
+ */
+#ifndef XPU_PROGRAM_H
+#define XPU_PROGRAM_H
+#include <stdint.h>
+
+#define XP_NREGS            32      /* largest per-lane register file */
+#define XP_NREGS_SMALL      16      /* register file kept in VGPRs */
+#define XP_REG_RESULT       1       /* scan result / hash / join status / key hash */
+#define XP_REG_TUPLE        2       /* inner tuple being joined */
+#define XP_REG_TRUE         0       /* R0 is the constant TRUE (guard "none") */
+#define XP_MAX_DEPTH        8       /* inner relations */
+#define XP_MAX_PROJ         64      /* projected / group-by attributes */
+#define XP_MAX_KEYS         8
+#define XP_MAX_OPS          2048
+
+/* value domains of a register */
+enum {
+    XD_INT = 0,     /* bool, int1/2/4/8, date: sign-extended int64 */
+    XD_F32 = 1,     /* float4 bits in the low word */
+    XD_F64 = 2,     /* float8 bits */
+    XD_TEXT = 3,    /* pointer in R[r], length in R[r+1] */
+};
+
+/* comparison operators (aux low byte of XO_CMP*) */
+enum { XC_EQ = 0, XC_NE, XC_LT, XC_LE, XC_GT, XC_GE };
+
+/* integer result classes for range checks */
+enum { XT_BOOL = 0, XT_INT1, XT_INT2, XT_INT4, XT_INT8 };
+
+enum xop_code {
+    XO_NOP = 0,
+    /* loads (never raise errors) */
+    XO_LDCOL,           /* dst <- Arrow fixed-width column aux; imm = byte width,
+                           b = domain, null from the validity bitmap */
+    XO_LDCOL_BOOL,      /* dst <- Arrow bit-packed boolean column aux */
+    XO_LDCOL_VAR,       /* dst,dst+1 <- Arrow utf8/binary column aux */
+    XO_LDTUP,           /* dst <- attribute imm (1-based) of the minimal tuple whose
+                           pointer is in R[a], inner KDS of depth aux, b = type code;
+                           text occupies dst,dst+1; R[a] == 0 loads NULL */
+    XO_CONST,           /* dst <- imm; aux & 1: NULL */
+    XO_CONST_TEXT,      /* dst <- const pool base + (imm & 0xffffffff),
+                           dst+1 <- imm >> 32 */
+    XO_MOV,             /* dst <- R[a] with null flag */
+    XO_MOV2,            /* dst,dst+1 <- R[a],R[a+1] with null flag */
+    XO_SETNULL,         /* dst <- NULL */
+    /* comparisons: dst(bool) <- R[a] op R[b]; aux = op | (compare_nulls << 8) */
+    XO_CMP_I, XO_CMP_F32, XO_CMP_F64, XO_CMP_TEXT,
+    /* integer arithmetic with range check into class aux; guard */
+    XO_ADD_I, XO_SUB_I, XO_MUL_I, XO_DIV_I, XO_MOD_I,
+    XO_NEG_I, XO_ABS_I,
+    XO_AND_I, XO_OR_I, XO_XOR_I, XO_NOT_I, XO_SHL_I, XO_SHR_I,
+    /* float arithmetic with overflow/zero-divide checks; guard */
+    XO_ADD_F32, XO_SUB_F32, XO_MUL_F32, XO_DIV_F32,
+    XO_ADD_F64, XO_SUB_F64, XO_MUL_F64, XO_DIV_F64,
+    XO_NEG_F32, XO_ABS_F32, XO_NEG_F64, XO_ABS_F64,
+    /* conversions; aux = integer class of the target where relevant; guard */
+    XO_I2I, XO_F32_I, XO_F64_I, XO_I_F32, XO_I_F64, XO_F32_F64, XO_F64_F32, XO_I_BOOL,
+    XO_TEXTLEN,
+    /* three-valued logic */
+    XO_AND3,            /* dst <- R[a] AND R[b] */
+    XO_OR3,
+    XO_NOT3,            /* dst <- NOT R[a] */
+    XO_ISNULL, XO_ISNOTNULL,
+    XO_BOOLTEST,        /* aux = kind 0..5 (IsTrue .. IsNotUnknown) */
+    XO_COALESCE,        /* if dst is NULL: dst <- R[a] (with null flag); text aware via b */
+    /* guards: dst <- R[b] && predicate(R[a]) (never NULL) */
+    XO_G_NOTFALSE,      /* R[a] is NULL or true */
+    XO_G_NOTTRUE,       /* R[a] is NULL or false */
+    XO_G_ISNULL,        /* R[a] is NULL */
+    XO_G_POSITIVE,      /* R[a] > 0 */
+    XO_G_NONZERO,       /* R[a] != 0 */
+    /* join-qual status: if R[b] && (R[a] NULL or false): dst <- imm */
+    XO_JQ_STATUS,
+    /* hashing: dst <- 0xffffffff; dst <- merge(dst, hash(R[a])) unless NULL;
+       aux = hash width in bytes (0 = text); dst <- dst ^ 0xffffffff */
+    XO_HASH_INIT, XO_HASH_ADD, XO_HASH_FIN,
+    XO_NUM_OPS
+};
+
+typedef struct
+{
+    uint16_t    op;
+    uint8_t     dst;
+    uint8_t     a;
+    uint8_t     b;
+    uint8_t     guard;
+    uint16_t    aux;
+    int64_t     imm;
+} xop;      /* 16 bytes */
+
+/* a program segment [begin, end) of the op list */
+typedef struct
+{
+    uint16_t    begin;
+    uint16_t    end;
+} xseg;
+
+/* where one value lives and how it is written into a tuple */
+typedef struct
+{
+    uint8_t     reg;        /* register (first of two for text) */
+    uint8_t     domain;     /* XD_* */
+    int8_t      attalign;   /* destination column alignment */
+    int8_t      width;      /* fixed byte width, or -1 for varlena */
+} xattr;
+
+/* group-by aggregate actions, in kds_final column order */
+typedef struct
+{
+    uint16_t    action;     /* KAGG_ACTION__* */
+    uint8_t     arg_reg;    /* register of arg0 (0xff: none) */
+    uint8_t     arg_domain;
+    uint8_t     filter_reg; /* 0xff: none */
+    uint8_t     key_index;  /* VREF: index among the grouping keys */
+    uint16_t    state_size; /* bytes of the partial state in the tuple */
+    uint32_t    state_off;  /* offset of the state from the payload start */
+    int32_t     lds_off;    /* first LDS array of this action (in entries) */
+} xagg;
+
+/* the compiled plan, uploaded once per session */
+typedef struct
+{
+    uint32_t    task_kind;          /* DEVTASK__* */
+    uint32_t    nops;
+    uint32_t    nrels;
+    uint32_t    nregs;
+    xseg        seg_scan;           /* depth-0 loads + scan quals */
+    uint8_t     r_scan_ok;          /* bool register, or XP_REG_TRUE */
+    uint8_t     r_hash;             /* outer key hash of the depth being probed */
+    uint8_t     r_status;           /* join status: 1 / 0 / -1 as ExecGpuJoinQuals */
+    uint8_t     r_tuple;            /* pointer to the inner tuple being joined */
+    struct {
+        xseg    seg_hash;           /* outer key hash of this depth */
+        xseg    seg_join;           /* inner loads + join quals */
+        xseg    seg_oj;             /* NULL inner loads + pushed-down quals only,
+                                       for the LEFT OUTER null-extended row */
+        uint8_t left_outer;
+        uint8_t __pad1[7];
+        uint64_t inner_kds;         /* device address of the inner KDS_FORMAT_HASH */
+    } depth[XP_MAX_DEPTH + 1];      /* index 1..nrels */
+    xseg        seg_final;          /* projection or group-by argument programs */
+    /* projection */
+    uint32_t    proj_nattrs;
+    xattr       proj[XP_MAX_PROJ];
+    /* group-by */
+    uint8_t     r_keyhash;
+    uint8_t     nkeys;
+    uint8_t     lds_fast;           /* keys pack into 63 bits: LDS partial table */
+    uint8_t     __pad2;
+    uint32_t    gb_nattrs;
+    uint32_t    gb_t_hoff;          /* MAXALIGN(offsetof(t_bits) + bitmap) */
+    uint32_t    gb_states_end;      /* payload offset after the last state */
+    uint32_t    gb_lds_arrays;      /* 8-byte LDS arrays per entry (excl. key) */
+    xseg        seg_keyhash_regs;   /* key hash from the key registers only */
+    xattr       keys[XP_MAX_KEYS];
+    uint8_t     key_bits[XP_MAX_KEYS];  /* packed width of each key in bits */
+    xagg        aggs[XP_MAX_PROJ];
+    uint64_t    const_pool;         /* device address of text constants */
+    xop         ops[XP_MAX_OPS];
+} xplan;
+
+#endif /* XPU_PROGRAM_H */

@@ -1,0 +1,842 @@
+/*
+ * xpuhip_runtime.cpp - host side of the MI355X xPU library: the C ABI of
+ * include/xpuhip.h over the HIP runtime.
+ *
+ * The roles it takes over from gpu_service.c:
+ *   session open         gpuservHandleOpenSession (:3319-3410) + pointer
+ *                        resolution (:3156-3317): here the blob is compiled
+ *                        into an xplan and uploaded once;
+ *   inner buffers        the kmrels import (:2197-2354): copied to HBM once
+ *                        per session, dimension tables stay resident;
+ *   task exec            __gpuservLaunchGpuTaskExecKernel (:3759-4003) with
+ *                        its suspend/resume loop, here a resume position
+ *                        returned to the caller;
+ *   final write-back     gpuservHandleGpuTaskFinal (:5684-5882) and the
+ *                        HASH->ROW conversion (:2780-2818).
+ * Device memory comes from hipMalloc (coarse-grained HBM); control blocks are
+ * cleared with a fill kernel, never hipMemsetAsync, so a caller may capture
+ * the launch sequence in a hipGraph.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "xpuhip.h"
+#include "xpu_compile.h"
+#include "xpu_kernels.h"
+
+#define XERR(eb, code, msg) \
+    xpu_set_error((eb), (code), "xpuhip_runtime.cpp", __LINE__, __func__, (msg))
+
+namespace {
+
+struct Device
+{
+    bool        ready = false;
+    int         id = 0;
+    int         ncu = 0;
+    size_t      lds_per_block = 0;
+    std::string name;
+};
+
+Device      g_dev;
+std::mutex  g_dev_lock;
+thread_local double t_last_kernel_ms = 0.0;
+
+bool
+hip_ok(hipError_t rc, kern_errorbuf *eb, const char *what)
+{
+    if (rc == hipSuccess)
+        return true;
+    char buf[200];
+    snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(rc));
+    XERR(eb, ERRCODE_DEVICE_ERROR, buf);
+    return false;
+}
+
+int
+ensure_device(kern_errorbuf *eb)
+{
+    std::lock_guard<std::mutex> g(g_dev_lock);
+    if (g_dev.ready)
+    {
+        if (!hip_ok(hipSetDevice(g_dev.id), eb, "hipSetDevice"))
+            return ERRCODE_DEVICE_ERROR;
+        return 0;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no HIP device is available");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    hipDeviceProp_t prop;
+    if (!hip_ok(hipSetDevice(g_dev.id), eb, "hipSetDevice") ||
+        !hip_ok(hipGetDeviceProperties(&prop, g_dev.id), eb, "hipGetDeviceProperties"))
+        return ERRCODE_DEVICE_ERROR;
+    g_dev.ncu = prop.multiProcessorCount;
+    g_dev.lds_per_block = prop.sharedMemPerBlock;
+    g_dev.name = prop.gcnArchName;
+    g_dev.ready = true;
+    return 0;
+}
+
+/* per-call device scratch: control block, look-back words, staging */
+struct Scratch
+{
+    hipStream_t stream = nullptr;
+    hipEvent_t  ev0 = nullptr, ev1 = nullptr;
+    xpu_task_ctl *d_ctl = nullptr;
+    xpu_task_ctl *h_ctl = nullptr;      /* pinned */
+    uint64_t   *d_status = nullptr;
+    size_t      status_cap = 0;
+    char       *d_src = nullptr;
+    size_t      src_cap = 0;
+    char       *d_dst = nullptr;
+    size_t      dst_cap = 0;
+};
+
+bool
+scratch_create(Scratch *s, kern_errorbuf *eb)
+{
+    return hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), eb, "hipStreamCreate") &&
+           hip_ok(hipEventCreate(&s->ev0), eb, "hipEventCreate") &&
+           hip_ok(hipEventCreate(&s->ev1), eb, "hipEventCreate") &&
+           hip_ok(hipMalloc((void **)&s->d_ctl, sizeof(xpu_task_ctl)), eb, "hipMalloc") &&
+           hip_ok(hipHostMalloc((void **)&s->h_ctl, sizeof(xpu_task_ctl), hipHostMallocDefault),
+                  eb, "hipHostMalloc");
+}
+
+void
+scratch_destroy(Scratch *s)
+{
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->d_ctl) (void)hipFree(s->d_ctl);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    if (s->d_status) (void)hipFree(s->d_status);
+    if (s->d_src) (void)hipFree(s->d_src);
+    if (s->d_dst) (void)hipFree(s->d_dst);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+bool
+grow(char **p, size_t *cap, size_t need, kern_errorbuf *eb)
+{
+    if (*cap >= need)
+        return true;
+    if (*p)
+        (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t sz = need + (need >> 3);
+    if (!hip_ok(hipMalloc((void **)p, sz), eb, "hipMalloc"))
+        return false;
+    *cap = sz;
+    return true;
+}
+
+}   /* namespace */
+
+struct xpuhip_session
+{
+    xpu_compiled    comp;
+    xplan          *d_plan = nullptr;
+    char           *d_const_pool = nullptr;
+    kern_data_store *d_kds_dst_head = nullptr;
+    kern_data_store *d_kds_final = nullptr;
+    uint64_t        final_length = 0;
+    char           *d_kmrels = nullptr;
+    bool            inner_ready = false;
+    xpuhip_tuning   tuning;
+    std::mutex      lock;
+    std::vector<Scratch *> pool;
+    uint64_t        ntasks = 0;
+};
+
+namespace {
+
+Scratch *
+scratch_get(xpuhip_session *s, kern_errorbuf *eb)
+{
+    {
+        std::lock_guard<std::mutex> g(s->lock);
+        if (!s->pool.empty())
+        {
+            Scratch *x = s->pool.back();
+            s->pool.pop_back();
+            return x;
+        }
+    }
+    Scratch *x = new Scratch();
+    if (!scratch_create(x, eb))
+    {
+        scratch_destroy(x);
+        return nullptr;
+    }
+    return x;
+}
+
+void
+scratch_put(xpuhip_session *s, Scratch *x)
+{
+    std::lock_guard<std::mutex> g(s->lock);
+    s->pool.push_back(x);
+}
+
+uint32_t
+lds_arrays_of(const xplan *P)
+{
+    return P->gb_lds_arrays;
+}
+
+size_t
+lds_bytes_for(const xplan *P, uint32_t nents)
+{
+    if (nents == 0)
+        return 0;
+    return 8ULL * nents + 8ULL * ((nents + 1) / 2) + 8ULL * lds_arrays_of(P) * nents;
+}
+
+/* default launch shape per task kind (the autotuner overrides it) */
+void
+default_tuning(const xpuhip_session *s, xpuhip_tuning *t)
+{
+    const xplan *P = &s->comp.plan;
+    memset(t, 0, sizeof(*t));
+    t->block_size = 256;
+    if (s->comp.has_groupby)
+    {
+        t->rows_per_thread = 4;
+        t->blocks_per_cu = 2;
+        if (P->lds_fast)
+        {
+            uint32_t n = 2048;
+            while (n > 64 && lds_bytes_for(P, n) > 48 * 1024)
+                n >>= 1;
+            t->lds_nents = n;
+        }
+    }
+    else
+    {
+        t->rows_per_thread = 1;
+        t->blocks_per_cu = 8;
+    }
+}
+
+/* host image of the single group tuple of a no-GROUP-BY kds_final
+ * (states in their initial values, cuda_gpupreagg.cu:18-223) */
+std::vector<char>
+nogroup_tuple(const xplan *P, uint32_t *p_unit)
+{
+    uint32_t t_hoff = P->gb_t_hoff;
+    uint32_t t_len = t_hoff + P->gb_states_end + ROWID_SIZE;
+    uint32_t unit = XPU_TYPEALIGN(XPU_L1_CACHELINE_SZ, offsetof(kern_hashitem, t) + t_len);
+    std::vector<char> buf(unit, 0);
+    kern_hashitem *hi = (kern_hashitem *)buf.data();
+    char *tb = (char *)&hi->t;
+    uint16_t infomask = HEAP_HASNULL;
+    for (uint32_t j = 0; j < P->gb_nattrs; j++)
+    {
+        const xagg *ag = &P->aggs[j];
+        char *s = tb + t_hoff + ag->state_off;
+        hi->t.t_bits[j >> 3] |= (uint8_t)(1 << (j & 7));
+        int64_t v;
+        double d;
+        switch (ag->action)
+        {
+            case KAGG_ACTION__NROWS_ANY:
+            case KAGG_ACTION__NROWS_COND:
+                break;
+            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
+                *(int32_t *)s = ag->state_size << 2;
+                v = INT64_MAX; memcpy(s + 8, &v, 8);
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
+                *(int32_t *)s = ag->state_size << 2;
+                v = INT64_MIN; memcpy(s + 8, &v, 8);
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            case KAGG_ACTION__PMIN_FP64:
+                *(int32_t *)s = ag->state_size << 2;
+                d = 1.7976931348623157e308; memcpy(s + 8, &d, 8);
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            case KAGG_ACTION__PMAX_FP64:
+                *(int32_t *)s = ag->state_size << 2;
+                d = -1.7976931348623157e308; memcpy(s + 8, &d, 8);
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            default:
+                *(int32_t *)s = ag->state_size << 2;
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+        }
+    }
+    hi->t.t_len = t_len;
+    hi->t.t_infomask2 = (uint16_t)(P->gb_nattrs & HEAP_NATTS_MASK);
+    hi->t.t_infomask = infomask;
+    hi->t.t_hoff = (uint8_t)(t_hoff + MINIMAL_TUPLE_OFFSET);
+    *p_unit = unit;
+    return buf;
+}
+
+/* (re)initialises the session's kds_final on the device */
+int
+init_kds_final(xpuhip_session *s, kern_errorbuf *eb)
+{
+    const kern_data_store *head = (const kern_data_store *)s->comp.kds_final_head.data();
+    uint64_t head_len = s->comp.kds_final_head.size();
+    const xplan *P = &s->comp.plan;
+    std::vector<char> img(head_len);
+    memcpy(img.data(), head, head_len);
+    kern_data_store *h = (kern_data_store *)img.data();
+    uint64_t length = s->final_length;
+
+    h->length = length;
+    h->usage = 0;
+    h->nitems = 0;
+    if (P->nkeys == 0)
+    {
+        /* ROW-format kds_final holding the single group */
+        uint32_t unit;
+        std::vector<char> tup = nogroup_tuple(P, &unit);
+        h->format = KDS_FORMAT_ROW;
+        h->hash_nslots = 0;
+        h->nitems = 1;
+        h->usage = unit;
+        uint64_t rowidx = unit - offsetof(kern_hashitem, t);
+        if (!hip_ok(hipMemcpy(s->d_kds_final, img.data(), head_len, hipMemcpyHostToDevice), eb, "hipMemcpy") ||
+            !hip_ok(hipMemcpy((char *)s->d_kds_final + head_len, &rowidx, 8, hipMemcpyHostToDevice), eb, "hipMemcpy") ||
+            !hip_ok(hipMemcpy((char *)s->d_kds_final + length - unit, tup.data(), unit,
+                              hipMemcpyHostToDevice), eb, "hipMemcpy"))
+            return ERRCODE_DEVICE_ERROR;
+        return 0;
+    }
+    if (!hip_ok(hipMemcpy(s->d_kds_final, img.data(), head_len, hipMemcpyHostToDevice), eb, "hipMemcpy"))
+        return ERRCODE_DEVICE_ERROR;
+    xpu_launch_fill((char *)s->d_kds_final + head_len, 0, h->hash_nslots, nullptr);
+    if (!hip_ok(hipDeviceSynchronize(), eb, "kds_final init"))
+        return ERRCODE_DEVICE_ERROR;
+    return 0;
+}
+
+const char *
+message_of(const xpuhip_session *s, uint32_t msgid)
+{
+    switch (msgid)
+    {
+        case XMSG_LOOKBACK_TIMEOUT: return "tile prefix look-back did not complete";
+        case XMSG_FINAL_NO_SPACE: return "GpuPreAgg(GroupBy) - no space to write on kds_final";
+        case XMSG_TEXT_EXTERNAL: return "text datum is compressed or external";
+        case XMSG_GROUP_RETRY: return "group-by slot lock retry limit exceeded";
+        case XMSG_TILE_TOO_LARGE: return "one tile of rows does not fit into kds_dst";
+    }
+    if (msgid < s->comp.messages.size())
+        return s->comp.messages[msgid].c_str();
+    return "device error";
+}
+
+}   /* namespace */
+
+/* ====================================================================
+ * C ABI
+ * ==================================================================== */
+extern "C" int
+xpuhip_init(int device, kern_errorbuf *eb)
+{
+    {
+        std::lock_guard<std::mutex> g(g_dev_lock);
+        if (!g_dev.ready)
+            g_dev.id = device < 0 ? 0 : device;
+    }
+    return ensure_device(eb);
+}
+
+extern "C" void
+xpuhip_shutdown(void)
+{
+    std::lock_guard<std::mutex> g(g_dev_lock);
+    if (g_dev.ready)
+        (void)hipDeviceSynchronize();
+    g_dev.ready = false;
+}
+
+extern "C" int
+xpuhip_session_check(const kern_session_info *session, size_t length, kern_errorbuf *eb)
+{
+    xpu_compiled *c = new xpu_compiled();
+    int rc = xpu_compile_session(session, length, c, eb);
+    delete c;
+    return rc;
+}
+
+extern "C" int
+xpuhip_session_open(const kern_session_info *session, size_t length,
+                    xpuhip_session **p_sess, kern_errorbuf *eb)
+{
+    int rc;
+    *p_sess = nullptr;
+    if ((rc = ensure_device(eb)) != 0)
+        return rc;
+    xpuhip_session *s = new xpuhip_session();
+    if ((rc = xpu_compile_session(session, length, &s->comp, eb)) != 0)
+    {
+        delete s;
+        return rc;
+    }
+    default_tuning(s, &s->tuning);
+    /* plan and constants */
+    size_t pool_len = s->comp.const_pool.size() ? s->comp.const_pool.size() : 8;
+    if (!hip_ok(hipMalloc((void **)&s->d_const_pool, pool_len), eb, "hipMalloc") ||
+        !hip_ok(hipMalloc((void **)&s->d_plan, sizeof(xplan)), eb, "hipMalloc"))
+    {
+        xpuhip_session_close(s);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!s->comp.const_pool.empty() &&
+        !hip_ok(hipMemcpy(s->d_const_pool, s->comp.const_pool.data(), s->comp.const_pool.size(),
+                          hipMemcpyHostToDevice), eb, "hipMemcpy"))
+    {
+        xpuhip_session_close(s);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    s->comp.plan.const_pool = (uint64_t)(uintptr_t)s->d_const_pool;
+    if (!s->comp.kds_dst_head.empty())
+    {
+        if (!hip_ok(hipMalloc((void **)&s->d_kds_dst_head, s->comp.kds_dst_head.size()), eb, "hipMalloc") ||
+            !hip_ok(hipMemcpy(s->d_kds_dst_head, s->comp.kds_dst_head.data(), s->comp.kds_dst_head.size(),
+                              hipMemcpyHostToDevice), eb, "hipMemcpy"))
+        {
+            xpuhip_session_close(s);
+            return ERRCODE_DEVICE_ERROR;
+        }
+    }
+    if (s->comp.has_groupby)
+    {
+        const kern_data_store *fh = (const kern_data_store *)s->comp.kds_final_head.data();
+        uint64_t len = fh->length;
+        uint64_t need = s->comp.kds_final_head.size() + 8ULL * fh->hash_nslots + (1ULL << 20);
+        if (len < need)
+            len = need + (64ULL << 20);
+        len &= ~(uint64_t)(XPU_L1_CACHELINE_SZ - 1);
+        s->final_length = len;
+        if (!hip_ok(hipMalloc((void **)&s->d_kds_final, len), eb, "hipMalloc(kds_final)") ||
+            (rc = init_kds_final(s, eb)) != 0)
+        {
+            xpuhip_session_close(s);
+            return rc ? rc : ERRCODE_DEVICE_ERROR;
+        }
+    }
+    if (!hip_ok(hipMemcpy(s->d_plan, &s->comp.plan, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy"))
+    {
+        xpuhip_session_close(s);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    s->inner_ready = (s->comp.plan.nrels == 0);
+    *p_sess = s;
+    return 0;
+}
+
+extern "C" int
+xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, size_t length,
+                            kern_errorbuf *eb)
+{
+    xplan *P = &s->comp.plan;
+    if (!kmrels || length < offsetof(kern_multirels, chunks) ||
+        kmrels->num_inner_rels < P->nrels ||
+        length < offsetof(kern_multirels, chunks) + sizeof(kern_multirels_chunk) * kmrels->num_inner_rels)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "kern_multirels does not match the session's joins");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    for (uint32_t d = 1; d <= P->nrels; d++)
+    {
+        const kern_multirels_chunk *ch = &kmrels->chunks[d - 1];
+        if (ch->kds_offset == 0 || ch->kds_offset + sizeof(kern_data_store) > length)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "inner KDS out of kern_multirels");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        const kern_data_store *k = (const kern_data_store *)((const char *)kmrels + ch->kds_offset);
+        if (k->format != KDS_FORMAT_HASH || k->hash_nslots == 0)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "only KDS_FORMAT_HASH inner relations are supported");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        if (ch->kds_offset + k->length > length || k->length >= (1ULL << 32))
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "inner KDS is truncated or larger than 4GB");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        if (ch->is_nestloop || ch->right_outer || ch->gist_offset)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "nest-loop, GiST and RIGHT OUTER joins are not supported");
+            return ERRCODE_DEVICE_ERROR;
+        }
+    }
+    if (s->d_kmrels)
+        (void)hipFree(s->d_kmrels);
+    s->d_kmrels = nullptr;
+    if (!hip_ok(hipMalloc((void **)&s->d_kmrels, length), eb, "hipMalloc(kmrels)") ||
+        !hip_ok(hipMemcpy(s->d_kmrels, kmrels, length, hipMemcpyHostToDevice), eb, "hipMemcpy(kmrels)"))
+        return ERRCODE_DEVICE_ERROR;
+    for (uint32_t d = 1; d <= P->nrels; d++)
+    {
+        P->depth[d].inner_kds = (uint64_t)(uintptr_t)(s->d_kmrels + kmrels->chunks[d - 1].kds_offset);
+        P->depth[d].left_outer = kmrels->chunks[d - 1].left_outer ? 1 : 0;
+    }
+    if (!hip_ok(hipMemcpy(s->d_plan, P, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy(plan)"))
+        return ERRCODE_DEVICE_ERROR;
+    s->inner_ready = true;
+    return 0;
+}
+
+static int
+task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, uint32_t nitems,
+                 kern_data_store *d_dst, size_t dst_length, uint64_t *resume_pos,
+                 kern_exec_results *res, kern_errorbuf *eb)
+{
+    const xplan *P = &s->comp.plan;
+    bool preagg = s->comp.has_groupby;
+    xpuhip_tuning t = s->tuning;
+    uint64_t row_begin = resume_pos ? *resume_pos : 0;
+
+    if (!s->inner_ready)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "inner relations are not attached");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg && (!d_dst || dst_length < s->comp.kds_dst_head.size() + 64 ||
+                    dst_length % XPU_MAXIMUM_ALIGNOF != 0 || dst_length >= (32ULL << 30)))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "projection needs a MAXALIGN'ed kds_dst buffer below 32GB");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (res)
+    {
+        memset(res, 0, sizeof(kern_exec_results));
+        res->num_rels = P->nrels;
+    }
+    if (row_begin >= nitems)
+    {
+        if (resume_pos)
+            *resume_pos = 0;
+        if (!preagg)
+        {
+            xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
+            xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                    dst_length, x->d_ctl, x->stream);
+            if (!hip_ok(hipStreamSynchronize(x->stream), eb, "hipStreamSynchronize"))
+                return ERRCODE_DEVICE_ERROR;
+        }
+        return 0;
+    }
+    uint32_t block = t.block_size ? t.block_size : 256;
+    uint32_t rpt = preagg ? (t.rows_per_thread ? t.rows_per_thread : 1) : 1;
+    uint32_t rows_per_tile = block * rpt;
+    uint64_t nrows = nitems - row_begin;
+    uint32_t ntiles = (uint32_t)((nrows + rows_per_tile - 1) / rows_per_tile);
+    uint32_t grid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 2);
+    if (grid > ntiles)
+        grid = ntiles;
+    uint32_t lds_nents = (preagg && P->lds_fast) ? t.lds_nents : 0;
+    size_t lds = lds_bytes_for(P, lds_nents);
+    if (lds > g_dev.lds_per_block)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "group-by LDS table exceeds the LDS of a block");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg && !grow((char **)&x->d_status, &x->status_cap, 8ULL * ntiles, eb))
+        return ERRCODE_DEVICE_ERROR;
+
+    xpu_kparams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.plan = s->d_plan;
+    kp.kds_src = d_src;
+    kp.kds_out = preagg ? s->d_kds_final : d_dst;
+    kp.ctl = x->d_ctl;
+    kp.tile_status = x->d_status;
+    kp.const_pool = s->d_const_pool;
+    kp.row_begin = (uint32_t)row_begin;
+    kp.row_end = nitems;
+    kp.rows_per_tile = rows_per_tile;
+    kp.ntiles = ntiles;
+    kp.lds_nents = lds_nents;
+
+    int rc = 0;
+    xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
+    if (!preagg)
+    {
+        /* kds_dst header first, so the kernel sees the capacity */
+        xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                dst_length, x->d_ctl, x->stream);
+        xpu_launch_fill(x->d_status, 0, ntiles, x->stream);
+    }
+    (void)hipEventRecord(x->ev0, x->stream);
+    rc = xpu_launch_pipeline(&kp, P->nregs, grid, block, lds, preagg, x->stream);
+    (void)hipEventRecord(x->ev1, x->stream);
+    if (rc != 0)
+    {
+        hip_ok((hipError_t)rc, eb, "pipeline kernel launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg)
+        xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                dst_length, x->d_ctl, x->stream);
+    (void)hipMemcpyAsync(x->h_ctl, x->d_ctl, sizeof(xpu_task_ctl), hipMemcpyDeviceToHost, x->stream);
+    if (!hip_ok(hipStreamSynchronize(x->stream), eb, "pipeline kernel"))
+        return ERRCODE_DEVICE_ERROR;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, x->ev0, x->ev1);
+    t_last_kernel_ms = ms;
+
+    const xpu_task_ctl *c = x->h_ctl;
+    if (res)
+    {
+        res->nitems_raw = (uint32_t)nrows;
+        res->nitems_in = c->nitems_in;
+        res->nitems_out = c->nitems_out;
+        for (uint32_t d = 1; d <= P->nrels && d <= 8; d++)
+            res->stats[d - 1].nitems_out = c->depth_out[d];
+    }
+    if (c->errcode && c->errcode != ERRCODE_SUSPEND_NO_SPACE)
+    {
+        XERR(eb, c->errcode, message_of(s, c->errmsg));
+        return (int)c->errcode;
+    }
+    if (c->errcode == ERRCODE_SUSPEND_NO_SPACE)
+    {
+        /* kds_final is full: not resumable in this build */
+        XERR(eb, ERRCODE_DEVICE_ERROR, message_of(s, c->errmsg));
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg && c->resume_tile != 0)
+    {
+        /* resume_tile holds (first tile that did not fit) + 1 */
+        uint64_t next = row_begin + (uint64_t)(c->resume_tile - 1) * rows_per_tile;
+        if (resume_pos)
+            *resume_pos = next;
+        XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "GpuProjection - no space to write");
+        return ERRCODE_SUSPEND_NO_SPACE;
+    }
+    if (resume_pos)
+        *resume_pos = 0;
+    return 0;
+}
+
+extern "C" int
+xpuhip_task_exec_device(xpuhip_session *s, const kern_data_store *d_kds_src, uint32_t src_nitems,
+                        kern_data_store *d_kds_dst, size_t dst_length, uint64_t *resume_pos,
+                        kern_exec_results *res, kern_errorbuf *eb)
+{
+    int rc;
+    if (!s)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no session");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if ((rc = ensure_device(eb)) != 0)
+        return rc;
+    Scratch *x = scratch_get(s, eb);
+    if (!x)
+        return ERRCODE_DEVICE_ERROR;
+    rc = task_exec_common(s, x, d_kds_src, src_nitems, d_kds_dst, dst_length, resume_pos, res, eb);
+    scratch_put(s, x);
+    return rc;
+}
+
+extern "C" int
+xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_length,
+                 kern_data_store *kds_dst, size_t dst_length, uint64_t *resume_pos,
+                 kern_exec_results *res, kern_errorbuf *eb)
+{
+    int rc;
+    if (!s || !kds_src || src_length < sizeof(kern_data_store) || kds_src->length > src_length)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "invalid kds_src");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (kds_src->format != KDS_FORMAT_ARROW)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "this build reads KDS_FORMAT_ARROW chunks only");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if ((rc = ensure_device(eb)) != 0)
+        return rc;
+    Scratch *x = scratch_get(s, eb);
+    if (!x)
+        return ERRCODE_DEVICE_ERROR;
+    bool preagg = s->comp.has_groupby;
+    if (!grow(&x->d_src, &x->src_cap, kds_src->length, eb) ||
+        (!preagg && !grow(&x->d_dst, &x->dst_cap, dst_length, eb)))
+    {
+        scratch_put(s, x);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!hip_ok(hipMemcpyAsync(x->d_src, kds_src, kds_src->length, hipMemcpyHostToDevice, x->stream),
+                eb, "hipMemcpyAsync(kds_src)"))
+    {
+        scratch_put(s, x);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    rc = task_exec_common(s, x, (const kern_data_store *)x->d_src, kds_src->nitems,
+                          preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
+                          resume_pos, res, eb);
+    if (!preagg && (rc == 0 || rc == ERRCODE_SUSPEND_NO_SPACE))
+    {
+        /* header + row index, then the tuples packed at the tail */
+        size_t head_len = s->comp.kds_dst_head.size();
+        kern_data_store h;
+        if (!hip_ok(hipMemcpy(&h, x->d_dst, sizeof(kern_data_store), hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+            rc = ERRCODE_DEVICE_ERROR;
+        else
+        {
+            size_t front = head_len + 8ULL * h.nitems;
+            if (!hip_ok(hipMemcpy(kds_dst, x->d_dst, front, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+                (h.usage > 0 &&
+                 !hip_ok(hipMemcpy((char *)kds_dst + dst_length - h.usage,
+                                   x->d_dst + dst_length - h.usage, h.usage,
+                                   hipMemcpyDeviceToHost), eb, "hipMemcpy")))
+                rc = ERRCODE_DEVICE_ERROR;
+        }
+    }
+    scratch_put(s, x);
+    return rc;
+}
+
+extern "C" int64_t
+xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
+                  kern_exec_results *res, kern_errorbuf *eb)
+{
+    if (!s || !s->comp.has_groupby)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "session has no group-by final buffer");
+        return -1;
+    }
+    if (ensure_device(eb) != 0)
+        return -1;
+    size_t head_len = s->comp.kds_final_head.size();
+    std::vector<char> hbuf(head_len);
+    if (!hip_ok(hipMemcpy(hbuf.data(), s->d_kds_final, head_len, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return -1;
+    kern_data_store *h = (kern_data_store *)hbuf.data();
+    uint64_t nitems = h->nitems, usage = h->usage, nslots = h->hash_nslots;
+    uint64_t need = head_len + 8 * nitems + usage;
+    if (res)
+    {
+        memset(res, 0, sizeof(kern_exec_results));
+        res->final_plan_task = true;
+        res->final_nitems = (uint32_t)nitems;
+        res->final_usage = usage;
+        res->final_total = need;
+    }
+    if (!out)
+        return (int64_t)need;
+    if (length < need)
+    {
+        XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "kds_final buffer is too small");
+        return -1;
+    }
+    /* HASH -> ROW on the wire (gpu_service.c:2780-2818) */
+    h->format = KDS_FORMAT_ROW;
+    h->hash_nslots = 0;
+    h->length = need;
+    memcpy(out, hbuf.data(), head_len);
+    if (!hip_ok(hipMemcpy((char *)out + head_len, (char *)s->d_kds_final + head_len + 8 * nslots,
+                          8 * nitems, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+        (usage > 0 &&
+         !hip_ok(hipMemcpy((char *)out + need - usage, (char *)s->d_kds_final + s->final_length - usage,
+                           usage, hipMemcpyDeviceToHost), eb, "hipMemcpy")))
+        return -1;
+    return (int64_t)need;
+}
+
+extern "C" void
+xpuhip_session_close(xpuhip_session *s)
+{
+    if (!s)
+        return;
+    for (Scratch *x : s->pool)
+        scratch_destroy(x);
+    s->pool.clear();
+    if (s->d_plan) (void)hipFree(s->d_plan);
+    if (s->d_const_pool) (void)hipFree(s->d_const_pool);
+    if (s->d_kds_dst_head) (void)hipFree(s->d_kds_dst_head);
+    if (s->d_kds_final) (void)hipFree(s->d_kds_final);
+    if (s->d_kmrels) (void)hipFree(s->d_kmrels);
+    delete s;
+}
+
+extern "C" int
+xpuhip_session_set_tuning(xpuhip_session *s, const xpuhip_tuning *t)
+{
+    if (!s || !t)
+        return ERRCODE_DEVICE_ERROR;
+    xpuhip_tuning d;
+    default_tuning(s, &d);
+    xpuhip_tuning n = *t;
+    if (n.block_size == 0) n.block_size = d.block_size;
+    if (n.rows_per_thread == 0) n.rows_per_thread = d.rows_per_thread;
+    if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
+    if (n.lds_nents == 0) n.lds_nents = d.lds_nents;
+    if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > 512 ||
+        (n.lds_nents & (n.lds_nents - 1)) != 0 || n.rows_per_thread > 64)
+        return ERRCODE_DEVICE_ERROR;
+    s->tuning = n;
+    return 0;
+}
+
+extern "C" int
+xpuhip_session_get_tuning(xpuhip_session *s, xpuhip_tuning *t)
+{
+    if (!s || !t)
+        return ERRCODE_DEVICE_ERROR;
+    *t = s->tuning;
+    return 0;
+}
+
+extern "C" int
+xpuhip_device_alloc(size_t length, void **p, kern_errorbuf *eb)
+{
+    if (ensure_device(eb) != 0)
+        return ERRCODE_DEVICE_ERROR;
+    return hip_ok(hipMalloc(p, length), eb, "hipMalloc") ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" int
+xpuhip_device_free(void *p)
+{
+    return hipFree(p) == hipSuccess ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" int
+xpuhip_memcpy_htod(void *d, const void *h, size_t len)
+{
+    return hipMemcpy(d, h, len, hipMemcpyHostToDevice) == hipSuccess ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" int
+xpuhip_memcpy_dtoh(void *h, const void *d, size_t len)
+{
+    return hipMemcpy(h, d, len, hipMemcpyDeviceToHost) == hipSuccess ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" int
+xpuhip_device_synchronize(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" double
+xpuhip_last_kernel_ms(void)
+{
+    return t_last_kernel_ms;
+}

@@ -1,0 +1,399 @@
+"""Builders and parsers for kern_data_store buffers (host side).
+
+Formats follow the reference (src/xpu_common.h):
+  * KDS_FORMAT_ARROW  (:1536-1628): columnar, Arrow validity bitmaps and
+    values/offsets buffers addressed by kern_colmeta offsets from the KDS head;
+  * KDS_FORMAT_ROW    (:771-798): row-index of u64 offsets counted from the
+    KDS tail, kern_tupitem payloads packed towards the tail;
+  * KDS_FORMAT_HASH   (:1383-1451): hash slots of u64 kern_hashitem offsets
+    (from the tail) in front of the row-index, chains through hitem->next,
+    built the way gpu_join.c:2128-2164 lays out an inner relation.
+"""
+import ctypes as C
+import struct
+
+import numpy as np
+
+from . import abi
+
+# name -> (oid, typlen, typbyval, typalign, TypeOpCode)
+PG_TYPES = {
+    "bool": (16, 1, True, 1, 1),
+    "int1": (0, 1, True, 1, 2),
+    "int2": (21, 2, True, 2, 3),
+    "int4": (23, 4, True, 4, 4),
+    "int8": (20, 8, True, 8, 5),
+    "float4": (700, 4, True, 4, 7),
+    "float8": (701, 8, True, 8, 8),
+    "numeric": (1700, -1, False, 4, 9),
+    "bytea": (17, -1, False, 4, 10),
+    "text": (25, -1, False, 4, 11),
+    "date": (1082, 4, True, 4, 13),
+}
+
+NP_DTYPES = {"bool": np.bool_, "int1": np.int8, "int2": np.int16, "int4": np.int32,
+             "int8": np.int64, "float4": np.float32, "float8": np.float64,
+             "date": np.int32}
+
+
+class ArrowVarlena:
+    """Text column already in Arrow form: int64 offsets[n+1] and data bytes."""
+
+    def __init__(self, offsets, data):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.data = data
+
+    def __len__(self):
+        return len(self.offsets) - 1
+
+    def __getitem__(self, i):
+        return self.data[self.offsets[i]:self.offsets[i + 1]]
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]
+
+
+class Column:
+    """One column of a table: typed numpy values plus an optional NULL mask
+    (True = NULL).  Text columns hold a list/array of bytes objects."""
+
+    def __init__(self, name, typname, values, nulls=None):
+        self.name = name
+        self.typname = typname
+        if typname in ("text", "bytea"):
+            if isinstance(values, ArrowVarlena):
+                self.values = values
+            else:
+                self.values = list(values)
+        else:
+            self.values = np.ascontiguousarray(values, dtype=NP_DTYPES[typname])
+        n = len(self.values)
+        if nulls is not None:
+            nulls = np.ascontiguousarray(nulls, dtype=np.bool_)
+            assert len(nulls) == n
+            if not nulls.any():
+                nulls = None
+        self.nulls = nulls
+
+    def __len__(self):
+        return len(self.values)
+
+    def isnull(self, i):
+        return self.nulls is not None and bool(self.nulls[i])
+
+
+def _fill_colmeta(cm, kds_offset, fmt, j, name, typname, attcacheoff):
+    oid, typlen, byval, align, _ = PG_TYPES[typname]
+    cm.attbyval = byval
+    cm.attalign = align
+    cm.attlen = typlen
+    cm.attnum = j + 1
+    cm.attcacheoff = attcacheoff
+    cm.atttypid = oid
+    cm.atttypmod = -1
+    cm.atttypkind = abi.TYPE_KIND__BASE
+    cm.kds_format = fmt
+    cm.kds_offset = kds_offset
+    cm.attname = name.encode()[:63]
+    cm.field_index = j
+
+
+def kds_header(fmt, coldefs, length=0, hash_nslots=0):
+    """Header of a KDS (struct + colmeta[]) as a bytearray of
+    KDS_HEAD_LENGTH bytes.  coldefs: list of (name, typname)."""
+    ncols = len(coldefs)
+    head_sz = abi.kds_head_length(ncols)
+    buf = bytearray(head_sz)
+    head = abi.kern_data_store_head.from_buffer(buf)
+    head.length = length
+    head.usage = 0
+    head.nitems = 0
+    head.ncols = ncols
+    head.format = fmt
+    head.tdtypeid = 0
+    head.tdtypmod = -1
+    head.hash_nslots = hash_nslots
+    head.nr_colmeta = ncols
+    attcacheoff = 0 if fmt in (abi.KDS_FORMAT_ROW, abi.KDS_FORMAT_HASH,
+                               abi.KDS_FORMAT_BLOCK) else -1
+    for j, (name, typname) in enumerate(coldefs):
+        off = abi.KDS_COLMETA_OFFSET + j * C.sizeof(abi.kern_colmeta)
+        cm = abi.kern_colmeta.from_buffer(buf, off)
+        _, typlen, _, align, _ = PG_TYPES[typname]
+        if typlen == -1:
+            head.has_varlena = True
+        # relscan.c:200-228 attcacheoff rule
+        if attcacheoff < 0:
+            co = -1
+        elif typlen > 0:
+            co = abi.typealign(align, attcacheoff)
+            attcacheoff = co + typlen
+        else:
+            a = abi.typealign(align, attcacheoff)
+            co = a if a == attcacheoff else -1
+            attcacheoff = -1
+        _fill_colmeta(cm, off, fmt, j, name, typname, co)
+    return buf
+
+
+def build_arrow_kds(columns):
+    """Columnar KDS (KDS_FORMAT_ARROW) holding `columns` (list of Column).
+    Returns a bytes-like numpy uint8 array."""
+    n = len(columns[0]) if columns else 0
+    coldefs = [(c.name, c.typname) for c in columns]
+    head = kds_header(abi.KDS_FORMAT_ARROW, coldefs)
+    pieces = []
+    pos = abi.typealign(64, len(head))
+    layout = []
+    for c in columns:
+        ent = {}
+        if c.nulls is not None:
+            valid = np.packbits(~c.nulls, bitorder="little")
+            ent["nullmap"] = (pos, valid.tobytes())
+            pos = abi.typealign(64, pos + len(valid))
+        if c.typname in ("text", "bytea"):
+            if isinstance(c.values, ArrowVarlena):
+                offs = c.values.offsets
+                data = c.values.data
+            else:
+                lens = np.fromiter((0 if (c.nulls is not None and c.nulls[i]) else len(v)
+                                    for i, v in enumerate(c.values)), dtype=np.int64, count=n)
+                offs = np.zeros(n + 1, dtype=np.int64)
+                np.cumsum(lens, out=offs[1:])
+                data = b"".join(v if not (c.nulls is not None and c.nulls[i]) else b""
+                                for i, v in enumerate(c.values))
+            assert offs[-1] < 2**31
+            ent["values"] = (pos, offs.astype(np.int32).tobytes())
+            pos = abi.typealign(64, pos + 4 * (n + 1))
+            ent["extra"] = (pos, data)
+            pos = abi.typealign(64, pos + len(data))
+        elif c.typname == "bool":
+            bits = np.packbits(c.values, bitorder="little")
+            ent["values"] = (pos, bits.tobytes())
+            pos = abi.typealign(64, pos + len(bits))
+        else:
+            raw = c.values.tobytes()
+            ent["values"] = (pos, raw)
+            pos = abi.typealign(64, pos + len(raw))
+        layout.append(ent)
+    total = pos
+    out = np.zeros(total, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(bytes(head), dtype=np.uint8)
+    hd = abi.kern_data_store_head.from_buffer(out)
+    hd.length = total
+    hd.nitems = n
+    hd.usage = 0
+    for j, (c, ent) in enumerate(zip(columns, layout)):
+        cm = abi.kern_colmeta.from_buffer(out, abi.KDS_COLMETA_OFFSET +
+                                          j * C.sizeof(abi.kern_colmeta))
+        t = c.typname
+        if t in ("int1", "int2", "int4", "int8", "date"):
+            cm.attopts.tag = abi.ArrowType__Int
+            cm.attopts.unitsz = PG_TYPES[t][1]
+            cm.attopts.u.integer.bitWidth = PG_TYPES[t][1] * 8
+            cm.attopts.u.integer.is_signed = 1
+        elif t in ("float4", "float8"):
+            cm.attopts.tag = abi.ArrowType__FloatingPoint
+            cm.attopts.unitsz = PG_TYPES[t][1]
+            cm.attopts.u.floating_point.precision = (
+                abi.ArrowPrecision__Single if t == "float4" else abi.ArrowPrecision__Double)
+        elif t in ("text", "bytea"):
+            cm.attopts.tag = abi.ArrowType__Utf8 if t == "text" else 4
+            cm.attopts.unitsz = 4
+        elif t == "bool":
+            cm.attopts.tag = abi.ArrowType__Bool
+        for key in ("nullmap", "values", "extra"):
+            if key in ent:
+                off, raw = ent[key]
+                out[off:off + len(raw)] = np.frombuffer(raw, dtype=np.uint8)
+                setattr(cm, key + "_offset", off)
+                setattr(cm, key + "_length", len(raw))
+    return out
+
+
+# ----------------------------------------------------------------------
+# datum encoding in heap/minimal tuples
+# ----------------------------------------------------------------------
+_FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
+        "float4": "<f", "float8": "<d", "date": "<i"}
+
+
+def encode_datum(typname, value):
+    if typname in ("text", "bytea"):
+        v = bytes(value)
+        return struct.pack("<I", (len(v) + 4) << 2) + v
+    return struct.pack(_FMT[typname], value)
+
+
+def form_tuple_payload(coldefs, values):
+    """(t_hoff, t_infomask, body bytes) of a minimal tuple holding values
+    (None = NULL), in the layout of kern_form_minimal_tuple
+    (xpu_common.cu:1096-1190)."""
+    natts = len(coldefs)
+    hasnull = any(v is None for v in values)
+    t_hoff = abi.T_BITS_OFFSET + (abi.bitmaplen(natts) if hasnull else 0)
+    t_hoff = abi.maxalign(t_hoff)
+    infomask = abi.HEAP_HASNULL if hasnull else 0
+    body = bytearray()
+    bits = bytearray(abi.bitmaplen(natts))
+    pos = t_hoff
+    for j, ((_, t), v) in enumerate(zip(coldefs, values)):
+        if v is None:
+            continue
+        align = PG_TYPES[t][3]
+        nxt = abi.typealign(align, pos)
+        body += b"\0" * (nxt - pos)
+        d = encode_datum(t, v)
+        body += d
+        if PG_TYPES[t][1] == -1:
+            infomask |= abi.HEAP_HASVARWIDTH
+        bits[j >> 3] |= 1 << (j & 7)
+        pos = nxt + len(d)
+    return t_hoff, infomask, bits if hasnull else bytearray(), bytes(body)
+
+
+def form_tupitem(coldefs, values, rowid, hash_value=0):
+    """A complete kern_tupitem (header, null bitmap, payload, row-id)."""
+    t_hoff, infomask, bits, body = form_tuple_payload(coldefs, values)
+    t_len = t_hoff + len(body) + abi.ROWID_SIZE
+    buf = bytearray(t_len)
+    struct.pack_into("<IIBBHHB", buf, 0, t_len, hash_value, 0, 0,
+                     len(coldefs) & abi.HEAP_NATTS_MASK, infomask,
+                     t_hoff + abi.MINIMAL_TUPLE_OFFSET)
+    buf[abi.T_BITS_OFFSET:abi.T_BITS_OFFSET + len(bits)] = bits
+    buf[t_hoff:t_hoff + len(body)] = body
+    struct.pack_into("<I", buf, t_len - 4, rowid)
+    return bytes(buf)
+
+
+def build_hash_kds(coldefs, rows, hashes):
+    """Inner relation as KDS_FORMAT_HASH, laid out like
+    __innerPreloadSetupHashBuffer (gpu_join.c:2128-2164): items are packed
+    from the tail, each slot holds the offset (from the tail) of the newest
+    kern_hashitem, and hitem->next chains to the previous head."""
+    n = len(rows)
+    nslots = abi.kds_get_hashslot_width(n)
+    head = kds_header(abi.KDS_FORMAT_HASH, coldefs, hash_nslots=nslots)
+    items = [form_tupitem(coldefs, r, i, h) for i, (r, h) in enumerate(zip(rows, hashes))]
+    usage = sum(abi.maxalign(8 + len(t)) for t in items)
+    length = len(head) + 8 * (nslots + n) + usage
+    length = abi.maxalign(length)
+    out = np.zeros(length, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(bytes(head), dtype=np.uint8)
+    hd = abi.kern_data_store_head.from_buffer(out)
+    hd.length = length
+    hd.nitems = n
+    hd.usage = usage
+    body = len(head)
+    slots = np.zeros(nslots, dtype=np.uint64)
+    rowidx = np.zeros(n, dtype=np.uint64)
+    curr = length
+    for i, (t, h) in enumerate(zip(items, hashes)):
+        curr -= abi.maxalign(8 + len(t))
+        self_off = length - curr
+        hidx = h % nslots
+        nxt = int(slots[hidx])
+        slots[hidx] = self_off
+        struct.pack_into("<Q", out, curr, nxt)
+        out[curr + 8:curr + 8 + len(t)] = np.frombuffer(t, dtype=np.uint8)
+        rowidx[i] = length - (curr + 8)
+    out[body:body + 8 * nslots] = slots.view(np.uint8)
+    out[body + 8 * nslots:body + 8 * (nslots + n)] = rowidx.view(np.uint8)
+    return out
+
+
+def build_multirels(inners):
+    """kern_multirels holding the inner KDS buffers (list of
+    (kds_bytes, left_outer)) one after another, as gpu_join.c lays out the
+    preload buffer (kds_offset counted from the kmrels head)."""
+    nrels = len(inners)
+    head_sz = abi.maxalign(abi.KMRELS_CHUNKS_OFFSET +
+                           C.sizeof(abi.kern_multirels_chunk) * nrels)
+    offs = []
+    pos = head_sz
+    for kds, _ in inners:
+        offs.append(pos)
+        pos = abi.typealign(64, pos + len(kds))
+    out = np.zeros(pos, dtype=np.uint8)
+    hd = abi.kern_multirels_head.from_buffer(out)
+    hd.length = pos
+    hd.num_inner_rels = nrels
+    for d, ((kds, left_outer), off) in enumerate(zip(inners, offs)):
+        ch = abi.kern_multirels_chunk.from_buffer(
+            out, abi.KMRELS_CHUNKS_OFFSET + d * C.sizeof(abi.kern_multirels_chunk))
+        ch.kds_offset = off
+        ch.left_outer = bool(left_outer)
+        out[off:off + len(kds)] = np.frombuffer(bytes(kds), dtype=np.uint8)
+    return out
+
+
+# ----------------------------------------------------------------------
+# parsers
+# ----------------------------------------------------------------------
+def kds_colmeta(buf):
+    hd = abi.kern_data_store_head.from_buffer_copy(bytes(buf[:C.sizeof(abi.kern_data_store_head)]))
+    cms = []
+    for j in range(hd.nr_colmeta):
+        off = abi.KDS_COLMETA_OFFSET + j * C.sizeof(abi.kern_colmeta)
+        cms.append(abi.kern_colmeta.from_buffer_copy(bytes(buf[off:off + C.sizeof(abi.kern_colmeta)])))
+    return hd, cms
+
+
+def _decode_datum(cm, raw, pos):
+    """(value, new_pos) of the attribute at pos (alignment already applied)."""
+    if cm.attlen == -1:
+        hdr = raw[pos]
+        if hdr & 0x01:      # 1-byte header
+            ln = (hdr >> 1) & 0x7f
+            return bytes(raw[pos + 1:pos + ln]), pos + ln
+        ln = (struct.unpack_from("<I", raw, pos)[0] >> 2) & 0x3fffffff
+        return bytes(raw[pos + 4:pos + ln]), pos + ln
+    fmt = {1: "<b", 2: "<h", 4: "<i", 8: "<q"}[cm.attlen]
+    if cm.atttypid == 700:
+        fmt = "<f"
+    elif cm.atttypid == 701:
+        fmt = "<d"
+    elif cm.atttypid == 16:
+        fmt = "<?"
+    return struct.unpack_from(fmt, raw, pos)[0], pos + cm.attlen
+
+
+def decode_tupitem(cms, raw, off):
+    """Decode the kern_tupitem at `off` of `raw`: returns (values, rowid, hash)."""
+    t_len, hashv, _, _, infomask2, infomask, t_hoff = struct.unpack_from("<IIBBHHB", raw, off)
+    natts = min(infomask2 & abi.HEAP_NATTS_MASK, len(cms))
+    hasnull = bool(infomask & abi.HEAP_HASNULL)
+    bits = raw[off + abi.T_BITS_OFFSET: off + abi.T_BITS_OFFSET + abi.bitmaplen(natts)]
+    pos = off + t_hoff - abi.MINIMAL_TUPLE_OFFSET
+    vals = []
+    for j in range(natts):
+        cm = cms[j]
+        if hasnull and not (bits[j >> 3] & (1 << (j & 7))):
+            vals.append(None)
+            continue
+        if cm.attlen > 0 or raw[pos] == 0:
+            pos = off + abi.typealign(cm.attalign, pos - off)
+        v, pos = _decode_datum(cm, raw, pos)
+        vals.append(v)
+    rowid = struct.unpack_from("<I", raw, off + t_len - 4)[0]
+    return vals, rowid, hashv
+
+
+def parse_row_kds(buf):
+    """List of (values, rowid) of a KDS_FORMAT_ROW (or FALLBACK-less HASH)
+    buffer, in row-index order."""
+    raw = memoryview(bytes(buf))
+    hd, cms = kds_colmeta(raw)
+    head_sz = abi.kds_head_length(hd.nr_colmeta)
+    ridx = np.frombuffer(raw, dtype=np.uint64, count=hd.nitems,
+                         offset=head_sz + 8 * hd.hash_nslots)
+    out = []
+    for i in range(hd.nitems):
+        o = int(ridx[i])
+        if o == 0:
+            out.append(None)
+            continue
+        vals, rowid, _ = decode_tupitem(cms, raw, hd.length - o)
+        out.append((tuple(vals), rowid))
+    return out

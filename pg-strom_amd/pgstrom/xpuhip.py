@@ -1,0 +1,198 @@
+"""ctypes binding of libxpuhip.so (include/xpuhip.h), the MI355X xPU library.
+
+Loading fails with an exception when the shared library is missing: there is
+no CPU fallback behind this module."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "libxpuhip.so")
+
+EXPORTS = [
+    "xpuhip_init", "xpuhip_shutdown", "xpuhip_session_open", "xpuhip_session_check",
+    "xpuhip_session_attach_inner", "xpuhip_task_exec", "xpuhip_task_exec_device",
+    "xpuhip_task_final", "xpuhip_session_close", "xpuhip_session_set_tuning",
+    "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
+    "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_device_synchronize",
+    "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json",
+]
+
+
+class XpuError(RuntimeError):
+    def __init__(self, code, errbuf):
+        self.code = code
+        self.errbuf = errbuf
+        super().__init__(errbuf.describe())
+
+
+class xpuhip_tuning(C.Structure):
+    _fields_ = [("block_size", C.c_uint32), ("rows_per_thread", C.c_uint32),
+                ("lds_nents", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libxpuhip.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        vp, sz = C.c_void_p, C.c_size_t
+        L.xpuhip_init.argtypes = [C.c_int, vp]
+        L.xpuhip_session_open.argtypes = [vp, sz, C.POINTER(vp), vp]
+        L.xpuhip_session_check.argtypes = [vp, sz, vp]
+        L.xpuhip_session_attach_inner.argtypes = [vp, vp, sz, vp]
+        L.xpuhip_task_exec.argtypes = [vp, vp, sz, vp, sz, C.POINTER(C.c_uint64), vp, vp]
+        L.xpuhip_task_exec_device.argtypes = [vp, vp, C.c_uint32, vp, sz,
+                                              C.POINTER(C.c_uint64), vp, vp]
+        L.xpuhip_task_final.restype = C.c_int64
+        L.xpuhip_task_final.argtypes = [vp, vp, sz, vp, vp]
+        L.xpuhip_session_close.argtypes = [vp]
+        L.xpuhip_session_set_tuning.argtypes = [vp, vp]
+        L.xpuhip_session_get_tuning.argtypes = [vp, vp]
+        L.xpuhip_device_alloc.argtypes = [sz, C.POINTER(vp), vp]
+        L.xpuhip_device_free.argtypes = [vp]
+        L.xpuhip_memcpy_htod.argtypes = [vp, vp, sz]
+        L.xpuhip_memcpy_dtoh.argtypes = [vp, vp, sz]
+        L.xpuhip_last_kernel_ms.restype = C.c_double
+        L.xpuhip_abi_layout_json.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _buf(a):
+    a = np.ascontiguousarray(np.frombuffer(bytes(a), dtype=np.uint8)
+                             if not isinstance(a, np.ndarray) else a)
+    return a
+
+
+def init(device=0):
+    err = abi.kern_errorbuf()
+    rc = lib().xpuhip_init(device, C.byref(err))
+    if rc:
+        raise XpuError(rc, err)
+
+
+def session_check(blob):
+    err = abi.kern_errorbuf()
+    b = _buf(blob)
+    rc = lib().xpuhip_session_check(b.ctypes.data, len(b), C.byref(err))
+    return rc, err
+
+
+class DeviceBuffer:
+    """A device allocation holding a copy of a host buffer (or empty)."""
+
+    def __init__(self, nbytes, host=None):
+        err = abi.kern_errorbuf()
+        p = C.c_void_p()
+        rc = lib().xpuhip_device_alloc(max(nbytes, 8), C.byref(p), C.byref(err))
+        if rc:
+            raise XpuError(rc, err)
+        self.ptr = p.value
+        self.nbytes = nbytes
+        if host is not None:
+            h = _buf(host)
+            assert len(h) <= nbytes
+            if lib().xpuhip_memcpy_htod(self.ptr, h.ctypes.data, len(h)):
+                raise RuntimeError("xpuhip_memcpy_htod failed")
+
+    def to_host(self, nbytes=None, offset=0):
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.empty(n, dtype=np.uint8)
+        if lib().xpuhip_memcpy_dtoh(out.ctypes.data, self.ptr + offset, n):
+            raise RuntimeError("xpuhip_memcpy_dtoh failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().xpuhip_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class Session:
+    def __init__(self, blob):
+        self.err = abi.kern_errorbuf()
+        self._blob = _buf(blob)
+        h = C.c_void_p()
+        rc = lib().xpuhip_session_open(self._blob.ctypes.data, len(self._blob), C.byref(h),
+                                       C.byref(self.err))
+        if rc:
+            raise XpuError(rc, self.err)
+        self.h = h.value
+
+    def attach_inner(self, kmrels):
+        self._kmrels = _buf(kmrels)
+        rc = lib().xpuhip_session_attach_inner(self.h, self._kmrels.ctypes.data,
+                                               len(self._kmrels), C.byref(self.err))
+        if rc:
+            raise XpuError(rc, self.err)
+
+    def set_tuning(self, **kw):
+        t = xpuhip_tuning(**kw)
+        rc = lib().xpuhip_session_set_tuning(self.h, C.byref(t))
+        if rc:
+            raise ValueError("invalid tuning %r" % kw)
+
+    def get_tuning(self):
+        t = xpuhip_tuning()
+        lib().xpuhip_session_get_tuning(self.h, C.byref(t))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def exec(self, kds_src, dst_length=0, resume_pos=0):
+        """Host-resident chunk; returns (rc, kds_dst array or None, results, resume_pos)."""
+        src = _buf(kds_src)
+        dst = np.zeros(dst_length, dtype=np.uint8) if dst_length else None
+        res = abi.kern_exec_results()
+        pos = C.c_uint64(resume_pos)
+        rc = lib().xpuhip_task_exec(self.h, src.ctypes.data, len(src),
+                                    dst.ctypes.data if dst is not None else None, dst_length,
+                                    C.byref(pos), C.byref(res), C.byref(self.err))
+        if rc and rc != abi.ERRCODE_SUSPEND_NO_SPACE:
+            raise XpuError(rc, self.err)
+        return rc, dst, res, pos.value
+
+    def exec_device(self, d_src, nitems, d_dst=None, dst_length=0, resume_pos=0):
+        res = abi.kern_exec_results()
+        pos = C.c_uint64(resume_pos)
+        rc = lib().xpuhip_task_exec_device(self.h, d_src.ptr if d_src else None, nitems,
+                                           d_dst.ptr if d_dst else None, dst_length,
+                                           C.byref(pos), C.byref(res), C.byref(self.err))
+        if rc and rc != abi.ERRCODE_SUSPEND_NO_SPACE:
+            raise XpuError(rc, self.err)
+        return rc, res, pos.value
+
+    def final(self):
+        need = lib().xpuhip_task_final(self.h, None, 0, None, C.byref(self.err))
+        if need < 0:
+            raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
+        buf = np.zeros(need, dtype=np.uint8)
+        got = lib().xpuhip_task_final(self.h, buf.ctypes.data, need, None, C.byref(self.err))
+        if got < 0:
+            raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
+        return buf
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xpuhip_session_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def last_kernel_ms():
+    return lib().xpuhip_last_kernel_ms()
+
+
+def abi_layout_json():
+    return lib().xpuhip_abi_layout_json().decode()

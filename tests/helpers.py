@@ -1,0 +1,80 @@
+"""Shared helpers of the parity tests: run a session through the HIP library
+and through the oracle on the same inputs, and compare."""
+import math
+
+import numpy as np
+
+from pgstrom import abi, kds, workloads as W, xpuhip
+import oracle
+
+
+def run_gpu_projection(blob, src, dst_length, kmrels=None):
+    """All rows of one chunk through xpuhip_task_exec, following 'd' resumes;
+    returns the list of kds_dst buffers."""
+    s = xpuhip.Session(blob)
+    if kmrels is not None:
+        s.attach_inner(kmrels)
+    outs = []
+    pos = 0
+    for _ in range(10000):
+        rc, dst, res, pos = s.exec(src, dst_length=dst_length, resume_pos=pos)
+        outs.append(dst)
+        if rc == 0:
+            break
+        assert rc == abi.ERRCODE_SUSPEND_NO_SPACE and pos > 0
+    s.close()
+    return outs
+
+
+def run_oracle_projection(blob, src, dst_length, kmrels=None, nthreads=4):
+    o = oracle.OracleSession(blob)
+    rc, dst, res = o.exec(src, kmrels=kmrels, dst_length=dst_length, nthreads=nthreads)
+    o.close()
+    assert rc == 0, rc
+    return dst, res
+
+
+def run_gpu_groupby(blob, chunks, kmrels=None, tuning=None):
+    s = xpuhip.Session(blob)
+    if tuning:
+        s.set_tuning(**tuning)
+    if kmrels is not None:
+        s.attach_inner(kmrels)
+    for src in chunks:
+        rc, _, _, _ = s.exec(src)
+        assert rc == 0
+    fin = s.final()
+    s.close()
+    return fin
+
+
+def run_oracle_groupby(blob, chunks, kmrels=None, nthreads=4):
+    o = oracle.OracleSession(blob)
+    for src in chunks:
+        rc, _, _ = o.exec(src, kmrels=kmrels, nthreads=nthreads)
+        assert rc == 0, o.err.describe()
+    fin = o.final()
+    o.close()
+    return fin
+
+
+FP_ACTIONS = (abi.KAGG_ACTION__PSUM_FP, abi.KAGG_ACTION__PAVG_FP, abi.KAGG_ACTION__STDDEV)
+
+
+def assert_groups_equal(got_kds, exp_kds, actions, rtol=1e-12):
+    """Group sets equal: integer states bit-exact, float8 sums within rtol
+    relative (their summation order differs by construction)."""
+    got = W.decode_groups(got_kds, actions)
+    exp = W.decode_groups(exp_kds, actions)
+    assert set(got) == set(exp), (len(got), len(exp))
+    state_actions = [a for a in actions if a != abi.KAGG_ACTION__VREF]
+    for key, es in exp.items():
+        gs = got[key]
+        for a, g, e in zip(state_actions, gs, es):
+            if a in FP_ACTIONS:
+                assert g[0] == e[0], (key, a, g, e)
+                for gv, ev in zip(g[1:], e[1:]):
+                    assert math.isclose(gv, ev, rel_tol=rtol, abs_tol=0.0) or gv == ev, (key, a, gv, ev)
+            else:
+                assert g == e, (key, a, g, e)
+    return got

@@ -1,0 +1,61 @@
+"""Session open validation without a GPU (xpuhip_session_check)."""
+import struct
+
+import numpy as np
+
+from pgstrom import abi, codegen as cg, datagen, workloads as W, xpuhip
+
+
+def test_workload_sessions_compile():
+    for blob in (W.scan_query(datagen.scan_table(10)),
+                 W.preagg_query(datagen.preagg_table(10)),
+                 W.star_query(datagen.star_tables(10, dim_sizes=(10, 10, 10))[0])):
+        rc, err = xpuhip.session_check(blob)
+        assert rc == 0, err.describe()
+
+
+def test_corrupted_magic_rejected():
+    blob = bytearray(W.scan_query(datagen.scan_table(10)))
+    s = abi.kern_session_info_head.from_buffer_copy(bytes(blob[:ctypes_size()]))
+    off = s.xpucode_scan_quals
+    ln = struct.unpack_from("<I", blob, off)[0]
+    blob[off + ln - 1] ^= 0xff
+    rc, err = xpuhip.session_check(bytes(blob))
+    assert rc == abi.ERRCODE_DEVICE_ERROR
+    assert b"magic" in err.message
+
+
+def ctypes_size():
+    import ctypes
+    return ctypes.sizeof(abi.kern_session_info_head)
+
+
+def test_unsupported_function_rejected():
+    """An opcode this build does not execute fails at session open, not in a kernel."""
+    blob = bytearray(W.scan_query(datagen.scan_table(10)))
+    s = abi.kern_session_info_head.from_buffer_copy(bytes(blob[:ctypes_size()]))
+    off = s.xpucode_scan_quals
+    # first argument of the AND is int4lt: retag it with an unknown opcode and fix its magic
+    arg = off + 40
+    ln = struct.unpack_from("<I", blob, arg)[0]
+    exptype = struct.unpack_from("<i", blob, arg + 4)[0]
+    struct.pack_into("<i", blob, arg + 20, 9000)
+    magic = (abi.KERN_EXPRESSION_MAGIC ^ (exptype << 6) ^ (9000 << 14)) & 0xffffffff
+    struct.pack_into("<I", blob, arg + ln - 4, magic)
+    rc, err = xpuhip.session_check(bytes(blob))
+    assert rc == abi.ERRCODE_DEVICE_ERROR
+    assert b"not supported" in err.message
+
+
+def test_register_pressure_uses_large_file():
+    """Expressions that need more than the 16-register file still compile."""
+    cols = datagen.scan_table(10)
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    a = plan.scan_var("a")
+    e = a
+    for i in range(12):
+        e = cg.Func("int4pl", e, cg.Func("int4mul", a, cg.Const("int4", i)))
+    plan.scan_quals = cg.Func("int4gt", e, cg.Const("int4", 0))
+    plan.set_projection([a], [("a", "int4")])
+    rc, err = xpuhip.session_check(plan.build())
+    assert rc == 0, err.describe()
