@@ -53,6 +53,11 @@ int     xpuhip_session_open(const kern_session_info *session, size_t length,
 int     xpuhip_session_check(const kern_session_info *session, size_t length,
                              kern_errorbuf *errbuf);
 
+/* Text listing of the compiled programs of a session blob (the EXPLAIN of
+ * this build's translation); returns its length, -1 on a compile error. */
+int64_t xpuhip_session_explain(const kern_session_info *session, size_t length,
+                               char *buf, size_t buflen, kern_errorbuf *errbuf);
+
 /* Imports the inner relations (kern_multirels with KDS_FORMAT_HASH chunks,
  * gpu_join.c:2128-2164) into device memory for the session's hash joins;
  * replaces the shm import of gpu_service.c:2197-2354. */

@@ -1123,3 +1123,50 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     return 0;
 #undef FAIL
 }
+
+/* textual dump of a compiled plan (debugging and CPU-side tests) */
+std::string
+xpu_plan_dump(const xpu_compiled *c)
+{
+    static const char *names[XO_NUM_OPS] = {
+        "NOP", "LDCOL", "LDCOL_BOOL", "LDCOL_VAR", "LDTUP", "CONST", "CONST_TEXT", "MOV", "MOV2",
+        "SETNULL", "CMP_I", "CMP_F32", "CMP_F64", "CMP_TEXT", "ADD_I", "SUB_I", "MUL_I", "DIV_I",
+        "MOD_I", "NEG_I", "ABS_I", "AND_I", "OR_I", "XOR_I", "NOT_I", "SHL_I", "SHR_I",
+        "ADD_F32", "SUB_F32", "MUL_F32", "DIV_F32", "ADD_F64", "SUB_F64", "MUL_F64", "DIV_F64",
+        "NEG_F32", "ABS_F32", "NEG_F64", "ABS_F64", "I2I", "F32_I", "F64_I", "I_F32", "I_F64",
+        "F32_F64", "F64_F32", "I_BOOL", "TEXTLEN", "AND3", "OR3", "NOT3", "ISNULL", "ISNOTNULL",
+        "BOOLTEST", "COALESCE", "G_NOTFALSE", "G_NOTTRUE", "G_ISNULL", "G_POSITIVE", "G_NONZERO",
+        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN"};
+    const xplan *P = &c->plan;
+    std::string s;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "nregs=%u nrels=%u nops=%u\n", P->nregs, P->nrels, P->nops);
+    s += buf;
+    auto seg = [&](const char *name, xseg g) {
+        snprintf(buf, sizeof(buf), "[%s %u..%u]\n", name, g.begin, g.end);
+        s += buf;
+        for (int pc = g.begin; pc < g.end; pc++)
+        {
+            const xop &o = P->ops[pc];
+            snprintf(buf, sizeof(buf), "  %3d %-10s d=%u a=%u b=%u g=%u aux=%u imm=%lld\n", pc,
+                     o.op < XO_NUM_OPS ? names[o.op] : "?", o.dst, o.a, o.b, o.guard, o.aux,
+                     (long long)o.imm);
+            s += buf;
+        }
+    };
+    seg("scan", P->seg_scan);
+    for (uint32_t d = 1; d <= P->nrels; d++)
+    {
+        seg("hash", P->depth[d].seg_hash);
+        seg("join", P->depth[d].seg_join);
+        seg("outer", P->depth[d].seg_oj);
+    }
+    seg("final", P->seg_final);
+    for (uint32_t j = 0; j < P->proj_nattrs; j++)
+    {
+        snprintf(buf, sizeof(buf), "proj[%u] reg=%u dom=%u align=%d width=%d\n", j, P->proj[j].reg,
+                 P->proj[j].domain, P->proj[j].attalign, P->proj[j].width);
+        s += buf;
+    }
+    return s;
+}

@@ -27,6 +27,8 @@ struct xpu_compiled
 int xpu_compile_session(const kern_session_info *session, size_t length,
                         xpu_compiled *out, kern_errorbuf *errbuf);
 
+std::string xpu_plan_dump(const xpu_compiled *c);
+
 void xpu_set_error(kern_errorbuf *eb, uint32_t errcode, const char *filename,
                    int lineno, const char *funcname, const char *message);
 

@@ -24,7 +24,7 @@
 typedef struct
 {
     uint32_t    tile_ticket;        /* dynamic tile counter */
-    uint32_t    resume_tile;        /* first tile that did not fit, plus one (0: none) */
+    uint32_t    resume_tile;        /* ~(first tile that did not fit); 0: none */
     uint32_t    errcode;            /* first error of the task */
     uint32_t    errmsg;             /* message id in the session's table */
     uint32_t    nitems_in;          /* rows that passed the scan quals */

@@ -1691,10 +1691,18 @@ xpu_pipeline_kernel(xpu_kparams kp)
                 }
                 if (act)
                     interp(P, seg.begin, seg.end, L, kds, row, kp.const_pool);
+                /* an error ends the lane's row and is reported for the task */
+                bool lane_err = false;
+                if (act && L.err)
+                {
+                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                    L.err = 0;
+                    lane_err = true;
+                }
                 /* ---- post ---- */
                 if (phase == PH_SCAN)
                 {
-                    bool ok = act && !L.err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
+                    bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
                     if (ok && pass == 0)
                         cnt_in++;
                     cd = ok ? 1 : 0;
@@ -1728,7 +1736,7 @@ xpu_pipeline_kernel(xpu_kparams kp)
                         kern_data_store *ik = (kern_data_store *)(uintptr_t)P->depth[d].inner_kds;
                         const kern_hashitem *cur = hashitem_at(ik, it[d]);
                         uint32_t h = cur->t.hash;       /* == the probe hash */
-                        int64_t status = L.err ? 0 : L.R[P->r_status];
+                        int64_t status = lane_err ? 0 : L.R[P->r_status];
                         /* next candidate with the same hash, found now so that
                          * exhausted chains unwind without another pass */
                         uint64_t off = cur->next;
@@ -1755,7 +1763,7 @@ xpu_pipeline_kernel(xpu_kparams kp)
                     if (act)
                     {
                         matched |= 1U << d;
-                        if (!L.err && L.R[P->r_status] > 0)
+                        if (!lane_err && L.R[P->r_status] > 0)
                         {
                             if (pass == 0)
                                 atomicAdd(&s_depth_out[d], 1U);
@@ -1788,7 +1796,7 @@ xpu_pipeline_kernel(xpu_kparams kp)
                 }
                 else    /* PH_EMIT */
                 {
-                    if (act && L.err == 0)
+                    if (act && !lane_err)
                     {
                         if (pass == 0)
                             cnt_out++;
@@ -1845,17 +1853,15 @@ xpu_pipeline_kernel(xpu_kparams kp)
                             }
                         }
                     }
+                    if (lane_err)
+                        cd = 0;
                     if (!wave_any(cd > 0))
                         break;
                     phase = PH_HASH;
                     d = 1;
                 }
-                if (L.err)
-                {
-                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
-                    L.err = 0;
+                if (lane_err)
                     cd = 0;
-                }
             }
 
             if (!PREAGG && pass == 0)
@@ -1923,10 +1929,10 @@ xpu_pipeline_kernel(xpu_kparams kp)
                         KDS_HEAD_LENGTH(dst) + 8 * (er + tc) + eb + tb > dst->length)
                     {
                         stop = true;
-                        /* only the first tile that does not fit gets here: later
-                         * tiles see its STOP word; resume_tile = tile + 1 */
-                        __hip_atomic_store(&ctl->resume_tile, tile + 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+                        /* several tiles may see the overflow (a successor can read
+                         * this tile's AGG word before its STOP); the resume point
+                         * is the earliest: keep max(~tile) */
+                        atomicMax(&ctl->resume_tile, 0xffffffffU - tile);
                         if (tile == 0)
                             record_error(ctl, XMSG_TILE_TOO_LARGE, ERRCODE_DEVICE_ERROR);
                     }

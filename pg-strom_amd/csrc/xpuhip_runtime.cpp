@@ -378,6 +378,28 @@ xpuhip_session_check(const kern_session_info *session, size_t length, kern_error
     return rc;
 }
 
+extern "C" int64_t
+xpuhip_session_explain(const kern_session_info *session, size_t length, char *buf, size_t buflen,
+                       kern_errorbuf *eb)
+{
+    xpu_compiled *c = new xpu_compiled();
+    int rc = xpu_compile_session(session, length, c, eb);
+    if (rc)
+    {
+        delete c;
+        return -1;
+    }
+    std::string s = xpu_plan_dump(c);
+    delete c;
+    if (buf && buflen > 0)
+    {
+        size_t n = s.size() < buflen - 1 ? s.size() : buflen - 1;
+        memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return (int64_t)s.size();
+}
+
 extern "C" int
 xpuhip_session_open(const kern_session_info *session, size_t length,
                     xpuhip_session **p_sess, kern_errorbuf *eb)
@@ -620,8 +642,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     }
     if (!preagg && c->resume_tile != 0)
     {
-        /* resume_tile holds (first tile that did not fit) + 1 */
-        uint64_t next = row_begin + (uint64_t)(c->resume_tile - 1) * rows_per_tile;
+        /* resume_tile holds ~(first tile that did not fit) */
+        uint64_t next = row_begin + (uint64_t)(0xffffffffU - c->resume_tile) * rows_per_tile;
         if (resume_pos)
             *resume_pos = next;
         XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "GpuProjection - no space to write");

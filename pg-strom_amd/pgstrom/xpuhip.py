@@ -18,7 +18,7 @@ EXPORTS = [
     "xpuhip_task_final", "xpuhip_session_close", "xpuhip_session_set_tuning",
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_device_synchronize",
-    "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json",
+    "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
 ]
 
 
@@ -62,6 +62,8 @@ def lib():
         L.xpuhip_memcpy_dtoh.argtypes = [vp, vp, sz]
         L.xpuhip_last_kernel_ms.restype = C.c_double
         L.xpuhip_abi_layout_json.restype = C.c_char_p
+        L.xpuhip_session_explain.restype = C.c_int64
+        L.xpuhip_session_explain.argtypes = [vp, sz, vp, sz, vp]
         _lib = L
     return _lib
 
@@ -70,6 +72,18 @@ def _buf(a):
     a = np.ascontiguousarray(np.frombuffer(bytes(a), dtype=np.uint8)
                              if not isinstance(a, np.ndarray) else a)
     return a
+
+
+def explain(blob):
+    """Listing of the programs the library compiles from a session blob."""
+    err = abi.kern_errorbuf()
+    b = _buf(blob)
+    n = lib().xpuhip_session_explain(b.ctypes.data, len(b), None, 0, C.byref(err))
+    if n < 0:
+        raise XpuError(abi.ERRCODE_DEVICE_ERROR, err)
+    out = C.create_string_buffer(n + 1)
+    lib().xpuhip_session_explain(b.ctypes.data, len(b), out, n + 1, C.byref(err))
+    return out.value.decode()
 
 
 def init(device=0):

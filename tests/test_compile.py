@@ -59,3 +59,21 @@ def test_register_pressure_uses_large_file():
     plan.set_projection([a], [("a", "int4")])
     rc, err = xpuhip.session_check(plan.build())
     assert rc == 0, err.describe()
+
+
+def test_explain_short_circuit_guard():
+    """The division inside an AND is compiled under the AND's guard register,
+    so a row where the first argument is false never raises."""
+    cols = [datagen.scan_table(10)[0]]
+    plan = cg.Plan([("a", "int4")])
+    a = plan.scan_var("a")
+    plan.scan_quals = cg.And(cg.Func("int4ne", a, cg.Const("int4", 0)),
+                             cg.Func("int4gt", cg.Func("int4div", cg.Const("int4", 10), a),
+                                     cg.Const("int4", 2)))
+    plan.set_projection([a], [("a", "int4")])
+    text = xpuhip.explain(plan.build())
+    lines = text.splitlines()
+    guard = [l for l in lines if "G_NOTFALSE" in l][0]
+    gdst = int(guard.split("d=")[1].split()[0])
+    div = [l for l in lines if "DIV_I" in l][0]
+    assert int(div.split("g=")[1].split()[0]) == gdst, text

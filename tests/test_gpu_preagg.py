@@ -33,7 +33,7 @@ def test_c1_preagg_multi_chunk(gpu):
     assert_groups_equal(got, exp, C1_ACTIONS)
 
 
-@pytest.mark.parametrize("tuning", [dict(lds_nents=64), dict(lds_nents=4096),
+@pytest.mark.parametrize("tuning", [dict(lds_nents=64), dict(lds_nents=512),
                                     dict(rows_per_thread=1, block_size=64),
                                     dict(rows_per_thread=16, block_size=512)])
 def test_c1_preagg_tunings(gpu, tuning):
@@ -84,3 +84,15 @@ def test_preagg_nogroup(gpu):
     sel = cols[0].values < 50
     assert groups[()][0] == int(sel.sum())
     assert groups[()][1] == (int(sel.sum()), int(cols[1].values[sel].astype(object).sum()))
+
+
+def test_lds_table_too_large_is_refused(gpu):
+    """An LDS table larger than a block's LDS is refused before any launch."""
+    from pgstrom import xpuhip
+    cols = datagen.preagg_table(1000)
+    s = xpuhip.Session(W.preagg_query(cols))
+    s.set_tuning(lds_nents=1 << 16)
+    with pytest.raises(xpuhip.XpuError) as ei:
+        s.exec(kds.build_arrow_kds(cols))
+    assert b"exceeds the LDS" in ei.value.errbuf.message
+    s.close()
