@@ -1269,19 +1269,26 @@ gb_global(const xplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bo
     {
         if (pending)
         {
-            uint64_t head = __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            /* Slots and the fields of published group tuples are read with
+             * relaxed device-coherent loads: the inserting lane writes a tuple
+             * into a fresh 128-byte unit no other XCD has cached and publishes
+             * it with a release store, so no per-row acquire (an L2
+             * invalidate on CDNA) is needed on the lookup path. */
+            uint64_t head = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             kern_hashitem *found = nullptr;
             if (head != XGB_LOCKED_SLOT)
             {
                 for (uint64_t off = head; off != 0; )
                 {
                     kern_hashitem *hi = hashitem_at(kds_final, off);
-                    if (hi->t.hash == hash && gb_keys_match(P, L, &hi->t))
+                    uint32_t h2 = __hip_atomic_load(&hi->t.hash, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                    if (h2 == hash && gb_keys_match(P, L, &hi->t))
                     {
                         found = hi;
                         break;
                     }
-                    off = hi->next;
+                    off = __hip_atomic_load(&hi->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 if (!found &&
                     atomicCAS((unsigned long long *)slot, (unsigned long long)head,
@@ -1299,7 +1306,7 @@ gb_global(const xplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bo
                     if (need > kds_final->length)
                     {
                         record_error(ctl, XMSG_FINAL_NO_SPACE, ERRCODE_SUSPEND_NO_SPACE);
-                        __hip_atomic_store(slot, head, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(slot, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         pending = false;
                     }
                     else
@@ -1309,7 +1316,6 @@ gb_global(const xplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bo
                         uint64_t *rowidx = slots + nslots;
                         rowidx[rowid] = (uint64_t)((char *)kds_final + kds_final->length -
                                                    (char *)&hi->t);
-                        __threadfence();
                         __hip_atomic_store(slot, (uint64_t)usage, __ATOMIC_RELEASE,
                                            __HIP_MEMORY_SCOPE_AGENT);
                         found = hi;
@@ -1893,12 +1899,12 @@ xpu_pipeline_kernel(xpu_kparams kp)
                     if (tile > 0)
                     {
                         __hip_atomic_store(&st[tile], lb_pack(XLB_AGG, tc, tb),
-                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         int64_t j = (int64_t)tile - 1;
                         uint32_t spins = 0;
                         while (j >= 0)
                         {
-                            uint64_t w = __hip_atomic_load(&st[j], __ATOMIC_ACQUIRE,
+                            uint64_t w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
                             uint64_t flag = w >> XLB_FLAG_SHIFT;
                             if (flag == 0)
@@ -1941,12 +1947,12 @@ xpu_pipeline_kernel(xpu_kparams kp)
                     {
                         stop = true;
                         __hip_atomic_store(&st[tile], lb_pack(XLB_STOP, 0, 0),
-                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     else
                     {
                         __hip_atomic_store(&st[tile], lb_pack(XLB_PREFIX, er + tc, eb + tb),
-                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         atomicMax((unsigned long long *)&ctl->out_nitems, (unsigned long long)(er + tc));
                         atomicMax((unsigned long long *)&ctl->out_usage, (unsigned long long)(eb + tb));
                     }
