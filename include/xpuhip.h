@@ -26,15 +26,21 @@ extern "C" {
 
 typedef struct xpuhip_session xpuhip_session;
 
-/* Tuning of the pipeline launch; zero fields keep the library's choice
- * (chosen per task kind by the autotuner, see xpuhip_autotune). */
+/* Tuning of the pipeline launch; zero fields keep the library's choice for
+ * the session's task kind. */
 typedef struct
 {
     uint32_t    block_size;         /* threads per block (64..512, multiple of 64) */
-    uint32_t    rows_per_thread;    /* group-by rows per thread per tile */
-    uint32_t    lds_nents;          /* group-by LDS table entries (power of 2) */
+    uint32_t    rows_per_thread;    /* group-by rows per lane per tile (1, 2 or 4
+                                       without joins) */
+    uint32_t    lds_nents;          /* group-by LDS table entries */
     uint32_t    blocks_per_cu;      /* persistent blocks per compute unit */
+    uint32_t    jit;                /* XPUHIP_JIT_*: kernel specialized for the plan */
 } xpuhip_tuning;
+
+#define XPUHIP_JIT_AUTO     0       /* specialize chunks of >= 1M rows */
+#define XPUHIP_JIT_ON       1       /* always specialize */
+#define XPUHIP_JIT_OFF      2       /* always run the op interpreter */
 
 /* Device setup; replaces gpuservSetupGpuContext (gpu_service.c:6462-6523).
  * device < 0 selects device 0. */
@@ -52,6 +58,12 @@ int     xpuhip_session_open(const kern_session_info *session, size_t length,
  * returns 0 if the session would open. */
 int     xpuhip_session_check(const kern_session_info *session, size_t length,
                              kern_errorbuf *errbuf);
+
+/* Compiles (and keeps for the process) the kernel specialized for the
+ * session blob, so that the first task does not wait for it; needs no GPU.
+ * Has no counterpart in the reference, which interprets every plan. */
+int     xpuhip_jit_prepare(const kern_session_info *session, size_t length,
+                           kern_errorbuf *errbuf);
 
 /* Text listing of the compiled programs of a session blob (the EXPLAIN of
  * this build's translation); returns its length, -1 on a compile error. */
@@ -108,6 +120,24 @@ int     xpuhip_device_synchronize(void);
 /* Kernel time of the last task executed by the calling thread, in
  * milliseconds, measured with HIP events on the stream it ran on. */
 double  xpuhip_last_kernel_ms(void);
+
+/* Launch shape and counters of the last task executed by the calling thread
+ * (the reference reports the row counts in kern_exec_results; the rest has no
+ * counterpart there and serves tuning and profiling). */
+typedef struct
+{
+    double      kernel_ms;          /* as xpuhip_last_kernel_ms */
+    uint32_t    grid;               /* blocks launched */
+    uint32_t    block;              /* threads per block */
+    uint32_t    ntiles;             /* row tiles of the chunk */
+    uint32_t    lds_nents;          /* group-by LDS table entries (0: none) */
+    uint64_t    lds_bytes;          /* dynamic LDS per block */
+    uint64_t    rows;               /* rows handed to the kernel */
+    uint64_t    lds_miss;           /* group-by rows merged straight into kds_final */
+    uint64_t    global_merges;      /* LDS entries merged into kds_final at block end */
+} xpuhip_task_stats;
+
+int     xpuhip_last_task_stats(xpuhip_task_stats *stats);
 
 /* Layout of this library's ABI structs as JSON (for conformance tests). */
 const char *xpuhip_abi_layout_json(void);

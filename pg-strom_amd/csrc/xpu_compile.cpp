@@ -739,22 +739,31 @@ layout_state_size(int action)
     return -1;
 }
 
-/* 8-byte LDS arrays an action needs in the partial table */
-int
-lds_arrays(int action)
+/* fields an action keeps in the LDS partial table, besides the entry's row
+ * count (rows that reach the table have every argument non-NULL and every
+ * FILTER true, so count(*), count(x) and the n of sums and averages are all
+ * the row count); false if the action has no LDS form */
+bool
+lds_fields(int action, int *n64, int *n32)
 {
+    *n64 = *n32 = 0;
     switch (action)
     {
-        case KAGG_ACTION__NROWS_ANY: case KAGG_ACTION__NROWS_COND: return 1;
-        case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
-        case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
-        case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64: return 1;
-        case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT: return 2;
-        case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64: return 3;
-        case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP: return 2;
-        case KAGG_ACTION__STDDEV: return 3;
+        case KAGG_ACTION__NROWS_ANY: case KAGG_ACTION__NROWS_COND:
+            return true;
+        case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMAX_INT32:
+            *n32 = 1; return true;
+        case KAGG_ACTION__PMIN_INT64: case KAGG_ACTION__PMAX_INT64:
+        case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64:
+        case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
+        case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
+            *n64 = 1; return true;
+        case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            *n64 = 1; *n32 = 1; return true;    /* low word + signed carry count */
+        case KAGG_ACTION__STDDEV:
+            *n64 = 2; return true;
     }
-    return -1;
+    return false;
 }
 
 }   /* namespace */
@@ -864,6 +873,12 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         }
         else
             C.emit(XO_CONST, P->r_scan_ok, 0, 0, XP_REG_TRUE, 0, 1);
+        if (nrels == 0)
+        {
+            /* the final stage runs in the same pass over the row */
+            C.emit(XO_KILL, 0, P->r_scan_ok);
+            P->fused = 1;
+        }
     }
     C.end_seg(P->seg_scan);
     if (!C.err.empty())
@@ -927,18 +942,14 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         P->gb_nattrs = nattrs;
 
         P->seg_final = C.begin_seg();
+        /* The key hash expression (HashValue over the grouping keys) only
+         * places groups in kds_final's hash slots, which the write-back turns
+         * into a ROW buffer; the kernels hash the key registers themselves,
+         * and only for rows that reach kds_final, so it is checked here but
+         * not compiled. */
         const kern_expression *kkh = C.kexp_at(s->xpucode_groupby_keyhash);
-        std::vector<int> key_slots;
-        if (kkh)
-        {
-            if (!C.check_node(nullptr, kkh) || kkh->opcode != FuncOpCode__HashValue)
-                FAIL("malformed group-by key hash");
-            Reg h = C.expr(nullptr, kkh, XP_REG_TRUE);
-            if (C.err.empty())
-                C.emit(XO_MOV, P->r_keyhash, h.r);
-        }
-        else
-            C.emit(XO_CONST, P->r_keyhash, 0, 0, XP_REG_TRUE, 0, 0);
+        if (kkh && (!C.check_node(nullptr, kkh) || kkh->opcode != FuncOpCode__HashValue))
+            FAIL("malformed group-by key hash");
         /* aggregate arguments (SaveExpr children of AggFuncs) */
         {
             const kern_expression *a = C.first_arg(kactions);
@@ -954,7 +965,7 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         bool seen_key = false;
         int nkeys = 0, key_bits = 0;
         bool lds_ok = true;
-        int lds_arr = 0;
+        int lds_n64 = 0, lds_n32 = 0;
         for (int j = 0; j < nattrs; j++)
         {
             const kern_aggregate_desc *d = &kactions->u.pagg.desc[j];
@@ -964,6 +975,8 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             ag->arg_reg = 0xff;
             ag->filter_reg = 0xff;
             ag->key_index = 0xff;
+            ag->lds64 = -1;
+            ag->lds32 = -1;
             if (d->arg0_slot_id >= 0)
             {
                 if (d->arg0_slot_id >= C.nslots)
@@ -1018,32 +1031,24 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             ag->state_off = off;
             ag->state_size = sz;
             off += sz;
-            int la = lds_arrays(d->action);
-            if (la < 0)
+            int f64, f32;
+            if (!lds_fields(d->action, &f64, &f32))
                 lds_ok = false;
-            ag->lds_off = lds_arr;
-            lds_arr += la > 0 ? la : 0;
+            if (f64 > 0)
+                ag->lds64 = lds_n64;
+            if (f32 > 0)
+                ag->lds32 = lds_n32;
+            lds_n64 += f64;
+            lds_n32 += f32;
         }
         P->nkeys = nkeys;
         P->gb_states_end = off;
         P->gb_t_hoff = XPU_MAXALIGN(KERN_TUPITEM_T_BITS_OFFSET + XPU_BITMAPLEN(nattrs));
-        P->gb_lds_arrays = lds_arr;
         P->lds_fast = (lds_ok && key_bits <= 63) ? 1 : 0;
-        out->lds_entry_bytes = 8 * (1 + lds_arr) + 4;
-
-        /* hash of the grouping keys from their registers (for flushing the
-         * LDS table, whose entries hold only the key values) */
-        P->seg_keyhash_regs = C.begin_seg();
-        {
-            int h = C.alloc_reg();
-            C.emit(XO_HASH_INIT, h);
-            for (int k = 0; k < nkeys; k++)
-                C.emit(XO_HASH_ADD, h, P->keys[k].reg, P->keys[k].domain, XP_REG_TRUE,
-                       P->keys[k].domain == XD_TEXT ? 0 : P->keys[k].width);
-            C.emit(XO_HASH_FIN, h);
-            C.emit(XO_MOV, P->r_keyhash, h);
-        }
-        C.end_seg(P->seg_keyhash_regs);
+        P->lds_n64 = lds_n64;
+        P->lds_n32 = lds_n32;
+        P->lds_entry_bytes = 8 * (1 + lds_n64) + 4 * (1 + lds_n32);
+        out->lds_entry_bytes = P->lds_entry_bytes;
         if (kf->format == KDS_FORMAT_HASH && kf->hash_nslots == 0)
             FAIL("kds_final in HASH format without hash slots");
         if (nkeys > 0 && kf->format != KDS_FORMAT_HASH)
@@ -1118,6 +1123,27 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         }
         P->nregs = nr;
     }
+    /* depth-0 fixed-width columns: the scan kernel loads them for a whole
+     * tile of rows before running the program (all loads in flight at once)
+     * and the program's own loads of them become no-ops there */
+    bool prefix = true;
+    for (int pc = P->seg_scan.begin; pc < P->seg_scan.end; pc++)
+    {
+        xop &o = C.ops[pc];
+        if (o.op != XO_LDCOL || P->nstage >= XP_MAX_STAGE)
+        {
+            prefix = false;
+            continue;
+        }
+        if (prefix)
+            P->stage_prefix++;
+        xstage *st = &P->stage[P->nstage++];
+        st->col = o.aux;
+        st->width = (uint8_t)(o.imm & XLD_WIDTH_MASK);
+        st->f32 = (o.b == XD_F32) ? 1 : 0;
+        st->reg = o.dst;
+        o.imm |= XLD_STAGED;
+    }
     P->nops = C.ops.size();
     memcpy(P->ops, C.ops.data(), sizeof(xop) * C.ops.size());
     return 0;
@@ -1136,7 +1162,7 @@ xpu_plan_dump(const xpu_compiled *c)
         "NEG_F32", "ABS_F32", "NEG_F64", "ABS_F64", "I2I", "F32_I", "F64_I", "I_F32", "I_F64",
         "F32_F64", "F64_F32", "I_BOOL", "TEXTLEN", "AND3", "OR3", "NOT3", "ISNULL", "ISNOTNULL",
         "BOOLTEST", "COALESCE", "G_NOTFALSE", "G_NOTTRUE", "G_ISNULL", "G_POSITIVE", "G_NONZERO",
-        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN"};
+        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL"};
     const xplan *P = &c->plan;
     std::string s;
     char buf[256];
@@ -1162,6 +1188,18 @@ xpu_plan_dump(const xpu_compiled *c)
         seg("outer", P->depth[d].seg_oj);
     }
     seg("final", P->seg_final);
+    for (uint32_t i = 0; i < P->nstage; i++)
+    {
+        snprintf(buf, sizeof(buf), "stage[%u] col=%u width=%u f32=%u reg=%u\n", i, P->stage[i].col,
+                 P->stage[i].width, P->stage[i].f32, P->stage[i].reg);
+        s += buf;
+    }
+    if (c->has_groupby)
+    {
+        snprintf(buf, sizeof(buf), "groupby nkeys=%u lds_fast=%u lds_entry_bytes=%u (u64 fields %u, u32 fields %u)\n",
+                 P->nkeys, P->lds_fast, P->lds_entry_bytes, P->lds_n64, P->lds_n32);
+        s += buf;
+    }
     for (uint32_t j = 0; j < P->proj_nattrs; j++)
     {
         snprintf(buf, sizeof(buf), "proj[%u] reg=%u dom=%u align=%d width=%d\n", j, P->proj[j].reg,

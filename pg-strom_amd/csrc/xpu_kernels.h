@@ -56,8 +56,10 @@ typedef struct
     uint32_t    row_end;                /* one past the last row */
     uint32_t    rows_per_tile;
     uint32_t    ntiles;
-    uint32_t    lds_nents;              /* group-by LDS table entries (power of 2; 0: off) */
+    uint32_t    lds_nents;              /* group-by LDS table entries (0: off) */
     uint32_t    __pad;
+    uint64_t    inner_kds[XP_MAX_DEPTH + 1];    /* device address of the inner KDS of
+                                                   each depth (index 1..nrels) */
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -66,6 +68,9 @@ extern "C" {
 /* host-side launchers (defined in xpu_kernels.hip) */
 int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
                         size_t lds_bytes, bool preagg, void *stream);
+/* scan + group-by without joins; rpt = rows per lane per wave tile (1, 2, 4) */
+int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
+                        size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
 int xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head,
                             uint32_t head_len, uint64_t length, const xpu_task_ctl *ctl,

@@ -1,2034 +1,22 @@
 /*
- * xpu_kernels.hip - the MI355X (gfx950) execution path of PG-Strom's xPU
- * scan / hash-join / pre-aggregation pipeline.
- *
- * One persistent kernel per task (the role of kern_gpujoin_main,
- * cuda_gpujoin.cu:1050-1279), redesigned for CDNA4:
- *
- *  - Expressions run as compiled register programs (xpu_program.h) with a
- *    wave-uniform program counter; there are no device function pointers,
- *    no alloca'd kern_context and no per-depth kvec rings in global memory
- *    (cuda_gpujoin.cu:1093-1096): a row stays in one lane from the column
- *    load to the final stage, its values in VGPRs.
- *  - Hash joins walk the inner KDS_FORMAT_HASH chains lane-by-lane in a
- *    depth-first state machine (one outer row per lane, any number of inner
- *    matches per depth, LEFT OUTER null extension), cuda_gpujoin.cu:304-494.
- *  - Projection output is stream-compacted with a wave64 shuffle scan and a
- *    decoupled look-back across tiles, so rows land in kds_dst in input order
- *    and an overflow ('d') leaves a clean resume point at a tile boundary
- *    (the reference suspends whole blocks, cuda_gpujoin.cu:836-878).
- *  - Group-by keeps a per-block partial table in LDS keyed by the packed
- *    grouping keys, updated with LDS atomics, and merges it into the
- *    reference-format kds_final (KDS_FORMAT_HASH group tuples) with 64-bit
- *    global atomics once per block; rows that miss the LDS table update
- *    kds_final directly (cuda_gpupreagg.cu:1607-1731 semantics).
+ * xpu_kernels.hip - ahead-of-time kernels of libxpuhip.so: the pipeline and
+ * scan + group-by kernels running the plan's programs through the op
+ * interpreter (device code in xpu_device.h), and small helper kernels.
  */
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdint.h>
-
-#include "xpu_kernels.h"
-
-#define XP_BLOCK_MAX        512
-
-/* ------------------------------------------------------------------
- * small helpers
- * ------------------------------------------------------------------ */
-#define RNULL(L, r)         (((L).nmask >> (r)) & 1U)
-#define SETNULL(L, r, n)    ((L).nmask = ((L).nmask & ~(1U << (r))) | ((uint32_t)(n) << (r)))
-
-/* The register file is a separate local array (a homogeneous alloca that
- * the backend keeps in VGPRs, indexed with uniform register numbers); a
- * struct holding it together with the scalars could not be split. */
-struct Lane
-{
-    int64_t    *R;              /* -> int64_t[XP_NREGS] in the kernel frame */
-    uint32_t    nmask;          /* bit r: R[r] is NULL */
-    uint32_t    err;            /* message id + 1 of the first error in this lane */
-};
-
-__device__ __forceinline__ double   as_f64(int64_t v) { return __longlong_as_double(v); }
-__device__ __forceinline__ int64_t  f64_bits(double d) { return __double_as_longlong(d); }
-__device__ __forceinline__ float    as_f32(int64_t v) { return __int_as_float((int)(uint32_t)v); }
-__device__ __forceinline__ int64_t  f32_bits(float f) { return (int64_t)(uint32_t)__float_as_int(f); }
-
-__device__ __forceinline__ uint64_t
-xalign(uint64_t a, uint64_t x)
-{
-    return (x + a - 1) & ~(a - 1);
-}
-
-__device__ __forceinline__ bool
-int_in_class(int cls, int64_t v)
-{
-    switch (cls)
-    {
-        case XT_BOOL:
-        case XT_INT1: return v >= -128 && v <= 127;
-        case XT_INT2: return v >= -32768 && v <= 32767;
-        case XT_INT4: return v >= INT32_MIN && v <= INT32_MAX;
-    }
-    return true;
-}
-
-__device__ __forceinline__ int64_t
-int_class_min(int cls)
-{
-    switch (cls)
-    {
-        case XT_BOOL:
-        case XT_INT1: return -128;
-        case XT_INT2: return -32768;
-        case XT_INT4: return INT32_MIN;
-    }
-    return INT64_MIN;
-}
-
-__device__ __forceinline__ int64_t
-wrap_class(int cls, int64_t v)
-{
-    switch (cls)
-    {
-        case XT_BOOL:
-        case XT_INT1: return (int8_t)v;
-        case XT_INT2: return (int16_t)v;
-        case XT_INT4: return (int32_t)v;
-    }
-    return v;
-}
-
-/* NaN-aware float ordering (xpu_basetype.cu:467-478) */
-__device__ __forceinline__ bool
-fcmp(int op, double x, double y)
-{
-    bool xn = isnan(x), yn = isnan(y);
-    switch (op)
-    {
-        case XC_EQ: return xn ? yn : (!yn && x == y);
-        case XC_NE: return xn ? !yn : (yn || x != y);
-        case XC_LT: return !xn && (yn || x < y);
-        case XC_LE: return yn || (!xn && x <= y);
-        case XC_GT: return !yn && (xn || x > y);
-        case XC_GE: return xn || (!yn && x >= y);
-    }
-    return false;
-}
-
-__device__ __forceinline__ bool
-cmp_by(int op, int c)
-{
-    switch (op)
-    {
-        case XC_EQ: return c == 0;
-        case XC_NE: return c != 0;
-        case XC_LT: return c < 0;
-        case XC_LE: return c <= 0;
-        case XC_GT: return c > 0;
-        case XC_GE: return c >= 0;
-    }
-    return false;
-}
-
-/* ------------------------------------------------------------------
- * lookup3 hash with PostgreSQL's seed (xpu_common.cu:2920-3041).
- * Fixed-width keys of up to 8 bytes never reach the 12-byte mixing loop,
- * so their hash is the final() mix of the two assembled words.
- * ------------------------------------------------------------------ */
-#define XROT(x, k) (((x) << (k)) | ((x) >> (32 - (k))))
-#define XMIX(a, b, c) do { \
-    a -= c; a ^= XROT(c, 4);  c += b; \
-    b -= a; b ^= XROT(a, 6);  a += c; \
-    c -= b; c ^= XROT(b, 8);  b += a; \
-    a -= c; a ^= XROT(c, 16); c += b; \
-    b -= a; b ^= XROT(a, 19); a += c; \
-    c -= b; c ^= XROT(b, 4);  b += a; } while (0)
-#define XFINAL(a, b, c) do { \
-    c ^= b; c -= XROT(b, 14); \
-    a ^= c; a -= XROT(c, 11); \
-    b ^= a; b -= XROT(a, 25); \
-    c ^= b; c -= XROT(b, 16); \
-    a ^= c; a -= XROT(c, 4);  \
-    b ^= a; b -= XROT(a, 14); \
-    c ^= b; c -= XROT(b, 24); } while (0)
-
-__device__ __forceinline__ uint32_t
-hash_fixed(uint64_t v, int len)
-{
-    uint32_t a, b, c;
-    a = b = c = 0x9e3779b9U + (uint32_t)len + 3923095U;
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-    if (len < 4)
-        lo &= (1U << (8 * len)) - 1;
-    if (len <= 4)
-        hi = 0;
-    else if (len < 8)
-        hi &= (1U << (8 * (len - 4))) - 1;
-    a += lo;
-    b += hi;
-    XFINAL(a, b, c);
-    return c;
-}
-
-__device__ uint32_t
-hash_bytes(const uint8_t *k, uint32_t len)
-{
-    uint32_t a, b, c;
-    a = b = c = 0x9e3779b9U + len + 3923095U;
-    while (len >= 12)
-    {
-        a += (uint32_t)k[0] | ((uint32_t)k[1] << 8) | ((uint32_t)k[2] << 16) | ((uint32_t)k[3] << 24);
-        b += (uint32_t)k[4] | ((uint32_t)k[5] << 8) | ((uint32_t)k[6] << 16) | ((uint32_t)k[7] << 24);
-        c += (uint32_t)k[8] | ((uint32_t)k[9] << 8) | ((uint32_t)k[10] << 16) | ((uint32_t)k[11] << 24);
-        XMIX(a, b, c);
-        k += 12;
-        len -= 12;
-    }
-    switch (len)
-    {
-        case 11: c += (uint32_t)k[10] << 24;    /* fall through */
-        case 10: c += (uint32_t)k[9] << 16;     /* fall through */
-        case 9:  c += (uint32_t)k[8] << 8;      /* fall through */
-        case 8:  b += (uint32_t)k[7] << 24;     /* fall through */
-        case 7:  b += (uint32_t)k[6] << 16;     /* fall through */
-        case 6:  b += (uint32_t)k[5] << 8;      /* fall through */
-        case 5:  b += k[4];                     /* fall through */
-        case 4:  a += (uint32_t)k[3] << 24;     /* fall through */
-        case 3:  a += (uint32_t)k[2] << 16;     /* fall through */
-        case 2:  a += (uint32_t)k[1] << 8;      /* fall through */
-        case 1:  a += k[0];
-    }
-    XFINAL(a, b, c);
-    return c;
-}
-
-__device__ __forceinline__ uint32_t
-hash_merge(uint32_t prev, uint32_t next)
-{
-    return ((prev >> 3) | (prev << 29)) ^ next;
-}
-
-/* ------------------------------------------------------------------
- * minimal-tuple attribute fetch (xpu_common.cu:60-130 deform rules)
- * ------------------------------------------------------------------ */
-__device__ __forceinline__ void
-ldtup(Lane &L, int d, const kern_tupitem *t, const kern_data_store *kds, int resno, int type)
-{
-    if (!t)
-    {
-        SETNULL(L, d, 1);
-        return;
-    }
-    uint16_t infomask2 = t->t_infomask2;
-    uint16_t infomask = t->t_infomask;
-    int ncols = infomask2 & HEAP_NATTS_MASK;
-    bool hasnull = (infomask & HEAP_HASNULL) != 0;
-    const char *base = (const char *)t;
-    uint32_t pos = (uint32_t)t->t_hoff - MINIMAL_TUPLE_OFFSET;
-
-    if (resno > ncols || resno > (int)kds->ncols)
-    {
-        SETNULL(L, d, 1);
-        return;
-    }
-    for (int j = 0; j < resno; j++)
-    {
-        const kern_colmeta *cm = &kds->colmeta[j];
-        int attlen = cm->attlen;
-        if (hasnull && !((t->t_bits[j >> 3] >> (j & 7)) & 1))
-        {
-            if (j == resno - 1)
-            {
-                SETNULL(L, d, 1);
-                return;
-            }
-            continue;
-        }
-        if (attlen > 0 || base[pos] == 0)
-            pos = (uint32_t)xalign(cm->attalign, pos);
-        const char *p = base + pos;
-        if (j == resno - 1)
-        {
-            int64_t v = 0;
-            switch (type)
-            {
-                case TypeOpCode__bool:
-                case TypeOpCode__int1: v = *(const int8_t *)p; break;
-                case TypeOpCode__int2: v = *(const int16_t *)p; break;
-                case TypeOpCode__int4:
-                case TypeOpCode__date: v = *(const int32_t *)p; break;
-                case TypeOpCode__float4: v = (int64_t)*(const uint32_t *)p; break;
-                case TypeOpCode__int8:
-                case TypeOpCode__float8: v = *(const int64_t *)p; break;
-                default:
-                {
-                    uint8_t b0 = (uint8_t)p[0];
-                    if (b0 == 0x01 || (b0 & 0x03) == 0x02)
-                    {
-                        if (!L.err)
-                            L.err = XMSG_TEXT_EXTERNAL + 1;
-                        SETNULL(L, d, 1);
-                        return;
-                    }
-                    if (b0 & 0x01)
-                    {
-                        L.R[d] = (int64_t)(uintptr_t)(p + 1);
-                        L.R[d + 1] = ((b0 >> 1) & 0x7f) - 1;
-                    }
-                    else
-                    {
-                        uint32_t h = *(const uint32_t *)p;
-                        L.R[d] = (int64_t)(uintptr_t)(p + 4);
-                        L.R[d + 1] = (int64_t)((h >> 2) & 0x3fffffff) - 4;
-                    }
-                    SETNULL(L, d, 0);
-                    return;
-                }
-            }
-            L.R[d] = v;
-            SETNULL(L, d, 0);
-            return;
-        }
-        if (attlen > 0)
-            pos += attlen;
-        else
-        {
-            uint8_t b0 = (uint8_t)p[0];
-            if (b0 == 0x01)
-            {
-                if (!L.err)
-                    L.err = XMSG_TEXT_EXTERNAL + 1;
-                SETNULL(L, d, 1);
-                return;
-            }
-            if (b0 & 0x01)
-                pos += (b0 >> 1) & 0x7f;
-            else
-                pos += ((*(const uint32_t *)p) >> 2) & 0x3fffffff;
-        }
-    }
-    SETNULL(L, d, 1);
-}
-
-/* ------------------------------------------------------------------
- * the register-program interpreter (one inlined copy per kernel)
- * ------------------------------------------------------------------ */
-#define RAISE(g, m) do { if (!RNULL(L, g) && L.R[g] != 0 && !L.err) L.err = (uint32_t)(m) + 1; } while (0)
-
-__device__ __forceinline__ void
-interp(const xplan *__restrict__ P, int begin, int end, Lane &L,
-       const kern_data_store *__restrict__ kds, uint32_t row, const char *const_pool)
-{
-    const char *kbase = (const char *)kds;
-    for (int pc = begin; pc < end; pc++)
-    {
-        const xop o = P->ops[pc];
-        const int d = o.dst, a = o.a, b = o.b, g = o.guard;
-        switch (o.op)
-        {
-            case XO_LDCOL:
-            {
-                const kern_colmeta *cm = &kds->colmeta[o.aux];
-                uint64_t nm_off = cm->nullmap_offset;
-                uint64_t v_off = cm->values_offset;
-                uint64_t v_len = cm->values_length;
-                int w = (int)o.imm;
-                bool isnull = false;
-                if (nm_off)
-                {
-                    uint32_t idx = row >> 3;
-                    isnull = (idx >= cm->nullmap_length) ||
-                             !((((const uint8_t *)kbase)[nm_off + idx] >> (row & 7)) & 1);
-                }
-                if ((uint64_t)w * (row + 1) > v_len)
-                    isnull = true;
-                int64_t v = 0;
-                if (!isnull)
-                {
-                    const char *p = kbase + v_off + (uint64_t)w * row;
-                    switch (w)
-                    {
-                        case 1: v = *(const int8_t *)p; break;
-                        case 2: v = *(const int16_t *)p; break;
-                        case 4: v = (b == XD_F32) ? (int64_t)*(const uint32_t *)p
-                                                  : (int64_t)*(const int32_t *)p; break;
-                        default: v = *(const int64_t *)p; break;
-                    }
-                }
-                L.R[d] = v;
-                SETNULL(L, d, isnull);
-                break;
-            }
-            case XO_LDCOL_BOOL:
-            {
-                const kern_colmeta *cm = &kds->colmeta[o.aux];
-                bool isnull = false;
-                if (cm->nullmap_offset)
-                {
-                    uint32_t idx = row >> 3;
-                    isnull = (idx >= cm->nullmap_length) ||
-                             !((((const uint8_t *)kbase)[cm->nullmap_offset + idx] >> (row & 7)) & 1);
-                }
-                if ((uint64_t)(row >> 3) >= cm->values_length)
-                    isnull = true;
-                L.R[d] = isnull ? 0 : ((((const uint8_t *)kbase)[cm->values_offset + (row >> 3)] >> (row & 7)) & 1);
-                SETNULL(L, d, isnull);
-                break;
-            }
-            case XO_LDCOL_VAR:
-            {
-                const kern_colmeta *cm = &kds->colmeta[o.aux];
-                bool isnull = false;
-                if (cm->nullmap_offset)
-                {
-                    uint32_t idx = row >> 3;
-                    isnull = (idx >= cm->nullmap_length) ||
-                             !((((const uint8_t *)kbase)[cm->nullmap_offset + idx] >> (row & 7)) & 1);
-                }
-                if (4ULL * ((uint64_t)row + 2) > cm->values_length)
-                    isnull = true;
-                int64_t ptr = 0, len = 0;
-                if (!isnull)
-                {
-                    const uint32_t *off = (const uint32_t *)(kbase + cm->values_offset);
-                    uint32_t o0 = off[row], o1 = off[row + 1];
-                    if (o0 > o1 || o1 > cm->extra_length)
-                        isnull = true;
-                    else
-                    {
-                        ptr = (int64_t)(uintptr_t)(kbase + cm->extra_offset + o0);
-                        len = o1 - o0;
-                    }
-                }
-                L.R[d] = ptr;
-                L.R[d + 1] = len;
-                SETNULL(L, d, isnull);
-                break;
-            }
-            case XO_LDTUP:
-                ldtup(L, d, (const kern_tupitem *)(uintptr_t)L.R[a],
-                      (const kern_data_store *)(uintptr_t)P->depth[o.aux].inner_kds,
-                      (int)o.imm, b);
-                break;
-            case XO_CONST:
-                L.R[d] = o.imm;
-                SETNULL(L, d, o.aux & 1);
-                break;
-            case XO_CONST_TEXT:
-                L.R[d] = (int64_t)(uintptr_t)(const_pool + (o.imm & 0xffffffffLL));
-                L.R[d + 1] = (int64_t)((uint64_t)o.imm >> 32);
-                SETNULL(L, d, 0);
-                break;
-            case XO_MOV:
-                L.R[d] = L.R[a];
-                SETNULL(L, d, RNULL(L, a));
-                break;
-            case XO_MOV2:
-                L.R[d] = L.R[a];
-                L.R[d + 1] = L.R[a + 1];
-                SETNULL(L, d, RNULL(L, a));
-                break;
-            case XO_SETNULL:
-                L.R[d] = 0;
-                SETNULL(L, d, 1);
-                break;
-            case XO_CMP_I:
-            case XO_CMP_F32:
-            case XO_CMP_F64:
-            case XO_CMP_TEXT:
-            {
-                int op = o.aux & 0xff;
-                bool na = RNULL(L, a), nb = RNULL(L, b);
-                if (na || nb)
-                {
-                    /* kmode_compare_nulls for eq/ne (xpu_basetype.h:30-60) */
-                    if ((o.aux & 0x100) && (op == XC_EQ || op == XC_NE))
-                    {
-                        bool both = na && nb;
-                        L.R[d] = (op == XC_EQ) ? both : !both;
-                        SETNULL(L, d, 0);
-                    }
-                    else
-                    {
-                        L.R[d] = 0;
-                        SETNULL(L, d, 1);
-                    }
-                    break;
-                }
-                bool r;
-                if (o.op == XO_CMP_I)
-                {
-                    int64_t x = L.R[a], y = L.R[b];
-                    r = cmp_by(op, x < y ? -1 : (x > y ? 1 : 0));
-                }
-                else if (o.op == XO_CMP_F32)
-                    r = fcmp(op, (double)as_f32(L.R[a]), (double)as_f32(L.R[b]));
-                else if (o.op == XO_CMP_F64)
-                    r = fcmp(op, as_f64(L.R[a]), as_f64(L.R[b]));
-                else
-                {
-                    const uint8_t *s1 = (const uint8_t *)(uintptr_t)L.R[a];
-                    const uint8_t *s2 = (const uint8_t *)(uintptr_t)L.R[b];
-                    int64_t l1 = L.R[a + 1], l2 = L.R[b + 1];
-                    int64_t n = l1 < l2 ? l1 : l2;
-                    int c = 0;
-                    for (int64_t i = 0; i < n && c == 0; i++)
-                        c = (int)s1[i] - (int)s2[i];
-                    if (c == 0)
-                        c = l1 < l2 ? -1 : (l1 > l2 ? 1 : 0);
-                    r = cmp_by(op, c);
-                }
-                L.R[d] = r;
-                SETNULL(L, d, 0);
-                break;
-            }
-            case XO_ADD_I: case XO_SUB_I: case XO_MUL_I: case XO_DIV_I: case XO_MOD_I:
-            {
-                bool n = RNULL(L, a) | RNULL(L, b);
-                int64_t x = L.R[a], y = L.R[b], v = 0;
-                int cls = o.aux;
-                if (!n)
-                {
-                    bool ovf = false;
-                    switch (o.op)
-                    {
-                        case XO_ADD_I: ovf = __builtin_add_overflow(x, y, &v); break;
-                        case XO_SUB_I: ovf = __builtin_sub_overflow(x, y, &v); break;
-                        case XO_MUL_I: ovf = __builtin_mul_overflow(x, y, &v); break;
-                        case XO_DIV_I:
-                            if (y == 0)
-                            {
-                                RAISE(g, o.imm & 0xffff);
-                                break;
-                            }
-                            if (y == -1)
-                            {
-                                ovf = (x == INT64_MIN);
-                                v = -x;
-                            }
-                            else
-                                v = x / y;
-                            break;
-                        case XO_MOD_I:
-                            if (y == 0)
-                            {
-                                RAISE(g, o.imm & 0xffff);
-                                break;
-                            }
-                            v = (y == -1) ? 0 : x % y;
-                            break;
-                    }
-                    if (ovf || !int_in_class(cls, v))
-                        RAISE(g, (o.imm >> 16) & 0xffff);
-                }
-                L.R[d] = v;
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_NEG_I: case XO_ABS_I:
-            {
-                bool n = RNULL(L, a);
-                int64_t x = L.R[a], v = 0;
-                if (!n)
-                {
-                    if (x == int_class_min(o.aux) && (o.op == XO_NEG_I || x < 0))
-                        RAISE(g, o.imm);
-                    v = (o.op == XO_NEG_I) ? -x : (x < 0 ? -x : x);
-                }
-                L.R[d] = v;
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_NOT_I:
-                L.R[d] = wrap_class(o.aux, ~L.R[a]);
-                SETNULL(L, d, RNULL(L, a));
-                break;
-            case XO_AND_I: case XO_OR_I: case XO_XOR_I: case XO_SHL_I: case XO_SHR_I:
-            {
-                bool n = RNULL(L, a) | RNULL(L, b);
-                int64_t x = L.R[a], y = L.R[b], v;
-                int bits = (o.aux == XT_INT2) ? 16 : (o.aux == XT_INT4) ? 32 :
-                           (o.aux == XT_INT8) ? 64 : 8;
-                switch (o.op)
-                {
-                    case XO_AND_I: v = x & y; break;
-                    case XO_OR_I:  v = x | y; break;
-                    case XO_XOR_I: v = x ^ y; break;
-                    case XO_SHL_I: v = (int64_t)((uint64_t)x << (y & (bits - 1))); break;
-                    default:       v = x >> (y & (bits - 1)); break;
-                }
-                L.R[d] = wrap_class(o.aux, v);
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_ADD_F32: case XO_SUB_F32: case XO_MUL_F32: case XO_DIV_F32:
-            {
-                bool n = RNULL(L, a) | RNULL(L, b);
-                float x = as_f32(L.R[a]), y = as_f32(L.R[b]), v = 0.0f;
-                if (!n)
-                {
-                    switch (o.op)
-                    {
-                        case XO_ADD_F32: v = x + y; break;
-                        case XO_SUB_F32: v = x - y; break;
-                        case XO_MUL_F32: v = x * y; break;
-                        default:
-                            if (y == 0.0f)
-                                RAISE(g, (o.imm >> 16) & 0xffff);
-                            v = x / y;
-                            if (v == 0.0f && x != 0.0f)
-                                RAISE(g, o.imm & 0xffff);
-                            break;
-                    }
-                    if (isinf(v) && !isinf(x) && !isinf(y))
-                        RAISE(g, o.imm & 0xffff);
-                }
-                L.R[d] = f32_bits(v);
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_ADD_F64: case XO_SUB_F64: case XO_MUL_F64: case XO_DIV_F64:
-            {
-                bool n = RNULL(L, a) | RNULL(L, b);
-                double x = as_f64(L.R[a]), y = as_f64(L.R[b]), v = 0.0;
-                if (!n)
-                {
-                    switch (o.op)
-                    {
-                        case XO_ADD_F64: v = x + y; break;
-                        case XO_SUB_F64: v = x - y; break;
-                        case XO_MUL_F64: v = x * y; break;
-                        default:
-                            if (y == 0.0)
-                                RAISE(g, (o.imm >> 16) & 0xffff);
-                            v = x / y;
-                            if (v == 0.0 && x != 0.0)
-                                RAISE(g, o.imm & 0xffff);
-                            break;
-                    }
-                    if (isinf(v) && !isinf(x) && !isinf(y))
-                        RAISE(g, o.imm & 0xffff);
-                }
-                L.R[d] = f64_bits(v);
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_NEG_F32: L.R[d] = f32_bits(-as_f32(L.R[a])); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_ABS_F32: L.R[d] = f32_bits(fabsf(as_f32(L.R[a]))); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_NEG_F64: L.R[d] = f64_bits(-as_f64(L.R[a])); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_ABS_F64: L.R[d] = f64_bits(fabs(as_f64(L.R[a]))); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_I2I:
-            {
-                bool n = RNULL(L, a);
-                if (!n && !int_in_class(o.aux, L.R[a]))
-                    RAISE(g, o.imm);
-                L.R[d] = L.R[a];
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_F32_I:
-            case XO_F64_I:
-            {
-                bool n = RNULL(L, a);
-                int64_t v = 0;
-                if (!n)
-                {
-                    /* rint then range check (xpu_basetype.cu:345-400) */
-                    bool bad;
-                    if (o.op == XO_F32_I)
-                    {
-                        float x = as_f32(L.R[a]);
-                        float r = rintf(x);
-                        bad = isnan(x) || r < (float)int_class_min(o.aux) ||
-                              (o.aux == XT_INT8 ? r >= 9223372036854775808.0f
-                                                : r > -(float)int_class_min(o.aux) - 1.0f);
-                        v = bad ? 0 : (int64_t)r;
-                    }
-                    else
-                    {
-                        double x = as_f64(L.R[a]);
-                        double r = rint(x);
-                        bad = isnan(x) || r < (double)int_class_min(o.aux) ||
-                              (o.aux == XT_INT8 ? r >= 9223372036854775808.0
-                                                : r > -(double)int_class_min(o.aux) - 1.0);
-                        v = bad ? 0 : (int64_t)r;
-                    }
-                    if (bad)
-                        RAISE(g, o.imm);
-                }
-                L.R[d] = v;
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_I_F32: L.R[d] = f32_bits((float)L.R[a]); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_I_F64: L.R[d] = f64_bits((double)L.R[a]); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_F32_F64: L.R[d] = f64_bits((double)as_f32(L.R[a])); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_F64_F32:
-            {
-                bool n = RNULL(L, a);
-                double x = as_f64(L.R[a]);
-                if (!n && (isinf(x) || isnan(x) || x < -3.40282346638528859812e+38 ||
-                           x > 3.40282346638528859812e+38))
-                    RAISE(g, o.imm);
-                L.R[d] = f32_bits((float)x);
-                SETNULL(L, d, n);
-                break;
-            }
-            case XO_I_BOOL: L.R[d] = (L.R[a] != 0); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_TEXTLEN: L.R[d] = L.R[a + 1]; SETNULL(L, d, RNULL(L, a)); break;
-            case XO_AND3:
-            {
-                bool na = RNULL(L, a), nb = RNULL(L, b);
-                bool fa = !na && L.R[a] == 0, fb = !nb && L.R[b] == 0;
-                if (fa || fb)
-                {
-                    L.R[d] = 0;
-                    SETNULL(L, d, 0);
-                }
-                else
-                {
-                    L.R[d] = !(na || nb);
-                    SETNULL(L, d, na || nb);
-                }
-                break;
-            }
-            case XO_OR3:
-            {
-                bool na = RNULL(L, a), nb = RNULL(L, b);
-                bool ta = !na && L.R[a] != 0, tb = !nb && L.R[b] != 0;
-                if (ta || tb)
-                {
-                    L.R[d] = 1;
-                    SETNULL(L, d, 0);
-                }
-                else
-                {
-                    L.R[d] = 0;
-                    SETNULL(L, d, na || nb);
-                }
-                break;
-            }
-            case XO_NOT3:
-                L.R[d] = RNULL(L, a) ? 0 : (L.R[a] == 0);
-                SETNULL(L, d, RNULL(L, a));
-                break;
-            case XO_ISNULL:
-                L.R[d] = RNULL(L, a);
-                SETNULL(L, d, 0);
-                break;
-            case XO_ISNOTNULL:
-                L.R[d] = !RNULL(L, a);
-                SETNULL(L, d, 0);
-                break;
-            case XO_BOOLTEST:
-            {
-                bool n = RNULL(L, a), v = L.R[a] != 0, r;
-                switch (o.aux)
-                {
-                    case 0: r = !n && v; break;
-                    case 1: r = n || !v; break;
-                    case 2: r = !n && !v; break;
-                    case 3: r = n || v; break;
-                    case 4: r = n; break;
-                    default: r = !n; break;
-                }
-                L.R[d] = r;
-                SETNULL(L, d, 0);
-                break;
-            }
-            case XO_COALESCE:
-                if (RNULL(L, d))
-                {
-                    L.R[d] = L.R[a];
-                    if (b)
-                        L.R[d + 1] = L.R[a + 1];
-                    SETNULL(L, d, RNULL(L, a));
-                }
-                break;
-            case XO_G_NOTFALSE:
-                L.R[d] = (L.R[b] != 0) && (RNULL(L, a) || L.R[a] != 0);
-                SETNULL(L, d, 0);
-                break;
-            case XO_G_NOTTRUE:
-                L.R[d] = (L.R[b] != 0) && (RNULL(L, a) || L.R[a] == 0);
-                SETNULL(L, d, 0);
-                break;
-            case XO_G_ISNULL:
-                L.R[d] = (L.R[b] != 0) && RNULL(L, a);
-                SETNULL(L, d, 0);
-                break;
-            case XO_G_POSITIVE:
-                L.R[d] = (L.R[b] != 0) && !RNULL(L, a) && L.R[a] > 0;
-                SETNULL(L, d, 0);
-                break;
-            case XO_G_NONZERO:
-                L.R[d] = (L.R[b] != 0) && !RNULL(L, a) && L.R[a] != 0;
-                SETNULL(L, d, 0);
-                break;
-            case XO_JQ_STATUS:
-                if (L.R[b] != 0 && (RNULL(L, a) || L.R[a] == 0))
-                    L.R[d] = o.imm;
-                break;
-            case XO_HASH_INIT:
-                L.R[d] = 0xffffffffLL;
-                SETNULL(L, d, 0);
-                break;
-            case XO_HASH_ADD:
-                if (!RNULL(L, a))
-                {
-                    uint32_t h;
-                    if (o.aux == 0)
-                        h = hash_bytes((const uint8_t *)(uintptr_t)L.R[a], (uint32_t)L.R[a + 1]);
-                    else
-                        h = hash_fixed((uint64_t)L.R[a], o.aux);
-                    L.R[d] = hash_merge((uint32_t)L.R[d], h);
-                }
-                break;
-            case XO_HASH_FIN:
-                L.R[d] = (uint32_t)L.R[d] ^ 0xffffffffU;
-                break;
-            default:
-                break;
-        }
-    }
-}
-
-/* ------------------------------------------------------------------
- * wave helpers (wave64)
- * ------------------------------------------------------------------ */
-__device__ __forceinline__ bool
-wave_any(bool p)
-{
-    return __ballot(p) != 0;
-}
-
-__device__ __forceinline__ uint64_t
-wave_incl_scan_u64(uint64_t v)
-{
-    int lane = __lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1)
-    {
-        uint64_t t = __shfl_up(v, off, 64);
-        if (lane >= off)
-            v += t;
-    }
-    return v;
-}
-
-__device__ __forceinline__ void
-record_error(xpu_task_ctl *ctl, uint32_t msgid, uint32_t code)
-{
-    if (atomicCAS(&ctl->errcode, 0U, code) == 0U)
-        __hip_atomic_store(&ctl->errmsg, msgid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-/* ------------------------------------------------------------------
- * projection: kern_form_minimal_tuple (xpu_common.cu:1096-1190)
- * ------------------------------------------------------------------ */
-__device__ __forceinline__ uint32_t
-proj_tuple_size(const xplan *P, const Lane &L)
-{
-    int nattrs = P->proj_nattrs;
-    bool hasnull = false;
-    for (int j = 0; j < nattrs; j++)
-    {
-        int r = P->proj[j].reg;
-        if (r == 0xff || RNULL(L, r))
-            hasnull = true;
-    }
-    uint32_t t_hoff = KERN_TUPITEM_T_BITS_OFFSET + (hasnull ? XPU_BITMAPLEN(nattrs) : 0);
-    t_hoff = (uint32_t)xalign(8, t_hoff);
-    for (int j = 0; j < nattrs; j++)
-    {
-        xattr pa = P->proj[j];
-        if (pa.reg == 0xff || RNULL(L, pa.reg))
-            continue;
-        t_hoff = (uint32_t)xalign(pa.attalign, t_hoff);
-        t_hoff += (pa.width > 0) ? pa.width : 4 + (uint32_t)L.R[pa.reg + 1];
-    }
-    return t_hoff + ROWID_SIZE;
-}
-
-__device__ __forceinline__ void
-proj_tuple_write(const xplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t rowid)
-{
-    int nattrs = P->proj_nattrs;
-    bool hasnull = false;
-    for (int j = 0; j < nattrs; j++)
-    {
-        int r = P->proj[j].reg;
-        if (r == 0xff || RNULL(L, r))
-            hasnull = true;
-    }
-    uint32_t t_hoff = KERN_TUPITEM_T_BITS_OFFSET + (hasnull ? XPU_BITMAPLEN(nattrs) : 0);
-    t_hoff = (uint32_t)xalign(8, t_hoff);
-    /* the tuple unit (MAXALIGN'ed) is cleared 8 bytes at a time */
-    for (uint32_t i = 0; i < unit; i += 8)
-        *(uint64_t *)(buf + i) = 0;
-    uint16_t infomask = hasnull ? HEAP_HASNULL : 0;
-    uint8_t *bits = (uint8_t *)buf + KERN_TUPITEM_T_BITS_OFFSET;
-    for (int j = 0; j < nattrs; j++)
-    {
-        xattr pa = P->proj[j];
-        if (pa.reg == 0xff || RNULL(L, pa.reg))
-            continue;
-        t_hoff = (uint32_t)xalign(pa.attalign, t_hoff);
-        char *p = buf + t_hoff;
-        int64_t v = L.R[pa.reg];
-        switch (pa.width)
-        {
-            case 1: *(int8_t *)p = (int8_t)v; t_hoff += 1; break;
-            case 2: *(int16_t *)p = (int16_t)v; t_hoff += 2; break;
-            case 4: *(int32_t *)p = (int32_t)v; t_hoff += 4; break;
-            case 8: *(int64_t *)p = v; t_hoff += 8; break;
-            default:
-            {
-                uint32_t len = (uint32_t)L.R[pa.reg + 1];
-                const uint8_t *s = (const uint8_t *)(uintptr_t)v;
-                *(uint32_t *)p = (len + 4) << 2;
-                for (uint32_t i = 0; i < len; i++)
-                    p[4 + i] = s[i];
-                t_hoff += 4 + len;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-            }
-        }
-        if (hasnull)
-            bits[j >> 3] |= (uint8_t)(1 << (j & 7));
-    }
-    t_hoff += ROWID_SIZE;
-    kern_tupitem *t = (kern_tupitem *)buf;
-    t->t_len = t_hoff;
-    t->hash = 0;
-    t->t_infomask2 = (uint16_t)(nattrs & HEAP_NATTS_MASK);
-    t->t_infomask = infomask;
-    t->t_hoff = (uint8_t)((uint32_t)xalign(8, KERN_TUPITEM_T_BITS_OFFSET +
-                                          (hasnull ? XPU_BITMAPLEN(nattrs) : 0)) +
-                          MINIMAL_TUPLE_OFFSET);
-    *(uint32_t *)(buf + t_hoff - ROWID_SIZE) = rowid;
-}
-
-/* ------------------------------------------------------------------
- * group-by: kds_final group tuples (cuda_gpupreagg.cu:18-223, 1112-1191)
- * ------------------------------------------------------------------ */
-__device__ __forceinline__ int64_t
-val_as_i64(int dom, int64_t raw)
-{
-    if (dom == XD_F32) return (int64_t)as_f32(raw);
-    if (dom == XD_F64) return (int64_t)as_f64(raw);
-    return raw;
-}
-
-__device__ __forceinline__ double
-val_as_f64(int dom, int64_t raw)
-{
-    if (dom == XD_F32) return (double)as_f32(raw);
-    if (dom == XD_F64) return as_f64(raw);
-    return (double)raw;
-}
-
-/* payload offset of grouping key k inside a group tuple, or -1 if NULL;
- * keys follow the states and NULL keys still take their alignment
- * (cuda_gpupreagg.cu:44-70) */
-__device__ __forceinline__ uint32_t
-gb_tuple_size(const xplan *P, const Lane &L)
-{
-    uint32_t t_hoff = P->gb_t_hoff + P->gb_states_end;
-    for (int j = 0; j < (int)P->gb_nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action != KAGG_ACTION__VREF)
-            continue;
-        xattr ka = P->keys[ag.key_index];
-        t_hoff = (uint32_t)xalign(ka.attalign, t_hoff);
-        if (!RNULL(L, ka.reg))
-            t_hoff += (ka.width > 0) ? ka.width : 4 + (uint32_t)L.R[ka.reg + 1];
-    }
-    return t_hoff + ROWID_SIZE;
-}
-
-__device__ __forceinline__ bool
-gb_keys_match(const xplan *P, const Lane &L, const kern_tupitem *t)
-{
-    const char *base = (const char *)t;
-    uint32_t pos = P->gb_t_hoff + P->gb_states_end;
-    for (int j = 0; j < (int)P->gb_nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action != KAGG_ACTION__VREF)
-            continue;
-        xattr ka = P->keys[ag.key_index];
-        pos = (uint32_t)xalign(ka.attalign, pos);
-        bool tnull = !((t->t_bits[j >> 3] >> (j & 7)) & 1);
-        bool lnull = RNULL(L, ka.reg);
-        if (tnull || lnull)
-        {
-            if (tnull != lnull)
-                return false;   /* NULLs group together (kmode_compare_nulls) */
-            continue;
-        }
-        const char *p = base + pos;
-        int64_t v = L.R[ka.reg];
-        switch (ka.width)
-        {
-            case 1: if (*(const int8_t *)p != (int8_t)v) return false; pos += 1; break;
-            case 2: if (*(const int16_t *)p != (int16_t)v) return false; pos += 2; break;
-            case 4:
-                if (ka.domain == XD_F32)
-                {
-                    if (!fcmp(XC_EQ, (double)__int_as_float(*(const int32_t *)p), (double)as_f32(v)))
-                        return false;
-                }
-                else if (*(const int32_t *)p != (int32_t)v)
-                    return false;
-                pos += 4;
-                break;
-            case 8:
-                if (ka.domain == XD_F64)
-                {
-                    if (!fcmp(XC_EQ, __longlong_as_double(*(const int64_t *)p), as_f64(v)))
-                        return false;
-                }
-                else if (*(const int64_t *)p != v)
-                    return false;
-                pos += 8;
-                break;
-            default:
-            {
-                uint32_t len = (uint32_t)L.R[ka.reg + 1];
-                uint32_t hdr = *(const uint32_t *)p;
-                if (((hdr >> 2) & 0x3fffffff) != len + 4)
-                    return false;
-                const uint8_t *s = (const uint8_t *)(uintptr_t)v;
-                for (uint32_t i = 0; i < len; i++)
-                    if ((uint8_t)p[4 + i] != s[i])
-                        return false;
-                pos += 4 + len;
-                break;
-            }
-        }
-    }
-    return true;
-}
-
-__device__ __forceinline__ void
-gb_tuple_write(const xplan *P, const Lane &L, kern_hashitem *hi, uint32_t unit,
-               uint32_t hash, uint32_t rowid, uint64_t next)
-{
-    char *ubuf = (char *)hi;
-    for (uint32_t i = 0; i < unit; i += 8)
-        *(uint64_t *)(ubuf + i) = 0;
-    hi->next = next;
-    char *buf = (char *)&hi->t;
-    kern_tupitem *t = &hi->t;
-    uint16_t infomask = HEAP_HASNULL;
-    uint8_t *bits = t->t_bits;
-    uint32_t nattrs = P->gb_nattrs;
-    uint32_t t_hoff = P->gb_t_hoff;
-    /* partial states, in their initial values */
-    for (int j = 0; j < (int)nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action == KAGG_ACTION__VREF)
-            continue;
-        char *s = buf + t_hoff + ag.state_off;
-        bits[j >> 3] |= (uint8_t)(1 << (j & 7));
-        switch (ag.action)
-        {
-            case KAGG_ACTION__NROWS_ANY:
-            case KAGG_ACTION__NROWS_COND:
-                break;
-            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
-                *(int32_t *)s = ag.state_size << 2;
-                *(int64_t *)(s + 8) = INT64_MAX;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
-                *(int32_t *)s = ag.state_size << 2;
-                *(int64_t *)(s + 8) = INT64_MIN;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-            case KAGG_ACTION__PMIN_FP64:
-                *(int32_t *)s = ag.state_size << 2;
-                *(double *)(s + 8) = 1.7976931348623157e308;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-            case KAGG_ACTION__PMAX_FP64:
-                *(int32_t *)s = ag.state_size << 2;
-                *(double *)(s + 8) = -1.7976931348623157e308;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-            default:
-                *(int32_t *)s = ag.state_size << 2;
-                infomask |= HEAP_HASVARWIDTH;
-                break;
-        }
-    }
-    /* grouping keys */
-    uint32_t pos = t_hoff + P->gb_states_end;
-    for (int j = 0; j < (int)nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action != KAGG_ACTION__VREF)
-            continue;
-        xattr ka = P->keys[ag.key_index];
-        pos = (uint32_t)xalign(ka.attalign, pos);
-        if (RNULL(L, ka.reg))
-            continue;
-        bits[j >> 3] |= (uint8_t)(1 << (j & 7));
-        char *p = buf + pos;
-        int64_t v = L.R[ka.reg];
-        switch (ka.width)
-        {
-            case 1: *(int8_t *)p = (int8_t)v; pos += 1; break;
-            case 2: *(int16_t *)p = (int16_t)v; pos += 2; break;
-            case 4: *(int32_t *)p = (int32_t)v; pos += 4; break;
-            case 8: *(int64_t *)p = v; pos += 8; break;
-            default:
-            {
-                uint32_t len = (uint32_t)L.R[ka.reg + 1];
-                const uint8_t *s = (const uint8_t *)(uintptr_t)v;
-                *(uint32_t *)p = (len + 4) << 2;
-                for (uint32_t i = 0; i < len; i++)
-                    p[4 + i] = s[i];
-                pos += 4 + len;
-                break;
-            }
-        }
-    }
-    pos += ROWID_SIZE;
-    t->t_len = pos;
-    t->hash = hash;
-    t->t_infomask2 = (uint16_t)(nattrs & HEAP_NATTS_MASK);
-    t->t_infomask = infomask;
-    t->t_hoff = (uint8_t)(t_hoff + MINIMAL_TUPLE_OFFSET);
-    *(uint32_t *)(buf + pos - ROWID_SIZE) = rowid;
-}
-
-/* LDS partial table view */
-struct LdsTable
-{
-    unsigned long long *keys;   /* [n] */
-    uint32_t   *flags;          /* [n] VALID bits of min/max actions */
-    unsigned long long *arr;    /* [arrays][n] */
-    uint32_t    n;
-};
-
-/* merge one contribution (a row, or an LDS entry) into a group's states */
-__device__ __forceinline__ void
-gb_merge_global(const xplan *P, const Lane &L, kern_tupitem *t, const LdsTable *T,
-                int lds_slot)
-{
-    char *payload = (char *)t + P->gb_t_hoff;
-    bool from_lds = (lds_slot >= 0);
-    for (int j = 0; j < (int)P->gb_nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action == KAGG_ACTION__VREF)
-            continue;
-        char *s = payload + ag.state_off;
-        unsigned long long *A = from_lds ? T->arr + (uint64_t)ag.lds_off * T->n + lds_slot : nullptr;
-        bool has = false;
-        int64_t iv = 0;
-        double fv = 0.0;
-        if (!from_lds)
-        {
-            bool pass = (ag.filter_reg == 0xff) ||
-                        (!RNULL(L, ag.filter_reg) && L.R[ag.filter_reg] != 0);
-            if (!pass)
-                continue;
-            if (ag.arg_reg != 0xff && !RNULL(L, ag.arg_reg))
-            {
-                has = true;
-                iv = val_as_i64(ag.arg_domain, L.R[ag.arg_reg]);
-                fv = val_as_f64(ag.arg_domain, L.R[ag.arg_reg]);
-            }
-        }
-        switch (ag.action)
-        {
-            case KAGG_ACTION__NROWS_ANY:
-            case KAGG_ACTION__NROWS_COND:
-            {
-                unsigned long long c = from_lds ? A[0] :
-                    ((ag.action == KAGG_ACTION__NROWS_ANY || has) ? 1ULL : 0ULL);
-                if (c)
-                    atomicAdd((unsigned long long *)s, c);
-                break;
-            }
-            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
-            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
-            {
-                bool valid = from_lds ? ((T->flags[lds_slot] >> (j & 31)) & 1) : has;
-                if (!valid)
-                    break;
-                long long v = from_lds ? (long long)A[0] : (long long)iv;
-                atomicOr((unsigned int *)(s + 4), __PAGG_MINMAX_ATTRS__VALID);
-                if (ag.action == KAGG_ACTION__PMIN_INT32 || ag.action == KAGG_ACTION__PMIN_INT64)
-                    atomicMin((long long *)(s + 8), v);
-                else
-                    atomicMax((long long *)(s + 8), v);
-                break;
-            }
-            case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64:
-            {
-                bool valid = from_lds ? ((T->flags[lds_slot] >> (j & 31)) & 1) : has;
-                if (!valid)
-                    break;
-                double v = from_lds ? __longlong_as_double((long long)A[0]) : fv;
-                atomicOr((unsigned int *)(s + 4), __PAGG_MINMAX_ATTRS__VALID);
-                if (ag.action == KAGG_ACTION__PMIN_FP64)
-                    atomicMin((double *)(s + 8), v);
-                else
-                    atomicMax((double *)(s + 8), v);
-                break;
-            }
-            case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
-            {
-                unsigned long long n = from_lds ? A[0] : (has ? 1ULL : 0ULL);
-                if (!n)
-                    break;
-                unsigned long long sum = from_lds ? A[T->n] : (unsigned long long)iv;
-                atomicAdd((unsigned long long *)(s + 8), n);
-                atomicAdd((unsigned long long *)(s + 16), sum);
-                break;
-            }
-            case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
-            {
-                unsigned long long n = from_lds ? A[0] : (has ? 1ULL : 0ULL);
-                if (!n)
-                    break;
-                unsigned long long lo = from_lds ? A[T->n] : (unsigned long long)iv;
-                unsigned long long hi = from_lds ? A[2 * T->n] : (iv < 0 ? ~0ULL : 0ULL);
-                atomicAdd((unsigned long long *)(s + 8), n);
-                /* int128 as lo/hi with the carry of the low word
-                 * (xpu_common.h:3659-3689) */
-                unsigned long long old = atomicAdd((unsigned long long *)(s + 16), lo);
-                hi += (old + lo < old) ? 1ULL : 0ULL;
-                if (hi)
-                    atomicAdd((unsigned long long *)(s + 24), hi);
-                break;
-            }
-            case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
-            {
-                unsigned long long n = from_lds ? A[0] : (has ? 1ULL : 0ULL);
-                if (!n)
-                    break;
-                double sum = from_lds ? __longlong_as_double((long long)A[T->n]) : fv;
-                atomicAdd((unsigned long long *)(s + 8), n);
-                atomicAdd((double *)(s + 16), sum);
-                break;
-            }
-            case KAGG_ACTION__STDDEV:
-            {
-                unsigned long long n = from_lds ? A[0] : (has ? 1ULL : 0ULL);
-                if (!n)
-                    break;
-                double sx = from_lds ? __longlong_as_double((long long)A[T->n]) : fv;
-                double sxx = from_lds ? __longlong_as_double((long long)A[2 * T->n]) : fv * fv;
-                atomicAdd((unsigned long long *)(s + 8), n);
-                atomicAdd((double *)(s + 16), sx);
-                atomicAdd((double *)(s + 24), sxx);
-                break;
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ kern_hashitem *
-hashitem_at(kern_data_store *kds, uint64_t off)
-{
-    return (kern_hashitem *)((char *)kds + kds->length - off);
-}
-
-/* Find-or-insert the group of the key registers of every pending lane and
- * merge its contribution; a wave-level retry loop in which a lane either
- * finds the group, or takes the slot lock once and inserts it, so lanes of
- * one wave never wait on each other's locks (the reference does the same at
- * block level, cuda_gpupreagg.cu:1607-1731). */
-__device__ __forceinline__ void
-gb_global(const xplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pending,
-          const LdsTable *T, int lds_slot, xpu_task_ctl *ctl)
-{
-    uint32_t nslots = kds_final->hash_nslots;
-    uint32_t iter = 0;
-
-    if (P->nkeys == 0)
-    {
-        /* no GROUP BY: the single group tuple of a ROW-format kds_final */
-        if (pending)
-        {
-            uint64_t *rowidx = (uint64_t *)((char *)kds_final + KDS_HEAD_LENGTH(kds_final));
-            kern_tupitem *t = (kern_tupitem *)((char *)kds_final + kds_final->length - rowidx[0]);
-            gb_merge_global(P, L, t, T, lds_slot);
-        }
-        return;
-    }
-    uint64_t *slots = (uint64_t *)((char *)kds_final + KDS_HEAD_LENGTH(kds_final));
-    uint64_t *slot = slots + (hash % nslots);
-    while (wave_any(pending))
-    {
-        if (pending)
-        {
-            /* Slots and the fields of published group tuples are read with
-             * relaxed device-coherent loads: the inserting lane writes a tuple
-             * into a fresh 128-byte unit no other XCD has cached and publishes
-             * it with a release store, so no per-row acquire (an L2
-             * invalidate on CDNA) is needed on the lookup path. */
-            uint64_t head = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            kern_hashitem *found = nullptr;
-            if (head != XGB_LOCKED_SLOT)
-            {
-                for (uint64_t off = head; off != 0; )
-                {
-                    kern_hashitem *hi = hashitem_at(kds_final, off);
-                    uint32_t h2 = __hip_atomic_load(&hi->t.hash, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-                    if (h2 == hash && gb_keys_match(P, L, &hi->t))
-                    {
-                        found = hi;
-                        break;
-                    }
-                    off = __hip_atomic_load(&hi->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                if (!found &&
-                    atomicCAS((unsigned long long *)slot, (unsigned long long)head,
-                              (unsigned long long)XGB_LOCKED_SLOT) == head)
-                {
-                    /* slot locked: nobody inserted since `head` was read */
-                    uint32_t tsz = gb_tuple_size(P, L);
-                    uint32_t unit = (uint32_t)xalign(XPU_L1_CACHELINE_SZ,
-                                                     offsetof(kern_hashitem, t) + tsz);
-                    uint32_t rowid = atomicAdd(&kds_final->nitems, 1U);
-                    unsigned long long usage =
-                        atomicAdd((unsigned long long *)&kds_final->usage, (unsigned long long)unit) + unit;
-                    uint64_t need = KDS_HEAD_LENGTH(kds_final) +
-                                    8ULL * ((uint64_t)nslots + rowid + 1) + usage;
-                    if (need > kds_final->length)
-                    {
-                        record_error(ctl, XMSG_FINAL_NO_SPACE, ERRCODE_SUSPEND_NO_SPACE);
-                        __hip_atomic_store(slot, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        pending = false;
-                    }
-                    else
-                    {
-                        kern_hashitem *hi = hashitem_at(kds_final, usage);
-                        gb_tuple_write(P, L, hi, unit, hash, rowid, head);
-                        uint64_t *rowidx = slots + nslots;
-                        rowidx[rowid] = (uint64_t)((char *)kds_final + kds_final->length -
-                                                   (char *)&hi->t);
-                        __hip_atomic_store(slot, (uint64_t)usage, __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                        found = hi;
-                    }
-                }
-            }
-            if (found)
-            {
-                gb_merge_global(P, L, &found->t, T, lds_slot);
-                pending = false;
-            }
-        }
-        if (++iter > (1U << 24))
-        {
-            if (pending)
-                record_error(ctl, XMSG_GROUP_RETRY, ERRCODE_DEVICE_ERROR);
-            break;
-        }
-    }
-}
-
-__device__ __forceinline__ uint64_t
-gb_pack_key(const xplan *P, const Lane &L)
-{
-    uint64_t pk = 0;
-    int shift = 0;
-    for (int k = 0; k < (int)P->nkeys; k++)
-    {
-        int bits = P->key_bits[k];
-        int r = P->keys[k].reg;
-        uint64_t v = RNULL(L, r) ? 0 : ((uint64_t)L.R[r] & ((bits >= 64) ? ~0ULL : ((1ULL << bits) - 1)));
-        pk |= v << shift;
-        shift += bits;
-    }
-    for (int k = 0; k < (int)P->nkeys; k++)
-        pk |= (uint64_t)RNULL(L, P->keys[k].reg) << (shift + k);
-    return pk;
-}
-
-__device__ __forceinline__ void
-gb_unpack_key(const xplan *P, Lane &L, uint64_t pk)
-{
-    int shift = 0;
-    for (int k = 0; k < (int)P->nkeys; k++)
-        shift += P->key_bits[k];
-    int vshift = 0;
-    for (int k = 0; k < (int)P->nkeys; k++)
-    {
-        int bits = P->key_bits[k];
-        int r = P->keys[k].reg;
-        uint64_t v = (pk >> vshift) & ((bits >= 64) ? ~0ULL : ((1ULL << bits) - 1));
-        int64_t sv;
-        if (P->keys[k].domain == XD_INT && bits < 64)
-            sv = (int64_t)(v << (64 - bits)) >> (64 - bits);     /* sign extend */
-        else
-            sv = (int64_t)v;
-        L.R[r] = sv;
-        SETNULL(L, r, (pk >> (shift + k)) & 1);
-        vshift += bits;
-    }
-}
-
-/* PG hash of the grouping keys held in registers (HashValue semantics) */
-__device__ __forceinline__ uint32_t
-gb_hash_regs(const xplan *P, const Lane &L)
-{
-    uint32_t h = 0xffffffffU;
-    for (int k = 0; k < (int)P->nkeys; k++)
-    {
-        xattr ka = P->keys[k];
-        if (RNULL(L, ka.reg))
-            continue;
-        uint32_t x = (ka.domain == XD_TEXT)
-            ? hash_bytes((const uint8_t *)(uintptr_t)L.R[ka.reg], (uint32_t)L.R[ka.reg + 1])
-            : hash_fixed((uint64_t)L.R[ka.reg], ka.width);
-        h = hash_merge(h, x);
-    }
-    return h ^ 0xffffffffU;
-}
-
-__device__ __forceinline__ uint32_t
-lds_hash(uint64_t k)
-{
-    k *= 0x9E3779B97F4A7C15ULL;
-    return (uint32_t)(k >> 32);
-}
-
-/* returns the LDS slot of packed key pk, inserting it; -1 if the probe fails */
-__device__ __forceinline__ int
-lds_probe(LdsTable &T, uint64_t pk)
-{
-    uint32_t mask = T.n - 1;
-    uint32_t h = lds_hash(pk) & mask;
-    for (int i = 0; i < 16; i++)
-    {
-        unsigned long long cur = T.keys[h];
-        if (cur == pk)
-            return (int)h;
-        if (cur == XLDS_EMPTY_KEY)
-        {
-            unsigned long long old = atomicCAS(&T.keys[h], XLDS_EMPTY_KEY, (unsigned long long)pk);
-            if (old == XLDS_EMPTY_KEY || old == pk)
-                return (int)h;
-        }
-        h = (h + 1) & mask;
-    }
-    return -1;
-}
-
-__device__ __forceinline__ void
-lds_update(const xplan *P, const Lane &L, LdsTable &T, int slot)
-{
-    for (int j = 0; j < (int)P->gb_nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action == KAGG_ACTION__VREF)
-            continue;
-        if (ag.filter_reg != 0xff && (RNULL(L, ag.filter_reg) || L.R[ag.filter_reg] == 0))
-            continue;
-        unsigned long long *A = T.arr + (uint64_t)ag.lds_off * T.n + slot;
-        bool has = ag.arg_reg != 0xff && !RNULL(L, ag.arg_reg);
-        int64_t raw = has ? L.R[ag.arg_reg] : 0;
-        switch (ag.action)
-        {
-            case KAGG_ACTION__NROWS_ANY:
-                atomicAdd(A, 1ULL);
-                break;
-            case KAGG_ACTION__NROWS_COND:
-                if (has)
-                    atomicAdd(A, 1ULL);
-                break;
-            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
-                if (has)
-                {
-                    atomicMin((long long *)A, (long long)val_as_i64(ag.arg_domain, raw));
-                    atomicOr(&T.flags[slot], 1U << (j & 31));
-                }
-                break;
-            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
-                if (has)
-                {
-                    atomicMax((long long *)A, (long long)val_as_i64(ag.arg_domain, raw));
-                    atomicOr(&T.flags[slot], 1U << (j & 31));
-                }
-                break;
-            case KAGG_ACTION__PMIN_FP64:
-                if (has)
-                {
-                    atomicMin((double *)A, val_as_f64(ag.arg_domain, raw));
-                    atomicOr(&T.flags[slot], 1U << (j & 31));
-                }
-                break;
-            case KAGG_ACTION__PMAX_FP64:
-                if (has)
-                {
-                    atomicMax((double *)A, val_as_f64(ag.arg_domain, raw));
-                    atomicOr(&T.flags[slot], 1U << (j & 31));
-                }
-                break;
-            case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
-                if (has)
-                {
-                    atomicAdd(A, 1ULL);
-                    atomicAdd(A + T.n, (unsigned long long)val_as_i64(ag.arg_domain, raw));
-                }
-                break;
-            case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
-                if (has)
-                {
-                    int64_t v = val_as_i64(ag.arg_domain, raw);
-                    atomicAdd(A, 1ULL);
-                    unsigned long long old = atomicAdd(A + T.n, (unsigned long long)v);
-                    unsigned long long hi = (v < 0 ? ~0ULL : 0ULL) +
-                                            (((unsigned long long)v + old < old) ? 1ULL : 0ULL);
-                    if (hi)
-                        atomicAdd(A + 2 * T.n, hi);
-                }
-                break;
-            case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
-                if (has)
-                {
-                    atomicAdd(A, 1ULL);
-                    atomicAdd((double *)(A + T.n), val_as_f64(ag.arg_domain, raw));
-                }
-                break;
-            case KAGG_ACTION__STDDEV:
-                if (has)
-                {
-                    double v = val_as_f64(ag.arg_domain, raw);
-                    atomicAdd(A, 1ULL);
-                    atomicAdd((double *)(A + T.n), v);
-                    atomicAdd((double *)(A + 2 * T.n), v * v);
-                }
-                break;
-        }
-    }
-}
-
-__device__ void
-lds_init(const xplan *P, LdsTable &T)
-{
-    for (uint32_t i = threadIdx.x; i < T.n; i += blockDim.x)
-    {
-        T.keys[i] = XLDS_EMPTY_KEY;
-        T.flags[i] = 0;
-    }
-    for (int j = 0; j < (int)P->gb_nattrs; j++)
-    {
-        xagg ag = P->aggs[j];
-        if (ag.action == KAGG_ACTION__VREF)
-            continue;
-        unsigned long long init = 0;
-        switch (ag.action)
-        {
-            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
-                init = (unsigned long long)INT64_MAX; break;
-            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
-                init = (unsigned long long)INT64_MIN; break;
-            case KAGG_ACTION__PMIN_FP64:
-                init = (unsigned long long)__double_as_longlong(1.7976931348623157e308); break;
-            case KAGG_ACTION__PMAX_FP64:
-                init = (unsigned long long)__double_as_longlong(-1.7976931348623157e308); break;
-        }
-        int narr = (ag.action == KAGG_ACTION__NROWS_ANY || ag.action == KAGG_ACTION__NROWS_COND ||
-                    ag.action == KAGG_ACTION__PMIN_INT32 || ag.action == KAGG_ACTION__PMIN_INT64 ||
-                    ag.action == KAGG_ACTION__PMAX_INT32 || ag.action == KAGG_ACTION__PMAX_INT64 ||
-                    ag.action == KAGG_ACTION__PMIN_FP64 || ag.action == KAGG_ACTION__PMAX_FP64) ? 1 :
-                   (ag.action == KAGG_ACTION__PSUM_INT64 || ag.action == KAGG_ACTION__PAVG_INT64 ||
-                    ag.action == KAGG_ACTION__STDDEV) ? 3 : 2;
-        for (int a = 0; a < narr; a++)
-        {
-            unsigned long long *A = T.arr + (uint64_t)(ag.lds_off + a) * T.n;
-            for (uint32_t i = threadIdx.x; i < T.n; i += blockDim.x)
-                A[i] = (a == 0) ? init : 0ULL;
-        }
-    }
-}
-
-/* ------------------------------------------------------------------
- * decoupled look-back over tiles (thread 0 of the block)
- * ------------------------------------------------------------------ */
-__device__ __forceinline__ uint64_t
-lb_pack(uint64_t flag, uint64_t rows, uint64_t bytes)
-{
-    return (flag << XLB_FLAG_SHIFT) | ((rows & XLB_ROWS_MASK) << 32) | ((bytes >> 3) & 0xffffffffULL);
-}
-
-/* ------------------------------------------------------------------
- * the pipeline kernel
- * ------------------------------------------------------------------ */
-enum { PH_SCAN = 0, PH_HASH, PH_JOIN, PH_OJ, PH_EMIT };
-#define IT_END      1U          /* chain exhausted (offsets are 8-aligned, never 1) */
+#include "xpu_device.h"
 
 template <bool PREAGG, int NR>
 __global__ void __launch_bounds__(XP_BLOCK_MAX)
 xpu_pipeline_kernel(xpu_kparams kp)
 {
-    extern __shared__ unsigned long long lds_dyn[];
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_stop;
-    __shared__ uint64_t s_wave_cnt[XP_BLOCK_MAX / 64];
-    __shared__ uint64_t s_wave_bytes[XP_BLOCK_MAX / 64];
-    __shared__ uint64_t s_base_rows;
-    __shared__ uint64_t s_base_bytes;
-    __shared__ uint32_t s_depth_out[XP_MAX_DEPTH + 1];
+    pipeline_body<PREAGG, NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
+}
 
-    const xplan *__restrict__ P = kp.plan;
-    const kern_data_store *__restrict__ kds = kp.kds_src;
-    xpu_task_ctl *ctl = kp.ctl;
-    const int tid = threadIdx.x;
-    const int nrels = P->nrels;
-    const int lane = __lane_id();
-    const int wid = tid >> 6;
-    const int nwaves = blockDim.x >> 6;
-
-    LdsTable T;
-    T.n = PREAGG ? kp.lds_nents : 0;
-    T.keys = lds_dyn;
-    T.flags = (uint32_t *)(lds_dyn + T.n);
-    T.arr = lds_dyn + T.n + (T.n + 1) / 2;
-    if (PREAGG && T.n > 0)
-    {
-        lds_init(P, T);
-        __syncthreads();
-    }
-
-    uint32_t cnt_in = 0, cnt_out = 0, lds_miss = 0;
-    if (tid <= XP_MAX_DEPTH)
-        s_depth_out[tid] = 0;
-    __syncthreads();
-
-    for (;;)
-    {
-        if (tid == 0)
-        {
-            uint32_t err = __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_tile = err ? kp.ntiles : atomicAdd(&ctl->tile_ticket, 1U);
-        }
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= kp.ntiles)
-            break;
-
-        /* rows of this tile: PREAGG handles rows_per_tile / blockDim rows per lane */
-        const uint32_t rpt = kp.rows_per_tile / blockDim.x;
-        uint64_t lane_cnt = 0, lane_bytes = 0, lane_wr_rows = 0, lane_wr_bytes = 0;
-        bool stop_writes = false;
-
-        for (int pass = 0; pass < (PREAGG ? 1 : 2); pass++)
-        for (uint32_t ri = 0; ri < rpt; ri++)
-        {
-            const uint64_t row64 = (uint64_t)kp.row_begin + (uint64_t)tile * kp.rows_per_tile +
-                                   (uint64_t)ri * blockDim.x + tid;
-            const bool valid = row64 < kp.row_end;
-            const uint32_t row = (uint32_t)row64;
-
-            int64_t Rfile[NR];
-            Lane L;
-            L.R = Rfile;
-            L.nmask = ~1U;      /* every register NULL but R0 */
-            L.err = 0;
-#pragma unroll
-            for (int i = 0; i < NR; i++)
-                L.R[i] = 0;
-            L.R[XP_REG_TRUE] = 1;
-
-            int cd = 0;
-            uint32_t it[XP_MAX_DEPTH + 2];   /* chain offsets (inner KDS < 4GB) */
-            uint32_t matched = 0;
-            for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
-                it[i] = 0;
-
-            /* a scan-only projection keeps its registers from pass 0 */
-            if (!PREAGG && pass == 1 && nrels == 0)
-                break;
-
-            int phase = PH_SCAN;
-            int d = 0;
-            for (;;)
-            {
-                xseg seg;
-                bool act = false;
-                seg.begin = seg.end = 0;
-                /* ---- pre ---- */
-                switch (phase)
-                {
-                    case PH_SCAN:
-                        seg = P->seg_scan;
-                        act = valid;
-                        break;
-                    case PH_HASH:
-                        seg = P->depth[d].seg_hash;
-                        act = (cd == d && it[d] == 0);
-                        break;
-                    case PH_JOIN:
-                        seg = P->depth[d].seg_join;
-                        act = (cd == d && it[d] > IT_END);
-                        if (act)
-                        {
-                            kern_data_store *ik = (kern_data_store *)(uintptr_t)P->depth[d].inner_kds;
-                            L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, it[d])->t;
-                            SETNULL(L, P->r_tuple, 0);
-                        }
-                        break;
-                    case PH_OJ:
-                        seg = P->depth[d].seg_oj;
-                        act = (cd == d && it[d] == IT_END && P->depth[d].left_outer &&
-                               !((matched >> d) & 1));
-                        if (act)
-                        {
-                            L.R[P->r_tuple] = 0;
-                            SETNULL(L, P->r_tuple, 1);
-                        }
-                        break;
-                    case PH_EMIT:
-                        seg = P->seg_final;
-                        act = (cd == nrels + 1);
-                        break;
-                }
-                if (act)
-                    interp(P, seg.begin, seg.end, L, kds, row, kp.const_pool);
-                /* an error ends the lane's row and is reported for the task */
-                bool lane_err = false;
-                if (act && L.err)
-                {
-                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
-                    L.err = 0;
-                    lane_err = true;
-                }
-                /* ---- post ---- */
-                if (phase == PH_SCAN)
-                {
-                    bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
-                    if (ok && pass == 0)
-                        cnt_in++;
-                    cd = ok ? 1 : 0;
-                    if (nrels > 0)
-                    {
-                        phase = PH_HASH;
-                        d = 1;
-                    }
-                    else
-                        phase = PH_EMIT;
-                }
-                else if (phase == PH_HASH)
-                {
-                    if (act)
-                    {
-                        kern_data_store *ik = (kern_data_store *)(uintptr_t)P->depth[d].inner_kds;
-                        uint32_t h = (uint32_t)L.R[P->r_hash];
-                        const uint64_t *hs = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
-                        uint64_t off = hs[h % ik->hash_nslots];
-                        uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
-                        while (off != 0 && off != ~0ULL && hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                            off = hashitem_at(ik, off)->next;
-                        it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
-                    }
-                    phase = PH_JOIN;
-                }
-                else if (phase == PH_JOIN)
-                {
-                    if (act)
-                    {
-                        kern_data_store *ik = (kern_data_store *)(uintptr_t)P->depth[d].inner_kds;
-                        const kern_hashitem *cur = hashitem_at(ik, it[d]);
-                        uint32_t h = cur->t.hash;       /* == the probe hash */
-                        int64_t status = lane_err ? 0 : L.R[P->r_status];
-                        /* next candidate with the same hash, found now so that
-                         * exhausted chains unwind without another pass */
-                        uint64_t off = cur->next;
-                        uint32_t steps = ik->nitems;
-                        while (off != 0 && off != ~0ULL && hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                            off = hashitem_at(ik, off)->next;
-                        it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
-                        if (status != 0)
-                            matched |= 1U << d;
-                        if (status > 0)
-                        {
-                            if (pass == 0)
-                                atomicAdd(&s_depth_out[d], 1U);
-                            cd = d + 1;
-                            if (d + 1 <= nrels)
-                                it[d + 1] = 0;
-                        }
-                    }
-                    phase = PH_OJ;
-                }
-                else if (phase == PH_OJ)
-                {
-                    bool adv = false;
-                    if (act)
-                    {
-                        matched |= 1U << d;
-                        if (!lane_err && L.R[P->r_status] > 0)
-                        {
-                            if (pass == 0)
-                                atomicAdd(&s_depth_out[d], 1U);
-                            cd = d + 1;
-                            if (d + 1 <= nrels)
-                                it[d + 1] = 0;
-                            adv = true;
-                        }
-                    }
-                    if (!adv && cd == d && it[d] == IT_END)
-                    {
-                        /* unwind exhausted depths (uniform indices only) */
-                        for (int dd = d; dd >= 1; dd--)
-                        {
-                            if (dd == cd && it[dd] == IT_END)
-                            {
-                                matched &= ~(1U << dd);
-                                it[dd] = 0;
-                                cd--;
-                            }
-                        }
-                    }
-                    if (d < nrels)
-                    {
-                        d++;
-                        phase = PH_HASH;
-                    }
-                    else
-                        phase = PH_EMIT;
-                }
-                else    /* PH_EMIT */
-                {
-                    if (act && !lane_err)
-                    {
-                        if (pass == 0)
-                            cnt_out++;
-                        if (PREAGG)
-                        {
-                            bool direct = true;
-                            int slot = -1;
-                            if (P->lds_fast && T.n > 0)
-                            {
-                                slot = lds_probe(T, gb_pack_key(P, L));
-                                if (slot >= 0)
-                                {
-                                    lds_update(P, L, T, slot);
-                                    direct = false;
-                                }
-                            }
-                            if (direct)
-                                lds_miss++;
-                            uint32_t hash = (uint32_t)L.R[P->r_keyhash];
-                            gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
-                        }
-                        else
-                        {
-                            uint32_t sz = (uint32_t)xalign(8, proj_tuple_size(P, L));
-                            if (pass == 0)
-                            {
-                                lane_cnt++;
-                                lane_bytes += sz;
-                            }
-                            else if (!stop_writes)
-                            {
-                                kern_data_store *dst = kp.kds_out;
-                                lane_wr_bytes += sz;
-                                uint64_t usage = s_base_bytes + lane_wr_bytes;
-                                uint64_t rowid = s_base_rows + lane_wr_rows;
-                                char *tp = (char *)dst + dst->length - usage;
-                                proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
-                                uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
-                                rowidx[rowid] = usage;
-                                lane_wr_rows++;
-                            }
-                        }
-                    }
-                    if (act)
-                    {
-                        cd = nrels;
-                        for (int dd = nrels; dd >= 1; dd--)
-                        {
-                            if (dd == cd && it[dd] == IT_END)
-                            {
-                                matched &= ~(1U << dd);
-                                it[dd] = 0;
-                                cd--;
-                            }
-                        }
-                    }
-                    if (lane_err)
-                        cd = 0;
-                    if (!wave_any(cd > 0))
-                        break;
-                    phase = PH_HASH;
-                    d = 1;
-                }
-                if (lane_err)
-                    cd = 0;
-            }
-
-            if (!PREAGG && pass == 0)
-            {
-                /* ---- tile prefix: wave scan, block scan, look-back ---- */
-                uint64_t ic = wave_incl_scan_u64(lane_cnt);
-                uint64_t ib = wave_incl_scan_u64(lane_bytes);
-                if (lane == 63)
-                {
-                    s_wave_cnt[wid] = ic;
-                    s_wave_bytes[wid] = ib;
-                }
-                __syncthreads();
-                if (tid == 0)
-                {
-                    uint64_t tc = 0, tb = 0;
-                    for (int w = 0; w < nwaves; w++)
-                    {
-                        uint64_t c = s_wave_cnt[w], b = s_wave_bytes[w];
-                        s_wave_cnt[w] = tc;
-                        s_wave_bytes[w] = tb;
-                        tc += c;
-                        tb += b;
-                    }
-                    /* exclusive prefix of earlier tiles */
-                    uint64_t er = 0, eb = 0;
-                    bool stop = false;
-                    uint64_t *st = kp.tile_status;
-                    if (tile > 0)
-                    {
-                        __hip_atomic_store(&st[tile], lb_pack(XLB_AGG, tc, tb),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        int64_t j = (int64_t)tile - 1;
-                        uint32_t spins = 0;
-                        while (j >= 0)
-                        {
-                            uint64_t w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-                            uint64_t flag = w >> XLB_FLAG_SHIFT;
-                            if (flag == 0)
-                            {
-                                if (++spins > XLB_SPIN_LIMIT)
-                                {
-                                    record_error(ctl, XMSG_LOOKBACK_TIMEOUT, ERRCODE_DEVICE_ERROR);
-                                    stop = true;
-                                    break;
-                                }
-                                __builtin_amdgcn_s_sleep(1);
-                                continue;
-                            }
-                            if (flag == XLB_STOP)
-                            {
-                                stop = true;
-                                break;
-                            }
-                            er += (w >> 32) & XLB_ROWS_MASK;
-                            eb += (w & 0xffffffffULL) << 3;
-                            if (flag == XLB_PREFIX)
-                                break;
-                            j--;
-                        }
-                    }
-                    kern_data_store *dst = kp.kds_out;
-                    if (!stop &&
-                        KDS_HEAD_LENGTH(dst) + 8 * (er + tc) + eb + tb > dst->length)
-                    {
-                        stop = true;
-                        /* several tiles may see the overflow (a successor can read
-                         * this tile's AGG word before its STOP); the resume point
-                         * is the earliest: keep max(~tile) */
-                        atomicMax(&ctl->resume_tile, 0xffffffffU - tile);
-                        if (tile == 0)
-                            record_error(ctl, XMSG_TILE_TOO_LARGE, ERRCODE_DEVICE_ERROR);
-                    }
-                    if (stop || __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
-                    {
-                        stop = true;
-                        __hip_atomic_store(&st[tile], lb_pack(XLB_STOP, 0, 0),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                    else
-                    {
-                        __hip_atomic_store(&st[tile], lb_pack(XLB_PREFIX, er + tc, eb + tb),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        atomicMax((unsigned long long *)&ctl->out_nitems, (unsigned long long)(er + tc));
-                        atomicMax((unsigned long long *)&ctl->out_usage, (unsigned long long)(eb + tb));
-                    }
-                    s_base_rows = er;
-                    s_base_bytes = eb;
-                    s_stop = stop;
-                }
-                __syncthreads();
-                stop_writes = s_stop;
-                /* this lane's first output row / byte offset within the tile */
-                uint64_t excl_c = ic - lane_cnt + s_wave_cnt[wid];
-                uint64_t excl_b = ib - lane_bytes + s_wave_bytes[wid];
-                lane_wr_rows = excl_c;
-                lane_wr_bytes = excl_b;
-                if (!stop_writes && nrels == 0 && lane_cnt > 0)
-                {
-                    /* scan-only projections emit at most one row per lane and
-                     * its registers are still live: write now, skip pass 1 */
-                    kern_data_store *dst = kp.kds_out;
-                    uint64_t usage = s_base_bytes + excl_b + lane_bytes;
-                    uint64_t rowid = s_base_rows + excl_c;
-                    char *tp = (char *)dst + dst->length - usage;
-                    proj_tuple_write(P, L, tp, (uint32_t)lane_bytes, (uint32_t)rowid);
-                    uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
-                    rowidx[rowid] = usage;
-                }
-            }
-        }
-        if (!PREAGG)
-            __syncthreads();    /* s_base_* / s_wave_* reuse by the next tile */
-    }
-
-    if (PREAGG && T.n > 0)
-    {
-        /* merge the block's partial table into kds_final */
-        __syncthreads();
-        int64_t Rfile[NR];
-        Lane L;
-        L.R = Rfile;
-        L.nmask = ~1U;
-        L.err = 0;
-#pragma unroll
-        for (int i = 0; i < NR; i++)
-            L.R[i] = 0;
-        L.R[XP_REG_TRUE] = 1;
-        uint32_t nflush = 0;
-        for (uint32_t base = 0; base < T.n; base += blockDim.x)
-        {
-            uint32_t h = base + tid;
-            bool pending = false;
-            uint32_t hash = 0;
-            if (h < T.n)
-            {
-                unsigned long long key = T.keys[h];
-                if (key != XLDS_EMPTY_KEY)
-                {
-                    gb_unpack_key(P, L, key);
-                    hash = gb_hash_regs(P, L);
-                    pending = true;
-                    nflush++;
-                }
-            }
-            gb_global(P, L, kp.kds_out, hash, pending, &T, pending ? (int)h : -1, ctl);
-        }
-        if (nflush)
-            atomicAdd((unsigned long long *)&ctl->global_merges, (unsigned long long)nflush);
-    }
-    if (cnt_in)
-        atomicAdd(&ctl->nitems_in, cnt_in);
-    if (cnt_out)
-        atomicAdd(&ctl->nitems_out, cnt_out);
-    if (lds_miss)
-        atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
-    __syncthreads();
-    if (tid >= 1 && tid <= nrels && s_depth_out[tid])
-        atomicAdd(&ctl->depth_out[tid], s_depth_out[tid]);
+template <int NR, int RPT>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_scan_agg_kernel(xpu_kparams kp)
+{
+    scan_agg_body<NR, RPT, Interp>((cplan *)(uintptr_t)kp.plan, kp);
 }
 
 /* ------------------------------------------------------------------
@@ -2088,6 +76,29 @@ xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block, size_
             hipLaunchKernelGGL((xpu_pipeline_kernel<false, XP_NREGS>), dim3(grid), dim3(block),
                                lds_bytes, s, *kp);
     }
+    return (int)hipGetLastError();
+}
+
+template <int NR>
+static void
+launch_scan_agg(const xpu_kparams *kp, int rpt, int grid, int block, size_t lds, hipStream_t s)
+{
+    if (rpt >= 4)
+        hipLaunchKernelGGL((xpu_scan_agg_kernel<NR, 4>), dim3(grid), dim3(block), lds, s, *kp);
+    else if (rpt == 2)
+        hipLaunchKernelGGL((xpu_scan_agg_kernel<NR, 2>), dim3(grid), dim3(block), lds, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_scan_agg_kernel<NR, 1>), dim3(grid), dim3(block), lds, s, *kp);
+}
+
+extern "C" int
+xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
+                    size_t lds_bytes, void *stream)
+{
+    if (nregs <= XP_NREGS_SMALL)
+        launch_scan_agg<XP_NREGS_SMALL>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
+    else
+        launch_scan_agg<XP_NREGS>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
     return (int)hipGetLastError();
 }
 

@@ -29,6 +29,12 @@
 #define XP_MAX_PROJ         64      /* projected / group-by attributes */
 #define XP_MAX_KEYS         8
 #define XP_MAX_OPS          2048
+#define XP_MAX_STAGE        6       /* depth-0 columns loaded by the tile prologue */
+
+/* XO_LDCOL imm: low byte = width; XLD_STAGED = the scan kernel's prologue has
+ * already put the value into the destination register */
+#define XLD_WIDTH_MASK      0xff
+#define XLD_STAGED          0x100
 
 /* value domains of a register */
 enum {
@@ -91,6 +97,9 @@ enum xop_code {
     /* hashing: dst <- 0xffffffff; dst <- merge(dst, hash(R[a])) unless NULL;
        aux = hash width in bytes (0 = text); dst <- dst ^ 0xffffffff */
     XO_HASH_INIT, XO_HASH_ADD, XO_HASH_FIN,
+    /* end of the scan quals when the final stage follows in the same pass:
+       a lane whose R[a] is not true raises no further errors */
+    XO_KILL,
     XO_NUM_OPS
 };
 
@@ -131,8 +140,20 @@ typedef struct
     uint8_t     key_index;  /* VREF: index among the grouping keys */
     uint16_t    state_size; /* bytes of the partial state in the tuple */
     uint32_t    state_off;  /* offset of the state from the payload start */
-    int32_t     lds_off;    /* first LDS array of this action (in entries) */
+    int8_t      lds64;      /* 8-byte LDS field array of this action (-1: none) */
+    int8_t      lds32;      /* 4-byte LDS field array of this action (-1: none) */
+    uint16_t    __pad;
 } xagg;
+
+/* a depth-0 fixed-width column loaded by the scan kernel's tile prologue */
+typedef struct
+{
+    uint16_t    col;        /* kds_src column index */
+    uint8_t     width;      /* 1, 2, 4 or 8 bytes */
+    uint8_t     f32;        /* float4: zero-extend the raw bits */
+    uint8_t     reg;        /* destination register */
+    uint8_t     __pad[3];
+} xstage;
 
 /* the compiled plan, uploaded once per session */
 typedef struct
@@ -142,6 +163,12 @@ typedef struct
     uint32_t    nrels;
     uint32_t    nregs;
     xseg        seg_scan;           /* depth-0 loads + scan quals */
+    uint32_t    nstage;             /* columns of stage[] */
+    xstage      stage[XP_MAX_STAGE];
+    uint32_t    fused;              /* nrels == 0: seg_scan ends with XO_KILL and
+                                       seg_final follows it directly */
+    uint32_t    stage_prefix;       /* leading ops of seg_scan that are staged
+                                       loads (the scan kernel starts after them) */
     uint8_t     r_scan_ok;          /* bool register, or XP_REG_TRUE */
     uint8_t     r_hash;             /* outer key hash of the depth being probed */
     uint8_t     r_status;           /* join status: 1 / 0 / -1 as ExecGpuJoinQuals */
@@ -167,8 +194,11 @@ typedef struct
     uint32_t    gb_nattrs;
     uint32_t    gb_t_hoff;          /* MAXALIGN(offsetof(t_bits) + bitmap) */
     uint32_t    gb_states_end;      /* payload offset after the last state */
-    uint32_t    gb_lds_arrays;      /* 8-byte LDS arrays per entry (excl. key) */
-    xseg        seg_keyhash_regs;   /* key hash from the key registers only */
+    /* LDS partial table, struct of arrays over its entries: u64 packed key,
+     * u64 fields [lds_n64], u32 row count, u32 fields [lds_n32] */
+    uint16_t    lds_n64;
+    uint16_t    lds_n32;
+    uint32_t    lds_entry_bytes;
     xattr       keys[XP_MAX_KEYS];
     uint8_t     key_bits[XP_MAX_KEYS];  /* packed width of each key in bits */
     xagg        aggs[XP_MAX_PROJ];

@@ -21,12 +21,15 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "xpuhip.h"
 #include "xpu_compile.h"
+#include "xpu_jit.h"
 #include "xpu_kernels.h"
 
 #define XERR(eb, code, msg) \
@@ -46,6 +49,7 @@ struct Device
 Device      g_dev;
 std::mutex  g_dev_lock;
 thread_local double t_last_kernel_ms = 0.0;
+thread_local xpuhip_task_stats t_last_stats;
 
 bool
 hip_ok(hipError_t rc, kern_errorbuf *eb, const char *what)
@@ -158,6 +162,10 @@ struct xpuhip_session
     std::mutex      lock;
     std::vector<Scratch *> pool;
     uint64_t        ntasks = 0;
+    /* kernel specialized for the plan (loaded on first use) */
+    hipFunction_t   jit_fn = nullptr;
+    int             jit_kind = 0;
+    int             jit_param = -1;
 };
 
 namespace {
@@ -190,18 +198,28 @@ scratch_put(xpuhip_session *s, Scratch *x)
     s->pool.push_back(x);
 }
 
-uint32_t
-lds_arrays_of(const xplan *P)
-{
-    return P->gb_lds_arrays;
-}
-
 size_t
 lds_bytes_for(const xplan *P, uint32_t nents)
 {
-    if (nents == 0)
+    return (size_t)nents * P->lds_entry_bytes;
+}
+
+/* plans without joins run the scan + group-by kernel */
+bool
+uses_scan_agg(const xpuhip_session *s)
+{
+    const xplan *P = &s->comp.plan;
+    return s->comp.has_groupby && P->nrels == 0 && P->fused;
+}
+
+/* largest LDS table (entries) that fits in budget bytes */
+uint32_t
+lds_nents_within(const xplan *P, size_t budget)
+{
+    if (!P->lds_fast || P->lds_entry_bytes == 0)
         return 0;
-    return 8ULL * nents + 8ULL * ((nents + 1) / 2) + 8ULL * lds_arrays_of(P) * nents;
+    size_t n = budget / P->lds_entry_bytes;
+    return (uint32_t)(n > (1U << 20) ? (1U << 20) : n);
 }
 
 /* default launch shape per task kind (the autotuner overrides it) */
@@ -210,21 +228,26 @@ default_tuning(const xpuhip_session *s, xpuhip_tuning *t)
 {
     const xplan *P = &s->comp.plan;
     memset(t, 0, sizeof(*t));
-    t->block_size = 256;
-    if (s->comp.has_groupby)
+    if (uses_scan_agg(s))
     {
+        /* one 512-thread block per CU owning (nearly) all of the CU's LDS as
+         * its partial table; 4 rows per lane per wave tile */
+        t->block_size = 512;
+        t->rows_per_thread = 4;
+        t->blocks_per_cu = 1;
+        size_t budget = g_dev.lds_per_block > 4096 ? g_dev.lds_per_block - 4096 : 0;
+        t->lds_nents = lds_nents_within(P, budget);
+    }
+    else if (s->comp.has_groupby)
+    {
+        t->block_size = 256;
         t->rows_per_thread = 4;
         t->blocks_per_cu = 2;
-        if (P->lds_fast)
-        {
-            uint32_t n = 2048;
-            while (n > 64 && lds_bytes_for(P, n) > 48 * 1024)
-                n >>= 1;
-            t->lds_nents = n;
-        }
+        t->lds_nents = lds_nents_within(P, 48 * 1024);
     }
     else
     {
+        t->block_size = 256;
         t->rows_per_thread = 1;
         t->blocks_per_cu = 8;
     }
@@ -378,6 +401,28 @@ xpuhip_session_check(const kern_session_info *session, size_t length, kern_error
     return rc;
 }
 
+extern "C" int
+xpuhip_jit_prepare(const kern_session_info *session, size_t length, kern_errorbuf *eb)
+{
+    std::unique_ptr<xpu_compiled> c(new xpu_compiled());
+    int rc = xpu_compile_session(session, length, c.get(), eb);
+    if (rc)
+        return rc;
+    const xplan *P = &c->plan;
+    int kind = (c->has_groupby && P->nrels == 0 && P->fused) ? XJ_SCAN_AGG : 0;
+    if (!kind)
+        return 0;       /* only scan + group-by plans are specialized */
+    std::string log;
+    if (!xpu_jit_code(P, kind, 4, &log))
+    {
+        if (log.size() > 150)
+            log.resize(150);
+        XERR(eb, ERRCODE_DEVICE_ERROR, ("specialized kernel does not compile: " + log).c_str());
+        return ERRCODE_DEVICE_ERROR;
+    }
+    return 0;
+}
+
 extern "C" int64_t
 xpuhip_session_explain(const kern_session_info *session, size_t length, char *buf, size_t buflen,
                        kern_errorbuf *eb)
@@ -521,6 +566,53 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
     return 0;
 }
 
+namespace {
+
+/* code objects of specialized kernels loaded into this process's device */
+std::mutex g_mod_lock;
+std::map<const std::vector<char> *, hipFunction_t> g_mods;
+
+hipFunction_t
+jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
+{
+    {
+        std::lock_guard<std::mutex> g(s->lock);
+        if (s->jit_fn && s->jit_kind == kind && s->jit_param == param)
+            return s->jit_fn;
+    }
+    std::string log;
+    const std::vector<char> *code = xpu_jit_code(&s->comp.plan, kind, param, &log);
+    if (!code)
+    {
+        if (log.size() > 150)
+            log.resize(150);
+        XERR(eb, ERRCODE_DEVICE_ERROR, ("specialized kernel does not compile: " + log).c_str());
+        return nullptr;
+    }
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_mod_lock);
+        auto it = g_mods.find(code);
+        if (it != g_mods.end())
+            fn = it->second;
+        else
+        {
+            hipModule_t mod;
+            if (!hip_ok(hipModuleLoadData(&mod, code->data()), eb, "hipModuleLoadData") ||
+                !hip_ok(hipModuleGetFunction(&fn, mod, "xpu_jit_kernel"), eb, "hipModuleGetFunction"))
+                return nullptr;
+            g_mods[code] = fn;
+        }
+    }
+    std::lock_guard<std::mutex> g(s->lock);
+    s->jit_fn = fn;
+    s->jit_kind = kind;
+    s->jit_param = param;
+    return fn;
+}
+
+}   /* namespace */
+
 static int
 task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, uint32_t nitems,
                  kern_data_store *d_dst, size_t dst_length, uint64_t *resume_pos,
@@ -561,6 +653,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         }
         return 0;
     }
+    const bool scan_agg = uses_scan_agg(s);
     uint32_t block = t.block_size ? t.block_size : 256;
     uint32_t rpt = preagg ? (t.rows_per_thread ? t.rows_per_thread : 1) : 1;
     uint32_t rows_per_tile = block * rpt;
@@ -568,7 +661,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     uint32_t ntiles = (uint32_t)((nrows + rows_per_tile - 1) / rows_per_tile);
     uint32_t grid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 2);
     if (grid > ntiles)
-        grid = ntiles;
+        grid = ntiles;      /* scan_agg: a block tile is its waves' first tiles */
     uint32_t lds_nents = (preagg && P->lds_fast) ? t.lds_nents : 0;
     size_t lds = lds_bytes_for(P, lds_nents);
     if (lds > g_dev.lds_per_block)
@@ -592,6 +685,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     kp.rows_per_tile = rows_per_tile;
     kp.ntiles = ntiles;
     kp.lds_nents = lds_nents;
+    for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
+        kp.inner_kds[d] = P->depth[d].inner_kds;
 
     int rc = 0;
     xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
@@ -602,8 +697,26 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
                                 dst_length, x->d_ctl, x->stream);
         xpu_launch_fill(x->d_status, 0, ntiles, x->stream);
     }
+    /* the kernel specialized for the plan, or the interpreter */
+    hipFunction_t jfn = nullptr;
+    if (scan_agg && t.jit != XPUHIP_JIT_OFF &&
+        (t.jit == XPUHIP_JIT_ON || nrows >= (1ULL << 20)))
+    {
+        jfn = jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb);
+        if (!jfn && t.jit == XPUHIP_JIT_ON)
+            return ERRCODE_DEVICE_ERROR;
+    }
     (void)hipEventRecord(x->ev0, x->stream);
-    rc = xpu_launch_pipeline(&kp, P->nregs, grid, block, lds, preagg, x->stream);
+    if (jfn)
+    {
+        void *args[] = {&kp};
+        rc = (int)hipModuleLaunchKernel(jfn, grid, 1, 1, block, 1, 1, (unsigned)lds, x->stream,
+                                        args, nullptr);
+    }
+    else if (scan_agg)
+        rc = xpu_launch_scan_agg(&kp, P->nregs, (int)rpt, grid, block, lds, x->stream);
+    else
+        rc = xpu_launch_pipeline(&kp, P->nregs, grid, block, lds, preagg, x->stream);
     (void)hipEventRecord(x->ev1, x->stream);
     if (rc != 0)
     {
@@ -621,6 +734,15 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_kernel_ms = ms;
 
     const xpu_task_ctl *c = x->h_ctl;
+    t_last_stats.kernel_ms = ms;
+    t_last_stats.grid = grid;
+    t_last_stats.block = block;
+    t_last_stats.ntiles = ntiles;
+    t_last_stats.lds_nents = lds_nents;
+    t_last_stats.lds_bytes = lds;
+    t_last_stats.rows = nrows;
+    t_last_stats.lds_miss = c->lds_miss;
+    t_last_stats.global_merges = c->global_merges;
     if (res)
     {
         res->nitems_raw = (uint32_t)nrows;
@@ -810,7 +932,10 @@ xpuhip_session_set_tuning(xpuhip_session *s, const xpuhip_tuning *t)
     if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
     if (n.lds_nents == 0) n.lds_nents = d.lds_nents;
     if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > 512 ||
-        (n.lds_nents & (n.lds_nents - 1)) != 0 || n.rows_per_thread > 64)
+        n.rows_per_thread > 64 || n.blocks_per_cu > 64 || n.jit > XPUHIP_JIT_OFF)
+        return ERRCODE_DEVICE_ERROR;
+    if (uses_scan_agg(s) && n.rows_per_thread != 1 && n.rows_per_thread != 2 &&
+        n.rows_per_thread != 4)
         return ERRCODE_DEVICE_ERROR;
     s->tuning = n;
     return 0;
@@ -861,4 +986,13 @@ extern "C" double
 xpuhip_last_kernel_ms(void)
 {
     return t_last_kernel_ms;
+}
+
+extern "C" int
+xpuhip_last_task_stats(xpuhip_task_stats *stats)
+{
+    if (!stats)
+        return ERRCODE_DEVICE_ERROR;
+    *stats = t_last_stats;
+    return 0;
 }

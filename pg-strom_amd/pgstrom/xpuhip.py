@@ -19,7 +19,10 @@ EXPORTS = [
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_device_synchronize",
     "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
+    "xpuhip_last_task_stats", "xpuhip_jit_prepare",
 ]
+
+JIT_AUTO, JIT_ON, JIT_OFF = 0, 1, 2
 
 
 class XpuError(RuntimeError):
@@ -31,7 +34,14 @@ class XpuError(RuntimeError):
 
 class xpuhip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("rows_per_thread", C.c_uint32),
-                ("lds_nents", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("lds_nents", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("jit", C.c_uint32)]
+
+
+class xpuhip_task_stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("grid", C.c_uint32), ("block", C.c_uint32),
+                ("ntiles", C.c_uint32), ("lds_nents", C.c_uint32), ("lds_bytes", C.c_uint64),
+                ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64)]
 
 
 _lib = None
@@ -64,6 +74,8 @@ def lib():
         L.xpuhip_abi_layout_json.restype = C.c_char_p
         L.xpuhip_session_explain.restype = C.c_int64
         L.xpuhip_session_explain.argtypes = [vp, sz, vp, sz, vp]
+        L.xpuhip_last_task_stats.argtypes = [vp]
+        L.xpuhip_jit_prepare.argtypes = [vp, sz, vp]
         _lib = L
     return _lib
 
@@ -89,6 +101,15 @@ def explain(blob):
 def init(device=0):
     err = abi.kern_errorbuf()
     rc = lib().xpuhip_init(device, C.byref(err))
+    if rc:
+        raise XpuError(rc, err)
+
+
+def jit_prepare(blob):
+    """Compiles the kernel specialized for a session blob (no GPU needed)."""
+    b = _buf(blob)
+    err = abi.kern_errorbuf()
+    rc = lib().xpuhip_jit_prepare(b.ctypes.data, len(b), C.byref(err))
     if rc:
         raise XpuError(rc, err)
 
@@ -206,6 +227,13 @@ class Session:
 
 def last_kernel_ms():
     return lib().xpuhip_last_kernel_ms()
+
+
+def last_task_stats():
+    """Launch shape and counters of the calling thread's last task."""
+    st = xpuhip_task_stats()
+    lib().xpuhip_last_task_stats(C.byref(st))
+    return {k: getattr(st, k) for k, _ in st._fields_}
 
 
 def abi_layout_json():

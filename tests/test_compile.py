@@ -2,8 +2,9 @@
 import struct
 
 import numpy as np
+import pytest
 
-from pgstrom import abi, codegen as cg, datagen, workloads as W, xpuhip
+from pgstrom import abi, codegen as cg, datagen, kds, workloads as W, xpuhip
 
 
 def test_workload_sessions_compile():
@@ -77,3 +78,21 @@ def test_explain_short_circuit_guard():
     gdst = int(guard.split("d=")[1].split()[0])
     div = [l for l in lines if "DIV_I" in l][0]
     assert int(div.split("g=")[1].split()[0]) == gdst, text
+
+
+@pytest.mark.parametrize("which", ["c1", "fragment", "nogroup"])
+def test_jit_prepare_compiles(which):
+    """The kernel specialized for a scan + group-by plan compiles with hiprtc
+    for gfx950 (no GPU needed)."""
+    cols = datagen.preagg_table(1000)
+    if which == "c1":
+        blob = W.preagg_query(cols)
+    elif which == "fragment":
+        a = kds.Column("a", "int4", np.arange(1000, dtype=np.int32))
+        blob = W.fragment_query([a] + cols, k=500, m=16)
+    else:
+        plan = cg.Plan([(c.name, c.typname) for c in cols])
+        plan.set_groupby([(abi.KAGG_ACTION__NROWS_ANY, None, ("n", "int8")),
+                          (abi.KAGG_ACTION__PSUM_FP, plan.scan_var("w"), ("f", "bytea"))])
+        blob = plan.build(final_length=4 << 20)
+    xpuhip.jit_prepare(blob)
