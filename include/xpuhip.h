@@ -135,6 +135,9 @@ typedef struct
     uint64_t    rows;               /* rows handed to the kernel */
     uint64_t    lds_miss;           /* group-by rows merged straight into kds_final */
     uint64_t    global_merges;      /* LDS entries merged into kds_final at block end */
+    uint64_t    spill_rows;         /* rows aggregated by the second (spill) pass */
+    uint32_t    jit;                /* 1: the kernel specialized for the plan ran */
+    uint32_t    __pad;
 } xpuhip_task_stats;
 
 int     xpuhip_last_task_stats(xpuhip_task_stats *stats);

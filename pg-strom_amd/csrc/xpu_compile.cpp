@@ -1045,6 +1045,29 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         P->gb_states_end = off;
         P->gb_t_hoff = XPU_MAXALIGN(KERN_TUPITEM_T_BITS_OFFSET + XPU_BITMAPLEN(nattrs));
         P->lds_fast = (lds_ok && key_bits <= 63) ? 1 : 0;
+        /* registers a spill record carries: the distinct aggregate arguments */
+        if (P->lds_fast && nkeys > 0)
+        {
+            int n = 0;
+            bool fits = true;
+            for (int j = 0; j < nattrs; j++)
+            {
+                const xagg *ag = &P->aggs[j];
+                if (ag->action == KAGG_ACTION__VREF || ag->arg_reg == 0xff)
+                    continue;
+                bool seen = false;
+                for (int k = 0; k < n; k++)
+                    seen = seen || P->spill_regs[k] == ag->arg_reg;
+                if (seen)
+                    continue;
+                if (n >= XP_MAX_SPILL)
+                    fits = false;
+                else
+                    P->spill_regs[n++] = ag->arg_reg;
+            }
+            P->spill_ok = fits ? 1 : 0;
+            P->spill_nregs = fits ? (uint8_t)n : 0;
+        }
         P->lds_n64 = lds_n64;
         P->lds_n32 = lds_n32;
         P->lds_entry_bytes = 8 * (1 + lds_n64) + 4 * (1 + lds_n32);
@@ -1196,8 +1219,9 @@ xpu_plan_dump(const xpu_compiled *c)
     }
     if (c->has_groupby)
     {
-        snprintf(buf, sizeof(buf), "groupby nkeys=%u lds_fast=%u lds_entry_bytes=%u (u64 fields %u, u32 fields %u)\n",
-                 P->nkeys, P->lds_fast, P->lds_entry_bytes, P->lds_n64, P->lds_n32);
+        snprintf(buf, sizeof(buf), "groupby nkeys=%u lds_fast=%u lds_entry_bytes=%u (u64 fields %u, u32 fields %u)"
+                 " spill=%u (%u regs)\n", P->nkeys, P->lds_fast, P->lds_entry_bytes, P->lds_n64,
+                 P->lds_n32, P->spill_ok, P->spill_nregs);
         s += buf;
     }
     for (uint32_t j = 0; j < P->proj_nattrs; j++)

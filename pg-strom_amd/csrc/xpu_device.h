@@ -2109,6 +2109,14 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
  * fused scan-quals + aggregate-argument program one at a time and folded
  * into the block's LDS partial table.
  * ------------------------------------------------------------------ */
+/* marks the unused records [pos, end) of a wave's spill reservation */
+__device__ __forceinline__ void
+spill_fill_holes(const xpu_kparams &kp, uint64_t pos, uint64_t end, int rec_words)
+{
+    for (uint64_t i = pos + __lane_id(); i < end; i += 64)
+        kp.spill[i * rec_words] = XLDS_EMPTY_KEY;
+}
+
 template <int RPT>
 __device__ __forceinline__ uint32_t
 pick_u32(const uint32_t (&a)[RPT], int i)
@@ -2130,7 +2138,9 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
     ckds *kmeta = (ckds *)(uintptr_t)kds;
     xpu_task_ctl *ctl = kp.ctl;
     const int lane = __lane_id();
-    const int wid = threadIdx.x >> 6;
+    /* wave-uniform (the backend cannot prove threadIdx.x >> 6 uniform, and
+     * everything derived from it, tile base included, would go divergent) */
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwaves = blockDim.x >> 6;
     const int nstage = (int)P->nstage;
 
@@ -2168,7 +2178,14 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
     const int pc_end = P->seg_final.end;
     __syncthreads();
 
-    uint32_t cnt_ok = 0, lds_miss = 0;
+    uint32_t cnt_ok = 0, lds_miss = 0, nspilled = 0;
+    /* rows that find no room in the LDS table are appended to the spill
+     * buffer (a second kernel aggregates them by hash partition); each wave
+     * reserves XSPILL_CHUNK records at a time, so appending costs one device
+     * atomic per chunk */
+    const bool can_spill = P->spill_ok && kp.spill != nullptr && kp.lds_nents > 0;
+    const int rec_words = 1 + P->spill_nregs;
+    uint64_t sp_pos = 0, sp_end = 0;
     const uint64_t wrows = 64ULL * RPT;
     const uint64_t nwt = ((uint64_t)(kp.row_end - kp.row_begin) + wrows - 1) / wrows;
     const uint64_t stride = (uint64_t)gridDim.x * nwaves;
@@ -2181,7 +2198,15 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
             break;
         const uint64_t base = (uint64_t)kp.row_begin + wt * wrows;
 
-        /* prologue: raw words of the staged columns */
+        /* prologue: raw words of the staged columns.  A tile that lies
+         * inside every staged column (all but the last tile) loads without
+         * per-lane bounds checks, so no load waits on another; the tail tile
+         * checks each element. */
+        bool full = base + wrows <= kp.row_end;
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+            if (st < nstage)
+                full = full && (uint64_t)s_w[st] * (base + wrows) <= s_vlen[st];
         uint32_t lo[XP_MAX_STAGE][RPT], hi[XP_MAX_STAGE][RPT];
 #pragma unroll
         for (int st = 0; st < XP_MAX_STAGE; st++)
@@ -2192,18 +2217,20 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                 lo[st][i] = 0;
                 hi[st][i] = 0;
             }
-            if (st < nstage)
-            {
-                const char *vbase = s_vbase[st];
-                const uint64_t vlen = s_vlen[st];
-                const int w = s_w[st];
+        }
+        if (full)
+        {
 #pragma unroll
-                for (int i = 0; i < RPT; i++)
+            for (int st = 0; st < XP_MAX_STAGE; st++)
+            {
+                if (st < nstage)
                 {
-                    const uint64_t row = base + (uint64_t)i * 64 + lane;
-                    if (row < kp.row_end && (uint64_t)w * (row + 1) <= vlen)
+                    const char *vbase = s_vbase[st];
+                    const int w = s_w[st];
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
                     {
-                        const char *p = vbase + (uint64_t)w * row;
+                        const char *p = vbase + (uint64_t)w * (base + (uint64_t)i * 64 + lane);
                         if (w == 8)
                         {
                             uint2 u = *(const uint2 *)p;
@@ -2218,6 +2245,58 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                             lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
                     }
                 }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int st = 0; st < XP_MAX_STAGE; st++)
+            {
+                if (st < nstage)
+                {
+                    const char *vbase = s_vbase[st];
+                    const uint64_t vlen = s_vlen[st];
+                    const int w = s_w[st];
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
+                    {
+                        const uint64_t row = base + (uint64_t)i * 64 + lane;
+                        if (row < kp.row_end && (uint64_t)w * (row + 1) <= vlen)
+                        {
+                            const char *p = vbase + (uint64_t)w * row;
+                            if (w == 8)
+                            {
+                                uint2 u = *(const uint2 *)p;
+                                lo[st][i] = u.x;
+                                hi[st][i] = u.y;
+                            }
+                            else if (w == 4)
+                                lo[st][i] = *(const uint32_t *)p;
+                            else if (w == 2)
+                                lo[st][i] = (uint32_t)(int32_t)*(const int16_t *)p;
+                            else
+                                lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
+                        }
+                    }
+                }
+            }
+        }
+        /* validity bitmaps: one 64-bit word per 64 rows, read with scalar
+         * loads when the tile is 64-row aligned inside the bitmap */
+        const bool nm_words = ((base & 63) == 0);
+        uint64_t nmw[XP_MAX_STAGE][RPT];
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+        {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+            {
+                nmw[st][i] = ~0ULL;
+                const uint64_t nm_off = s_nmoff[st];
+                const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
+                if (st < nstage && nm_off && nm_words && (nm_off & 7) == 0 &&
+                    byte + 8 <= s_nmlen[st])
+                    nmw[st][i] = *(const XCONST uint64_t *)(uintptr_t)(kbase + nm_off + byte);
             }
         }
 
@@ -2253,8 +2332,12 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                     if (nm_off)
                     {
                         const uint64_t idx = row >> 3;
-                        isnull = isnull || idx >= s_nmlen[st] ||
-                                 !(((((const uint8_t *)kbase)[nm_off + idx]) >> (row & 7)) & 1);
+                        const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
+                        if (nm_words && (nm_off & 7) == 0 && byte + 8 <= s_nmlen[st])
+                            isnull = isnull || !((nmw[st][i] >> lane) & 1);
+                        else
+                            isnull = isnull || idx >= s_nmlen[st] ||
+                                     !(((((const uint8_t *)kbase)[nm_off + idx]) >> (row & 7)) & 1);
                     }
                     int64_t v = (w == 8) ? (int64_t)(((uint64_t)h << 32) | l)
                                          : (s_f32[st] ? (int64_t)l : (int64_t)(int32_t)l);
@@ -2273,13 +2356,55 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
             if (ok)
                 cnt_ok++;
             bool direct = ok;
+            bool spill = false;
+            uint64_t pk = 0;
             if (ok && T.n > 0 && gb_row_fast(P, L))
             {
-                int slot = lds_probe(T, gb_pack_key(P, L));
+                pk = gb_pack_key(P, L);
+                int slot = lds_probe(T, pk);
                 if (slot >= 0)
                 {
                     lds_update(P, L, T, slot);
                     direct = false;
+                }
+                else if (can_spill)
+                {
+                    spill = true;
+                    direct = false;
+                }
+            }
+            if (can_spill)
+            {
+                const uint64_t m = __ballot(spill);
+                if (m)
+                {
+                    const uint64_t k = __popcll(m);
+                    if (sp_pos + k > sp_end)
+                    {
+                        spill_fill_holes(kp, sp_pos, sp_end, rec_words);
+                        uint64_t b0 = 0;
+                        if (lane == 0)
+                            b0 = atomicAdd((unsigned long long *)&ctl->spill_next,
+                                           (unsigned long long)XSPILL_CHUNK);
+                        b0 = __shfl(b0, 0);
+                        sp_pos = b0 < kp.spill_cap ? b0 : kp.spill_cap;
+                        sp_end = b0 + XSPILL_CHUNK < kp.spill_cap ? b0 + XSPILL_CHUNK : kp.spill_cap;
+                    }
+                    const uint64_t pos = sp_pos + __popcll(m & ((1ULL << lane) - 1));
+                    if (spill)
+                    {
+                        if (pos < sp_end)
+                        {
+                            uint64_t *r = kp.spill + pos * rec_words;
+                            r[0] = pk;
+                            for (int j = 0; j < (int)P->spill_nregs; j++)
+                                r[1 + j] = (uint64_t)L.R[P->spill_regs[j]];
+                            nspilled++;
+                        }
+                        else
+                            direct = true;      /* spill buffer full */
+                    }
+                    sp_pos = sp_pos + k < sp_end ? sp_pos + k : sp_end;
                 }
             }
             if (direct)
@@ -2303,6 +2428,8 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                 row_step(i);
         }
     }
+    if (can_spill)
+        spill_fill_holes(kp, sp_pos, sp_end, rec_words);
     __syncthreads();
     if (T.n > 0)
         lds_flush<NR>(P, T, kp.kds_out, ctl);
@@ -2311,6 +2438,111 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
         atomicAdd(&ctl->nitems_in, cnt_ok);
         atomicAdd(&ctl->nitems_out, cnt_ok);
     }
+    if (lds_miss)
+        atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
+    if (nspilled)
+        atomicAdd((unsigned long long *)&ctl->spill_rows, (unsigned long long)nspilled);
+}
+
+/* ------------------------------------------------------------------
+ * second pass of a scan + group-by launch: the spill records
+ *
+ * Every block owns one hash partition of the packed keys (the number of
+ * partitions is chosen so that a partition's groups fit one LDS table) and
+ * a slice of the records; it aggregates the records of its partition in
+ * LDS and merges the table into kds_final like the first pass.
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t
+spill_partition(uint64_t pk, uint32_t nparts)
+{
+    uint64_t h = pk * 0xD6E8FEB86659FD93ULL;
+    return (uint32_t)(((h >> 32) * (uint64_t)nparts) >> 32);
+}
+
+template <int NR, class PROG>
+__device__ __forceinline__ void
+spill_agg_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    xpu_task_ctl *ctl = kp.ctl;
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n > 0)
+        lds_init(P, T);
+    __syncthreads();
+    const uint64_t reserved = __hip_atomic_load(&ctl->spill_next, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t nrec = reserved < kp.spill_cap ? reserved : kp.spill_cap;
+    if (nrec == 0 || T.n == 0 || kp.spill == nullptr)
+        return;
+    /* partitions: about half an LDS table of groups each */
+    const uint32_t ngroups = kp.kds_out->nitems;
+    uint32_t nparts = (uint32_t)((2ULL * ngroups + T.n - 1) / T.n);
+    const uint32_t maxparts = gridDim.x < 64 ? gridDim.x : 64;
+    nparts = nparts < 1 ? 1 : (nparts > maxparts ? maxparts : nparts);
+    const uint32_t part = blockIdx.x % nparts;
+    const uint32_t kblk = blockIdx.x / nparts;
+    const uint32_t nblk = (gridDim.x - part + nparts - 1) / nparts;
+    const uint64_t lo = nrec * kblk / nblk;
+    const uint64_t hi = nrec * (kblk + 1) / nblk;
+    const int rec_words = 1 + P->spill_nregs;
+
+    int64_t Rfile[NR];
+    Lane L;
+    L.R = Rfile;
+    L.nmask = ~1U;
+    L.err = 0;
+    L.dead = 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        L.R[r] = 0;
+    L.R[XP_REG_TRUE] = 1;
+    /* FILTERs passed for every spilled row */
+    for (int j = 0; j < (int)P->gb_nattrs; j++)
+    {
+        const int fr = P->aggs[j].filter_reg;
+        if (fr != 0xff)
+        {
+            L.R[fr] = 1;
+            SETNULL(L, fr, 0);
+        }
+    }
+    uint32_t lds_miss = 0;
+    for (uint64_t base = lo; base < hi; base += blockDim.x)
+    {
+        const uint64_t i = base + threadIdx.x;
+        bool mine = false;
+        uint64_t pk = XLDS_EMPTY_KEY;
+        if (i < hi)
+        {
+            pk = kp.spill[i * rec_words];
+            mine = (pk != XLDS_EMPTY_KEY) && spill_partition(pk, nparts) == part;
+        }
+        bool direct = false;
+        if (mine)
+        {
+            gb_unpack_key(P, L, pk);
+            for (int j = 0; j < (int)P->spill_nregs; j++)
+            {
+                const int r = P->spill_regs[j];
+                L.R[r] = (int64_t)kp.spill[i * rec_words + 1 + j];
+                SETNULL(L, r, 0);
+            }
+            int slot = lds_probe(T, pk);
+            if (slot >= 0)
+                lds_update(P, L, T, slot);
+            else
+                direct = true;
+        }
+        if (direct)
+            lds_miss++;
+        if (wave_any(direct))
+        {
+            uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
+            gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+        }
+    }
+    __syncthreads();
+    lds_flush<NR>(P, T, kp.kds_out, ctl);
     if (lds_miss)
         atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
 }

@@ -127,6 +127,12 @@ xpu_jit_source(const xplan *P, int kind, int param)
         append(s, "    pipeline_body<%s, %d, JitProg>((cplan *)xjit_plan, kp);\n",
                param ? "true" : "false", nr);
     s += "}\n";
+    if (kind == XJ_SCAN_AGG)
+    {
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_spill_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    spill_agg_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
+    }
     return s;
 }
 

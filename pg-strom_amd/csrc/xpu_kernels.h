@@ -19,6 +19,7 @@
 
 #define XLDS_EMPTY_KEY      0xffffffffffffffffULL
 #define XGB_LOCKED_SLOT     0xffffffffffffffffULL
+#define XSPILL_CHUNK        512     /* spill records a wave reserves at once */
 
 /* per-task control block in device memory (cleared before each launch) */
 typedef struct
@@ -35,6 +36,8 @@ typedef struct
     uint64_t    out_usage;          /* projection bytes written */
     uint64_t    lds_miss;           /* group-by rows sent straight to kds_final */
     uint64_t    global_merges;      /* group-by merges into kds_final */
+    uint64_t    spill_next;         /* spill records reserved (may pass the capacity) */
+    uint64_t    spill_rows;         /* rows written to the spill buffer */
 } xpu_task_ctl;
 
 /* error messages the kernels raise themselves (beyond the compiled table) */
@@ -60,6 +63,8 @@ typedef struct
     uint32_t    __pad;
     uint64_t    inner_kds[XP_MAX_DEPTH + 1];    /* device address of the inner KDS of
                                                    each depth (index 1..nrels) */
+    uint64_t   *spill;                  /* group-by spill records (nullptr: none) */
+    uint64_t    spill_cap;              /* capacity in records */
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -68,6 +73,9 @@ extern "C" {
 /* host-side launchers (defined in xpu_kernels.hip) */
 int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
                         size_t lds_bytes, bool preagg, void *stream);
+/* second pass over the spill records of a scan + group-by launch */
+int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
+                         size_t lds_bytes, void *stream);
 /* scan + group-by without joins; rpt = rows per lane per wave tile (1, 2, 4) */
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);

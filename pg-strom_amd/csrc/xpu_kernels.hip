@@ -19,6 +19,13 @@ xpu_scan_agg_kernel(xpu_kparams kp)
     scan_agg_body<NR, RPT, Interp>((cplan *)(uintptr_t)kp.plan, kp);
 }
 
+template <int NR>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_spill_agg_kernel(xpu_kparams kp)
+{
+    spill_agg_body<NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
+}
+
 /* ------------------------------------------------------------------
  * helper kernels
  * ------------------------------------------------------------------ */
@@ -99,6 +106,20 @@ xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int blo
         launch_scan_agg<XP_NREGS_SMALL>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
     else
         launch_scan_agg<XP_NREGS>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block, size_t lds_bytes,
+                     void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nregs <= XP_NREGS_SMALL)
+        hipLaunchKernelGGL((xpu_spill_agg_kernel<XP_NREGS_SMALL>), dim3(grid), dim3(block),
+                           lds_bytes, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_spill_agg_kernel<XP_NREGS>), dim3(grid), dim3(block),
+                           lds_bytes, s, *kp);
     return (int)hipGetLastError();
 }
 

@@ -30,6 +30,7 @@
 #define XP_MAX_KEYS         8
 #define XP_MAX_OPS          2048
 #define XP_MAX_STAGE        6       /* depth-0 columns loaded by the tile prologue */
+#define XP_MAX_SPILL        8       /* registers carried by a group-by spill record */
 
 /* XO_LDCOL imm: low byte = width; XLD_STAGED = the scan kernel's prologue has
  * already put the value into the destination register */
@@ -199,6 +200,12 @@ typedef struct
     uint16_t    lds_n64;
     uint16_t    lds_n32;
     uint32_t    lds_entry_bytes;
+    /* spill records of rows that find no room in the LDS table: packed key,
+     * then the listed argument registers (8 bytes each) */
+    uint8_t     spill_ok;
+    uint8_t     spill_nregs;
+    uint8_t     spill_regs[XP_MAX_SPILL];
+    uint8_t     __pad3[6];
     xattr       keys[XP_MAX_KEYS];
     uint8_t     key_bits[XP_MAX_KEYS];  /* packed width of each key in bits */
     xagg        aggs[XP_MAX_PROJ];

@@ -98,6 +98,8 @@ struct Scratch
     xpu_task_ctl *h_ctl = nullptr;      /* pinned */
     uint64_t   *d_status = nullptr;
     size_t      status_cap = 0;
+    uint64_t   *d_spill = nullptr;      /* group-by spill records */
+    size_t      spill_cap = 0;          /* bytes */
     char       *d_src = nullptr;
     size_t      src_cap = 0;
     char       *d_dst = nullptr;
@@ -122,6 +124,7 @@ scratch_destroy(Scratch *s)
     if (s->d_ctl) (void)hipFree(s->d_ctl);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->d_status) (void)hipFree(s->d_status);
+    if (s->d_spill) (void)hipFree(s->d_spill);
     if (s->d_src) (void)hipFree(s->d_src);
     if (s->d_dst) (void)hipFree(s->d_dst);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -162,8 +165,9 @@ struct xpuhip_session
     std::mutex      lock;
     std::vector<Scratch *> pool;
     uint64_t        ntasks = 0;
-    /* kernel specialized for the plan (loaded on first use) */
+    /* kernels specialized for the plan (loaded on first use) */
     hipFunction_t   jit_fn = nullptr;
+    hipFunction_t   jit_spill_fn = nullptr;
     int             jit_kind = 0;
     int             jit_param = -1;
 };
@@ -570,15 +574,18 @@ namespace {
 
 /* code objects of specialized kernels loaded into this process's device */
 std::mutex g_mod_lock;
-std::map<const std::vector<char> *, hipFunction_t> g_mods;
+std::map<const std::vector<char> *, std::pair<hipFunction_t, hipFunction_t>> g_mods;
 
 hipFunction_t
-jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
+jit_function(xpuhip_session *s, int kind, int param, hipFunction_t *p_spill, kern_errorbuf *eb)
 {
     {
         std::lock_guard<std::mutex> g(s->lock);
         if (s->jit_fn && s->jit_kind == kind && s->jit_param == param)
+        {
+            *p_spill = s->jit_spill_fn;
             return s->jit_fn;
+        }
     }
     std::string log;
     const std::vector<char> *code = xpu_jit_code(&s->comp.plan, kind, param, &log);
@@ -589,25 +596,34 @@ jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
         XERR(eb, ERRCODE_DEVICE_ERROR, ("specialized kernel does not compile: " + log).c_str());
         return nullptr;
     }
-    hipFunction_t fn = nullptr;
+    hipFunction_t fn = nullptr, sfn = nullptr;
     {
         std::lock_guard<std::mutex> g(g_mod_lock);
         auto it = g_mods.find(code);
         if (it != g_mods.end())
-            fn = it->second;
+        {
+            fn = it->second.first;
+            sfn = it->second.second;
+        }
         else
         {
             hipModule_t mod;
             if (!hip_ok(hipModuleLoadData(&mod, code->data()), eb, "hipModuleLoadData") ||
                 !hip_ok(hipModuleGetFunction(&fn, mod, "xpu_jit_kernel"), eb, "hipModuleGetFunction"))
                 return nullptr;
-            g_mods[code] = fn;
+            if (kind == XJ_SCAN_AGG &&
+                !hip_ok(hipModuleGetFunction(&sfn, mod, "xpu_jit_spill_kernel"), eb,
+                        "hipModuleGetFunction"))
+                return nullptr;
+            g_mods[code] = std::make_pair(fn, sfn);
         }
     }
     std::lock_guard<std::mutex> g(s->lock);
     s->jit_fn = fn;
+    s->jit_spill_fn = sfn;
     s->jit_kind = kind;
     s->jit_param = param;
+    *p_spill = sfn;
     return fn;
 }
 
@@ -687,6 +703,18 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     kp.lds_nents = lds_nents;
     for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
         kp.inner_kds[d] = P->depth[d].inner_kds;
+    /* spill buffer of a scan + group-by: room for a quarter of the rows (at
+     * least 1M records); rows beyond it merge into kds_final directly */
+    const bool spill = scan_agg && P->spill_ok && lds_nents > 0;
+    if (spill)
+    {
+        uint64_t rec = 8ULL * (1 + P->spill_nregs);
+        uint64_t nrec = nrows / 4 > (1ULL << 20) ? nrows / 4 : (1ULL << 20);
+        if (!grow((char **)&x->d_spill, &x->spill_cap, nrec * rec, eb))
+            return ERRCODE_DEVICE_ERROR;
+        kp.spill = x->d_spill;
+        kp.spill_cap = x->spill_cap / rec;
+    }
 
     int rc = 0;
     xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
@@ -698,11 +726,11 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         xpu_launch_fill(x->d_status, 0, ntiles, x->stream);
     }
     /* the kernel specialized for the plan, or the interpreter */
-    hipFunction_t jfn = nullptr;
+    hipFunction_t jfn = nullptr, jsfn = nullptr;
     if (scan_agg && t.jit != XPUHIP_JIT_OFF &&
         (t.jit == XPUHIP_JIT_ON || nrows >= (1ULL << 20)))
     {
-        jfn = jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb);
+        jfn = jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), &jsfn, eb);
         if (!jfn && t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
@@ -717,6 +745,20 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         rc = xpu_launch_scan_agg(&kp, P->nregs, (int)rpt, grid, block, lds, x->stream);
     else
         rc = xpu_launch_pipeline(&kp, P->nregs, grid, block, lds, preagg, x->stream);
+    /* second pass over the spilled rows, stream-ordered after the first (it
+     * reads the spill count and kds_final's group count on the device) */
+    if (rc == 0 && spill)
+    {
+        uint32_t sgrid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
+        if (jfn && jsfn)
+        {
+            void *args[] = {&kp};
+            rc = (int)hipModuleLaunchKernel(jsfn, sgrid, 1, 1, block, 1, 1, (unsigned)lds,
+                                            x->stream, args, nullptr);
+        }
+        else
+            rc = xpu_launch_spill_agg(&kp, P->nregs, sgrid, block, lds, x->stream);
+    }
     (void)hipEventRecord(x->ev1, x->stream);
     if (rc != 0)
     {
@@ -743,6 +785,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_stats.rows = nrows;
     t_last_stats.lds_miss = c->lds_miss;
     t_last_stats.global_merges = c->global_merges;
+    t_last_stats.spill_rows = c->spill_rows;
+    t_last_stats.jit = jfn ? 1 : 0;
     if (res)
     {
         res->nitems_raw = (uint32_t)nrows;

@@ -41,7 +41,8 @@ class xpuhip_tuning(C.Structure):
 class xpuhip_task_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("grid", C.c_uint32), ("block", C.c_uint32),
                 ("ntiles", C.c_uint32), ("lds_nents", C.c_uint32), ("lds_bytes", C.c_uint64),
-                ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64)]
+                ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64),
+                ("spill_rows", C.c_uint64), ("jit", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 _lib = None

@@ -197,3 +197,25 @@ def test_preagg_error_in_argument(gpu, jit):
         s.exec(kds.build_arrow_kds(cols))
     assert b"division by zero" in ei.value.errbuf.message
     s.close()
+
+
+def test_spill_overflow(gpu, jit):
+    """A 7-entry LDS table sends almost every row to the spill buffer, which
+    holds a quarter of the rows: the rest merge into kds_final directly.
+    Both paths together give the oracle's groups."""
+    from pgstrom import xpuhip
+    n = 5_000_000
+    cols = datagen.preagg_table(n, fragment=15)
+    src = kds.build_arrow_kds(cols)
+    blob = W.preagg_query(cols)
+    s = xpuhip.Session(blob)
+    s.set_tuning(lds_nents=7, jit=jit)
+    rc, _, _, _ = s.exec(src)
+    assert rc == 0
+    st = xpuhip.last_task_stats()
+    assert st["spill_rows"] > 0 and st["lds_miss"] > 0
+    got = s.final()
+    s.close()
+    exp = run_oracle_groupby(blob, [src])
+    groups = assert_groups_equal(got, exp, C1_ACTIONS)
+    assert sum(g[0] for g in groups.values()) == n
