@@ -28,18 +28,22 @@ def log(msg):
 
 
 def time_steps(run_step, warmup, steps):
+    """Runs warmup + steps tasks; returns wall seconds per step, the mean
+    kernel time of all passes (HIP events on the task's stream) and the
+    per-step stats of the library."""
     from pgstrom import xpuhip
     for _ in range(warmup):
         run_step()
     xpuhip.lib().xpuhip_device_synchronize()
-    kms = []
+    kms, stats = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         run_step()
         kms.append(xpuhip.last_kernel_ms())
+        stats.append(xpuhip.last_task_stats())
     xpuhip.lib().xpuhip_device_synchronize()
     wall = (time.perf_counter() - t0) / steps
-    return wall, float(np.mean(kms)), kms
+    return wall, float(np.mean(kms)), stats
 
 
 def bench_preagg(n, warmup, steps):
@@ -56,7 +60,7 @@ def bench_preagg(n, warmup, steps):
         rc, res, _ = s.exec_device(d_src, n)
         assert rc == 0
 
-    wall, kms, _ = time_steps(step, warmup, steps)
+    wall, kms, stats = time_steps(step, warmup, steps)
     fin = s.final()
     groups = W.decode_groups(fin, [a for a, _, _ in W.preagg_tlist(None, cols[0], None, None, None)])
     total = sum(g[0] for g in groups.values())
@@ -67,7 +71,11 @@ def bench_preagg(n, warmup, steps):
     d_src.free()
     algo_bytes = n * (4 + 8 + 8 + 4)
     return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo_bytes,
-                groups=len(groups), tuning=tuning, cols=cols)
+                groups=len(groups), tuning=tuning, cols=cols,
+                pass1_ms=float(np.mean([st["pass1_ms"] for st in stats])),
+                spill_ms=float(np.mean([st["spill_ms"] for st in stats])),
+                spill_frac=stats[-1]["spill_rows"] / n, jit=stats[-1]["jit"],
+                lds_nents=stats[-1]["lds_nents"])
 
 
 def bench_scan(n, warmup, steps):
@@ -123,6 +131,32 @@ def bench_star(n, warmup, steps):
     return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo)
 
 
+def traffic_of(name):
+    """HBM bytes per launch of the dominant kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("traffic_bytes"), t.get("source")
+
+
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (run under --pmc)."""
+    from pgstrom import xpuhip
+    xpuhip.init(0)
+    a = xpuhip.DeviceBuffer(nbytes)
+    b = xpuhip.DeviceBuffer(nbytes)
+    for _ in range(3):
+        xpuhip.lib().xpuhip_memcpy_dtod(b.ptr, a.ptr, nbytes)
+    xpuhip.lib().xpuhip_device_synchronize()
+    a.free()
+    b.free()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
 def cpu_baseline(cols_fn, sample_rows):
     """The oracle (own CPU restatement, multi-threaded) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -149,8 +183,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--only", default="all", choices=["all", "preagg"])
-    ap.add_argument("--cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=20_000_000,
+                    help="rows of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many bytes (counter scaling)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not run)" % args.gpus, file=sys.stderr)
@@ -161,7 +201,9 @@ def main():
 
     pa = bench_preagg(args.rows, args.warmup, args.steps)
     kms = pa["kernel_ms"]
-    achieved = pa["algo_bytes"] / (kms / 1e3)
+    # dominant kernel: the scan + group-by pass, which reads every column byte
+    achieved = pa["algo_bytes"] / (pa["pass1_ms"] / 1e3)
+    traffic, traffic_src = traffic_of("preagg_c1_traffic.json")
     others = {}
     if args.only == "all":
         sc = bench_scan(1_000_000, 3, 20)
@@ -175,8 +217,10 @@ def main():
                                         kernel_ms=st["kernel_ms"], wall_ms=st["wall_ms"],
                                         hbm_gbs=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
                                         frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK)
-    log("cpu baseline")
-    cpu = cpu_baseline(None, args.cpu_sample)
+    cpu = None
+    if args.cpu_sample > 0:
+        log("cpu baseline")
+        cpu = cpu_baseline(None, args.cpu_sample)
 
     line = {
         "metric": "rows/s & %HBM-peak per scan/join/preagg; p99 latency, batched vs plain",
@@ -195,8 +239,13 @@ def main():
                                "w float8, x int4; GROUP BY key count(*), sum(v), avg(w), min(x), max(x)",
                    "rows": pa["rows"], "groups": pa["groups"], "tuning": pa["tuning"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel_ms": kms, "algo_bytes_per_launch": pa["algo_bytes"]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": "xpu_jit_kernel" if pa["jit"] else "xpu_scan_agg_kernel",
+                     "kernel_ms": pa["pass1_ms"], "spill_pass_ms": pa["spill_ms"],
+                     "all_passes_ms": kms, "algo_bytes_per_launch": pa["algo_bytes"],
+                     "algo_bytes_per_row": 24, "spilled_row_frac": pa["spill_frac"],
+                     "lds_table_entries": pa["lds_nents"]},
         "cpu_baseline": cpu,
         "workloads": others,
     }

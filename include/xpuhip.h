@@ -115,6 +115,7 @@ int     xpuhip_device_alloc(size_t length, void **p_dptr, kern_errorbuf *errbuf)
 int     xpuhip_device_free(void *dptr);
 int     xpuhip_memcpy_htod(void *dptr, const void *src, size_t length);
 int     xpuhip_memcpy_dtoh(void *dst, const void *dptr, size_t length);
+int     xpuhip_memcpy_dtod(void *dst, const void *src, size_t length);   /* copy kernel */
 int     xpuhip_device_synchronize(void);
 
 /* Kernel time of the last task executed by the calling thread, in
@@ -126,7 +127,9 @@ double  xpuhip_last_kernel_ms(void);
  * counterpart there and serves tuning and profiling). */
 typedef struct
 {
-    double      kernel_ms;          /* as xpuhip_last_kernel_ms */
+    double      kernel_ms;          /* as xpuhip_last_kernel_ms (all passes) */
+    double      pass1_ms;           /* the pipeline / scan kernel alone */
+    double      spill_ms;           /* the group-by spill pass alone */
     uint32_t    grid;               /* blocks launched */
     uint32_t    block;              /* threads per block */
     uint32_t    ntiles;             /* row tiles of the chunk */

@@ -80,6 +80,7 @@ int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
+int xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream);
 int xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head,
                             uint32_t head_len, uint64_t length, const xpu_task_ctl *ctl,
                             void *stream);

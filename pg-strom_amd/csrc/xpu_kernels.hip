@@ -123,6 +123,37 @@ xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block, size
     return (int)hipGetLastError();
 }
 
+/* device-to-device copy in 16-byte words (plus a byte tail) */
+__global__ void
+xpu_copy_kernel(uint4 *dst, const uint4 *src, size_t nwords, char *dtail, const char *stail,
+                size_t ntail)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < nwords; i += stride)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < ntail)
+        dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+extern "C" int
+xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream)
+{
+    if (nbytes == 0)
+        return 0;
+    size_t nwords = nbytes / 16;
+    size_t ntail = nbytes % 16;
+    size_t blocks = (nwords + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(xpu_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       (uint4 *)dst, (const uint4 *)src, nwords, (char *)dst + nwords * 16,
+                       (const char *)src + nwords * 16, ntail);
+    return (int)hipGetLastError();
+}
+
 extern "C" int
 xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream)
 {

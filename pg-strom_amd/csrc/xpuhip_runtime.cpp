@@ -93,7 +93,7 @@ ensure_device(kern_errorbuf *eb)
 struct Scratch
 {
     hipStream_t stream = nullptr;
-    hipEvent_t  ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t  ev0 = nullptr, ev1 = nullptr, evm = nullptr;
     xpu_task_ctl *d_ctl = nullptr;
     xpu_task_ctl *h_ctl = nullptr;      /* pinned */
     uint64_t   *d_status = nullptr;
@@ -112,6 +112,7 @@ scratch_create(Scratch *s, kern_errorbuf *eb)
     return hip_ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), eb, "hipStreamCreate") &&
            hip_ok(hipEventCreate(&s->ev0), eb, "hipEventCreate") &&
            hip_ok(hipEventCreate(&s->ev1), eb, "hipEventCreate") &&
+           hip_ok(hipEventCreate(&s->evm), eb, "hipEventCreate") &&
            hip_ok(hipMalloc((void **)&s->d_ctl, sizeof(xpu_task_ctl)), eb, "hipMalloc") &&
            hip_ok(hipHostMalloc((void **)&s->h_ctl, sizeof(xpu_task_ctl), hipHostMallocDefault),
                   eb, "hipHostMalloc");
@@ -129,6 +130,7 @@ scratch_destroy(Scratch *s)
     if (s->d_dst) (void)hipFree(s->d_dst);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->evm) (void)hipEventDestroy(s->evm);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -749,6 +751,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
      * reads the spill count and kds_final's group count on the device) */
     if (rc == 0 && spill)
     {
+        (void)hipEventRecord(x->evm, x->stream);
         uint32_t sgrid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
         if (jfn && jsfn)
         {
@@ -773,10 +776,15 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         return ERRCODE_DEVICE_ERROR;
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, x->ev0, x->ev1);
+    float ms1 = ms;
+    if (spill)
+        (void)hipEventElapsedTime(&ms1, x->ev0, x->evm);
     t_last_kernel_ms = ms;
 
     const xpu_task_ctl *c = x->h_ctl;
     t_last_stats.kernel_ms = ms;
+    t_last_stats.pass1_ms = ms1;
+    t_last_stats.spill_ms = ms - ms1;
     t_last_stats.grid = grid;
     t_last_stats.block = block;
     t_last_stats.ntiles = ntiles;
@@ -1018,6 +1026,17 @@ extern "C" int
 xpuhip_memcpy_dtoh(void *h, const void *d, size_t len)
 {
     return hipMemcpy(h, d, len, hipMemcpyDeviceToHost) == hipSuccess ? 0 : ERRCODE_DEVICE_ERROR;
+}
+
+extern "C" int
+xpuhip_memcpy_dtod(void *dst, const void *src, size_t len)
+{
+    kern_errorbuf eb;
+    if (ensure_device(&eb) != 0)
+        return ERRCODE_DEVICE_ERROR;
+    if (xpu_launch_copy(dst, src, len, nullptr) != 0 || hipDeviceSynchronize() != hipSuccess)
+        return ERRCODE_DEVICE_ERROR;
+    return 0;
 }
 
 extern "C" int

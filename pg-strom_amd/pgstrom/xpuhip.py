@@ -17,7 +17,7 @@ EXPORTS = [
     "xpuhip_session_attach_inner", "xpuhip_task_exec", "xpuhip_task_exec_device",
     "xpuhip_task_final", "xpuhip_session_close", "xpuhip_session_set_tuning",
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
-    "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_device_synchronize",
+    "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_memcpy_dtod", "xpuhip_device_synchronize",
     "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
     "xpuhip_last_task_stats", "xpuhip_jit_prepare",
 ]
@@ -39,7 +39,8 @@ class xpuhip_tuning(C.Structure):
 
 
 class xpuhip_task_stats(C.Structure):
-    _fields_ = [("kernel_ms", C.c_double), ("grid", C.c_uint32), ("block", C.c_uint32),
+    _fields_ = [("kernel_ms", C.c_double), ("pass1_ms", C.c_double), ("spill_ms", C.c_double),
+                ("grid", C.c_uint32), ("block", C.c_uint32),
                 ("ntiles", C.c_uint32), ("lds_nents", C.c_uint32), ("lds_bytes", C.c_uint64),
                 ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64),
                 ("spill_rows", C.c_uint64), ("jit", C.c_uint32), ("_pad", C.c_uint32)]
@@ -71,6 +72,7 @@ def lib():
         L.xpuhip_device_free.argtypes = [vp]
         L.xpuhip_memcpy_htod.argtypes = [vp, vp, sz]
         L.xpuhip_memcpy_dtoh.argtypes = [vp, vp, sz]
+        L.xpuhip_memcpy_dtod.argtypes = [vp, vp, sz]
         L.xpuhip_last_kernel_ms.restype = C.c_double
         L.xpuhip_abi_layout_json.restype = C.c_char_p
         L.xpuhip_session_explain.restype = C.c_int64
