@@ -38,7 +38,7 @@ typedef struct
     uint32_t    jit;                /* XPUHIP_JIT_*: kernel specialized for the plan */
 } xpuhip_tuning;
 
-#define XPUHIP_JIT_AUTO     0       /* specialize chunks of >= 1M rows */
+#define XPUHIP_JIT_AUTO     0       /* specialize chunks of >= 1,000,000 rows */
 #define XPUHIP_JIT_ON       1       /* always specialize */
 #define XPUHIP_JIT_OFF      2       /* always run the op interpreter */
 
@@ -60,7 +60,8 @@ int     xpuhip_session_check(const kern_session_info *session, size_t length,
                              kern_errorbuf *errbuf);
 
 /* Compiles (and keeps for the process) the kernel specialized for the
- * session blob, so that the first task does not wait for it; needs no GPU.
+ * session blob with the default launch shape, so that the first task does
+ * not wait for it; needs no GPU.
  * Has no counterpart in the reference, which interprets every plan. */
 int     xpuhip_jit_prepare(const kern_session_info *session, size_t length,
                            kern_errorbuf *errbuf);

@@ -18,3 +18,10 @@ def gpu():
     from pgstrom import xpuhip
     xpuhip.init(0)
     return xpuhip
+
+
+@pytest.fixture(params=[2, 1], ids=["interp", "jit"])
+def jit(request):
+    """Both kernels of a plan: the op interpreter (XPUHIP_JIT_OFF) and the
+    kernel specialized for the plan with hiprtc (XPUHIP_JIT_ON)."""
+    return request.param

@@ -8,10 +8,12 @@ from pgstrom import abi, kds, workloads as W, xpuhip
 import oracle
 
 
-def run_gpu_projection(blob, src, dst_length, kmrels=None):
+def run_gpu_projection(blob, src, dst_length, kmrels=None, tuning=None):
     """All rows of one chunk through xpuhip_task_exec, following 'd' resumes;
     returns the list of kds_dst buffers."""
     s = xpuhip.Session(blob)
+    if tuning:
+        s.set_tuning(**tuning)
     if kmrels is not None:
         s.attach_inner(kmrels)
     outs = []

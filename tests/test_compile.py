@@ -96,3 +96,14 @@ def test_jit_prepare_compiles(which):
                           (abi.KAGG_ACTION__PSUM_FP, plan.scan_var("w"), ("f", "bytea"))])
         blob = plan.build(final_length=4 << 20)
     xpuhip.jit_prepare(blob)
+
+
+@pytest.mark.parametrize("which", ["c0", "c2", "c2_left"])
+def test_jit_prepare_pipeline_compiles(which):
+    """Projection and join plans specialise too (the pipeline kernel)."""
+    if which == "c0":
+        blob = W.scan_query(datagen.scan_table(10))
+    else:
+        fact, _ = datagen.star_tables(10, dim_sizes=(10, 10, 10))
+        blob = W.star_query(fact, left_outer=(which == "c2_left"))
+    xpuhip.jit_prepare(blob)

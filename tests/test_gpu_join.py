@@ -13,17 +13,17 @@ C2_ACTIONS = [abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__VREF, abi.KAGG_ACTIO
 
 
 @pytest.mark.parametrize("left_outer", [False, True])
-def test_c2_star_join_preagg(gpu, left_outer):
+def test_c2_star_join_preagg(gpu, left_outer, jit):
     fact, dims = datagen.star_tables(300_000, dim_sizes=(20_000, 2_000, 2_556))
     src = kds.build_arrow_kds(fact)
     km = W.star_inner(dims)
     blob = W.star_query(fact, left_outer=left_outer)
-    got = run_gpu_groupby(blob, [src], kmrels=km)
+    got = run_gpu_groupby(blob, [src], kmrels=km, tuning=dict(jit=jit))
     exp = run_oracle_groupby(blob, [src], kmrels=km)
     assert_groups_equal(got, exp, C2_ACTIONS)
 
 
-def test_join_projection_multi_match(gpu):
+def test_join_projection_multi_match(gpu, jit):
     """Inner relation with duplicate keys: every match is emitted, in the
     oracle's (row-major) order."""
     r = datagen.rng(9)
@@ -49,7 +49,7 @@ def test_join_projection_multi_match(gpu):
     plan.set_projection([v, iid, it], [("v", "int8"), ("id", "int2"), ("t", "text")])
     blob = plan.build()
     src = kds.build_arrow_kds(outer)
-    got = run_gpu_projection(blob, src, 32 << 20, kmrels=km)
+    got = run_gpu_projection(blob, src, 32 << 20, kmrels=km, tuning=dict(jit=jit))
     exp, res = run_oracle_projection(blob, src, 32 << 20, kmrels=km)
     assert len(got) == 1
     assert np.array_equal(got[0], exp)

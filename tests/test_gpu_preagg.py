@@ -8,14 +8,6 @@ from helpers import assert_groups_equal, run_gpu_groupby, run_oracle_groupby
 
 pytestmark = pytest.mark.gpu
 
-JIT_OFF, JIT_ON = 2, 1
-
-
-@pytest.fixture(params=[JIT_OFF, JIT_ON], ids=["interp", "jit"])
-def jit(request):
-    """Both kernels of a plan without joins: the op interpreter and the
-    kernel specialized for the plan (compiled with hiprtc)."""
-    return request.param
 
 C1_ACTIONS = [abi.KAGG_ACTION__NROWS_ANY, abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__PAVG_FP,
               abi.KAGG_ACTION__PMIN_INT32, abi.KAGG_ACTION__PMAX_INT32, abi.KAGG_ACTION__VREF]

@@ -10,12 +10,12 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 10000, 1_000_000])
-def test_c0_scan_bytes(gpu, n):
+def test_c0_scan_bytes(gpu, n, jit):
     cols = datagen.scan_table(n)
     src = kds.build_arrow_kds(cols)
     blob = W.scan_query(cols)
     dst_len = 64 << 20
-    got = run_gpu_projection(blob, src, dst_len)
+    got = run_gpu_projection(blob, src, dst_len, tuning=dict(jit=jit))
     exp, res = run_oracle_projection(blob, src, dst_len)
     assert len(got) == 1
     assert np.array_equal(got[0], exp)
@@ -30,14 +30,14 @@ def test_c0_scan_param(gpu):
     assert np.array_equal(got[0], exp)
 
 
-def test_c0_scan_resume(gpu):
+def test_c0_scan_resume(gpu, jit):
     """A kds_dst too small for the chunk: 'd' returns, the caller ships the
     partial buffer and resumes; the concatenation equals one big run."""
     cols = datagen.scan_table(200_000)
     src = kds.build_arrow_kds(cols)
     blob = W.scan_query(cols)
     small = 1 << 20
-    parts = run_gpu_projection(blob, src, small)
+    parts = run_gpu_projection(blob, src, small, tuning=dict(jit=jit))
     assert len(parts) > 1
     rows = []
     for p in parts:
@@ -80,18 +80,18 @@ EXPRS = [
 
 
 @pytest.mark.parametrize("i", range(len(EXPRS)))
-def test_scan_expressions(gpu, i):
+def test_scan_expressions(gpu, i, jit):
     quals, targets, coldefs = EXPRS[i]
     cols = datagen.scan_table(30000, null_frac=0.2)
     cols[0].values[:50] = 0          # a = 0 rows for the division guard
     src = kds.build_arrow_kds(cols)
     blob = _expr_plan(cols, quals, targets, coldefs)
-    got = run_gpu_projection(blob, src, 16 << 20)
+    got = run_gpu_projection(blob, src, 16 << 20, tuning=dict(jit=jit))
     exp, _ = run_oracle_projection(blob, src, 16 << 20)
     assert np.array_equal(got[0], exp)
 
 
-def test_scan_error_raised(gpu):
+def test_scan_error_raised(gpu, jit):
     """int4 overflow in a target raises the reference's error text."""
     from pgstrom import xpuhip
     cols = datagen.scan_table(1000, null_frac=0.0)
@@ -100,6 +100,6 @@ def test_scan_error_raised(gpu):
     blob = _expr_plan(cols, lambda v, p: cg.Const("bool", True),
                       lambda v: [cg.Func("int4pl", v["a"], cg.Const("int4", 1))], [("a", "int4")])
     with pytest.raises(xpuhip.XpuError) as ei:
-        run_gpu_projection(blob, src, 4 << 20)
+        run_gpu_projection(blob, src, 4 << 20, tuning=dict(jit=jit))
     assert ei.value.code == abi.ERRCODE_DEVICE_ERROR
     assert b"int4pl: value out of range" in ei.value.errbuf.message
