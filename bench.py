@@ -131,6 +131,72 @@ def bench_star(n, warmup, steps):
     return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo)
 
 
+class _DevPtr:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def bench_batch(nfrag, base_rows, warmup, steps):
+    """Batched form: nfrag scan + pre-agg fragments (own predicate constant
+    a < U[1e5,9e5] and GROUP BY key mod m, m log-uniform in [16,1e5]) over
+    row windows of log-uniform size [64K, 64M] of one resident columnar table
+    (key int4 Zipf(1.1) over 10,000, v int8, w float8, x int4, a int4); one
+    persistent graph-captured launch against one launch per fragment."""
+    from pgstrom import datagen, kds, workloads as W, xpuhip
+    r = np.random.default_rng(20261202 + 3)
+    log("generating %d-row base table for %d fragments" % (base_rows, nfrag))
+    cols = datagen.preagg_table(base_rows, seed_workload=3)
+    cols = [kds.Column("a", "int4", r.integers(0, 1_000_000, size=base_rows, dtype=np.int32))] + cols
+    base = kds.build_arrow_kds(cols)
+    sizes = np.exp(r.uniform(np.log(64 << 10), np.log(64 << 20), size=nfrag)).astype(np.int64)
+    sizes = np.minimum(sizes, base_rows)
+    windows = [(int(r.integers(0, (base_rows - n) // 64 + 1)) * 64, int(n)) for n in sizes]
+    stride = 1024
+    hdrs = kds.window_headers(base, windows, nfrag * stride, stride)
+    dev = xpuhip.DeviceBuffer(len(hdrs) + len(base))
+    xpuhip.lib().xpuhip_memcpy_htod(dev.ptr, hdrs.ctypes.data, len(hdrs))
+    xpuhip.lib().xpuhip_memcpy_htod(dev.ptr + len(hdrs), base.ctypes.data, len(base))
+    del base
+    sessions = []
+    for i in range(nfrag):
+        k = int(r.integers(100_000, 900_001))
+        m = int(round(np.exp(r.uniform(np.log(16), np.log(100_000)))))
+        s = xpuhip.Session(W.fragment_query(cols, k=k, m=m))
+        s.set_tuning(jit=xpuhip.JIT_ON)
+        sessions.append(s)
+    total = int(sizes.sum())
+    tasks = [(s, dev.ptr + i * stride, n) for i, (s, (_, n)) in enumerate(zip(sessions, windows))]
+
+    def plain():
+        for s, p, n in tasks:
+            rc, _, _ = s.exec_device(_DevPtr(p), n)
+            assert rc == 0
+
+    def batched():
+        xpuhip.batch_exec(tasks, graph=True)
+
+    out = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for s in sessions:
+            s.reset()
+        for _ in range(warmup):
+            fn()
+        xpuhip.lib().xpuhip_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        xpuhip.lib().xpuhip_device_synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        out[name] = dict(rows_per_s=total / wall, ms_per_step=wall * 1e3)
+    for s in sessions:
+        s.close()
+    dev.free()
+    out["fragments"] = nfrag
+    out["rows_per_step"] = total
+    out["speedup"] = out["batched"]["rows_per_s"] / out["plain"]["rows_per_s"]
+    return out
+
+
 def traffic_of(name):
     """HBM bytes per launch of the dominant kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null if absent)."""
@@ -182,7 +248,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--only", default="all", choices=["all", "preagg"])
+    ap.add_argument("--only", default="all", choices=["all", "preagg", "batch"])
+    ap.add_argument("--batch-frags", type=int, default=256)
+    ap.add_argument("--batch-base-rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
                     help="rows of the CPU baseline sample (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -199,6 +267,9 @@ def main():
     from pgstrom import xpuhip
     xpuhip.init(0)
 
+    if args.only == "batch":
+        print(json.dumps(bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)))
+        return
     pa = bench_preagg(args.rows, args.warmup, args.steps)
     kms = pa["kernel_ms"]
     # dominant kernel: the scan + group-by pass, which reads every column byte
@@ -212,6 +283,7 @@ def main():
                                  hbm_gbs=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / 1e9,
                                  frac=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / HBM_PEAK,
                                  rows_out=sc["nout"])
+        others["batched_256"] = bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)
         st = bench_star(200_000_000, 1, 5)
         others["star_join_200M"] = dict(rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3),
                                         kernel_ms=st["kernel_ms"], wall_ms=st["wall_ms"],

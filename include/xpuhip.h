@@ -107,6 +107,34 @@ int64_t xpuhip_task_final(xpuhip_session *sess, kern_data_store *kds_final, size
 /* Releases the session and its device memory (gpuservCloseSession). */
 void    xpuhip_session_close(xpuhip_session *sess);
 
+/* One fragment of a batched launch: a scan + group-by session (no joins)
+ * and its chunk already in device memory. */
+typedef struct
+{
+    xpuhip_session *sess;
+    const kern_data_store *d_kds_src;   /* KDS_FORMAT_ARROW, device address */
+    uint32_t    nitems;                 /* rows of the chunk */
+    uint32_t    __pad;
+    kern_exec_results *results;         /* per-fragment statistics (may be NULL) */
+} xpuhip_batch_task;
+
+#define XPUHIP_BATCH_GRAPH  1   /* capture the launch sequence as a hipGraph; an
+                                   identical later call replays it */
+
+/* XpuTaskExec of many fragments in one persistent launch: the sessions must
+ * share one plan shape (they may differ in parameter values, chunks and
+ * kds_final).  Work items of all fragments are taken from one queue; each
+ * fragment's kds_final accumulates exactly what xpuhip_task_exec_device of
+ * that fragment alone would.  Launch shape from the first session's tuning.
+ * Returns 0 or the first fragment's error (errbuf names the fragment).
+ * Has no counterpart in the reference, which launches one kernel per chunk. */
+int     xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flags,
+                          kern_errorbuf *errbuf);
+
+/* Clears a group-by session's kds_final (the session can run again from
+ * scratch, e.g. after tuning trials). */
+int     xpuhip_session_reset(xpuhip_session *sess, kern_errorbuf *errbuf);
+
 /* Launch tuning of one session (see xpuhip_tuning). */
 int     xpuhip_session_set_tuning(xpuhip_session *sess, const xpuhip_tuning *tuning);
 int     xpuhip_session_get_tuning(xpuhip_session *sess, xpuhip_tuning *tuning);

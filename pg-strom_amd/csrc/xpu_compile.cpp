@@ -98,6 +98,7 @@ struct Compiler
     xplan      *P;
     std::vector<xop> ops;
     int         slot_reg[256];      /* -1 until first use */
+    int         param_index[XP_MAX_PARAMS];     /* param_id -> XO_PARAM index, -1 */
     int         slot_type[256];
     int         nslots = 0;
     int         slot_top = 3;       /* R0 TRUE, R1 result, R2 tuple */
@@ -340,7 +341,37 @@ struct Compiler
                     fail("parameter offset out of the session blob");
                     return bad;
                 }
-                return const_value(k->exptype, off == 0, (const char *)s + off);
+                int dom = type_domain(k->exptype);
+                if (dom < 0 || dom == XD_TEXT || id >= XP_MAX_PARAMS)
+                    return const_value(k->exptype, off == 0, (const char *)s + off);
+                /* fixed width: a run-time value of the launch */
+                if (param_index[id] < 0)
+                {
+                    if (out->params.size() >= XP_MAX_PARAMS)
+                        return const_value(k->exptype, off == 0, (const char *)s + off);
+                    int64_t v = 0;
+                    const char *val = (const char *)s + off;
+                    if (off)
+                    {
+                        switch (k->exptype)
+                        {
+                            case TypeOpCode__bool: v = *(const uint8_t *)val ? 1 : 0; break;
+                            case TypeOpCode__int1: v = *(const int8_t *)val; break;
+                            case TypeOpCode__int2: { int16_t x; memcpy(&x, val, 2); v = x; } break;
+                            case TypeOpCode__int4: case TypeOpCode__date:
+                                { int32_t x; memcpy(&x, val, 4); v = x; } break;
+                            case TypeOpCode__float4: { uint32_t x; memcpy(&x, val, 4); v = x; } break;
+                            default: memcpy(&v, val, 8); break;
+                        }
+                    }
+                    param_index[id] = (int)out->params.size();
+                    if (!off)
+                        out->param_nulls |= 1ULL << out->params.size();
+                    out->params.push_back(v);
+                }
+                int r = alloc_reg();
+                emit(XO_PARAM, r, 0, 0, XP_REG_TRUE, 0, param_index[id]);
+                return Reg{r, dom, k->exptype};
             }
             case FuncOpCode__VarExpr:
             {
@@ -790,6 +821,8 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     xplan *P = &out->plan;
 
     memset(P, 0, sizeof(xplan));
+    for (int i = 0; i < XP_MAX_PARAMS; i++)
+        C.param_index[i] = -1;
     C.s = s;
     C.length = length;
     C.out = out;
@@ -1185,7 +1218,7 @@ xpu_plan_dump(const xpu_compiled *c)
         "NEG_F32", "ABS_F32", "NEG_F64", "ABS_F64", "I2I", "F32_I", "F64_I", "I_F32", "I_F64",
         "F32_F64", "F64_F32", "I_BOOL", "TEXTLEN", "AND3", "OR3", "NOT3", "ISNULL", "ISNOTNULL",
         "BOOLTEST", "COALESCE", "G_NOTFALSE", "G_NOTTRUE", "G_ISNULL", "G_POSITIVE", "G_NONZERO",
-        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL"};
+        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL", "PARAM"};
     const xplan *P = &c->plan;
     std::string s;
     char buf[256];

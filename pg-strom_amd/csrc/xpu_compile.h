@@ -20,6 +20,8 @@ struct xpu_compiled
     bool                has_groupby = false;
     uint32_t            nslots = 0;
     uint32_t            lds_entry_bytes = 0;    /* bytes per LDS group entry */
+    std::vector<int64_t> params;                /* XO_PARAM values (raw register form) */
+    uint64_t            param_nulls = 0;        /* bit i: parameter i is NULL */
 };
 
 /* Validates and translates the session blob; returns 0 or an ERRCODE_*

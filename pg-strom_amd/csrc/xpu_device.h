@@ -817,6 +817,10 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
             case XO_KILL:
                 L.dead = RNULL(L, a) || L.R[a] == 0;
                 break;
+            case XO_PARAM:
+                L.R[d] = kp.params[o.imm];
+                SETNULL(L, d, (kp.param_nulls >> o.imm) & 1);
+                break;
             default:
                 break;
         }
@@ -2128,25 +2132,28 @@ pick_u32(const uint32_t (&a)[RPT], int i)
     return v;
 }
 
+/* counters of one wave over the ranges it processed */
+struct ScanAggWave
+{
+    uint32_t    cnt_ok;         /* rows that passed the scan quals */
+    uint32_t    lds_miss;       /* rows merged into kds_final directly */
+    uint32_t    nspilled;       /* rows written to the spill buffer */
+};
+
+/* the wave tiles wt_first, wt_first + wt_stride, ... of the rows
+ * [kp.row_begin, kp.row_end) of kp.kds_src, folded into T (whose entries
+ * belong to kp.kds_out) */
 template <int NR, int RPT, class PROG>
 __device__ __forceinline__ void
-scan_agg_body(cplan *P, const xpu_kparams &kp)
+scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
+               uint64_t wt_stride, ScanAggWave &W)
 {
-    extern __shared__ unsigned long long lds_dyn[];
     const kern_data_store *__restrict__ kds = kp.kds_src;
     const char *kbase = (const char *)kds;
     ckds *kmeta = (ckds *)(uintptr_t)kds;
     xpu_task_ctl *ctl = kp.ctl;
     const int lane = __lane_id();
-    /* wave-uniform (the backend cannot prove threadIdx.x >> 6 uniform, and
-     * everything derived from it, tile base included, would go divergent) */
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nwaves = blockDim.x >> 6;
     const int nstage = (int)P->nstage;
-
-    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
-    if (T.n > 0)
-        lds_init(P, T);
 
     /* staged columns' geometry, loaded once (uniform: scalar registers) */
     const char *s_vbase[XP_MAX_STAGE];
@@ -2176,9 +2183,7 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
     }
     const int pc_begin = P->seg_scan.begin + P->stage_prefix;
     const int pc_end = P->seg_final.end;
-    __syncthreads();
 
-    uint32_t cnt_ok = 0, lds_miss = 0, nspilled = 0;
     /* rows that find no room in the LDS table are appended to the spill
      * buffer (a second kernel aggregates them by hash partition); each wave
      * reserves XSPILL_CHUNK records at a time, so appending costs one device
@@ -2188,9 +2193,8 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
     uint64_t sp_pos = 0, sp_end = 0;
     const uint64_t wrows = 64ULL * RPT;
     const uint64_t nwt = ((uint64_t)(kp.row_end - kp.row_begin) + wrows - 1) / wrows;
-    const uint64_t stride = (uint64_t)gridDim.x * nwaves;
     uint32_t iter = 0;
-    for (uint64_t wt = (uint64_t)blockIdx.x * nwaves + wid; wt < nwt; wt += stride, iter++)
+    for (uint64_t wt = wt_first; wt < nwt; wt += wt_stride, iter++)
     {
         /* another wave's error ends the task (checked now and then) */
         if ((iter & 31) == 31 &&
@@ -2354,7 +2358,7 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                 ok = false;
             }
             if (ok)
-                cnt_ok++;
+                W.cnt_ok++;
             bool direct = ok;
             bool spill = false;
             uint64_t pk = 0;
@@ -2399,7 +2403,7 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                             r[0] = pk;
                             for (int j = 0; j < (int)P->spill_nregs; j++)
                                 r[1 + j] = (uint64_t)L.R[P->spill_regs[j]];
-                            nspilled++;
+                            W.nspilled++;
                         }
                         else
                             direct = true;      /* spill buffer full */
@@ -2408,7 +2412,7 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                 }
             }
             if (direct)
-                lds_miss++;
+                W.lds_miss++;
             if (wave_any(direct))
             {
                 uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
@@ -2430,19 +2434,48 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
     }
     if (can_spill)
         spill_fill_holes(kp, sp_pos, sp_end, rec_words);
+}
+
+/* adds a wave's counters to a task's control block */
+__device__ __forceinline__ void
+scan_agg_report(xpu_task_ctl *ctl, ScanAggWave &W)
+{
+    if (W.cnt_ok)
+    {
+        atomicAdd(&ctl->nitems_in, W.cnt_ok);
+        atomicAdd(&ctl->nitems_out, W.cnt_ok);
+    }
+    if (W.lds_miss)
+        atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)W.lds_miss);
+    if (W.nspilled)
+        atomicAdd((unsigned long long *)&ctl->spill_rows, (unsigned long long)W.nspilled);
+    W.cnt_ok = W.lds_miss = W.nspilled = 0;
+}
+
+/* one task: every block takes the wave tiles blockIdx.x * nwaves + wid,
+ * strided by the grid's wave count */
+template <int NR, int RPT, class PROG>
+__device__ __forceinline__ void
+scan_agg_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    /* wave-uniform (the backend cannot prove threadIdx.x >> 6 uniform, and
+     * everything derived from it, tile base included, would go divergent) */
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n > 0)
+        lds_init(P, T);
+    __syncthreads();
+    ScanAggWave W = {0, 0, 0};
+    scan_agg_range<NR, RPT, PROG>(P, kp, T, (uint64_t)blockIdx.x * nwaves + wid,
+                                  (uint64_t)gridDim.x * nwaves, W);
     __syncthreads();
     if (T.n > 0)
-        lds_flush<NR>(P, T, kp.kds_out, ctl);
-    if (cnt_ok)
-    {
-        atomicAdd(&ctl->nitems_in, cnt_ok);
-        atomicAdd(&ctl->nitems_out, cnt_ok);
-    }
-    if (lds_miss)
-        atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
-    if (nspilled)
-        atomicAdd((unsigned long long *)&ctl->spill_rows, (unsigned long long)nspilled);
+        lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+    scan_agg_report(kp.ctl, W);
 }
+
 
 /* ------------------------------------------------------------------
  * second pass of a scan + group-by launch: the spill records
@@ -2459,33 +2492,15 @@ spill_partition(uint64_t pk, uint32_t nparts)
     return (uint32_t)(((h >> 32) * (uint64_t)nparts) >> 32);
 }
 
-template <int NR, class PROG>
-__device__ __forceinline__ void
-spill_agg_body(cplan *P, const xpu_kparams &kp)
+/* the spill records [lo, hi) of kp.spill whose key falls in partition part
+ * of nparts, folded into T (merged into kp.kds_out by the caller) */
+template <int NR>
+__device__ __forceinline__ uint32_t
+spill_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t lo, uint64_t hi,
+                uint32_t part, uint32_t nparts)
 {
-    extern __shared__ unsigned long long lds_dyn[];
     xpu_task_ctl *ctl = kp.ctl;
-    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
-    if (T.n > 0)
-        lds_init(P, T);
-    __syncthreads();
-    const uint64_t reserved = __hip_atomic_load(&ctl->spill_next, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t nrec = reserved < kp.spill_cap ? reserved : kp.spill_cap;
-    if (nrec == 0 || T.n == 0 || kp.spill == nullptr)
-        return;
-    /* partitions: about half an LDS table of groups each */
-    const uint32_t ngroups = kp.kds_out->nitems;
-    uint32_t nparts = (uint32_t)((2ULL * ngroups + T.n - 1) / T.n);
-    const uint32_t maxparts = gridDim.x < 64 ? gridDim.x : 64;
-    nparts = nparts < 1 ? 1 : (nparts > maxparts ? maxparts : nparts);
-    const uint32_t part = blockIdx.x % nparts;
-    const uint32_t kblk = blockIdx.x / nparts;
-    const uint32_t nblk = (gridDim.x - part + nparts - 1) / nparts;
-    const uint64_t lo = nrec * kblk / nblk;
-    const uint64_t hi = nrec * (kblk + 1) / nblk;
     const int rec_words = 1 + P->spill_nregs;
-
     int64_t Rfile[NR];
     Lane L;
     L.R = Rfile;
@@ -2541,10 +2556,202 @@ spill_agg_body(cplan *P, const xpu_kparams &kp)
             gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
         }
     }
+    return lds_miss;
+}
+
+/* partitions of a task's spill: about half an LDS table of groups each */
+__device__ __forceinline__ uint32_t
+spill_nparts(uint32_t ngroups, uint32_t table_n, uint32_t maxparts)
+{
+    uint32_t nparts = (uint32_t)((2ULL * ngroups + table_n - 1) / table_n);
+    return nparts < 1 ? 1 : (nparts > maxparts ? maxparts : nparts);
+}
+
+template <int NR, class PROG>
+__device__ __forceinline__ void
+spill_agg_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    xpu_task_ctl *ctl = kp.ctl;
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n > 0)
+        lds_init(P, T);
+    __syncthreads();
+    const uint64_t reserved = __hip_atomic_load(&ctl->spill_next, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t nrec = reserved < kp.spill_cap ? reserved : kp.spill_cap;
+    if (nrec == 0 || T.n == 0 || kp.spill == nullptr)
+        return;
+    const uint32_t nparts = spill_nparts(kp.kds_out->nitems, T.n,
+                                         gridDim.x < 64 ? gridDim.x : 64);
+    const uint32_t part = blockIdx.x % nparts;
+    const uint32_t kblk = blockIdx.x / nparts;
+    const uint32_t nblk = (gridDim.x - part + nparts - 1) / nparts;
+    const uint64_t lo = nrec * kblk / nblk;
+    const uint64_t hi = nrec * (kblk + 1) / nblk;
+    uint32_t lds_miss = spill_agg_range<NR>(P, kp, T, lo, hi, part, nparts);
     __syncthreads();
     lds_flush<NR>(P, T, kp.kds_out, ctl);
     if (lds_miss)
         atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
+}
+
+/* ------------------------------------------------------------------
+ * batched fragments: many scan + group-by tasks of one plan shape (their
+ * own chunk, kds_final, parameters and control block each) in one
+ * persistent launch
+ *
+ * Pass 1: blocks take work items (a row range of one fragment; items are
+ * ordered by fragment) from a ticket; a block's LDS table holds one
+ * fragment at a time and is merged into that fragment's kds_final when the
+ * block moves on.  Rows without room spill into the fragment's own region
+ * of the spill buffer.  A one-block setup kernel then lists the spill work
+ * (fragment, hash partition, slice of records) and pass 2 takes those items
+ * from a second ticket.
+ * ------------------------------------------------------------------ */
+
+/* the task parameters of fragment f on top of the launch's common ones */
+__device__ __forceinline__ xpu_kparams
+batch_kparams(const xpu_kparams &kp, const xbatch_frag &f)
+{
+    xpu_kparams k = kp;
+    k.kds_src = f.kds_src;
+    k.kds_out = f.kds_final;
+    k.ctl = f.ctl;
+    k.const_pool = f.const_pool;
+    k.params = f.params;
+    k.param_nulls = f.param_nulls;
+    k.spill = f.spill;
+    k.spill_cap = f.spill_cap;
+    k.row_begin = 0;
+    k.row_end = f.nrows;
+    return k;
+}
+
+template <int NR, int RPT, class PROG>
+__device__ __forceinline__ void
+batch_agg_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    __shared__ uint32_t s_item;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwaves = blockDim.x >> 6;
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n > 0)
+        lds_init(P, T);
+    ScanAggWave W = {0, 0, 0};
+    int cur = -1;
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+            s_item = atomicAdd(kp.batch_ticket, 1U);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= kp.batch_nitems)
+            break;
+        const xbatch_item item = kp.batch_items[it];
+        if ((int)item.frag != cur)
+        {
+            if (cur >= 0)
+            {
+                const xbatch_frag &fc = kp.batch_frags[cur];
+                if (T.n > 0)
+                {
+                    lds_flush<NR>(P, T, fc.kds_final, fc.ctl);
+                    __syncthreads();
+                    lds_init(P, T);
+                }
+                scan_agg_report(fc.ctl, W);
+            }
+            __syncthreads();
+            cur = (int)item.frag;
+        }
+        xpu_kparams k = batch_kparams(kp, kp.batch_frags[item.frag]);
+        k.row_begin = item.row_begin;
+        k.row_end = item.row_end;
+        scan_agg_range<NR, RPT, PROG>(P, k, T, wid, nwaves, W);
+        __syncthreads();
+    }
+    if (cur >= 0)
+    {
+        const xbatch_frag &fc = kp.batch_frags[cur];
+        if (T.n > 0)
+            lds_flush<NR>(P, T, fc.kds_final, fc.ctl);
+        scan_agg_report(fc.ctl, W);
+    }
+}
+
+/* lists the spill work of a batch (one thread) */
+__device__ __forceinline__ void
+batch_spill_setup_body(const xpu_kparams &kp)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    uint32_t n = 0;
+    for (uint32_t f = 0; f < kp.batch_nfrags; f++)
+    {
+        const xbatch_frag &fr = kp.batch_frags[f];
+        if (!fr.spill || kp.lds_nents == 0)
+            continue;
+        const uint64_t reserved = __hip_atomic_load(&fr.ctl->spill_next, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t nrec = reserved < fr.spill_cap ? reserved : fr.spill_cap;
+        if (nrec == 0)
+            continue;
+        const uint32_t nparts = spill_nparts(fr.kds_final->nitems, kp.lds_nents, 64);
+        uint64_t nslices = (nrec + XBATCH_SPILL_SLICE - 1) / XBATCH_SPILL_SLICE;
+        if (nslices > 64)
+            nslices = 64;
+        for (uint32_t p = 0; p < nparts; p++)
+            for (uint64_t sl = 0; sl < nslices; sl++)
+            {
+                if (n >= kp.batch_nsitems_cap)
+                    break;
+                xbatch_sitem si;
+                si.frag = f;
+                si.part = (uint16_t)p;
+                si.nparts = (uint16_t)nparts;
+                si.lo = nrec * sl / nslices;
+                si.hi = nrec * (sl + 1) / nslices;
+                kp.batch_sitems[n++] = si;
+            }
+    }
+    kp.batch_ticket[1] = 0;
+    kp.batch_ticket[2] = n;
+}
+
+template <int NR, class PROG>
+__device__ __forceinline__ void
+batch_spill_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    __shared__ uint32_t s_item;
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n == 0)
+        return;
+    const uint32_t nsitems = __hip_atomic_load(&kp.batch_ticket[2], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+            s_item = atomicAdd(&kp.batch_ticket[1], 1U);
+        __syncthreads();
+        const uint32_t it = s_item;
+        __syncthreads();
+        if (it >= nsitems)
+            break;
+        const xbatch_sitem si = kp.batch_sitems[it];
+        const xpu_kparams k = batch_kparams(kp, kp.batch_frags[si.frag]);
+        lds_init(P, T);
+        __syncthreads();
+        uint32_t miss = spill_agg_range<NR>(P, k, T, si.lo, si.hi, si.part, si.nparts);
+        __syncthreads();
+        lds_flush<NR>(P, T, k.kds_out, k.ctl);
+        if (miss)
+            atomicAdd((unsigned long long *)&k.ctl->lds_miss, (unsigned long long)miss);
+        __syncthreads();
+    }
 }
 
 #endif /* XPU_DEVICE_H */

@@ -47,6 +47,42 @@ typedef struct
 #define XMSG_GROUP_RETRY        0xfff3
 #define XMSG_TILE_TOO_LARGE     0xfff4
 
+/* one fragment of a batched launch */
+typedef struct
+{
+    const kern_data_store *kds_src;     /* KDS_FORMAT_ARROW in device memory */
+    kern_data_store *kds_final;         /* the fragment's group-by result */
+    xpu_task_ctl *ctl;                  /* the fragment's control block */
+    const char *const_pool;
+    const int64_t *params;              /* XO_PARAM values */
+    uint64_t   *spill;                  /* the fragment's spill region */
+    uint64_t    spill_cap;              /* records */
+    uint64_t    param_nulls;
+    uint32_t    nrows;
+    uint32_t    __pad;
+} xbatch_frag;
+
+/* a row range of one fragment (pass 1) */
+typedef struct
+{
+    uint32_t    frag;
+    uint32_t    row_begin;
+    uint32_t    row_end;
+    uint32_t    __pad;
+} xbatch_item;
+
+/* a hash partition of a slice of one fragment's spill records (pass 2) */
+typedef struct
+{
+    uint32_t    frag;
+    uint16_t    part;
+    uint16_t    nparts;
+    uint64_t    lo;
+    uint64_t    hi;
+} xbatch_sitem;
+
+#define XBATCH_SPILL_SLICE  (2ULL << 20)    /* spill records per pass-2 item */
+
 typedef struct
 {
     const xplan *plan;
@@ -65,6 +101,18 @@ typedef struct
                                                    each depth (index 1..nrels) */
     uint64_t   *spill;                  /* group-by spill records (nullptr: none) */
     uint64_t    spill_cap;              /* capacity in records */
+    const int64_t *params;              /* XO_PARAM values of this launch */
+    uint64_t    param_nulls;            /* bit i: parameter i is NULL */
+    /* batched launches */
+    const xbatch_frag *batch_frags;
+    const xbatch_item *batch_items;
+    xbatch_sitem *batch_sitems;
+    uint32_t   *batch_ticket;           /* [0] pass-1 ticket, [1] pass-2 ticket,
+                                           [2] pass-2 items */
+    uint32_t    batch_nfrags;
+    uint32_t    batch_nitems;
+    uint32_t    batch_nsitems_cap;
+    uint32_t    __pad2;
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -73,6 +121,11 @@ extern "C" {
 /* host-side launchers (defined in xpu_kernels.hip) */
 int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
                         size_t lds_bytes, bool preagg, void *stream);
+/* the spill setup kernel of a batch alone (for specialized pass kernels) */
+int xpu_launch_batch_setup(const xpu_kparams *kp, void *stream);
+/* batched fragments: pass 1, spill setup, pass 2 (all stream-ordered) */
+int xpu_launch_batch(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
+                     size_t lds_bytes, void *stream);
 /* second pass over the spill records of a scan + group-by launch */
 int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
                          size_t lds_bytes, void *stream);

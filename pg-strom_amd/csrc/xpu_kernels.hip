@@ -26,6 +26,26 @@ xpu_spill_agg_kernel(xpu_kparams kp)
     spill_agg_body<NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
 }
 
+template <int NR, int RPT>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_batch_agg_kernel(xpu_kparams kp)
+{
+    batch_agg_body<NR, RPT, Interp>((cplan *)(uintptr_t)kp.plan, kp);
+}
+
+__global__ void
+xpu_batch_spill_setup_kernel(xpu_kparams kp)
+{
+    batch_spill_setup_body(kp);
+}
+
+template <int NR>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_batch_spill_kernel(xpu_kparams kp)
+{
+    batch_spill_body<NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
+}
+
 /* ------------------------------------------------------------------
  * helper kernels
  * ------------------------------------------------------------------ */
@@ -106,6 +126,38 @@ xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int blo
         launch_scan_agg<XP_NREGS_SMALL>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
     else
         launch_scan_agg<XP_NREGS>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
+    return (int)hipGetLastError();
+}
+
+template <int NR>
+static void
+launch_batch(const xpu_kparams *kp, int rpt, int grid, int block, size_t lds, hipStream_t s)
+{
+    if (rpt >= 4)
+        hipLaunchKernelGGL((xpu_batch_agg_kernel<NR, 4>), dim3(grid), dim3(block), lds, s, *kp);
+    else if (rpt == 2)
+        hipLaunchKernelGGL((xpu_batch_agg_kernel<NR, 2>), dim3(grid), dim3(block), lds, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_batch_agg_kernel<NR, 1>), dim3(grid), dim3(block), lds, s, *kp);
+    hipLaunchKernelGGL(xpu_batch_spill_setup_kernel, dim3(1), dim3(64), 0, s, *kp);
+    hipLaunchKernelGGL((xpu_batch_spill_kernel<NR>), dim3(grid), dim3(block), lds, s, *kp);
+}
+
+extern "C" int
+xpu_launch_batch_setup(const xpu_kparams *kp, void *stream)
+{
+    hipLaunchKernelGGL(xpu_batch_spill_setup_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, *kp);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_batch(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
+                 size_t lds_bytes, void *stream)
+{
+    if (nregs <= XP_NREGS_SMALL)
+        launch_batch<XP_NREGS_SMALL>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
+    else
+        launch_batch<XP_NREGS>(kp, rpt, grid, block, lds_bytes, (hipStream_t)stream);
     return (int)hipGetLastError();
 }
 

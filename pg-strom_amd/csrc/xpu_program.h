@@ -31,6 +31,7 @@
 #define XP_MAX_OPS          2048
 #define XP_MAX_STAGE        6       /* depth-0 columns loaded by the tile prologue */
 #define XP_MAX_SPILL        8       /* registers carried by a group-by spill record */
+#define XP_MAX_PARAMS       64      /* fixed-width query parameters per session */
 
 /* XO_LDCOL imm: low byte = width; XLD_STAGED = the scan kernel's prologue has
  * already put the value into the destination register */
@@ -101,6 +102,9 @@ enum xop_code {
     /* end of the scan quals when the final stage follows in the same pass:
        a lane whose R[a] is not true raises no further errors */
     XO_KILL,
+    /* dst <- query parameter imm of the launch (fixed-width parameters are
+       run-time values, so plans differing only in them share a kernel) */
+    XO_PARAM,
     XO_NUM_OPS
 };
 

@@ -162,16 +162,12 @@ struct xpuhip_session
     kern_data_store *d_kds_final = nullptr;
     uint64_t        final_length = 0;
     char           *d_kmrels = nullptr;
+    int64_t        *d_params = nullptr;     /* XO_PARAM values */
     bool            inner_ready = false;
     xpuhip_tuning   tuning;
     std::mutex      lock;
     std::vector<Scratch *> pool;
     uint64_t        ntasks = 0;
-    /* kernels specialized for the plan (loaded on first use) */
-    hipFunction_t   jit_fn = nullptr;
-    hipFunction_t   jit_spill_fn = nullptr;
-    int             jit_kind = 0;
-    int             jit_param = -1;
 };
 
 namespace {
@@ -485,6 +481,17 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
         return ERRCODE_DEVICE_ERROR;
     }
     s->comp.plan.const_pool = (uint64_t)(uintptr_t)s->d_const_pool;
+    {
+        size_t np = s->comp.params.size() ? s->comp.params.size() : 1;
+        if (!hip_ok(hipMalloc((void **)&s->d_params, 8 * np), eb, "hipMalloc") ||
+            (!s->comp.params.empty() &&
+             !hip_ok(hipMemcpy(s->d_params, s->comp.params.data(), 8 * s->comp.params.size(),
+                               hipMemcpyHostToDevice), eb, "hipMemcpy")))
+        {
+            xpuhip_session_close(s);
+            return ERRCODE_DEVICE_ERROR;
+        }
+    }
     if (!s->comp.kds_dst_head.empty())
     {
         if (!hip_ok(hipMalloc((void **)&s->d_kds_dst_head, s->comp.kds_dst_head.size()), eb, "hipMalloc") ||
@@ -581,20 +588,19 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
 namespace {
 
 /* code objects of specialized kernels loaded into this process's device */
-std::mutex g_mod_lock;
-std::map<const std::vector<char> *, std::pair<hipFunction_t, hipFunction_t>> g_mods;
-
-hipFunction_t
-jit_function(xpuhip_session *s, int kind, int param, hipFunction_t *p_spill, kern_errorbuf *eb)
+struct JitFns
 {
-    {
-        std::lock_guard<std::mutex> g(s->lock);
-        if (s->jit_fn && s->jit_kind == kind && s->jit_param == param)
-        {
-            *p_spill = s->jit_spill_fn;
-            return s->jit_fn;
-        }
-    }
+    hipFunction_t   main = nullptr;         /* scan + group-by or pipeline kernel */
+    hipFunction_t   spill = nullptr;        /* spill pass (scan + group-by) */
+    hipFunction_t   batch = nullptr;        /* batched pass 1 */
+    hipFunction_t   batch_spill = nullptr;  /* batched pass 2 */
+};
+std::mutex g_mod_lock;
+std::map<const std::vector<char> *, JitFns> g_mods;
+
+const JitFns *
+jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
+{
     std::string log;
     const std::vector<char> *code = xpu_jit_code(&s->comp.plan, kind, param, &log);
     if (!code)
@@ -604,35 +610,24 @@ jit_function(xpuhip_session *s, int kind, int param, hipFunction_t *p_spill, ker
         XERR(eb, ERRCODE_DEVICE_ERROR, ("specialized kernel does not compile: " + log).c_str());
         return nullptr;
     }
-    hipFunction_t fn = nullptr, sfn = nullptr;
-    {
-        std::lock_guard<std::mutex> g(g_mod_lock);
-        auto it = g_mods.find(code);
-        if (it != g_mods.end())
-        {
-            fn = it->second.first;
-            sfn = it->second.second;
-        }
-        else
-        {
-            hipModule_t mod;
-            if (!hip_ok(hipModuleLoadData(&mod, code->data()), eb, "hipModuleLoadData") ||
-                !hip_ok(hipModuleGetFunction(&fn, mod, "xpu_jit_kernel"), eb, "hipModuleGetFunction"))
-                return nullptr;
-            if (kind == XJ_SCAN_AGG &&
-                !hip_ok(hipModuleGetFunction(&sfn, mod, "xpu_jit_spill_kernel"), eb,
-                        "hipModuleGetFunction"))
-                return nullptr;
-            g_mods[code] = std::make_pair(fn, sfn);
-        }
-    }
-    std::lock_guard<std::mutex> g(s->lock);
-    s->jit_fn = fn;
-    s->jit_spill_fn = sfn;
-    s->jit_kind = kind;
-    s->jit_param = param;
-    *p_spill = sfn;
-    return fn;
+    std::lock_guard<std::mutex> g(g_mod_lock);
+    auto it = g_mods.find(code);
+    if (it != g_mods.end())
+        return &it->second;
+    hipModule_t mod;
+    JitFns f;
+    if (!hip_ok(hipModuleLoadData(&mod, code->data()), eb, "hipModuleLoadData") ||
+        !hip_ok(hipModuleGetFunction(&f.main, mod, "xpu_jit_kernel"), eb, "hipModuleGetFunction"))
+        return nullptr;
+    if (kind == XJ_SCAN_AGG &&
+        (!hip_ok(hipModuleGetFunction(&f.spill, mod, "xpu_jit_spill_kernel"), eb,
+                 "hipModuleGetFunction") ||
+         !hip_ok(hipModuleGetFunction(&f.batch, mod, "xpu_jit_batch_kernel"), eb,
+                 "hipModuleGetFunction") ||
+         !hip_ok(hipModuleGetFunction(&f.batch_spill, mod, "xpu_jit_batch_spill_kernel"), eb,
+                 "hipModuleGetFunction")))
+        return nullptr;
+    return &(g_mods[code] = f);
 }
 
 }   /* namespace */
@@ -709,6 +704,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     kp.rows_per_tile = rows_per_tile;
     kp.ntiles = ntiles;
     kp.lds_nents = lds_nents;
+    kp.params = s->d_params;
+    kp.param_nulls = s->comp.param_nulls;
     for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
         kp.inner_kds[d] = P->depth[d].inner_kds;
     /* spill buffer of a scan + group-by: room for a quarter of the rows (at
@@ -737,11 +734,14 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     hipFunction_t jfn = nullptr, jsfn = nullptr;
     if (t.jit != XPUHIP_JIT_OFF && (t.jit == XPUHIP_JIT_ON || nrows >= XPUHIP_JIT_AUTO_ROWS))
     {
-        if (scan_agg)
-            jfn = jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), &jsfn, eb);
-        else
-            jfn = jit_function(s, XJ_PIPELINE, preagg ? 1 : 0, &jsfn, eb);
-        if (!jfn && t.jit == XPUHIP_JIT_ON)
+        const JitFns *jf = scan_agg ? jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb)
+                                    : jit_function(s, XJ_PIPELINE, preagg ? 1 : 0, eb);
+        if (jf)
+        {
+            jfn = jf->main;
+            jsfn = jf->spill;
+        }
+        else if (t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
     (void)hipEventRecord(x->ev0, x->stream);
@@ -976,6 +976,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_kds_dst_head) (void)hipFree(s->d_kds_dst_head);
     if (s->d_kds_final) (void)hipFree(s->d_kds_final);
     if (s->d_kmrels) (void)hipFree(s->d_kmrels);
+    if (s->d_params) (void)hipFree(s->d_params);
     delete s;
 }
 
@@ -1066,4 +1067,365 @@ xpuhip_last_task_stats(xpuhip_task_stats *stats)
         return ERRCODE_DEVICE_ERROR;
     *stats = t_last_stats;
     return 0;
+}
+
+/* ------------------------------------------------------------------
+ * session reset and batched launches
+ * ------------------------------------------------------------------ */
+extern "C" int
+xpuhip_session_reset(xpuhip_session *s, kern_errorbuf *eb)
+{
+    if (!s)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no session");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!s->comp.has_groupby)
+        return 0;
+    return init_kds_final(s, eb);
+}
+
+namespace {
+
+/* device buffers of batched launches, reused across calls (one batch runs
+ * at a time in a process); an identical call replays the captured graph */
+struct BatchScratch
+{
+    hipStream_t stream = nullptr;
+    hipEvent_t  ev0 = nullptr, ev1 = nullptr;
+    xbatch_frag *d_frags = nullptr;     size_t frags_cap = 0;
+    xbatch_item *d_items = nullptr;     size_t items_cap = 0;
+    xbatch_sitem *d_sitems = nullptr;   size_t sitems_cap = 0;
+    xpu_task_ctl *d_ctl = nullptr;      size_t ctl_cap = 0;
+    xpu_task_ctl *h_ctl = nullptr;      size_t hctl_cap = 0;
+    uint32_t   *d_ticket = nullptr;     size_t ticket_cap = 0;
+    uint64_t   *d_spill = nullptr;      size_t spill_cap = 0;
+    std::vector<uint64_t> sig;
+    hipGraph_t  graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+std::mutex g_batch_lock;
+BatchScratch g_batch;
+
+#define XBATCH_MAX_SITEMS   16384
+#define XBATCH_SPILL_BYTES  (8ULL << 30)    /* spill buffer of a batch at most */
+
+void
+batch_drop_graph(BatchScratch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->sig.clear();
+}
+
+/* grows a device buffer; a reallocation invalidates the captured graph */
+bool
+batch_grow(BatchScratch *b, char **p, size_t *cap, size_t need, kern_errorbuf *eb)
+{
+    if (need <= *cap)
+        return true;
+    batch_drop_graph(b);
+    return grow(p, cap, need, eb);
+}
+
+bool
+same_shape(const xplan *a, const xplan *b)
+{
+    xplan x, y;
+    memcpy(&x, a, sizeof(xplan));
+    memcpy(&y, b, sizeof(xplan));
+    for (int d = 0; d <= XP_MAX_DEPTH; d++)
+        x.depth[d].inner_kds = y.depth[d].inner_kds = 0;
+    x.const_pool = y.const_pool = 0;
+    if (x.nops != y.nops)
+        return false;
+    size_t n = offsetof(xplan, ops) + sizeof(xop) * x.nops;
+    return memcmp(&x, &y, n) == 0;
+}
+
+}   /* namespace */
+
+extern "C" int
+xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flags,
+                  kern_errorbuf *eb)
+{
+    if (ensure_device(eb) != 0)
+        return ERRCODE_DEVICE_ERROR;
+    if (!tasks || ntasks == 0)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "empty batch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    xpuhip_session *s0 = tasks[0].sess;
+    for (uint32_t f = 0; f < ntasks; f++)
+    {
+        xpuhip_session *s = tasks[f].sess;
+        if (!s || !uses_scan_agg(s) || !tasks[f].d_kds_src)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "batch fragments must be scan + group-by sessions with a chunk");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        if (!same_shape(&s->comp.plan, &s0->comp.plan))
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "batch fragments must share one plan shape");
+            return ERRCODE_DEVICE_ERROR;
+        }
+    }
+    const xplan *P = &s0->comp.plan;
+    const xpuhip_tuning t = s0->tuning;
+    const uint32_t block = t.block_size ? t.block_size : 512;
+    const uint32_t rpt = t.rows_per_thread >= 4 ? 4 : (t.rows_per_thread == 2 ? 2 : 1);
+    const uint32_t grid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
+    const uint32_t lds_nents = P->lds_fast ? t.lds_nents : 0;
+    const size_t lds = lds_bytes_for(P, lds_nents);
+    if (lds > g_dev.lds_per_block)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "group-by LDS table exceeds the LDS of a block");
+        return ERRCODE_DEVICE_ERROR;
+    }
+
+    std::lock_guard<std::mutex> g(g_batch_lock);
+    BatchScratch *b = &g_batch;
+    if (!b->stream &&
+        (!hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), eb, "hipStreamCreate") ||
+         !hip_ok(hipEventCreate(&b->ev0), eb, "hipEventCreate") ||
+         !hip_ok(hipEventCreate(&b->ev1), eb, "hipEventCreate")))
+        return ERRCODE_DEVICE_ERROR;
+
+    /* work items: row ranges of about a quarter of a block's fair share
+     * (multiples of 4096 rows), in fragment order */
+    uint64_t total = 0;
+    for (uint32_t f = 0; f < ntasks; f++)
+        total += tasks[f].nitems;
+    uint64_t item_rows = total / ((uint64_t)grid * 4);
+    item_rows = item_rows < (64ULL << 10) ? (64ULL << 10) : (item_rows > (16ULL << 20) ? (16ULL << 20) : item_rows);
+    item_rows = (item_rows + 4095) & ~4095ULL;
+    std::vector<xbatch_item> items;
+    for (uint32_t f = 0; f < ntasks; f++)
+        for (uint64_t r = 0; r < tasks[f].nitems; r += item_rows)
+        {
+            xbatch_item it;
+            it.frag = f;
+            it.row_begin = (uint32_t)r;
+            it.row_end = (uint32_t)(r + item_rows < tasks[f].nitems ? r + item_rows : tasks[f].nitems);
+            it.__pad = 0;
+            items.push_back(it);
+        }
+
+    /* spill regions: an eighth of each fragment's rows (at least 64K records),
+     * scaled down to the batch's spill budget */
+    const bool spill = P->spill_ok && lds_nents > 0;
+    const uint64_t rec = 8ULL * (1 + P->spill_nregs);
+    std::vector<uint64_t> caps(ntasks, 0);
+    uint64_t spill_recs = 0;
+    if (spill)
+    {
+        for (uint32_t f = 0; f < ntasks; f++)
+        {
+            caps[f] = tasks[f].nitems / 8 > 65536 ? tasks[f].nitems / 8 : 65536;
+            spill_recs += caps[f];
+        }
+        if (spill_recs * rec > XBATCH_SPILL_BYTES)
+        {
+            double k = (double)XBATCH_SPILL_BYTES / (double)(spill_recs * rec);
+            spill_recs = 0;
+            for (uint32_t f = 0; f < ntasks; f++)
+            {
+                caps[f] = (uint64_t)(caps[f] * k);
+                spill_recs += caps[f];
+            }
+        }
+    }
+
+    std::vector<uint64_t> sig;
+    sig.push_back(ntasks);
+    sig.push_back(flags);
+    sig.push_back(((uint64_t)block << 32) | rpt);
+    sig.push_back(((uint64_t)grid << 32) | lds_nents);
+    sig.push_back(t.jit);
+    for (uint32_t f = 0; f < ntasks; f++)
+    {
+        sig.push_back((uint64_t)(uintptr_t)tasks[f].sess);
+        sig.push_back((uint64_t)(uintptr_t)tasks[f].d_kds_src);
+        sig.push_back(tasks[f].nitems);
+    }
+
+    const JitFns *jf = nullptr;
+    if (t.jit != XPUHIP_JIT_OFF && (t.jit == XPUHIP_JIT_ON || total >= XPUHIP_JIT_AUTO_ROWS))
+    {
+        jf = jit_function(s0, XJ_SCAN_AGG, (int)rpt, eb);
+        if (!jf && t.jit == XPUHIP_JIT_ON)
+            return ERRCODE_DEVICE_ERROR;
+    }
+
+    const bool replay = (flags & XPUHIP_BATCH_GRAPH) && b->gexec && b->sig == sig;
+    if (!replay)
+    {
+        if (!batch_grow(b, (char **)&b->d_frags, &b->frags_cap, sizeof(xbatch_frag) * ntasks, eb) ||
+            !batch_grow(b, (char **)&b->d_items, &b->items_cap, sizeof(xbatch_item) * items.size(), eb) ||
+            !batch_grow(b, (char **)&b->d_sitems, &b->sitems_cap, sizeof(xbatch_sitem) * XBATCH_MAX_SITEMS, eb) ||
+            !batch_grow(b, (char **)&b->d_ctl, &b->ctl_cap, sizeof(xpu_task_ctl) * ntasks, eb) ||
+            !batch_grow(b, (char **)&b->d_ticket, &b->ticket_cap, 64, eb) ||
+            (spill && !batch_grow(b, (char **)&b->d_spill, &b->spill_cap, spill_recs * rec, eb)))
+            return ERRCODE_DEVICE_ERROR;
+        if (b->hctl_cap < sizeof(xpu_task_ctl) * ntasks)
+        {
+            if (b->h_ctl)
+                (void)hipHostFree(b->h_ctl);
+            b->h_ctl = nullptr;
+            b->hctl_cap = 0;
+            if (!hip_ok(hipHostMalloc((void **)&b->h_ctl, sizeof(xpu_task_ctl) * ntasks,
+                                      hipHostMallocDefault), eb, "hipHostMalloc"))
+                return ERRCODE_DEVICE_ERROR;
+            b->hctl_cap = sizeof(xpu_task_ctl) * ntasks;
+        }
+        std::vector<xbatch_frag> frags(ntasks);
+        uint64_t off = 0;
+        for (uint32_t f = 0; f < ntasks; f++)
+        {
+            xpuhip_session *s = tasks[f].sess;
+            xbatch_frag &fr = frags[f];
+            memset(&fr, 0, sizeof(fr));
+            fr.kds_src = tasks[f].d_kds_src;
+            fr.kds_final = s->d_kds_final;
+            fr.ctl = b->d_ctl + f;
+            fr.const_pool = s->d_const_pool;
+            fr.params = s->d_params;
+            fr.param_nulls = s->comp.param_nulls;
+            fr.spill = spill ? b->d_spill + off * (rec / 8) : nullptr;
+            fr.spill_cap = caps[f];
+            fr.nrows = tasks[f].nitems;
+            off += caps[f];
+        }
+        if (!hip_ok(hipMemcpy(b->d_frags, frags.data(), sizeof(xbatch_frag) * ntasks,
+                              hipMemcpyHostToDevice), eb, "hipMemcpy") ||
+            !hip_ok(hipMemcpy(b->d_items, items.data(), sizeof(xbatch_item) * items.size(),
+                              hipMemcpyHostToDevice), eb, "hipMemcpy"))
+            return ERRCODE_DEVICE_ERROR;
+        batch_drop_graph(b);
+    }
+
+    xpu_kparams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.plan = s0->d_plan;
+    kp.const_pool = s0->d_const_pool;
+    kp.lds_nents = lds_nents;
+    kp.batch_frags = b->d_frags;
+    kp.batch_items = b->d_items;
+    kp.batch_sitems = b->d_sitems;
+    kp.batch_ticket = b->d_ticket;
+    kp.batch_nfrags = ntasks;
+    kp.batch_nitems = (uint32_t)items.size();
+    kp.batch_nsitems_cap = XBATCH_MAX_SITEMS;
+
+    /* the launch sequence: clear control blocks and tickets, pass 1, spill
+     * setup, pass 2 (no memset nodes: capturable) */
+    auto enqueue = [&]() -> int {
+        int rc = xpu_launch_fill(b->d_ctl, 0, sizeof(xpu_task_ctl) * ntasks / 8, b->stream);
+        if (!rc)
+            rc = xpu_launch_fill(b->d_ticket, 0, 8, b->stream);
+        if (rc)
+            return rc;
+        if (jf)
+        {
+            void *args[] = {&kp};
+            rc = (int)hipModuleLaunchKernel(jf->batch, grid, 1, 1, block, 1, 1, (unsigned)lds,
+                                            b->stream, args, nullptr);
+            if (!rc)
+                rc = xpu_launch_batch_setup(&kp, b->stream);
+            if (!rc)
+                rc = (int)hipModuleLaunchKernel(jf->batch_spill, grid, 1, 1, block, 1, 1,
+                                                (unsigned)lds, b->stream, args, nullptr);
+            return rc;
+        }
+        return xpu_launch_batch(&kp, P->nregs, (int)rpt, grid, block, lds, b->stream);
+    };
+
+    int rc = 0;
+    if (flags & XPUHIP_BATCH_GRAPH)
+    {
+        if (!replay)
+        {
+            if (!hip_ok(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), eb,
+                        "hipStreamBeginCapture"))
+                return ERRCODE_DEVICE_ERROR;
+            rc = enqueue();
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (rc || e != hipSuccess)
+            {
+                if (graph)
+                    (void)hipGraphDestroy(graph);
+                hip_ok(rc ? (hipError_t)rc : e, eb, "batch graph capture");
+                return ERRCODE_DEVICE_ERROR;
+            }
+            if (!hip_ok(hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0), eb,
+                        "hipGraphInstantiate"))
+            {
+                (void)hipGraphDestroy(graph);
+                return ERRCODE_DEVICE_ERROR;
+            }
+            b->graph = graph;
+            b->sig = sig;
+        }
+        (void)hipEventRecord(b->ev0, b->stream);
+        rc = (int)hipGraphLaunch(b->gexec, b->stream);
+        (void)hipEventRecord(b->ev1, b->stream);
+    }
+    else
+    {
+        (void)hipEventRecord(b->ev0, b->stream);
+        rc = enqueue();
+        (void)hipEventRecord(b->ev1, b->stream);
+    }
+    if (rc)
+    {
+        hip_ok((hipError_t)rc, eb, "batch launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    (void)hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(xpu_task_ctl) * ntasks, hipMemcpyDeviceToHost,
+                         b->stream);
+    if (!hip_ok(hipStreamSynchronize(b->stream), eb, "batch"))
+        return ERRCODE_DEVICE_ERROR;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    t_last_kernel_ms = ms;
+    memset(&t_last_stats, 0, sizeof(t_last_stats));
+    t_last_stats.kernel_ms = ms;
+    t_last_stats.pass1_ms = ms;
+    t_last_stats.grid = grid;
+    t_last_stats.block = block;
+    t_last_stats.ntiles = (uint32_t)items.size();
+    t_last_stats.lds_nents = lds_nents;
+    t_last_stats.lds_bytes = lds;
+    t_last_stats.rows = total;
+    t_last_stats.jit = jf ? 1 : 0;
+
+    int first_err = 0;
+    for (uint32_t f = 0; f < ntasks; f++)
+    {
+        const xpu_task_ctl *c = b->h_ctl + f;
+        t_last_stats.lds_miss += c->lds_miss;
+        t_last_stats.global_merges += c->global_merges;
+        t_last_stats.spill_rows += c->spill_rows;
+        if (tasks[f].results)
+        {
+            kern_exec_results *r = tasks[f].results;
+            memset(r, 0, sizeof(*r));
+            r->nitems_raw = tasks[f].nitems;
+            r->nitems_in = c->nitems_in;
+            r->nitems_out = c->nitems_out;
+        }
+        if (c->errcode && !first_err)
+        {
+            char msg[200];
+            snprintf(msg, sizeof(msg), "batch fragment %u: %s", f,
+                     message_of(tasks[f].sess, c->errmsg));
+            XERR(eb, ERRCODE_DEVICE_ERROR, msg);
+            first_err = ERRCODE_DEVICE_ERROR;
+        }
+    }
+    return first_err;
 }

@@ -397,3 +397,42 @@ def parse_row_kds(buf):
         vals, rowid, _ = decode_tupitem(cms, raw, hd.length - o)
         out.append((tuple(vals), rowid))
     return out
+
+
+def window_headers(base, windows, base_offset, stride):
+    """KDS_FORMAT_ARROW headers of row windows over one columnar chunk.
+
+    `base` is a chunk built by build_arrow_kds (fixed-width columns, validity
+    bitmaps allowed when every window starts at a multiple of 8 rows).  Header
+    i, placed `i * stride` bytes after the start of a buffer that holds `base`
+    at `base_offset`, describes rows [r0, r0 + n) of `base` for windows[i] =
+    (r0, n), with its column offsets pointing into `base` (as an Arrow record
+    batch producer hands out slices of one file).  Returns one uint8 array of
+    len(windows) * stride bytes."""
+    hd = abi.kern_data_store_head.from_buffer_copy(bytes(base[:C.sizeof(abi.kern_data_store_head)]))
+    head_len = abi.kds_head_length(hd.ncols)
+    assert head_len <= stride and stride % 64 == 0
+    out = np.zeros(len(windows) * stride, dtype=np.uint8)
+    proto = np.array(base[:head_len], dtype=np.uint8)
+    for i, (r0, n) in enumerate(windows):
+        h0 = i * stride
+        out[h0:h0 + head_len] = proto
+        h = abi.kern_data_store_head.from_buffer(out, h0)
+        h.nitems = n
+        h.length = base_offset + hd.length - h0
+        shift = base_offset - h0
+        for j in range(hd.ncols):
+            cm = abi.kern_colmeta.from_buffer(out, h0 + abi.KDS_COLMETA_OFFSET +
+                                              j * C.sizeof(abi.kern_colmeta))
+            w = cm.attopts.unitsz
+            if w <= 0 or cm.extra_offset:
+                raise ValueError("window_headers: fixed-width columns only")
+            cm.values_offset = shift + cm.values_offset + r0 * w
+            cm.values_length = n * w
+            if cm.nullmap_offset:
+                if r0 % 8:
+                    raise ValueError("window_headers: windows over NULL-bearing columns "
+                                     "must start at a multiple of 8 rows")
+                cm.nullmap_offset = shift + cm.nullmap_offset + r0 // 8
+                cm.nullmap_length = (n + 7) // 8
+    return out

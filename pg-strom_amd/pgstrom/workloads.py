@@ -64,7 +64,10 @@ def fragment_query(cols, k, m, final_nrows=None):
     plan.scan_quals = cg.Func("int4lt", a, plan.param("int4", int(k)))
     kmod = cg.Func("int4mod", key, plan.param("int4", int(m)))
     plan.set_groupby(preagg_tlist(plan, kmod, v, w, x))
-    return plan.build(final_nrows=final_nrows or max(int(m), 100))
+    nrows = final_nrows or max(int(m), 100)
+    # kds_final sized for the groups (at most m, and at most the 10,000 keys
+    # of the generator): many fragment sessions stay resident at once
+    return plan.build(final_nrows=nrows, final_length=(4 << 20) + 400 * min(nrows, 20000))
 
 
 def dim_filter_mod(dim_index):

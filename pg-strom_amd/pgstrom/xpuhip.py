@@ -19,8 +19,9 @@ EXPORTS = [
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_memcpy_dtod", "xpuhip_device_synchronize",
     "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
-    "xpuhip_last_task_stats", "xpuhip_jit_prepare",
+    "xpuhip_last_task_stats", "xpuhip_jit_prepare", "xpuhip_batch_exec", "xpuhip_session_reset",
 ]
+BATCH_GRAPH = 1
 
 JIT_AUTO, JIT_ON, JIT_OFF = 0, 1, 2
 
@@ -36,6 +37,11 @@ class xpuhip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("rows_per_thread", C.c_uint32),
                 ("lds_nents", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("jit", C.c_uint32)]
+
+
+class xpuhip_batch_task(C.Structure):
+    _fields_ = [("sess", C.c_void_p), ("d_kds_src", C.c_void_p), ("nitems", C.c_uint32),
+                ("_pad", C.c_uint32), ("results", C.c_void_p)]
 
 
 class xpuhip_task_stats(C.Structure):
@@ -79,6 +85,8 @@ def lib():
         L.xpuhip_session_explain.argtypes = [vp, sz, vp, sz, vp]
         L.xpuhip_last_task_stats.argtypes = [vp]
         L.xpuhip_jit_prepare.argtypes = [vp, sz, vp]
+        L.xpuhip_batch_exec.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
+        L.xpuhip_session_reset.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -209,6 +217,12 @@ class Session:
             raise XpuError(rc, self.err)
         return rc, res, pos.value
 
+    def reset(self):
+        """Clears the session's kds_final."""
+        rc = lib().xpuhip_session_reset(self.h, C.byref(self.err))
+        if rc:
+            raise XpuError(rc, self.err)
+
     def final(self):
         need = lib().xpuhip_task_final(self.h, None, 0, None, C.byref(self.err))
         if need < 0:
@@ -226,6 +240,23 @@ class Session:
 
     def __del__(self):
         self.close()
+
+
+def batch_exec(tasks, graph=True):
+    """One launch over many (Session, DeviceBuffer, nitems) fragments of one
+    plan shape; returns the per-fragment kern_exec_results."""
+    arr = (xpuhip_batch_task * len(tasks))()
+    results = [abi.kern_exec_results() for _ in tasks]
+    for i, (sess, dbuf, n) in enumerate(tasks):
+        arr[i].sess = sess.h
+        arr[i].d_kds_src = dbuf if isinstance(dbuf, int) else dbuf.ptr
+        arr[i].nitems = n
+        arr[i].results = C.addressof(results[i])
+    err = abi.kern_errorbuf()
+    rc = lib().xpuhip_batch_exec(arr, len(tasks), BATCH_GRAPH if graph else 0, C.byref(err))
+    if rc:
+        raise XpuError(rc, err)
+    return results
 
 
 def last_kernel_ms():

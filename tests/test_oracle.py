@@ -113,3 +113,30 @@ def test_oracle_error_planted():
     rc, _, _ = o.exec(kds.build_arrow_kds(cols), dst_length=1 << 20)
     assert rc == abi.ERRCODE_DEVICE_ERROR
     assert b"division by zero" in o.err.message
+
+
+def test_window_headers_match_sliced_chunks():
+    """Window headers over one columnar chunk describe the same rows as
+    chunks built from the sliced columns (checked through the oracle)."""
+    n = 5000
+    cols = datagen.preagg_table(n)
+    cols[2] = kds.Column("w", "float8", cols[2].values, nulls=np.arange(n) % 7 == 0)
+    base = kds.build_arrow_kds(cols)
+    windows = [(0, 10), (64, 1000), (4096, 904), (8, 0)]
+    stride = 4096
+    hdrs = kds.window_headers(base, windows, len(windows) * stride, stride)
+    buf = np.concatenate([hdrs, base])
+    blob = W.preagg_query(cols)
+    acts = [a for a, _, _ in W.preagg_tlist(None, cols[0], None, None, None)]
+    for i, (r0, m) in enumerate(windows):
+        o = oracle.OracleSession(blob)
+        o.exec(buf[i * stride:], nthreads=2)
+        got = W.decode_groups(o.final(), acts)
+        o.close()
+        sl = [kds.Column(c.name, c.typname, c.values[r0:r0 + m],
+                         nulls=None if c.nulls is None else c.nulls[r0:r0 + m]) for c in cols]
+        o = oracle.OracleSession(blob)
+        o.exec(kds.build_arrow_kds(sl), nthreads=2)
+        exp = W.decode_groups(o.final(), acts)
+        o.close()
+        assert got == exp
