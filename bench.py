@@ -55,6 +55,8 @@ def bench_preagg(n, warmup, steps):
     d_src = xpuhip.DeviceBuffer(len(src), src)
     del src
     s = xpuhip.Session(blob)
+    log("autotuning")
+    s.autotune(d_src, n)
 
     def step():
         rc, res, _ = s.exec_device(d_src, n)
@@ -175,7 +177,9 @@ def bench_batch(nfrag, base_rows, warmup, steps):
     def batched():
         xpuhip.batch_exec(tasks, graph=True)
 
-    out = {}
+    log("autotuning the batch")
+    tuned = xpuhip.batch_autotune(tasks)
+    out = {"tuning": tuned}
     for name, fn in (("plain", plain), ("batched", batched)):
         for s in sessions:
             s.reset()

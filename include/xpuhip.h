@@ -131,9 +131,73 @@ typedef struct
 int     xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flags,
                           kern_errorbuf *errbuf);
 
+/* Autotuner: runs the session's task on the caller's chunk (twice per
+ * candidate: the first run compiles and warms) under each candidate launch
+ * shape of its task kind - block size, rows per lane, LDS table entries,
+ * blocks per CU - and keeps the fastest by kernel time.  The winner also
+ * becomes the starting shape of later sessions of the same plan shape in the
+ * process.  kds_final is reset afterwards.  d_dst/dst_length: the kds_dst of
+ * a projection (NULL for group-by).  No counterpart in the reference. */
+int     xpuhip_session_autotune(xpuhip_session *sess, const kern_data_store *d_kds_src,
+                                uint32_t src_nitems, kern_data_store *d_kds_dst,
+                                size_t dst_length, xpuhip_tuning *best, kern_errorbuf *errbuf);
+/* The same over a whole batch (xpuhip_batch_exec without graph capture per
+ * trial); every fragment's session gets the winner and is reset. */
+int     xpuhip_batch_autotune(const xpuhip_batch_task *tasks, uint32_t ntasks,
+                              xpuhip_tuning *best, kern_errorbuf *errbuf);
+
 /* Clears a group-by session's kds_final (the session can run again from
  * scratch, e.g. after tuning trials). */
 int     xpuhip_session_reset(xpuhip_session *sess, kern_errorbuf *errbuf);
+
+/* ------------------------------------------------------------------
+ * The GPU service as a request loop (continuous batching).  No counterpart
+ * in the reference, whose service runs one task per worker thread
+ * (gpu_service.c:5912-5993): requests from many backends are admitted in
+ * arrival order as in-flight slots and the HBM budget allow, queued
+ * group-by fragments of one plan shape are coalesced into one batched
+ * launch, and admitted work overlaps on the device.  slots = 1 and
+ * max_batch = 1 give the one-query-at-a-time service.
+ * ------------------------------------------------------------------ */
+typedef struct xpuhip_server xpuhip_server;
+
+typedef struct
+{
+    uint64_t    hbm_budget;         /* bytes of requests admitted at once */
+    uint32_t    slots;              /* jobs in flight (worker threads) */
+    uint32_t    max_batch;          /* fragments coalesced into one launch */
+} xpuhip_server_config;
+
+#define XPUHIP_REQ_BATCHABLE    1   /* scan + group-by fragment (xpuhip_batch_exec) */
+
+typedef struct
+{
+    xpuhip_session *sess;           /* the backend's session (kept open across requests) */
+    const kern_data_store *d_kds_src;
+    uint32_t    nitems;
+    uint32_t    flags;              /* XPUHIP_REQ_* */
+    kern_data_store *d_kds_dst;     /* projection output (device), or NULL */
+    size_t      dst_length;
+    uint64_t    hbm_bytes;          /* memory the request holds while admitted */
+    uint64_t    shape;              /* requests coalesce only with equal shapes */
+} xpuhip_request;
+
+typedef struct
+{
+    int         rc;
+    uint32_t    batched;            /* fragments of the launch it ran in (1: alone) */
+    double      queue_ms;           /* submit -> admitted */
+    double      exec_ms;            /* admitted -> done */
+    double      total_ms;           /* submit -> done */
+    kern_exec_results results;
+    kern_errorbuf errbuf;
+} xpuhip_request_status;
+
+xpuhip_server *xpuhip_server_create(const xpuhip_server_config *cfg, kern_errorbuf *errbuf);
+uint64_t xpuhip_server_submit(xpuhip_server *srv, const xpuhip_request *req);   /* ticket */
+int     xpuhip_server_wait(xpuhip_server *srv, uint64_t ticket, xpuhip_request_status *status);
+void    xpuhip_server_stats(xpuhip_server *srv, uint64_t *launches, uint64_t *coalesced);
+void    xpuhip_server_destroy(xpuhip_server *srv);      /* drains, then stops */
 
 /* Launch tuning of one session (see xpuhip_tuning). */
 int     xpuhip_session_set_tuning(xpuhip_session *sess, const xpuhip_tuning *tuning);

@@ -153,6 +153,8 @@ grow(char **p, size_t *cap, size_t need, kern_errorbuf *eb)
 
 }   /* namespace */
 
+bool xpuhip_tuned_shape(const xplan *P, xpuhip_tuning *t);
+
 struct xpuhip_session
 {
     xpu_compiled    comp;
@@ -465,6 +467,12 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
         return rc;
     }
     default_tuning(s, &s->tuning);
+    {
+        /* a shape tuned earlier in this process */
+        xpuhip_tuning tt;
+        if (xpuhip_tuned_shape(&s->comp.plan, &tt))
+            s->tuning = tt;
+    }
     /* plan and constants */
     size_t pool_len = s->comp.const_pool.size() ? s->comp.const_pool.size() : 8;
     if (!hip_ok(hipMalloc((void **)&s->d_const_pool, pool_len), eb, "hipMalloc") ||
@@ -1428,4 +1436,180 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
         }
     }
     return first_err;
+}
+
+/* ------------------------------------------------------------------
+ * autotuner: launch shapes tried on the caller's own data
+ * ------------------------------------------------------------------ */
+namespace {
+
+/* tuned shapes by plan shape (plan bytes without session addresses), so
+ * that later sessions of the same shape start tuned */
+std::mutex g_tune_lock;
+std::map<std::string, xpuhip_tuning> g_tuned;
+
+std::string
+shape_key(const xplan *P)
+{
+    xplan x;
+    memcpy(&x, P, sizeof(xplan));
+    for (int d = 0; d <= XP_MAX_DEPTH; d++)
+        x.depth[d].inner_kds = 0;
+    x.const_pool = 0;
+    return std::string((const char *)&x, offsetof(xplan, ops) + sizeof(xop) * x.nops);
+}
+
+/* candidate launch shapes for a session's task kind */
+std::vector<xpuhip_tuning>
+tuning_candidates(const xpuhip_session *s)
+{
+    const xplan *P = &s->comp.plan;
+    std::vector<xpuhip_tuning> c;
+    auto add = [&](uint32_t block, uint32_t rpt, uint32_t bpc, double lds_frac) {
+        xpuhip_tuning t;
+        memset(&t, 0, sizeof(t));
+        t.block_size = block;
+        t.rows_per_thread = rpt;
+        t.blocks_per_cu = bpc;
+        t.jit = s->tuning.jit;
+        size_t budget = g_dev.lds_per_block > 4096 ? g_dev.lds_per_block - 4096 : 0;
+        t.lds_nents = lds_nents_within(P, (size_t)(budget * lds_frac / bpc));
+        if (P->lds_fast && t.lds_nents == 0)
+            return;
+        c.push_back(t);
+    };
+    if (uses_scan_agg(s))
+    {
+        add(512, 4, 1, 1.0);
+        add(512, 2, 1, 1.0);
+        add(256, 4, 1, 1.0);
+        add(512, 4, 1, 0.75);
+        add(512, 4, 2, 1.0);
+        add(256, 4, 2, 1.0);
+    }
+    else if (s->comp.has_groupby)
+    {
+        add(512, 4, 2, 0.3);
+        add(256, 4, 2, 0.3);
+        add(512, 4, 1, 0.3);
+        add(256, 4, 4, 0.3);
+        add(512, 2, 2, 0.3);
+    }
+    else
+    {
+        add(512, 1, 8, 0.0);
+        add(256, 1, 8, 0.0);
+        add(512, 1, 4, 0.0);
+        add(256, 1, 4, 0.0);
+    }
+    return c;
+}
+
+}   /* namespace */
+
+extern "C" int
+xpuhip_session_autotune(xpuhip_session *s, const kern_data_store *d_src, uint32_t nitems,
+                        kern_data_store *d_dst, size_t dst_length, xpuhip_tuning *best,
+                        kern_errorbuf *eb)
+{
+    if (!s || !d_src)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "autotune needs a session and a chunk");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    std::vector<xpuhip_tuning> cands = tuning_candidates(s);
+    xpuhip_tuning keep = s->tuning, win = s->tuning;
+    double win_ms = 1e30;
+    for (const xpuhip_tuning &t : cands)
+    {
+        if (xpuhip_session_set_tuning(s, &t) != 0)
+            continue;
+        double ms = 1e30;
+        for (int rep = 0; rep < 2; rep++)      /* the first run compiles / warms */
+        {
+            uint64_t pos = 0;
+            int rc = xpuhip_task_exec_device(s, d_src, nitems, d_dst, dst_length, &pos, nullptr, eb);
+            if (rc == ERRCODE_SUSPEND_NO_SPACE)
+                rc = 0;         /* a projection that overflows still measures */
+            if (rc != 0)
+            {
+                s->tuning = keep;
+                xpuhip_session_reset(s, eb);
+                return rc;
+            }
+            ms = t_last_kernel_ms;
+        }
+        if (ms < win_ms)
+        {
+            win_ms = ms;
+            win = s->tuning;
+        }
+    }
+    s->tuning = win;
+    {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        g_tuned[shape_key(&s->comp.plan)] = win;
+    }
+    if (best)
+        *best = win;
+    return xpuhip_session_reset(s, eb);
+}
+
+extern "C" int
+xpuhip_batch_autotune(const xpuhip_batch_task *tasks, uint32_t ntasks, xpuhip_tuning *best,
+                      kern_errorbuf *eb)
+{
+    if (!tasks || ntasks == 0 || !tasks[0].sess)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "empty batch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    xpuhip_session *s0 = tasks[0].sess;
+    std::vector<xpuhip_tuning> cands = tuning_candidates(s0);
+    xpuhip_tuning win = s0->tuning;
+    double win_ms = 1e30;
+    for (const xpuhip_tuning &t : cands)
+    {
+        if (xpuhip_session_set_tuning(s0, &t) != 0)
+            continue;
+        double ms = 1e30;
+        for (int rep = 0; rep < 2; rep++)
+        {
+            int rc = xpuhip_batch_exec(tasks, ntasks, 0, eb);
+            if (rc != 0)
+                return rc;
+            ms = t_last_kernel_ms;
+        }
+        if (ms < win_ms)
+        {
+            win_ms = ms;
+            win = s0->tuning;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> g(g_tune_lock);
+        g_tuned[shape_key(&s0->comp.plan)] = win;
+    }
+    for (uint32_t f = 0; f < ntasks; f++)
+    {
+        tasks[f].sess->tuning = win;
+        int rc = xpuhip_session_reset(tasks[f].sess, eb);
+        if (rc)
+            return rc;
+    }
+    if (best)
+        *best = win;
+    return 0;
+}
+
+/* the tuned shape of an earlier session of the same plan, if any */
+bool
+xpuhip_tuned_shape(const xplan *P, xpuhip_tuning *t)
+{
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    auto it = g_tuned.find(shape_key(P));
+    if (it == g_tuned.end())
+        return false;
+    *t = it->second;
+    return true;
 }

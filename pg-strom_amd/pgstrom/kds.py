@@ -402,8 +402,9 @@ def parse_row_kds(buf):
 def window_headers(base, windows, base_offset, stride):
     """KDS_FORMAT_ARROW headers of row windows over one columnar chunk.
 
-    `base` is a chunk built by build_arrow_kds (fixed-width columns, validity
-    bitmaps allowed when every window starts at a multiple of 8 rows).  Header
+    `base` is a chunk built by build_arrow_kds (fixed-width and utf8/binary
+    columns; validity bitmaps need every window to start at a multiple of 8
+    rows).  Header
     i, placed `i * stride` bytes after the start of a buffer that holds `base`
     at `base_offset`, describes rows [r0, r0 + n) of `base` for windows[i] =
     (r0, n), with its column offsets pointing into `base` (as an Arrow record
@@ -424,11 +425,18 @@ def window_headers(base, windows, base_offset, stride):
         for j in range(hd.ncols):
             cm = abi.kern_colmeta.from_buffer(out, h0 + abi.KDS_COLMETA_OFFSET +
                                               j * C.sizeof(abi.kern_colmeta))
-            w = cm.attopts.unitsz
-            if w <= 0 or cm.extra_offset:
-                raise ValueError("window_headers: fixed-width columns only")
-            cm.values_offset = shift + cm.values_offset + r0 * w
-            cm.values_length = n * w
+            if cm.extra_offset:
+                # utf8 / binary: the window's slice of the u32 offsets; the
+                # offsets stay absolute into the shared extra buffer
+                cm.values_offset = shift + cm.values_offset + r0 * 4
+                cm.values_length = (n + 1) * 4
+                cm.extra_offset = shift + cm.extra_offset
+            else:
+                w = cm.attopts.unitsz
+                if w <= 0:
+                    raise ValueError("window_headers: column without a fixed width")
+                cm.values_offset = shift + cm.values_offset + r0 * w
+                cm.values_length = n * w
             if cm.nullmap_offset:
                 if r0 % 8:
                     raise ValueError("window_headers: windows over NULL-bearing columns "

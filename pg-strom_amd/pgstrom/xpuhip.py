@@ -20,7 +20,11 @@ EXPORTS = [
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_memcpy_dtod", "xpuhip_device_synchronize",
     "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
     "xpuhip_last_task_stats", "xpuhip_jit_prepare", "xpuhip_batch_exec", "xpuhip_session_reset",
+    "xpuhip_session_autotune", "xpuhip_batch_autotune",
+    "xpuhip_server_create", "xpuhip_server_submit", "xpuhip_server_wait", "xpuhip_server_stats",
+    "xpuhip_server_destroy",
 ]
+REQ_BATCHABLE = 1
 BATCH_GRAPH = 1
 
 JIT_AUTO, JIT_ON, JIT_OFF = 0, 1, 2
@@ -42,6 +46,22 @@ class xpuhip_tuning(C.Structure):
 class xpuhip_batch_task(C.Structure):
     _fields_ = [("sess", C.c_void_p), ("d_kds_src", C.c_void_p), ("nitems", C.c_uint32),
                 ("_pad", C.c_uint32), ("results", C.c_void_p)]
+
+
+class xpuhip_server_config(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("max_batch", C.c_uint32)]
+
+
+class xpuhip_request(C.Structure):
+    _fields_ = [("sess", C.c_void_p), ("d_kds_src", C.c_void_p), ("nitems", C.c_uint32),
+                ("flags", C.c_uint32), ("d_kds_dst", C.c_void_p), ("dst_length", C.c_size_t),
+                ("hbm_bytes", C.c_uint64), ("shape", C.c_uint64)]
+
+
+class xpuhip_request_status(C.Structure):
+    _fields_ = [("rc", C.c_int), ("batched", C.c_uint32), ("queue_ms", C.c_double),
+                ("exec_ms", C.c_double), ("total_ms", C.c_double),
+                ("results", abi.kern_exec_results), ("errbuf", abi.kern_errorbuf)]
 
 
 class xpuhip_task_stats(C.Structure):
@@ -87,6 +107,15 @@ def lib():
         L.xpuhip_jit_prepare.argtypes = [vp, sz, vp]
         L.xpuhip_batch_exec.argtypes = [vp, C.c_uint32, C.c_uint32, vp]
         L.xpuhip_session_reset.argtypes = [vp, vp]
+        L.xpuhip_session_autotune.argtypes = [vp, vp, C.c_uint32, vp, sz, vp, vp]
+        L.xpuhip_batch_autotune.argtypes = [vp, C.c_uint32, vp, vp]
+        L.xpuhip_server_create.restype = vp
+        L.xpuhip_server_create.argtypes = [vp, vp]
+        L.xpuhip_server_submit.restype = C.c_uint64
+        L.xpuhip_server_submit.argtypes = [vp, vp]
+        L.xpuhip_server_wait.argtypes = [vp, C.c_uint64, vp]
+        L.xpuhip_server_stats.argtypes = [vp, vp, vp]
+        L.xpuhip_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -217,6 +246,15 @@ class Session:
             raise XpuError(rc, self.err)
         return rc, res, pos.value
 
+    def autotune(self, d_src, nitems, d_dst=None, dst_length=0):
+        """Tries the candidate launch shapes on this chunk; keeps the fastest."""
+        t = xpuhip_tuning()
+        rc = lib().xpuhip_session_autotune(self.h, d_src.ptr, nitems, d_dst.ptr if d_dst else None,
+                                           dst_length, C.byref(t), C.byref(self.err))
+        if rc:
+            raise XpuError(rc, self.err)
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
     def reset(self):
         """Clears the session's kds_final."""
         rc = lib().xpuhip_session_reset(self.h, C.byref(self.err))
@@ -257,6 +295,59 @@ def batch_exec(tasks, graph=True):
     if rc:
         raise XpuError(rc, err)
     return results
+
+
+class Server:
+    """The request loop of the GPU service (include/xpuhip.h, xpuhip_server_*):
+    slots = 1, max_batch = 1 is the one-query-at-a-time service."""
+
+    def __init__(self, hbm_budget, slots, max_batch):
+        cfg = xpuhip_server_config(hbm_budget, slots, max_batch)
+        err = abi.kern_errorbuf()
+        self.h = lib().xpuhip_server_create(C.byref(cfg), C.byref(err))
+        if not self.h:
+            raise XpuError(abi.ERRCODE_DEVICE_ERROR, err)
+
+    def submit(self, sess, d_src, nitems, d_dst=None, dst_length=0, hbm_bytes=0, shape=0,
+               batchable=False):
+        r = xpuhip_request(sess.h, d_src if isinstance(d_src, int) else d_src.ptr, nitems,
+                           REQ_BATCHABLE if batchable else 0,
+                           (d_dst if isinstance(d_dst, int) else d_dst.ptr) if d_dst else None,
+                           dst_length, hbm_bytes, shape)
+        return lib().xpuhip_server_submit(self.h, C.byref(r))
+
+    def wait(self, ticket):
+        st = xpuhip_request_status()
+        if lib().xpuhip_server_wait(self.h, ticket, C.byref(st)):
+            raise KeyError(ticket)
+        return st
+
+    def stats(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        lib().xpuhip_server_stats(self.h, C.byref(a), C.byref(b))
+        return {"launches": a.value, "coalesced": b.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xpuhip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def batch_autotune(tasks):
+    arr = (xpuhip_batch_task * len(tasks))()
+    for i, (sess, dbuf, n) in enumerate(tasks):
+        arr[i].sess = sess.h
+        arr[i].d_kds_src = dbuf if isinstance(dbuf, int) else dbuf.ptr
+        arr[i].nitems = n
+    t = xpuhip_tuning()
+    err = abi.kern_errorbuf()
+    rc = lib().xpuhip_batch_autotune(arr, len(tasks), C.byref(t), C.byref(err))
+    if rc:
+        raise XpuError(rc, err)
+    return {k: getattr(t, k) for k, _ in t._fields_}
 
 
 def last_kernel_ms():

@@ -84,3 +84,17 @@ def test_batch_rejects_mixed_shapes(frags):
     assert b"one plan shape" in ei.value.errbuf.message
     s1.close()
     s2.close()
+
+
+def test_batch_autotune_then_exec(frags):
+    """The autotuner tries launch shapes over the batch, leaves every
+    kds_final empty, and the tuned batch gives the oracle's groups."""
+    sessions = [xpuhip.Session(blob) for _, blob, _, _ in frags]
+    tasks = [(s, d, n) for s, (n, _, _, d) in zip(sessions, frags)]
+    best = xpuhip.batch_autotune(tasks)
+    assert best["block_size"] in (256, 512)
+    xpuhip.batch_exec(tasks, graph=True)
+    for s, (n, blob, src, _) in zip(sessions, frags):
+        exp = run_oracle_groupby(blob, [src])
+        assert_groups_equal(s.final(), exp, ACTIONS)
+        s.close()

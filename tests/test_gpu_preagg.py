@@ -211,3 +211,24 @@ def test_spill_overflow(gpu, jit):
     exp = run_oracle_groupby(blob, [src])
     groups = assert_groups_equal(got, exp, C1_ACTIONS)
     assert sum(g[0] for g in groups.values()) == n
+
+
+def test_session_autotune(gpu):
+    """Autotuning on the caller's chunk resets kds_final; the tuned session
+    gives the oracle's groups."""
+    from pgstrom import xpuhip
+    n = 1_000_000
+    cols = datagen.preagg_table(n, fragment=16)
+    src = kds.build_arrow_kds(cols)
+    blob = W.preagg_query(cols)
+    d = xpuhip.DeviceBuffer(len(src), src)
+    s = xpuhip.Session(blob)
+    best = s.autotune(d, n)
+    assert best["rows_per_thread"] in (1, 2, 4)
+    rc, _, _ = s.exec_device(d, n)
+    assert rc == 0
+    got = s.final()
+    s.close()
+    d.free()
+    exp = run_oracle_groupby(blob, [src])
+    assert_groups_equal(got, exp, C1_ACTIONS)
