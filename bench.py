@@ -201,6 +201,53 @@ def bench_batch(nfrag, base_rows, warmup, steps):
     return out
 
 
+def bench_served(nreq, nback, min_rows, max_rows, table_rows):
+    """Served form: open-loop Poisson arrivals of nreq requests from nback
+    backends (60% scans, 25% pre-aggs, 15% star joins over row windows of
+    log-uniform size of resident tables; dims resident), through the request
+    loop with 8 slots and coalesced group-by fragments, against the same
+    requests one at a time.  Loads: 0.8x and 3x the one-at-a-time capacity."""
+    from pgstrom import serving, xpuhip
+    log("served: generating resident tables (%d rows each)" % table_rows)
+    res = serving.Resident(scan_rows=table_rows // 2, preagg_rows=table_rows, fact_rows=table_rows,
+                           dim_sizes=(2_000_000, 200_000, 2_556), nwin=nreq + 8)
+    reqs = serving.make_requests(res, nreq, min_rows, max_rows)
+    rows = sum(q["n"] for q in reqs)
+    dsts = [xpuhip.DeviceBuffer(serving.dst_bytes(q["n"])) if q["kind"] == "scan" else None
+            for q in reqs]
+    backends = serving.Backends(res, nback)
+
+    def form(slots, max_batch, arrivals):
+        srv = xpuhip.Server(hbm_budget=200 << 30, slots=slots, max_batch=max_batch)
+        st, span = serving.run(srv, backends, reqs, arrivals, dsts)
+        stats = srv.stats()
+        srv.close()
+        assert all(x.rc == 0 for x in st)
+        lat = np.array([x.total_ms for x in st])
+        return dict(rows_per_s=rows / span, req_per_s=len(reqs) / span, makespan_s=span,
+                    p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                    launches=stats["launches"], coalesced=stats["coalesced"])
+
+    # warm: specialised kernels of every kind compile once
+    form(8, 16, [0.0] * len(reqs))
+    cap = form(1, 1, [0.0] * len(reqs))
+    mu = cap["req_per_s"]
+    rng = np.random.default_rng(20261202 + 5)
+    out = {"requests": nreq, "backends": nback, "rows": rows, "plain_capacity_req_per_s": mu}
+    for load in (0.8, 3.0):
+        gaps = rng.exponential(1.0 / (load * mu), size=len(reqs))
+        arr = list(np.cumsum(gaps) - gaps[0])
+        out["load_%.1fx" % load] = {"plain": form(1, 1, arr), "served": form(8, 16, arr)}
+    for d in dsts:
+        if d is not None:
+            d.free()
+    backends.close()
+    res.free()
+    hi = out["load_3.0x"]
+    out["served_vs_plain_rows_per_s_at_3x"] = hi["served"]["rows_per_s"] / hi["plain"]["rows_per_s"]
+    return out
+
+
 def traffic_of(name):
     """HBM bytes per launch of the dominant kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (null if absent)."""
@@ -252,7 +299,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--only", default="all", choices=["all", "preagg", "batch"])
+    ap.add_argument("--only", default="all", choices=["all", "preagg", "batch", "served"])
+    ap.add_argument("--served-requests", type=int, default=300)
+    ap.add_argument("--served-table-rows", type=int, default=50_000_000)
     ap.add_argument("--batch-frags", type=int, default=256)
     ap.add_argument("--batch-base-rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
@@ -274,6 +323,10 @@ def main():
     if args.only == "batch":
         print(json.dumps(bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)))
         return
+    if args.only == "served":
+        print(json.dumps(bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
+                                      args.served_table_rows)))
+        return
     pa = bench_preagg(args.rows, args.warmup, args.steps)
     kms = pa["kernel_ms"]
     # dominant kernel: the scan + group-by pass, which reads every column byte
@@ -288,6 +341,8 @@ def main():
                                  frac=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / HBM_PEAK,
                                  rows_out=sc["nout"])
         others["batched_256"] = bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)
+        others["served"] = bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
+                                        args.served_table_rows)
         st = bench_star(200_000_000, 1, 5)
         others["star_join_200M"] = dict(rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3),
                                         kernel_ms=st["kernel_ms"], wall_ms=st["wall_ms"],

@@ -1675,10 +1675,20 @@ lb_pack(uint64_t flag, uint64_t rows, uint64_t bytes)
 enum { PH_SCAN = 0, PH_HASH, PH_JOIN, PH_OJ, PH_EMIT };
 #define IT_END      1U          /* chain exhausted (offsets are 8-aligned, never 1) */
 
-template <bool PREAGG, int NR, class PROG>
+/* how a projection launch finds its tiles' output offsets */
+enum
+{
+    PM_LOOKBACK = 0,    /* one kernel: decoupled look-back over tile status words */
+    PM_COUNT = 1,       /* count pass: tile totals only, no output */
+    PM_WRITE = 2,       /* write pass: offsets from the scanned tile totals */
+};
+
+template <bool PREAGG, int NR, class PROG, int PM = PM_LOOKBACK>
 __device__ __forceinline__ void
 pipeline_body(cplan *P, const xpu_kparams &kp)
 {
+    /* statistics are counted once: by the count pass of a two-pass projection */
+    constexpr bool COUNTS = (PM != PM_WRITE);
     extern __shared__ unsigned long long lds_dyn[];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_stop;
@@ -1708,15 +1718,32 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
         s_depth_out[tid] = 0;
     __syncthreads();
 
+    /* the write pass stops at the first tile that does not fit kds_dst */
+    uint32_t stop_tile = kp.ntiles;
+    if (PM == PM_WRITE)
+    {
+        uint32_t rt = __hip_atomic_load(&ctl->resume_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (rt)
+            stop_tile = 0xffffffffU - rt;
+    }
+    uint32_t k_iter = 0;
     for (;;)
     {
         if (tid == 0)
         {
             uint32_t err = __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_tile = err ? kp.ntiles : atomicAdd(&ctl->tile_ticket, 1U);
+            if (PM == PM_LOOKBACK)
+                s_tile = err ? kp.ntiles : atomicAdd(&ctl->tile_ticket, 1U);
+            else
+            {
+                /* no inter-tile dependency: static assignment */
+                uint64_t t = (uint64_t)blockIdx.x + (uint64_t)k_iter * gridDim.x;
+                s_tile = (err || t >= stop_tile) ? kp.ntiles : (uint32_t)t;
+            }
         }
         __syncthreads();
         const uint32_t tile = s_tile;
+        k_iter++;
         if (tile >= kp.ntiles)
             break;
 
@@ -1750,8 +1777,9 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
             for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
                 it[i] = 0;
 
-            /* a scan-only projection keeps its registers from pass 0 */
-            if (!PREAGG && pass == 1 && nrels == 0)
+            /* a scan-only projection keeps its registers from pass 0; the
+             * count pass of a two-pass projection writes nothing */
+            if (!PREAGG && pass == 1 && (nrels == 0 || PM == PM_COUNT))
                 break;
 
             int phase = PH_SCAN;
@@ -1811,7 +1839,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                 if (phase == PH_SCAN)
                 {
                     bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
-                    if (ok && pass == 0)
+                    if (ok && pass == 0 && COUNTS)
                         cnt_in++;
                     cd = ok ? 1 : 0;
                     if (nrels > 0)
@@ -1856,7 +1884,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                             matched |= 1U << d;
                         if (status > 0)
                         {
-                            if (pass == 0)
+                            if (pass == 0 && COUNTS)
                                 atomicAdd(&s_depth_out[d], 1U);
                             cd = d + 1;
                             if (d + 1 <= nrels)
@@ -1873,7 +1901,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                         matched |= 1U << d;
                         if (!lane_err && L.R[P->r_status] > 0)
                         {
-                            if (pass == 0)
+                            if (pass == 0 && COUNTS)
                                 atomicAdd(&s_depth_out[d], 1U);
                             cd = d + 1;
                             if (d + 1 <= nrels)
@@ -1906,7 +1934,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                 {
                     if (act && !lane_err)
                     {
-                        if (pass == 0)
+                        if (pass == 0 && COUNTS)
                             cnt_out++;
                         if (PREAGG)
                         {
@@ -1997,7 +2025,20 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                     uint64_t er = 0, eb = 0;
                     bool stop = false;
                     uint64_t *st = kp.tile_status;
-                    if (tile > 0)
+                    if (PM == PM_COUNT)
+                    {
+                        /* tile totals, scanned by xpu_tile_scan_kernel */
+                        st[2ULL * tile] = tc;
+                        st[2ULL * tile + 1] = tb;
+                        stop = true;
+                    }
+                    else if (PM == PM_WRITE)
+                    {
+                        er = st[2ULL * tile];
+                        eb = st[2ULL * tile + 1];
+                        stop = (tile >= stop_tile);
+                    }
+                    else if (tile > 0)
                     {
                         __hip_atomic_store(&st[tile], lb_pack(XLB_AGG, tc, tb),
                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -2032,7 +2073,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                         }
                     }
                     kern_data_store *dst = kp.kds_out;
-                    if (!stop &&
+                    if (PM == PM_LOOKBACK && !stop &&
                         KDS_HEAD_LENGTH(dst) + 8 * (er + tc) + eb + tb > dst->length)
                     {
                         stop = true;
@@ -2043,20 +2084,28 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                         if (tile == 0)
                             record_error(ctl, XMSG_TILE_TOO_LARGE, ERRCODE_DEVICE_ERROR);
                     }
-                    if (stop || __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT) != 0)
+                    if (PM == PM_LOOKBACK)
                     {
+                        if (stop || __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) != 0)
+                        {
+                            stop = true;
+                            __hip_atomic_store(&st[tile], lb_pack(XLB_STOP, 0, 0),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                        else
+                        {
+                            __hip_atomic_store(&st[tile], lb_pack(XLB_PREFIX, er + tc, eb + tb),
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            atomicMax((unsigned long long *)&ctl->out_nitems,
+                                      (unsigned long long)(er + tc));
+                            atomicMax((unsigned long long *)&ctl->out_usage,
+                                      (unsigned long long)(eb + tb));
+                        }
+                    }
+                    else if (__hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) != 0)
                         stop = true;
-                        __hip_atomic_store(&st[tile], lb_pack(XLB_STOP, 0, 0),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    }
-                    else
-                    {
-                        __hip_atomic_store(&st[tile], lb_pack(XLB_PREFIX, er + tc, eb + tb),
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        atomicMax((unsigned long long *)&ctl->out_nitems, (unsigned long long)(er + tc));
-                        atomicMax((unsigned long long *)&ctl->out_usage, (unsigned long long)(eb + tb));
-                    }
                     s_base_rows = er;
                     s_base_bytes = eb;
                     s_stop = stop;
@@ -2751,6 +2800,84 @@ batch_spill_body(cplan *P, const xpu_kparams &kp)
         if (miss)
             atomicAdd((unsigned long long *)&k.ctl->lds_miss, (unsigned long long)miss);
         __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------
+ * the scan between the count and write passes of a projection: exclusive
+ * prefixes of the tile totals (rows, bytes) in place, and the first tile
+ * that does not fit kds_dst as the resume point (one block)
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ void
+tile_scan_body(uint64_t *st, uint32_t ntiles, const kern_data_store *dst, xpu_task_ctl *ctl)
+{
+    __shared__ uint64_t s_wr[XP_BLOCK_MAX / 64], s_wb[XP_BLOCK_MAX / 64];
+    __shared__ uint64_t s_carry_r, s_carry_b;
+    __shared__ uint32_t s_stop;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int wid = tid >> 6;
+    if (tid == 0)
+    {
+        s_carry_r = 0;
+        s_carry_b = 0;
+        s_stop = ntiles;
+    }
+    __syncthreads();
+    const uint64_t head = KDS_HEAD_LENGTH(dst);
+    const uint64_t len = dst->length;
+    for (uint32_t base = 0; base < ntiles; base += blockDim.x)
+    {
+        const uint32_t t = base + tid;
+        const uint64_t r = t < ntiles ? st[2ULL * t] : 0;
+        const uint64_t b = t < ntiles ? st[2ULL * t + 1] : 0;
+        const uint64_t ir = wave_incl_scan_u64(r);
+        const uint64_t ib = wave_incl_scan_u64(b);
+        if (lane == 63)
+        {
+            s_wr[wid] = ir;
+            s_wb[wid] = ib;
+        }
+        __syncthreads();
+        uint64_t wr = s_carry_r, wb = s_carry_b;
+        for (int w = 0; w < wid; w++)
+        {
+            wr += s_wr[w];
+            wb += s_wb[w];
+        }
+        const uint64_t er = wr + ir - r;
+        const uint64_t eb = wb + ib - b;
+        if (t < ntiles)
+        {
+            st[2ULL * t] = er;
+            st[2ULL * t + 1] = eb;
+            if (head + 8 * (er + r) + eb + b > len)
+                atomicMin(&s_stop, t);
+        }
+        __syncthreads();
+        if (tid == (int)blockDim.x - 1)
+        {
+            s_carry_r = er + r;
+            s_carry_b = eb + b;
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        const uint32_t stop = s_stop;
+        if (stop < ntiles)
+        {
+            ctl->resume_tile = 0xffffffffU - stop;
+            if (stop == 0)
+                record_error(ctl, XMSG_TILE_TOO_LARGE, ERRCODE_DEVICE_ERROR);
+            ctl->out_nitems = st[2ULL * stop];
+            ctl->out_usage = st[2ULL * stop + 1];
+        }
+        else
+        {
+            ctl->out_nitems = s_carry_r;
+            ctl->out_usage = s_carry_b;
+        }
     }
 }
 

@@ -127,6 +127,15 @@ xpu_jit_source(const xplan *P, int kind, int param)
         append(s, "    pipeline_body<%s, %d, JitProg>((cplan *)xjit_plan, kp);\n",
                param ? "true" : "false", nr);
     s += "}\n";
+    if (kind == XJ_PIPELINE && !param)
+    {
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_count_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    pipeline_body<false, %d, JitProg, PM_COUNT>((cplan *)xjit_plan, kp);\n}\n", nr);
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_write_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    pipeline_body<false, %d, JitProg, PM_WRITE>((cplan *)xjit_plan, kp);\n}\n", nr);
+    }
     if (kind == XJ_SCAN_AGG)
     {
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"

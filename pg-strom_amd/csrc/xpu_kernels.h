@@ -121,6 +121,9 @@ extern "C" {
 /* host-side launchers (defined in xpu_kernels.hip) */
 int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
                         size_t lds_bytes, bool preagg, void *stream);
+/* projection in two passes: count (tile totals), tile scan, write */
+int xpu_launch_projection(const xpu_kparams *kp, int nregs, int grid, int block, void *stream);
+int xpu_launch_tile_scan(const xpu_kparams *kp, void *stream);
 /* the spill setup kernel of a batch alone (for specialized pass kernels) */
 int xpu_launch_batch_setup(const xpu_kparams *kp, void *stream);
 /* batched fragments: pass 1, spill setup, pass 2 (all stream-ordered) */

@@ -12,6 +12,19 @@ xpu_pipeline_kernel(xpu_kparams kp)
     pipeline_body<PREAGG, NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
 }
 
+template <int NR, int PM>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_projection_kernel(xpu_kparams kp)
+{
+    pipeline_body<false, NR, Interp, PM>((cplan *)(uintptr_t)kp.plan, kp);
+}
+
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_tile_scan_kernel(uint64_t *st, uint32_t ntiles, const kern_data_store *dst, xpu_task_ctl *ctl)
+{
+    tile_scan_body(st, ntiles, dst, ctl);
+}
+
 template <int NR, int RPT>
 __global__ void __launch_bounds__(XP_BLOCK_MAX)
 xpu_scan_agg_kernel(xpu_kparams kp)
@@ -203,6 +216,36 @@ xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream)
     hipLaunchKernelGGL(xpu_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        (uint4 *)dst, (const uint4 *)src, nwords, (char *)dst + nwords * 16,
                        (const char *)src + nwords * 16, ntail);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_tile_scan(const xpu_kparams *kp, void *stream)
+{
+    hipLaunchKernelGGL(xpu_tile_scan_kernel, dim3(1), dim3(XP_BLOCK_MAX), 0, (hipStream_t)stream,
+                       kp->tile_status, kp->ntiles, (const kern_data_store *)kp->kds_out, kp->ctl);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_projection(const xpu_kparams *kp, int nregs, int grid, int block, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nregs <= XP_NREGS_SMALL)
+        hipLaunchKernelGGL((xpu_projection_kernel<XP_NREGS_SMALL, PM_COUNT>), dim3(grid), dim3(block),
+                           0, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_projection_kernel<XP_NREGS, PM_COUNT>), dim3(grid), dim3(block),
+                           0, s, *kp);
+    int rc = xpu_launch_tile_scan(kp, stream);
+    if (rc)
+        return rc;
+    if (nregs <= XP_NREGS_SMALL)
+        hipLaunchKernelGGL((xpu_projection_kernel<XP_NREGS_SMALL, PM_WRITE>), dim3(grid), dim3(block),
+                           0, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_projection_kernel<XP_NREGS, PM_WRITE>), dim3(grid), dim3(block),
+                           0, s, *kp);
     return (int)hipGetLastError();
 }
 

@@ -602,6 +602,8 @@ struct JitFns
     hipFunction_t   spill = nullptr;        /* spill pass (scan + group-by) */
     hipFunction_t   batch = nullptr;        /* batched pass 1 */
     hipFunction_t   batch_spill = nullptr;  /* batched pass 2 */
+    hipFunction_t   count = nullptr;        /* projection count pass */
+    hipFunction_t   write = nullptr;        /* projection write pass */
 };
 std::mutex g_mod_lock;
 std::map<const std::vector<char> *, JitFns> g_mods;
@@ -633,6 +635,12 @@ jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
          !hip_ok(hipModuleGetFunction(&f.batch, mod, "xpu_jit_batch_kernel"), eb,
                  "hipModuleGetFunction") ||
          !hip_ok(hipModuleGetFunction(&f.batch_spill, mod, "xpu_jit_batch_spill_kernel"), eb,
+                 "hipModuleGetFunction")))
+        return nullptr;
+    if (kind == XJ_PIPELINE && !param &&
+        (!hip_ok(hipModuleGetFunction(&f.count, mod, "xpu_jit_count_kernel"), eb,
+                 "hipModuleGetFunction") ||
+         !hip_ok(hipModuleGetFunction(&f.write, mod, "xpu_jit_write_kernel"), eb,
                  "hipModuleGetFunction")))
         return nullptr;
     return &(g_mods[code] = f);
@@ -696,7 +704,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         XERR(eb, ERRCODE_DEVICE_ERROR, "group-by LDS table exceeds the LDS of a block");
         return ERRCODE_DEVICE_ERROR;
     }
-    if (!preagg && !grow((char **)&x->d_status, &x->status_cap, 8ULL * ntiles, eb))
+    if (!preagg && !grow((char **)&x->d_status, &x->status_cap, 16ULL * ntiles, eb))
         return ERRCODE_DEVICE_ERROR;
 
     xpu_kparams kp;
@@ -736,10 +744,10 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         /* kds_dst header first, so the kernel sees the capacity */
         xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
                                 dst_length, x->d_ctl, x->stream);
-        xpu_launch_fill(x->d_status, 0, ntiles, x->stream);
+        xpu_launch_fill(x->d_status, 0, 2ULL * ntiles, x->stream);
     }
     /* the kernel specialized for the plan, or the interpreter */
-    hipFunction_t jfn = nullptr, jsfn = nullptr;
+    hipFunction_t jfn = nullptr, jsfn = nullptr, jcount = nullptr, jwrite = nullptr;
     if (t.jit != XPUHIP_JIT_OFF && (t.jit == XPUHIP_JIT_ON || nrows >= XPUHIP_JIT_AUTO_ROWS))
     {
         const JitFns *jf = scan_agg ? jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb)
@@ -748,12 +756,32 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         {
             jfn = jf->main;
             jsfn = jf->spill;
+            jcount = jf->count;
+            jwrite = jf->write;
         }
         else if (t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
     (void)hipEventRecord(x->ev0, x->stream);
-    if (jfn)
+    if (!preagg)
+    {
+        /* projection: count pass (tile totals), tile scan, write pass; no
+         * tile waits on another */
+        if (jcount && jwrite)
+        {
+            void *args[] = {&kp};
+            rc = (int)hipModuleLaunchKernel(jcount, grid, 1, 1, block, 1, 1, 0, x->stream,
+                                            args, nullptr);
+            if (!rc)
+                rc = xpu_launch_tile_scan(&kp, x->stream);
+            if (!rc)
+                rc = (int)hipModuleLaunchKernel(jwrite, grid, 1, 1, block, 1, 1, 0, x->stream,
+                                                args, nullptr);
+        }
+        else
+            rc = xpu_launch_projection(&kp, P->nregs, grid, block, x->stream);
+    }
+    else if (jfn)
     {
         void *args[] = {&kp};
         rc = (int)hipModuleLaunchKernel(jfn, grid, 1, 1, block, 1, 1, (unsigned)lds, x->stream,
