@@ -201,7 +201,7 @@ def bench_batch(nfrag, base_rows, warmup, steps):
     return out
 
 
-def bench_served(nreq, nback, min_rows, max_rows, table_rows):
+def bench_served(nreq, nback, min_rows, max_rows, table_rows, slots=8, max_batch=32):
     """Served form: open-loop Poisson arrivals of nreq requests from nback
     backends (60% scans, 25% pre-aggs, 15% star joins over row windows of
     log-uniform size of resident tables; dims resident), through the request
@@ -224,20 +224,27 @@ def bench_served(nreq, nback, min_rows, max_rows, table_rows):
         srv.close()
         assert all(x.rc == 0 for x in st)
         lat = np.array([x.total_ms for x in st])
+        per_kind = {}
+        for kind in serving.KINDS:
+            ex = [x.exec_ms for x, q in zip(st, reqs) if q["kind"] == kind]
+            rk = sum(q["n"] for q in reqs if q["kind"] == kind)
+            per_kind[kind] = dict(requests=len(ex), exec_ms_mean=float(np.mean(ex)) if ex else 0.0,
+                                  rows=rk)
         return dict(rows_per_s=rows / span, req_per_s=len(reqs) / span, makespan_s=span,
                     p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                    launches=stats["launches"], coalesced=stats["coalesced"])
+                    launches=stats["launches"], coalesced=stats["coalesced"], per_kind=per_kind)
 
     # warm: specialised kernels of every kind compile once
-    form(8, 16, [0.0] * len(reqs))
+    form(slots, max_batch, [0.0] * len(reqs))
     cap = form(1, 1, [0.0] * len(reqs))
     mu = cap["req_per_s"]
     rng = np.random.default_rng(20261202 + 5)
-    out = {"requests": nreq, "backends": nback, "rows": rows, "plain_capacity_req_per_s": mu}
+    out = {"requests": nreq, "backends": nback, "rows": rows, "plain_capacity_req_per_s": mu,
+           "slots": slots, "max_batch": max_batch}
     for load in (0.8, 3.0):
         gaps = rng.exponential(1.0 / (load * mu), size=len(reqs))
         arr = list(np.cumsum(gaps) - gaps[0])
-        out["load_%.1fx" % load] = {"plain": form(1, 1, arr), "served": form(8, 16, arr)}
+        out["load_%.1fx" % load] = {"plain": form(1, 1, arr), "served": form(slots, max_batch, arr)}
     for d in dsts:
         if d is not None:
             d.free()
@@ -302,6 +309,8 @@ def main():
     ap.add_argument("--only", default="all", choices=["all", "preagg", "batch", "served"])
     ap.add_argument("--served-requests", type=int, default=300)
     ap.add_argument("--served-table-rows", type=int, default=50_000_000)
+    ap.add_argument("--served-slots", type=int, default=8)
+    ap.add_argument("--served-max-batch", type=int, default=32)
     ap.add_argument("--batch-frags", type=int, default=256)
     ap.add_argument("--batch-base-rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=20_000_000,
@@ -325,7 +334,8 @@ def main():
         return
     if args.only == "served":
         print(json.dumps(bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
-                                      args.served_table_rows)))
+                                      args.served_table_rows, args.served_slots,
+                                      args.served_max_batch)))
         return
     pa = bench_preagg(args.rows, args.warmup, args.steps)
     kms = pa["kernel_ms"]
