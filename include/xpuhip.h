@@ -36,11 +36,20 @@ typedef struct
     uint32_t    lds_nents;          /* group-by LDS table entries */
     uint32_t    blocks_per_cu;      /* persistent blocks per compute unit */
     uint32_t    jit;                /* XPUHIP_JIT_*: kernel specialized for the plan */
+    uint32_t    join_filter;        /* XPUHIP_JOIN_FILTER_*: inner-relation Bloom filters */
 } xpuhip_tuning;
 
 #define XPUHIP_JIT_AUTO     0       /* specialize chunks of >= 1,000,000 rows */
 #define XPUHIP_JIT_ON       1       /* always specialize */
 #define XPUHIP_JIT_OFF      2       /* always run the op interpreter */
+
+/* Joins probe a Bloom filter of each inner relation's join hashes (built by
+ * xpuhip_session_attach_inner) before walking its hash chain, and a group-by
+ * over joins screens its rows with the depth-1 filter before the joins.
+ * Results are the same either way. */
+#define XPUHIP_JOIN_FILTER_AUTO 0   /* on */
+#define XPUHIP_JOIN_FILTER_ON   1
+#define XPUHIP_JOIN_FILTER_OFF  2
 
 /* Device setup; replaces gpuservSetupGpuContext (gpu_service.c:6462-6523).
  * device < 0 selects device 0. */

@@ -35,7 +35,6 @@
 
 #include "xpu_kernels.h"
 
-#define XP_BLOCK_MAX        512
 
 /* The plan and the headers of the source and inner KDS are read-only while
  * a kernel runs.  Reading them through the constant address space lets the
@@ -1327,6 +1326,64 @@ hashitem_at(kern_data_store *kds, uint64_t off)
     return (kern_hashitem *)((char *)kds + kds->length - off);
 }
 
+/* ------------------------------------------------------------------
+ * join-hash Bloom filter of one inner relation
+ *
+ * Register-blocked (Putze, Sanders, Singler, "Cache-, Hash- and
+ * Space-Efficient Bloom Filters", WEA 2007): a key selects one 64-bit word
+ * and sets four bits in it, so a probe is one 8-byte load.  The key is the
+ * 32-bit join hash that the bucket walk compares (kern_hashitem.t.hash):
+ * every hash that can find an item has its bits set, so a clear bit proves
+ * that the walk would end without a candidate.  The mixing function is the
+ * 64-bit finalizer of MurmurHash3 (Appleby, public domain).
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ uint64_t
+bloom_mix(uint32_t h)
+{
+    uint64_t x = (uint64_t)h;
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+/* word index from the low bits (mask < 2^32), bit positions from bits 40..63 */
+__device__ __forceinline__ uint64_t
+bloom_bits(uint64_t x)
+{
+    return (1ULL << ((x >> 40) & 63)) | (1ULL << ((x >> 46) & 63)) |
+           (1ULL << ((x >> 52) & 63)) | (1ULL << (x >> 58));
+}
+
+__device__ __forceinline__ bool
+bloom_maybe(const uint64_t *__restrict__ words, uint32_t mask, uint32_t h)
+{
+    uint64_t x = bloom_mix(h);
+    uint64_t m = bloom_bits(x);
+    return (words[x & mask] & m) == m;
+}
+
+/* sets the bits of every item reachable from the hash slots [s0, s1) */
+__device__ __forceinline__ void
+bloom_build_range(kern_data_store *ik, uint64_t *words, uint32_t mask, uint64_t s0, uint64_t s1)
+{
+    const uint64_t *slots = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
+    for (uint64_t s = s0; s < s1; s++)
+    {
+        uint64_t off = slots[s];
+        uint64_t steps = (uint64_t)ik->nitems + 1;     /* as many as a probe inspects */
+        while (off != 0 && off != ~0ULL && off < ik->length && steps-- > 0)
+        {
+            const kern_hashitem *hi = hashitem_at(ik, off);
+            uint64_t x = bloom_mix(hi->t.hash);
+            atomicOr((unsigned long long *)&words[x & mask], (unsigned long long)bloom_bits(x));
+            off = hi->next;
+        }
+    }
+}
+
 /* Find-or-insert the group of the key registers of every pending lane and
  * merge its contribution; a wave-level retry loop in which a lane either
  * finds the group, or takes the slot lock once and inserts it, so lanes of
@@ -1624,7 +1681,7 @@ lds_init(cplan *P, LdsTable &T)
 /* merges every entry of the block's LDS table into kds_final (after a
  * barrier that ends all updates of the table) */
 template <int NR>
-__device__ void
+__device__ __forceinline__ void
 lds_flush(cplan *P, LdsTable &T, kern_data_store *kds_final, xpu_task_ctl *ctl)
 {
     int64_t Rfile[NR];
@@ -1661,6 +1718,217 @@ lds_flush(cplan *P, LdsTable &T, kern_data_store *kds_final, xpu_task_ctl *ctl)
 }
 
 /* ------------------------------------------------------------------
+ * partial-state dumps: instead of merging its LDS entries into kds_final
+ * with device atomics (about nine per entry), a block writes them as
+ * records, hash-partitioned, and one block per partition later folds a
+ * partition's records in LDS and merges each group once.  A record is the
+ * LDS entry: packed key, u64 fields, row count, u32 fields.
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t
+dump_partition(uint64_t pk, uint32_t nparts)
+{
+    uint64_t h = pk * 0x9FB21C651E98DF25ULL;
+    return (uint32_t)(((h >> 32) * (uint64_t)nparts) >> 32);
+}
+
+__device__ __forceinline__ int
+dump_rec_words(cplan *P)
+{
+    return 1 + P->lds_n64 + (1 + P->lds_n32 + 1) / 2;
+}
+
+/* writes the block's table as partial-state records (entries beyond a
+ * partition's capacity merge into kds_final directly) */
+template <int NR>
+__device__ void
+lds_dump(cplan *P, LdsTable &T, const xpu_kparams &kp)
+{
+    __shared__ uint32_t s_cnt[XP_DUMP_MAXPARTS];
+    __shared__ uint64_t s_base[XP_DUMP_MAXPARTS];
+    const uint32_t np = kp.dump_nparts;
+    const int rw = dump_rec_words(P);
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+        s_cnt[i] = 0;
+    __syncthreads();
+    for (uint32_t h = threadIdx.x; h < T.n; h += blockDim.x)
+    {
+        unsigned long long key = T.keys[h];
+        if (key != XLDS_EMPTY_KEY && T.rows[h] != 0)
+            atomicAdd(&s_cnt[dump_partition(key, np)], 1U);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x)
+    {
+        uint32_t c = s_cnt[i];
+        s_base[i] = c ? atomicAdd((unsigned long long *)&kp.dump_next[i], (unsigned long long)c) : 0;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
+    int64_t Rfile[NR];
+    Lane L;
+    L.R = Rfile;
+    L.nmask = ~1U;
+    L.err = 0;
+    L.dead = 0;
+#pragma unroll
+    for (int i = 0; i < NR; i++)
+        L.R[i] = 0;
+    L.R[XP_REG_TRUE] = 1;
+    uint32_t ndirect = 0;
+    for (uint32_t base = 0; base < T.n; base += blockDim.x)
+    {
+        const uint32_t h = base + threadIdx.x;
+        bool pending = false;
+        uint32_t hash = 0;
+        if (h < T.n)
+        {
+            unsigned long long key = T.keys[h];
+            if (key != XLDS_EMPTY_KEY && T.rows[h] != 0)
+            {
+                const uint32_t part = dump_partition(key, np);
+                const uint64_t idx = s_base[part] + atomicAdd(&s_cnt[part], 1U);
+                if (idx < kp.dump_cap)
+                {
+                    uint64_t *rec = kp.dump + ((uint64_t)part * kp.dump_cap + idx) * rw;
+                    rec[0] = key;
+                    for (int f = 0; f < (int)P->lds_n64; f++)
+                        rec[1 + f] = T.f64[(uint64_t)f * T.n + h];
+                    uint32_t *r32 = (uint32_t *)(rec + 1 + P->lds_n64);
+                    r32[0] = T.rows[h];
+                    for (int f = 0; f < (int)P->lds_n32; f++)
+                        r32[1 + f] = T.f32[(uint64_t)f * T.n + h];
+                }
+                else
+                {
+                    gb_unpack_key(P, L, key);
+                    hash = gb_hash_regs(P, L);
+                    pending = true;
+                    ndirect++;
+                }
+            }
+        }
+        gb_global(P, L, kp.kds_out, hash, pending, &T, pending ? (int)h : -1, kp.ctl);
+    }
+    if (ndirect)
+        atomicAdd((unsigned long long *)&kp.ctl->global_merges, (unsigned long long)ndirect);
+}
+
+/* folds a partial-state record into LDS entry slot */
+__device__ __forceinline__ void
+lds_merge_entry(cplan *P, LdsTable &T, int slot, const uint64_t *rec)
+{
+    const uint32_t *r32 = (const uint32_t *)(rec + 1 + P->lds_n64);
+    atomicAdd(&T.rows[slot], r32[0]);
+    for (int j = 0; j < (int)P->gb_nattrs; j++)
+    {
+        xagg ag = P->aggs[j];
+        if (ag.action == KAGG_ACTION__VREF)
+            continue;
+        unsigned long long *F64 = T.f64 + (uint64_t)ag.lds64 * T.n + slot;
+        uint32_t *F32 = T.f32 + (uint64_t)ag.lds32 * T.n + slot;
+        const uint64_t v64 = ag.lds64 >= 0 ? rec[1 + ag.lds64] : 0;
+        const uint32_t v32 = ag.lds32 >= 0 ? r32[1 + ag.lds32] : 0;
+        switch (ag.action)
+        {
+            case KAGG_ACTION__PMIN_INT32:
+                atomicMin((int *)F32, (int)v32);
+                break;
+            case KAGG_ACTION__PMAX_INT32:
+                atomicMax((int *)F32, (int)v32);
+                break;
+            case KAGG_ACTION__PMIN_INT64:
+                atomicMin((long long *)F64, (long long)v64);
+                break;
+            case KAGG_ACTION__PMAX_INT64:
+                atomicMax((long long *)F64, (long long)v64);
+                break;
+            case KAGG_ACTION__PMIN_FP64:
+                atomicMin((double *)F64, __longlong_as_double((long long)v64));
+                break;
+            case KAGG_ACTION__PMAX_FP64:
+                atomicMax((double *)F64, __longlong_as_double((long long)v64));
+                break;
+            case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
+                atomicAdd(F64, (unsigned long long)v64);
+                break;
+            case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            {
+                unsigned long long old = atomicAdd(F64, (unsigned long long)v64);
+                int hi = (int)v32 + ((old + v64 < old) ? 1 : 0);
+                if (hi)
+                    atomicAdd((int *)F32, hi);
+                break;
+            }
+            case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
+                atomicAdd((double *)F64, __longlong_as_double((long long)v64));
+                break;
+            case KAGG_ACTION__STDDEV:
+                atomicAdd((double *)F64, __longlong_as_double((long long)v64));
+                atomicAdd((double *)(F64 + T.n), __longlong_as_double((long long)rec[2 + ag.lds64]));
+                break;
+        }
+    }
+}
+
+/* one block per partition: fold the partition's records in LDS, merge each
+ * group into kds_final (a table that fills up is merged and restarted) */
+template <int NR, class PROG>
+__device__ __forceinline__ void
+dump_merge_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    LdsTable T = lds_table_at(P, lds_dyn, kp.lds_nents);
+    if (T.n == 0 || kp.dump == nullptr)
+        return;
+    const int rw = dump_rec_words(P);
+    for (uint32_t part = blockIdx.x; part < kp.dump_nparts; part += gridDim.x)
+    {
+        lds_init(P, T);
+        __syncthreads();
+        const uint64_t reserved = __hip_atomic_load(&kp.dump_next[part], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t n = reserved < kp.dump_cap ? reserved : kp.dump_cap;
+        const uint64_t *recs = kp.dump + (uint64_t)part * kp.dump_cap * rw;
+        for (uint64_t i0 = 0; i0 < n; i0 += blockDim.x)
+        {
+            const uint64_t i = i0 + threadIdx.x;
+            const uint64_t *rec = recs + i * rw;
+            bool fail = false;
+            if (i < n)
+            {
+                int slot = lds_probe(T, rec[0]);
+                if (slot >= 0)
+                    lds_merge_entry(P, T, slot, rec);
+                else
+                    fail = true;
+            }
+            while (__syncthreads_or(fail))
+            {
+                /* the table is full: merge it, start over, then fold the
+                 * records that found no room (each round places at least
+                 * one of them in the emptied table) */
+                lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+                __syncthreads();
+                lds_init(P, T);
+                __syncthreads();
+                if (fail)
+                {
+                    int slot = lds_probe(T, rec[0]);
+                    if (slot >= 0)
+                    {
+                        lds_merge_entry(P, T, slot, rec);
+                        fail = false;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------
  * decoupled look-back over tiles (thread 0 of the block)
  * ------------------------------------------------------------------ */
 __device__ __forceinline__ uint64_t
@@ -1683,6 +1951,24 @@ enum
     PM_WRITE = 2,       /* write pass: offsets from the scanned tile totals */
 };
 
+
+/*
+ * The pipeline kernel: scan quals, the joins of every depth, then either the
+ * group-by (PREAGG) or the projection into kds_dst.
+ *
+ * A lane runs one outer row through a small state machine of phases (scan,
+ * then per depth: bucket walk, join quals, outer-join fill) whose segments
+ * run only for the lanes they apply to; the wave loops until every lane has
+ * exhausted its bucket chains.
+ *
+ * Group-by joins first screen the rows: the scan quals (and, with
+ * kp.prefilter, the depth-1 key hash) run for every row, and a row goes on,
+ * through a per-wave LDS queue, only if it passes the quals and its hash may
+ * be in the depth-1 relation (its Bloom filter, kp.bloom[1]).  The joins then
+ * run over 64 queued rows at a time, so a wave's bucket walks are not spent
+ * on rows that the quals or a selective first join reject.  Rows of an outer
+ * join at depth 1 are not screened by the filter.
+ */
 template <bool PREAGG, int NR, class PROG, int PM = PM_LOOKBACK>
 __device__ __forceinline__ void
 pipeline_body(cplan *P, const xpu_kparams &kp)
@@ -1697,6 +1983,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
     __shared__ uint64_t s_base_rows;
     __shared__ uint64_t s_base_bytes;
     __shared__ uint32_t s_depth_out[XP_MAX_DEPTH + 1];
+    __shared__ uint32_t s_queue[PREAGG ? (XP_BLOCK_MAX / 64) * XQ_WAVE : 1];
 
     const kern_data_store *__restrict__ kds = kp.kds_src;
     xpu_task_ctl *ctl = kp.ctl;
@@ -1717,6 +2004,259 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
     if (tid <= XP_MAX_DEPTH)
         s_depth_out[tid] = 0;
     __syncthreads();
+
+    /* queue of screened rows (group-by joins only) */
+    const bool qmode = PREAGG && nrels > 0;
+    const bool qbloom = qmode && kp.prefilter != 0 && kp.bloom[1] != nullptr &&
+                        !P->depth[1].left_outer;
+    uint32_t *wq = &s_queue[PREAGG ? wid * XQ_WAVE : 0];
+
+    /* projection output of the current tile */
+    uint64_t lane_cnt = 0, lane_bytes = 0, lane_wr_rows = 0, lane_wr_bytes = 0;
+    bool stop_writes = false;
+
+    /* one outer row through the joins and the final stage; 'queued' rows
+     * passed the scan quals (counted) and the depth-1 filter already */
+    auto run_row = [&](Lane &L, const uint32_t row, const bool valid, const int pass,
+                       const bool queued) {
+        L.nmask = ~1U;      /* every register NULL but R0 */
+        L.err = 0;
+        L.dead = 0;
+#pragma unroll
+        for (int i = 0; i < NR; i++)
+            L.R[i] = 0;
+        L.R[XP_REG_TRUE] = 1;
+
+        int cd = 0;
+        uint32_t it[XP_MAX_DEPTH + 2];   /* chain offsets (inner KDS < 4GB) */
+        uint32_t matched = 0;
+        for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
+            it[i] = 0;
+
+        int phase = PH_SCAN;
+        int d = 0;
+        for (;;)
+        {
+            xseg seg;
+            bool act = false;
+            seg.begin = seg.end = 0;
+            /* ---- pre ---- */
+            switch (phase)
+            {
+                case PH_SCAN:
+                    seg = P->seg_scan;
+                    act = valid;
+                    break;
+                case PH_HASH:
+                    seg = P->depth[d].seg_hash;
+                    act = (cd == d && it[d] == 0);
+                    break;
+                case PH_JOIN:
+                    seg = P->depth[d].seg_join;
+                    act = (cd == d && it[d] > IT_END);
+                    if (act)
+                    {
+                        kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+                        L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, it[d])->t;
+                        SETNULL(L, P->r_tuple, 0);
+                    }
+                    break;
+                case PH_OJ:
+                    seg = P->depth[d].seg_oj;
+                    act = (cd == d && it[d] == IT_END && P->depth[d].left_outer &&
+                           !((matched >> d) & 1));
+                    if (act)
+                    {
+                        L.R[P->r_tuple] = 0;
+                        SETNULL(L, P->r_tuple, 1);
+                    }
+                    break;
+                case PH_EMIT:
+                    seg = P->seg_final;
+                    act = (cd == nrels + 1);
+                    break;
+            }
+            if (act)
+                PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
+            /* an error ends the lane's row and is reported for the task */
+            bool lane_err = false;
+            if (act && L.err)
+            {
+                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                L.err = 0;
+                lane_err = true;
+            }
+            /* ---- post ---- */
+            if (phase == PH_SCAN)
+            {
+                bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
+                if (ok && pass == 0 && COUNTS && !queued)
+                    cnt_in++;
+                cd = ok ? 1 : 0;
+                if (nrels > 0)
+                {
+                    phase = PH_HASH;
+                    d = 1;
+                }
+                else
+                    phase = PH_EMIT;
+            }
+            else if (phase == PH_HASH)
+            {
+                if (act)
+                {
+                    kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+                    uint32_t h = (uint32_t)L.R[P->r_hash];
+                    const uint64_t *bf = kp.bloom[d];
+                    uint64_t off = 0;
+                    if (!bf || (queued && d == 1) || bloom_maybe(bf, kp.bloom_mask[d], h))
+                    {
+                        const uint64_t *hs = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
+                        off = hs[h % ik->hash_nslots];
+                        uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
+                        while (off != 0 && off != ~0ULL && off < ik->length &&
+                               hashitem_at(ik, off)->t.hash != h && steps-- > 0)
+                            off = hashitem_at(ik, off)->next;
+                    }
+                    it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+                }
+                phase = PH_JOIN;
+            }
+            else if (phase == PH_JOIN)
+            {
+                if (act)
+                {
+                    kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+                    const kern_hashitem *cur = hashitem_at(ik, it[d]);
+                    uint32_t h = cur->t.hash;       /* == the probe hash */
+                    int64_t status = lane_err ? 0 : L.R[P->r_status];
+                    /* next candidate with the same hash, found now so that
+                     * exhausted chains unwind without another pass */
+                    uint64_t off = cur->next;
+                    uint32_t steps = ik->nitems;
+                    while (off != 0 && off != ~0ULL && off < ik->length &&
+                           hashitem_at(ik, off)->t.hash != h && steps-- > 0)
+                        off = hashitem_at(ik, off)->next;
+                    it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+                    if (status != 0)
+                        matched |= 1U << d;
+                    if (status > 0)
+                    {
+                        if (pass == 0 && COUNTS)
+                            atomicAdd(&s_depth_out[d], 1U);
+                        cd = d + 1;
+                        if (d + 1 <= nrels)
+                            it[d + 1] = 0;
+                    }
+                }
+                phase = PH_OJ;
+            }
+            else if (phase == PH_OJ)
+            {
+                bool adv = false;
+                if (act)
+                {
+                    matched |= 1U << d;
+                    if (!lane_err && L.R[P->r_status] > 0)
+                    {
+                        if (pass == 0 && COUNTS)
+                            atomicAdd(&s_depth_out[d], 1U);
+                        cd = d + 1;
+                        if (d + 1 <= nrels)
+                            it[d + 1] = 0;
+                        adv = true;
+                    }
+                }
+                if (!adv && cd == d && it[d] == IT_END)
+                {
+                    /* unwind exhausted depths (uniform indices only) */
+                    for (int dd = d; dd >= 1; dd--)
+                    {
+                        if (dd == cd && it[dd] == IT_END)
+                        {
+                            matched &= ~(1U << dd);
+                            it[dd] = 0;
+                            cd--;
+                        }
+                    }
+                }
+                if (d < nrels)
+                {
+                    d++;
+                    phase = PH_HASH;
+                }
+                else
+                    phase = PH_EMIT;
+            }
+            else    /* PH_EMIT */
+            {
+                if (act && !lane_err)
+                {
+                    if (pass == 0 && COUNTS)
+                        cnt_out++;
+                    if (PREAGG)
+                    {
+                        bool direct = true;
+                        if (T.n > 0 && gb_row_fast(P, L))
+                        {
+                            int slot = lds_probe(T, gb_pack_key(P, L));
+                            if (slot >= 0)
+                            {
+                                lds_update(P, L, T, slot);
+                                direct = false;
+                            }
+                        }
+                        if (direct)
+                            lds_miss++;
+                        uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
+                        gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+                    }
+                    else
+                    {
+                        uint32_t sz = (uint32_t)xalign(8, proj_tuple_size(P, L));
+                        if (pass == 0)
+                        {
+                            lane_cnt++;
+                            lane_bytes += sz;
+                        }
+                        else if (!stop_writes)
+                        {
+                            kern_data_store *dst = kp.kds_out;
+                            lane_wr_bytes += sz;
+                            uint64_t usage = s_base_bytes + lane_wr_bytes;
+                            uint64_t rowid = s_base_rows + lane_wr_rows;
+                            char *tp = (char *)dst + dst->length - usage;
+                            proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
+                            uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
+                            rowidx[rowid] = usage;
+                            lane_wr_rows++;
+                        }
+                    }
+                }
+                if (act)
+                {
+                    cd = nrels;
+                    for (int dd = nrels; dd >= 1; dd--)
+                    {
+                        if (dd == cd && it[dd] == IT_END)
+                        {
+                            matched &= ~(1U << dd);
+                            it[dd] = 0;
+                            cd--;
+                        }
+                    }
+                }
+                if (lane_err)
+                    cd = 0;
+                if (!wave_any(cd > 0))
+                    break;
+                phase = PH_HASH;
+                d = 1;
+            }
+            if (lane_err)
+                cd = 0;
+        }
+    };
 
     /* the write pass stops at the first tile that does not fit kds_dst */
     uint32_t stop_tile = kp.ntiles;
@@ -1749,8 +2289,76 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
 
         /* rows of this tile: PREAGG handles rows_per_tile / blockDim rows per lane */
         const uint32_t rpt = kp.rows_per_tile / blockDim.x;
-        uint64_t lane_cnt = 0, lane_bytes = 0, lane_wr_rows = 0, lane_wr_bytes = 0;
-        bool stop_writes = false;
+        lane_cnt = lane_bytes = lane_wr_rows = lane_wr_bytes = 0;
+        stop_writes = false;
+
+        if (qmode)
+        {
+            /* screen up to XQ_BATCHES row batches into the wave's queue, then
+             * run the joins over the queued rows 64 at a time */
+            for (uint32_t r0 = 0; r0 < rpt; r0 += XQ_BATCHES)
+            {
+                const uint32_t r1 = rpt - r0 < XQ_BATCHES ? rpt : r0 + XQ_BATCHES;
+                uint32_t q_count = 0;      /* wave-uniform */
+                for (uint32_t ri = r0; ri < r1; ri++)
+                {
+                    const uint64_t row64 = (uint64_t)kp.row_begin + (uint64_t)tile * kp.rows_per_tile +
+                                           (uint64_t)ri * blockDim.x + tid;
+                    const uint32_t row = (uint32_t)row64;
+                    int64_t Rfile[NR];
+                    Lane L;
+                    L.R = Rfile;
+                    L.nmask = ~1U;
+                    L.err = 0;
+                    L.dead = 0;
+#pragma unroll
+                    for (int i = 0; i < NR; i++)
+                        L.R[i] = 0;
+                    L.R[XP_REG_TRUE] = 1;
+                    bool cand = false;
+                    if (row64 < kp.row_end)
+                    {
+                        PROG::template run<false>(P, P->seg_scan.begin, P->seg_scan.end, L, kds, row, kp);
+                        bool ok = false;
+                        if (L.err)
+                            record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                        else
+                            ok = !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
+                        if (ok && COUNTS)
+                            cnt_in++;
+                        if (ok && qbloom)
+                        {
+                            PROG::template run<false>(P, P->depth[1].seg_hash.begin,
+                                                      P->depth[1].seg_hash.end, L, kds, row, kp);
+                            if (L.err)
+                            {
+                                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                                ok = false;
+                            }
+                            else
+                                ok = bloom_maybe(kp.bloom[1], kp.bloom_mask[1], (uint32_t)L.R[P->r_hash]);
+                        }
+                        cand = ok;
+                    }
+                    const uint64_t m = __ballot(cand);
+                    if (cand)
+                        wq[q_count + (uint32_t)__popcll(m & ((1ULL << lane) - 1))] = row;
+                    q_count += (uint32_t)__popcll(m);
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t q0 = 0; q0 < q_count; q0 += 64)
+                {
+                    const bool has = q0 + (uint32_t)lane < q_count;
+                    const uint32_t qrow = has ? wq[q0 + lane] : 0;
+                    int64_t Rfile[NR];
+                    Lane L;
+                    L.R = Rfile;
+                    run_row(L, qrow, has, 0, true);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            continue;
+        }
 
         for (int pass = 0; pass < (PREAGG ? 1 : 2); pass++)
         for (uint32_t ri = 0; ri < rpt; ri++)
@@ -1763,241 +2371,13 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
             int64_t Rfile[NR];
             Lane L;
             L.R = Rfile;
-            L.nmask = ~1U;      /* every register NULL but R0 */
-            L.err = 0;
-            L.dead = 0;
-#pragma unroll
-            for (int i = 0; i < NR; i++)
-                L.R[i] = 0;
-            L.R[XP_REG_TRUE] = 1;
-
-            int cd = 0;
-            uint32_t it[XP_MAX_DEPTH + 2];   /* chain offsets (inner KDS < 4GB) */
-            uint32_t matched = 0;
-            for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
-                it[i] = 0;
 
             /* a scan-only projection keeps its registers from pass 0; the
              * count pass of a two-pass projection writes nothing */
             if (!PREAGG && pass == 1 && (nrels == 0 || PM == PM_COUNT))
                 break;
 
-            int phase = PH_SCAN;
-            int d = 0;
-            for (;;)
-            {
-                xseg seg;
-                bool act = false;
-                seg.begin = seg.end = 0;
-                /* ---- pre ---- */
-                switch (phase)
-                {
-                    case PH_SCAN:
-                        seg = P->seg_scan;
-                        act = valid;
-                        break;
-                    case PH_HASH:
-                        seg = P->depth[d].seg_hash;
-                        act = (cd == d && it[d] == 0);
-                        break;
-                    case PH_JOIN:
-                        seg = P->depth[d].seg_join;
-                        act = (cd == d && it[d] > IT_END);
-                        if (act)
-                        {
-                            kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
-                            L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, it[d])->t;
-                            SETNULL(L, P->r_tuple, 0);
-                        }
-                        break;
-                    case PH_OJ:
-                        seg = P->depth[d].seg_oj;
-                        act = (cd == d && it[d] == IT_END && P->depth[d].left_outer &&
-                               !((matched >> d) & 1));
-                        if (act)
-                        {
-                            L.R[P->r_tuple] = 0;
-                            SETNULL(L, P->r_tuple, 1);
-                        }
-                        break;
-                    case PH_EMIT:
-                        seg = P->seg_final;
-                        act = (cd == nrels + 1);
-                        break;
-                }
-                if (act)
-                    PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
-                /* an error ends the lane's row and is reported for the task */
-                bool lane_err = false;
-                if (act && L.err)
-                {
-                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
-                    L.err = 0;
-                    lane_err = true;
-                }
-                /* ---- post ---- */
-                if (phase == PH_SCAN)
-                {
-                    bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
-                    if (ok && pass == 0 && COUNTS)
-                        cnt_in++;
-                    cd = ok ? 1 : 0;
-                    if (nrels > 0)
-                    {
-                        phase = PH_HASH;
-                        d = 1;
-                    }
-                    else
-                        phase = PH_EMIT;
-                }
-                else if (phase == PH_HASH)
-                {
-                    if (act)
-                    {
-                        kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
-                        uint32_t h = (uint32_t)L.R[P->r_hash];
-                        const uint64_t *hs = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
-                        uint64_t off = hs[h % ik->hash_nslots];
-                        uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
-                        while (off != 0 && off != ~0ULL && hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                            off = hashitem_at(ik, off)->next;
-                        it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
-                    }
-                    phase = PH_JOIN;
-                }
-                else if (phase == PH_JOIN)
-                {
-                    if (act)
-                    {
-                        kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
-                        const kern_hashitem *cur = hashitem_at(ik, it[d]);
-                        uint32_t h = cur->t.hash;       /* == the probe hash */
-                        int64_t status = lane_err ? 0 : L.R[P->r_status];
-                        /* next candidate with the same hash, found now so that
-                         * exhausted chains unwind without another pass */
-                        uint64_t off = cur->next;
-                        uint32_t steps = ik->nitems;
-                        while (off != 0 && off != ~0ULL && hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                            off = hashitem_at(ik, off)->next;
-                        it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
-                        if (status != 0)
-                            matched |= 1U << d;
-                        if (status > 0)
-                        {
-                            if (pass == 0 && COUNTS)
-                                atomicAdd(&s_depth_out[d], 1U);
-                            cd = d + 1;
-                            if (d + 1 <= nrels)
-                                it[d + 1] = 0;
-                        }
-                    }
-                    phase = PH_OJ;
-                }
-                else if (phase == PH_OJ)
-                {
-                    bool adv = false;
-                    if (act)
-                    {
-                        matched |= 1U << d;
-                        if (!lane_err && L.R[P->r_status] > 0)
-                        {
-                            if (pass == 0 && COUNTS)
-                                atomicAdd(&s_depth_out[d], 1U);
-                            cd = d + 1;
-                            if (d + 1 <= nrels)
-                                it[d + 1] = 0;
-                            adv = true;
-                        }
-                    }
-                    if (!adv && cd == d && it[d] == IT_END)
-                    {
-                        /* unwind exhausted depths (uniform indices only) */
-                        for (int dd = d; dd >= 1; dd--)
-                        {
-                            if (dd == cd && it[dd] == IT_END)
-                            {
-                                matched &= ~(1U << dd);
-                                it[dd] = 0;
-                                cd--;
-                            }
-                        }
-                    }
-                    if (d < nrels)
-                    {
-                        d++;
-                        phase = PH_HASH;
-                    }
-                    else
-                        phase = PH_EMIT;
-                }
-                else    /* PH_EMIT */
-                {
-                    if (act && !lane_err)
-                    {
-                        if (pass == 0 && COUNTS)
-                            cnt_out++;
-                        if (PREAGG)
-                        {
-                            bool direct = true;
-                            if (T.n > 0 && gb_row_fast(P, L))
-                            {
-                                int slot = lds_probe(T, gb_pack_key(P, L));
-                                if (slot >= 0)
-                                {
-                                    lds_update(P, L, T, slot);
-                                    direct = false;
-                                }
-                            }
-                            if (direct)
-                                lds_miss++;
-                            uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-                            gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
-                        }
-                        else
-                        {
-                            uint32_t sz = (uint32_t)xalign(8, proj_tuple_size(P, L));
-                            if (pass == 0)
-                            {
-                                lane_cnt++;
-                                lane_bytes += sz;
-                            }
-                            else if (!stop_writes)
-                            {
-                                kern_data_store *dst = kp.kds_out;
-                                lane_wr_bytes += sz;
-                                uint64_t usage = s_base_bytes + lane_wr_bytes;
-                                uint64_t rowid = s_base_rows + lane_wr_rows;
-                                char *tp = (char *)dst + dst->length - usage;
-                                proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
-                                uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
-                                rowidx[rowid] = usage;
-                                lane_wr_rows++;
-                            }
-                        }
-                    }
-                    if (act)
-                    {
-                        cd = nrels;
-                        for (int dd = nrels; dd >= 1; dd--)
-                        {
-                            if (dd == cd && it[dd] == IT_END)
-                            {
-                                matched &= ~(1U << dd);
-                                it[dd] = 0;
-                                cd--;
-                            }
-                        }
-                    }
-                    if (lane_err)
-                        cd = 0;
-                    if (!wave_any(cd > 0))
-                        break;
-                    phase = PH_HASH;
-                    d = 1;
-                }
-                if (lane_err)
-                    cd = 0;
-            }
+            run_row(L, row, valid, pass, false);
 
             if (!PREAGG && pass == 0)
             {
@@ -2134,7 +2514,6 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
         if (!PREAGG)
             __syncthreads();    /* s_base_* / s_wave_* reuse by the next tile */
     }
-
     if (PREAGG && T.n > 0)
     {
         __syncthreads();
@@ -2521,7 +2900,12 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
                                   (uint64_t)gridDim.x * nwaves, W);
     __syncthreads();
     if (T.n > 0)
-        lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+    {
+        if (kp.dump)
+            lds_dump<NR>(P, T, kp);
+        else
+            lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+    }
     scan_agg_report(kp.ctl, W);
 }
 
@@ -2616,6 +3000,64 @@ spill_nparts(uint32_t ngroups, uint32_t table_n, uint32_t maxparts)
     return nparts < 1 ? 1 : (nparts > maxparts ? maxparts : nparts);
 }
 
+/* the spill records [lo, hi) of kp.spill, every key, folded into T; a full
+ * table is dumped (partial-state records) and restarted */
+template <int NR>
+__device__ __forceinline__ void
+spill_fold_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t lo, uint64_t hi)
+{
+    const int rec_words = 1 + P->spill_nregs;
+    int64_t Rfile[NR];
+    Lane L;
+    L.R = Rfile;
+    L.nmask = ~1U;
+    L.err = 0;
+    L.dead = 0;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+        L.R[r] = 0;
+    L.R[XP_REG_TRUE] = 1;
+    for (uint64_t base = lo; base < hi; base += blockDim.x)
+    {
+        const uint64_t i = base + threadIdx.x;
+        uint64_t pk = XLDS_EMPTY_KEY;
+        if (i < hi)
+            pk = kp.spill[i * rec_words];
+        bool fail = false;
+        if (pk != XLDS_EMPTY_KEY)
+        {
+            gb_unpack_key(P, L, pk);
+            for (int j = 0; j < (int)P->spill_nregs; j++)
+            {
+                const int r = P->spill_regs[j];
+                L.R[r] = (int64_t)kp.spill[i * rec_words + 1 + j];
+                SETNULL(L, r, 0);
+            }
+            int slot = lds_probe(T, pk);
+            if (slot >= 0)
+                lds_update(P, L, T, slot);
+            else
+                fail = true;
+        }
+        while (__syncthreads_or(fail))
+        {
+            lds_dump<NR>(P, T, kp);
+            __syncthreads();
+            lds_init(P, T);
+            __syncthreads();
+            if (fail)
+            {
+                int slot = lds_probe(T, pk);
+                if (slot >= 0)
+                {
+                    lds_update(P, L, T, slot);
+                    fail = false;
+                }
+            }
+        }
+    }
+}
+
 template <int NR, class PROG>
 __device__ __forceinline__ void
 spill_agg_body(cplan *P, const xpu_kparams &kp)
@@ -2631,6 +3073,16 @@ spill_agg_body(cplan *P, const xpu_kparams &kp)
     const uint64_t nrec = reserved < kp.spill_cap ? reserved : kp.spill_cap;
     if (nrec == 0 || T.n == 0 || kp.spill == nullptr)
         return;
+    if (kp.dump)
+    {
+        /* each block folds one slice of the records (read once) */
+        const uint64_t lo = nrec * blockIdx.x / gridDim.x;
+        const uint64_t hi = nrec * (blockIdx.x + 1) / gridDim.x;
+        spill_fold_range<NR>(P, kp, T, lo, hi);
+        __syncthreads();
+        lds_dump<NR>(P, T, kp);
+        return;
+    }
     const uint32_t nparts = spill_nparts(kp.kds_out->nitems, T.n,
                                          gridDim.x < 64 ? gridDim.x : 64);
     const uint32_t part = blockIdx.x % nparts;

@@ -142,6 +142,9 @@ xpu_jit_source(const xplan *P, int kind, int param)
              "xpu_jit_spill_kernel(xpu_kparams kp)\n{\n";
         append(s, "    spill_agg_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_dump_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    dump_merge_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
              "xpu_jit_batch_kernel(xpu_kparams kp)\n{\n";
         append(s, "    batch_agg_body<%d, %d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr, param);
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"

@@ -20,6 +20,14 @@
 #define XLDS_EMPTY_KEY      0xffffffffffffffffULL
 #define XGB_LOCKED_SLOT     0xffffffffffffffffULL
 #define XSPILL_CHUNK        512     /* spill records a wave reserves at once */
+#define XP_DUMP_MAXPARTS    256     /* partial-state dump partitions at most */
+#define XP_BLOCK_MAX        512     /* threads per block at most */
+/* group-by joins: 64-row batches a wave screens before running the joins
+ * over the rows that pass, and the static LDS of the pipeline kernel (the
+ * screened-row queues and the block's counters) */
+#define XQ_BATCHES          8
+#define XQ_WAVE             (64 * XQ_BATCHES)
+#define XP_PIPELINE_STATIC_LDS  ((XP_BLOCK_MAX / 64) * XQ_WAVE * 4 + 1024)
 
 /* per-task control block in device memory (cleared before each launch) */
 typedef struct
@@ -112,7 +120,14 @@ typedef struct
     uint32_t    batch_nfrags;
     uint32_t    batch_nitems;
     uint32_t    batch_nsitems_cap;
-    uint32_t    __pad2;
+    uint32_t    dump_nparts;            /* partial-state dump partitions (0: none) */
+    uint64_t   *dump;                   /* partial-state records, [part][dump_cap] */
+    uint64_t    dump_cap;               /* records per partition */
+    uint64_t   *dump_next;              /* [dump_nparts] records reserved */
+    /* join-hash Bloom filters of the inner relations (nullptr: none) */
+    const uint64_t *bloom[XP_MAX_DEPTH + 1];
+    uint32_t    bloom_mask[XP_MAX_DEPTH + 1];   /* words - 1 (words: a power of two) */
+    uint32_t    prefilter;              /* group-by joins: screen rows before the joins */
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -124,6 +139,9 @@ int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
 /* projection in two passes: count (tile totals), tile scan, write */
 int xpu_launch_projection(const xpu_kparams *kp, int nregs, int grid, int block, void *stream);
 int xpu_launch_tile_scan(const xpu_kparams *kp, void *stream);
+/* folds the partial-state dumps of a scan + group-by task into kds_final */
+int xpu_launch_dump_merge(const xpu_kparams *kp, int nregs, int grid, int block,
+                          size_t lds_bytes, void *stream);
 /* the spill setup kernel of a batch alone (for specialized pass kernels) */
 int xpu_launch_batch_setup(const xpu_kparams *kp, void *stream);
 /* batched fragments: pass 1, spill setup, pass 2 (all stream-ordered) */
@@ -133,6 +151,10 @@ int xpu_launch_batch(const xpu_kparams *kp, int nregs, int rpt, int grid, int bl
 int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
                          size_t lds_bytes, void *stream);
 /* scan + group-by without joins; rpt = rows per lane per wave tile (1, 2, 4) */
+/* sets the Bloom filter bits of the join hashes of one inner KDS_FORMAT_HASH
+ * (words zeroed by the caller) */
+int xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words,
+                           uint32_t mask, void *stream);
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);

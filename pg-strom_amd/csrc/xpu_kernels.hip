@@ -39,6 +39,13 @@ xpu_spill_agg_kernel(xpu_kparams kp)
     spill_agg_body<NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
 }
 
+template <int NR>
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_dump_merge_kernel(xpu_kparams kp)
+{
+    dump_merge_body<NR, Interp>((cplan *)(uintptr_t)kp.plan, kp);
+}
+
 template <int NR, int RPT>
 __global__ void __launch_bounds__(XP_BLOCK_MAX)
 xpu_batch_agg_kernel(xpu_kparams kp)
@@ -72,6 +79,17 @@ xpu_fill_kernel(uint64_t *ptr, uint64_t value, size_t nwords)
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < nwords; i += stride)
         ptr[i] = value;
+}
+
+/* Bloom filter of an inner relation's join hashes: one lane per hash slot
+ * walks that slot's chain */
+__global__ void
+xpu_bloom_build_kernel(kern_data_store *inner, uint64_t nslots, uint64_t *words, uint32_t mask)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < nslots; i += stride)
+        bloom_build_range(inner, words, mask, i, i + 1);
 }
 
 /* writes the kds_dst header: the session's KDS_FORMAT_ROW header with the
@@ -175,6 +193,20 @@ xpu_launch_batch(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
 }
 
 extern "C" int
+xpu_launch_dump_merge(const xpu_kparams *kp, int nregs, int grid, int block, size_t lds_bytes,
+                      void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nregs <= XP_NREGS_SMALL)
+        hipLaunchKernelGGL((xpu_dump_merge_kernel<XP_NREGS_SMALL>), dim3(grid), dim3(block),
+                           lds_bytes, s, *kp);
+    else
+        hipLaunchKernelGGL((xpu_dump_merge_kernel<XP_NREGS>), dim3(grid), dim3(block),
+                           lds_bytes, s, *kp);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
 xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block, size_t lds_bytes,
                      void *stream)
 {
@@ -259,6 +291,20 @@ xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream)
         blocks = 4096;
     hipLaunchKernelGGL(xpu_fill_kernel, dim3((unsigned)blocks), dim3(256), 0,
                        (hipStream_t)stream, (uint64_t *)ptr, value, nwords);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words, uint32_t mask,
+                       void *stream)
+{
+    if (nslots == 0)
+        return 0;
+    uint64_t blocks = (nslots + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(xpu_bloom_build_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, inner, nslots, words, mask);
     return (int)hipGetLastError();
 }
 

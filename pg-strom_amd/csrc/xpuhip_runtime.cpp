@@ -101,6 +101,10 @@ struct Scratch
     size_t      status_cap = 0;
     uint64_t   *d_spill = nullptr;      /* group-by spill records */
     size_t      spill_cap = 0;          /* bytes */
+    uint64_t   *d_dump = nullptr;       /* partial-state dump records */
+    size_t      dump_cap = 0;           /* bytes */
+    uint64_t   *d_dump_next = nullptr;  /* per-partition reservation counters */
+    size_t      dump_next_cap = 0;
     char       *d_src = nullptr;
     size_t      src_cap = 0;
     char       *d_dst = nullptr;
@@ -127,6 +131,8 @@ scratch_destroy(Scratch *s)
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->d_status) (void)hipFree(s->d_status);
     if (s->d_spill) (void)hipFree(s->d_spill);
+    if (s->d_dump) (void)hipFree(s->d_dump);
+    if (s->d_dump_next) (void)hipFree(s->d_dump_next);
     if (s->d_src) (void)hipFree(s->d_src);
     if (s->d_dst) (void)hipFree(s->d_dst);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -165,6 +171,9 @@ struct xpuhip_session
     kern_data_store *d_kds_final = nullptr;
     uint64_t        final_length = 0;
     char           *d_kmrels = nullptr;
+    uint64_t       *d_bloom = nullptr;      /* Bloom filters of the inner relations */
+    uint64_t        bloom_off[XP_MAX_DEPTH + 1] = {};   /* words from d_bloom */
+    uint32_t        bloom_mask[XP_MAX_DEPTH + 1] = {};  /* words - 1 (0: no filter) */
     int64_t        *d_params = nullptr;     /* XO_PARAM values */
     bool            inner_ready = false;
     xpuhip_tuning   tuning;
@@ -246,9 +255,11 @@ default_tuning(const xpuhip_session *s, xpuhip_tuning *t)
     else if (s->comp.has_groupby)
     {
         /* joins + group-by: two 512-thread blocks per CU (measured on the
-         * 200M-row star join: 15 ms against 25 ms with 256-thread blocks) */
+         * 200M-row star join: 15 ms against 25 ms with 256-thread blocks);
+         * a wave screens 8 row batches of a tile before its joins run over
+         * the rows that pass */
         t->block_size = 512;
-        t->rows_per_thread = 4;
+        t->rows_per_thread = 8;
         t->blocks_per_cu = 2;
         t->lds_nents = lds_nents_within(P, 48 * 1024);
     }
@@ -565,7 +576,8 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
             XERR(eb, ERRCODE_DEVICE_ERROR, "only KDS_FORMAT_HASH inner relations are supported");
             return ERRCODE_DEVICE_ERROR;
         }
-        if (ch->kds_offset + k->length > length || k->length >= (1ULL << 32))
+        if (ch->kds_offset + k->length > length || k->length >= (1ULL << 32) ||
+            KDS_HEAD_LENGTH(k) + 8ULL * k->hash_nslots > k->length)
         {
             XERR(eb, ERRCODE_DEVICE_ERROR, "inner KDS is truncated or larger than 4GB");
             return ERRCODE_DEVICE_ERROR;
@@ -586,6 +598,41 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
     {
         P->depth[d].inner_kds = (uint64_t)(uintptr_t)(s->d_kmrels + kmrels->chunks[d - 1].kds_offset);
         P->depth[d].left_outer = kmrels->chunks[d - 1].left_outer ? 1 : 0;
+    }
+    /* Bloom filters: 16 to 32 bits per item in 64-bit words (a power of two,
+     * at most 2^23 words), built from the hash chains just copied */
+    if (s->d_bloom)
+        (void)hipFree(s->d_bloom);
+    s->d_bloom = nullptr;
+    uint64_t nwords = 0;
+    for (uint32_t d = 1; d <= P->nrels; d++)
+    {
+        const kern_data_store *k = (const kern_data_store *)((const char *)kmrels +
+                                                             kmrels->chunks[d - 1].kds_offset);
+        uint64_t w = 1;
+        while (w < (k->nitems + 3) / 4 && w < (1ULL << 23))
+            w <<= 1;
+        s->bloom_off[d] = nwords;
+        s->bloom_mask[d] = (uint32_t)(w - 1);
+        nwords += w;
+    }
+    if (nwords > 0)
+    {
+        if (!hip_ok(hipMalloc((void **)&s->d_bloom, 8 * nwords), eb, "hipMalloc(bloom)") ||
+            !hip_ok((hipError_t)xpu_launch_fill(s->d_bloom, 0, nwords, nullptr), eb, "bloom fill"))
+            return ERRCODE_DEVICE_ERROR;
+        for (uint32_t d = 1; d <= P->nrels; d++)
+        {
+            const kern_data_store *k = (const kern_data_store *)((const char *)kmrels +
+                                                                 kmrels->chunks[d - 1].kds_offset);
+            if (!hip_ok((hipError_t)xpu_launch_bloom_build((kern_data_store *)(uintptr_t)P->depth[d].inner_kds,
+                                                           k->hash_nslots, s->d_bloom + s->bloom_off[d],
+                                                           s->bloom_mask[d], nullptr),
+                        eb, "bloom build"))
+                return ERRCODE_DEVICE_ERROR;
+        }
+        if (!hip_ok(hipStreamSynchronize(nullptr), eb, "bloom build"))
+            return ERRCODE_DEVICE_ERROR;
     }
     if (!hip_ok(hipMemcpy(s->d_plan, P, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy(plan)"))
         return ERRCODE_DEVICE_ERROR;
@@ -609,6 +656,7 @@ struct JitFns
     hipFunction_t   spill = nullptr;        /* spill pass (scan + group-by) */
     hipFunction_t   batch = nullptr;        /* batched pass 1 */
     hipFunction_t   batch_spill = nullptr;  /* batched pass 2 */
+    hipFunction_t   dump = nullptr;         /* partial-state dump merge */
     hipFunction_t   count = nullptr;        /* projection count pass */
     hipFunction_t   write = nullptr;        /* projection write pass */
 };
@@ -642,6 +690,8 @@ jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
          !hip_ok(hipModuleGetFunction(&f.batch, mod, "xpu_jit_batch_kernel"), eb,
                  "hipModuleGetFunction") ||
          !hip_ok(hipModuleGetFunction(&f.batch_spill, mod, "xpu_jit_batch_spill_kernel"), eb,
+                 "hipModuleGetFunction") ||
+         !hip_ok(hipModuleGetFunction(&f.dump, mod, "xpu_jit_dump_kernel"), eb,
                  "hipModuleGetFunction")))
         return nullptr;
     if (kind == XJ_PIPELINE && !param &&
@@ -714,7 +764,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         grid = ntiles;      /* scan_agg: a block tile is its waves' first tiles */
     uint32_t lds_nents = (preagg && P->lds_fast) ? t.lds_nents : 0;
     size_t lds = lds_bytes_for(P, lds_nents);
-    if (lds > g_dev.lds_per_block)
+    if (lds + (scan_agg ? 0 : XP_PIPELINE_STATIC_LDS) > g_dev.lds_per_block)
     {
         XERR(eb, ERRCODE_DEVICE_ERROR, "group-by LDS table exceeds the LDS of a block");
         return ERRCODE_DEVICE_ERROR;
@@ -739,6 +789,15 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     kp.param_nulls = s->comp.param_nulls;
     for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
         kp.inner_kds[d] = P->depth[d].inner_kds;
+    if (t.join_filter != XPUHIP_JOIN_FILTER_OFF && s->d_bloom)
+    {
+        for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
+        {
+            kp.bloom[d] = s->d_bloom + s->bloom_off[d];
+            kp.bloom_mask[d] = s->bloom_mask[d];
+        }
+        kp.prefilter = (preagg && P->nrels > 0) ? 1 : 0;
+    }
     /* spill buffer of a scan + group-by: room for a quarter of the rows (at
      * least 1M records); rows beyond it merge into kds_final directly */
     const bool spill = scan_agg && P->spill_ok && lds_nents > 0;
@@ -750,10 +809,27 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
             return ERRCODE_DEVICE_ERROR;
         kp.spill = x->d_spill;
         kp.spill_cap = x->spill_cap / rec;
+        /* LDS entries leave the scan and spill passes as hash-partitioned
+         * partial-state records; one block per partition merges each group
+         * into kds_final once */
+        uint32_t nparts = (uint32_t)g_dev.ncu < XP_DUMP_MAXPARTS ? (uint32_t)g_dev.ncu : XP_DUMP_MAXPARTS;
+        uint64_t rw = 1 + P->lds_n64 + (1 + P->lds_n32 + 1) / 2;
+        /* one table per block from the scan pass plus the spill pass's
+         * dumps of full tables: room for eight tables per block */
+        uint64_t per_part = (8ULL * grid * lds_nents) / nparts + 4096;
+        if (!grow((char **)&x->d_dump, &x->dump_cap, per_part * nparts * rw * 8, eb) ||
+            !grow((char **)&x->d_dump_next, &x->dump_next_cap, 8ULL * XP_DUMP_MAXPARTS, eb))
+            return ERRCODE_DEVICE_ERROR;
+        kp.dump = x->d_dump;
+        kp.dump_cap = x->dump_cap / (nparts * rw * 8);
+        kp.dump_next = x->d_dump_next;
+        kp.dump_nparts = nparts;
     }
 
     int rc = 0;
     xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
+    if (kp.dump)
+        xpu_launch_fill(kp.dump_next, 0, kp.dump_nparts, x->stream);
     if (!preagg)
     {
         /* kds_dst header first, so the kernel sees the capacity */
@@ -762,7 +838,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         xpu_launch_fill(x->d_status, 0, 2ULL * ntiles, x->stream);
     }
     /* the kernel specialized for the plan, or the interpreter */
-    hipFunction_t jfn = nullptr, jsfn = nullptr, jcount = nullptr, jwrite = nullptr;
+    hipFunction_t jfn = nullptr, jsfn = nullptr, jcount = nullptr, jwrite = nullptr, jdump = nullptr;
     if (t.jit != XPUHIP_JIT_OFF && (t.jit == XPUHIP_JIT_ON || nrows >= XPUHIP_JIT_AUTO_ROWS))
     {
         const JitFns *jf = scan_agg ? jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb)
@@ -773,6 +849,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
             jsfn = jf->spill;
             jcount = jf->count;
             jwrite = jf->write;
+            jdump = jf->dump;
         }
         else if (t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
@@ -820,6 +897,19 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         }
         else
             rc = xpu_launch_spill_agg(&kp, P->nregs, sgrid, block, lds, x->stream);
+        /* the partial-state dumps of both passes, one block per partition */
+        if (rc == 0 && kp.dump)
+        {
+            uint32_t dgrid = kp.dump_nparts;
+            if (jfn && jdump)
+            {
+                void *args[] = {&kp};
+                rc = (int)hipModuleLaunchKernel(jdump, dgrid, 1, 1, block, 1, 1, (unsigned)lds,
+                                                x->stream, args, nullptr);
+            }
+            else
+                rc = xpu_launch_dump_merge(&kp, P->nregs, dgrid, block, lds, x->stream);
+        }
     }
     (void)hipEventRecord(x->ev1, x->stream);
     if (rc != 0)
@@ -1027,6 +1117,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_kds_dst_head) (void)hipFree(s->d_kds_dst_head);
     if (s->d_kds_final) (void)hipFree(s->d_kds_final);
     if (s->d_kmrels) (void)hipFree(s->d_kmrels);
+    if (s->d_bloom) (void)hipFree(s->d_bloom);
     if (s->d_params) (void)hipFree(s->d_params);
     delete s;
 }
@@ -1044,7 +1135,8 @@ xpuhip_session_set_tuning(xpuhip_session *s, const xpuhip_tuning *t)
     if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
     if (n.lds_nents == 0) n.lds_nents = d.lds_nents;
     if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > 512 ||
-        n.rows_per_thread > 64 || n.blocks_per_cu > 64 || n.jit > XPUHIP_JIT_OFF)
+        n.rows_per_thread > 64 || n.blocks_per_cu > 64 || n.jit > XPUHIP_JIT_OFF ||
+        n.join_filter > XPUHIP_JOIN_FILTER_OFF)
         return ERRCODE_DEVICE_ERROR;
     if (uses_scan_agg(s) && n.rows_per_thread != 1 && n.rows_per_thread != 2 &&
         n.rows_per_thread != 4)
