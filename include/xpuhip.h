@@ -37,6 +37,7 @@ typedef struct
     uint32_t    blocks_per_cu;      /* persistent blocks per compute unit */
     uint32_t    jit;                /* XPUHIP_JIT_*: kernel specialized for the plan */
     uint32_t    join_filter;        /* XPUHIP_JOIN_FILTER_*: inner-relation Bloom filters */
+    uint32_t    join_passes;        /* XPUHIP_JOIN_PASSES_*: group-by joins in passes */
 } xpuhip_tuning;
 
 #define XPUHIP_JIT_AUTO     0       /* specialize chunks of >= 1,000,000 rows */
@@ -50,6 +51,16 @@ typedef struct
 #define XPUHIP_JOIN_FILTER_AUTO 0   /* on */
 #define XPUHIP_JOIN_FILTER_ON   1
 #define XPUHIP_JOIN_FILTER_OFF  2
+
+/* A group-by over joins that runs the specialized kernels (xpuhip_tuning.jit)
+ * runs as a screen pass (scan quals and the depth-1 filter) and one pass per
+ * depth over the rows that reach it, each its own launch over dense records,
+ * instead of one kernel that carries every row through every depth.  Results
+ * are the same either way. */
+#define XPUHIP_JOIN_PASSES_AUTO 0   /* passes, unless the inner relations' duplicate
+                                       keys need more than the record budget */
+#define XPUHIP_JOIN_PASSES_ON   1
+#define XPUHIP_JOIN_PASSES_OFF  2
 
 /* Device setup; replaces gpuservSetupGpuContext (gpu_service.c:6462-6523).
  * device < 0 selects device 0. */

@@ -1929,6 +1929,210 @@ dump_merge_body(cplan *P, const xpu_kparams &kp)
 }
 
 /* ------------------------------------------------------------------
+ * staged columns: a wave's loads of the columns the plan stages
+ * (XLD_STAGED LDCOLs) for 64*RPT consecutive rows, issued together before
+ * any loaded value is used, so a lane keeps nstage*RPT loads in flight
+ * instead of one dependent load per column per row.  fill() sets a row's
+ * registers with the NULL rules of XO_LDCOL; the program then runs with
+ * STAGED=true, which skips the staged LDCOLs.
+ * ------------------------------------------------------------------ */
+template <int RPT>
+__device__ __forceinline__ uint32_t
+pick_u32(const uint32_t (&a)[RPT], int i)
+{
+    uint32_t v = a[0];
+#pragma unroll
+    for (int k = 1; k < RPT; k++)
+        v = (i == k) ? a[k] : v;
+    return v;
+}
+
+template <int RPT>
+struct StagedCols
+{
+    /* geometry of the staged columns (uniform: scalar registers) */
+    const char *kbase;
+    int         nstage;
+    const char *vbase[XP_MAX_STAGE];
+    uint64_t    vlen[XP_MAX_STAGE], nmoff[XP_MAX_STAGE], nmlen[XP_MAX_STAGE];
+    int         w[XP_MAX_STAGE], reg[XP_MAX_STAGE];
+    bool        f32[XP_MAX_STAGE];
+    /* the rows [base, base + 64*RPT): raw words and validity-bitmap words */
+    uint64_t    base;
+    bool        nm_words;
+    uint32_t    lo[XP_MAX_STAGE][RPT], hi[XP_MAX_STAGE][RPT];
+    uint64_t    nmw[XP_MAX_STAGE][RPT];
+
+    __device__ __forceinline__ void
+    setup(cplan *P, const kern_data_store *kds)
+    {
+        kbase = (const char *)kds;
+        ckds *kmeta = (ckds *)(uintptr_t)kds;
+        nstage = (int)P->nstage;
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+        {
+            vbase[st] = kbase;
+            vlen[st] = nmoff[st] = nmlen[st] = 0;
+            w[st] = 1;
+            reg[st] = 0;
+            f32[st] = false;
+            if (st < nstage)
+            {
+                const xstage sd = P->stage[st];
+                ccolmeta *cm = &kmeta->colmeta[sd.col];
+                vbase[st] = kbase + cm->values_offset;
+                vlen[st] = cm->values_length;
+                nmoff[st] = cm->nullmap_offset;
+                nmlen[st] = cm->nullmap_length;
+                w[st] = sd.width;
+                reg[st] = sd.reg;
+                f32[st] = sd.f32 != 0;
+            }
+        }
+    }
+
+    /* A tile that lies inside every staged column (all but the last tile)
+     * loads without per-lane bounds checks, so no load waits on another; the
+     * tail tile checks each element. */
+    __device__ __forceinline__ void
+    load(uint64_t tile_base, uint64_t row_end)
+    {
+        const int lane = __lane_id();
+        const uint64_t wrows = 64ULL * RPT;
+        base = tile_base;
+        bool full = base + wrows <= row_end;
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+            if (st < nstage)
+                full = full && (uint64_t)w[st] * (base + wrows) <= vlen[st];
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+        {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+            {
+                lo[st][i] = 0;
+                hi[st][i] = 0;
+            }
+        }
+        if (full)
+        {
+#pragma unroll
+            for (int st = 0; st < XP_MAX_STAGE; st++)
+            {
+                if (st < nstage)
+                {
+                    const char *vb = vbase[st];
+                    const int ww = w[st];
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
+                    {
+                        const char *p = vb + (uint64_t)ww * (base + (uint64_t)i * 64 + lane);
+                        if (ww == 8)
+                        {
+                            uint2 u = *(const uint2 *)p;
+                            lo[st][i] = u.x;
+                            hi[st][i] = u.y;
+                        }
+                        else if (ww == 4)
+                            lo[st][i] = *(const uint32_t *)p;
+                        else if (ww == 2)
+                            lo[st][i] = (uint32_t)(int32_t)*(const int16_t *)p;
+                        else
+                            lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
+                    }
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int st = 0; st < XP_MAX_STAGE; st++)
+            {
+                if (st < nstage)
+                {
+                    const char *vb = vbase[st];
+                    const uint64_t vl = vlen[st];
+                    const int ww = w[st];
+#pragma unroll
+                    for (int i = 0; i < RPT; i++)
+                    {
+                        const uint64_t row = base + (uint64_t)i * 64 + lane;
+                        if (row < row_end && (uint64_t)ww * (row + 1) <= vl)
+                        {
+                            const char *p = vb + (uint64_t)ww * row;
+                            if (ww == 8)
+                            {
+                                uint2 u = *(const uint2 *)p;
+                                lo[st][i] = u.x;
+                                hi[st][i] = u.y;
+                            }
+                            else if (ww == 4)
+                                lo[st][i] = *(const uint32_t *)p;
+                            else if (ww == 2)
+                                lo[st][i] = (uint32_t)(int32_t)*(const int16_t *)p;
+                            else
+                                lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
+                        }
+                    }
+                }
+            }
+        }
+        /* validity bitmaps: one 64-bit word per 64 rows, read with scalar
+         * loads when the tile is 64-row aligned inside the bitmap */
+        nm_words = ((base & 63) == 0);
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+        {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+            {
+                nmw[st][i] = ~0ULL;
+                const uint64_t nm_off = nmoff[st];
+                const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
+                if (st < nstage && nm_off && nm_words && (nm_off & 7) == 0 &&
+                    byte + 8 <= nmlen[st])
+                    nmw[st][i] = *(const XCONST uint64_t *)(uintptr_t)(kbase + nm_off + byte);
+            }
+        }
+    }
+
+    /* the staged registers of row base + 64*i + lane */
+    __device__ __forceinline__ void
+    fill(Lane &L, int i, uint64_t row, bool valid) const
+    {
+        const int lane = __lane_id();
+#pragma unroll
+        for (int st = 0; st < XP_MAX_STAGE; st++)
+        {
+            if (st < nstage)
+            {
+                const int ww = w[st];
+                const uint32_t l = pick_u32<RPT>(lo[st], i);
+                const uint32_t h = pick_u32<RPT>(hi[st], i);
+                bool isnull = !valid || (uint64_t)ww * (row + 1) > vlen[st];
+                const uint64_t nm_off = nmoff[st];
+                if (nm_off)
+                {
+                    const uint64_t idx = row >> 3;
+                    const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
+                    if (nm_words && (nm_off & 7) == 0 && byte + 8 <= nmlen[st])
+                        isnull = isnull || !((nmw[st][i] >> lane) & 1);
+                    else
+                        isnull = isnull || idx >= nmlen[st] ||
+                                 !(((((const uint8_t *)kbase)[nm_off + idx]) >> (row & 7)) & 1);
+                }
+                int64_t v = (ww == 8) ? (int64_t)(((uint64_t)h << 32) | l)
+                                      : (f32[st] ? (int64_t)l : (int64_t)(int32_t)l);
+                L.R[reg[st]] = isnull ? 0 : v;
+                SETNULL(L, reg[st], isnull);
+            }
+        }
+    }
+};
+
+/* ------------------------------------------------------------------
  * decoupled look-back over tiles (thread 0 of the block)
  * ------------------------------------------------------------------ */
 __device__ __forceinline__ uint64_t
@@ -1940,7 +2144,6 @@ lb_pack(uint64_t flag, uint64_t rows, uint64_t bytes)
 /* ------------------------------------------------------------------
  * the pipeline kernel
  * ------------------------------------------------------------------ */
-enum { PH_SCAN = 0, PH_HASH, PH_JOIN, PH_OJ, PH_EMIT };
 #define IT_END      1U          /* chain exhausted (offsets are 8-aligned, never 1) */
 
 /* how a projection launch finds its tiles' output offsets */
@@ -1951,6 +2154,49 @@ enum
     PM_WRITE = 2,       /* write pass: offsets from the scanned tile totals */
 };
 
+
+/* a lane's registers before a row: all NULL but R0, and TRUE */
+template <int NR>
+__device__ __forceinline__ void
+lane_reset(Lane &L)
+{
+    L.nmask = ~1U;
+    L.err = 0;
+    L.dead = 0;
+#pragma unroll
+    for (int i = 0; i < NR; i++)
+        L.R[i] = 0;
+    L.R[XP_REG_TRUE] = 1;
+}
+
+/* The scan quals of one row and, for a row that passes them and with
+ * 'hash', the depth-1 key hash (*h); an error is reported and fails the
+ * row. */
+template <int NR, class PROG, bool STAGED>
+__device__ __forceinline__ bool
+screen_row(cplan *P, const xpu_kparams &kp, Lane &L, uint32_t row, bool hash, uint32_t *h)
+{
+    PROG::template run<STAGED>(P, P->seg_scan.begin, P->seg_scan.end, L, kp.kds_src, row, kp);
+    if (L.err)
+    {
+        record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+        return false;
+    }
+    if (RNULL(L, P->r_scan_ok) || L.R[P->r_scan_ok] == 0)
+        return false;
+    if (hash)
+    {
+        PROG::template run<STAGED>(P, P->depth[1].seg_hash.begin, P->depth[1].seg_hash.end, L,
+                                   kp.kds_src, row, kp);
+        if (L.err)
+        {
+            record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+            return false;
+        }
+        *h = (uint32_t)L.R[P->r_hash];
+    }
+    return true;
+}
 
 /*
  * The pipeline kernel: scan quals, the joins of every depth, then either the
@@ -2016,128 +2262,89 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
     bool stop_writes = false;
 
     /* one outer row through the joins and the final stage; 'queued' rows
-     * passed the scan quals (counted) and the depth-1 filter already */
+     * passed the scan quals (counted) and the depth-1 filter already.
+     * A sweep runs, for depth 1..nrels, the bucket walk, the join quals of
+     * the current candidate and the outer-join fill, then the final stage;
+     * the wave sweeps again while a lane still has candidates.  The
+     * specialized kernel unrolls the depths (nrels is a constant there), so
+     * every segment and chain offset is addressed statically. */
     auto run_row = [&](Lane &L, const uint32_t row, const bool valid, const int pass,
                        const bool queued) {
-        L.nmask = ~1U;      /* every register NULL but R0 */
-        L.err = 0;
-        L.dead = 0;
-#pragma unroll
-        for (int i = 0; i < NR; i++)
-            L.R[i] = 0;
-        L.R[XP_REG_TRUE] = 1;
-
+        lane_reset<NR>(L);
         int cd = 0;
         uint32_t it[XP_MAX_DEPTH + 2];   /* chain offsets (inner KDS < 4GB) */
         uint32_t matched = 0;
+#pragma unroll
         for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
             it[i] = 0;
 
-        int phase = PH_SCAN;
-        int d = 0;
-        for (;;)
-        {
-            xseg seg;
-            bool act = false;
-            seg.begin = seg.end = 0;
-            /* ---- pre ---- */
-            switch (phase)
-            {
-                case PH_SCAN:
-                    seg = P->seg_scan;
-                    act = valid;
-                    break;
-                case PH_HASH:
-                    seg = P->depth[d].seg_hash;
-                    act = (cd == d && it[d] == 0);
-                    break;
-                case PH_JOIN:
-                    seg = P->depth[d].seg_join;
-                    act = (cd == d && it[d] > IT_END);
-                    if (act)
-                    {
-                        kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
-                        L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, it[d])->t;
-                        SETNULL(L, P->r_tuple, 0);
-                    }
-                    break;
-                case PH_OJ:
-                    seg = P->depth[d].seg_oj;
-                    act = (cd == d && it[d] == IT_END && P->depth[d].left_outer &&
-                           !((matched >> d) & 1));
-                    if (act)
-                    {
-                        L.R[P->r_tuple] = 0;
-                        SETNULL(L, P->r_tuple, 1);
-                    }
-                    break;
-                case PH_EMIT:
-                    seg = P->seg_final;
-                    act = (cd == nrels + 1);
-                    break;
-            }
-            if (act)
-                PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
-            /* an error ends the lane's row and is reported for the task */
-            bool lane_err = false;
-            if (act && L.err)
+        /* an error ends the lane's row and is reported for the task */
+        auto run_seg = [&](const xseg seg, const bool act) -> bool {
+            if (!act)
+                return false;
+            PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
+            if (L.err)
             {
                 record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
                 L.err = 0;
-                lane_err = true;
+                return true;
             }
-            /* ---- post ---- */
-            if (phase == PH_SCAN)
+            return false;
+        };
+        /* drops the exhausted depths d, d-1, ... of the current candidate */
+        auto unwind = [&](const int d) {
+            for (int dd = d; dd >= 1; dd--)
             {
-                bool ok = act && !lane_err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
-                if (ok && pass == 0 && COUNTS && !queued)
-                    cnt_in++;
-                cd = ok ? 1 : 0;
-                if (nrels > 0)
+                if (dd == cd && it[dd] == IT_END)
                 {
-                    phase = PH_HASH;
-                    d = 1;
+                    matched &= ~(1U << dd);
+                    it[dd] = 0;
+                    cd--;
                 }
-                else
-                    phase = PH_EMIT;
             }
-            else if (phase == PH_HASH)
+        };
+        /* the next item of inner relation d whose hash is h, from offset off */
+        auto chain_find = [&](const int d, uint64_t off, const uint32_t h) -> uint32_t {
+            kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+            uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
+            while (off != 0 && off != ~0ULL && off < ik->length &&
+                   hashitem_at(ik, off)->t.hash != h && steps-- > 0)
+                off = hashitem_at(ik, off)->next;
+            return (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+        };
+        auto depth_step = [&](const int d) {
+            kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+            /* bucket walk to the first candidate */
             {
-                if (act)
+                const bool act = (cd == d && it[d] == 0);
+                const bool err = run_seg(P->depth[d].seg_hash, act);
+                if (act && !err)
                 {
-                    kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
-                    uint32_t h = (uint32_t)L.R[P->r_hash];
+                    const uint32_t h = (uint32_t)L.R[P->r_hash];
                     const uint64_t *bf = kp.bloom[d];
-                    uint64_t off = 0;
+                    uint64_t head = 0;
                     if (!bf || (queued && d == 1) || bloom_maybe(bf, kp.bloom_mask[d], h))
-                    {
-                        const uint64_t *hs = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
-                        off = hs[h % ik->hash_nslots];
-                        uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
-                        while (off != 0 && off != ~0ULL && off < ik->length &&
-                               hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                            off = hashitem_at(ik, off)->next;
-                    }
-                    it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+                        head = ((const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik)))[h % ik->hash_nslots];
+                    it[d] = chain_find(d, head, h);
                 }
-                phase = PH_JOIN;
+                if (err)
+                    cd = 0;
             }
-            else if (phase == PH_JOIN)
+            /* join quals of the candidate; the next candidate is found now so
+             * that exhausted chains unwind without another sweep */
             {
+                const bool act = (cd == d && it[d] > IT_END);
                 if (act)
                 {
-                    kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+                    L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, it[d])->t;
+                    SETNULL(L, P->r_tuple, 0);
+                }
+                const bool err = run_seg(P->depth[d].seg_join, act);
+                if (act)
+                {
                     const kern_hashitem *cur = hashitem_at(ik, it[d]);
-                    uint32_t h = cur->t.hash;       /* == the probe hash */
-                    int64_t status = lane_err ? 0 : L.R[P->r_status];
-                    /* next candidate with the same hash, found now so that
-                     * exhausted chains unwind without another pass */
-                    uint64_t off = cur->next;
-                    uint32_t steps = ik->nitems;
-                    while (off != 0 && off != ~0ULL && off < ik->length &&
-                           hashitem_at(ik, off)->t.hash != h && steps-- > 0)
-                        off = hashitem_at(ik, off)->next;
-                    it[d] = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+                    const int64_t status = err ? 0 : L.R[P->r_status];
+                    it[d] = chain_find(d, cur->next, cur->t.hash);
                     if (status != 0)
                         matched |= 1U << d;
                     if (status > 0)
@@ -2145,116 +2352,119 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                         if (pass == 0 && COUNTS)
                             atomicAdd(&s_depth_out[d], 1U);
                         cd = d + 1;
-                        if (d + 1 <= nrels)
-                            it[d + 1] = 0;
+                        it[d + 1] = 0;
                     }
                 }
-                phase = PH_OJ;
+                if (err)
+                    cd = 0;
             }
-            else if (phase == PH_OJ)
+            /* outer-join fill of a row without matches */
             {
+                const bool act = (cd == d && it[d] == IT_END && P->depth[d].left_outer &&
+                                  !((matched >> d) & 1));
+                if (act)
+                {
+                    L.R[P->r_tuple] = 0;
+                    SETNULL(L, P->r_tuple, 1);
+                }
+                const bool err = run_seg(P->depth[d].seg_oj, act);
                 bool adv = false;
                 if (act)
                 {
                     matched |= 1U << d;
-                    if (!lane_err && L.R[P->r_status] > 0)
+                    if (!err && L.R[P->r_status] > 0)
                     {
                         if (pass == 0 && COUNTS)
                             atomicAdd(&s_depth_out[d], 1U);
                         cd = d + 1;
-                        if (d + 1 <= nrels)
-                            it[d + 1] = 0;
+                        it[d + 1] = 0;
                         adv = true;
                     }
                 }
                 if (!adv && cd == d && it[d] == IT_END)
+                    unwind(d);
+                if (err)
+                    cd = 0;
+            }
+        };
+
+        /* scan quals */
+        {
+            const bool err = run_seg(P->seg_scan, valid);
+            const bool ok = valid && !err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
+            if (ok && pass == 0 && COUNTS && !queued)
+                cnt_in++;
+            cd = ok ? 1 : 0;
+        }
+        for (;;)
+        {
+            if constexpr (PROG::kUnrollRows)
+            {
+#pragma unroll
+                for (int d = 1; d <= XP_MAX_DEPTH; d++)
+                    if (d <= nrels)
+                        depth_step(d);
+            }
+            else
+            {
+                for (int d = 1; d <= nrels; d++)
+                    depth_step(d);
+            }
+            /* the final stage */
+            const bool act = (cd == nrels + 1);
+            const bool err = run_seg(P->seg_final, act);
+            if (act && !err)
+            {
+                if (pass == 0 && COUNTS)
+                    cnt_out++;
+                if (PREAGG)
                 {
-                    /* unwind exhausted depths (uniform indices only) */
-                    for (int dd = d; dd >= 1; dd--)
+                    bool direct = true;
+                    if (T.n > 0 && gb_row_fast(P, L))
                     {
-                        if (dd == cd && it[dd] == IT_END)
+                        int slot = lds_probe(T, gb_pack_key(P, L));
+                        if (slot >= 0)
                         {
-                            matched &= ~(1U << dd);
-                            it[dd] = 0;
-                            cd--;
+                            lds_update(P, L, T, slot);
+                            direct = false;
                         }
                     }
-                }
-                if (d < nrels)
-                {
-                    d++;
-                    phase = PH_HASH;
+                    if (direct)
+                        lds_miss++;
+                    uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
+                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
                 }
                 else
-                    phase = PH_EMIT;
+                {
+                    uint32_t sz = (uint32_t)xalign(8, proj_tuple_size(P, L));
+                    if (pass == 0)
+                    {
+                        lane_cnt++;
+                        lane_bytes += sz;
+                    }
+                    else if (!stop_writes)
+                    {
+                        kern_data_store *dst = kp.kds_out;
+                        lane_wr_bytes += sz;
+                        uint64_t usage = s_base_bytes + lane_wr_bytes;
+                        uint64_t rowid = s_base_rows + lane_wr_rows;
+                        char *tp = (char *)dst + dst->length - usage;
+                        proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
+                        uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
+                        rowidx[rowid] = usage;
+                        lane_wr_rows++;
+                    }
+                }
             }
-            else    /* PH_EMIT */
+            if (act)
             {
-                if (act && !lane_err)
-                {
-                    if (pass == 0 && COUNTS)
-                        cnt_out++;
-                    if (PREAGG)
-                    {
-                        bool direct = true;
-                        if (T.n > 0 && gb_row_fast(P, L))
-                        {
-                            int slot = lds_probe(T, gb_pack_key(P, L));
-                            if (slot >= 0)
-                            {
-                                lds_update(P, L, T, slot);
-                                direct = false;
-                            }
-                        }
-                        if (direct)
-                            lds_miss++;
-                        uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-                        gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
-                    }
-                    else
-                    {
-                        uint32_t sz = (uint32_t)xalign(8, proj_tuple_size(P, L));
-                        if (pass == 0)
-                        {
-                            lane_cnt++;
-                            lane_bytes += sz;
-                        }
-                        else if (!stop_writes)
-                        {
-                            kern_data_store *dst = kp.kds_out;
-                            lane_wr_bytes += sz;
-                            uint64_t usage = s_base_bytes + lane_wr_bytes;
-                            uint64_t rowid = s_base_rows + lane_wr_rows;
-                            char *tp = (char *)dst + dst->length - usage;
-                            proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
-                            uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
-                            rowidx[rowid] = usage;
-                            lane_wr_rows++;
-                        }
-                    }
-                }
-                if (act)
-                {
-                    cd = nrels;
-                    for (int dd = nrels; dd >= 1; dd--)
-                    {
-                        if (dd == cd && it[dd] == IT_END)
-                        {
-                            matched &= ~(1U << dd);
-                            it[dd] = 0;
-                            cd--;
-                        }
-                    }
-                }
-                if (lane_err)
-                    cd = 0;
-                if (!wave_any(cd > 0))
-                    break;
-                phase = PH_HASH;
-                d = 1;
+                cd = nrels;
+                unwind(nrels);
             }
-            if (lane_err)
+            if (err)
                 cd = 0;
+            if (!wave_any(cd > 0))
+                break;
         }
     };
 
@@ -2296,54 +2506,32 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
         {
             /* screen up to XQ_BATCHES row batches into the wave's queue, then
              * run the joins over the queued rows 64 at a time */
+            const uint64_t tile_base = (uint64_t)kp.row_begin + (uint64_t)tile * kp.rows_per_tile;
             for (uint32_t r0 = 0; r0 < rpt; r0 += XQ_BATCHES)
             {
-                const uint32_t r1 = rpt - r0 < XQ_BATCHES ? rpt : r0 + XQ_BATCHES;
+                const uint32_t nb = rpt - r0 < XQ_BATCHES ? rpt - r0 : XQ_BATCHES;
                 uint32_t q_count = 0;      /* wave-uniform */
-                for (uint32_t ri = r0; ri < r1; ri++)
-                {
-                    const uint64_t row64 = (uint64_t)kp.row_begin + (uint64_t)tile * kp.rows_per_tile +
-                                           (uint64_t)ri * blockDim.x + tid;
-                    const uint32_t row = (uint32_t)row64;
-                    int64_t Rfile[NR];
-                    Lane L;
-                    L.R = Rfile;
-                    L.nmask = ~1U;
-                    L.err = 0;
-                    L.dead = 0;
-#pragma unroll
-                    for (int i = 0; i < NR; i++)
-                        L.R[i] = 0;
-                    L.R[XP_REG_TRUE] = 1;
-                    bool cand = false;
-                    if (row64 < kp.row_end)
-                    {
-                        PROG::template run<false>(P, P->seg_scan.begin, P->seg_scan.end, L, kds, row, kp);
-                        bool ok = false;
-                        if (L.err)
-                            record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
-                        else
-                            ok = !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
-                        if (ok && COUNTS)
-                            cnt_in++;
-                        if (ok && qbloom)
-                        {
-                            PROG::template run<false>(P, P->depth[1].seg_hash.begin,
-                                                      P->depth[1].seg_hash.end, L, kds, row, kp);
-                            if (L.err)
-                            {
-                                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
-                                ok = false;
-                            }
-                            else
-                                ok = bloom_maybe(kp.bloom[1], kp.bloom_mask[1], (uint32_t)L.R[P->r_hash]);
-                        }
-                        cand = ok;
-                    }
+                auto enqueue = [&](const bool cand, const uint32_t row) {
                     const uint64_t m = __ballot(cand);
                     if (cand)
                         wq[q_count + (uint32_t)__popcll(m & ((1ULL << lane) - 1))] = row;
                     q_count += (uint32_t)__popcll(m);
+                };
+                for (uint32_t ri = r0; ri < r0 + nb; ri++)
+                {
+                    const uint64_t row64 = tile_base + (uint64_t)ri * blockDim.x + tid;
+                    const bool valid = row64 < kp.row_end;
+                    int64_t Rfile[NR];
+                    Lane L;
+                    L.R = Rfile;
+                    lane_reset<NR>(L);
+                    uint32_t h = 0;
+                    bool ok = valid && screen_row<NR, PROG, false>(P, kp, L, (uint32_t)row64, qbloom, &h);
+                    if (ok && COUNTS)
+                        cnt_in++;
+                    if (ok && qbloom)
+                        ok = bloom_maybe(kp.bloom[1], kp.bloom_mask[1], h);
+                    enqueue(ok, (uint32_t)row64);
                 }
                 __builtin_amdgcn_wave_barrier();
                 for (uint32_t q0 = 0; q0 < q_count; q0 += 64)
@@ -2531,6 +2719,331 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
 }
 
 /* ------------------------------------------------------------------
+ * multi-pass joins of a group-by
+ *
+ * The rows of a slice are screened by one kernel (the scan quals and the
+ * depth-1 Bloom filter) into level-1 records; pass d then joins the level-d
+ * records with inner relation d and writes level-(d+1) records, or, at the
+ * last depth, runs the final stage into the group-by.  Each pass is a
+ * launch of its own, so it holds only its own registers and runs at the
+ * occupancy they allow, and its bucket walks run over dense batches of 64
+ * records instead of over the few lanes of a wave whose rows survived.
+ * The screen reads only the columns its own program needs (the specialized
+ * kernel drops the other loads); a pass reads the rest for the rows that
+ * reach it.
+ *
+ * A level-d record holds the row and the matched item of each depth below d
+ * (0: the NULL row of an outer join).  A pass restores a record's registers
+ * by running the scan program for the row and the join program of each
+ * depth below d with its recorded item: every register is then what the
+ * single-kernel pipeline would hold at depth d.
+ *
+ * Records are column-major within per-block regions: block b of every pass
+ * reads region b of its input and appends to region b of its output, so
+ * appends reserve room with LDS atomics only.  A region of level d+1 holds
+ * as many records as region b of level d times the most items that share a
+ * join hash at depth d, so it cannot overflow (a full region is reported,
+ * never truncated).
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t *
+rec_word(uint32_t *base, uint32_t nwords, uint32_t cap, uint32_t b, uint32_t k, uint32_t r)
+{
+    return base + ((uint64_t)b * nwords + k) * cap + r;
+}
+
+/* The screen pass: one 64-row batch per wave step (static assignment of
+ * the batches to blocks and waves), the scan quals and the depth-1 filter,
+ * and the append of the rows that pass.  Few registers: a wide block keeps
+ * many waves in flight. */
+template <int NR, class PROG>
+__device__ __forceinline__ void
+join_screen_body(cplan *P, const xpu_kparams &kp)
+{
+    __shared__ uint32_t s_cnt;
+    xpu_task_ctl *ctl = kp.ctl;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);     /* uniform */
+    const int nwaves = blockDim.x >> 6;
+    const uint32_t b = blockIdx.x;
+    const bool bloom = kp.prefilter && kp.bloom[1] != nullptr && !P->depth[1].left_outer;
+
+    if (tid == 0)
+        s_cnt = 0;
+    __syncthreads();
+    uint32_t cnt_in = 0;
+    bool full = false;
+    const uint64_t step = (uint64_t)gridDim.x * nwaves * 64;
+    for (uint64_t r0 = (uint64_t)kp.row_begin + ((uint64_t)b * nwaves + wid) * 64;
+         r0 < kp.row_end; r0 += step)
+    {
+        const uint64_t row = r0 + lane;
+        const bool valid = row < kp.row_end;
+        int64_t Rfile[NR];
+        Lane L;
+        L.R = Rfile;
+        lane_reset<NR>(L);
+        uint32_t h = 0;
+        bool ok = valid && screen_row<NR, PROG, false>(P, kp, L, (uint32_t)row, bloom, &h);
+        if (ok)
+            cnt_in++;
+        if (ok && bloom)
+            ok = bloom_maybe(kp.bloom[1], kp.bloom_mask[1], h);
+        const uint64_t m = __ballot(ok);
+        if (!m)
+            continue;
+        uint32_t pos0 = 0;
+        if (lane == 0)
+            pos0 = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+        pos0 = __shfl(pos0, 0);
+        const uint32_t r = pos0 + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+        if (ok && r >= kp.rec_out_cap)
+            full = true;
+        else if (ok)
+            *rec_word(kp.rec_out, 1, kp.rec_out_cap, b, 0, r) = (uint32_t)row;
+    }
+    if (full)
+        record_error(ctl, XMSG_JOIN_RECORDS, ERRCODE_DEVICE_ERROR);
+    if (cnt_in)
+        atomicAdd(&ctl->nitems_in, cnt_in);
+    __syncthreads();
+    if (tid == 0)
+        kp.rec_out_cnt[b] = s_cnt < kp.rec_out_cap ? s_cnt : kp.rec_out_cap;
+}
+
+/* The join pass of depth D (0: kp.pass_depth) over the records of block
+ * b's region: restore each record's registers, walk its bucket chain, run
+ * the join quals of every candidate, then append a level-(D+1) record per
+ * match (after the next depth's filter) or, at the last depth, run the
+ * final stage into the group-by. */
+template <int NR, class PROG, int D>
+__device__ __forceinline__ void
+join_probe_body(cplan *P, const xpu_kparams &kp)
+{
+    extern __shared__ unsigned long long lds_dyn[];
+    __shared__ uint32_t s_cnt;
+    __shared__ uint32_t s_depth_out;
+    const kern_data_store *__restrict__ kds = kp.kds_src;
+    xpu_task_ctl *ctl = kp.ctl;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);     /* uniform */
+    const int nwaves = blockDim.x >> 6;
+    const uint32_t b = blockIdx.x;
+    const int d = D ? D : (int)kp.pass_depth;
+    const int nrels = P->nrels;
+    const bool last = (d == nrels);
+    const uint32_t nrec = kp.rec_in_cnt[b] < kp.rec_in_cap ? kp.rec_in_cnt[b] : kp.rec_in_cap;
+    const uint32_t nw_in = (uint32_t)d;     /* row, items of depths 1..d-1 */
+    const uint32_t nw_out = nw_in + 1;
+
+    /* the group-by table pays its init and merge only for a block with more
+     * records than entries */
+    LdsTable T = lds_table_at(P, lds_dyn, (last && nrec > kp.lds_nents) ? kp.lds_nents : 0);
+    if (T.n > 0)
+        lds_init(P, T);
+    if (tid == 0)
+    {
+        s_cnt = 0;
+        s_depth_out = 0;
+    }
+    __syncthreads();
+
+    kern_data_store *ik = (kern_data_store *)(uintptr_t)kp.inner_kds[d];
+    const bool left_outer = P->depth[d].left_outer;
+    const uint64_t *bf = (d == 1 && kp.screened) ? nullptr : kp.bloom[d];
+    const bool nbloom = !last && kp.prefilter && kp.bloom[d + 1] != nullptr &&
+                        !P->depth[d + 1].left_outer;
+    uint32_t cnt_out = 0, lds_miss = 0, depth_out = 0;
+    bool full = false;
+
+    for (uint32_t r0 = (uint32_t)wid * 64; r0 < nrec; r0 += (uint32_t)nwaves * 64)
+    {
+        const uint32_t r = r0 + lane;
+        const bool valid = r < nrec;
+        int64_t Rfile[NR];
+        Lane L;
+        L.R = Rfile;
+        lane_reset<NR>(L);
+        uint32_t row = 0;
+        bool alive = valid;
+        /* an error ends the record and is reported for the task */
+        auto run_seg = [&](const xseg seg, const bool act) -> bool {
+            if (!act)
+                return false;
+            PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
+            if (L.err)
+            {
+                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                L.err = 0;
+                return true;
+            }
+            return false;
+        };
+        /* the record's registers: the scan program for its row, then the
+         * join program of each depth below d with its recorded item */
+        if (valid)
+            row = *rec_word(kp.rec_in, nw_in, kp.rec_in_cap, b, 0, r);
+        alive = alive && !run_seg(P->seg_scan, alive);
+        for (int j = 1; j < XP_MAX_DEPTH && j < d; j++)
+        {
+            const uint32_t itj = valid ? *rec_word(kp.rec_in, nw_in, kp.rec_in_cap, b, j, r) : 0;
+            if (alive)
+            {
+                kern_data_store *jk = (kern_data_store *)(uintptr_t)kp.inner_kds[j];
+                L.R[P->r_tuple] = itj ? (int64_t)(uintptr_t)&hashitem_at(jk, itj)->t : 0;
+                SETNULL(L, P->r_tuple, itj == 0);
+            }
+            const bool e1 = run_seg(P->depth[j].seg_join, alive && itj != 0);
+            const bool e2 = run_seg(P->depth[j].seg_oj, alive && itj == 0);
+            alive = alive && !e1 && !e2;
+        }
+        /* bucket walk to the first candidate */
+        uint32_t h = 0;
+        alive = alive && !run_seg(P->depth[d].seg_hash, alive);
+        if (alive)
+            h = (uint32_t)L.R[P->r_hash];
+        uint64_t off = 0;
+        if (alive && (!bf || bloom_maybe(bf, kp.bloom_mask[d], h)))
+            off = ((const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik)))[h % ik->hash_nslots];
+        uint32_t steps = ik->nitems;    /* chains are acyclic: bound the walk */
+        while (off != 0 && off != ~0ULL && off < ik->length &&
+               hashitem_at(ik, off)->t.hash != h && steps-- > 0)
+            off = hashitem_at(ik, off)->next;
+        uint32_t cur = (off == 0 || off == ~0ULL || off >= ik->length) ? IT_END : (uint32_t)off;
+        bool matched = false;
+
+        for (;;)
+        {
+            const bool join = alive && cur > IT_END;
+            const bool fill = alive && !join && left_outer && !matched;
+            if (!wave_any(join || fill))
+                break;
+            /* the join quals of the candidate, or the outer-join NULL row */
+            bool emit = false;
+            uint32_t emit_it = 0;
+            if (join)
+            {
+                L.R[P->r_tuple] = (int64_t)(uintptr_t)&hashitem_at(ik, cur)->t;
+                SETNULL(L, P->r_tuple, 0);
+            }
+            else if (fill)
+            {
+                L.R[P->r_tuple] = 0;
+                SETNULL(L, P->r_tuple, 1);
+            }
+            const bool ej = run_seg(P->depth[d].seg_join, join);
+            const bool eo = run_seg(P->depth[d].seg_oj, fill);
+            if (join)
+            {
+                const kern_hashitem *hi = hashitem_at(ik, cur);
+                const int64_t status = ej ? 0 : L.R[P->r_status];
+                if (status != 0)
+                    matched = true;
+                emit = status > 0;
+                emit_it = cur;
+                /* the next candidate with the same hash */
+                uint64_t nx = hi->next;
+                uint32_t ns = ik->nitems;
+                while (nx != 0 && nx != ~0ULL && nx < ik->length &&
+                       hashitem_at(ik, nx)->t.hash != h && ns-- > 0)
+                    nx = hashitem_at(ik, nx)->next;
+                cur = (nx == 0 || nx == ~0ULL || nx >= ik->length) ? IT_END : (uint32_t)nx;
+            }
+            if (fill)
+            {
+                matched = true;
+                emit = !eo && L.R[P->r_status] > 0;
+                emit_it = 0;
+            }
+            if (ej || eo)
+            {
+                alive = false;
+                emit = false;
+            }
+            if (emit)
+                depth_out++;
+            if (last)
+            {
+                const bool fin = emit && !run_seg(P->seg_final, emit);
+                if (fin)
+                {
+                    cnt_out++;
+                    bool direct = true;
+                    if (T.n > 0 && gb_row_fast(P, L))
+                    {
+                        int slot = lds_probe(T, gb_pack_key(P, L));
+                        if (slot >= 0)
+                        {
+                            lds_update(P, L, T, slot);
+                            direct = false;
+                        }
+                    }
+                    if (direct)
+                        lds_miss++;
+                    uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
+                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+                }
+            }
+            else
+            {
+                /* the next depth's filter, then the append */
+                bool out = emit;
+                if (out && nbloom)
+                {
+                    if (run_seg(P->depth[d + 1].seg_hash, true))
+                    {
+                        out = false;
+                        alive = false;
+                    }
+                    else
+                        out = bloom_maybe(kp.bloom[d + 1], kp.bloom_mask[d + 1], (uint32_t)L.R[P->r_hash]);
+                }
+                const uint64_t m = __ballot(out);
+                if (m)
+                {
+                    uint32_t pos0 = 0;
+                    if (lane == 0)
+                        pos0 = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+                    pos0 = __shfl(pos0, 0);
+                    const uint32_t q = pos0 + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
+                    if (out && q >= kp.rec_out_cap)
+                        full = true;
+                    else if (out)
+                    {
+                        for (uint32_t k = 0; k < nw_in; k++)
+                            *rec_word(kp.rec_out, nw_out, kp.rec_out_cap, b, k, q) =
+                                *rec_word(kp.rec_in, nw_in, kp.rec_in_cap, b, k, r);
+                        *rec_word(kp.rec_out, nw_out, kp.rec_out_cap, b, nw_in, q) = emit_it;
+                    }
+                }
+            }
+        }
+    }
+    if (full)
+        record_error(ctl, XMSG_JOIN_RECORDS, ERRCODE_DEVICE_ERROR);
+    if (depth_out)
+        atomicAdd(&s_depth_out, depth_out);
+    if (T.n > 0)
+    {
+        __syncthreads();
+        lds_flush<NR>(P, T, kp.kds_out, ctl);
+    }
+    if (cnt_out)
+        atomicAdd(&ctl->nitems_out, cnt_out);
+    if (lds_miss)
+        atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
+    __syncthreads();
+    if (tid == 0)
+    {
+        if (!last)
+            kp.rec_out_cnt[b] = s_cnt < kp.rec_out_cap ? s_cnt : kp.rec_out_cap;
+        if (s_depth_out)
+            atomicAdd(&ctl->depth_out[d], s_depth_out);
+    }
+}
+
+/* ------------------------------------------------------------------
  * scan + group-by kernel for plans without joins
  *
  * Each wave owns tiles of 64*RPT consecutive rows (static assignment, so
@@ -2547,17 +3060,6 @@ spill_fill_holes(const xpu_kparams &kp, uint64_t pos, uint64_t end, int rec_word
 {
     for (uint64_t i = pos + __lane_id(); i < end; i += 64)
         kp.spill[i * rec_words] = XLDS_EMPTY_KEY;
-}
-
-template <int RPT>
-__device__ __forceinline__ uint32_t
-pick_u32(const uint32_t (&a)[RPT], int i)
-{
-    uint32_t v = a[0];
-#pragma unroll
-    for (int k = 1; k < RPT; k++)
-        v = (i == k) ? a[k] : v;
-    return v;
 }
 
 /* counters of one wave over the ranges it processed */
@@ -2577,38 +3079,10 @@ scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
                uint64_t wt_stride, ScanAggWave &W)
 {
     const kern_data_store *__restrict__ kds = kp.kds_src;
-    const char *kbase = (const char *)kds;
-    ckds *kmeta = (ckds *)(uintptr_t)kds;
     xpu_task_ctl *ctl = kp.ctl;
     const int lane = __lane_id();
-    const int nstage = (int)P->nstage;
-
-    /* staged columns' geometry, loaded once (uniform: scalar registers) */
-    const char *s_vbase[XP_MAX_STAGE];
-    uint64_t s_vlen[XP_MAX_STAGE], s_nmoff[XP_MAX_STAGE], s_nmlen[XP_MAX_STAGE];
-    int s_w[XP_MAX_STAGE], s_reg[XP_MAX_STAGE];
-    bool s_f32[XP_MAX_STAGE];
-#pragma unroll
-    for (int st = 0; st < XP_MAX_STAGE; st++)
-    {
-        s_vbase[st] = kbase;
-        s_vlen[st] = s_nmoff[st] = s_nmlen[st] = 0;
-        s_w[st] = 1;
-        s_reg[st] = 0;
-        s_f32[st] = false;
-        if (st < nstage)
-        {
-            const xstage sd = P->stage[st];
-            ccolmeta *cm = &kmeta->colmeta[sd.col];
-            s_vbase[st] = kbase + cm->values_offset;
-            s_vlen[st] = cm->values_length;
-            s_nmoff[st] = cm->nullmap_offset;
-            s_nmlen[st] = cm->nullmap_length;
-            s_w[st] = sd.width;
-            s_reg[st] = sd.reg;
-            s_f32[st] = sd.f32 != 0;
-        }
-    }
+    StagedCols<RPT> S;
+    S.setup(P, kds);
     const int pc_begin = P->seg_scan.begin + P->stage_prefix;
     const int pc_end = P->seg_final.end;
 
@@ -2629,108 +3103,7 @@ scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
             __hip_atomic_load(&ctl->errcode, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
             break;
         const uint64_t base = (uint64_t)kp.row_begin + wt * wrows;
-
-        /* prologue: raw words of the staged columns.  A tile that lies
-         * inside every staged column (all but the last tile) loads without
-         * per-lane bounds checks, so no load waits on another; the tail tile
-         * checks each element. */
-        bool full = base + wrows <= kp.row_end;
-#pragma unroll
-        for (int st = 0; st < XP_MAX_STAGE; st++)
-            if (st < nstage)
-                full = full && (uint64_t)s_w[st] * (base + wrows) <= s_vlen[st];
-        uint32_t lo[XP_MAX_STAGE][RPT], hi[XP_MAX_STAGE][RPT];
-#pragma unroll
-        for (int st = 0; st < XP_MAX_STAGE; st++)
-        {
-#pragma unroll
-            for (int i = 0; i < RPT; i++)
-            {
-                lo[st][i] = 0;
-                hi[st][i] = 0;
-            }
-        }
-        if (full)
-        {
-#pragma unroll
-            for (int st = 0; st < XP_MAX_STAGE; st++)
-            {
-                if (st < nstage)
-                {
-                    const char *vbase = s_vbase[st];
-                    const int w = s_w[st];
-#pragma unroll
-                    for (int i = 0; i < RPT; i++)
-                    {
-                        const char *p = vbase + (uint64_t)w * (base + (uint64_t)i * 64 + lane);
-                        if (w == 8)
-                        {
-                            uint2 u = *(const uint2 *)p;
-                            lo[st][i] = u.x;
-                            hi[st][i] = u.y;
-                        }
-                        else if (w == 4)
-                            lo[st][i] = *(const uint32_t *)p;
-                        else if (w == 2)
-                            lo[st][i] = (uint32_t)(int32_t)*(const int16_t *)p;
-                        else
-                            lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
-                    }
-                }
-            }
-        }
-        else
-        {
-#pragma unroll
-            for (int st = 0; st < XP_MAX_STAGE; st++)
-            {
-                if (st < nstage)
-                {
-                    const char *vbase = s_vbase[st];
-                    const uint64_t vlen = s_vlen[st];
-                    const int w = s_w[st];
-#pragma unroll
-                    for (int i = 0; i < RPT; i++)
-                    {
-                        const uint64_t row = base + (uint64_t)i * 64 + lane;
-                        if (row < kp.row_end && (uint64_t)w * (row + 1) <= vlen)
-                        {
-                            const char *p = vbase + (uint64_t)w * row;
-                            if (w == 8)
-                            {
-                                uint2 u = *(const uint2 *)p;
-                                lo[st][i] = u.x;
-                                hi[st][i] = u.y;
-                            }
-                            else if (w == 4)
-                                lo[st][i] = *(const uint32_t *)p;
-                            else if (w == 2)
-                                lo[st][i] = (uint32_t)(int32_t)*(const int16_t *)p;
-                            else
-                                lo[st][i] = (uint32_t)(int32_t)*(const int8_t *)p;
-                        }
-                    }
-                }
-            }
-        }
-        /* validity bitmaps: one 64-bit word per 64 rows, read with scalar
-         * loads when the tile is 64-row aligned inside the bitmap */
-        const bool nm_words = ((base & 63) == 0);
-        uint64_t nmw[XP_MAX_STAGE][RPT];
-#pragma unroll
-        for (int st = 0; st < XP_MAX_STAGE; st++)
-        {
-#pragma unroll
-            for (int i = 0; i < RPT; i++)
-            {
-                nmw[st][i] = ~0ULL;
-                const uint64_t nm_off = s_nmoff[st];
-                const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
-                if (st < nstage && nm_off && nm_words && (nm_off & 7) == 0 &&
-                    byte + 8 <= s_nmlen[st])
-                    nmw[st][i] = *(const XCONST uint64_t *)(uintptr_t)(kbase + nm_off + byte);
-            }
-        }
+        S.load(base, kp.row_end);
 
         /* one row of the lane's tile (a lambda: the specialized kernel
          * unrolls the rows, so the staged words stay in registers; the
@@ -2749,34 +3122,7 @@ scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
             for (int r = 0; r < NR; r++)
                 L.R[r] = 0;
             L.R[XP_REG_TRUE] = 1;
-            /* staged values into their registers, with the NULL rules of
-             * XO_LDCOL (validity bitmap; values beyond the buffer) */
-#pragma unroll
-            for (int st = 0; st < XP_MAX_STAGE; st++)
-            {
-                if (st < nstage)
-                {
-                    const int w = s_w[st];
-                    const uint32_t l = pick_u32<RPT>(lo[st], i);
-                    const uint32_t h = pick_u32<RPT>(hi[st], i);
-                    bool isnull = !valid || (uint64_t)w * (row + 1) > s_vlen[st];
-                    const uint64_t nm_off = s_nmoff[st];
-                    if (nm_off)
-                    {
-                        const uint64_t idx = row >> 3;
-                        const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
-                        if (nm_words && (nm_off & 7) == 0 && byte + 8 <= s_nmlen[st])
-                            isnull = isnull || !((nmw[st][i] >> lane) & 1);
-                        else
-                            isnull = isnull || idx >= s_nmlen[st] ||
-                                     !(((((const uint8_t *)kbase)[nm_off + idx]) >> (row & 7)) & 1);
-                    }
-                    int64_t v = (w == 8) ? (int64_t)(((uint64_t)h << 32) | l)
-                                         : (s_f32[st] ? (int64_t)l : (int64_t)(int32_t)l);
-                    L.R[s_reg[st]] = isnull ? 0 : v;
-                    SETNULL(L, s_reg[st], isnull);
-                }
-            }
+            S.fill(L, i, row, valid);
             if (valid)
                 PROG::template run<true>(P, pc_begin, pc_end, L, kds, (uint32_t)row, kp);
             bool ok = valid && !L.dead;

@@ -127,6 +127,18 @@ xpu_jit_source(const xplan *P, int kind, int param)
         append(s, "    pipeline_body<%s, %d, JitProg>((cplan *)xjit_plan, kp);\n",
                param ? "true" : "false", nr);
     s += "}\n";
+    if (kind == XJ_PIPELINE && param && P->nrels > 0)
+    {
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XQ_SCREEN_BLOCK)\n"
+             "xpu_jit_screen_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    join_screen_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
+        for (uint32_t d = 1; d <= P->nrels; d++)
+        {
+            append(s, "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+                      "xpu_jit_probe_kernel_%u(xpu_kparams kp)\n{\n", d);
+            append(s, "    join_probe_body<%d, JitProg, %u>((cplan *)xjit_plan, kp);\n}\n", nr, d);
+        }
+    }
     if (kind == XJ_PIPELINE && !param)
     {
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"

@@ -28,6 +28,10 @@
 #define XQ_BATCHES          8
 #define XQ_WAVE             (64 * XQ_BATCHES)
 #define XP_PIPELINE_STATIC_LDS  ((XP_BLOCK_MAX / 64) * XQ_WAVE * 4 + 1024)
+/* multi-pass joins: threads per block of the screen pass (wide blocks: the
+ * screen holds few registers, and its blocks match the join passes' blocks
+ * one to one) */
+#define XQ_SCREEN_BLOCK     1024
 
 /* per-task control block in device memory (cleared before each launch) */
 typedef struct
@@ -54,6 +58,7 @@ typedef struct
 #define XMSG_TEXT_EXTERNAL      0xfff2
 #define XMSG_GROUP_RETRY        0xfff3
 #define XMSG_TILE_TOO_LARGE     0xfff4
+#define XMSG_JOIN_RECORDS       0xfff5
 
 /* one fragment of a batched launch */
 typedef struct
@@ -128,6 +133,16 @@ typedef struct
     const uint64_t *bloom[XP_MAX_DEPTH + 1];
     uint32_t    bloom_mask[XP_MAX_DEPTH + 1];   /* words - 1 (words: a power of two) */
     uint32_t    prefilter;              /* group-by joins: screen rows before the joins */
+    /* multi-pass joins of a group-by: the records a pass reads and writes,
+     * in per-block regions of rec_*_cap records (column-major words) */
+    uint32_t   *rec_in;
+    uint32_t   *rec_out;
+    uint32_t   *rec_in_cnt;             /* [grid] records of each region */
+    uint32_t   *rec_out_cnt;
+    uint32_t    rec_in_cap;
+    uint32_t    rec_out_cap;
+    uint32_t    pass_depth;             /* 0: screen pass; d: join with inner relation d */
+    uint32_t    screened;               /* level-1 records passed the depth-1 filter */
 } xpu_kparams;
 
 #ifdef __cplusplus

@@ -21,6 +21,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -109,6 +110,10 @@ struct Scratch
     size_t      src_cap = 0;
     char       *d_dst = nullptr;
     size_t      dst_cap = 0;
+    uint32_t   *d_rec[2] = {};          /* multi-pass join records (odd, even levels) */
+    size_t      rec_cap[2] = {};        /* bytes */
+    uint32_t   *d_rec_cnt = nullptr;    /* [2][grid] records per region */
+    size_t      rec_cnt_cap = 0;
 };
 
 bool
@@ -135,6 +140,9 @@ scratch_destroy(Scratch *s)
     if (s->d_dump_next) (void)hipFree(s->d_dump_next);
     if (s->d_src) (void)hipFree(s->d_src);
     if (s->d_dst) (void)hipFree(s->d_dst);
+    for (int i = 0; i < 2; i++)
+        if (s->d_rec[i]) (void)hipFree(s->d_rec[i]);
+    if (s->d_rec_cnt) (void)hipFree(s->d_rec_cnt);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->evm) (void)hipEventDestroy(s->evm);
@@ -174,6 +182,7 @@ struct xpuhip_session
     uint64_t       *d_bloom = nullptr;      /* Bloom filters of the inner relations */
     uint64_t        bloom_off[XP_MAX_DEPTH + 1] = {};   /* words from d_bloom */
     uint32_t        bloom_mask[XP_MAX_DEPTH + 1] = {};  /* words - 1 (0: no filter) */
+    uint32_t        maxdup[XP_MAX_DEPTH + 1] = {};      /* most items sharing a join hash */
     int64_t        *d_params = nullptr;     /* XO_PARAM values */
     bool            inner_ready = false;
     xpuhip_tuning   tuning;
@@ -382,6 +391,7 @@ message_of(const xpuhip_session *s, uint32_t msgid)
         case XMSG_TEXT_EXTERNAL: return "text datum is compressed or external";
         case XMSG_GROUP_RETRY: return "group-by slot lock retry limit exceeded";
         case XMSG_TILE_TOO_LARGE: return "one tile of rows does not fit into kds_dst";
+        case XMSG_JOIN_RECORDS: return "join records exceed their buffer";
     }
     if (msgid < s->comp.messages.size())
         return s->comp.messages[msgid].c_str();
@@ -550,6 +560,41 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
     return 0;
 }
 
+/* the most items of an inner KDS_FORMAT_HASH that share one join hash: a
+ * probe finds at most that many candidates (items of one hash share a
+ * chain, so each chain is counted on its own) */
+static uint32_t
+inner_max_dup(const kern_data_store *k)
+{
+    const uint64_t *slots = (const uint64_t *)((const char *)k + KDS_HEAD_LENGTH(k));
+    std::vector<uint32_t> hs;
+    uint32_t best = 0;
+    for (uint64_t i = 0; i < k->hash_nslots; i++)
+    {
+        hs.clear();
+        uint64_t off = slots[i];
+        uint64_t steps = (uint64_t)k->nitems + 1;
+        while (off != 0 && off != ~0ULL && off < k->length && off >= sizeof(kern_hashitem) &&
+               steps-- > 0)
+        {
+            const kern_hashitem *hi = (const kern_hashitem *)((const char *)k + k->length - off);
+            hs.push_back(hi->t.hash);
+            off = hi->next;
+        }
+        if (hs.size() <= best)
+            continue;
+        std::sort(hs.begin(), hs.end());
+        uint32_t run = 0;
+        for (size_t j = 0; j < hs.size(); j++)
+        {
+            run = (j > 0 && hs[j] == hs[j - 1]) ? run + 1 : 1;
+            if (run > best)
+                best = run;
+        }
+    }
+    return best;
+}
+
 extern "C" int
 xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, size_t length,
                             kern_errorbuf *eb)
@@ -634,6 +679,9 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
         if (!hip_ok(hipStreamSynchronize(nullptr), eb, "bloom build"))
             return ERRCODE_DEVICE_ERROR;
     }
+    for (uint32_t d = 1; d <= P->nrels; d++)
+        s->maxdup[d] = inner_max_dup((const kern_data_store *)((const char *)kmrels +
+                                                               kmrels->chunks[d - 1].kds_offset));
     if (!hip_ok(hipMemcpy(s->d_plan, P, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy(plan)"))
         return ERRCODE_DEVICE_ERROR;
     s->inner_ready = true;
@@ -647,7 +695,55 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
 /* XPUHIP_JIT_AUTO specializes chunks of at least this many rows */
 #define XPUHIP_JIT_AUTO_ROWS    1000000ULL
 
+/* device memory for the records of multi-pass joins, per task */
+#define XPU_JOIN_REC_BUDGET     (4ULL << 30)
+
 namespace {
+
+/* Sizes of multi-pass join records: the rows per slice (*slice), the
+ * per-block capacity and words of each level's records (cap[l], nw[l]: the
+ * input of the pass of depth l), and the bytes of the two buffers that the
+ * levels alternate between.  False when a slice of one grid-wide round of
+ * wave tiles would not fit the budget. */
+bool
+join_pass_sizes(const xplan *P, const uint32_t *maxdup, uint64_t nrows, uint32_t grid,
+                uint32_t nwaves, uint64_t *slice, uint32_t *cap, uint32_t *nw, size_t *bytes)
+{
+    const uint64_t wrows = 64;
+    const uint64_t round = (uint64_t)grid * nwaves * wrows;
+    /* record words per row of a slice, at the level's most matches per row
+     * (a level-l record: the row and the items of depths 1..l-1) */
+    uint64_t words[2] = {0, 0}, mult = 1;
+    for (uint32_t l = 1; l <= P->nrels; l++)
+    {
+        uint64_t w = l * mult;
+        words[(l - 1) & 1] = w > words[(l - 1) & 1] ? w : words[(l - 1) & 1];
+        mult *= maxdup[l] > 1 ? maxdup[l] : 1;
+        if (mult > (1ULL << 20))
+            return false;
+    }
+    uint64_t n = XPU_JOIN_REC_BUDGET / (4 * (words[0] + words[1]));
+    n = n / round * round;
+    if (n == 0)
+        return false;
+    if (n > nrows)
+        n = nrows;
+    const uint64_t nwt = (n + wrows - 1) / wrows;
+    uint64_t c = ((nwt + (uint64_t)grid * nwaves - 1) / ((uint64_t)grid * nwaves)) * nwaves * wrows;
+    bytes[0] = bytes[1] = 0;
+    for (uint32_t l = 1; l <= P->nrels; l++)
+    {
+        if (c > 0xffffffffULL)
+            return false;
+        cap[l] = (uint32_t)c;
+        nw[l] = l;
+        size_t b = 4ULL * grid * nw[l] * c;
+        bytes[(l - 1) & 1] = b > bytes[(l - 1) & 1] ? b : bytes[(l - 1) & 1];
+        c *= maxdup[l] > 1 ? maxdup[l] : 1;
+    }
+    *slice = n;
+    return true;
+}
 
 /* code objects of specialized kernels loaded into this process's device */
 struct JitFns
@@ -659,6 +755,8 @@ struct JitFns
     hipFunction_t   dump = nullptr;         /* partial-state dump merge */
     hipFunction_t   count = nullptr;        /* projection count pass */
     hipFunction_t   write = nullptr;        /* projection write pass */
+    hipFunction_t   screen = nullptr;       /* multi-pass joins: screen pass */
+    hipFunction_t   probe[XP_MAX_DEPTH + 1] = {};  /* multi-pass joins: pass of depth d */
 };
 std::mutex g_mod_lock;
 std::map<const std::vector<char> *, JitFns> g_mods;
@@ -700,6 +798,19 @@ jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
          !hip_ok(hipModuleGetFunction(&f.write, mod, "xpu_jit_write_kernel"), eb,
                  "hipModuleGetFunction")))
         return nullptr;
+    if (kind == XJ_PIPELINE && param && s->comp.plan.nrels > 0)
+    {
+        if (!hip_ok(hipModuleGetFunction(&f.screen, mod, "xpu_jit_screen_kernel"), eb,
+                    "hipModuleGetFunction"))
+            return nullptr;
+        for (uint32_t d = 1; d <= s->comp.plan.nrels && d <= XP_MAX_DEPTH; d++)
+        {
+            char name[64];
+            snprintf(name, sizeof(name), "xpu_jit_probe_kernel_%u", d);
+            if (!hip_ok(hipModuleGetFunction(&f.probe[d], mod, name), eb, "hipModuleGetFunction"))
+                return nullptr;
+        }
+    }
     return &(g_mods[code] = f);
 }
 
@@ -826,6 +937,28 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         kp.dump_nparts = nparts;
     }
 
+    /* a group-by over joins with the specialized kernels: a screen pass
+     * and one pass per depth, over slices of the rows sized to the record
+     * budget */
+    const bool use_jit = t.jit == XPUHIP_JIT_ON ||
+                         (t.jit != XPUHIP_JIT_OFF && nrows >= XPUHIP_JIT_AUTO_ROWS);
+    bool passes = false;
+    uint64_t slice = 0;
+    uint32_t rcap[XP_MAX_DEPTH + 2] = {}, rnw[XP_MAX_DEPTH + 2] = {};
+    if (preagg && !scan_agg && P->nrels > 0 && use_jit && t.join_passes != XPUHIP_JOIN_PASSES_OFF)
+    {
+        size_t bytes[2];
+        passes = join_pass_sizes(P, s->maxdup, nrows, grid, XQ_SCREEN_BLOCK / 64, &slice, rcap, rnw,
+                                 bytes) &&
+                 (t.join_passes == XPUHIP_JOIN_PASSES_ON || slice >= nrows || slice >= (1ULL << 20));
+        if (passes &&
+            (!grow((char **)&x->d_rec[0], &x->rec_cap[0], bytes[0], eb) ||
+             !grow((char **)&x->d_rec[1], &x->rec_cap[1], bytes[1], eb) ||
+             !grow((char **)&x->d_rec_cnt, &x->rec_cnt_cap, 8ULL * grid, eb)))
+            return ERRCODE_DEVICE_ERROR;
+        kp.screened = (kp.prefilter && kp.bloom[1] && !P->depth[1].left_outer) ? 1 : 0;
+    }
+
     int rc = 0;
     xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
     if (kp.dump)
@@ -839,10 +972,11 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     }
     /* the kernel specialized for the plan, or the interpreter */
     hipFunction_t jfn = nullptr, jsfn = nullptr, jcount = nullptr, jwrite = nullptr, jdump = nullptr;
-    if (t.jit != XPUHIP_JIT_OFF && (t.jit == XPUHIP_JIT_ON || nrows >= XPUHIP_JIT_AUTO_ROWS))
+    const JitFns *jf = nullptr;
+    if (use_jit)
     {
-        const JitFns *jf = scan_agg ? jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb)
-                                    : jit_function(s, XJ_PIPELINE, preagg ? 1 : 0, eb);
+        jf = scan_agg ? jit_function(s, XJ_SCAN_AGG, (int)(rpt >= 4 ? 4 : rpt), eb)
+                      : jit_function(s, XJ_PIPELINE, preagg ? 1 : 0, eb);
         if (jf)
         {
             jfn = jf->main;
@@ -854,6 +988,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         else if (t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
+    if (passes && !(jf && jf->screen))
+        passes = false;
     (void)hipEventRecord(x->ev0, x->stream);
     if (!preagg)
     {
@@ -872,6 +1008,38 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         }
         else
             rc = xpu_launch_projection(&kp, P->nregs, grid, block, x->stream);
+    }
+    else if (passes)
+    {
+        /* per slice: the screen pass into level-1 records (buffer 0), then
+         * the pass of each depth from level d (buffer (d-1)&1) to level d+1 */
+        for (uint64_t sb = row_begin; rc == 0 && sb < nitems; sb += slice)
+        {
+            xpu_kparams q = kp;
+            void *args[] = {&q};
+            q.row_begin = (uint32_t)sb;
+            q.row_end = (uint32_t)(sb + slice < nitems ? sb + slice : nitems);
+            q.pass_depth = 0;
+            q.rec_out = x->d_rec[0];
+            q.rec_out_cnt = x->d_rec_cnt;
+            q.rec_out_cap = rcap[1];
+            rc = (int)hipModuleLaunchKernel(jf->screen, grid, 1, 1, XQ_SCREEN_BLOCK, 1, 1, 0,
+                                            x->stream, args, nullptr);
+            for (uint32_t d = 1; rc == 0 && d <= P->nrels; d++)
+            {
+                const bool last = (d == P->nrels);
+                q.pass_depth = d;
+                q.rec_in = x->d_rec[(d - 1) & 1];
+                q.rec_in_cnt = x->d_rec_cnt + (size_t)((d - 1) & 1) * grid;
+                q.rec_in_cap = rcap[d];
+                q.rec_out = last ? nullptr : x->d_rec[d & 1];
+                q.rec_out_cnt = last ? nullptr : x->d_rec_cnt + (size_t)(d & 1) * grid;
+                q.rec_out_cap = last ? 0 : rcap[d + 1];
+                rc = (int)hipModuleLaunchKernel(jf->probe[d], grid, 1, 1, block, 1, 1,
+                                                (unsigned)(last ? lds : 0), x->stream, args,
+                                                nullptr);
+            }
+        }
     }
     else if (jfn)
     {
@@ -943,7 +1111,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_stats.lds_miss = c->lds_miss;
     t_last_stats.global_merges = c->global_merges;
     t_last_stats.spill_rows = c->spill_rows;
-    t_last_stats.jit = jfn ? 1 : 0;
+    t_last_stats.jit = jf ? 1 : 0;
     if (res)
     {
         res->nitems_raw = (uint32_t)nrows;
@@ -1136,7 +1304,7 @@ xpuhip_session_set_tuning(xpuhip_session *s, const xpuhip_tuning *t)
     if (n.lds_nents == 0) n.lds_nents = d.lds_nents;
     if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > 512 ||
         n.rows_per_thread > 64 || n.blocks_per_cu > 64 || n.jit > XPUHIP_JIT_OFF ||
-        n.join_filter > XPUHIP_JOIN_FILTER_OFF)
+        n.join_filter > XPUHIP_JOIN_FILTER_OFF || n.join_passes > XPUHIP_JOIN_PASSES_OFF)
         return ERRCODE_DEVICE_ERROR;
     if (uses_scan_agg(s) && n.rows_per_thread != 1 && n.rows_per_thread != 2 &&
         n.rows_per_thread != 4)

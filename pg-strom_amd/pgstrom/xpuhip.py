@@ -29,6 +29,7 @@ BATCH_GRAPH = 1
 
 JIT_AUTO, JIT_ON, JIT_OFF = 0, 1, 2
 JOIN_FILTER_AUTO, JOIN_FILTER_ON, JOIN_FILTER_OFF = 0, 1, 2
+JOIN_PASSES_AUTO, JOIN_PASSES_ON, JOIN_PASSES_OFF = 0, 1, 2
 
 
 class XpuError(RuntimeError):
@@ -41,7 +42,7 @@ class XpuError(RuntimeError):
 class xpuhip_tuning(C.Structure):
     _fields_ = [("block_size", C.c_uint32), ("rows_per_thread", C.c_uint32),
                 ("lds_nents", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("jit", C.c_uint32), ("join_filter", C.c_uint32)]
+                ("jit", C.c_uint32), ("join_filter", C.c_uint32), ("join_passes", C.c_uint32)]
 
 
 class xpuhip_batch_task(C.Structure):

@@ -15,22 +15,27 @@ C2_ACTIONS = [abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__VREF, abi.KAGG_ACTIO
 
 FILTERS = [xpuhip.JOIN_FILTER_AUTO, xpuhip.JOIN_FILTER_OFF]
 FILTER_IDS = ["filter", "nofilter"]
+PASSES = [xpuhip.JOIN_PASSES_ON, xpuhip.JOIN_PASSES_OFF]
+PASSES_IDS = ["passes", "fused"]
 
 
+@pytest.mark.parametrize("join_passes", PASSES, ids=PASSES_IDS)
 @pytest.mark.parametrize("join_filter", FILTERS, ids=FILTER_IDS)
 @pytest.mark.parametrize("left_outer", [False, True])
-def test_c2_star_join_preagg(gpu, left_outer, jit, join_filter):
+def test_c2_star_join_preagg(gpu, left_outer, jit, join_filter, join_passes):
     fact, dims = datagen.star_tables(300_000, dim_sizes=(20_000, 2_000, 2_556))
     src = kds.build_arrow_kds(fact)
     km = W.star_inner(dims)
     blob = W.star_query(fact, left_outer=left_outer)
-    got = run_gpu_groupby(blob, [src], kmrels=km, tuning=dict(jit=jit, join_filter=join_filter))
+    got = run_gpu_groupby(blob, [src], kmrels=km,
+                          tuning=dict(jit=jit, join_filter=join_filter, join_passes=join_passes))
     exp = run_oracle_groupby(blob, [src], kmrels=km)
     assert_groups_equal(got, exp, C2_ACTIONS)
 
 
+@pytest.mark.parametrize("join_passes", PASSES, ids=PASSES_IDS)
 @pytest.mark.parametrize("join_filter", FILTERS, ids=FILTER_IDS)
-def test_c2_star_join_stats(gpu, jit, join_filter):
+def test_c2_star_join_stats(gpu, jit, join_filter, join_passes):
     """Rows in, rows out of every depth and groups out: the same counts as
     the oracle's with and without the inner-relation filters (the filters
     skip bucket walks that cannot match and change nothing else)."""
@@ -39,7 +44,7 @@ def test_c2_star_join_stats(gpu, jit, join_filter):
     km = W.star_inner(dims)
     blob = W.star_query(fact)
     s = xpuhip.Session(blob)
-    s.set_tuning(jit=jit, join_filter=join_filter)
+    s.set_tuning(jit=jit, join_filter=join_filter, join_passes=join_passes)
     s.attach_inner(km)
     rc, _, res, _ = s.exec(src)
     s.close()
@@ -55,8 +60,9 @@ def test_c2_star_join_stats(gpu, jit, join_filter):
     assert 0 < res.stats[2].nitems_out < res.stats[0].nitems_out
 
 
+@pytest.mark.parametrize("join_passes", PASSES, ids=PASSES_IDS)
 @pytest.mark.parametrize("join_filter", FILTERS, ids=FILTER_IDS)
-def test_join_groupby_multi_match(gpu, jit, join_filter):
+def test_join_groupby_multi_match(gpu, jit, join_filter, join_passes):
     """Group-by over a join whose inner relation holds duplicate keys, with
     scan quals rejecting a third of the rows: every (row, match) pair is
     aggregated once."""
@@ -87,7 +93,8 @@ def test_join_groupby_multi_match(gpu, jit, join_filter):
     blob = plan.build(final_nrows=16)
     src = kds.build_arrow_kds(outer)
     acts = [abi.KAGG_ACTION__NROWS_ANY, abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__VREF]
-    got = run_gpu_groupby(blob, [src], kmrels=km, tuning=dict(jit=jit, join_filter=join_filter))
+    got = run_gpu_groupby(blob, [src], kmrels=km,
+                          tuning=dict(jit=jit, join_filter=join_filter, join_passes=join_passes))
     exp = run_oracle_groupby(blob, [src], kmrels=km)
     groups = assert_groups_equal(got, exp, acts)
     assert len(groups) == 7
