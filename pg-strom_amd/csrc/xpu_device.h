@@ -366,10 +366,11 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     isnull = (idx >= cm->nullmap_length) ||
                              !((((const uint8_t *)kbase)[nm_off + idx] >> (row & 7)) & 1);
                 }
-                if ((uint64_t)w * (row + 1) > v_len)
-                    isnull = true;
+                /* the value is read whenever it lies inside the column, so
+                 * its load does not wait for the validity bit */
+                const bool inside = (uint64_t)w * (row + 1) <= v_len;
                 int64_t v = 0;
-                if (!isnull)
+                if (inside)
                 {
                     const char *p = kbase + v_off + (uint64_t)w * row;
                     switch (w)
@@ -381,7 +382,8 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                         default: v = *(const int64_t *)p; break;
                     }
                 }
-                L.R[d] = v;
+                isnull = isnull || !inside;
+                L.R[d] = isnull ? 0 : v;
                 SETNULL(L, d, isnull);
                 break;
             }
@@ -2751,11 +2753,13 @@ rec_word(uint32_t *base, uint32_t nwords, uint32_t cap, uint32_t b, uint32_t k, 
     return base + ((uint64_t)b * nwords + k) * cap + r;
 }
 
-/* The screen pass: one 64-row batch per wave step (static assignment of
- * the batches to blocks and waves), the scan quals and the depth-1 filter,
- * and the append of the rows that pass.  Few registers: a wide block keeps
- * many waves in flight. */
-template <int NR, class PROG>
+/* The screen pass: wave tiles of 64*RPT rows (static assignment of the
+ * tiles to blocks and waves), their staged loads issued together (the
+ * specialized kernel keeps only the columns the screen program reads), the
+ * scan quals and depth-1 filter probes of every row, then the appends of
+ * the rows that pass.  Few registers: a wide block keeps many waves in
+ * flight. */
+template <int NR, int RPT, class PROG>
 __device__ __forceinline__ void
 join_screen_body(cplan *P, const xpu_kparams &kp)
 {
@@ -2773,34 +2777,66 @@ join_screen_body(cplan *P, const xpu_kparams &kp)
     __syncthreads();
     uint32_t cnt_in = 0;
     bool full = false;
-    const uint64_t step = (uint64_t)gridDim.x * nwaves * 64;
-    for (uint64_t r0 = (uint64_t)kp.row_begin + ((uint64_t)b * nwaves + wid) * 64;
-         r0 < kp.row_end; r0 += step)
+    StagedCols<RPT> S;
+    S.setup(P, kp.kds_src);
+    const uint64_t wrows = 64ULL * RPT;
+    const uint64_t nwt = ((uint64_t)(kp.row_end - kp.row_begin) + wrows - 1) / wrows;
+    for (uint64_t wt = (uint64_t)b * nwaves + wid; wt < nwt; wt += (uint64_t)gridDim.x * nwaves)
     {
-        const uint64_t row = r0 + lane;
-        const bool valid = row < kp.row_end;
-        int64_t Rfile[NR];
-        Lane L;
-        L.R = Rfile;
-        lane_reset<NR>(L);
-        uint32_t h = 0;
-        bool ok = valid && screen_row<NR, PROG, false>(P, kp, L, (uint32_t)row, bloom, &h);
-        if (ok)
-            cnt_in++;
-        if (ok && bloom)
-            ok = bloom_maybe(kp.bloom[1], kp.bloom_mask[1], h);
-        const uint64_t m = __ballot(ok);
-        if (!m)
+        const uint64_t base = (uint64_t)kp.row_begin + wt * wrows;
+        S.load(base, kp.row_end);
+        bool ok[RPT];
+        uint64_t bw[RPT], bm[RPT];
+        auto screen_step = [&](const int i) __attribute__((always_inline)) {
+            const uint64_t row = base + (uint64_t)i * 64 + lane;
+            const bool valid = row < kp.row_end;
+            int64_t Rfile[NR];
+            Lane L;
+            L.R = Rfile;
+            lane_reset<NR>(L);
+            S.fill(L, i, row, valid);
+            uint32_t h = 0;
+            ok[i] = valid && screen_row<NR, PROG, true>(P, kp, L, (uint32_t)row, bloom, &h);
+            if (ok[i])
+                cnt_in++;
+            bw[i] = bm[i] = 0;
+            if (ok[i] && bloom)
+            {
+                const uint64_t x = bloom_mix(h);
+                bm[i] = bloom_bits(x);
+                bw[i] = kp.bloom[1][x & kp.bloom_mask[1]];
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+            screen_step(i);
+        /* one reservation for the tile's rows that pass */
+        bool cand[RPT];
+        uint64_t m[RPT];
+        uint32_t total = 0;
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+        {
+            cand[i] = ok[i] && (bw[i] & bm[i]) == bm[i];
+            m[i] = __ballot(cand[i]);
+            total += (uint32_t)__popcll(m[i]);
+        }
+        if (total == 0)
             continue;
-        uint32_t pos0 = 0;
+        uint32_t pos = 0;
         if (lane == 0)
-            pos0 = atomicAdd(&s_cnt, (uint32_t)__popcll(m));
-        pos0 = __shfl(pos0, 0);
-        const uint32_t r = pos0 + (uint32_t)__popcll(m & ((1ULL << lane) - 1));
-        if (ok && r >= kp.rec_out_cap)
-            full = true;
-        else if (ok)
-            *rec_word(kp.rec_out, 1, kp.rec_out_cap, b, 0, r) = (uint32_t)row;
+            pos = atomicAdd(&s_cnt, total);
+        pos = __builtin_amdgcn_readfirstlane(pos);
+#pragma unroll
+        for (int i = 0; i < RPT; i++)
+        {
+            const uint32_t r = pos + (uint32_t)__popcll(m[i] & ((1ULL << lane) - 1));
+            if (cand[i] && r >= kp.rec_out_cap)
+                full = true;
+            else if (cand[i])
+                *rec_word(kp.rec_out, 1, kp.rec_out_cap, b, 0, r) = (uint32_t)(base + (uint64_t)i * 64 + lane);
+            pos += (uint32_t)__popcll(m[i]);
+        }
     }
     if (full)
         record_error(ctl, XMSG_JOIN_RECORDS, ERRCODE_DEVICE_ERROR);

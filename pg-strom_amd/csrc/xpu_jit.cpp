@@ -131,7 +131,7 @@ xpu_jit_source(const xplan *P, int kind, int param)
     {
         s += "\nextern \"C\" __global__ void __launch_bounds__(XQ_SCREEN_BLOCK)\n"
              "xpu_jit_screen_kernel(xpu_kparams kp)\n{\n";
-        append(s, "    join_screen_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
+        append(s, "    join_screen_body<%d, XQ_SCREEN_RPT, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
         for (uint32_t d = 1; d <= P->nrels; d++)
         {
             append(s, "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"

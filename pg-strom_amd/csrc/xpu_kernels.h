@@ -32,6 +32,7 @@
  * screen holds few registers, and its blocks match the join passes' blocks
  * one to one) */
 #define XQ_SCREEN_BLOCK     1024
+#define XQ_SCREEN_RPT       4       /* 64-row batches per wave tile */
 
 /* per-task control block in device memory (cleared before each launch) */
 typedef struct

@@ -709,7 +709,7 @@ bool
 join_pass_sizes(const xplan *P, const uint32_t *maxdup, uint64_t nrows, uint32_t grid,
                 uint32_t nwaves, uint64_t *slice, uint32_t *cap, uint32_t *nw, size_t *bytes)
 {
-    const uint64_t wrows = 64;
+    const uint64_t wrows = 64ULL * XQ_SCREEN_RPT;
     const uint64_t round = (uint64_t)grid * nwaves * wrows;
     /* record words per row of a slice, at the level's most matches per row
      * (a level-l record: the row and the items of depths 1..l-1) */
