@@ -3080,6 +3080,275 @@ join_probe_body(cplan *P, const xpu_kparams &kp)
 }
 
 /* ------------------------------------------------------------------
+ * scan + projection for plans without joins
+ *
+ * Two passes over the same wave tiles of 64*RPT rows, each block owning a
+ * contiguous run of tiles (so tile order is row order): the count pass
+ * stores each tile's output rows and bytes and each block's totals; the
+ * tile scan turns them into every tile's exclusive prefix and finds the
+ * first tile that does not fit kds_dst; the write pass runs the rows again
+ * and writes each output tuple at its tile's prefix plus the outputs of
+ * the rows before it in the tile, so kds_dst holds the rows in scan order.
+ * The staged loads of a tile are issued together, as in the scan +
+ * group-by kernel, and no pass waits on another block.
+ * ------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t
+wave_incl_scan_u32(uint32_t v)
+{
+    int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v += t;
+    }
+    return v;
+}
+
+template <int NR, int RPT, class PROG, int PM>
+__device__ __forceinline__ void
+scan_proj_body(cplan *P, const xpu_kparams &kp)
+{
+    __shared__ uint64_t s_tot[2 * (XP_BLOCK_MAX / 64)];
+    const kern_data_store *__restrict__ kds = kp.kds_src;
+    xpu_task_ctl *ctl = kp.ctl;
+    kern_data_store *dst = kp.kds_out;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);     /* uniform */
+    const int nwaves = blockDim.x >> 6;
+    const uint64_t wrows = 64ULL * RPT;
+    const uint64_t nwt = kp.ntiles;
+    const uint64_t per = (nwt + gridDim.x - 1) / gridDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * per;
+    const uint64_t t1 = t0 + per < nwt ? t0 + per : nwt;
+    uint64_t *st = kp.tile_status;
+    uint64_t stop = nwt;
+    if (PM == PM_WRITE)
+    {
+        uint32_t rt = __hip_atomic_load(&ctl->resume_tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (rt)
+            stop = 0xffffffffU - rt;
+        /* the rows and bytes kds_dst ends up with */
+        if (blockIdx.x == 0 && tid == 0 && stop < nwt)
+        {
+            ctl->out_nitems = st[2 * stop];
+            ctl->out_usage = st[2 * stop + 1];
+        }
+    }
+    StagedCols<RPT> S;
+    S.setup(P, kds);
+    const int pc_begin = P->seg_scan.begin + P->stage_prefix;
+    const int pc_end = P->seg_final.end;
+    uint64_t *rowidx = (uint64_t *)((char *)dst + KDS_HEAD_LENGTH(dst));
+    uint32_t cnt = 0;
+    uint64_t wc = 0, wb = 0;
+    for (uint64_t wt = t0 + wid; wt < t1; wt += nwaves)
+    {
+        if (PM == PM_WRITE && wt >= stop)
+            break;
+        const uint64_t base = (uint64_t)kp.row_begin + wt * wrows;
+        S.load(base, kp.row_end);
+        /* count pass: the tile's totals so far; write pass: the outputs
+         * before the current row batch */
+        uint64_t tr = 0, tb = 0;
+        if (PM == PM_WRITE)
+        {
+            tr = st[2 * wt];
+            tb = st[2 * wt + 1];
+        }
+        auto row_step = [&](const int i) __attribute__((always_inline))
+        {
+            const uint64_t row = base + (uint64_t)i * 64 + lane;
+            const bool valid = row < kp.row_end;
+            int64_t Rfile[NR];
+            Lane L;
+            L.R = Rfile;
+            lane_reset<NR>(L);
+            S.fill(L, i, row, valid);
+            if (valid)
+                PROG::template run<true>(P, pc_begin, pc_end, L, kds, (uint32_t)row, kp);
+            bool ok = valid && !L.dead;
+            if (valid && L.err)
+            {
+                if (PM == PM_COUNT)
+                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                ok = false;
+            }
+            const uint32_t sz = ok ? (uint32_t)xalign(8, proj_tuple_size(P, L)) : 0;
+            const uint64_t m = __ballot(ok);
+            const uint32_t ib = wave_incl_scan_u32(sz);
+            if (PM == PM_COUNT)
+            {
+                if (ok)
+                    cnt++;
+            }
+            else if (ok)
+            {
+                const uint64_t rowid = tr + (uint64_t)__popcll(m & ((1ULL << lane) - 1));
+                const uint64_t usage = tb + ib;
+                char *tp = (char *)dst + dst->length - usage;
+                proj_tuple_write(P, L, tp, sz, (uint32_t)rowid);
+                rowidx[rowid] = usage;
+            }
+            tr += (uint64_t)__popcll(m);
+            tb += (uint64_t)__shfl(ib, 63);
+        };
+        if constexpr (PROG::kUnrollRows)
+        {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                row_step(i);
+        }
+        else
+        {
+#pragma unroll 1
+            for (int i = 0; i < RPT; i++)
+                row_step(i);
+        }
+        if (PM == PM_COUNT)
+        {
+            if (lane == 0)
+            {
+                st[2 * wt] = tr;
+                st[2 * wt + 1] = tb;
+            }
+            wc += tr;
+            wb += tb;
+        }
+    }
+    if (PM == PM_COUNT)
+    {
+        if (lane == 0)
+        {
+            s_tot[2 * wid] = wc;
+            s_tot[2 * wid + 1] = wb;
+        }
+        __syncthreads();
+        if (tid == 0)
+        {
+            uint64_t r = 0, bsum = 0;
+            for (int w = 0; w < nwaves; w++)
+            {
+                r += s_tot[2 * w];
+                bsum += s_tot[2 * w + 1];
+            }
+            kp.block_status[2 * blockIdx.x] = r;
+            kp.block_status[2 * blockIdx.x + 1] = bsum;
+        }
+        if (cnt)
+        {
+            atomicAdd(&ctl->nitems_in, cnt);
+            atomicAdd(&ctl->nitems_out, cnt);
+        }
+    }
+}
+
+/* The tile scan of a scan + projection: block b sums the totals of the
+ * count-pass blocks before it, then scans the tiles block b of the count
+ * pass owned into exclusive prefixes, and reports the first tile that does
+ * not fit kds_dst.  The last block stores the grand totals. */
+__device__ __forceinline__ void
+tile_scan_blocks_body(uint64_t *st, const uint64_t *bt, uint32_t nwt, const kern_data_store *dst,
+                      xpu_task_ctl *ctl)
+{
+    __shared__ uint64_t s_wr[XP_BLOCK_MAX / 64], s_wb[XP_BLOCK_MAX / 64];
+    __shared__ uint64_t s_carry_r, s_carry_b;
+    __shared__ uint32_t s_stop;
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int wid = tid >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const uint32_t b = blockIdx.x;
+    const uint64_t per = ((uint64_t)nwt + gridDim.x - 1) / gridDim.x;
+    const uint64_t t0 = (uint64_t)b * per;
+    const uint64_t t1 = t0 + per < nwt ? t0 + per : nwt;
+
+    /* the outputs of the blocks before b */
+    uint64_t pr = 0, pb = 0;
+    for (uint32_t j = tid; j < b; j += blockDim.x)
+    {
+        pr += bt[2 * j];
+        pb += bt[2 * j + 1];
+    }
+    pr = wave_incl_scan_u64(pr);
+    pb = wave_incl_scan_u64(pb);
+    if (lane == 63)
+    {
+        s_wr[wid] = pr;
+        s_wb[wid] = pb;
+    }
+    __syncthreads();
+    if (tid == 0)
+    {
+        uint64_t r = 0, bb = 0;
+        for (int w = 0; w < nwaves; w++)
+        {
+            r += s_wr[w];
+            bb += s_wb[w];
+        }
+        s_carry_r = r;
+        s_carry_b = bb;
+        s_stop = nwt;
+    }
+    __syncthreads();
+    const uint64_t head = KDS_HEAD_LENGTH(dst);
+    const uint64_t len = dst->length;
+    for (uint64_t base = t0; base < t1; base += blockDim.x)
+    {
+        const uint64_t t = base + tid;
+        const uint64_t r = t < t1 ? st[2 * t] : 0;
+        const uint64_t by = t < t1 ? st[2 * t + 1] : 0;
+        const uint64_t ir = wave_incl_scan_u64(r);
+        const uint64_t ib = wave_incl_scan_u64(by);
+        if (lane == 63)
+        {
+            s_wr[wid] = ir;
+            s_wb[wid] = ib;
+        }
+        __syncthreads();
+        uint64_t wr = s_carry_r, wb = s_carry_b;
+        for (int w = 0; w < wid; w++)
+        {
+            wr += s_wr[w];
+            wb += s_wb[w];
+        }
+        const uint64_t er = wr + ir - r;
+        const uint64_t eb = wb + ib - by;
+        if (t < t1)
+        {
+            st[2 * t] = er;
+            st[2 * t + 1] = eb;
+            if (head + 8 * (er + r) + eb + by > len)
+                atomicMin(&s_stop, (uint32_t)t);
+        }
+        __syncthreads();
+        if (tid == (int)blockDim.x - 1)
+        {
+            s_carry_r = er + r;
+            s_carry_b = eb + by;
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+    {
+        const uint32_t stop = s_stop;
+        if (stop < nwt)
+        {
+            atomicMax(&ctl->resume_tile, 0xffffffffU - stop);
+            if (stop == 0)
+                record_error(ctl, XMSG_TILE_TOO_LARGE, ERRCODE_DEVICE_ERROR);
+        }
+        if (b == gridDim.x - 1)
+        {
+            ctl->out_nitems = s_carry_r;
+            ctl->out_usage = s_carry_b;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------
  * scan + group-by kernel for plans without joins
  *
  * Each wave owns tiles of 64*RPT consecutive rows (static assignment, so

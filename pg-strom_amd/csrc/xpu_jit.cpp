@@ -116,6 +116,10 @@ xpu_jit_source(const xplan *P, int kind, int param)
             emit_segment(s, P, P->depth[d].seg_oj.begin, P->depth[d].seg_oj.end);
         }
         emit_segment(s, P, P->seg_final.begin, P->seg_final.end);
+        /* a projection without joins runs its quals and targets as one
+         * program after the staged loads */
+        if (P->nrels == 0)
+            emit_segment(s, P, P->seg_scan.begin + P->stage_prefix, P->seg_final.end);
     }
     s += "    }\n};\n\n";
     s += "extern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
@@ -138,6 +142,15 @@ xpu_jit_source(const xplan *P, int kind, int param)
                       "xpu_jit_probe_kernel_%u(xpu_kparams kp)\n{\n", d);
             append(s, "    join_probe_body<%d, JitProg, %u>((cplan *)xjit_plan, kp);\n}\n", nr, d);
         }
+    }
+    if (kind == XJ_PIPELINE && !param && P->nrels == 0)
+    {
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_scount_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    scan_proj_body<%d, XP_PROJ_RPT, JitProg, PM_COUNT>((cplan *)xjit_plan, kp);\n}\n", nr);
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+             "xpu_jit_swrite_kernel(xpu_kparams kp)\n{\n";
+        append(s, "    scan_proj_body<%d, XP_PROJ_RPT, JitProg, PM_WRITE>((cplan *)xjit_plan, kp);\n}\n", nr);
     }
     if (kind == XJ_PIPELINE && !param)
     {

@@ -33,6 +33,8 @@
  * one to one) */
 #define XQ_SCREEN_BLOCK     1024
 #define XQ_SCREEN_RPT       4       /* 64-row batches per wave tile */
+/* scan + projection: 64-row batches per wave tile */
+#define XP_PROJ_RPT         4
 
 /* per-task control block in device memory (cleared before each launch) */
 typedef struct
@@ -144,6 +146,8 @@ typedef struct
     uint32_t    rec_out_cap;
     uint32_t    pass_depth;             /* 0: screen pass; d: join with inner relation d */
     uint32_t    screened;               /* level-1 records passed the depth-1 filter */
+    /* scan + projection: rows and bytes of each count-pass block */
+    uint64_t   *block_status;
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -155,6 +159,9 @@ int xpu_launch_pipeline(const xpu_kparams *kp, int nregs, int grid, int block,
 /* projection in two passes: count (tile totals), tile scan, write */
 int xpu_launch_projection(const xpu_kparams *kp, int nregs, int grid, int block, void *stream);
 int xpu_launch_tile_scan(const xpu_kparams *kp, void *stream);
+/* scan + projection: the tile scan over the count pass's wave tiles (one
+ * block per count-pass block) */
+int xpu_launch_tile_scan_blocks(const xpu_kparams *kp, int grid, void *stream);
 /* folds the partial-state dumps of a scan + group-by task into kds_final */
 int xpu_launch_dump_merge(const xpu_kparams *kp, int nregs, int grid, int block,
                           size_t lds_bytes, void *stream);

@@ -251,6 +251,22 @@ xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream)
     return (int)hipGetLastError();
 }
 
+__global__ void __launch_bounds__(XP_BLOCK_MAX)
+xpu_tile_scan_blocks_kernel(uint64_t *st, const uint64_t *bt, uint32_t nwt, const kern_data_store *dst,
+                            xpu_task_ctl *ctl)
+{
+    tile_scan_blocks_body(st, bt, nwt, dst, ctl);
+}
+
+extern "C" int
+xpu_launch_tile_scan_blocks(const xpu_kparams *kp, int grid, void *stream)
+{
+    hipLaunchKernelGGL(xpu_tile_scan_blocks_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream,
+                       kp->tile_status, kp->block_status, kp->ntiles, (const kern_data_store *)kp->kds_out,
+                       kp->ctl);
+    return (int)hipGetLastError();
+}
+
 extern "C" int
 xpu_launch_tile_scan(const xpu_kparams *kp, void *stream)
 {
