@@ -202,11 +202,15 @@ def bench_batch(nfrag, base_rows, warmup, steps):
 
 
 def bench_served(nreq, nback, min_rows, max_rows, table_rows, slots=8, max_batch=32):
-    """Served form: open-loop Poisson arrivals of nreq requests from nback
-    backends (60% scans, 25% pre-aggs, 15% star joins over row windows of
-    log-uniform size of resident tables; dims resident), through the request
-    loop with 8 slots and coalesced group-by fragments, against the same
-    requests one at a time.  Loads: 0.8x and 3x the one-at-a-time capacity."""
+    """Served form against the one-query-at-a-time form on the same
+    open-loop Poisson arrivals of nreq queries (60% scans, 25% pre-aggs, 15%
+    star joins over row windows of log-uniform size of tables resident in
+    HBM).  Served: nback backends with sessions (and dimension hash tables)
+    kept across their queries, each running its queries one at a time,
+    through the request loop with `slots` slots and coalesced group-by
+    fragments.  One query at a time: each query opens its own session,
+    attaches its dimension tables, runs, reads its result and closes.
+    Loads: 0.8x and 3x the one-query-at-a-time capacity."""
     from pgstrom import serving, xpuhip
     log("served: generating resident tables (%d rows each)" % table_rows)
     res = serving.Resident(scan_rows=table_rows // 2, preagg_rows=table_rows, fact_rows=table_rows,
@@ -217,12 +221,8 @@ def bench_served(nreq, nback, min_rows, max_rows, table_rows, slots=8, max_batch
             for q in reqs]
     backends = serving.Backends(res, nback)
 
-    def form(slots, max_batch, arrivals):
-        srv = xpuhip.Server(hbm_budget=200 << 30, slots=slots, max_batch=max_batch)
-        st, span = serving.run(srv, backends, reqs, arrivals, dsts)
-        stats = srv.stats()
-        srv.close()
-        assert all(x.rc == 0 for x in st)
+    def summary(st, span, stats):
+        assert all(x.rc == 0 for x in st), [x.err for x in st if x.rc][:3]
         lat = np.array([x.total_ms for x in st])
         per_kind = {}
         for kind in serving.KINDS:
@@ -230,21 +230,40 @@ def bench_served(nreq, nback, min_rows, max_rows, table_rows, slots=8, max_batch
             rk = sum(q["n"] for q in reqs if q["kind"] == kind)
             per_kind[kind] = dict(requests=len(ex), exec_ms_mean=float(np.mean(ex)) if ex else 0.0,
                                   rows=rk)
-        return dict(rows_per_s=rows / span, req_per_s=len(reqs) / span, makespan_s=span,
-                    p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                    launches=stats["launches"], coalesced=stats["coalesced"], per_kind=per_kind)
+        out = dict(rows_per_s=rows / span, req_per_s=len(reqs) / span, makespan_s=span,
+                   p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                   per_kind=per_kind)
+        if stats:
+            out.update(launches=stats["launches"], coalesced=stats["coalesced"])
+        return out
+
+    def served(arrivals):
+        srv = xpuhip.Server(hbm_budget=200 << 30, slots=slots, max_batch=max_batch)
+        st, span = serving.run(srv, backends, reqs, arrivals, dsts)
+        stats = srv.stats()
+        srv.close()
+        return summary(st, span, stats)
+
+    def plain(arrivals):
+        st, span = serving.run_plain(res, reqs, arrivals, dsts)
+        return summary(st, span, None)
 
     # warm: specialised kernels of every kind compile once
-    form(slots, max_batch, [0.0] * len(reqs))
-    cap = form(1, 1, [0.0] * len(reqs))
+    served([0.0] * len(reqs))
+    # capacities: every query arrives at once
+    cap = plain([0.0] * len(reqs))
+    cap_served = served([0.0] * len(reqs))
     mu = cap["req_per_s"]
     rng = np.random.default_rng(20261202 + 5)
-    out = {"requests": nreq, "backends": nback, "rows": rows, "plain_capacity_req_per_s": mu,
-           "slots": slots, "max_batch": max_batch}
+    out = {"requests": nreq, "backends": nback, "rows": rows, "slots": slots, "max_batch": max_batch,
+           "plain_capacity_req_per_s": mu, "plain_capacity_rows_per_s": cap["rows_per_s"],
+           "served_capacity_req_per_s": cap_served["req_per_s"],
+           "served_capacity_rows_per_s": cap_served["rows_per_s"],
+           "served_vs_plain_capacity": cap_served["rows_per_s"] / cap["rows_per_s"]}
     for load in (0.8, 3.0):
         gaps = rng.exponential(1.0 / (load * mu), size=len(reqs))
         arr = list(np.cumsum(gaps) - gaps[0])
-        out["load_%.1fx" % load] = {"plain": form(1, 1, arr), "served": form(slots, max_batch, arr)}
+        out["load_%.1fx" % load] = {"plain": plain(arr), "served": served(arr)}
     for d in dsts:
         if d is not None:
             d.free()
@@ -252,6 +271,8 @@ def bench_served(nreq, nback, min_rows, max_rows, table_rows, slots=8, max_batch
     res.free()
     hi = out["load_3.0x"]
     out["served_vs_plain_rows_per_s_at_3x"] = hi["served"]["rows_per_s"] / hi["plain"]["rows_per_s"]
+    out["served_p99_at_3x_vs_plain_p99_at_0.8x"] = (hi["served"]["p99_ms"] /
+                                                    out["load_0.8x"]["plain"]["p99_ms"])
     return out
 
 

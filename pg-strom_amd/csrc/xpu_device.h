@@ -1367,22 +1367,52 @@ bloom_maybe(const uint64_t *__restrict__ words, uint32_t mask, uint32_t h)
     return (words[x & mask] & m) == m;
 }
 
-/* sets the bits of every item reachable from the hash slots [s0, s1) */
+/* Sets the bits of every item reachable from the hash slots [s0, s1), and
+ * raises *maxdup to the most items of one chain that share a join hash (a
+ * probe finds at most that many candidates, since items of one hash share a
+ * chain); a chain of more than 64 items counts as its length, which bounds
+ * the same from above. */
 __device__ __forceinline__ void
-bloom_build_range(kern_data_store *ik, uint64_t *words, uint32_t mask, uint64_t s0, uint64_t s1)
+bloom_build_range(kern_data_store *ik, uint64_t *words, uint32_t mask, uint64_t s0, uint64_t s1,
+                  uint32_t *maxdup)
 {
     const uint64_t *slots = (const uint64_t *)((const char *)ik + KDS_HEAD_LENGTH(ik));
     for (uint64_t s = s0; s < s1; s++)
     {
-        uint64_t off = slots[s];
+        const uint64_t head = slots[s];
+        uint64_t off = head;
         uint64_t steps = (uint64_t)ik->nitems + 1;     /* as many as a probe inspects */
+        uint32_t len = 0;
         while (off != 0 && off != ~0ULL && off < ik->length && steps-- > 0)
         {
             const kern_hashitem *hi = hashitem_at(ik, off);
             uint64_t x = bloom_mix(hi->t.hash);
             atomicOr((unsigned long long *)&words[x & mask], (unsigned long long)bloom_bits(x));
             off = hi->next;
+            len++;
         }
+        uint32_t most = len;
+        if (len <= 64)
+        {
+            most = 0;
+            uint64_t a = head;
+            for (uint32_t i = 0; i < len; i++)
+            {
+                const kern_hashitem *ha = hashitem_at(ik, a);
+                uint32_t same = 0;
+                uint64_t c = a;
+                for (uint32_t j = i; j < len; j++)
+                {
+                    const kern_hashitem *hc = hashitem_at(ik, c);
+                    same += (hc->t.hash == ha->t.hash);
+                    c = hc->next;
+                }
+                most = same > most ? same : most;
+                a = ha->next;
+            }
+        }
+        if (most)
+            atomicMax(maxdup, most);
     }
 }
 

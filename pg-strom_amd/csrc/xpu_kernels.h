@@ -175,9 +175,10 @@ int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
                          size_t lds_bytes, void *stream);
 /* scan + group-by without joins; rpt = rows per lane per wave tile (1, 2, 4) */
 /* sets the Bloom filter bits of the join hashes of one inner KDS_FORMAT_HASH
- * (words zeroed by the caller) */
+ * (words zeroed by the caller) and raises *maxdup to the most items that
+ * share a join hash */
 int xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words,
-                           uint32_t mask, void *stream);
+                           uint32_t mask, uint32_t *maxdup, void *stream);
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);

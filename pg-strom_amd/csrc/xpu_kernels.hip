@@ -84,12 +84,13 @@ xpu_fill_kernel(uint64_t *ptr, uint64_t value, size_t nwords)
 /* Bloom filter of an inner relation's join hashes: one lane per hash slot
  * walks that slot's chain */
 __global__ void
-xpu_bloom_build_kernel(kern_data_store *inner, uint64_t nslots, uint64_t *words, uint32_t mask)
+xpu_bloom_build_kernel(kern_data_store *inner, uint64_t nslots, uint64_t *words, uint32_t mask,
+                       uint32_t *maxdup)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (; i < nslots; i += stride)
-        bloom_build_range(inner, words, mask, i, i + 1);
+        bloom_build_range(inner, words, mask, i, i + 1, maxdup);
 }
 
 /* writes the kds_dst header: the session's KDS_FORMAT_ROW header with the
@@ -312,7 +313,7 @@ xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream)
 
 extern "C" int
 xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words, uint32_t mask,
-                       void *stream)
+                       uint32_t *maxdup, void *stream)
 {
     if (nslots == 0)
         return 0;
@@ -320,7 +321,7 @@ xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words,
     if (blocks > 4096)
         blocks = 4096;
     hipLaunchKernelGGL(xpu_bloom_build_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       (hipStream_t)stream, inner, nslots, words, mask);
+                       (hipStream_t)stream, inner, nslots, words, mask, maxdup);
     return (int)hipGetLastError();
 }
 

@@ -196,6 +196,51 @@ struct xpuhip_session
 
 namespace {
 
+/* Task scratch (stream, events, pinned control block, grown buffers) of
+ * closed sessions, kept for the next sessions: opening and closing a
+ * session per query then costs no stream or pinned-memory setup. */
+std::mutex g_idle_lock;
+std::vector<Scratch *> g_idle;
+#define XPU_IDLE_SCRATCH        64
+#define XPU_IDLE_KEEP_BYTES     (256ULL << 20)
+
+void
+scratch_trim(Scratch *x)
+{
+    auto trim = [](char **p, size_t *cap) {
+        if (*p && *cap > XPU_IDLE_KEEP_BYTES)
+        {
+            (void)hipFree(*p);
+            *p = nullptr;
+            *cap = 0;
+        }
+    };
+    trim((char **)&x->d_status, &x->status_cap);
+    trim((char **)&x->d_spill, &x->spill_cap);
+    trim((char **)&x->d_dump, &x->dump_cap);
+    trim(&x->d_src, &x->src_cap);
+    trim(&x->d_dst, &x->dst_cap);
+    for (int i = 0; i < 2; i++)
+        trim((char **)&x->d_rec[i], &x->rec_cap[i]);
+}
+
+/* a closed session's scratch, to the idle list (or freed beyond it) */
+void
+scratch_retire(Scratch *x)
+{
+    (void)hipStreamSynchronize(x->stream);
+    scratch_trim(x);
+    {
+        std::lock_guard<std::mutex> g(g_idle_lock);
+        if (g_idle.size() < XPU_IDLE_SCRATCH)
+        {
+            g_idle.push_back(x);
+            return;
+        }
+    }
+    scratch_destroy(x);
+}
+
 Scratch *
 scratch_get(xpuhip_session *s, kern_errorbuf *eb)
 {
@@ -205,6 +250,15 @@ scratch_get(xpuhip_session *s, kern_errorbuf *eb)
         {
             Scratch *x = s->pool.back();
             s->pool.pop_back();
+            return x;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> g(g_idle_lock);
+        if (!g_idle.empty())
+        {
+            Scratch *x = g_idle.back();
+            g_idle.pop_back();
             return x;
         }
     }
@@ -562,41 +616,6 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
     return 0;
 }
 
-/* the most items of an inner KDS_FORMAT_HASH that share one join hash: a
- * probe finds at most that many candidates (items of one hash share a
- * chain, so each chain is counted on its own) */
-static uint32_t
-inner_max_dup(const kern_data_store *k)
-{
-    const uint64_t *slots = (const uint64_t *)((const char *)k + KDS_HEAD_LENGTH(k));
-    std::vector<uint32_t> hs;
-    uint32_t best = 0;
-    for (uint64_t i = 0; i < k->hash_nslots; i++)
-    {
-        hs.clear();
-        uint64_t off = slots[i];
-        uint64_t steps = (uint64_t)k->nitems + 1;
-        while (off != 0 && off != ~0ULL && off < k->length && off >= sizeof(kern_hashitem) &&
-               steps-- > 0)
-        {
-            const kern_hashitem *hi = (const kern_hashitem *)((const char *)k + k->length - off);
-            hs.push_back(hi->t.hash);
-            off = hi->next;
-        }
-        if (hs.size() <= best)
-            continue;
-        std::sort(hs.begin(), hs.end());
-        uint32_t run = 0;
-        for (size_t j = 0; j < hs.size(); j++)
-        {
-            run = (j > 0 && hs[j] == hs[j - 1]) ? run + 1 : 1;
-            if (run > best)
-                best = run;
-        }
-    }
-    return best;
-}
-
 extern "C" int
 xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, size_t length,
                             kern_errorbuf *eb)
@@ -665,25 +684,30 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
     }
     if (nwords > 0)
     {
-        if (!hip_ok(hipMalloc((void **)&s->d_bloom, 8 * nwords), eb, "hipMalloc(bloom)") ||
-            !hip_ok((hipError_t)xpu_launch_fill(s->d_bloom, 0, nwords, nullptr), eb, "bloom fill"))
+        /* the filter words, then one counter per depth for the most items
+         * sharing a join hash */
+        const uint64_t ndup = (P->nrels + 1 + 1) / 2;
+        if (!hip_ok(hipMalloc((void **)&s->d_bloom, 8 * (nwords + ndup)), eb, "hipMalloc(bloom)") ||
+            !hip_ok((hipError_t)xpu_launch_fill(s->d_bloom, 0, nwords + ndup, nullptr), eb, "bloom fill"))
             return ERRCODE_DEVICE_ERROR;
+        uint32_t *d_dup = (uint32_t *)(s->d_bloom + nwords);
         for (uint32_t d = 1; d <= P->nrels; d++)
         {
             const kern_data_store *k = (const kern_data_store *)((const char *)kmrels +
                                                                  kmrels->chunks[d - 1].kds_offset);
             if (!hip_ok((hipError_t)xpu_launch_bloom_build((kern_data_store *)(uintptr_t)P->depth[d].inner_kds,
                                                            k->hash_nslots, s->d_bloom + s->bloom_off[d],
-                                                           s->bloom_mask[d], nullptr),
+                                                           s->bloom_mask[d], d_dup + d, nullptr),
                         eb, "bloom build"))
                 return ERRCODE_DEVICE_ERROR;
         }
-        if (!hip_ok(hipStreamSynchronize(nullptr), eb, "bloom build"))
+        uint32_t dup[XP_MAX_DEPTH + 2] = {};
+        if (!hip_ok(hipMemcpy(dup, d_dup, 4 * (P->nrels + 1), hipMemcpyDeviceToHost), eb,
+                    "bloom build"))
             return ERRCODE_DEVICE_ERROR;
+        for (uint32_t d = 1; d <= P->nrels; d++)
+            s->maxdup[d] = dup[d];
     }
-    for (uint32_t d = 1; d <= P->nrels; d++)
-        s->maxdup[d] = inner_max_dup((const kern_data_store *)((const char *)kmrels +
-                                                               kmrels->chunks[d - 1].kds_offset));
     if (!hip_ok(hipMemcpy(s->d_plan, P, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy(plan)"))
         return ERRCODE_DEVICE_ERROR;
     s->inner_ready = true;
@@ -1309,7 +1333,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (!s)
         return;
     for (Scratch *x : s->pool)
-        scratch_destroy(x);
+        scratch_retire(x);
     s->pool.clear();
     if (s->d_plan) (void)hipFree(s->d_plan);
     if (s->d_const_pool) (void)hipFree(s->d_const_pool);

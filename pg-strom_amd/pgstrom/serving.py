@@ -105,35 +105,101 @@ def dst_bytes(n):
     return 4096 + n * 96
 
 
-def run(server, backends, reqs, arrivals, dsts):
-    """Open-loop run: request i is submitted at arrivals[i] seconds (from its
-    backend, round robin) and waited for by a collector thread; returns the
+class Status:
+    """Outcome of one query: return code, times (ms) and, for a group-by,
+    its kds_final as read back by the backend."""
+
+    def __init__(self, rc, queue_ms, exec_ms, total_ms, batched=0, result=None, err=""):
+        self.rc = rc
+        self.queue_ms = queue_ms
+        self.exec_ms = exec_ms
+        self.total_ms = total_ms
+        self.batched = batched
+        self.result = result
+        self.err = err
+
+
+def _wait_until(t):
+    delay = t - time.perf_counter()
+    if delay > 0:
+        time.sleep(delay)
+
+
+def run(server, backends, reqs, arrivals, dsts, keep_results=False):
+    """The served form, open loop: request i arrives at arrivals[i] seconds
+    at backend i % nback.  A backend runs its queries one at a time, as a
+    PostgreSQL backend does: it submits a query once it has arrived and the
+    backend's previous query is done, waits for it, and reads a group-by's
+    result back (and clears its session for the next query).  Returns the
     per-request statuses in request order and the makespan."""
-    tickets = [None] * len(reqs)
+    nback = len(backends.sess)
     done = [None] * len(reqs)
     shape_of = {k: i + 1 for i, k in enumerate(KINDS)}
+    t0 = time.perf_counter()
 
-    def collector():
-        for i in range(len(reqs)):
-            while tickets[i] is None:
-                time.sleep(0.0002)
-            done[i] = server.wait(tickets[i])
+    def backend(b):
+        for i in range(b, len(reqs), nback):
+            q = reqs[i]
+            _wait_until(t0 + arrivals[i])
+            s = backends.sess[b][q["kind"]]
+            if q["kind"] == "scan":
+                d = dsts[i]
+                tk = server.submit(s, q["src"], q["n"], d.ptr, d.nbytes,
+                                   hbm_bytes=q["n"] * 40 + d.nbytes, shape=shape_of["scan"])
+            else:
+                tk = server.submit(s, q["src"], q["n"], hbm_bytes=q["n"] * 28,
+                                   shape=shape_of[q["kind"]], batchable=(q["kind"] == "preagg"))
+            st = server.wait(tk)
+            result = None
+            if st.rc == 0 and q["kind"] != "scan":
+                result = s.final()
+                s.reset()
+            total = (time.perf_counter() - (t0 + arrivals[i])) * 1e3
+            done[i] = Status(st.rc, st.queue_ms, st.exec_ms, total, st.batched,
+                             result if keep_results else None,
+                             st.errbuf.message.decode(errors="replace") if st.rc else "")
 
-    th = threading.Thread(target=collector)
-    th.start()
+    threads = [threading.Thread(target=backend, args=(b,)) for b in range(nback)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    return done, time.perf_counter() - t0
+
+
+def run_plain(res, reqs, arrivals, dsts, keep_results=False, tuning=None):
+    """The one-query-at-a-time form, open loop: queries run one after
+    another in arrival order, each as its own session (opened, its dimension
+    tables attached, run, its group-by result read back, closed), with no
+    state kept from one query to the next.  Returns the statuses in request
+    order and the makespan."""
+    done = [None] * len(reqs)
+    blobs = {k: res.blob(k) for k in KINDS}
     t0 = time.perf_counter()
     for i, q in enumerate(reqs):
-        delay = t0 + arrivals[i] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        s = backends.sess[i % len(backends.sess)][q["kind"]]
+        _wait_until(t0 + arrivals[i])
+        start = time.perf_counter()
+        s = xpuhip.Session(blobs[q["kind"]])
+        if tuning:
+            s.set_tuning(**tuning)
+        if q["kind"] == "join":
+            s.attach_inner(res.kmrels)
+        t1 = time.perf_counter()
         if q["kind"] == "scan":
             d = dsts[i]
-            tickets[i] = server.submit(s, q["src"], q["n"], d.ptr, d.nbytes,
-                                       hbm_bytes=q["n"] * 40 + d.nbytes, shape=shape_of["scan"])
+            rc, _, _ = s.exec_device(_Addr(q["src"]), q["n"], d, d.nbytes)
         else:
-            tickets[i] = server.submit(s, q["src"], q["n"], hbm_bytes=q["n"] * 28,
-                                       shape=shape_of[q["kind"]],
-                                       batchable=(q["kind"] == "preagg"))
-    th.join()
+            rc, _, _ = s.exec_device(_Addr(q["src"]), q["n"])
+        result = s.final() if rc == 0 and q["kind"] != "scan" else None
+        s.close()
+        end = time.perf_counter()
+        done[i] = Status(rc, (start - (t0 + arrivals[i])) * 1e3, (end - t1) * 1e3,
+                         (end - (t0 + arrivals[i])) * 1e3, 0, result if keep_results else None)
     return done, time.perf_counter() - t0
+
+
+class _Addr:
+    """A device address where the session API takes a device buffer."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr

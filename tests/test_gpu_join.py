@@ -130,3 +130,40 @@ def test_join_projection_multi_match(gpu, jit):
     assert len(got) == 1
     assert np.array_equal(got[0], exp)
     assert res.nitems_out > n
+
+
+@pytest.mark.parametrize("join_passes", PASSES, ids=PASSES_IDS)
+def test_join_groupby_heavy_duplicates(gpu, join_passes):
+    """One join key held by 600 inner items (a chain longer than the exact
+    multiplicity count covers): every outer row with that key matches all
+    of them, and the passes size their records for it."""
+    r = datagen.rng(12)
+    n = 20_000
+    outer = [kds.Column("k", "int4", r.integers(0, 10, size=n, dtype=np.int32)),
+             kds.Column("v", "int8", r.integers(0, 1000, size=n, dtype=np.int64))]
+    keys = np.concatenate([np.full(600, 3, dtype=np.int32),
+                           r.integers(100, 200, size=200, dtype=np.int32)])
+    rows = [(int(k), int(i % 5), bytes("t%d" % i, "ascii")) for i, k in enumerate(keys)]
+    hashes = hashing.hash_keys_np([(keys, 4)])
+    inner = kds.build_hash_kds([("k", "int4"), ("g", "int2"), ("t", "text")], rows,
+                               [int(h) for h in hashes])
+    km = kds.build_multirels([(inner, False)])
+    plan = cg.Plan([("k", "int4"), ("v", "int8")])
+    k = plan.scan_var("k")
+    v = plan.scan_var("v")
+    d = plan.add_inner([("k", "int4"), ("g", "int2"), ("t", "text")])
+    ik = plan.inner_var(d, "k")
+    g = plan.inner_var(d, "g")
+    plan.add_join(d, [k], [cg.Func("int4eq", k, ik)])
+    plan.set_groupby([
+        (abi.KAGG_ACTION__NROWS_ANY, None, ("nrows", "int8")),
+        (abi.KAGG_ACTION__PSUM_INT64, v, ("psum_v", "bytea")),
+        (abi.KAGG_ACTION__VREF, g, ("g", "int2")),
+    ])
+    blob = plan.build(final_nrows=16)
+    src = kds.build_arrow_kds(outer)
+    acts = [abi.KAGG_ACTION__NROWS_ANY, abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__VREF]
+    got = run_gpu_groupby(blob, [src], kmrels=km, tuning=dict(jit=xpuhip.JIT_ON, join_passes=join_passes))
+    exp = run_oracle_groupby(blob, [src], kmrels=km)
+    groups = assert_groups_equal(got, exp, acts)
+    assert sum(g[0] for g in groups.values()) == 600 * int(np.sum(outer[0].values == 3))

@@ -23,55 +23,40 @@ def resident(gpu):
     res.free()
 
 
-def results_of(reqs, backends, dsts):
-    out = []
-    for i, q in enumerate(reqs):
-        s = backends.sess[i][q["kind"]]
-        if q["kind"] == "scan":
-            out.append([r for r in kds.parse_row_kds(dsts[i].to_host())])
-        else:
-            out.append(s.final())
-    return out
-
-
-def run_form(resident, reqs, slots, max_batch, burst):
+def run_served(resident, reqs, slots, max_batch):
     backends = serving.Backends(resident, len(reqs))
     dsts = [xpuhip.DeviceBuffer(serving.dst_bytes(q["n"])) if q["kind"] == "scan" else None
             for q in reqs]
     srv = xpuhip.Server(hbm_budget=200 << 30, slots=slots, max_batch=max_batch)
-    if burst:
-        arrivals = [0.0] * len(reqs)
-        st, _ = serving.run(srv, backends, reqs, arrivals, dsts)
-    else:
-        st = []
-        for i, q in enumerate(reqs):
-            one, _ = serving.run(srv, _Single(backends, i), [q], [0.0], [dsts[i]])
-            st += one
+    st, _ = serving.run(srv, backends, reqs, [0.0] * len(reqs), dsts, keep_results=True)
     stats = srv.stats()
     srv.close()
-    assert all(s.rc == 0 for s in st), [s.errbuf.message for s in st if s.rc]
-    got = results_of(reqs, backends, dsts)
     backends.close()
-    for d in dsts:
-        if d is not None:
-            d.free()
-    return got, st, stats
+    return st, dsts, stats
 
 
-class _Single:
-    """The sessions of request i alone (its own backend)."""
-
-    def __init__(self, backends, i):
-        self.sess = [backends.sess[i]]
+def run_alone(resident, reqs):
+    dsts = [xpuhip.DeviceBuffer(serving.dst_bytes(q["n"])) if q["kind"] == "scan" else None
+            for q in reqs]
+    st, _ = serving.run_plain(resident, reqs, [0.0] * len(reqs), dsts, keep_results=True)
+    return st, dsts
 
 
 def test_served_equals_alone(resident):
+    """Queries of many backends arriving at once (so group-by fragments of
+    different backends share launches) give each query the result of
+    running it alone, in its own session."""
     reqs = serving.make_requests(resident, 40, 2_000, 600_000, seed=7)
-    alone, _, _ = run_form(resident, reqs, slots=1, max_batch=1, burst=False)
-    mix, st, stats = run_form(resident, reqs, slots=8, max_batch=16, burst=True)
+    alone, adst = run_alone(resident, reqs)
+    mix, mdst, stats = run_served(resident, reqs, slots=8, max_batch=16)
+    assert all(s.rc == 0 for s in alone), [s.err for s in alone if s.rc]
+    assert all(s.rc == 0 for s in mix), [s.err for s in mix if s.rc]
     assert stats["coalesced"] > 0, stats     # group-by fragments shared launches
-    for q, a, b in zip(reqs, alone, mix):
+    for q, a, b, da, db in zip(reqs, alone, mix, adst, mdst):
         if q["kind"] == "scan":
-            assert a == b
+            assert kds.parse_row_kds(da.to_host()) == kds.parse_row_kds(db.to_host())
         else:
-            assert_groups_equal(b, a, C1_ACTIONS if q["kind"] == "preagg" else C2_ACTIONS)
+            assert_groups_equal(b.result, a.result, C1_ACTIONS if q["kind"] == "preagg" else C2_ACTIONS)
+    for d in adst + mdst:
+        if d is not None:
+            d.free()
