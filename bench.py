@@ -133,6 +133,21 @@ def bench_star(n, warmup, steps):
     return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo)
 
 
+def scan_line(sc):
+    return dict(workload="GpuScan 1M rows: WHERE a < 500000 AND b > 0.0 projecting (c, d)",
+                rows_per_s=sc["rows"] / (sc["kernel_ms"] / 1e3), kernel_ms=sc["kernel_ms"],
+                wall_ms=sc["wall_ms"], hbm_gbs=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / 1e9,
+                frac=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / HBM_PEAK, rows_out=sc["nout"])
+
+
+def join_line(st):
+    return dict(workload="star GpuJoin + GpuPreAgg 200M fact rows, dims 2M/200K/2556 (10% filters), "
+                         "GROUP BY (dim1.category, dim3.category) sum(quantity*price)",
+                rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3), kernel_ms=st["kernel_ms"],
+                wall_ms=st["wall_ms"], hbm_gbs=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
+                frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK)
+
+
 class _DevPtr:
     def __init__(self, ptr):
         self.ptr = ptr
@@ -327,7 +342,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--only", default="all", choices=["all", "preagg", "batch", "served"])
+    ap.add_argument("--only", default="all",
+                    choices=["all", "preagg", "batch", "served", "join", "scan"])
     ap.add_argument("--served-requests", type=int, default=300)
     ap.add_argument("--served-table-rows", type=int, default=50_000_000)
     ap.add_argument("--served-slots", type=int, default=8)
@@ -353,6 +369,13 @@ def main():
     if args.only == "batch":
         print(json.dumps(bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)))
         return
+    if args.only == "join":
+        st = bench_star(200_000_000, args.warmup, args.steps)
+        print(json.dumps(join_line(st)))
+        return
+    if args.only == "scan":
+        print(json.dumps(scan_line(bench_scan(1_000_000, args.warmup, args.steps))))
+        return
     if args.only == "served":
         print(json.dumps(bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
                                       args.served_table_rows, args.served_slots,
@@ -365,20 +388,11 @@ def main():
     traffic, traffic_src = traffic_of("preagg_c1_traffic.json")
     others = {}
     if args.only == "all":
-        sc = bench_scan(1_000_000, 3, 20)
-        others["scan_1M"] = dict(rows_per_s=sc["rows"] / (sc["kernel_ms"] / 1e3),
-                                 kernel_ms=sc["kernel_ms"], wall_ms=sc["wall_ms"],
-                                 hbm_gbs=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / 1e9,
-                                 frac=sc["algo_bytes"] / (sc["kernel_ms"] / 1e3) / HBM_PEAK,
-                                 rows_out=sc["nout"])
+        others["scan_1M"] = scan_line(bench_scan(1_000_000, 3, 20))
         others["batched_256"] = bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)
         others["served"] = bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
                                         args.served_table_rows)
-        st = bench_star(200_000_000, 1, 5)
-        others["star_join_200M"] = dict(rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3),
-                                        kernel_ms=st["kernel_ms"], wall_ms=st["wall_ms"],
-                                        hbm_gbs=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
-                                        frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK)
+        others["star_join_200M"] = join_line(bench_star(200_000_000, 1, 5))
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline")
