@@ -122,8 +122,9 @@ xpu_jit_source(const xplan *P, int kind, int param)
             emit_segment(s, P, P->seg_scan.begin + P->stage_prefix, P->seg_final.end);
     }
     s += "    }\n};\n\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
-         "xpu_jit_kernel(xpu_kparams kp)\n{\n";
+    append(s, "extern \"C\" __global__ void __launch_bounds__(%s)\n"
+              "xpu_jit_kernel(xpu_kparams kp)\n{\n",
+           kind == XJ_SCAN_AGG ? "XP_AGG_BLOCK_MAX" : "XP_BLOCK_MAX");
     int nr = (P->nregs <= XP_NREGS_SMALL) ? XP_NREGS_SMALL : XP_NREGS;
     if (kind == XJ_SCAN_AGG)
         append(s, "    scan_agg_body<%d, %d, JitProg>((cplan *)xjit_plan, kp);\n", nr, param);
@@ -169,7 +170,7 @@ xpu_jit_source(const xplan *P, int kind, int param)
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
              "xpu_jit_dump_kernel(xpu_kparams kp)\n{\n";
         append(s, "    dump_merge_body<%d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr);
-        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"
+        s += "\nextern \"C\" __global__ void __launch_bounds__(XP_AGG_BLOCK_MAX)\n"
              "xpu_jit_batch_kernel(xpu_kparams kp)\n{\n";
         append(s, "    batch_agg_body<%d, %d, JitProg>((cplan *)xjit_plan, kp);\n}\n", nr, param);
         s += "\nextern \"C\" __global__ void __launch_bounds__(XP_BLOCK_MAX)\n"

@@ -22,6 +22,8 @@
 #define XSPILL_CHUNK        512     /* spill records a wave reserves at once */
 #define XP_DUMP_MAXPARTS    256     /* partial-state dump partitions at most */
 #define XP_BLOCK_MAX        512     /* threads per block at most */
+#define XP_AGG_BLOCK_MAX    1024    /* ... of the scan + group-by kernels (one block per
+                                       CU sharing the CU's LDS table) */
 /* group-by joins: 64-row batches a wave screens before running the joins
  * over the rows that pass, and the static LDS of the pipeline kernel (the
  * screened-row queues and the block's counters) */

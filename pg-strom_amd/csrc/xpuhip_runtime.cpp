@@ -310,9 +310,11 @@ default_tuning(const xpuhip_session *s, xpuhip_tuning *t)
     memset(t, 0, sizeof(*t));
     if (uses_scan_agg(s))
     {
-        /* one 512-thread block per CU owning (nearly) all of the CU's LDS as
-         * its partial table; 4 rows per lane per wave tile */
-        t->block_size = 512;
+        /* one 1024-thread block per CU owning (nearly) all of the CU's LDS as
+         * its partial table; 4 rows per lane per wave tile (C1, 100M rows:
+         * scan pass 1.23 ms against 1.55 ms with 512 threads; the
+         * interpreter's kernels run 512) */
+        t->block_size = XP_AGG_BLOCK_MAX;
         t->rows_per_thread = 4;
         t->blocks_per_cu = 1;
         size_t budget = g_dev.lds_per_block > 4096 ? g_dev.lds_per_block - 4096 : 0;
@@ -915,6 +917,9 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
         else if (t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
+    /* the interpreter's kernels take blocks of up to XP_BLOCK_MAX threads */
+    if (!jf && block > XP_BLOCK_MAX)
+        block = XP_BLOCK_MAX;
     /* a projection without joins runs over wave tiles of 64*XP_PROJ_RPT rows */
     const bool sproj = !preagg && P->nrels == 0 && jf && jf->scount;
     uint32_t rows_per_tile = sproj ? 64 * XP_PROJ_RPT : block * rpt;
@@ -1112,14 +1117,16 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     {
         (void)hipEventRecord(x->evm, x->stream);
         uint32_t sgrid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
+        /* the spill and dump passes run blocks of at most XP_BLOCK_MAX threads */
+        const uint32_t sblock = block < XP_BLOCK_MAX ? block : XP_BLOCK_MAX;
         if (jfn && jsfn)
         {
             void *args[] = {&kp};
-            rc = (int)hipModuleLaunchKernel(jsfn, sgrid, 1, 1, block, 1, 1, (unsigned)lds,
+            rc = (int)hipModuleLaunchKernel(jsfn, sgrid, 1, 1, sblock, 1, 1, (unsigned)lds,
                                             x->stream, args, nullptr);
         }
         else
-            rc = xpu_launch_spill_agg(&kp, P->nregs, sgrid, block, lds, x->stream);
+            rc = xpu_launch_spill_agg(&kp, P->nregs, sgrid, sblock, lds, x->stream);
         /* the partial-state dumps of both passes, one block per partition */
         if (rc == 0 && kp.dump)
         {
@@ -1127,11 +1134,11 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
             if (jfn && jdump)
             {
                 void *args[] = {&kp};
-                rc = (int)hipModuleLaunchKernel(jdump, dgrid, 1, 1, block, 1, 1, (unsigned)lds,
+                rc = (int)hipModuleLaunchKernel(jdump, dgrid, 1, 1, sblock, 1, 1, (unsigned)lds,
                                                 x->stream, args, nullptr);
             }
             else
-                rc = xpu_launch_dump_merge(&kp, P->nregs, dgrid, block, lds, x->stream);
+                rc = xpu_launch_dump_merge(&kp, P->nregs, dgrid, sblock, lds, x->stream);
         }
     }
     (void)hipEventRecord(x->ev1, x->stream);
@@ -1357,7 +1364,8 @@ xpuhip_session_set_tuning(xpuhip_session *s, const xpuhip_tuning *t)
     if (n.rows_per_thread == 0) n.rows_per_thread = d.rows_per_thread;
     if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
     if (n.lds_nents == 0) n.lds_nents = d.lds_nents;
-    if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > 512 ||
+    const uint32_t max_block = uses_scan_agg(s) ? XP_AGG_BLOCK_MAX : XP_BLOCK_MAX;
+    if (n.block_size % 64 != 0 || n.block_size < 64 || n.block_size > max_block ||
         n.rows_per_thread > 64 || n.blocks_per_cu > 64 || n.jit > XPUHIP_JIT_OFF ||
         n.join_filter > XPUHIP_JOIN_FILTER_OFF || n.join_passes > XPUHIP_JOIN_PASSES_OFF)
         return ERRCODE_DEVICE_ERROR;
@@ -1541,7 +1549,7 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
     }
     const xplan *P = &s0->comp.plan;
     const xpuhip_tuning t = s0->tuning;
-    const uint32_t block = t.block_size ? t.block_size : 512;
+    uint32_t block = t.block_size ? t.block_size : 512;
     const uint32_t rpt = t.rows_per_thread >= 4 ? 4 : (t.rows_per_thread == 2 ? 2 : 1);
     const uint32_t grid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
     const uint32_t lds_nents = P->lds_fast ? t.lds_nents : 0;
@@ -1625,6 +1633,9 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
         if (!jf && t.jit == XPUHIP_JIT_ON)
             return ERRCODE_DEVICE_ERROR;
     }
+    /* the interpreter's kernels take blocks of up to XP_BLOCK_MAX threads */
+    if (!jf && block > XP_BLOCK_MAX)
+        block = XP_BLOCK_MAX;
 
     const bool replay = (flags & XPUHIP_BATCH_GRAPH) && b->gexec && b->sig == sig;
     if (!replay)
@@ -1702,7 +1713,8 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
             if (!rc)
                 rc = xpu_launch_batch_setup(&kp, b->stream);
             if (!rc)
-                rc = (int)hipModuleLaunchKernel(jf->batch_spill, grid, 1, 1, block, 1, 1,
+                rc = (int)hipModuleLaunchKernel(jf->batch_spill, grid, 1, 1,
+                                                block < XP_BLOCK_MAX ? block : XP_BLOCK_MAX, 1, 1,
                                                 (unsigned)lds, b->stream, args, nullptr);
             return rc;
         }
@@ -1838,12 +1850,13 @@ tuning_candidates(const xpuhip_session *s)
     };
     if (uses_scan_agg(s))
     {
+        add(1024, 2, 1, 1.0);
+        add(1024, 4, 1, 1.0);
         add(512, 4, 1, 1.0);
         add(512, 2, 1, 1.0);
         add(256, 4, 1, 1.0);
         add(512, 4, 1, 0.75);
         add(512, 4, 2, 1.0);
-        add(256, 4, 2, 1.0);
     }
     else if (s->comp.has_groupby)
     {
@@ -1855,10 +1868,10 @@ tuning_candidates(const xpuhip_session *s)
     }
     else
     {
-        add(512, 1, 8, 0.0);
-        add(256, 1, 8, 0.0);
-        add(512, 1, 4, 0.0);
+        add(256, 1, 2, 0.0);
         add(256, 1, 4, 0.0);
+        add(512, 1, 2, 0.0);
+        add(512, 1, 4, 0.0);
     }
     return c;
 }
