@@ -39,10 +39,13 @@ def test_c1_preagg_multi_chunk(gpu, jit):
                                     dict(rows_per_thread=2, block_size=192),
                                     dict(rows_per_thread=4, block_size=512, blocks_per_cu=2,
                                          lds_nents=1500),
-                                    dict(rows_per_thread=4, block_size=512)])
+                                    dict(rows_per_thread=4, block_size=512),
+                                    dict(rows_per_thread=4, block_size=1024),
+                                    dict(rows_per_thread=2, block_size=1024, lds_nents=64)])
 def test_c1_preagg_tunings(gpu, tuning, jit):
     """LDS table overflow (64 and 7 entries) and launch shapes (rows per lane,
-    partial waves per block) give the same groups."""
+    partial waves per block, 1024-thread blocks, which the interpreter runs
+    as 512) give the same groups."""
     cols = datagen.preagg_table(400_000, fragment=7)
     src = kds.build_arrow_kds(cols)
     blob = W.preagg_query(cols)
