@@ -1982,6 +1982,7 @@ pick_u32(const uint32_t (&a)[RPT], int i)
 template <int RPT>
 struct StagedCols
 {
+    static_assert(XP_MAX_STAGE * RPT <= 32, "validity bits of a tile fit one word");
     /* geometry of the staged columns (uniform: scalar registers) */
     const char *kbase;
     int         nstage;
@@ -1989,11 +1990,13 @@ struct StagedCols
     uint64_t    vlen[XP_MAX_STAGE], nmoff[XP_MAX_STAGE], nmlen[XP_MAX_STAGE];
     int         w[XP_MAX_STAGE], reg[XP_MAX_STAGE];
     bool        f32[XP_MAX_STAGE];
-    /* the rows [base, base + 64*RPT): raw words and validity-bitmap words */
+    /* the rows [base, base + 64*RPT): raw words, and the lane's validity
+     * bits (bit st*RPT + i: row i of column st is not NULL by its bitmap);
+     * 'full': every row of the tile lies inside every staged column */
     uint64_t    base;
-    bool        nm_words;
+    bool        full;
     uint32_t    lo[XP_MAX_STAGE][RPT], hi[XP_MAX_STAGE][RPT];
-    uint64_t    nmw[XP_MAX_STAGE][RPT];
+    uint32_t    vbits;
 
     __device__ __forceinline__ void
     setup(cplan *P, const kern_data_store *kds)
@@ -2033,7 +2036,7 @@ struct StagedCols
         const int lane = __lane_id();
         const uint64_t wrows = 64ULL * RPT;
         base = tile_base;
-        bool full = base + wrows <= row_end;
+        full = base + wrows <= row_end;
 #pragma unroll
         for (int st = 0; st < XP_MAX_STAGE; st++)
             if (st < nstage)
@@ -2112,20 +2115,33 @@ struct StagedCols
             }
         }
         /* validity bitmaps: one 64-bit word per 64 rows, read with scalar
-         * loads when the tile is 64-row aligned inside the bitmap */
-        nm_words = ((base & 63) == 0);
+         * loads when the tile is 64-row aligned inside the bitmap, else one
+         * byte per lane; kept as one bit per (column, row batch) */
+        const bool nm_words = ((base & 63) == 0);
+        vbits = ~0U;
 #pragma unroll
         for (int st = 0; st < XP_MAX_STAGE; st++)
         {
+            const uint64_t nm_off = nmoff[st];
+            if (st >= nstage || !nm_off)
+                continue;
 #pragma unroll
             for (int i = 0; i < RPT; i++)
             {
-                nmw[st][i] = ~0ULL;
-                const uint64_t nm_off = nmoff[st];
                 const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
-                if (st < nstage && nm_off && nm_words && (nm_off & 7) == 0 &&
-                    byte + 8 <= nmlen[st])
-                    nmw[st][i] = *(const XCONST uint64_t *)(uintptr_t)(kbase + nm_off + byte);
+                uint32_t bit;
+                if (nm_words && (nm_off & 7) == 0 && byte + 8 <= nmlen[st])
+                {
+                    const uint64_t word = *(const XCONST uint64_t *)(uintptr_t)(kbase + nm_off + byte);
+                    bit = (uint32_t)(word >> lane) & 1;
+                }
+                else
+                {
+                    const uint64_t row = base + (uint64_t)i * 64 + lane;
+                    const uint64_t idx = row >> 3;
+                    bit = idx < nmlen[st] ? (((const uint8_t *)kbase)[nm_off + idx] >> (row & 7)) & 1 : 0;
+                }
+                vbits &= ~((bit ^ 1U) << (st * RPT + i));
             }
         }
     }
@@ -2143,18 +2159,9 @@ struct StagedCols
                 const int ww = w[st];
                 const uint32_t l = pick_u32<RPT>(lo[st], i);
                 const uint32_t h = pick_u32<RPT>(hi[st], i);
-                bool isnull = !valid || (uint64_t)ww * (row + 1) > vlen[st];
-                const uint64_t nm_off = nmoff[st];
-                if (nm_off)
-                {
-                    const uint64_t idx = row >> 3;
-                    const uint64_t byte = (base + (uint64_t)i * 64) >> 3;
-                    if (nm_words && (nm_off & 7) == 0 && byte + 8 <= nmlen[st])
-                        isnull = isnull || !((nmw[st][i] >> lane) & 1);
-                    else
-                        isnull = isnull || idx >= nmlen[st] ||
-                                 !(((((const uint8_t *)kbase)[nm_off + idx]) >> (row & 7)) & 1);
-                }
+                /* a full tile needs no per-row bounds test */
+                bool isnull = full ? false : (!valid || (uint64_t)ww * (row + 1) > vlen[st]);
+                isnull = isnull || !((vbits >> (st * RPT + i)) & 1);
                 int64_t v = (ww == 8) ? (int64_t)(((uint64_t)h << 32) | l)
                                       : (f32[st] ? (int64_t)l : (int64_t)(int32_t)l);
                 L.R[reg[st]] = isnull ? 0 : v;
