@@ -226,6 +226,65 @@ typedef struct kern_fallbackitem
     kern_tupitem t XPU_MAXALIGNED;
 } kern_fallbackitem;
 
+/* Transaction attributes embedded in a kern_tupitem of a KDS_FORMAT_ROW
+ * source, in front of its payload (xpu_common.h:1199-1205, 1267-1283) */
+typedef struct
+{
+    uint32_t    xmin;
+    uint32_t    xmax;
+    uint16_t    __padding__;
+    uint16_t    ctid[3];        /* ItemPointerData: bi_hi, bi_lo, ip_posid */
+} kern_tupitem_xact_attrs;
+
+#define InvalidTransactionId    0U
+#define FrozenTransactionId     2U
+
+/* SerializedTransactionState at session_xact_state (xpu_common.h:3156-3165) */
+typedef struct
+{
+    int32_t     xactIsoLevel;
+    bool        xactDeferrable;
+    uint64_t    topFullTransactionId;
+    uint64_t    currentFullTransactionId;
+    uint32_t    currentCommandId;
+    int32_t     nParallelCurrentXids;
+    uint32_t    parallelCurrentXids[1];
+} SerializedTransactionState;
+
+/* ---------------------------------------------------------------
+ * Heap pages of a KDS_FORMAT_BLOCK chunk (xpu_common.h:1058-1134,
+ * 1478-1483): nitems pages of BLCKSZ bytes at block_offset, their block
+ * numbers (u32) at the KDS body.  A page is PageHeaderData (24 bytes) and
+ * its ItemIdData line pointers; tuples are HeapTupleHeaderData.
+ * --------------------------------------------------------------- */
+#define XPU_PAGE_HEADER_SIZE    24      /* SizeOfPageHeaderData */
+#define XPU_PAGE_PD_LOWER       12      /* offsetof(PageHeaderData, pd_lower) */
+#define LP_UNUSED               0
+#define LP_NORMAL               1
+#define LP_REDIRECT             2
+#define LP_DEAD                 3
+/* ItemIdData bit fields: lp_off:15, lp_flags:2, lp_len:15 */
+#define XPU_ITEMID_OFF(lp)      ((uint32_t)(lp) & 0x7fffU)
+#define XPU_ITEMID_FLAGS(lp)    (((uint32_t)(lp) >> 15) & 0x3U)
+#define XPU_ITEMID_LEN(lp)      ((uint32_t)(lp) >> 17)
+/* HeapTupleHeaderData (xpu_common.h:906-938) */
+#define XPU_HTUP_CTID           12
+#define XPU_HTUP_INFOMASK2      18
+#define XPU_HTUP_INFOMASK       20
+#define XPU_HTUP_HOFF           22
+#define XPU_HTUP_BITS           23
+#define XPU_SIZEOF_HTUP_HEADER  23
+/* size of an external datum (1-byte header 0x01, then its tag):
+ * VARSIZE_EXTERNAL (xpu_common.h:1743-1751), 2 + sizeof(varatt_indirect)
+ * or 2 + sizeof(varatt_external), 2 for other tags */
+#define XPU_VARTAG_INDIRECT     1
+#define XPU_VARTAG_ONDISK       18
+XPU_INLINE uint32_t
+XPU_VARSIZE_EXTERNAL(uint8_t tag)
+{
+    return tag == XPU_VARTAG_INDIRECT ? 10 : tag == XPU_VARTAG_ONDISK ? 18 : 2;
+}
+
 /* Hash-slot width chosen by the host for a KDS_FORMAT_HASH buffer
  * (xpu_common.h:1389-1396). */
 XPU_INLINE uint64_t

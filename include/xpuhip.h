@@ -97,7 +97,8 @@ int64_t xpuhip_session_explain(const kern_session_info *session, size_t length,
 int     xpuhip_session_attach_inner(xpuhip_session *sess, const kern_multirels *kmrels,
                                     size_t length, kern_errorbuf *errbuf);
 
-/* XpuTaskExec on a host-resident chunk (KDS_FORMAT_ARROW): copies it to the
+/* XpuTaskExec on a host-resident chunk (KDS_FORMAT_ARROW, or a heap-page /
+ * row chunk as xpuhip_task_exec_device_tuples describes): copies it to the
  * device, runs the pipeline, and for a scan/join projection copies the rows
  * into kds_dst (KDS_FORMAT_ROW, dst_length bytes, header from the session).
  * *resume_pos must be 0 for a new chunk.  Replaces gpuservHandleGpuTaskExec
@@ -116,6 +117,23 @@ int     xpuhip_task_exec_device(xpuhip_session *sess,
                                 kern_data_store *d_kds_dst, size_t dst_length,
                                 uint64_t *resume_pos,
                                 kern_exec_results *results, kern_errorbuf *errbuf);
+
+/* The same on a heap-page chunk (KDS_FORMAT_BLOCK: kds_head->nitems pages of
+ * BLCKSZ bytes with their line pointers, as relscan.c:413-742 builds it) or a
+ * row chunk (KDS_FORMAT_ROW / KDS_FORMAT_HASH: kds_head->nitems row-index
+ * entries) already in device memory at d_kds_src.  kds_head is a host copy
+ * of the chunk's header and column metadata (KDS_HEAD_LENGTH bytes).  The
+ * tuples the reference's scan would load - LP_NORMAL items of each page
+ * (cuda_gpuscan.cu:405-534), visible rows of a row chunk (:610-700) - are
+ * deformed on the device into column tiles, then run as an Arrow chunk of
+ * that many rows in page / row-index order (*resume_pos counts those rows).
+ * xpuhip_task_exec takes such chunks from host memory the same way. */
+int     xpuhip_task_exec_device_tuples(xpuhip_session *sess,
+                                       const kern_data_store *d_kds_src,
+                                       const kern_data_store *kds_head,
+                                       kern_data_store *d_kds_dst, size_t dst_length,
+                                       uint64_t *resume_pos,
+                                       kern_exec_results *results, kern_errorbuf *errbuf);
 
 /* XpuTaskFinal of a group-by session: writes the session's kds_final into
  * kds_final (host memory) as the service writes it back, HASH converted to
@@ -254,6 +272,9 @@ typedef struct
     uint64_t    spill_rows;         /* rows aggregated by the second (spill) pass */
     uint32_t    jit;                /* 1: the kernel specialized for the plan ran */
     uint32_t    __pad;
+    double      heap_ms;            /* heap-page / row chunk: its count, scan and
+                                       deform passes (0 for Arrow chunks) */
+    uint64_t    heap_rows;          /* tuples those passes loaded */
 } xpuhip_task_stats;
 
 int     xpuhip_last_task_stats(xpuhip_task_stats *stats);

@@ -12,6 +12,9 @@
  *   datum operators      xpu_basetype.cu:325-772 (casts, NaN-aware compares,
  *                        overflow-checked arithmetic), xpu_textlib.cu:321-380
  *   hash                 xpu_common.cu:2920-3041 (lookup3, PG seed)
+ *   tuple sources        cuda_gpuscan.cu:405-534 (BLOCK pages: LP_NORMAL
+ *                        items), :610-700 (ROW chunks: visibility),
+ *                        xpu_common.cu:194-257 (heap tuple payload)
  *   column loads         xpu_basetype.cu:242-306 (Arrow fixed width),
  *                        xpu_textlib.cu:237-274 (Arrow utf8/binary),
  *                        xpu_common.cu:60-130 (minimal-tuple deform)
@@ -839,6 +842,224 @@ exec_loadvars_tuple(octx *c, const kern_expression *kload, const kern_data_store
 }
 
 /* ------------------------------------------------------------------
+ * heap tuples of KDS_FORMAT_BLOCK pages and KDS_FORMAT_ROW/HASH row chunks
+ * ------------------------------------------------------------------ */
+
+/* one tuple of a heap / row chunk: its HeapTupleHeaderData (for a
+ * kern_tupitem, the virtual header MINIMAL_TUPLE_OFFSET bytes before it) and
+ * the length of its item from there */
+typedef struct
+{
+    const uint8_t  *h;
+    uint32_t        len;
+} otuple;
+
+/* VARSIZE_ANY (xpu_common.h:1812-1815, 1743-1751) */
+static uint32_t
+varsize_any(const uint8_t *p)
+{
+    if (p[0] == 0x01)
+        return 2 + (p[1] == 1 ? 8 : p[1] == 18 ? 16 : 0);
+    if (p[0] & 0x01)
+        return p[0] >> 1;
+    uint32_t w;
+    memcpy(&w, p, 4);
+    return (w >> 2) & 0x3fffffff;
+}
+
+/* attribute `resno` of a heap tuple, walked as __kern_extract_common_tuple
+ * (xpu_common.cu:60-130) on the payload kern_extract_heap_tuple hands it
+ * (:194-257): ncols = Min(natts, kds->ncols), the null bitmap when
+ * HEAP_HASNULL, positions aligned relative to the payload */
+static bool
+load_heap_attr(octx *c, const kern_data_store *kds, const otuple *tp, int resno, int type, od *r)
+{
+    const uint8_t *h = tp->h;
+    uint16_t im2, im;
+    memcpy(&im2, h + 18, 2);
+    memcpy(&im, h + 20, 2);
+    int ncols = im2 & HEAP_NATTS_MASK;
+    if (ncols > (int)kds->ncols)
+        ncols = kds->ncols;
+    const uint8_t *nullmap = (im & HEAP_HASNULL) ? h + 23 : NULL;
+    uint32_t hoff = h[22];
+    const uint8_t *payload = h + hoff;
+    uint32_t plen = tp->len > hoff ? tp->len - hoff : 0;
+    uint64_t off = 0;
+
+    memset(r, 0, sizeof(od));
+    if (hoff < 23 || hoff > tp->len)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: heap tuple header out of its item", "oracle");
+    for (int resno_j = 1; resno_j <= ncols && resno_j <= resno; resno_j++)
+    {
+        int j = resno_j - 1;
+        const kern_colmeta *cm = &kds->colmeta[j];
+        if (nullmap && !(nullmap[j >> 3] & (1 << (j & 7))))
+        {
+            if (resno_j == resno)
+            {
+                r->isnull = true;
+                return true;
+            }
+            continue;
+        }
+        if (cm->attlen > 0)
+            off = XPU_TYPEALIGN(cm->attalign, off);
+        else if (off < plen && payload[off] == 0)
+            off = XPU_TYPEALIGN(cm->attalign, off);
+        if (off >= plen)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: heap tuple attribute beyond its item", "oracle");
+        uint64_t sz = cm->attlen > 0 ? (uint64_t)cm->attlen : varsize_any(payload + off);
+        if (off + sz > plen || (cm->attlen <= 0 && cm->attlen != -1))
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: heap tuple attribute beyond its item", "oracle");
+        if (resno_j == resno)
+            return datum_from_image(c, type, (const char *)payload + off, r);
+        off += sz;
+    }
+    r->isnull = true;
+    return true;
+}
+
+static bool
+exec_loadvars_heap(octx *c, const kern_expression *kload, const kern_data_store *kds,
+                   const otuple *tp)
+{
+    if (!kload)
+        return true;
+    for (int i = 0; i < kload->u.load.nitems; i++)
+    {
+        int resno = kload->u.load.desc[i].vl_resno;
+        int slot = kload->u.load.desc[i].vl_slot_id;
+        if (resno <= 0 || resno > (int)kds->ncols)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: resno out of range", "LoadVars");
+        if (!load_heap_attr(c, kds, tp, resno, c->vdesc[slot].vs_type_code, &c->slots[slot]))
+            return false;
+    }
+    return true;
+}
+
+/* __gpuscan_check_row_visibility (cuda_gpuscan.cu:614-642) against the
+ * session's parallelCurrentXids */
+static bool
+row_visible(const kern_session_info *s, size_t slen, const uint8_t *t)
+{
+    if (!t[8])
+        return true;
+    uint32_t xmin, xmax, n = 0;
+    const uint32_t *xids = NULL;
+    const uint8_t *xa = t + t[14] - MINIMAL_TUPLE_OFFSET - sizeof(kern_tupitem_xact_attrs);
+    memcpy(&xmin, xa, 4);
+    memcpy(&xmax, xa + 4, 4);
+    if (s->session_xact_state &&
+        s->session_xact_state + offsetof(SerializedTransactionState, parallelCurrentXids) <= slen)
+    {
+        const SerializedTransactionState *xs =
+            (const SerializedTransactionState *)((const char *)s + s->session_xact_state);
+        n = xs->nParallelCurrentXids > 0 ? (uint32_t)xs->nParallelCurrentXids : 0;
+        xids = xs->parallelCurrentXids;
+    }
+    if (xmin == InvalidTransactionId)
+        return false;
+    if (xmin != FrozenTransactionId)
+    {
+        bool found = false;
+        for (uint32_t i = 0; i < n; i++)
+            if (xids[i] == xmin)
+                found = true;
+        if (!found)
+            return false;
+    }
+    if (xmax == InvalidTransactionId)
+        return true;
+    if (xmax == FrozenTransactionId)
+        return false;
+    for (uint32_t i = 0; i < n; i++)
+        if (xids[i] == xmax)
+            return false;
+    return true;
+}
+
+/* The tuples a scan of the chunk loads, in order: the LP_NORMAL items of
+ * each page (cuda_gpuscan.cu:479-507; PageGetMaxOffsetNumber,
+ * xpu_common.h:1123-1127), or the visible entries of the row index
+ * (:662-672).  Returns the count, or -1 with the error set. */
+static void set_error(kern_errorbuf *eb, uint32_t code, const char *func, const char *msg);
+
+static int64_t
+collect_tuples(const kern_session_info *sess, size_t slen, const kern_data_store *kds,
+               otuple **out, kern_errorbuf *eb)
+{
+    const uint8_t *base = (const uint8_t *)kds;
+    size_t hl = KDS_HEAD_LENGTH(kds);
+    size_t cap = 1024, n = 0;
+    otuple *v = malloc(sizeof(otuple) * cap);
+
+    if (kds->format == KDS_FORMAT_BLOCK)
+    {
+        if (kds->block_offset < hl || kds->block_offset + (uint64_t)XPU_BLCKSZ * kds->nitems > kds->length)
+            goto bad;
+        for (uint32_t b = 0; b < kds->nitems; b++)
+        {
+            const uint8_t *pg = base + kds->block_offset + (uint64_t)XPU_BLCKSZ * b;
+            uint16_t pd_lower;
+            memcpy(&pd_lower, pg + 12, 2);
+            if (pd_lower > XPU_BLCKSZ)
+                goto bad;
+            uint32_t nlp = pd_lower <= 24 ? 0 : (pd_lower - 24) / 4;
+            for (uint32_t k = 0; k < nlp; k++)
+            {
+                uint32_t lp;
+                memcpy(&lp, pg + 24 + 4 * k, 4);
+                if (((lp >> 15) & 3) != LP_NORMAL)
+                    continue;
+                uint32_t off = lp & 0x7fff, len = lp >> 17;
+                if (off < 24 || len < 23 || off + len > XPU_BLCKSZ)
+                    goto bad;
+                if (n == cap)
+                    v = realloc(v, sizeof(otuple) * (cap *= 2));
+                v[n].h = pg + off;
+                v[n].len = len;
+                n++;
+            }
+        }
+    }
+    else
+    {
+        uint64_t ridx_off = hl + 8ULL * kds->hash_nslots;
+        if (ridx_off + 8ULL * kds->nitems > kds->length)
+            goto bad;
+        for (uint32_t i = 0; i < kds->nitems; i++)
+        {
+            uint64_t off;
+            memcpy(&off, base + ridx_off + 8ULL * i, 8);
+            if (off == 0)
+                continue;
+            if (off < 16 || off > kds->length - (ridx_off + 8ULL * kds->nitems))
+                goto bad;
+            const uint8_t *t = base + kds->length - off;
+            uint32_t t_len;
+            memcpy(&t_len, t, 4);
+            if (t_len > off || t[14] < MINIMAL_TUPLE_OFFSET + KERN_TUPITEM_T_BITS_OFFSET ||
+                (uint32_t)t[14] - MINIMAL_TUPLE_OFFSET > t_len)
+                goto bad;
+            if (!row_visible(sess, slen, t))
+                continue;
+            if (n == cap)
+                v = realloc(v, sizeof(otuple) * (cap *= 2));
+            v[n].h = t - MINIMAL_TUPLE_OFFSET;
+            v[n].len = MINIMAL_TUPLE_OFFSET + t_len;
+            n++;
+        }
+    }
+    *out = v;
+    return (int64_t)n;
+bad:
+    free(v);
+    set_error(eb, ERRCODE_DEVICE_ERROR, "collect_tuples", "page item or row-index entry out of the chunk");
+    return -1;
+}
+
+/* ------------------------------------------------------------------
  * session
  * ------------------------------------------------------------------ */
 typedef struct
@@ -1561,8 +1782,8 @@ typedef struct
 } othread;
 
 static void
-run_rows(const oracle_session *os, const kern_data_store *kds_src, const kern_multirels *kmrels,
-         uint32_t row_begin, uint32_t row_end, othread *th)
+run_rows(const oracle_session *os, const kern_data_store *kds_src, const otuple *tups,
+         const kern_multirels *kmrels, uint32_t row_begin, uint32_t row_end, othread *th)
 {
     octx *c = malloc(sizeof(octx));
     orow w;
@@ -1582,7 +1803,8 @@ run_rows(const oracle_session *os, const kern_data_store *kds_src, const kern_mu
             memset(&c->slots[s], 0, sizeof(od));
             c->slots[s].isnull = true;
         }
-        if (!exec_loadvars_arrow(c, os->load[0], kds_src, row))
+        if (tups ? !exec_loadvars_heap(c, os->load[0], kds_src, &tups[row])
+                 : !exec_loadvars_arrow(c, os->load[0], kds_src, row))
             break;
         if (os->scan_quals)
         {
@@ -1609,12 +1831,21 @@ oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
                  int nthreads, kern_exec_results *results, kern_errorbuf *eb)
 {
     uint32_t nitems = kds_src->nitems;
+    otuple *tups = NULL;
     othread *th;
     int rc = 0;
 
-    if (kds_src->format != KDS_FORMAT_ARROW)
+    if (kds_src->format == KDS_FORMAT_BLOCK || kds_src->format == KDS_FORMAT_ROW ||
+        kds_src->format == KDS_FORMAT_HASH)
     {
-        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "oracle reads KDS_FORMAT_ARROW only");
+        int64_t n = collect_tuples(os->session, os->length, kds_src, &tups, eb);
+        if (n < 0)
+            return ERRCODE_DEVICE_ERROR;
+        nitems = (uint32_t)n;
+    }
+    else if (kds_src->format != KDS_FORMAT_ARROW)
+    {
+        set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "oracle reads KDS_FORMAT_ARROW, BLOCK, ROW, HASH");
         return ERRCODE_DEVICE_ERROR;
     }
     if (os->nrels > 0 && (!kmrels || (int)kmrels->num_inner_rels < os->nrels))
@@ -1630,7 +1861,7 @@ oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
     {
         uint64_t b = (uint64_t)nitems * t / nthreads;
         uint64_t e = (uint64_t)nitems * (t + 1) / nthreads;
-        run_rows(os, kds_src, kmrels, (uint32_t)b, (uint32_t)e, &th[t]);
+        run_rows(os, kds_src, tups, kmrels, (uint32_t)b, (uint32_t)e, &th[t]);
     }
     if (results)
     {
@@ -1729,6 +1960,7 @@ oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
         free(th[t].gt.slots);
     }
     free(th);
+    free(tups);
     return rc;
 }
 

@@ -7,11 +7,13 @@
  * oracle/_ref/libpgstrom_ref.so by oracle/Makefile.  It is never part of the
  * product: only tests and tests/golden generators load it, to
  *   - dump the reference's struct layouts and opcode numbers (ref_abi_json),
- *   - compute the reference's pg_hash_any (xpu_common.cu:2920-3041).
+ *   - compute the reference's pg_hash_any (xpu_common.cu:2920-3041),
+ *   - deform heap tuples with the reference's ExecLoadVarsOuterHeap.
  */
 #include "xpu_common.h"
 #include <stdlib.h>
 #include <string>
+#include <vector>
 
 static std::string g_json;
 
@@ -209,4 +211,113 @@ extern "C" uint32_t
 ref_pg_hash_any(const void *ptr, int sz)
 {
     return pg_hash_any(ptr, sz);
+}
+
+/*
+ * The reference's own heap-tuple deform: ExecLoadVarsOuterHeap
+ * (xpu_common.cu:1587-1620 -> kern_extract_heap_tuple :194-257 ->
+ * __kern_extract_common_tuple :60-130) on the tuple at htup of a kds whose
+ * colmeta describes the columns.  Loads nload attributes resnos[i] of type
+ * codes types[i] (bool, int1/2/4/8, float4/8, text); for each writes
+ * nulls[i], and vals[i] (the value's bits, sign-extended integers) or, for
+ * text, offs[i]/lens[i] (offset of the data from htup and its length; -1
+ * for a compressed or external datum).  Returns 0, or -1 if the reference
+ * raised an error.
+ */
+extern "C" int
+ref_heap_deform(const void *kds, const void *htup, int nload, const int16_t *resnos,
+                const int32_t *types, int64_t *vals, int64_t *offs, int32_t *lens,
+                uint8_t *nulls)
+{
+    if (nload <= 0 || nload > 64)
+        return -1;
+    std::vector<kern_varslot_desc> desc(nload);
+    std::vector<xpu_datum_t *> slots(nload);
+    std::vector<std::vector<char>> store(nload);
+    for (int i = 0; i < nload; i++)
+    {
+        const xpu_datum_operators *ops;
+        switch (types[i])
+        {
+            case TypeOpCode__bool: ops = &xpu_bool_ops; break;
+            case TypeOpCode__int1: ops = &xpu_int1_ops; break;
+            case TypeOpCode__int2: ops = &xpu_int2_ops; break;
+            case TypeOpCode__int4: ops = &xpu_int4_ops; break;
+            case TypeOpCode__int8: ops = &xpu_int8_ops; break;
+            case TypeOpCode__float4: ops = &xpu_float4_ops; break;
+            case TypeOpCode__float8: ops = &xpu_float8_ops; break;
+            case TypeOpCode__text: ops = &xpu_text_ops; break;
+            default: return -1;
+        }
+        memset(&desc[i], 0, sizeof(kern_varslot_desc));
+        desc[i].vs_type_code = ops->xpu_type_code;
+        desc[i].vs_typbyval = ops->xpu_type_byval;
+        desc[i].vs_typalign = ops->xpu_type_align;
+        desc[i].vs_typlen = ops->xpu_type_length;
+        desc[i].vs_offset = -1;
+        desc[i].vs_ops = ops;
+        store[i].assign(ops->xpu_type_sizeof + 16, 0);
+        slots[i] = (xpu_datum_t *)store[i].data();
+        slots[i]->expr_ops = NULL;
+    }
+    std::vector<char> kbuf(offsetof(kern_context, vlbuf) + 1024, 0);
+    kern_context *kcxt = (kern_context *)kbuf.data();
+    kcxt->kvars_slot = slots.data();
+    kcxt->kvars_desc = desc.data();
+    kcxt->kvars_nslots = nload;
+    kcxt->kvars_nrooms = nload;
+    kcxt->vlpos = kcxt->vlbuf;
+    kcxt->vlend = kcxt->vlbuf + 1024;
+
+    size_t klen = offsetof(kern_expression, u.load.desc) + sizeof(kern_varload_desc) * nload;
+    std::vector<char> ebuf(klen + 16, 0);
+    kern_expression *kexp = (kern_expression *)ebuf.data();
+    kexp->len = klen;
+    kexp->exptype = TypeOpCode__int4;
+    kexp->opcode = FuncOpCode__LoadVars;
+    kexp->nr_args = 0;
+    kexp->u.load.depth = 0;
+    kexp->u.load.nitems = nload;
+    for (int i = 0; i < nload; i++)
+    {
+        kexp->u.load.desc[i].vl_resno = resnos[i];
+        kexp->u.load.desc[i].vl_slot_id = i;
+    }
+    if (!ExecLoadVarsOuterHeap(kcxt, kexp, NULL, (const kern_data_store *)kds,
+                               (const HeapTupleHeaderData *)htup))
+        return -1;
+    for (int i = 0; i < nload; i++)
+    {
+        xpu_datum_t *d = slots[i];
+        nulls[i] = d->expr_ops == NULL;
+        vals[i] = 0;
+        offs[i] = 0;
+        lens[i] = 0;
+        if (nulls[i])
+            continue;
+        switch (types[i])
+        {
+            case TypeOpCode__bool: vals[i] = ((xpu_bool_t *)d)->value ? 1 : 0; break;
+            case TypeOpCode__int1: vals[i] = ((xpu_int1_t *)d)->value; break;
+            case TypeOpCode__int2: vals[i] = ((xpu_int2_t *)d)->value; break;
+            case TypeOpCode__int4: vals[i] = ((xpu_int4_t *)d)->value; break;
+            case TypeOpCode__int8: vals[i] = ((xpu_int8_t *)d)->value; break;
+            case TypeOpCode__float4:
+            {
+                uint32_t b;
+                memcpy(&b, &((xpu_float4_t *)d)->value, 4);
+                vals[i] = b;
+                break;
+            }
+            case TypeOpCode__float8: memcpy(&vals[i], &((xpu_float8_t *)d)->value, 8); break;
+            case TypeOpCode__text:
+            {
+                xpu_text_t *t = (xpu_text_t *)d;
+                offs[i] = (int64_t)((const char *)t->value - (const char *)htup);
+                lens[i] = t->length;
+                break;
+            }
+        }
+    }
+    return 0;
 }

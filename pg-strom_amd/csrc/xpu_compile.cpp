@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include "xpu_kernels.h"
 #include "xpu_opcodes_table.h"
 
 namespace {
@@ -1202,6 +1203,49 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     }
     P->nops = C.ops.size();
     memcpy(P->ops, C.ops.data(), sizeof(xop) * C.ops.size());
+
+    /* source columns the programs read (heap / row chunks are deformed into
+     * exactly these) */
+    for (const xop &o : C.ops)
+    {
+        uint8_t kind, width = 0;
+        if (o.op == XO_LDCOL)
+        {
+            kind = XH_FIXED;
+            width = (uint8_t)(o.imm & XLD_WIDTH_MASK);
+        }
+        else if (o.op == XO_LDCOL_BOOL)
+        {
+            kind = XH_BOOL;
+            width = 1;
+        }
+        else if (o.op == XO_LDCOL_VAR)
+            kind = XH_VAR;
+        else
+            continue;
+        if (o.aux >= out->src_cols.size())
+            out->src_cols.resize(o.aux + 1, xpu_src_col{0, 0});
+        xpu_src_col &sc = out->src_cols[o.aux];
+        if (sc.kind && (sc.kind != kind || sc.width != width))
+            FAIL("a source column is loaded with two types");
+        sc.kind = kind;
+        sc.width = width;
+    }
+    /* parallelCurrentXids, for the visibility of KDS_FORMAT_ROW sources */
+    if (s->session_xact_state)
+    {
+        const size_t xoff = s->session_xact_state;
+        const size_t xids_at = offsetof(SerializedTransactionState, parallelCurrentXids);
+        if (xoff + xids_at > length)
+            FAIL("session_xact_state out of the session blob");
+        const SerializedTransactionState *xs =
+            (const SerializedTransactionState *)((const char *)s + xoff);
+        int32_t n = xs->nParallelCurrentXids;
+        if (n < 0 || xoff + xids_at + 4ULL * (uint32_t)n > length)
+            FAIL("session_xact_state out of the session blob");
+        out->has_xact_state = true;
+        out->xids.assign(xs->parallelCurrentXids, xs->parallelCurrentXids + n);
+    }
     return 0;
 #undef FAIL
 }

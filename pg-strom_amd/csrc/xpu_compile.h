@@ -9,6 +9,14 @@
 #include "pgstrom_xpu.h"
 #include "xpu_program.h"
 
+/* a kds_src column the programs load: XH_FIXED (width bytes), XH_BOOL or
+ * XH_VAR (xpu_kernels.h); kind 0: not loaded */
+struct xpu_src_col
+{
+    uint8_t     kind;
+    uint8_t     width;
+};
+
 struct xpu_compiled
 {
     xplan               plan;
@@ -22,6 +30,9 @@ struct xpu_compiled
     uint32_t            lds_entry_bytes = 0;    /* bytes per LDS group entry */
     std::vector<int64_t> params;                /* XO_PARAM values (raw register form) */
     uint64_t            param_nulls = 0;        /* bit i: parameter i is NULL */
+    std::vector<xpu_src_col> src_cols;          /* by kds_src column index */
+    bool                has_xact_state = false;
+    std::vector<uint32_t> xids;                 /* parallelCurrentXids */
 };
 
 /* Validates and translates the session blob; returns 0 or an ERRCODE_*

@@ -413,19 +413,46 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     isnull = (idx >= cm->nullmap_length) ||
                              !((((const uint8_t *)kbase)[cm->nullmap_offset + idx] >> (row & 7)) & 1);
                 }
-                if (4ULL * ((uint64_t)row + 2) > cm->values_length)
-                    isnull = true;
                 int64_t ptr = 0, len = 0;
-                if (!isnull)
+                if (cm->attopts.tag == XARROW_TAG_HEAPREF)
                 {
-                    const uint32_t *off = (const uint32_t *)(kbase + cm->values_offset);
-                    uint32_t o0 = off[row], o1 = off[row + 1];
-                    if (o0 > o1 || o1 > cm->extra_length)
+                    /* column tiles of a heap / row chunk: (offset, length)
+                     * pairs into the source chunk */
+                    if (8ULL * ((uint64_t)row + 1) > cm->values_length)
                         isnull = true;
-                    else
+                    if (!isnull)
                     {
-                        ptr = (int64_t)(uintptr_t)(kbase + cm->extra_offset + o0);
-                        len = o1 - o0;
+                        uint2 pr = ((const uint2 *)(kbase + cm->values_offset))[row];
+                        if (pr.y == XHEAP_LEN_EXTERNAL)
+                        {
+                            if (!L.err)
+                                L.err = XMSG_TEXT_EXTERNAL + 1;
+                            isnull = true;
+                        }
+                        else if ((uint64_t)pr.x + pr.y > cm->extra_length)
+                            isnull = true;
+                        else
+                        {
+                            ptr = (int64_t)((uintptr_t)kbase + cm->extra_offset + pr.x);
+                            len = pr.y;
+                        }
+                    }
+                }
+                else
+                {
+                    if (4ULL * ((uint64_t)row + 2) > cm->values_length)
+                        isnull = true;
+                    if (!isnull)
+                    {
+                        const uint32_t *off = (const uint32_t *)(kbase + cm->values_offset);
+                        uint32_t o0 = off[row], o1 = off[row + 1];
+                        if (o0 > o1 || o1 > cm->extra_length)
+                            isnull = true;
+                        else
+                        {
+                            ptr = (int64_t)(uintptr_t)(kbase + cm->extra_offset + o0);
+                            len = o1 - o0;
+                        }
                     }
                 }
                 L.R[d] = ptr;

@@ -64,6 +64,68 @@ typedef struct
 #define XMSG_GROUP_RETRY        0xfff3
 #define XMSG_TILE_TOO_LARGE     0xfff4
 #define XMSG_JOIN_RECORDS       0xfff5
+#define XMSG_HEAP_ITEM          0xfff6
+#define XMSG_HEAP_TUPLE         0xfff7
+
+/* ------------------------------------------------------------------
+ * Heap-page (KDS_FORMAT_BLOCK) and row (KDS_FORMAT_ROW / HASH) sources are
+ * read into column tiles before the pipeline: a count pass over the pages
+ * (or 64-row units), a scan of the unit counts, and a deform pass writing
+ * the referenced attributes of every visible tuple, in page / row-index
+ * order, into a KDS_FORMAT_ARROW buffer whose layout the host sets up.
+ * Fixed-width attributes become Arrow values, booleans bit-packed values;
+ * text/bytea attributes become (offset, length) pairs of u32 that address
+ * the datum's bytes inside the source chunk (no bytes are copied): such a
+ * column carries attopts.tag XARROW_TAG_HEAPREF and its extra_offset is the
+ * source chunk's address minus the tile buffer's, modulo 2^64.  A length of
+ * XHEAP_LEN_EXTERNAL marks a compressed or external datum.
+ * ------------------------------------------------------------------ */
+#define XARROW_TAG_HEAPREF      0x4845      /* not an ArrowTypeTag */
+#define XHEAP_LEN_EXTERNAL      0xffffffffU
+#define XHEAP_BLOCK             64          /* one wave per page / unit */
+#define XHEAP_COUNT_BLOCK       256
+#define XHEAP_SCAN_BLOCK        1024
+
+enum { XH_SKIP = 0, XH_FIXED = 1, XH_BOOL = 2, XH_VAR = 3 };
+
+/* one source attribute up to the last referenced one */
+typedef struct
+{
+    int16_t     attlen;
+    int8_t      attalign;
+    uint8_t     kind;               /* XH_* */
+    uint32_t    width;              /* XH_FIXED: 1, 2, 4 or 8 */
+    uint64_t    values_off;         /* from the tile buffer */
+    uint64_t    nullmap_off;        /* validity bits (u32 words, zeroed) */
+} xheap_col;
+
+typedef struct
+{
+    uint32_t    total;              /* visible tuples of the chunk */
+    uint32_t    errcode;
+    uint32_t    errmsg;
+    uint32_t    __pad;
+} xheap_status;
+
+typedef struct
+{
+    const kern_data_store *src;     /* device address of the source chunk */
+    char       *dst;                /* tile buffer (KDS_FORMAT_ARROW) */
+    const xheap_col *cols;          /* [natts_walk] */
+    uint32_t   *units;              /* [nunits + 1]: counts, then exclusive prefix */
+    xheap_status *st;
+    const uint32_t *xids;           /* parallelCurrentXids of the session */
+    uint64_t    src_length;
+    uint64_t    rowindex_off;       /* ROW / HASH: row index from the source head */
+    uint32_t    format;             /* KDS_FORMAT_BLOCK, _ROW or _HASH */
+    uint32_t    nitems;             /* pages (BLOCK) or tuples */
+    uint32_t    nunits;             /* pages, or 64-tuple units */
+    uint32_t    block_offset;
+    uint32_t    src_ncols;
+    uint32_t    natts_walk;
+    uint32_t    nxids;
+    uint32_t    capacity;           /* rows the tile buffer holds (deform pass) */
+} xheap_params;
 
 /* one fragment of a batched launch */
 typedef struct
@@ -184,6 +246,9 @@ int xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *wo
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
+/* heap / row sources: count pass, unit scan (total into st->total), deform pass */
+int xpu_launch_heap_count(const xheap_params *hp, void *stream);
+int xpu_launch_heap_deform(const xheap_params *hp, void *stream);
 int xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream);
 int xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head,
                             uint32_t head_len, uint64_t length, const xpu_task_ctl *ctl,

@@ -116,6 +116,16 @@ struct Scratch
     size_t      rec_cnt_cap = 0;
     uint64_t   *d_block_status = nullptr;   /* scan + projection: block totals */
     size_t      block_status_cap = 0;
+    /* heap-page / row chunks: their column tiles, the passes' status, column
+     * table and unit counts, and host copies of what is uploaded */
+    char       *d_tiles = nullptr;
+    size_t      tiles_cap = 0;
+    char       *d_hmeta = nullptr;
+    size_t      hmeta_cap = 0;
+    xheap_status *h_hstat = nullptr;    /* pinned */
+    hipEvent_t  evh0 = nullptr, evh1 = nullptr;
+    std::vector<char> h_tiles_head;
+    std::vector<xheap_col> h_hcols;
 };
 
 bool
@@ -127,7 +137,11 @@ scratch_create(Scratch *s, kern_errorbuf *eb)
            hip_ok(hipEventCreate(&s->evm), eb, "hipEventCreate") &&
            hip_ok(hipMalloc((void **)&s->d_ctl, sizeof(xpu_task_ctl)), eb, "hipMalloc") &&
            hip_ok(hipHostMalloc((void **)&s->h_ctl, sizeof(xpu_task_ctl), hipHostMallocDefault),
-                  eb, "hipHostMalloc");
+                  eb, "hipHostMalloc") &&
+           hip_ok(hipHostMalloc((void **)&s->h_hstat, sizeof(xheap_status), hipHostMallocDefault),
+                  eb, "hipHostMalloc") &&
+           hip_ok(hipEventCreate(&s->evh0), eb, "hipEventCreate") &&
+           hip_ok(hipEventCreate(&s->evh1), eb, "hipEventCreate");
 }
 
 void
@@ -146,6 +160,11 @@ scratch_destroy(Scratch *s)
         if (s->d_rec[i]) (void)hipFree(s->d_rec[i]);
     if (s->d_rec_cnt) (void)hipFree(s->d_rec_cnt);
     if (s->d_block_status) (void)hipFree(s->d_block_status);
+    if (s->d_tiles) (void)hipFree(s->d_tiles);
+    if (s->d_hmeta) (void)hipFree(s->d_hmeta);
+    if (s->h_hstat) (void)hipHostFree(s->h_hstat);
+    if (s->evh0) (void)hipEventDestroy(s->evh0);
+    if (s->evh1) (void)hipEventDestroy(s->evh1);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->evm) (void)hipEventDestroy(s->evm);
@@ -187,6 +206,7 @@ struct xpuhip_session
     uint32_t        bloom_mask[XP_MAX_DEPTH + 1] = {};  /* words - 1 (0: no filter) */
     uint32_t        maxdup[XP_MAX_DEPTH + 1] = {};      /* most items sharing a join hash */
     int64_t        *d_params = nullptr;     /* XO_PARAM values */
+    uint32_t       *d_xids = nullptr;       /* parallelCurrentXids (row-chunk visibility) */
     bool            inner_ready = false;
     xpuhip_tuning   tuning;
     std::mutex      lock;
@@ -220,6 +240,7 @@ scratch_trim(Scratch *x)
     trim((char **)&x->d_dump, &x->dump_cap);
     trim(&x->d_src, &x->src_cap);
     trim(&x->d_dst, &x->dst_cap);
+    trim(&x->d_tiles, &x->tiles_cap);
     for (int i = 0; i < 2; i++)
         trim((char **)&x->d_rec[i], &x->rec_cap[i]);
 }
@@ -450,6 +471,8 @@ message_of(const xpuhip_session *s, uint32_t msgid)
         case XMSG_GROUP_RETRY: return "group-by slot lock retry limit exceeded";
         case XMSG_TILE_TOO_LARGE: return "one tile of rows does not fit into kds_dst";
         case XMSG_JOIN_RECORDS: return "join records exceed their buffer";
+        case XMSG_HEAP_ITEM: return "heap page line pointer or row-index entry out of its chunk";
+        case XMSG_HEAP_TUPLE: return "heap tuple attribute beyond its item";
     }
     if (msgid < s->comp.messages.size())
         return s->comp.messages[msgid].c_str();
@@ -581,6 +604,14 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
             xpuhip_session_close(s);
             return ERRCODE_DEVICE_ERROR;
         }
+    }
+    if (!s->comp.xids.empty() &&
+        (!hip_ok(hipMalloc((void **)&s->d_xids, 4 * s->comp.xids.size()), eb, "hipMalloc") ||
+         !hip_ok(hipMemcpy(s->d_xids, s->comp.xids.data(), 4 * s->comp.xids.size(),
+                           hipMemcpyHostToDevice), eb, "hipMemcpy")))
+    {
+        xpuhip_session_close(s);
+        return ERRCODE_DEVICE_ERROR;
     }
     if (!s->comp.kds_dst_head.empty())
     {
@@ -1174,6 +1205,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_stats.global_merges = c->global_merges;
     t_last_stats.spill_rows = c->spill_rows;
     t_last_stats.jit = jf ? 1 : 0;
+    t_last_stats.heap_ms = 0.0;
+    t_last_stats.heap_rows = 0;
     if (res)
     {
         res->nitems_raw = (uint32_t)nrows;
@@ -1205,6 +1238,282 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     if (resume_pos)
         *resume_pos = 0;
     return 0;
+}
+
+/* ------------------------------------------------------------------
+ * heap-page (KDS_FORMAT_BLOCK) and row (KDS_FORMAT_ROW / HASH) chunks
+ *
+ * The chunk at d_src (host copy of its header: head) is read into column
+ * tiles in x->d_tiles (xpu_heap.hip): a count pass and a scan of the per-unit
+ * counts, a read-back of the total (it sizes the tiles and the pipeline's
+ * launch), then the deform pass.  The tiles form a KDS_FORMAT_ARROW chunk of
+ * *rows rows holding the columns the session's programs load.
+ * ------------------------------------------------------------------ */
+static uint64_t
+align64(uint64_t v)
+{
+    return (v + 63) & ~63ULL;
+}
+
+static int
+heap_tiles(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, const kern_data_store *head,
+           uint32_t *rows, double *ms, kern_errorbuf *eb)
+{
+    const xpu_compiled *C = &s->comp;
+    const char fmt = head->format;
+    char msg[160];
+    *rows = 0;
+    *ms = 0.0;
+    if (fmt != KDS_FORMAT_BLOCK && fmt != KDS_FORMAT_ROW && fmt != KDS_FORMAT_HASH)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "kds_src is not a KDS_FORMAT_BLOCK, ROW or HASH chunk");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    const uint64_t hl = KDS_HEAD_LENGTH(head);
+    if (head->nr_colmeta < head->ncols || head->length < hl || head->length >= (1ULL << 32))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "heap / row chunk header is malformed or 4GB or larger");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    const uint32_t natts = (uint32_t)C->src_cols.size();
+    if (natts > head->ncols)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "kds_src has fewer columns than the plan loads");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    for (uint32_t j = 0; j < natts; j++)
+    {
+        const kern_colmeta *cm = &head->colmeta[j];
+        const xpu_src_col &sc = C->src_cols[j];
+        int a = cm->attalign;
+        bool ok = (cm->attlen > 0 || cm->attlen == -1) && (a == 1 || a == 2 || a == 4 || a == 8);
+        if (sc.kind == XH_FIXED)
+            ok = ok && cm->attlen == (int)sc.width;
+        else if (sc.kind == XH_BOOL)
+            ok = ok && cm->attlen == 1;
+        else if (sc.kind == XH_VAR)
+            ok = ok && cm->attlen == -1;
+        if (!ok)
+        {
+            snprintf(msg, sizeof(msg), "kds_src column %u (attlen %d, attalign %d) cannot be read "
+                     "as the plan loads it", j + 1, (int)cm->attlen, a);
+            XERR(eb, ERRCODE_DEVICE_ERROR, msg);
+            return ERRCODE_DEVICE_ERROR;
+        }
+    }
+    xheap_params hp;
+    memset(&hp, 0, sizeof(hp));
+    hp.src = d_src;
+    hp.src_length = head->length;
+    hp.format = (uint32_t)(uint8_t)fmt;
+    hp.nitems = head->nitems;
+    hp.src_ncols = head->ncols;
+    hp.natts_walk = natts;
+    hp.xids = s->d_xids;
+    hp.nxids = (uint32_t)C->xids.size();
+    if (fmt == KDS_FORMAT_BLOCK)
+    {
+        if (head->block_offset % 8 != 0 || head->block_offset < hl + 4ULL * head->nitems ||
+            head->block_offset + (uint64_t)XPU_BLCKSZ * head->nitems > head->length)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "KDS_FORMAT_BLOCK pages out of the chunk");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        hp.block_offset = head->block_offset;
+        hp.nunits = head->nitems;
+    }
+    else
+    {
+        hp.rowindex_off = hl + 8ULL * head->hash_nslots;
+        if (hp.rowindex_off + 8ULL * head->nitems > head->length)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "row index out of the chunk");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        hp.nunits = (head->nitems + 63) / 64;
+    }
+    /* status, column table, unit counts */
+    const size_t cols_off = 16;
+    const size_t units_off = (cols_off + sizeof(xheap_col) * natts + 15) & ~(size_t)15;
+    if (!grow(&x->d_hmeta, &x->hmeta_cap, units_off + 4ULL * (hp.nunits + 1), eb))
+        return ERRCODE_DEVICE_ERROR;
+    hp.st = (xheap_status *)x->d_hmeta;
+    hp.cols = (const xheap_col *)(x->d_hmeta + cols_off);
+    hp.units = (uint32_t *)(x->d_hmeta + units_off);
+
+    (void)hipEventRecord(x->evh0, x->stream);
+    xpu_launch_fill(hp.st, 0, sizeof(xheap_status) / 8, x->stream);
+    int rc = xpu_launch_heap_count(&hp, x->stream);
+    (void)hipEventRecord(x->evh1, x->stream);
+    if (rc != 0)
+    {
+        hip_ok((hipError_t)rc, eb, "heap count launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    (void)hipMemcpyAsync(x->h_hstat, hp.st, sizeof(xheap_status), hipMemcpyDeviceToHost, x->stream);
+    if (!hip_ok(hipStreamSynchronize(x->stream), eb, "heap count pass"))
+        return ERRCODE_DEVICE_ERROR;
+    float ms0 = 0.0f;
+    (void)hipEventElapsedTime(&ms0, x->evh0, x->evh1);
+    *ms = ms0;
+    if (x->h_hstat->errcode)
+    {
+        XERR(eb, x->h_hstat->errcode, message_of(s, x->h_hstat->errmsg));
+        return (int)x->h_hstat->errcode;
+    }
+    const uint32_t total = x->h_hstat->total;
+
+    /* tile layout: header, validity bitmaps and boolean values (zeroed in
+     * one fill), then values and text pairs */
+    std::vector<char> &H = x->h_tiles_head;
+    H.assign((const char *)head, (const char *)head + hl);
+    kern_data_store *th = (kern_data_store *)H.data();
+    th->format = KDS_FORMAT_ARROW;
+    th->nitems = total;
+    th->usage = 0;
+    th->hash_nslots = 0;
+    th->block_offset = 0;
+    th->block_nloaded = 0;
+    for (uint32_t j = 0; j < th->ncols; j++)
+    {
+        kern_colmeta *cm = &th->colmeta[j];
+        cm->kds_format = KDS_FORMAT_ARROW;
+        cm->nullmap_offset = cm->nullmap_length = 0;
+        cm->values_offset = cm->values_length = 0;
+        cm->extra_offset = cm->extra_length = 0;
+    }
+    const uint64_t bm_bytes = 4ULL * ((total + 31) / 32 > 0 ? (total + 31) / 32 : 1);
+    uint64_t pos = align64(hl);
+    const uint64_t bm_begin = pos;
+    for (uint32_t j = 0; j < natts; j++)
+    {
+        const xpu_src_col &sc = C->src_cols[j];
+        if (!sc.kind)
+            continue;
+        kern_colmeta *cm = &th->colmeta[j];
+        cm->nullmap_offset = pos;
+        cm->nullmap_length = bm_bytes;
+        pos += align64(bm_bytes);
+        if (sc.kind == XH_BOOL)
+        {
+            cm->values_offset = pos;
+            cm->values_length = bm_bytes;
+            cm->attopts.tag = ArrowType__Bool;
+            pos += align64(bm_bytes);
+        }
+    }
+    const uint64_t bm_end = pos;
+    for (uint32_t j = 0; j < natts; j++)
+    {
+        const xpu_src_col &sc = C->src_cols[j];
+        kern_colmeta *cm = &th->colmeta[j];
+        if (sc.kind == XH_FIXED)
+        {
+            cm->values_offset = pos;
+            cm->values_length = (uint64_t)total * sc.width;
+            cm->attopts.unitsz = sc.width;
+            pos += align64(cm->values_length);
+        }
+        else if (sc.kind == XH_VAR)
+        {
+            cm->values_offset = pos;
+            cm->values_length = 8ULL * total;
+            cm->attopts.tag = XARROW_TAG_HEAPREF;
+            cm->attopts.unitsz = 8;
+            cm->extra_length = head->length;
+            pos += align64(cm->values_length);
+        }
+    }
+    th->length = pos;
+    if (!grow(&x->d_tiles, &x->tiles_cap, pos, eb))
+        return ERRCODE_DEVICE_ERROR;
+    const uint64_t src_rel = (uint64_t)((uintptr_t)d_src - (uintptr_t)x->d_tiles);
+    x->h_hcols.assign(natts, xheap_col{});
+    for (uint32_t j = 0; j < natts; j++)
+    {
+        const kern_colmeta *sm = &head->colmeta[j];
+        kern_colmeta *cm = &th->colmeta[j];
+        xheap_col &c = x->h_hcols[j];
+        c.attlen = sm->attlen;
+        c.attalign = sm->attalign;
+        c.kind = C->src_cols[j].kind;
+        c.width = C->src_cols[j].width;
+        c.values_off = cm->values_offset;
+        c.nullmap_off = cm->nullmap_offset;
+        if (c.kind == XH_VAR)
+            cm->extra_offset = src_rel;
+    }
+    hp.dst = x->d_tiles;
+    hp.capacity = total;
+    (void)hipEventRecord(x->evh0, x->stream);
+    if (!hip_ok(hipMemcpyAsync(x->d_tiles, H.data(), H.size(), hipMemcpyHostToDevice, x->stream),
+                eb, "hipMemcpyAsync(tiles)") ||
+        (natts > 0 &&
+         !hip_ok(hipMemcpyAsync((void *)hp.cols, x->h_hcols.data(), sizeof(xheap_col) * natts,
+                                hipMemcpyHostToDevice, x->stream), eb, "hipMemcpyAsync(tiles)")))
+        return ERRCODE_DEVICE_ERROR;
+    xpu_launch_fill(x->d_tiles + bm_begin, 0, (bm_end - bm_begin) / 8, x->stream);
+    rc = xpu_launch_heap_deform(&hp, x->stream);
+    (void)hipEventRecord(x->evh1, x->stream);
+    if (rc != 0)
+    {
+        hip_ok((hipError_t)rc, eb, "heap deform launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    *rows = total;
+    return 0;
+}
+
+/* a heap-page / row chunk at d_src: tiles, then the pipeline over them */
+static int
+task_exec_tuples(xpuhip_session *s, Scratch *x, const kern_data_store *d_src,
+                 const kern_data_store *head, kern_data_store *d_dst, size_t dst_length,
+                 uint64_t *resume_pos, kern_exec_results *res, kern_errorbuf *eb)
+{
+    uint32_t rows = 0;
+    double ms0 = 0.0;
+    int rc = heap_tiles(s, x, d_src, head, &rows, &ms0, eb);
+    if (rc != 0)
+        return rc;
+    rc = task_exec_common(s, x, (const kern_data_store *)x->d_tiles, rows, d_dst, dst_length,
+                          resume_pos, res, eb);
+    /* errors of the deform pass come first: the tiles are not valid then */
+    (void)hipMemcpyAsync(x->h_hstat, x->d_hmeta, sizeof(xheap_status), hipMemcpyDeviceToHost,
+                         x->stream);
+    if (!hip_ok(hipStreamSynchronize(x->stream), eb, "heap deform pass"))
+        return ERRCODE_DEVICE_ERROR;
+    if (x->h_hstat->errcode)
+    {
+        XERR(eb, x->h_hstat->errcode, message_of(s, x->h_hstat->errmsg));
+        return (int)x->h_hstat->errcode;
+    }
+    float ms1 = 0.0f;
+    (void)hipEventElapsedTime(&ms1, x->evh0, x->evh1);
+    t_last_stats.heap_ms = ms0 + ms1;
+    t_last_stats.heap_rows = rows;
+    return rc;
+}
+
+extern "C" int
+xpuhip_task_exec_device_tuples(xpuhip_session *s, const kern_data_store *d_kds_src,
+                               const kern_data_store *kds_head, kern_data_store *d_kds_dst,
+                               size_t dst_length, uint64_t *resume_pos, kern_exec_results *res,
+                               kern_errorbuf *eb)
+{
+    int rc;
+    if (!s || !d_kds_src || !kds_head)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no session or chunk");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if ((rc = ensure_device(eb)) != 0)
+        return rc;
+    Scratch *x = scratch_get(s, eb);
+    if (!x)
+        return ERRCODE_DEVICE_ERROR;
+    rc = task_exec_tuples(s, x, d_kds_src, kds_head, d_kds_dst, dst_length, resume_pos, res, eb);
+    scratch_put(s, x);
+    return rc;
 }
 
 extern "C" int
@@ -1239,9 +1548,16 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
         XERR(eb, ERRCODE_DEVICE_ERROR, "invalid kds_src");
         return ERRCODE_DEVICE_ERROR;
     }
-    if (kds_src->format != KDS_FORMAT_ARROW)
+    const bool tuples = kds_src->format == KDS_FORMAT_BLOCK || kds_src->format == KDS_FORMAT_ROW ||
+                        kds_src->format == KDS_FORMAT_HASH;
+    if (kds_src->format != KDS_FORMAT_ARROW && !tuples)
     {
-        XERR(eb, ERRCODE_DEVICE_ERROR, "this build reads KDS_FORMAT_ARROW chunks only");
+        XERR(eb, ERRCODE_DEVICE_ERROR, "kds_src must be KDS_FORMAT_ARROW, BLOCK, ROW or HASH");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (tuples && (src_length < KDS_HEAD_LENGTH(kds_src) || kds_src->nr_colmeta < kds_src->ncols))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "invalid kds_src");
         return ERRCODE_DEVICE_ERROR;
     }
     if ((rc = ensure_device(eb)) != 0)
@@ -1262,9 +1578,14 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
         scratch_put(s, x);
         return ERRCODE_DEVICE_ERROR;
     }
-    rc = task_exec_common(s, x, (const kern_data_store *)x->d_src, kds_src->nitems,
-                          preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
-                          resume_pos, res, eb);
+    if (tuples)
+        rc = task_exec_tuples(s, x, (const kern_data_store *)x->d_src, kds_src,
+                              preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
+                              resume_pos, res, eb);
+    else
+        rc = task_exec_common(s, x, (const kern_data_store *)x->d_src, kds_src->nitems,
+                              preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
+                              resume_pos, res, eb);
     if (!preagg && (rc == 0 || rc == ERRCODE_SUSPEND_NO_SPACE))
     {
         /* header + row index, then the tuples packed at the tail */
@@ -1349,6 +1670,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_kmrels) (void)hipFree(s->d_kmrels);
     if (s->d_bloom) (void)hipFree(s->d_bloom);
     if (s->d_params) (void)hipFree(s->d_params);
+    if (s->d_xids) (void)hipFree(s->d_xids);
     delete s;
 }
 
@@ -1780,6 +2102,8 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
     t_last_stats.lds_bytes = lds;
     t_last_stats.rows = total;
     t_last_stats.jit = jf ? 1 : 0;
+    t_last_stats.heap_ms = 0.0;
+    t_last_stats.heap_rows = 0;
 
     int first_err = 0;
     for (uint32_t f = 0; f < ntasks; f++)

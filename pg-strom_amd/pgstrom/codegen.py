@@ -338,7 +338,10 @@ class Plan:
             slots.append(slot)
         return slots
 
-    def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000):
+    def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None):
+        """The session blob.  xids: parallelCurrentXids of the transaction
+        (SerializedTransactionState at session_xact_state, xpu_common.h:3156-3165),
+        which decide the visibility of KDS_FORMAT_ROW sources."""
         nrels = len(self.inner_coldefs)
         codes = {}
         # scan quals
@@ -488,6 +491,10 @@ class Plan:
             sess.groupby_kds_final = _append_aligned(buf, bytes(head))
             sess.groupby_ngroups_estimation = float(final_nrows)
             flags |= abi.DEVTASK__PINNED_HASH_RESULTS
+        if xids is not None:
+            xs = struct.pack("<i?3xQQIi", 1, False, 0, 0, 0, len(xids)) + \
+                struct.pack("<%dI" % len(xids), *xids)
+            sess.session_xact_state = _append_aligned(buf, xs)
         sess.query_plan_id = self.query_plan_id
         sess.xpu_task_flags = flags
         sess.kcxt_kvecs_ndims = nrels + 2

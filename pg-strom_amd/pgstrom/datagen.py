@@ -78,3 +78,43 @@ def star_tables(n_fact, dim_sizes=(2_000_000, 200_000, 2_556), seed_workload=2):
         dims.append([Column("pk", "int4", pk), Column("category", "int2", cat),
                      Column("txt", "text", txt)])
     return fact, dims
+
+
+def heap_table(n, seed_workload=5, null_frac=0.1, fragment=0):
+    """Mixed-width columns whose heap layout exercises every alignment rule:
+    e int2, a int4, f bool, b float8, g text (0-300 bytes: short and 4-byte
+    varlena headers), c int8, h float4, d text 8-24 chars; null_frac NULLs
+    per column."""
+    r = rng(seed_workload, fragment)
+    nulls = [r.random(n) < null_frac for _ in range(8)]
+    e = r.integers(-2**15, 2**15, size=n, dtype=np.int16)
+    a = r.integers(0, 1_000_000, size=n, dtype=np.int32)
+    f = r.random(n) < 0.5
+    b = r.standard_normal(n)
+    glen = np.where(r.random(n) < 0.3, r.integers(127, 301, size=n), r.integers(0, 40, size=n))
+    glen[nulls[4]] = 0
+    goffs = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(glen, out=goffs[1:])
+    g = ArrowVarlena(goffs, ALNUM[r.integers(0, len(ALNUM), size=int(goffs[-1]))].tobytes())
+    c = r.integers(-10**12, 10**12, size=n, dtype=np.int64, endpoint=True)
+    h = r.standard_normal(n).astype(np.float32)
+    d = random_text(r, n, nulls=nulls[7])
+    return [Column("e", "int2", e, nulls[0]), Column("a", "int4", a, nulls[1]),
+            Column("f", "bool", f, nulls[2]), Column("b", "float8", b, nulls[3]),
+            Column("g", "text", g, nulls[4]), Column("c", "int8", c, nulls[5]),
+            Column("h", "float4", h, nulls[6]), Column("d", "text", d, nulls[7])]
+
+
+def take_rows(cols, keep):
+    """The columns restricted to the rows where keep is True."""
+    keep = np.asarray(keep, dtype=bool)
+    idx = np.nonzero(keep)[0]
+    out = []
+    for c in cols:
+        nulls = None if c.nulls is None else c.nulls[keep]
+        if c.typname in ("text", "bytea"):
+            vals = [bytes(c.values[i]) for i in idx]
+        else:
+            vals = c.values[keep]
+        out.append(Column(c.name, c.typname, vals, nulls))
+    return out

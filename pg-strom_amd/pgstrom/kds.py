@@ -444,3 +444,236 @@ def window_headers(base, windows, base_offset, stride):
                 cm.nullmap_offset = shift + cm.nullmap_offset + r0 // 8
                 cm.nullmap_length = (n + 7) // 8
     return out
+
+
+# ----------------------------------------------------------------------
+# heap tuples: 8 KB pages (KDS_FORMAT_BLOCK) and row chunks (KDS_FORMAT_ROW)
+# ----------------------------------------------------------------------
+BLCKSZ = 8192
+PAGE_HEADER_SIZE = 24
+HTUP_HEADER_SIZE = 23
+LP_UNUSED, LP_NORMAL, LP_REDIRECT, LP_DEAD = 0, 1, 2, 3
+PD_ALL_VISIBLE = 0x0004
+XACT_ATTRS_SIZE = 16        # sizeof(kern_tupitem_xact_attrs)
+FROZEN_XID = 2
+
+
+class ExternalDatum:
+    """A TOAST pointer in a text/bytea column: 1-byte header 0x01, the tag,
+    then the pointer body (16 bytes for VARTAG_ONDISK)."""
+
+    def __init__(self, tag=18):
+        self.tag = tag
+
+    def encode(self):
+        body = {1: 8, 18: 16}.get(self.tag, 0)
+        return bytes([0x01, self.tag]) + bytes(range(1, body + 1))
+
+
+def _varlena(v):
+    """PostgreSQL's on-page varlena form: a 1-byte header for data of up to
+    126 bytes (VARATT_CAN_MAKE_SHORT), else a 4-byte header.  Returns
+    (bytes, needs_alignment)."""
+    if isinstance(v, ExternalDatum):
+        return v.encode(), False
+    v = bytes(v)
+    if len(v) + 1 <= 0x7f:
+        return bytes([((len(v) + 1) << 1) | 0x01]) + v, False
+    return struct.pack("<I", (len(v) + 4) << 2) + v, True
+
+
+def heap_payload(coldefs, values, start=0):
+    """(infomask, null bitmap, data bytes) of a tuple in heap_fill_tuple's
+    layout: NULLs take no space, fixed-width values are aligned to typalign,
+    short varlena values are not aligned, 4-byte-header ones are.  `start`
+    is the data's offset from a MAXALIGN'ed base (alignment is relative to
+    it); the data bytes returned start at `start`."""
+    natts = len(coldefs)
+    hasnull = any(v is None for v in values)
+    infomask = abi.HEAP_HASNULL if hasnull else 0
+    bits = bytearray(abi.bitmaplen(natts)) if hasnull else bytearray()
+    body = bytearray()
+    pos = start
+    for j, ((_, t), v) in enumerate(zip(coldefs, values)):
+        if v is None:
+            continue
+        if hasnull:
+            bits[j >> 3] |= 1 << (j & 7)
+        _, typlen, _, align, _ = PG_TYPES[t]
+        if typlen == -1:
+            d, aligned = _varlena(v)
+            infomask |= abi.HEAP_HASVARWIDTH
+            if aligned:
+                nxt = abi.typealign(align, pos)
+                body += b"\0" * (nxt - pos)
+                pos = nxt
+        else:
+            d = encode_datum(t, v)
+            nxt = abi.typealign(align, pos)
+            body += b"\0" * (nxt - pos)
+            pos = nxt
+        body += d
+        pos += len(d)
+    return infomask, bytes(bits), bytes(body)
+
+
+def form_heap_tuple(coldefs, values, xmin=FROZEN_XID, xmax=0, ctid=(0, 0)):
+    """HeapTupleHeaderData + null bitmap + data, as heap_form_tuple lays it
+    out (t_hoff = MAXALIGN(23 + bitmap))."""
+    natts = len(coldefs)
+    hasnull = any(v is None for v in values)
+    hoff = abi.maxalign(HTUP_HEADER_SIZE + (abi.bitmaplen(natts) if hasnull else 0))
+    infomask, bits, body = heap_payload(coldefs, values, 0)
+    blk, posid = ctid
+    head = struct.pack("<IIIHHHHHB", xmin, xmax, 0, (blk >> 16) & 0xffff, blk & 0xffff, posid,
+                       natts & abi.HEAP_NATTS_MASK, infomask | 0x0900, hoff)
+    return head + bits + b"\0" * (hoff - HTUP_HEADER_SIZE - len(bits)) + body
+
+
+def build_block_kds(columns, invisible=None, extra_items=False, max_per_page=None, seed=0,
+                    empty_pages=()):
+    """KDS_FORMAT_BLOCK chunk (relscan.c:554-742 layout): the rows of
+    `columns` as heap tuples packed into 8 KB pages (PageAddItem order:
+    line pointers up from the header, tuples MAXALIGN'ed down from the end),
+    the block numbers at the KDS body, the pages at block_offset.
+
+    invisible[i] marks rows the host's visibility check rejected: their line
+    pointer is set LP_UNUSED as relscan.c:480-495 does (the tuple bytes stay
+    on the page).  extra_items adds LP_DEAD items with storage and
+    LP_REDIRECT items between the rows; max_per_page caps the tuples per
+    page; empty_pages lists page numbers left without items.  The scan loads
+    the rows that are not invisible, in row order."""
+    n = len(columns[0]) if columns else 0
+    coldefs = [(c.name, c.typname) for c in columns]
+    rng = np.random.default_rng(seed)
+    invisible = np.zeros(n, dtype=bool) if invisible is None else np.asarray(invisible, dtype=bool)
+
+    def value(c, i):
+        if c.nulls is not None and c.nulls[i]:
+            return None
+        v = c.values[i]
+        return v.item() if hasattr(v, "item") else v
+
+    pages = []
+    page = None
+
+    def new_page():
+        p = {"buf": bytearray(BLCKSZ), "lps": [], "upper": BLCKSZ, "ntup": 0}
+        pages.append(p)
+        return p
+
+    def add_item(p, raw, flags):
+        alen = abi.maxalign(len(raw))
+        if PAGE_HEADER_SIZE + 4 * (len(p["lps"]) + 1) > p["upper"] - alen:
+            return False
+        p["upper"] -= alen
+        off = p["upper"]
+        p["buf"][off:off + len(raw)] = raw
+        p["lps"].append(off | (flags << 15) | (len(raw) << 17))
+        return True
+
+    page = new_page()
+    while len(pages) - 1 in empty_pages:
+        page = new_page()
+    for i in range(n):
+        tup = form_heap_tuple(coldefs, [value(c, i) for c in columns],
+                              ctid=(len(pages) - 1, len(page["lps"]) + 1))
+        if extra_items and rng.random() < 0.05:
+            # a dead item keeping its storage, a redirect to the previous item
+            dead = form_heap_tuple(coldefs, [value(c, i) for c in columns])
+            if add_item(page, dead, LP_DEAD) and \
+                    PAGE_HEADER_SIZE + 4 * (len(page["lps"]) + 1) <= page["upper"]:
+                page["lps"].append(max(1, len(page["lps"]) - 1) | (LP_REDIRECT << 15))
+        full = max_per_page is not None and page["ntup"] >= max_per_page
+        if full or not add_item(page, tup, LP_NORMAL):
+            page = new_page()
+            while len(pages) - 1 in empty_pages:
+                page = new_page()
+            if not add_item(page, tup, LP_NORMAL):
+                raise ValueError("a tuple does not fit into an empty page")
+        page["ntup"] += 1
+        if invisible[i]:
+            page["lps"][-1] = 0         # ItemIdSetUnused
+    npages = len(pages)
+    head = kds_header(abi.KDS_FORMAT_BLOCK, coldefs)
+    block_offset = abi.maxalign(len(head) + 4 * npages)
+    length = block_offset + BLCKSZ * npages
+    out = np.zeros(length, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(bytes(head), dtype=np.uint8)
+    hd = abi.kern_data_store_head.from_buffer(out)
+    hd.length = length
+    hd.nitems = npages
+    hd.block_offset = block_offset
+    hd.block_nloaded = npages
+    blknums = np.arange(npages, dtype=np.uint32) + 1000
+    out[len(head):len(head) + 4 * npages] = blknums.view(np.uint8)
+    for b, p in enumerate(pages):
+        buf = p["buf"]
+        nlp = len(p["lps"])
+        struct.pack_into("<IIHHHHHHI", buf, 0, 0, 0, 0, PD_ALL_VISIBLE,
+                         PAGE_HEADER_SIZE + 4 * nlp, p["upper"], BLCKSZ, BLCKSZ | 4, 0)
+        if nlp:
+            buf[PAGE_HEADER_SIZE:PAGE_HEADER_SIZE + 4 * nlp] = \
+                np.array(p["lps"], dtype=np.uint32).tobytes()
+        o = block_offset + BLCKSZ * b
+        out[o:o + BLCKSZ] = np.frombuffer(bytes(buf), dtype=np.uint8)
+    return out
+
+
+def form_row_tupitem(coldefs, values, rowid, xact=None):
+    """kern_tupitem of a row chunk.  With xact = (xmin, xmax) the item embeds
+    kern_tupitem_xact_attrs in front of its payload, laid out as the GPU
+    cache writes it (gpu_cache.c:868-960): header MAXALIGN(offsetof(t_bits)
+    + bitmap + 16), payload from there, INTALIGN'ed, then the row-id."""
+    natts = len(coldefs)
+    hasnull = any(v is None for v in values)
+    bl = abi.bitmaplen(natts) if hasnull else 0
+    head_sz = abi.maxalign(abi.T_BITS_OFFSET + bl + (XACT_ATTRS_SIZE if xact else 0))
+    infomask, bits, body = heap_payload(coldefs, values, 0)
+    t_len = abi.typealign(4, head_sz + len(body)) + abi.ROWID_SIZE
+    buf = bytearray(t_len)
+    struct.pack_into("<IIBBHHB", buf, 0, t_len, 0, 1 if xact else 0, 0,
+                     natts & abi.HEAP_NATTS_MASK, infomask, head_sz + abi.MINIMAL_TUPLE_OFFSET)
+    buf[abi.T_BITS_OFFSET:abi.T_BITS_OFFSET + len(bits)] = bits
+    if xact:
+        struct.pack_into("<IIHHHH", buf, head_sz - XACT_ATTRS_SIZE, xact[0], xact[1], 0, 0,
+                         rowid & 0xffff, 1)
+    buf[head_sz:head_sz + len(body)] = body
+    struct.pack_into("<I", buf, t_len - 4, rowid)
+    return bytes(buf)
+
+
+def build_row_kds(columns, xact=None, holes=()):
+    """KDS_FORMAT_ROW chunk of the rows of `columns` (row index of u64
+    offsets from the tail, items packed towards the tail).  xact: per-row
+    (xmin, xmax) to embed, or None; holes: row-index entries left 0 (no
+    tuple)."""
+    n = len(columns[0]) if columns else 0
+    coldefs = [(c.name, c.typname) for c in columns]
+
+    def value(c, i):
+        if c.nulls is not None and c.nulls[i]:
+            return None
+        v = c.values[i]
+        return v.item() if hasattr(v, "item") else v
+
+    items = [form_row_tupitem(coldefs, [value(c, i) for c in columns], i,
+                              None if xact is None else xact[i]) for i in range(n)]
+    head = kds_header(abi.KDS_FORMAT_ROW, coldefs)
+    usage = sum(abi.maxalign(len(t)) for t in items)
+    length = abi.maxalign(len(head) + 8 * n + usage)
+    out = np.zeros(length, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(bytes(head), dtype=np.uint8)
+    hd = abi.kern_data_store_head.from_buffer(out)
+    hd.length = length
+    hd.nitems = n
+    hd.usage = usage
+    rowidx = np.zeros(n, dtype=np.uint64)
+    curr = length
+    holes = set(holes)
+    for i, t in enumerate(items):
+        curr -= abi.maxalign(len(t))
+        out[curr:curr + len(t)] = np.frombuffer(t, dtype=np.uint8)
+        rowidx[i] = 0 if i in holes else length - curr
+    out[len(head):len(head) + 8 * n] = rowidx.view(np.uint8)
+    return out

@@ -80,3 +80,22 @@ def assert_groups_equal(got_kds, exp_kds, actions, rtol=1e-12):
             else:
                 assert g == e, (key, a, g, e)
     return got
+
+
+def project_all_query(cols, names=None, quals=None):
+    """SELECT <names, default all columns> FROM the chunk [WHERE quals(plan,
+    vars)]; returns the session blob."""
+    from pgstrom import codegen as cg
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    names = names or [c.name for c in cols]
+    types = {c.name: c.typname for c in cols}
+    vs = {n: plan.scan_var(n) for n in names}
+    if quals is not None:
+        plan.scan_quals = quals(plan, vs)
+    plan.set_projection([vs[n] for n in names], [(n, types[n]) for n in names])
+    return plan.build()
+
+
+def rows_of(dst):
+    """(values tuple) of every row of a kds_dst, in row-index order."""
+    return [r[0] for r in kds.parse_row_kds(dst)]
