@@ -110,6 +110,57 @@ def bench_scan(n, warmup, steps):
     return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo, nout=nout)
 
 
+def bench_heap(n, warmup, steps):
+    """C0's scan over heap pages: the 1M-row table as a KDS_FORMAT_BLOCK chunk
+    of 8 KB pages in device memory (xpuhip_task_exec_device_tuples)."""
+    from pgstrom import abi, datagen, kds, workloads as W, xpuhip
+    log("generating %d-row scan table on heap pages" % n)
+    cols = datagen.scan_table(n)
+    src = kds.build_block_kds(cols)
+    hd = abi.kern_data_store_head.from_buffer_copy(bytes(src[:256]))
+    npages = hd.nitems
+    blob = W.scan_query(cols)
+    d_src = xpuhip.DeviceBuffer(len(src), src)
+    head = np.array(src[:abi.kds_head_length(hd.nr_colmeta)])
+    dst_len = 64 << 20
+    d_dst = xpuhip.DeviceBuffer(dst_len)
+    s = xpuhip.Session(blob)
+    out = {}
+
+    def step():
+        rc, res, _ = s.exec_device_tuples(d_src, head, d_dst, dst_len)
+        assert rc == 0
+        out["nout"] = res.nitems_out
+
+    wall, kms, stats = time_steps(step, warmup, steps)
+    s.close()
+    usage = int(np.frombuffer(d_dst.to_host(256)[8:16].tobytes(), dtype=np.uint64)[0])
+    d_src.free()
+    d_dst.free()
+    heap_ms = float(np.mean([st["heap_ms"] for st in stats]))
+    nout = out["nout"]
+    page_bytes = npages * kds.BLCKSZ
+    # deform pass: every page read once; tiles of the four referenced columns
+    # written (a 4 B, b 8 B, c 8 B, d an 8 B (offset, length) pair, plus a
+    # validity bit each)
+    tile_bytes = n * (4 + 8 + 8 + 8) + 4 * (n // 8)
+    return dict(rows=n, pages=npages, wall_ms=wall * 1e3, kernel_ms=kms, heap_ms=heap_ms,
+                nout=nout, page_bytes=page_bytes, tile_bytes=tile_bytes,
+                algo_bytes=page_bytes + usage + 8 * nout)
+
+
+def heap_line(hp):
+    total_ms = hp["heap_ms"] + hp["kernel_ms"]
+    return dict(workload="GpuScan 1M rows on 8 KB heap pages (KDS_FORMAT_BLOCK, %d pages): "
+                         "WHERE a < 500000 AND b > 0.0 projecting (c, d)" % hp["pages"],
+                rows_per_s=hp["rows"] / (total_ms / 1e3), device_ms=total_ms,
+                heap_passes_ms=hp["heap_ms"], pipeline_ms=hp["kernel_ms"], wall_ms=hp["wall_ms"],
+                hbm_gbs=hp["algo_bytes"] / (total_ms / 1e3) / 1e9,
+                frac=hp["algo_bytes"] / (total_ms / 1e3) / HBM_PEAK,
+                heap_passes_gbs=(hp["page_bytes"] + hp["tile_bytes"]) / (hp["heap_ms"] / 1e3) / 1e9,
+                rows_out=hp["nout"])
+
+
 def bench_star(n, warmup, steps):
     from pgstrom import datagen, kds, workloads as W, xpuhip
     log("generating %d-row star-join fact table and dimensions" % n)
@@ -343,7 +394,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--only", default="all",
-                    choices=["all", "preagg", "batch", "served", "join", "scan"])
+                    choices=["all", "preagg", "batch", "served", "join", "scan", "heap"])
     ap.add_argument("--served-requests", type=int, default=300)
     ap.add_argument("--served-table-rows", type=int, default=50_000_000)
     ap.add_argument("--served-slots", type=int, default=8)
@@ -376,6 +427,9 @@ def main():
     if args.only == "scan":
         print(json.dumps(scan_line(bench_scan(1_000_000, args.warmup, args.steps))))
         return
+    if args.only == "heap":
+        print(json.dumps(heap_line(bench_heap(1_000_000, args.warmup, args.steps))))
+        return
     if args.only == "served":
         print(json.dumps(bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
                                       args.served_table_rows, args.served_slots,
@@ -389,6 +443,7 @@ def main():
     others = {}
     if args.only == "all":
         others["scan_1M"] = scan_line(bench_scan(1_000_000, 3, 20))
+        others["scan_1M_heap_pages"] = heap_line(bench_heap(1_000_000, 3, 20))
         others["batched_256"] = bench_batch(args.batch_frags, args.batch_base_rows, 1, 3)
         others["served"] = bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
                                         args.served_table_rows)

@@ -15,8 +15,10 @@
  *                visible rows), validated against the page and chunk bounds
  *   unit scan    exclusive prefix of the per-unit counts: every page owns a
  *                contiguous range of output rows, so rows keep page order
- *   deform pass  one 64-lane block per page: the 8 KB page is copied into
- *                LDS with coalesced 8-byte loads, the normal items are
+ *   deform pass  persistent waves, one page at a time each: the 8 KB page
+ *                is copied into the wave's LDS slot with coalesced 8-byte
+ *                loads (the next page's loads overlap this page's work),
+ *                the normal items are
  *                compacted into a list, and each lane deforms one tuple from
  *                LDS (attribute walk with the reference's alignment rules),
  *                writing fixed-width values, validity bits and (offset,
@@ -27,6 +29,9 @@
 #include <hip/hip_runtime.h>
 
 #include "xpu_kernels.h"
+
+/* walks of up to this many attributes keep the column table in registers */
+#define XHEAP_REG_COLS  8
 
 namespace {
 
@@ -184,10 +189,13 @@ put_bits(uint32_t *words, uint32_t row0, uint64_t mask)
  * aligned unless their first byte is non-zero (a short header), sizes by
  * VARSIZE_ANY; positions are aligned relative to the payload.  h_src is the
  * offset of h from the chunk head (text datums are addressed by it).
+ * NC > 0: the column table (at most NC entries) is in registers, cols[j]
+ * with the walk unrolled; NC == 0: cols is the table in memory.
  */
+template <int NC>
 __device__ __forceinline__ void
-deform_tuple(const xheap_params &p, const uint8_t *h, uint32_t t, uint32_t end, uint64_t h_src,
-             uint32_t row, uint32_t row0, bool active, bool *bad)
+deform_tuple(const xheap_params &p, const xheap_col *cols, const uint8_t *h, uint32_t t,
+             uint32_t end, uint64_t h_src, uint32_t row, uint32_t row0, bool active, bool *bad)
 {
     uint32_t natts = 0, hasnull = 0, p0 = 0;
     if (active)
@@ -209,9 +217,14 @@ deform_tuple(const xheap_params &p, const uint8_t *h, uint32_t t, uint32_t end, 
     }
     uint32_t pos = p0;
     char *dst = p.dst;
-    for (uint32_t j = 0; j < p.natts_walk; j++)
+    const uint32_t nwalk = NC > 0 ? (p.natts_walk < (uint32_t)NC ? p.natts_walk : (uint32_t)NC)
+                                  : p.natts_walk;
+#pragma unroll
+    for (uint32_t j = 0; j < (NC > 0 ? (uint32_t)NC : nwalk); j++)
     {
-        const xheap_col c = p.cols[j];
+        if (NC > 0 && j >= nwalk)
+            break;
+        const xheap_col c = cols[j];
         bool isnull = !active || j >= natts ||
                       (hasnull && !((h[t + XPU_HTUP_BITS + (j >> 3)] >> (j & 7)) & 1));
         uint64_t v = 0;
@@ -350,88 +363,154 @@ xpu_heap_count_kernel(xheap_params p)
         p.units[unit] = cnt;
 }
 
-/* exclusive prefix of units[0..n) in place; units[n] and st->total get the sum */
+/* exclusive prefix of units[0..n) in place; units[n] and st->total get the
+ * sum.  Rounds of XHEAP_SCAN_BLOCK * XHEAP_SCAN_SEG units: every thread
+ * loads its XHEAP_SCAN_SEG contiguous counts at once, the block scans the
+ * thread sums, and the round's total carries into the next round. */
+#define XHEAP_SCAN_SEG  16
 __global__ void __launch_bounds__(XHEAP_SCAN_BLOCK)
 xpu_heap_scan_kernel(uint32_t *units, uint32_t n, xheap_status *st)
 {
     __shared__ uint32_t wsum[XHEAP_SCAN_BLOCK / 64];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
-    const uint32_t seg = (n + XHEAP_SCAN_BLOCK - 1) / XHEAP_SCAN_BLOCK;
-    uint32_t b = tid * seg < n ? tid * seg : n;
-    uint32_t e = b + seg < n ? b + seg : n;
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; i++)
-        s += units[i];
-    uint32_t incl = s;
-    for (uint32_t off = 1; off < 64; off <<= 1)
+    uint32_t carry = 0;
+    for (uint32_t r0 = 0; r0 < n; r0 += XHEAP_SCAN_BLOCK * XHEAP_SCAN_SEG)
     {
-        uint32_t v = __shfl_up(incl, off, 64);
-        if (lane >= off)
-            incl += v;
+        const uint32_t b = r0 + tid * XHEAP_SCAN_SEG;
+        uint32_t c[XHEAP_SCAN_SEG];
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < XHEAP_SCAN_SEG; i++)
+        {
+            c[i] = b + i < n ? units[b + i] : 0;
+            s += c[i];
+        }
+        uint32_t incl = s;
+        for (uint32_t off = 1; off < 64; off <<= 1)
+        {
+            uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= off)
+                incl += v;
+        }
+        __syncthreads();
+        if (lane == 63)
+            wsum[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry, round = 0;
+        for (uint32_t w = 0; w < XHEAP_SCAN_BLOCK / 64; w++)
+        {
+            uint32_t ws = wsum[w];
+            before += w < wave ? ws : 0;
+            round += ws;
+        }
+        uint32_t run = before + incl - s;
+#pragma unroll
+        for (uint32_t i = 0; i < XHEAP_SCAN_SEG; i++)
+        {
+            if (b + i < n)
+                units[b + i] = run;
+            run += c[i];
+        }
+        carry += round;
     }
-    if (lane == 63)
-        wsum[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0;
-    for (uint32_t w = 0; w < wave; w++)
-        before += wsum[w];
-    uint32_t run = before + incl - s;
-    for (uint32_t i = b; i < e; i++)
+    if (tid == 0)
     {
-        uint32_t c = units[i];
-        units[i] = run;
-        run += c;
-    }
-    if (tid == XHEAP_SCAN_BLOCK - 1)
-    {
-        units[n] = run;
-        st->total = run;
+        units[n] = carry;
+        st->total = carry;
     }
 }
 
-/* one page per 64-lane block: stage the page in LDS, list its normal items,
- * deform them 64 at a time */
-__global__ void __launch_bounds__(XHEAP_BLOCK)
+/* LDS ordering among the lanes of one wave (its waves run independently) */
+__device__ __forceinline__ void
+wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* persistent waves over the pages: each wave stages its page in its own LDS
+ * slot, lists the normal items and deforms them 64 at a time, while the
+ * next page's loads are already in flight in registers */
+template <int NC>
+__global__ void __launch_bounds__(XHEAP_DEFORM_BLOCK)
 xpu_heap_block_deform_kernel(xheap_params p)
 {
-    __shared__ uint2 pg[XPU_BLCKSZ / 8];
-    __shared__ uint16_t items[XPU_BLCKSZ / 4];
-    const uint32_t page = blockIdx.x, lane = lane_id();
-    const uint64_t page_src = p.block_offset + (uint64_t)page * XPU_BLCKSZ;
-    const uint2 *gp = (const uint2 *)((const uint8_t *)p.src + page_src);
-#pragma unroll 4
-    for (uint32_t i = lane; i < XPU_BLCKSZ / 8; i += 64)
-        pg[i] = gp[i];
-    __syncthreads();
-    const uint8_t *pb = (const uint8_t *)pg;
-    bool bad = false;
-    uint32_t nlp = page_nlp((uint32_t)pb[XPU_PAGE_PD_LOWER] | ((uint32_t)pb[XPU_PAGE_PD_LOWER + 1] << 8),
-                            &bad);
-    const uint32_t *lps = (const uint32_t *)(pb + XPU_PAGE_HEADER_SIZE);
-    uint32_t n = 0;
-    for (uint32_t k0 = 0; k0 < nlp; k0 += 64)
+    xheap_col regc[NC > 0 ? NC : 1];
+    const xheap_col *cols = p.cols;
+    if (NC > 0)
     {
-        uint32_t k = k0 + lane;
-        bool ok = k < nlp && page_item_ok(lps[k], &bad);
-        uint64_t m = __ballot(ok);
-        if (ok)
-            items[n + (uint32_t)__popcll(m & ((1ULL << lane) - 1))] = (uint16_t)k;
-        n += (uint32_t)__popcll(m);
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+            if ((uint32_t)j < p.natts_walk)
+                regc[j] = p.cols[j];
+        cols = regc;
     }
-    __syncthreads();
-    const uint32_t base = p.units[page];
-    for (uint32_t b0 = 0; b0 < n; b0 += 64)
+    constexpr uint32_t NW = XHEAP_DEFORM_BLOCK / 64;
+    constexpr uint32_t NV = XPU_BLCKSZ / 8 / 64;
+    __shared__ uint2 pg_all[NW][XPU_BLCKSZ / 8];
+    __shared__ uint16_t items_all[NW][XPU_BLCKSZ / 4];
+    const uint32_t wave = threadIdx.x / 64, lane = lane_id();
+    uint2 *pg = pg_all[wave];
+    uint16_t *items = items_all[wave];
+    const uint8_t *pb = (const uint8_t *)pg;
+    const uint32_t *lps = (const uint32_t *)(pb + XPU_PAGE_HEADER_SIZE);
+    const uint32_t stride = gridDim.x * NW;
+    bool bad = false;
+    uint32_t page = blockIdx.x * NW + wave;
+    uint2 v[NV];
+    if (page < p.nunits)
     {
-        uint32_t idx = b0 + lane;
-        bool active = idx < n && base + idx < p.capacity;
-        uint32_t off = 0, len = 0;
-        if (active)
+        const uint2 *gp = (const uint2 *)((const uint8_t *)p.src + p.block_offset +
+                                          (uint64_t)page * XPU_BLCKSZ);
+#pragma unroll
+        for (uint32_t i = 0; i < NV; i++)
+            v[i] = gp[lane + 64 * i];
+    }
+    for (; page < p.nunits; page += stride)
+    {
+        const uint64_t page_src = p.block_offset + (uint64_t)page * XPU_BLCKSZ;
+        wave_sync();
+#pragma unroll
+        for (uint32_t i = 0; i < NV; i++)
+            pg[lane + 64 * i] = v[i];
+        if (page + stride < p.nunits)
         {
-            uint32_t lp = lps[items[idx]];
-            off = XPU_ITEMID_OFF(lp);
-            len = XPU_ITEMID_LEN(lp);
+            const uint2 *gp = (const uint2 *)((const uint8_t *)p.src + page_src +
+                                              (uint64_t)stride * XPU_BLCKSZ);
+#pragma unroll
+            for (uint32_t i = 0; i < NV; i++)
+                v[i] = gp[lane + 64 * i];
         }
-        deform_tuple(p, pb, off, off + len, page_src, base + idx, base + b0, active, &bad);
+        wave_sync();
+        uint32_t nlp = page_nlp((uint32_t)pb[XPU_PAGE_PD_LOWER] |
+                                ((uint32_t)pb[XPU_PAGE_PD_LOWER + 1] << 8), &bad);
+        uint32_t n = 0;
+        for (uint32_t k0 = 0; k0 < nlp; k0 += 64)
+        {
+            uint32_t k = k0 + lane;
+            bool ok = k < nlp && page_item_ok(lps[k], &bad);
+            uint64_t m = __ballot(ok);
+            if (ok)
+                items[n + (uint32_t)__popcll(m & ((1ULL << lane) - 1))] = (uint16_t)k;
+            n += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        const uint32_t base = p.units[page];
+        for (uint32_t b0 = 0; b0 < n; b0 += 64)
+        {
+            uint32_t idx = b0 + lane;
+            bool active = idx < n && base + idx < p.capacity;
+            uint32_t off = 0, len = 0;
+            if (active)
+            {
+                uint32_t lp = lps[items[idx]];
+                off = XPU_ITEMID_OFF(lp);
+                len = XPU_ITEMID_LEN(lp);
+            }
+            deform_tuple<NC>(p, cols, pb, off, off + len, page_src, base + idx, base + b0, active,
+                             &bad);
+        }
     }
     if (__ballot(bad) && lane == 0)
         heap_error(p.st, XMSG_HEAP_TUPLE);
@@ -439,9 +518,20 @@ xpu_heap_block_deform_kernel(xheap_params p)
 
 /* 64 row-index entries per 64-lane block: compact the visible rows onto the
  * first lanes, deform them from global memory */
+template <int NC>
 __global__ void __launch_bounds__(XHEAP_BLOCK)
 xpu_heap_row_deform_kernel(xheap_params p)
 {
+    xheap_col regc[NC > 0 ? NC : 1];
+    const xheap_col *cols = p.cols;
+    if (NC > 0)
+    {
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+            if ((uint32_t)j < p.natts_walk)
+                regc[j] = p.cols[j];
+        cols = regc;
+    }
     __shared__ uint32_t slot[64];
     const uint32_t unit = blockIdx.x, lane = lane_id();
     bool bad = false;
@@ -459,7 +549,7 @@ xpu_heap_row_deform_kernel(xheap_params p)
     if (active)
         active = row_item(p, slot[lane], &hoff, &hend, &bad);
     const uint8_t *h = (const uint8_t *)p.src + hoff;
-    deform_tuple(p, h, 0, hend, hoff, base + lane, base, active, &bad);
+    deform_tuple<NC>(p, cols, h, 0, hend, hoff, base + lane, base, active, &bad);
     if (__ballot(bad) && lane == 0)
         heap_error(p.st, XMSG_HEAP_TUPLE);
 }
@@ -479,15 +569,33 @@ xpu_launch_heap_count(const xheap_params *hp, void *stream)
 }
 
 extern "C" int
-xpu_launch_heap_deform(const xheap_params *hp, void *stream)
+xpu_launch_heap_deform(const xheap_params *hp, int ncu, void *stream)
 {
     if (hp->nunits == 0 || hp->capacity == 0)
         return 0;
     if (hp->format == KDS_FORMAT_BLOCK)
-        hipLaunchKernelGGL(xpu_heap_block_deform_kernel, dim3(hp->nunits), dim3(XHEAP_BLOCK), 0,
-                           (hipStream_t)stream, *hp);
+    {
+        /* three 48 KB blocks of four waves per compute unit, or fewer when
+         * the pages run out */
+        uint32_t per = XHEAP_DEFORM_BLOCK / 64;
+        uint32_t grid = (hp->nunits + per - 1) / per;
+        if (grid > 3U * (uint32_t)ncu)
+            grid = 3U * (uint32_t)ncu;
+        if (hp->natts_walk <= XHEAP_REG_COLS)
+            hipLaunchKernelGGL(xpu_heap_block_deform_kernel<XHEAP_REG_COLS>, dim3(grid),
+                               dim3(XHEAP_DEFORM_BLOCK), 0, (hipStream_t)stream, *hp);
+        else
+            hipLaunchKernelGGL(xpu_heap_block_deform_kernel<0>, dim3(grid),
+                               dim3(XHEAP_DEFORM_BLOCK), 0, (hipStream_t)stream, *hp);
+    }
     else
-        hipLaunchKernelGGL(xpu_heap_row_deform_kernel, dim3(hp->nunits), dim3(XHEAP_BLOCK), 0,
-                           (hipStream_t)stream, *hp);
+    {
+        if (hp->natts_walk <= XHEAP_REG_COLS)
+            hipLaunchKernelGGL(xpu_heap_row_deform_kernel<XHEAP_REG_COLS>, dim3(hp->nunits),
+                               dim3(XHEAP_BLOCK), 0, (hipStream_t)stream, *hp);
+        else
+            hipLaunchKernelGGL(xpu_heap_row_deform_kernel<0>, dim3(hp->nunits), dim3(XHEAP_BLOCK),
+                               0, (hipStream_t)stream, *hp);
+    }
     return (int)hipGetLastError();
 }

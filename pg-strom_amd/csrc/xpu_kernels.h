@@ -82,7 +82,8 @@ typedef struct
  * ------------------------------------------------------------------ */
 #define XARROW_TAG_HEAPREF      0x4845      /* not an ArrowTypeTag */
 #define XHEAP_LEN_EXTERNAL      0xffffffffU
-#define XHEAP_BLOCK             64          /* one wave per page / unit */
+#define XHEAP_BLOCK             64          /* row chunks: one wave per 64-row unit */
+#define XHEAP_DEFORM_BLOCK      256         /* heap pages: four persistent waves */
 #define XHEAP_COUNT_BLOCK       256
 #define XHEAP_SCAN_BLOCK        1024
 
@@ -248,7 +249,7 @@ int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
 /* heap / row sources: count pass, unit scan (total into st->total), deform pass */
 int xpu_launch_heap_count(const xheap_params *hp, void *stream);
-int xpu_launch_heap_deform(const xheap_params *hp, void *stream);
+int xpu_launch_heap_deform(const xheap_params *hp, int ncu, void *stream);
 int xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream);
 int xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head,
                             uint32_t head_len, uint64_t length, const xpu_task_ctl *ctl,

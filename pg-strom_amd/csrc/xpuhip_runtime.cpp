@@ -1453,7 +1453,7 @@ heap_tiles(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, const ke
                                 hipMemcpyHostToDevice, x->stream), eb, "hipMemcpyAsync(tiles)")))
         return ERRCODE_DEVICE_ERROR;
     xpu_launch_fill(x->d_tiles + bm_begin, 0, (bm_end - bm_begin) / 8, x->stream);
-    rc = xpu_launch_heap_deform(&hp, x->stream);
+    rc = xpu_launch_heap_deform(&hp, g_dev.ncu, x->stream);
     (void)hipEventRecord(x->evh1, x->stream);
     if (rc != 0)
     {
