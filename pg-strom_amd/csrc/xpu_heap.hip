@@ -364,27 +364,31 @@ xpu_heap_count_kernel(xheap_params p)
 }
 
 /* exclusive prefix of units[0..n) in place; units[n] and st->total get the
- * sum.  Rounds of XHEAP_SCAN_BLOCK * XHEAP_SCAN_SEG units: every thread
- * loads its XHEAP_SCAN_SEG contiguous counts at once, the block scans the
- * thread sums, and the round's total carries into the next round. */
-#define XHEAP_SCAN_SEG  16
+ * sum.  Rounds of 4 * XHEAP_SCAN_BLOCK units: every thread holds 4
+ * contiguous counts (one coalesced 16-byte load; the next round's load is
+ * issued before this round's scan), the block scans the thread sums, and
+ * the round's total carries into the next round.  The units array is padded
+ * to 4 entries past n, which read as 0 and receive the total. */
 __global__ void __launch_bounds__(XHEAP_SCAN_BLOCK)
 xpu_heap_scan_kernel(uint32_t *units, uint32_t n, xheap_status *st)
 {
+    constexpr uint32_t ROUND = 4 * XHEAP_SCAN_BLOCK;
     __shared__ uint32_t wsum[XHEAP_SCAN_BLOCK / 64];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / 64;
+    uint4 *u4 = (uint4 *)units;
     uint32_t carry = 0;
-    for (uint32_t r0 = 0; r0 < n; r0 += XHEAP_SCAN_BLOCK * XHEAP_SCAN_SEG)
+    uint4 cur = make_uint4(0, 0, 0, 0);
+    if (4 * tid < n)
+        cur = u4[tid];
+    for (uint32_t r0 = 0; r0 < n; r0 += ROUND)
     {
-        const uint32_t b = r0 + tid * XHEAP_SCAN_SEG;
-        uint32_t c[XHEAP_SCAN_SEG];
-        uint32_t s = 0;
-#pragma unroll
-        for (uint32_t i = 0; i < XHEAP_SCAN_SEG; i++)
-        {
-            c[i] = b + i < n ? units[b + i] : 0;
-            s += c[i];
-        }
+        const uint32_t b = r0 + 4 * tid;
+        uint4 nxt = make_uint4(0, 0, 0, 0);
+        if (b + ROUND < n)
+            nxt = u4[(b + ROUND) / 4];
+        uint32_t c0 = b < n ? cur.x : 0, c1 = b + 1 < n ? cur.y : 0;
+        uint32_t c2 = b + 2 < n ? cur.z : 0, c3 = b + 3 < n ? cur.w : 0;
+        uint32_t s = c0 + c1 + c2 + c3;
         uint32_t incl = s;
         for (uint32_t off = 1; off < 64; off <<= 1)
         {
@@ -403,19 +407,16 @@ xpu_heap_scan_kernel(uint32_t *units, uint32_t n, xheap_status *st)
             before += w < wave ? ws : 0;
             round += ws;
         }
-        uint32_t run = before + incl - s;
-#pragma unroll
-        for (uint32_t i = 0; i < XHEAP_SCAN_SEG; i++)
-        {
-            if (b + i < n)
-                units[b + i] = run;
-            run += c[i];
-        }
+        uint32_t e0 = before + incl - s;
+        if (b <= n)
+            u4[b / 4] = make_uint4(e0, e0 + c0, e0 + c0 + c1, e0 + c0 + c1 + c2);
         carry += round;
+        cur = nxt;
     }
     if (tid == 0)
     {
-        units[n] = carry;
+        if (n % ROUND == 0)
+            units[n] = carry;
         st->total = carry;
     }
 }

@@ -1335,7 +1335,8 @@ heap_tiles(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, const ke
     /* status, column table, unit counts */
     const size_t cols_off = 16;
     const size_t units_off = (cols_off + sizeof(xheap_col) * natts + 15) & ~(size_t)15;
-    if (!grow(&x->d_hmeta, &x->hmeta_cap, units_off + 4ULL * (hp.nunits + 1), eb))
+    /* the unit scan reads and writes whole groups of 4 units past nunits */
+    if (!grow(&x->d_hmeta, &x->hmeta_cap, units_off + 4ULL * (hp.nunits + 8), eb))
         return ERRCODE_DEVICE_ERROR;
     hp.st = (xheap_status *)x->d_hmeta;
     hp.cols = (const xheap_col *)(x->d_hmeta + cols_off);
