@@ -791,7 +791,7 @@ lds_fields(int action, int *n64, int *n32)
         case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
             *n64 = 1; return true;
         case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
-            *n64 = 1; *n32 = 1; return true;    /* low word + signed carry count */
+            *n64 = 1; *n32 = 1; return true;    /* low-half sum + count of negatives */
         case KAGG_ACTION__STDDEV:
             *n64 = 2; return true;
     }

@@ -1208,7 +1208,8 @@ lds_table_at(cplan *P, unsigned long long *base, uint32_t n)
 
 /* A row may use the LDS table when every aggregate argument is non-NULL and
  * every FILTER is true: then one row counter stands for count(*), count(x)
- * and the n of every sum and average. */
+ * and the n of every sum and average.  The argument of a 128-bit sum must
+ * also lie in [-2^32, 2^32), the range the table's two-field sum holds. */
 __device__ __forceinline__ bool
 gb_row_fast(cplan *P, const Lane &L)
 {
@@ -1222,6 +1223,12 @@ gb_row_fast(cplan *P, const Lane &L)
             ok = ok && !RNULL(L, ag.filter_reg) && L.R[ag.filter_reg] != 0;
         if (ag.arg_reg != 0xff)
             ok = ok && !RNULL(L, ag.arg_reg);
+        if ((ag.action == KAGG_ACTION__PSUM_INT64 || ag.action == KAGG_ACTION__PAVG_INT64) &&
+            ag.arg_reg != 0xff)
+        {
+            const int64_t hi = val_as_i64(ag.arg_domain, L.R[ag.arg_reg]) >> 32;
+            ok = ok && (hi == 0 || hi == -1);
+        }
     }
     return ok;
 }
@@ -1315,9 +1322,22 @@ gb_merge_global(cplan *P, const Lane &L, kern_tupitem *t, const LdsTable *T,
             {
                 if (!n)
                     break;
-                unsigned long long lo = from_lds ? F64[0] : (unsigned long long)iv;
-                unsigned long long hi = from_lds ? (unsigned long long)(long long)(int32_t)F32[0]
-                                                 : (iv < 0 ? ~0ULL : 0ULL);
+                /* an LDS entry holds the sum of the values' low 32-bit
+                 * halves and the count of negative values (every value lies
+                 * in [-2^32, 2^32)): lo - count * 2^32 as 128 bits */
+                unsigned long long lo, hi;
+                if (from_lds)
+                {
+                    const unsigned long long sl = F64[0];
+                    const unsigned long long sub = (unsigned long long)F32[0] << 32;
+                    lo = sl - sub;
+                    hi = sl < sub ? ~0ULL : 0ULL;
+                }
+                else
+                {
+                    lo = (unsigned long long)iv;
+                    hi = iv < 0 ? ~0ULL : 0ULL;
+                }
                 atomicAdd((unsigned long long *)(s + 8), n);
                 unsigned long long old = atomicAdd((unsigned long long *)(s + 16), lo);
                 hi += (old + lo < old) ? 1ULL : 0ULL;
@@ -1668,13 +1688,14 @@ lds_update(cplan *P, const Lane &L, LdsTable &T, int slot)
                 break;
             case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
             {
-                /* 64-bit low word; the high word of the 128-bit sum is the
-                 * signed count of carries and sign extensions */
+                /* v in [-2^32, 2^32) (gb_row_fast): the sum of the low
+                 * 32-bit halves and the count of negative values hold the
+                 * 128-bit sum exactly below 2^32 rows, with no returning
+                 * atomic */
                 int64_t v = val_as_i64(ag.arg_domain, raw);
-                unsigned long long old = atomicAdd(F64, (unsigned long long)v);
-                int hi = (v < 0 ? -1 : 0) + (((unsigned long long)v + old < old) ? 1 : 0);
-                if (hi)
-                    atomicAdd((int *)F32, hi);
+                atomicAdd(F64, (unsigned long long)(uint32_t)v);
+                if (v < 0)
+                    atomicAdd(F32, 1U);
                 break;
             }
             case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
@@ -1911,13 +1932,9 @@ lds_merge_entry(cplan *P, LdsTable &T, int slot, const uint64_t *rec)
                 atomicAdd(F64, (unsigned long long)v64);
                 break;
             case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
-            {
-                unsigned long long old = atomicAdd(F64, (unsigned long long)v64);
-                int hi = (int)v32 + ((old + v64 < old) ? 1 : 0);
-                if (hi)
-                    atomicAdd((int *)F32, hi);
+                atomicAdd(F64, (unsigned long long)v64);
+                atomicAdd(F32, v32);
                 break;
-            }
             case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
                 atomicAdd((double *)F64, __longlong_as_double((long long)v64));
                 break;

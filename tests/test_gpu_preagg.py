@@ -235,3 +235,36 @@ def test_session_autotune(gpu):
     d.free()
     exp = run_oracle_groupby(blob, [src])
     assert_groups_equal(got, exp, C1_ACTIONS)
+
+
+@pytest.mark.parametrize("lds_nents", [0, 64])
+def test_preagg_int128_sums(gpu, jit, lds_nents):
+    """sum(int8) over full-range values: group sums leave the int64 range, so
+    the 128-bit state (LDS halves, spill records, partial-state dumps and the
+    kds_final carry) must be exact; a 64-entry LDS table forces spills."""
+    r = datagen.rng(9, 1)
+    n = 400_000
+    key = r.integers(0, 300, size=n, dtype=np.int32)
+    v = r.integers(-2**63, 2**63 - 1, size=n, dtype=np.int64, endpoint=True)
+    v[::7] = 2**63 - 1
+    v[3::11] = -2**63
+    cols = [kds.Column("key", "int4", key), kds.Column("v", "int8", v)]
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    kv = plan.scan_var("key")
+    plan.set_groupby([(abi.KAGG_ACTION__NROWS_ANY, None, ("n", "int8")),
+                      (abi.KAGG_ACTION__PSUM_INT64, plan.scan_var("v"), ("s", "bytea")),
+                      (abi.KAGG_ACTION__VREF, kv, ("key", "int4"))])
+    blob = plan.build(final_nrows=1000)
+    src = kds.build_arrow_kds(cols)
+    tuning = dict(jit=jit)
+    if lds_nents:
+        tuning["lds_nents"] = lds_nents
+    got = run_gpu_groupby(blob, [src, src], tuning=tuning)
+    exp = run_oracle_groupby(blob, [src, src])
+    acts = [abi.KAGG_ACTION__NROWS_ANY, abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__VREF]
+    groups = assert_groups_equal(got, exp, acts)
+    vs = v.astype(object)
+    for k in (0, 17, 299):
+        sel = key == k
+        assert groups[(k,)][1] == (2 * int(sel.sum()), 2 * int(vs[sel].sum()))
+    assert any(abs(g[1][1]) > 2**63 for g in groups.values())
