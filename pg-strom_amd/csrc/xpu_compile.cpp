@@ -957,6 +957,21 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             FAIL("projection_kds_dst out of the session blob");
         out->kds_dst_head.assign((const char *)h, (const char *)h + hl);
     }
+    if (!kactions && (s->xpu_task_flags & DEVTASK__PINNED_HASH_RESULTS))
+    {
+        /* results kept on the device as an inner buffer of a later join
+         * (executor.c:655-672): the buffer's KDS_FORMAT_HASH header is the
+         * session's groupby_kds_final */
+        if (!s->groupby_kds_final || s->groupby_kds_final + sizeof(kern_data_store) > length)
+            FAIL("pinned hash results without a buffer header");
+        const kern_data_store *kf = (const kern_data_store *)((const char *)s + s->groupby_kds_final);
+        size_t hl = KDS_HEAD_LENGTH(kf);
+        if (s->groupby_kds_final + hl > length || kf->format != KDS_FORMAT_HASH ||
+            kf->hash_nslots == 0)
+            FAIL("pinned hash results need a KDS_FORMAT_HASH buffer header");
+        out->kds_final_head.assign((const char *)kf, (const char *)kf + hl);
+        out->pinned_hash = true;
+    }
     if (kactions)
     {
         /* ---- group-by ---- */
@@ -1124,6 +1139,21 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
         const kern_expression *a = C.first_arg(kproj);
         for (uint32_t i = 0; i < kproj->nr_args; i++, a = C.next_arg(a))
             C.expr(kproj, a, XP_REG_TRUE);
+        /* pinned inner buffer: the projection's HashValue over the join keys
+         * (codegen.c:3933-3941) becomes each row's kern_tupitem.hash
+         * (cuda_gpujoin.cu:799-829) */
+        if (s->xpu_task_flags & DEVTASK__PINNED_HASH_RESULTS)
+        {
+            if (kproj->u.proj.hash == 0)
+                FAIL("pinned hash results without a projection hash");
+            const kern_expression *kh = (const kern_expression *)((const char *)kproj +
+                                                                  kproj->u.proj.hash);
+            if (!C.check_node(kproj, kh) || kh->opcode != FuncOpCode__HashValue)
+                FAIL("malformed projection hash");
+            Reg h = C.expr(kproj, kh, XP_REG_TRUE);
+            C.emit(XO_MOV, XP_REG_RESULT, h.r);
+            P->r_projhash = XP_REG_RESULT;
+        }
         C.end_seg(P->seg_final);
         if (!C.err.empty())
             FAIL(C.err.c_str());

@@ -26,6 +26,8 @@ struct xpu_compiled
     std::vector<char>   kds_final_head;     /* header of kds_final (HASH or ROW) */
     bool                has_projection = false;
     bool                has_groupby = false;
+    bool                pinned_hash = false;    /* projection rows go to a device-resident
+                                                   KDS_FORMAT_HASH buffer (kds_final_head) */
     uint32_t            nslots = 0;
     uint32_t            lds_entry_bytes = 0;    /* bytes per LDS group entry */
     std::vector<int64_t> params;                /* XO_PARAM values (raw register form) */

@@ -978,7 +978,8 @@ proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t row
     t_hoff += ROWID_SIZE;
     kern_tupitem *t = (kern_tupitem *)buf;
     t->t_len = t_hoff;
-    t->hash = 0;
+    /* rows of a pinned inner buffer carry their join-key hash */
+    t->hash = P->r_projhash ? (uint32_t)L.R[P->r_projhash] : 0U;
     t->t_infomask2 = (uint16_t)(nattrs & HEAP_NATTS_MASK);
     t->t_infomask = infomask;
     t->t_hoff = (uint8_t)((uint32_t)xalign(8, KERN_TUPITEM_T_BITS_OFFSET +

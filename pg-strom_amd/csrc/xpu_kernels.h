@@ -247,6 +247,12 @@ int xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *wo
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);
+/* pinned inner buffers: append a ROW chunk's rows (nrows: an upper bound
+ * for the grid) to a KDS_FORMAT_HASH buffer (base[2]: reservation, see the
+ * kernel), and re-chain a copied buffer into its own slots */
+int xpu_launch_hash_append(kern_data_store *hk, const kern_data_store *rk, uint64_t *base,
+                           uint32_t nrows, void *stream);
+int xpu_launch_hash_rechain(kern_data_store *hk, uint32_t nitems, void *stream);
 /* heap / row sources: count pass, unit scan (total into st->total), deform pass */
 int xpu_launch_heap_count(const xheap_params *hp, void *stream);
 int xpu_launch_heap_deform(const xheap_params *hp, int ncu, void *stream);

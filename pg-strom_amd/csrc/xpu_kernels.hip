@@ -298,6 +298,136 @@ xpu_launch_projection(const xpu_kparams *kp, int nregs, int grid, int block, voi
     return (int)hipGetLastError();
 }
 
+/* ------------------------------------------------------------------
+ * pinned inner buffers: KDS_FORMAT_HASH chunks kept in device memory
+ * ------------------------------------------------------------------ */
+
+/* One thread: reserves the rows and bytes of the ROW chunk rk (rows as
+ * kern_hashitems) in hk under the buffer's lock, nitems = UINT_MAX while
+ * held (the allocation protocol of cuda_gpujoin.cu:836-878).  base[0] and
+ * base[1] get the first row id and byte offset, or base[0] = ~0 and base[1]
+ * = 1 (the buffer is full) or 2 (the lock was not obtained). */
+__global__ void
+xpu_hash_reserve_kernel(kern_data_store *hk, const kern_data_store *rk, uint64_t *base)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    const uint64_t n = rk->nitems;
+    const uint64_t bytes = rk->usage + 8 * n;
+    for (uint32_t spin = 0; spin < XLB_SPIN_LIMIT; spin++)
+    {
+        uint32_t cur = __hip_atomic_load(&hk->nitems, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == 0xffffffffU || atomicCAS(&hk->nitems, cur, 0xffffffffU) != cur)
+            continue;
+        const uint64_t usage = __hip_atomic_load(&hk->usage, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t need = KDS_HEAD_LENGTH(hk) + 8 * (hk->hash_nslots + cur + n) + usage + bytes;
+        if (cur + n >= 0xffffffffULL || need > hk->length)
+        {
+            base[0] = ~0ULL;
+            base[1] = 1;
+        }
+        else
+        {
+            __hip_atomic_store(&hk->usage, usage + bytes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            base[0] = cur;
+            base[1] = usage;
+            cur += (uint32_t)n;
+        }
+        __hip_atomic_store(&hk->nitems, cur, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    base[0] = ~0ULL;
+    base[1] = 2;
+}
+
+/* Appends every row of the ROW chunk rk (rows packed from its tail in row
+ * order) into hk at the reserved base: row i becomes a kern_hashitem
+ * base[1] + U_i + 8 (i + 1) bytes from hk's tail (U_i: its tuple's offset in
+ * rk), takes row id base[0] + i and is pushed onto the chain of its
+ * tupitem's hash (the HASH branch of cuda_gpujoin.cu:898-916). */
+__global__ void
+xpu_hash_append_kernel(kern_data_store *hk, const kern_data_store *rk, const uint64_t *base)
+{
+    const uint64_t b0 = base[0], u0 = base[1];
+    if (b0 == ~0ULL)
+        return;
+    const uint32_t n = rk->nitems;
+    const uint64_t *rrow = (const uint64_t *)((const char *)rk + KDS_HEAD_LENGTH(rk));
+    unsigned long long *slots = (unsigned long long *)((char *)hk + KDS_HEAD_LENGTH(hk));
+    const uint64_t nslots = hk->hash_nslots;
+    uint64_t *hrow = (uint64_t *)(slots + nslots);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint64_t U = rrow[i];
+        if (U > rk->usage || U < 16)
+            continue;
+        const kern_tupitem *t = (const kern_tupitem *)((const char *)rk + rk->length - U);
+        const uint32_t t_len = t->t_len;
+        if (t_len > U || t_len < KERN_TUPITEM_T_BITS_OFFSET + ROWID_SIZE)
+            continue;
+        const uint64_t H = u0 + U + 8ULL * (i + 1);
+        kern_hashitem *hi = (kern_hashitem *)((char *)hk + hk->length - H);
+        const uint64_t *src = (const uint64_t *)t;
+        uint64_t *dst = (uint64_t *)&hi->t;
+        for (uint32_t w = 0; w < (t_len + 7) / 8; w++)
+            dst[w] = src[w];
+        *(uint32_t *)((char *)&hi->t + t_len - ROWID_SIZE) = (uint32_t)(b0 + i);
+        hi->next = atomicExch(&slots[t->hash % nslots], (unsigned long long)H);
+        hrow[b0 + i] = H - offsetof(kern_hashitem, t);
+    }
+}
+
+/* Re-chains the items of a KDS_FORMAT_HASH chunk whose row index and tuple
+ * area were copied from another buffer, into its own (zeroed) hash slots:
+ * kern_buffer_reconstruction (cuda_gpujoin.cu:18-74) with the items left
+ * where they are. */
+__global__ void
+xpu_hash_rechain_kernel(kern_data_store *hk)
+{
+    const uint32_t n = hk->nitems;
+    unsigned long long *slots = (unsigned long long *)((char *)hk + KDS_HEAD_LENGTH(hk));
+    const uint64_t nslots = hk->hash_nslots;
+    const uint64_t *hrow = (const uint64_t *)(slots + nslots);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint64_t H = hrow[i] + offsetof(kern_hashitem, t);
+        if (H > hk->usage)
+            continue;
+        kern_hashitem *hi = (kern_hashitem *)((char *)hk + hk->length - H);
+        hi->next = atomicExch(&slots[hi->t.hash % nslots], (unsigned long long)H);
+    }
+}
+
+extern "C" int
+xpu_launch_hash_append(kern_data_store *hk, const kern_data_store *rk, uint64_t *base,
+                       uint32_t nrows, void *stream)
+{
+    hipLaunchKernelGGL(xpu_hash_reserve_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                       hk, rk, base);
+    if (nrows > 0)
+    {
+        uint32_t blocks = (nrows + 255) / 256;
+        if (blocks > 4096)
+            blocks = 4096;
+        hipLaunchKernelGGL(xpu_hash_append_kernel, dim3(blocks), dim3(256), 0,
+                           (hipStream_t)stream, hk, rk, (const uint64_t *)base);
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_hash_rechain(kern_data_store *hk, uint32_t nitems, void *stream)
+{
+    if (nitems == 0)
+        return 0;
+    uint32_t blocks = (nitems + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(xpu_hash_rechain_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, hk);
+    return (int)hipGetLastError();
+}
+
 extern "C" int
 xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream)
 {

@@ -195,7 +195,8 @@ typedef struct
     uint8_t     r_keyhash;
     uint8_t     nkeys;
     uint8_t     lds_fast;           /* keys pack into 63 bits: LDS partial table */
-    uint8_t     __pad2;
+    uint8_t     r_projhash;         /* projection into a pinned hash buffer: the
+                                       register of the row's join-key hash (0: none) */
     uint32_t    gb_nattrs;
     uint32_t    gb_t_hoff;          /* MAXALIGN(offsetof(t_bits) + bitmap) */
     uint32_t    gb_states_end;      /* payload offset after the last state */

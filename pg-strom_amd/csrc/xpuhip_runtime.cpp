@@ -207,6 +207,9 @@ struct xpuhip_session
     uint32_t        maxdup[XP_MAX_DEPTH + 1] = {};      /* most items sharing a join hash */
     int64_t        *d_params = nullptr;     /* XO_PARAM values */
     uint32_t       *d_xids = nullptr;       /* parallelCurrentXids (row-chunk visibility) */
+    struct PinnedBuf *pinned = nullptr;     /* pinned hash results: the shared buffer */
+    char           *d_inner_own[XP_MAX_DEPTH + 1] = {};  /* inner buffers rebuilt from pinned
+                                                            results (owned) */
     bool            inner_ready = false;
     xpuhip_tuning   tuning;
     std::mutex      lock;
@@ -214,7 +217,22 @@ struct xpuhip_session
     uint64_t        ntasks = 0;
 };
 
+/* Pinned inner buffers (gpu_service.c:2099-2196): the KDS_FORMAT_HASH
+ * buffer that the projections of every session of one query_plan_id append
+ * to, kept while a session of that plan is open.  A join names it by the
+ * plan id in its kern_multirels chunk (buffer_id) and attaches a copy. */
+struct PinnedBuf
+{
+    uint64_t    plan_id = 0;
+    char       *d_kds = nullptr;
+    uint64_t    length = 0;
+    int         refs = 0;
+};
+
 namespace {
+
+std::mutex g_pinned_lock;
+std::map<uint64_t, PinnedBuf *> g_pinned;
 
 /* Task scratch (stream, events, pinned control block, grown buffers) of
  * closed sessions, kept for the next sessions: opening and closing a
@@ -435,7 +453,7 @@ init_kds_final(xpuhip_session *s, kern_errorbuf *eb)
     h->length = length;
     h->usage = 0;
     h->nitems = 0;
-    if (P->nkeys == 0)
+    if (P->nkeys == 0 && !s->comp.pinned_hash)
     {
         /* ROW-format kds_final holding the single group */
         uint32_t unit;
@@ -639,6 +657,42 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
             return rc ? rc : ERRCODE_DEVICE_ERROR;
         }
     }
+    if (s->comp.pinned_hash)
+    {
+        /* the plan's pinned buffer: shared by its sessions, created by the
+         * first one */
+        std::lock_guard<std::mutex> g(g_pinned_lock);
+        const uint64_t plan_id = session->query_plan_id;
+        auto it = g_pinned.find(plan_id);
+        PinnedBuf *pb = it != g_pinned.end() ? it->second : nullptr;
+        if (!pb)
+        {
+            /* the header's length, as the reference allocates the pieces of a
+             * pinned buffer (executor.c:664-668), at least its slots */
+            const kern_data_store *fh = (const kern_data_store *)s->comp.kds_final_head.data();
+            uint64_t len = fh->length;
+            uint64_t need = s->comp.kds_final_head.size() + 8ULL * fh->hash_nslots + (64ULL << 10);
+            if (len < need)
+                len = need;
+            len = (len + XPU_L1_CACHELINE_SZ - 1) & ~(uint64_t)(XPU_L1_CACHELINE_SZ - 1);
+            s->final_length = len;
+            if (!hip_ok(hipMalloc((void **)&s->d_kds_final, len), eb, "hipMalloc(pinned buffer)") ||
+                (rc = init_kds_final(s, eb)) != 0)
+            {
+                xpuhip_session_close(s);
+                return rc ? rc : ERRCODE_DEVICE_ERROR;
+            }
+            pb = new PinnedBuf();
+            pb->plan_id = plan_id;
+            pb->d_kds = (char *)s->d_kds_final;
+            pb->length = len;
+            g_pinned[plan_id] = pb;
+        }
+        pb->refs++;
+        s->pinned = pb;
+        s->d_kds_final = (kern_data_store *)pb->d_kds;
+        s->final_length = pb->length;
+    }
     if (!hip_ok(hipMemcpy(s->d_plan, &s->comp.plan, sizeof(xplan), hipMemcpyHostToDevice), eb, "hipMemcpy"))
     {
         xpuhip_session_close(s);
@@ -646,6 +700,71 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
     }
     s->inner_ready = (s->comp.plan.nrels == 0);
     *p_sess = s;
+    return 0;
+}
+
+/* A copy of the pinned buffer of plan_id with hash slots for its rows
+ * (KDS_GET_HASHSLOT_WIDTH), owned by the attaching session: the row index
+ * and the tuple area are copied, the items re-chained
+ * (gpu_service.c:1906-2056 with kern_buffer_reconstruction). */
+static int
+attach_pinned(xpuhip_session *s, uint32_t d, uint64_t plan_id, uint64_t *nitems_out,
+              uint64_t *nslots_out, kern_errorbuf *eb)
+{
+    PinnedBuf *pb = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_pinned_lock);
+        auto it = g_pinned.find(plan_id);
+        if (it != g_pinned.end())
+            pb = it->second;
+    }
+    char msg[128];
+    if (!pb)
+    {
+        snprintf(msg, sizeof(msg), "pinned inner buffer %llx was not found",
+                 (unsigned long long)plan_id);
+        XERR(eb, ERRCODE_DEVICE_ERROR, msg);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!hip_ok(hipDeviceSynchronize(), eb, "pinned buffer"))
+        return ERRCODE_DEVICE_ERROR;
+    kern_data_store h0;
+    if (!hip_ok(hipMemcpy(&h0, pb->d_kds, sizeof(h0), hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return ERRCODE_DEVICE_ERROR;
+    std::vector<char> head(KDS_HEAD_LENGTH(&h0));
+    if (!hip_ok(hipMemcpy(head.data(), pb->d_kds, head.size(), hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return ERRCODE_DEVICE_ERROR;
+    kern_data_store *h = (kern_data_store *)head.data();
+    const uint64_t hl = head.size();
+    const uint64_t n = h->nitems, usage = h->usage, src_slots = h->hash_nslots;
+    const uint64_t nslots = KDS_GET_HASHSLOT_WIDTH(n);
+    const uint64_t length = XPU_MAXALIGN(hl + 8 * nslots + 8 * n + usage);
+    if (length >= (1ULL << 32))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "pinned inner buffer is 4GB or larger");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    h->length = length;
+    h->hash_nslots = nslots;
+    if (s->d_inner_own[d])
+        (void)hipFree(s->d_inner_own[d]);
+    s->d_inner_own[d] = nullptr;
+    char *dst = nullptr;
+    if (!hip_ok(hipMalloc((void **)&dst, length), eb, "hipMalloc(inner buffer)"))
+        return ERRCODE_DEVICE_ERROR;
+    s->d_inner_own[d] = dst;
+    if (!hip_ok(hipMemcpy(dst, head.data(), hl, hipMemcpyHostToDevice), eb, "hipMemcpy") ||
+        !hip_ok((hipError_t)xpu_launch_fill(dst + hl, 0, nslots, nullptr), eb, "fill") ||
+        (n > 0 && !hip_ok(hipMemcpy(dst + hl + 8 * nslots, pb->d_kds + hl + 8 * src_slots, 8 * n,
+                                    hipMemcpyDeviceToDevice), eb, "hipMemcpy")) ||
+        (usage > 0 && !hip_ok(hipMemcpy(dst + length - usage, pb->d_kds + pb->length - usage, usage,
+                                        hipMemcpyDeviceToDevice), eb, "hipMemcpy")) ||
+        !hip_ok((hipError_t)xpu_launch_hash_rechain((kern_data_store *)dst, (uint32_t)n, nullptr),
+                eb, "rechain") ||
+        !hip_ok(hipDeviceSynchronize(), eb, "pinned buffer copy"))
+        return ERRCODE_DEVICE_ERROR;
+    *nitems_out = n;
+    *nslots_out = nslots;
     return 0;
 }
 
@@ -661,9 +780,19 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
         XERR(eb, ERRCODE_DEVICE_ERROR, "kern_multirels does not match the session's joins");
         return ERRCODE_DEVICE_ERROR;
     }
+    int rc;
+    if ((rc = ensure_device(eb)) != 0)
+        return rc;
     for (uint32_t d = 1; d <= P->nrels; d++)
     {
         const kern_multirels_chunk *ch = &kmrels->chunks[d - 1];
+        if (ch->is_nestloop || ch->right_outer || ch->gist_offset)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "nest-loop, GiST and RIGHT OUTER joins are not supported");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        if (ch->pinned_buffer)
+            continue;       /* a pinned inner buffer on the device, by buffer_id */
         if (ch->kds_offset == 0 || ch->kds_offset + sizeof(kern_data_store) > length)
         {
             XERR(eb, ERRCODE_DEVICE_ERROR, "inner KDS out of kern_multirels");
@@ -681,11 +810,6 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
             XERR(eb, ERRCODE_DEVICE_ERROR, "inner KDS is truncated or larger than 4GB");
             return ERRCODE_DEVICE_ERROR;
         }
-        if (ch->is_nestloop || ch->right_outer || ch->gist_offset)
-        {
-            XERR(eb, ERRCODE_DEVICE_ERROR, "nest-loop, GiST and RIGHT OUTER joins are not supported");
-            return ERRCODE_DEVICE_ERROR;
-        }
     }
     if (s->d_kmrels)
         (void)hipFree(s->d_kmrels);
@@ -693,10 +817,24 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
     if (!hip_ok(hipMalloc((void **)&s->d_kmrels, length), eb, "hipMalloc(kmrels)") ||
         !hip_ok(hipMemcpy(s->d_kmrels, kmrels, length, hipMemcpyHostToDevice), eb, "hipMemcpy(kmrels)"))
         return ERRCODE_DEVICE_ERROR;
+    uint64_t d_nitems[XP_MAX_DEPTH + 1] = {}, d_nslots[XP_MAX_DEPTH + 1] = {};
     for (uint32_t d = 1; d <= P->nrels; d++)
     {
-        P->depth[d].inner_kds = (uint64_t)(uintptr_t)(s->d_kmrels + kmrels->chunks[d - 1].kds_offset);
-        P->depth[d].left_outer = kmrels->chunks[d - 1].left_outer ? 1 : 0;
+        const kern_multirels_chunk *ch = &kmrels->chunks[d - 1];
+        if (ch->pinned_buffer)
+        {
+            if ((rc = attach_pinned(s, d, ch->buffer_id, &d_nitems[d], &d_nslots[d], eb)) != 0)
+                return rc;
+            P->depth[d].inner_kds = (uint64_t)(uintptr_t)s->d_inner_own[d];
+        }
+        else
+        {
+            const kern_data_store *k = (const kern_data_store *)((const char *)kmrels + ch->kds_offset);
+            d_nitems[d] = k->nitems;
+            d_nslots[d] = k->hash_nslots;
+            P->depth[d].inner_kds = (uint64_t)(uintptr_t)(s->d_kmrels + ch->kds_offset);
+        }
+        P->depth[d].left_outer = ch->left_outer ? 1 : 0;
     }
     /* Bloom filters: 16 to 32 bits per item in 64-bit words (a power of two,
      * at most 2^23 words), built from the hash chains just copied */
@@ -706,10 +844,8 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
     uint64_t nwords = 0;
     for (uint32_t d = 1; d <= P->nrels; d++)
     {
-        const kern_data_store *k = (const kern_data_store *)((const char *)kmrels +
-                                                             kmrels->chunks[d - 1].kds_offset);
         uint64_t w = 1;
-        while (w < (k->nitems + 3) / 4 && w < (1ULL << 23))
+        while (w < (d_nitems[d] + 3) / 4 && w < (1ULL << 23))
             w <<= 1;
         s->bloom_off[d] = nwords;
         s->bloom_mask[d] = (uint32_t)(w - 1);
@@ -726,10 +862,8 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
         uint32_t *d_dup = (uint32_t *)(s->d_bloom + nwords);
         for (uint32_t d = 1; d <= P->nrels; d++)
         {
-            const kern_data_store *k = (const kern_data_store *)((const char *)kmrels +
-                                                                 kmrels->chunks[d - 1].kds_offset);
             if (!hip_ok((hipError_t)xpu_launch_bloom_build((kern_data_store *)(uintptr_t)P->depth[d].inner_kds,
-                                                           k->hash_nslots, s->d_bloom + s->bloom_off[d],
+                                                           d_nslots[d], s->d_bloom + s->bloom_off[d],
                                                            s->bloom_mask[d], d_dup + d, nullptr),
                         eb, "bloom build"))
                 return ERRCODE_DEVICE_ERROR;
@@ -1240,6 +1374,60 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     return 0;
 }
 
+/* projected rows staged per round before they go to a pinned buffer */
+#define XPU_PINNED_STAGING      (64ULL << 20)
+
+/* A projection into the session's pinned hash buffer: the chunk's rows go
+ * through the projection into a ROW staging buffer (resuming while it
+ * fills), and each round's rows are appended to the buffer as hash items
+ * (cuda_gpujoin.cu:898-916).  Nothing returns to the host. */
+static int
+task_exec_pinned(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, uint32_t nitems,
+                 kern_exec_results *res, kern_errorbuf *eb)
+{
+    if (!grow(&x->d_dst, &x->dst_cap, XPU_PINNED_STAGING + 64, eb))
+        return ERRCODE_DEVICE_ERROR;
+    uint64_t *d_base = (uint64_t *)(x->d_dst + XPU_PINNED_STAGING);
+    kern_exec_results sum;
+    memset(&sum, 0, sizeof(sum));
+    sum.num_rels = s->comp.plan.nrels;
+    uint64_t pos = 0;
+    for (;;)
+    {
+        kern_exec_results part;
+        int rc = task_exec_common(s, x, d_src, nitems, (kern_data_store *)x->d_dst,
+                                  XPU_PINNED_STAGING, &pos, &part, eb);
+        if (rc != 0 && rc != ERRCODE_SUSPEND_NO_SPACE)
+            return rc;
+        const uint32_t nrows = (uint32_t)x->h_ctl->out_nitems;
+        if (!hip_ok((hipError_t)xpu_launch_hash_append(s->d_kds_final, (const kern_data_store *)x->d_dst,
+                                                      d_base, nrows, x->stream),
+                    eb, "pinned buffer append") ||
+            !hip_ok(hipMemcpyAsync(x->h_hstat, d_base, 16, hipMemcpyDeviceToHost, x->stream), eb,
+                    "hipMemcpyAsync") ||
+            !hip_ok(hipStreamSynchronize(x->stream), eb, "pinned buffer append"))
+            return ERRCODE_DEVICE_ERROR;
+        const uint64_t *base = (const uint64_t *)x->h_hstat;
+        if (base[0] == ~0ULL)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, base[1] == 1 ? "pinned inner buffer is full"
+                                                        : "pinned inner buffer lock timed out");
+            return ERRCODE_DEVICE_ERROR;
+        }
+        sum.nitems_raw = part.nitems_raw > sum.nitems_raw ? part.nitems_raw : sum.nitems_raw;
+        sum.nitems_in += part.nitems_in;
+        sum.nitems_out += part.nitems_out;
+        for (uint32_t d = 0; d < s->comp.plan.nrels && d < 8; d++)
+            sum.stats[d].nitems_out += part.stats[d].nitems_out;
+        if (rc == 0)
+            break;
+    }
+    sum.nitems_raw = nitems;
+    if (res)
+        *res = sum;
+    return 0;
+}
+
 /* ------------------------------------------------------------------
  * heap-page (KDS_FORMAT_BLOCK) and row (KDS_FORMAT_ROW / HASH) chunks
  *
@@ -1476,8 +1664,11 @@ task_exec_tuples(xpuhip_session *s, Scratch *x, const kern_data_store *d_src,
     int rc = heap_tiles(s, x, d_src, head, &rows, &ms0, eb);
     if (rc != 0)
         return rc;
-    rc = task_exec_common(s, x, (const kern_data_store *)x->d_tiles, rows, d_dst, dst_length,
-                          resume_pos, res, eb);
+    if (s->comp.pinned_hash)
+        rc = task_exec_pinned(s, x, (const kern_data_store *)x->d_tiles, rows, res, eb);
+    else
+        rc = task_exec_common(s, x, (const kern_data_store *)x->d_tiles, rows, d_dst, dst_length,
+                              resume_pos, res, eb);
     /* errors of the deform pass come first: the tiles are not valid then */
     (void)hipMemcpyAsync(x->h_hstat, x->d_hmeta, sizeof(xheap_status), hipMemcpyDeviceToHost,
                          x->stream);
@@ -1533,7 +1724,10 @@ xpuhip_task_exec_device(xpuhip_session *s, const kern_data_store *d_kds_src, uin
     Scratch *x = scratch_get(s, eb);
     if (!x)
         return ERRCODE_DEVICE_ERROR;
-    rc = task_exec_common(s, x, d_kds_src, src_nitems, d_kds_dst, dst_length, resume_pos, res, eb);
+    if (s->comp.pinned_hash)
+        rc = task_exec_pinned(s, x, d_kds_src, src_nitems, res, eb);
+    else
+        rc = task_exec_common(s, x, d_kds_src, src_nitems, d_kds_dst, dst_length, resume_pos, res, eb);
     scratch_put(s, x);
     return rc;
 }
@@ -1566,7 +1760,8 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
     Scratch *x = scratch_get(s, eb);
     if (!x)
         return ERRCODE_DEVICE_ERROR;
-    bool preagg = s->comp.has_groupby;
+    /* group-by and pinned hash results stay on the device */
+    bool preagg = s->comp.has_groupby || s->comp.pinned_hash;
     if (!grow(&x->d_src, &x->src_cap, kds_src->length, eb) ||
         (!preagg && !grow(&x->d_dst, &x->dst_cap, dst_length, eb)))
     {
@@ -1583,6 +1778,8 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
         rc = task_exec_tuples(s, x, (const kern_data_store *)x->d_src, kds_src,
                               preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
                               resume_pos, res, eb);
+    else if (s->comp.pinned_hash)
+        rc = task_exec_pinned(s, x, (const kern_data_store *)x->d_src, kds_src->nitems, res, eb);
     else
         rc = task_exec_common(s, x, (const kern_data_store *)x->d_src, kds_src->nitems,
                               preagg ? nullptr : (kern_data_store *)x->d_dst, dst_length,
@@ -1613,7 +1810,7 @@ extern "C" int64_t
 xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
                   kern_exec_results *res, kern_errorbuf *eb)
 {
-    if (!s || !s->comp.has_groupby)
+    if (!s || (!s->comp.has_groupby && !s->comp.pinned_hash))
     {
         XERR(eb, ERRCODE_DEVICE_ERROR, "session has no group-by final buffer");
         return -1;
@@ -1667,11 +1864,28 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_plan) (void)hipFree(s->d_plan);
     if (s->d_const_pool) (void)hipFree(s->d_const_pool);
     if (s->d_kds_dst_head) (void)hipFree(s->d_kds_dst_head);
-    if (s->d_kds_final) (void)hipFree(s->d_kds_final);
+    if (s->pinned)
+    {
+        /* the plan's last session frees its pinned buffer */
+        std::lock_guard<std::mutex> g(g_pinned_lock);
+        if (--s->pinned->refs == 0)
+        {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(s->pinned->d_kds);
+            g_pinned.erase(s->pinned->plan_id);
+            delete s->pinned;
+        }
+        s->pinned = nullptr;
+        s->d_kds_final = nullptr;
+    }
+    else if (s->d_kds_final)
+        (void)hipFree(s->d_kds_final);
     if (s->d_kmrels) (void)hipFree(s->d_kmrels);
     if (s->d_bloom) (void)hipFree(s->d_bloom);
     if (s->d_params) (void)hipFree(s->d_params);
     if (s->d_xids) (void)hipFree(s->d_xids);
+    for (int d = 0; d <= XP_MAX_DEPTH; d++)
+        if (s->d_inner_own[d]) (void)hipFree(s->d_inner_own[d]);
     delete s;
 }
 
@@ -1777,7 +1991,7 @@ xpuhip_session_reset(xpuhip_session *s, kern_errorbuf *eb)
         XERR(eb, ERRCODE_DEVICE_ERROR, "no session");
         return ERRCODE_DEVICE_ERROR;
     }
-    if (!s->comp.has_groupby)
+    if (!s->comp.has_groupby && !s->comp.pinned_hash)
         return 0;
     return init_kds_final(s, eb);
 }

@@ -338,10 +338,16 @@ class Plan:
             slots.append(slot)
         return slots
 
-    def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None):
+    def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None,
+              pinned_hash=None, pinned_nslots=137, pinned_length=512 << 20):
         """The session blob.  xids: parallelCurrentXids of the transaction
         (SerializedTransactionState at session_xact_state, xpu_common.h:3156-3165),
-        which decide the visibility of KDS_FORMAT_ROW sources."""
+        which decide the visibility of KDS_FORMAT_ROW sources.  pinned_hash:
+        join-key expressions of a projection whose rows stay on the device
+        as a pinned inner buffer (DEVTASK__PINNED_HASH_RESULTS): the
+        projection carries their HashValue (codegen.c:3933-3941) and
+        groupby_kds_final the buffer's KDS_FORMAT_HASH header
+        (executor.c:655-672)."""
         nrels = len(self.inner_coldefs)
         codes = {}
         # scan quals
@@ -380,6 +386,14 @@ class Plan:
             struct.pack_into("<Ii", b, 40, 0, n)
             for i, s in enumerate(slots):
                 struct.pack_into("<H", b, 48 + 2 * i, s)
+            if pinned_hash:
+                hv = bytearray(_kexp_head(TYPE_CODE["int4"], SPECIAL["HashValue"],
+                                          len(pinned_hash), 40))
+                for k in pinned_hash:
+                    emit_expr(hv, k)
+                _finish(hv, 0)
+                hpos = _append_aligned(b, bytes(hv))
+                struct.pack_into("<I", b, 40, hpos)
             _finish(b, 0)
             codes["projection"] = bytes(b)
         # group-by
@@ -483,6 +497,11 @@ class Plan:
         if self.proj_coldefs is not None:
             head = kdsmod.kds_header(abi.KDS_FORMAT_ROW, self.proj_coldefs, kds_dst_length)
             sess.projection_kds_dst = _append_aligned(buf, bytes(head))
+            if pinned_hash:
+                head = kdsmod.kds_header(abi.KDS_FORMAT_HASH, self.proj_coldefs, pinned_length,
+                                         pinned_nslots)
+                sess.groupby_kds_final = _append_aligned(buf, bytes(head))
+                flags |= abi.DEVTASK__PINNED_HASH_RESULTS
         if final_coldefs is not None:
             head = kdsmod.kds_header(abi.KDS_FORMAT_ROW, final_coldefs, 0)
             sess.projection_kds_dst = _append_aligned(buf, bytes(head))

@@ -303,18 +303,32 @@ def build_hash_kds(coldefs, rows, hashes):
     return out
 
 
+class PinnedInner:
+    """A kern_multirels chunk naming a pinned inner buffer on the device: the
+    projection results of the session whose query_plan_id is buffer_id
+    (gpu_join.c:1925-1932)."""
+
+    def __init__(self, buffer_id):
+        self.buffer_id = buffer_id
+
+    def __len__(self):
+        return 0
+
+
 def build_multirels(inners):
     """kern_multirels holding the inner KDS buffers (list of
-    (kds_bytes, left_outer)) one after another, as gpu_join.c lays out the
-    preload buffer (kds_offset counted from the kmrels head)."""
+    (kds_bytes, left_outer), or (PinnedInner, left_outer) for a pinned
+    buffer) one after another, as gpu_join.c lays out the preload buffer
+    (kds_offset counted from the kmrels head)."""
     nrels = len(inners)
     head_sz = abi.maxalign(abi.KMRELS_CHUNKS_OFFSET +
                            C.sizeof(abi.kern_multirels_chunk) * nrels)
     offs = []
     pos = head_sz
     for kds, _ in inners:
-        offs.append(pos)
-        pos = abi.typealign(64, pos + len(kds))
+        offs.append(0 if isinstance(kds, PinnedInner) else pos)
+        if not isinstance(kds, PinnedInner):
+            pos = abi.typealign(64, pos + len(kds))
     out = np.zeros(pos, dtype=np.uint8)
     hd = abi.kern_multirels_head.from_buffer(out)
     hd.length = pos
@@ -322,8 +336,13 @@ def build_multirels(inners):
     for d, ((kds, left_outer), off) in enumerate(zip(inners, offs)):
         ch = abi.kern_multirels_chunk.from_buffer(
             out, abi.KMRELS_CHUNKS_OFFSET + d * C.sizeof(abi.kern_multirels_chunk))
-        ch.kds_offset = off
         ch.left_outer = bool(left_outer)
+        if isinstance(kds, PinnedInner):
+            ch.kds_offset = 0
+            ch.pinned_buffer = True
+            ch.buffer_id = kds.buffer_id
+            continue
+        ch.kds_offset = off
         out[off:off + len(kds)] = np.frombuffer(bytes(kds), dtype=np.uint8)
     return out
 

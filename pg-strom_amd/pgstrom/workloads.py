@@ -91,6 +91,33 @@ def star_inner(dims):
     return kds.build_multirels(inners)
 
 
+PINNED_PLAN_ID = 0x51a0
+
+
+def dim_scan_pinned(dim_cols, dim_index, pinned_length=64 << 20):
+    """The inner side of the star join as the GPU builds it: a scan of one
+    dimension with its filter (pk % 10 = i) projecting (pk, category, txt)
+    into a pinned inner buffer keyed by HashValue(pk)
+    (DEVTASK__PINNED_HASH_RESULTS).  Its query_plan_id, PINNED_PLAN_ID + i,
+    is the buffer_id the join's kern_multirels names."""
+    plan = cg.Plan([(c.name, c.typname) for c in dim_cols])
+    plan.query_plan_id = PINNED_PLAN_ID + dim_index
+    pk = plan.scan_var("pk")
+    cat = plan.scan_var("category")
+    txt = plan.scan_var("txt")
+    mod, rem = dim_filter_mod(dim_index)
+    plan.scan_quals = cg.Func("int4eq", cg.Func("int4mod", pk, cg.Const("int4", mod)),
+                              cg.Const("int4", rem))
+    plan.set_projection([pk, cat, txt], [("pk", "int4"), ("category", "int2"), ("txt", "text")])
+    return plan.build(pinned_hash=[pk], pinned_length=pinned_length)
+
+
+def star_inner_pinned(ndims=3, left_outer=False):
+    """kern_multirels whose chunks name the pinned buffers of dim_scan_pinned."""
+    return kds.build_multirels([(kds.PinnedInner(PINNED_PLAN_ID + i), left_outer)
+                                for i in range(ndims)])
+
+
 def star_query(fact_cols, ndims=3, left_outer=False):
     plan = cg.Plan([(c.name, c.typname) for c in fact_cols])
     fks = [plan.scan_var("fk%d" % (i + 1)) for i in range(ndims)]
