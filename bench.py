@@ -165,7 +165,9 @@ def bench_star(n, warmup, steps):
     from pgstrom import datagen, kds, workloads as W, xpuhip
     log("generating %d-row star-join fact table and dimensions" % n)
     fact, dims = datagen.star_tables(n)
+    t0 = time.perf_counter()
     km = W.star_inner(dims)
+    host_build_s = time.perf_counter() - t0
     src = kds.build_arrow_kds(fact)
     blob = W.star_query(fact)
     d_src = xpuhip.DeviceBuffer(len(src), src)
@@ -181,7 +183,50 @@ def bench_star(n, warmup, steps):
     s.close()
     d_src.free()
     algo = n * (4 * 3 + 8 + 8)
-    return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo)
+    return dict(rows=n, wall_ms=wall * 1e3, kernel_ms=kms, algo_bytes=algo,
+                host_build_s=host_build_s)
+
+
+def bench_pinned(warmup, steps):
+    """C2's three dimensions (2M / 200K / 2,556 rows, resident in HBM as Arrow
+    chunks) built on the GPU: one scan session per dimension with its filter
+    projects (pk, category, txt) into a pinned inner buffer, and the star
+    join's session attaches the three buffers by buffer_id (a re-chained copy
+    each, plus the Bloom filters).  A step is that whole build."""
+    from pgstrom import datagen, kds, workloads as W, xpuhip
+    fact, dims = datagen.star_tables(10)
+    chunks = []
+    for d in dims:
+        a = kds.build_arrow_kds(d)
+        chunks.append((xpuhip.DeviceBuffer(len(a), a), len(d[0])))
+    blobs = [W.dim_scan_pinned(d, i) for i, d in enumerate(dims)]
+    star_blob = W.star_query(fact)
+    km = W.star_inner_pinned()
+    out = {}
+
+    def step():
+        ss = [xpuhip.Session(b) for b in blobs]
+        kept = 0
+        for s, (dbuf, n) in zip(ss, chunks):
+            s.set_tuning(jit=xpuhip.JIT_ON)
+            rc, res, _ = s.exec_device(dbuf, n)
+            assert rc == 0
+            kept += res.nitems_out
+        j = xpuhip.Session(star_blob)
+        j.attach_inner(km)
+        j.close()
+        for s in ss:
+            s.close()
+        out["kept"] = kept
+
+    wall, _, _ = time_steps(step, warmup, steps)
+    for dbuf, _ in chunks:
+        dbuf.free()
+    rows = sum(n for _, n in chunks)
+    return dict(workload="C2's dimensions (2M / 200K / 2,556 rows, pk % 10 filters) built on the "
+                         "GPU into pinned inner buffers and attached to the star join",
+                build_ms=wall * 1e3, dim_rows=rows, rows_kept=out["kept"],
+                rows_per_s=rows / wall)
 
 
 def scan_line(sc):
@@ -196,7 +241,8 @@ def join_line(st):
                          "GROUP BY (dim1.category, dim3.category) sum(quantity*price)",
                 rows_per_s=st["rows"] / (st["kernel_ms"] / 1e3), kernel_ms=st["kernel_ms"],
                 wall_ms=st["wall_ms"], hbm_gbs=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / 1e9,
-                frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK)
+                frac=st["algo_bytes"] / (st["kernel_ms"] / 1e3) / HBM_PEAK,
+                host_inner_build_s=st["host_build_s"])
 
 
 class _DevPtr:
@@ -394,7 +440,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--only", default="all",
-                    choices=["all", "preagg", "batch", "served", "join", "scan", "heap"])
+                    choices=["all", "preagg", "batch", "served", "join", "scan", "heap", "pinned"])
     ap.add_argument("--served-requests", type=int, default=300)
     ap.add_argument("--served-table-rows", type=int, default=50_000_000)
     ap.add_argument("--served-slots", type=int, default=8)
@@ -427,6 +473,9 @@ def main():
     if args.only == "scan":
         print(json.dumps(scan_line(bench_scan(1_000_000, args.warmup, args.steps))))
         return
+    if args.only == "pinned":
+        print(json.dumps(bench_pinned(args.warmup, args.steps)))
+        return
     if args.only == "heap":
         print(json.dumps(heap_line(bench_heap(1_000_000, args.warmup, args.steps))))
         return
@@ -448,6 +497,7 @@ def main():
         others["served"] = bench_served(args.served_requests, 32, 1_000_000, 50_000_000,
                                         args.served_table_rows)
         others["star_join_200M"] = join_line(bench_star(200_000_000, 1, 5))
+        others["pinned_inner_build"] = bench_pinned(2, 10)
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline")
