@@ -107,3 +107,50 @@ def test_jit_prepare_pipeline_compiles(which):
         fact, _ = datagen.star_tables(10, dim_sizes=(10, 10, 10))
         blob = W.star_query(fact, left_outer=(which == "c2_left"))
     xpuhip.jit_prepare(blob)
+
+
+def test_pinned_hash_session_compiles():
+    """A projection into a pinned inner buffer: the projection's HashValue is
+    compiled after the projected values, into the register the tuple's hash
+    comes from; the blob carries the buffer's KDS_FORMAT_HASH header."""
+    _, dims = datagen.star_tables(10, dim_sizes=(100, 10, 10))
+    blob = W.dim_scan_pinned(dims[0], 0)
+    rc, err = xpuhip.session_check(blob)
+    assert rc == 0, err.describe()
+    s = abi.kern_session_info_head.from_buffer_copy(bytes(blob[:ctypes_size()]))
+    assert s.xpu_task_flags & abi.DEVTASK__PINNED_HASH_RESULTS
+    hd = abi.kern_data_store_head.from_buffer_copy(bytes(blob[s.groupby_kds_final:
+                                                              s.groupby_kds_final + ctypes_kds()]))
+    assert hd.format == abi.KDS_FORMAT_HASH and hd.hash_nslots == 137
+    text = xpuhip.explain(blob)
+    final = text[text.index("[final"):]
+    assert "HASH_FIN" in final and "MOV        d=1" in final
+
+
+def test_pinned_hash_session_needs_its_hash():
+    """DEVTASK__PINNED_HASH_RESULTS without the projection's hash is refused."""
+    _, dims = datagen.star_tables(10, dim_sizes=(100, 10, 10))
+    blob = bytearray(W.dim_scan_pinned(dims[0], 0))
+    s = abi.kern_session_info_head.from_buffer_copy(bytes(blob[:ctypes_size()]))
+    struct.pack_into("<I", blob, s.xpucode_projection + 40, 0)      # u.proj.hash = 0
+    rc, err = xpuhip.session_check(bytes(blob))
+    assert rc == abi.ERRCODE_DEVICE_ERROR
+    assert b"projection hash" in err.message
+
+
+def ctypes_kds():
+    import ctypes
+    return ctypes.sizeof(abi.kern_data_store_head)
+
+
+def test_pinned_multirels_chunks():
+    """kern_multirels chunks naming pinned buffers: pinned_buffer set,
+    buffer_id the inner plan's query_plan_id, no KDS in the buffer."""
+    import ctypes
+    km = W.star_inner_pinned(ndims=3)
+    for d in range(3):
+        ch = abi.kern_multirels_chunk.from_buffer_copy(
+            bytes(km[abi.KMRELS_CHUNKS_OFFSET + d * ctypes.sizeof(abi.kern_multirels_chunk):
+                     abi.KMRELS_CHUNKS_OFFSET + (d + 1) * ctypes.sizeof(abi.kern_multirels_chunk)]))
+        assert ch.pinned_buffer and ch.kds_offset == 0
+        assert ch.buffer_id == W.PINNED_PLAN_ID + d
