@@ -723,15 +723,60 @@ load_arrow(octx *c, const kern_data_store *kds, int resno, uint32_t row, int typ
     }
     if (type == T_text || type == TypeOpCode__bytea)
     {
-        const uint32_t *off = (const uint32_t *)(base + cm->values_offset);
-        if (4 * ((uint64_t)row + 1) > cm->values_length ||
-            off[row] > off[row + 1] || off[row + 1] > cm->extra_length)
+        /* __xpu_varlena_datum_arrow_ref (xpu_textlib.cu:19-54): by the
+         * column's Arrow type; an offset out of range reads as NULL */
+        uint64_t o0, o1;
+        switch (cm->attopts.tag)
+        {
+            case ArrowType__Utf8:
+            case ArrowType__Binary:
+            {
+                const uint32_t *off = (const uint32_t *)(base + cm->values_offset);
+                if (4 * ((uint64_t)row + 2) > cm->values_length)
+                {
+                    r->isnull = true;
+                    return true;
+                }
+                o0 = off[row];
+                o1 = off[row + 1];
+                break;
+            }
+            case ArrowType__LargeUtf8:
+            case ArrowType__LargeBinary:
+            {
+                const uint64_t *off = (const uint64_t *)(base + cm->values_offset);
+                if (8 * ((uint64_t)row + 2) > cm->values_length)
+                {
+                    r->isnull = true;
+                    return true;
+                }
+                o0 = off[row];
+                o1 = off[row + 1];
+                break;
+            }
+            case ArrowType__FixedSizeBinary:
+            {
+                uint64_t w = cm->attopts.u.fixed_size_binary.byteWidth;
+                if (w * ((uint64_t)row + 1) > cm->values_length)
+                {
+                    r->isnull = true;
+                    return true;
+                }
+                r->p = base + cm->values_offset + w * row;
+                r->len = (int32_t)w;
+                return true;
+            }
+            default:
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "not a mappable Arrow data type");
+        }
+        /* KDS_ARROW_REF_VARLENA32/64_DATUM (xpu_common.h:1577-1626) */
+        if (o0 > o1 || o1 > cm->extra_length || o1 - o0 > 0x4ffffff8U)
         {
             r->isnull = true;
             return true;
         }
-        r->p = base + cm->extra_offset + off[row];
-        r->len = (int32_t)(off[row + 1] - off[row]);
+        r->p = base + cm->extra_offset + o0;
+        r->len = (int32_t)(o1 - o0);
         return true;
     }
     if (type == T_bool)

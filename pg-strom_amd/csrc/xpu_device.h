@@ -438,15 +438,16 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                         }
                     }
                 }
-                else
+                else if (cm->attopts.tag == ArrowType__Utf8 || cm->attopts.tag == ArrowType__Binary)
                 {
+                    /* KDS_ARROW_REF_VARLENA32_DATUM (xpu_common.h:1577-1601) */
                     if (4ULL * ((uint64_t)row + 2) > cm->values_length)
                         isnull = true;
                     if (!isnull)
                     {
                         const uint32_t *off = (const uint32_t *)(kbase + cm->values_offset);
                         uint32_t o0 = off[row], o1 = off[row + 1];
-                        if (o0 > o1 || o1 > cm->extra_length)
+                        if (o0 > o1 || o1 > cm->extra_length || o1 - o0 > XARROW_VARATT_MAX)
                             isnull = true;
                         else
                         {
@@ -454,6 +455,45 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                             len = o1 - o0;
                         }
                     }
+                }
+                else if (cm->attopts.tag == ArrowType__LargeUtf8 ||
+                         cm->attopts.tag == ArrowType__LargeBinary)
+                {
+                    /* KDS_ARROW_REF_VARLENA64_DATUM (xpu_common.h:1603-1626) */
+                    if (8ULL * ((uint64_t)row + 2) > cm->values_length)
+                        isnull = true;
+                    if (!isnull)
+                    {
+                        const uint64_t *off = (const uint64_t *)(kbase + cm->values_offset);
+                        uint64_t o0 = off[row], o1 = off[row + 1];
+                        if (o0 > o1 || o1 > cm->extra_length || o1 - o0 > XARROW_VARATT_MAX)
+                            isnull = true;
+                        else
+                        {
+                            ptr = (int64_t)(uintptr_t)(kbase + cm->extra_offset + o0);
+                            len = (int64_t)(o1 - o0);
+                        }
+                    }
+                }
+                else if (cm->attopts.tag == ArrowType__FixedSizeBinary)
+                {
+                    /* KDS_ARROW_REF_SIMPLE_DATUM with the byte width
+                     * (xpu_textlib.cu:30-35, xpu_common.h:1556-1573) */
+                    const uint64_t w = cm->attopts.u.fixed_size_binary.byteWidth;
+                    if (w * ((uint64_t)row + 1) > cm->values_length)
+                        isnull = true;
+                    if (!isnull)
+                    {
+                        ptr = (int64_t)(uintptr_t)(kbase + cm->values_offset + w * row);
+                        len = (int64_t)w;
+                    }
+                }
+                else
+                {
+                    /* "not a mappable Arrow data type" (xpu_textlib.cu:49-51) */
+                    if (!L.err)
+                        L.err = XMSG_ARROW_TYPE + 1;
+                    isnull = true;
                 }
                 L.R[d] = ptr;
                 L.R[d + 1] = len;

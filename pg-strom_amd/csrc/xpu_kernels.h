@@ -66,6 +66,9 @@ typedef struct
 #define XMSG_JOIN_RECORDS       0xfff5
 #define XMSG_HEAP_ITEM          0xfff6
 #define XMSG_HEAP_TUPLE         0xfff7
+#define XMSG_ARROW_TYPE         0xfff8
+/* the longest varlena an Arrow column may hold (VARATT_MAX, xpu_common.h:1575) */
+#define XARROW_VARATT_MAX       0x4ffffff8U
 
 /* ------------------------------------------------------------------
  * Heap-page (KDS_FORMAT_BLOCK) and row (KDS_FORMAT_ROW / HASH) sources are

@@ -491,6 +491,7 @@ message_of(const xpuhip_session *s, uint32_t msgid)
         case XMSG_JOIN_RECORDS: return "join records exceed their buffer";
         case XMSG_HEAP_ITEM: return "heap page line pointer or row-index entry out of its chunk";
         case XMSG_HEAP_TUPLE: return "heap tuple attribute beyond its item";
+        case XMSG_ARROW_TYPE: return "not a mappable Arrow data type";
     }
     if (msgid < s->comp.messages.size())
         return s->comp.messages[msgid].c_str();

@@ -116,9 +116,13 @@ class _ArrowFloat(C.Structure):
     _fields_ = [("precision", C.c_int32)]
 
 
+class _ArrowFixedBinary(C.Structure):
+    _fields_ = [("byteWidth", C.c_uint32)]
+
+
 class _ArrowOptsUnion(C.Union):
     _fields_ = [("integer", _ArrowInteger), ("floating_point", _ArrowFloat),
-                ("_pad", C.c_uint16 * 4)]
+                ("fixed_size_binary", _ArrowFixedBinary), ("_pad", C.c_uint16 * 4)]
 
 
 class ArrowTypeOptions(C.Structure):

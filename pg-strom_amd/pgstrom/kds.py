@@ -31,6 +31,11 @@ PG_TYPES = {
     "date": (1082, 4, True, 4, 13),
 }
 
+# Arrow type tags of the other varlena layouts (arrow_defs.h)
+ARROW_FIXED_SIZE_BINARY = 15
+ARROW_LARGE_BINARY = 19
+ARROW_LARGE_UTF8 = 20
+
 NP_DTYPES = {"bool": np.bool_, "int1": np.int8, "int2": np.int16, "int4": np.int32,
              "int8": np.int64, "float4": np.float32, "float8": np.float64,
              "date": np.int32}
@@ -137,9 +142,12 @@ def kds_header(fmt, coldefs, length=0, hash_nslots=0):
     return buf
 
 
-def build_arrow_kds(columns):
+def build_arrow_kds(columns, text_layout="utf8"):
     """Columnar KDS (KDS_FORMAT_ARROW) holding `columns` (list of Column).
-    Returns a bytes-like numpy uint8 array."""
+    text_layout sets the Arrow layout of text/bytea columns: "utf8" (Utf8 /
+    Binary, 32-bit offsets), "large" (LargeUtf8 / LargeBinary, 64-bit
+    offsets) or "fixed" (FixedSizeBinary; every non-NULL value of a column
+    has the same length).  Returns a bytes-like numpy uint8 array."""
     n = len(columns[0]) if columns else 0
     coldefs = [(c.name, c.typname) for c in columns]
     head = kds_header(abi.KDS_FORMAT_ARROW, coldefs)
@@ -163,9 +171,28 @@ def build_arrow_kds(columns):
                 np.cumsum(lens, out=offs[1:])
                 data = b"".join(v if not (c.nulls is not None and c.nulls[i]) else b""
                                 for i, v in enumerate(c.values))
-            assert offs[-1] < 2**31
-            ent["values"] = (pos, offs.astype(np.int32).tobytes())
-            pos = abi.typealign(64, pos + 4 * (n + 1))
+            if text_layout == "fixed":
+                lens = np.diff(offs)
+                live = lens if c.nulls is None else lens[~c.nulls]
+                w = int(live[0]) if len(live) else 1
+                if len(live) and (live != w).any():
+                    raise ValueError("FixedSizeBinary needs values of one length")
+                raw = bytearray(w * n)
+                for i in range(n):
+                    if offs[i + 1] > offs[i]:
+                        raw[w * i:w * (i + 1)] = data[offs[i]:offs[i + 1]]
+                ent["values"] = (pos, bytes(raw))
+                ent["fixed_width"] = w
+                pos = abi.typealign(64, pos + len(raw))
+                layout.append(ent)
+                continue
+            if text_layout == "large":
+                ent["values"] = (pos, offs.astype(np.int64).tobytes())
+                pos = abi.typealign(64, pos + 8 * (n + 1))
+            else:
+                assert offs[-1] < 2**31
+                ent["values"] = (pos, offs.astype(np.int32).tobytes())
+                pos = abi.typealign(64, pos + 4 * (n + 1))
             ent["extra"] = (pos, data)
             pos = abi.typealign(64, pos + len(data))
         elif c.typname == "bool":
@@ -198,6 +225,13 @@ def build_arrow_kds(columns):
             cm.attopts.unitsz = PG_TYPES[t][1]
             cm.attopts.u.floating_point.precision = (
                 abi.ArrowPrecision__Single if t == "float4" else abi.ArrowPrecision__Double)
+        elif t in ("text", "bytea") and "fixed_width" in ent:
+            cm.attopts.tag = ARROW_FIXED_SIZE_BINARY
+            cm.attopts.unitsz = ent["fixed_width"]
+            cm.attopts.u.fixed_size_binary.byteWidth = ent["fixed_width"]
+        elif t in ("text", "bytea") and text_layout == "large":
+            cm.attopts.tag = ARROW_LARGE_UTF8 if t == "text" else ARROW_LARGE_BINARY
+            cm.attopts.unitsz = 8
         elif t in ("text", "bytea"):
             cm.attopts.tag = abi.ArrowType__Utf8 if t == "text" else 4
             cm.attopts.unitsz = 4

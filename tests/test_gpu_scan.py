@@ -103,3 +103,42 @@ def test_scan_error_raised(gpu, jit):
         run_gpu_projection(blob, src, 4 << 20, tuning=dict(jit=jit))
     assert ei.value.code == abi.ERRCODE_DEVICE_ERROR
     assert b"int4pl: value out of range" in ei.value.errbuf.message
+
+
+@pytest.mark.parametrize("layout", ["utf8", "large", "fixed"])
+def test_arrow_text_layouts(gpu, jit, layout):
+    """Text columns as Utf8, LargeUtf8 (64-bit offsets) and FixedSizeBinary:
+    kds_dst bytes equal the oracle's (xpu_textlib.cu:19-54)."""
+    n = 50_000
+    r = datagen.rng(7, 2)
+    a = kds.Column("a", "int4", r.integers(0, 100, size=n, dtype=np.int32))
+    nulls = r.random(n) < 0.1
+    t = kds.Column("t", "text", datagen.random_text(r, n, 16, 16 if layout == "fixed" else 60,
+                                                    nulls=nulls), nulls)
+    cols = [a, t]
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    va, vt = plan.scan_var("a"), plan.scan_var("t")
+    plan.scan_quals = cg.Func("int4lt", va, cg.Const("int4", 60))
+    plan.set_projection([vt, va], [("t", "text"), ("a", "int4")])
+    blob = plan.build()
+    src = kds.build_arrow_kds(cols, text_layout=layout)
+    got = run_gpu_projection(blob, src, 16 << 20, tuning=dict(jit=jit))
+    exp, _ = run_oracle_projection(blob, src, 16 << 20)
+    assert np.array_equal(got[0], exp)
+
+
+def test_unmappable_arrow_type_is_an_error(gpu, jit):
+    from pgstrom import xpuhip
+    cols = [kds.Column("t", "text", [b"x", b"yy"] * 100)]
+    src = kds.build_arrow_kds(cols)
+    cm = abi.kern_colmeta.from_buffer(src, abi.KDS_COLMETA_OFFSET)
+    cm.attopts.tag = abi.ArrowType__Int
+    plan = cg.Plan([("t", "text")])
+    vt = plan.scan_var("t")
+    plan.set_projection([vt], [("t", "text")])
+    s = xpuhip.Session(plan.build())
+    s.set_tuning(jit=jit)
+    with pytest.raises(xpuhip.XpuError) as ei:
+        s.exec(src, dst_length=1 << 20)
+    s.close()
+    assert "not a mappable Arrow data type" in str(ei.value)

@@ -140,3 +140,47 @@ def test_window_headers_match_sliced_chunks():
         exp = W.decode_groups(o.final(), acts)
         o.close()
         assert got == exp
+
+
+@pytest.mark.parametrize("layout", ["large", "fixed"])
+def test_oracle_arrow_text_layouts(layout):
+    """Text columns as LargeUtf8 (64-bit offsets) and FixedSizeBinary read the
+    same values as Utf8 (xpu_textlib.cu:19-54): identical kds_dst bytes."""
+    from pgstrom import codegen as cg
+    n = 3000
+    r = datagen.rng(7, 1)
+    a = kds.Column("a", "int4", r.integers(0, 100, size=n, dtype=np.int32))
+    nulls = r.random(n) < 0.1
+    t = kds.Column("t", "text", datagen.random_text(r, n, 12, 12 if layout == "fixed" else 40,
+                                                    nulls=nulls), nulls)
+    cols = [a, t]
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    va, vt = plan.scan_var("a"), plan.scan_var("t")
+    plan.scan_quals = cg.Func("int4lt", va, cg.Const("int4", 60))
+    plan.set_projection([vt, va], [("t", "text"), ("a", "int4")])
+    blob = plan.build()
+    outs = []
+    for lay in ("utf8", layout):
+        o = oracle.OracleSession(blob)
+        rc, dst, _ = o.exec(kds.build_arrow_kds(cols, text_layout=lay), dst_length=4 << 20)
+        o.close()
+        assert rc == 0
+        outs.append(dst)
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_oracle_unmappable_arrow_type():
+    """A text column whose Arrow type is not a varlena layout is an error."""
+    from pgstrom import codegen as cg
+    import ctypes
+    cols = [kds.Column("t", "text", [b"x", b"yy"])]
+    src = kds.build_arrow_kds(cols)
+    cm = abi.kern_colmeta.from_buffer(src, abi.KDS_COLMETA_OFFSET)
+    cm.attopts.tag = abi.ArrowType__Int
+    plan = cg.Plan([("t", "text")])
+    vt = plan.scan_var("t")
+    plan.set_projection([vt], [("t", "text")])
+    o = oracle.OracleSession(plan.build())
+    rc, _, _ = o.exec(src, dst_length=1 << 20)
+    o.close()
+    assert rc == abi.ERRCODE_DEVICE_ERROR
