@@ -2885,6 +2885,7 @@ template <int NR, int RPT, class PROG>
 __device__ __forceinline__ void
 join_screen_body(cplan *P, const xpu_kparams &kp)
 {
+    extern __shared__ unsigned long long lds_dyn[];
     __shared__ uint32_t s_cnt;
     xpu_task_ctl *ctl = kp.ctl;
     const int tid = threadIdx.x;
@@ -2894,6 +2895,11 @@ join_screen_body(cplan *P, const xpu_kparams &kp)
     const uint32_t b = blockIdx.x;
     const bool bloom = kp.prefilter && kp.bloom[1] != nullptr && !P->depth[1].left_outer;
 
+    /* the folded depth-1 filter in LDS: a first probe that keeps most
+     * rejected rows off the full filter in L2 */
+    const uint32_t lwords = (bloom && kp.bloom_lds) ? kp.bloom_lds_words : 0;
+    for (uint32_t i = tid; i < lwords; i += blockDim.x)
+        lds_dyn[i] = kp.bloom_lds[i];
     if (tid == 0)
         s_cnt = 0;
     __syncthreads();
@@ -2926,7 +2932,11 @@ join_screen_body(cplan *P, const xpu_kparams &kp)
             {
                 const uint64_t x = bloom_mix(h);
                 bm[i] = bloom_bits(x);
-                bw[i] = kp.bloom[1][x & kp.bloom_mask[1]];
+                const bool lds_pass = lwords == 0 ||
+                    (lds_dyn[x & (lwords - 1)] & bm[i]) == bm[i];
+                if (lds_pass)
+                    bw[i] = (lwords - 1 == kp.bloom_mask[1]) ? bm[i]     /* the whole filter */
+                                                            : kp.bloom[1][x & kp.bloom_mask[1]];
             }
         };
 #pragma unroll

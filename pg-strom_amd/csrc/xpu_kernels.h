@@ -203,6 +203,11 @@ typedef struct
     /* join-hash Bloom filters of the inner relations (nullptr: none) */
     const uint64_t *bloom[XP_MAX_DEPTH + 1];
     uint32_t    bloom_mask[XP_MAX_DEPTH + 1];   /* words - 1 (words: a power of two) */
+    /* the depth-1 filter folded to bloom_lds_words words (a power of two),
+     * which the screen pass holds in LDS: a row it rejects is rejected by the
+     * full filter too (nullptr: none) */
+    const uint64_t *bloom_lds;
+    uint32_t    bloom_lds_words;
     uint32_t    prefilter;              /* group-by joins: screen rows before the joins */
     /* multi-pass joins of a group-by: the records a pass reads and writes,
      * in per-block regions of rec_*_cap records (column-major words) */
@@ -247,6 +252,11 @@ int xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block,
  * share a join hash */
 int xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words,
                            uint32_t mask, uint32_t *maxdup, void *stream);
+/* folds a filter of nwords words into fwords words (both powers of two):
+ * folded word i is the OR of words i, i + fwords, ... */
+int xpu_launch_bloom_fold(const uint64_t *words, uint64_t nwords, uint64_t *folded,
+                          uint64_t fwords, void *stream);
+#define XQ_SCREEN_LDS_WORDS (16384)     /* 128 KB of folded depth-1 filter per screen block */
 int xpu_launch_scan_agg(const xpu_kparams *kp, int nregs, int rpt, int grid, int block,
                         size_t lds_bytes, void *stream);
 int xpu_launch_fill(void *ptr, uint64_t value, size_t nwords, void *stream);

@@ -455,6 +455,29 @@ xpu_launch_bloom_build(kern_data_store *inner, uint64_t nslots, uint64_t *words,
     return (int)hipGetLastError();
 }
 
+__global__ void
+xpu_bloom_fold_kernel(const uint64_t *words, uint64_t nwords, uint64_t *folded, uint64_t fwords)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= fwords)
+        return;
+    uint64_t w = 0;
+    for (uint64_t j = i; j < nwords; j += fwords)
+        w |= words[j];
+    folded[i] = w;
+}
+
+extern "C" int
+xpu_launch_bloom_fold(const uint64_t *words, uint64_t nwords, uint64_t *folded, uint64_t fwords,
+                      void *stream)
+{
+    if (fwords == 0)
+        return 0;
+    hipLaunchKernelGGL(xpu_bloom_fold_kernel, dim3((unsigned)((fwords + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, words, nwords, folded, fwords);
+    return (int)hipGetLastError();
+}
+
 extern "C" int
 xpu_launch_finalize_dst(kern_data_store *kds_dst, const kern_data_store *head, uint32_t head_len,
                         uint64_t length, const xpu_task_ctl *ctl, void *stream)

@@ -205,6 +205,8 @@ struct xpuhip_session
     uint64_t        bloom_off[XP_MAX_DEPTH + 1] = {};   /* words from d_bloom */
     uint32_t        bloom_mask[XP_MAX_DEPTH + 1] = {};  /* words - 1 (0: no filter) */
     uint32_t        maxdup[XP_MAX_DEPTH + 1] = {};      /* most items sharing a join hash */
+    uint64_t        bloom_lds_off = 0;      /* the depth-1 filter folded for the screen's LDS */
+    uint32_t        bloom_lds_words = 0;
     int64_t        *d_params = nullptr;     /* XO_PARAM values */
     uint32_t       *d_xids = nullptr;       /* parallelCurrentXids (row-chunk visibility) */
     struct PinnedBuf *pinned = nullptr;     /* pinned hash results: the shared buffer */
@@ -857,7 +859,12 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
         /* the filter words, then one counter per depth for the most items
          * sharing a join hash */
         const uint64_t ndup = (P->nrels + 1 + 1) / 2;
-        if (!hip_ok(hipMalloc((void **)&s->d_bloom, 8 * (nwords + ndup)), eb, "hipMalloc(bloom)") ||
+        /* then the depth-1 filter folded to at most XQ_SCREEN_LDS_WORDS words */
+        const uint64_t w1 = (uint64_t)s->bloom_mask[1] + 1;
+        s->bloom_lds_words = (uint32_t)(w1 < XQ_SCREEN_LDS_WORDS ? w1 : XQ_SCREEN_LDS_WORDS);
+        s->bloom_lds_off = nwords + ndup;
+        if (!hip_ok(hipMalloc((void **)&s->d_bloom, 8 * (nwords + ndup + s->bloom_lds_words)), eb,
+                    "hipMalloc(bloom)") ||
             !hip_ok((hipError_t)xpu_launch_fill(s->d_bloom, 0, nwords + ndup, nullptr), eb, "bloom fill"))
             return ERRCODE_DEVICE_ERROR;
         uint32_t *d_dup = (uint32_t *)(s->d_bloom + nwords);
@@ -869,6 +876,11 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
                         eb, "bloom build"))
                 return ERRCODE_DEVICE_ERROR;
         }
+        if (!hip_ok((hipError_t)xpu_launch_bloom_fold(s->d_bloom + s->bloom_off[1],
+                                                      (uint64_t)s->bloom_mask[1] + 1,
+                                                      s->d_bloom + s->bloom_lds_off,
+                                                      s->bloom_lds_words, nullptr), eb, "bloom fold"))
+            return ERRCODE_DEVICE_ERROR;
         uint32_t dup[XP_MAX_DEPTH + 2] = {};
         if (!hip_ok(hipMemcpy(dup, d_dup, 4 * (P->nrels + 1), hipMemcpyDeviceToHost), eb,
                     "bloom build"))
@@ -1190,6 +1202,11 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
              !grow((char **)&x->d_rec_cnt, &x->rec_cnt_cap, 8ULL * grid, eb)))
             return ERRCODE_DEVICE_ERROR;
         kp.screened = (kp.prefilter && kp.bloom[1] && !P->depth[1].left_outer) ? 1 : 0;
+        if (kp.screened && s->bloom_lds_words > 0)
+        {
+            kp.bloom_lds = s->d_bloom + s->bloom_lds_off;
+            kp.bloom_lds_words = s->bloom_lds_words;
+        }
     }
 
     int rc = 0;
@@ -1249,8 +1266,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
             q.rec_out = x->d_rec[0];
             q.rec_out_cnt = x->d_rec_cnt;
             q.rec_out_cap = rcap[1];
-            rc = (int)hipModuleLaunchKernel(jf->screen, grid, 1, 1, XQ_SCREEN_BLOCK, 1, 1, 0,
-                                            x->stream, args, nullptr);
+            rc = (int)hipModuleLaunchKernel(jf->screen, grid, 1, 1, XQ_SCREEN_BLOCK, 1, 1,
+                                            8U * kp.bloom_lds_words, x->stream, args, nullptr);
             for (uint32_t d = 1; rc == 0 && d <= P->nrels; d++)
             {
                 const bool last = (d == P->nrels);
