@@ -407,6 +407,46 @@ typedef struct
     int32_t     typmod;
 } kern_aggregate_desc;
 
+/* ---------------------------------------------------------------
+ * GPU-Sort key descriptors of FuncOpCode__SortKeys (xpu_common.h:2474-2520)
+ * --------------------------------------------------------------- */
+#define KSORT_KEY_ATTR__NULLS_FIRST         0x0400U
+#define KSORT_KEY_ATTR__ORDER_ASC           0x8000U
+#define KSORT_KEY_KIND__MASK                0x03ffU
+#define KSORT_KEY_KIND__VREF                0
+#define KSORT_KEY_KIND__PMINMAX_INT64       1
+#define KSORT_KEY_KIND__PMINMAX_FP64        2
+#define KSORT_KEY_KIND__PSUM_INT64          3
+#define KSORT_KEY_KIND__PSUM_FP64           4
+#define KSORT_KEY_KIND__PSUM_NUMERIC        5
+#define KSORT_KEY_KIND__PAVG_INT64          6
+#define KSORT_KEY_KIND__PAVG_FP64           7
+#define KSORT_KEY_KIND__PAVG_NUMERIC        8
+#define KSORT_KEY_KIND__PVARIANCE_SAMP      9
+#define KSORT_KEY_KIND__NITEMS              23
+
+#define KSORT_WINDOW_FUNC__ROW_NUMBER       'n'
+#define KSORT_WINDOW_FUNC__RANK             'r'
+#define KSORT_WINDOW_FUNC__DENSE_RANK       'd'
+
+typedef struct
+{
+    uint16_t    kind;           /* KSORT_KEY_KIND__* */
+    int8_t      nulls_first;
+    int8_t      order_asc;
+    uint16_t    src_anum;       /* attribute number in the kds_final tuple */
+    uint16_t    buf_offset;     /* finalized-value offset past the tuple (unused here) */
+    TypeOpCode  key_type_code;
+    const void *key_ops;        /* resolved by the xPU service; ignored here */
+} kern_sortkey_desc;
+
+/* special values of a numeric sum state's attrs (xpu_common.h:2430-2434) */
+#define __PAGG_NUMERIC_ATTRS__WEIGHT    0x00ffffU
+#define __PAGG_NUMERIC_ATTRS__NAN       0x010000U
+#define __PAGG_NUMERIC_ATTRS__PINF      0x020000U
+#define __PAGG_NUMERIC_ATTRS__NINF      0x040000U
+#define __PAGG_NUMERIC_ATTRS__MASK      0x070000U
+
 typedef struct kern_expression
 {
     uint32_t    len;
@@ -465,6 +505,15 @@ typedef struct kern_expression
             int32_t     nattrs;
             uint16_t    slot_id[1];
         } proj;
+        struct {
+            int32_t     nkeys;
+            bool        needs_finalization;
+            char        window_rank_func;       /* KSORT_WINDOW_FUNC__* or 0 */
+            uint32_t    window_rank_limit;
+            uint16_t    window_partby_nkeys;
+            uint16_t    window_orderby_nkeys;
+            kern_sortkey_desc desc[1];
+        } sort;
         struct {
             uint32_t    npacked;
             uint32_t    offset[1];

@@ -138,7 +138,13 @@ int     xpuhip_task_exec_device_tuples(xpuhip_session *sess,
 /* XpuTaskFinal of a group-by session: writes the session's kds_final into
  * kds_final (host memory) as the service writes it back, HASH converted to
  * ROW (gpu_service.c:2780-2818).  With kds_final == NULL returns the number of
- * bytes needed.  Returns the bytes written, or -1 with errbuf filled. */
+ * bytes needed.  Returns the bytes written, or -1 with errbuf filled.
+ * A session with FuncOpCode__SortKeys (xpucode_gpusort_keydesc) gets its
+ * row index sorted by those keys on the device first, and with a positive
+ * gpusort_limit_count only that many leading rows (gpuservSortingFinalBuffer,
+ * gpu_service.c:5199-5345); the tuples of the rows past the limit stay in
+ * the written buffer's tail, unreferenced.  The device buffer itself is not
+ * reordered. */
 int64_t xpuhip_task_final(xpuhip_session *sess, kern_data_store *kds_final, size_t length,
                           kern_exec_results *results, kern_errorbuf *errbuf);
 
@@ -275,6 +281,10 @@ typedef struct
     double      heap_ms;            /* heap-page / row chunk: its count, scan and
                                        deform passes (0 for Arrow chunks) */
     uint64_t    heap_rows;          /* tuples those passes loaded */
+    double      sort_ms;            /* GPU-Sort of the last xpuhip_task_final:
+                                       key, radix and permute passes */
+    uint32_t    sort_items;         /* items it sorted (0: no sort) */
+    uint32_t    sort_passes;        /* radix passes run (constant digits skipped) */
 } xpuhip_task_stats;
 
 int     xpuhip_last_task_stats(xpuhip_task_stats *stats);

@@ -2144,6 +2144,206 @@ write_group_tuple(const oracle_session *os, const ogroup *g, char *buf)
     return t_hoff;
 }
 
+/* ------------------------------------------------------------------
+ * GPU-Sort of the final buffer: the comparator of cuda_gpusort.cu:15-343
+ * over the row index, then the LIMIT of gpu_service.c:5099-5197 (the rows
+ * past the limit leave the row index).  qsort is not stable; neither is the
+ * reference's bitonic sort, so rows whose keys compare equal may come out
+ * in either order.
+ * ------------------------------------------------------------------ */
+/* IEEE binary16 -> float, from its fields (exact: every half is a float) */
+static float
+oracle_half_to_float(uint16_t h)
+{
+    const int sgn = h >> 15, e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    float v;
+    if (e == 0)
+        v = ldexpf((float)m, -24);                  /* zero, subnormal */
+    else if (e == 31)
+        v = m ? NAN : INFINITY;
+    else
+        v = ldexpf((float)(1024 + m), e - 25);
+    return sgn ? -v : v;
+}
+
+static const kern_data_store *sort_kds;
+static const kern_expression *sort_kexp;
+
+/* kern_fetch_minimal_tuple_attr: attribute anum of a kds_final tuple */
+static const char *
+sort_fetch_attr(const kern_data_store *kds, const kern_tupitem *t, int anum)
+{
+    const int natts = t->t_infomask2 & HEAP_NATTS_MASK;
+    const bool hasnull = (t->t_infomask & HEAP_HASNULL) != 0;
+    const uint8_t *bits = (const uint8_t *)t + KERN_TUPITEM_T_BITS_OFFSET;
+    const char *base = (const char *)t;
+    uint32_t off = (uint32_t)t->t_hoff - MINIMAL_TUPLE_OFFSET;
+
+    if (anum < 1 || anum > natts)
+        return NULL;
+    for (int j = 0; j < anum; j++)
+    {
+        const kern_colmeta *cm = &kds->colmeta[j];
+        if (hasnull && !(bits[j >> 3] & (1 << (j & 7))))
+        {
+            if (j == anum - 1)
+                return NULL;
+            continue;
+        }
+        if (cm->attlen > 0 || base[off] == 0)
+            off = XPU_TYPEALIGN(cm->attalign > 0 ? cm->attalign : 1, off);
+        if (j == anum - 1)
+            return base + off;
+        off += cm->attlen > 0 ? (uint32_t)cm->attlen : varsize_any((const uint8_t *)base + off);
+    }
+    return NULL;
+}
+
+/* x NULL / y NULL cases shared by every key kind */
+static int
+sort_comp_nulls(const kern_sortkey_desc *d, bool x_ok, bool y_ok)
+{
+    if (x_ok && !y_ok)
+        return d->nulls_first ? 1 : -1;
+    if (!x_ok && y_ok)
+        return d->nulls_first ? -1 : 1;
+    return 0;
+}
+
+#define SORT_CMP(a, b)  ((a) < (b) ? -1 : (a) > (b) ? 1 : 0)
+
+static int64_t
+sort_i64(const char *p) { int64_t v; memcpy(&v, p, 8); return v; }
+static double
+sort_f64(const char *p) { double v; memcpy(&v, p, 8); return v; }
+
+/* a numeric sum state as (kind, value): kind 0 -inf, 1 finite, 2 +inf, 3 NaN
+ * (NaN above every value, as numeric's comparison) */
+static int
+sort_comp_numeric(const char *x, const char *y)
+{
+    uint32_t ax, ay;
+    memcpy(&ax, x + 4, 4);
+    memcpy(&ay, y + 4, 4);
+    int kx = (ax & __PAGG_NUMERIC_ATTRS__NAN) ? 3 : (ax & __PAGG_NUMERIC_ATTRS__PINF) ? 2
+           : (ax & __PAGG_NUMERIC_ATTRS__NINF) ? 0 : 1;
+    int ky = (ay & __PAGG_NUMERIC_ATTRS__NAN) ? 3 : (ay & __PAGG_NUMERIC_ATTRS__PINF) ? 2
+           : (ay & __PAGG_NUMERIC_ATTRS__NINF) ? 0 : 1;
+    if (kx != ky || kx != 1)
+        return SORT_CMP(kx, ky);
+    __int128 vx, vy;
+    memcpy(&vx, x + 16, 16);
+    memcpy(&vy, y + 16, 16);
+    int wx = (int16_t)(ax & __PAGG_NUMERIC_ATTRS__WEIGHT);
+    int wy = (int16_t)(ay & __PAGG_NUMERIC_ATTRS__WEIGHT);
+    for (; wx < wy; wx++) vx *= 10;     /* same number of decimal digits */
+    for (; wy < wx; wy++) vy *= 10;
+    return SORT_CMP(vx, vy);
+}
+
+/* __gpusort_comp_keys (cuda_gpusort.cu:273-342) */
+static int
+sort_comp_rows(const void *a, const void *b)
+{
+    const uint64_t ra = *(const uint64_t *)a, rb = *(const uint64_t *)b;
+    const char *tail = (const char *)sort_kds + sort_kds->length;
+    const kern_tupitem *tx = (const kern_tupitem *)(tail - ra);
+    const kern_tupitem *ty = (const kern_tupitem *)(tail - rb);
+
+    for (int k = 0; k < sort_kexp->u.sort.nkeys; k++)
+    {
+        const kern_sortkey_desc *d = &sort_kexp->u.sort.desc[k];
+        const char *x = sort_fetch_attr(sort_kds, tx, d->src_anum);
+        const char *y = sort_fetch_attr(sort_kds, ty, d->src_anum);
+        bool x_ok = x != NULL, y_ok = y != NULL;
+        int comp = 0;
+        switch (d->kind)
+        {
+            case KSORT_KEY_KIND__PMINMAX_INT64:
+            case KSORT_KEY_KIND__PMINMAX_FP64:
+            {
+                uint32_t fx = 0, fy = 0;
+                if (x) memcpy(&fx, x + 4, 4);
+                if (y) memcpy(&fy, y + 4, 4);
+                x_ok = x_ok && (fx & __PAGG_MINMAX_ATTRS__VALID);
+                y_ok = y_ok && (fy & __PAGG_MINMAX_ATTRS__VALID);
+                break;
+            }
+            case KSORT_KEY_KIND__PSUM_INT64:
+            case KSORT_KEY_KIND__PSUM_FP64:
+            case KSORT_KEY_KIND__PSUM_NUMERIC:
+            case KSORT_KEY_KIND__PAVG_INT64:
+            case KSORT_KEY_KIND__PAVG_FP64:
+                x_ok = x_ok && sort_i64(x + 8) != 0;
+                y_ok = y_ok && sort_i64(y + 8) != 0;
+                break;
+            default:
+                break;
+        }
+        if (!x_ok || !y_ok)
+            comp = sort_comp_nulls(d, x_ok, y_ok);
+        else
+        {
+            switch (d->kind)
+            {
+                case KSORT_KEY_KIND__VREF:
+                    switch (d->key_type_code)
+                    {
+                        case TypeOpCode__bool: comp = SORT_CMP(*(const uint8_t *)x != 0, *(const uint8_t *)y != 0); break;
+                        case TypeOpCode__int1: comp = SORT_CMP(*(const int8_t *)x, *(const int8_t *)y); break;
+                        case TypeOpCode__int2: comp = SORT_CMP(*(const int16_t *)x, *(const int16_t *)y); break;
+                        case TypeOpCode__int4:
+                        case TypeOpCode__date: comp = SORT_CMP(*(const int32_t *)x, *(const int32_t *)y); break;
+                        case TypeOpCode__int8: comp = SORT_CMP(sort_i64(x), sort_i64(y)); break;
+                        case TypeOpCode__float4: comp = SORT_CMP(*(const float *)x, *(const float *)y); break;
+                        case TypeOpCode__float8: comp = SORT_CMP(sort_f64(x), sort_f64(y)); break;
+                        case TypeOpCode__float2:
+                        {
+                            /* half -> float, exact */
+                            uint16_t hx = *(const uint16_t *)x, hy = *(const uint16_t *)y;
+                            comp = SORT_CMP(oracle_half_to_float(hx), oracle_half_to_float(hy));
+                            break;
+                        }
+                        default: comp = 0; break;
+                    }
+                    break;
+                case KSORT_KEY_KIND__PMINMAX_INT64:
+                    comp = SORT_CMP(sort_i64(x + 8), sort_i64(y + 8));
+                    break;
+                case KSORT_KEY_KIND__PMINMAX_FP64:
+                    comp = SORT_CMP(sort_f64(x + 8), sort_f64(y + 8));
+                    break;
+                case KSORT_KEY_KIND__PSUM_INT64:
+                    comp = SORT_CMP(sort_i64(x + 16), sort_i64(y + 16));
+                    break;
+                case KSORT_KEY_KIND__PSUM_FP64:
+                    comp = SORT_CMP(sort_f64(x + 16), sort_f64(y + 16));
+                    break;
+                case KSORT_KEY_KIND__PSUM_NUMERIC:
+                    comp = sort_comp_numeric(x, y);
+                    break;
+                case KSORT_KEY_KIND__PAVG_INT64:
+                    /* __gpusort_prep_pavg_int64 (:401-429) */
+                    comp = SORT_CMP((double)sort_i64(x + 16) / (double)sort_i64(x + 8),
+                                    (double)sort_i64(y + 16) / (double)sort_i64(y + 8));
+                    break;
+                case KSORT_KEY_KIND__PAVG_FP64:
+                    comp = SORT_CMP(sort_f64(x + 16) / (double)sort_i64(x + 8),
+                                    sort_f64(y + 16) / (double)sort_i64(y + 8));
+                    break;
+                default:
+                    comp = 0;
+                    break;
+            }
+            if (!d->order_asc)
+                comp = -comp;
+        }
+        if (comp != 0)
+            return comp;
+    }
+    return 0;
+}
+
 int64_t
 oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_length,
                   kern_errorbuf *eb)
@@ -2188,5 +2388,17 @@ oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_l
     }
     kds_final->nitems = (uint32_t)os->gt.ngroups;
     kds_final->usage = usage;
+    if (os->session->xpucode_gpusort_keydesc)
+    {
+        /* gpuservSortingFinalBuffer (gpu_service.c:5199-5345) */
+        sort_kds = kds_final;
+        sort_kexp = (const kern_expression *)((const char *)os->session +
+                                              os->session->xpucode_gpusort_keydesc);
+        if (kds_final->nitems > 1)
+            qsort(rowidx, kds_final->nitems, sizeof(uint64_t), sort_comp_rows);
+        if (os->session->gpusort_limit_count > 0 &&
+            (uint64_t)os->session->gpusort_limit_count < kds_final->nitems)
+            kds_final->nitems = (uint32_t)os->session->gpusort_limit_count;
+    }
     return (int64_t)need;
 }

@@ -833,8 +833,6 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     if (length < offsetof(kern_session_info, poffset) ||
         length < offsetof(kern_session_info, poffset) + 4ULL * s->nparams)
         FAIL("session blob is too short");
-    if (s->gpusort_htup_margin != 0 || s->xpucode_gpusort_keydesc != 0)
-        FAIL("GPU-Sort is not supported by this build");
     if (s->xpucode_gist_evals_packed != 0)
         FAIL("GiST joins are not supported by this build");
     if (s->select_into_kds_head != 0)
@@ -1260,6 +1258,80 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             FAIL("a source column is loaded with two types");
         sc.kind = kind;
         sc.width = width;
+    }
+    /* GPU-Sort of the group-by results (codegen.c:4360-4525): each key is
+     * a column of the final tuple or a partial aggregate state; finalized
+     * values (averages) are computed while the keys are encoded, so the
+     * tuple margin (gpusort_htup_margin) is not used */
+    if (s->xpucode_gpusort_keydesc)
+    {
+        const kern_expression *ks = C.kexp_at(s->xpucode_gpusort_keydesc);
+        if (!ks || !C.check_node(nullptr, ks) || ks->opcode != FuncOpCode__SortKeys)
+            FAIL("malformed SortKeys");
+        if (!out->has_groupby)
+            FAIL("GPU-Sort of scan or join results is not supported by this build");
+        if (ks->u.sort.window_rank_func != 0)
+            FAIL("GPU-Sort window rank functions are not supported by this build");
+        const int nk = ks->u.sort.nkeys;
+        if (nk < 1 || nk > XSORT_MAX_KEYS)
+            FAIL("GPU-Sort key count out of range");
+        if (offsetof(kern_expression, u.sort.desc) + (size_t)nk * sizeof(kern_sortkey_desc) > ks->len)
+            FAIL("SortKeys descriptors out of the expression");
+        const kern_data_store *kf = (const kern_data_store *)out->kds_final_head.data();
+        uint32_t nw = 0;
+        for (int k = 0; k < nk; k++)
+        {
+            const kern_sortkey_desc &d = ks->u.sort.desc[k];
+            if (d.src_anum < 1 || d.src_anum > kf->ncols || d.src_anum > kf->nr_colmeta)
+                FAIL("GPU-Sort key attribute out of range");
+            const kern_colmeta &cm = kf->colmeta[d.src_anum - 1];
+            xsort_key sk;
+            memset(&sk, 0, sizeof(sk));
+            sk.kind = d.kind;
+            sk.nulls_first = d.nulls_first ? 1 : 0;
+            sk.order_asc = d.order_asc ? 1 : 0;
+            sk.anum = d.src_anum;
+            sk.word = (uint8_t)nw;
+            sk.type_code = d.key_type_code;
+            switch (d.kind)
+            {
+                case KSORT_KEY_KIND__VREF:
+                {
+                    int width = 0;
+                    switch (d.key_type_code)
+                    {
+                        case TypeOpCode__bool: case TypeOpCode__int1: width = 1; break;
+                        case TypeOpCode__int2: case TypeOpCode__float2: width = 2; break;
+                        case TypeOpCode__int4: case TypeOpCode__float4:
+                        case TypeOpCode__date: width = 4; break;
+                        case TypeOpCode__int8: case TypeOpCode__float8: width = 8; break;
+                        default:
+                            FAIL("GPU-Sort key type is not supported by this build");
+                    }
+                    if (cm.attlen != width)
+                        FAIL("GPU-Sort key type does not match the final buffer column");
+                    sk.nwords = width == 8 ? 2 : 1;
+                    break;
+                }
+                case KSORT_KEY_KIND__PMINMAX_INT64:
+                case KSORT_KEY_KIND__PMINMAX_FP64:
+                case KSORT_KEY_KIND__PSUM_INT64:
+                case KSORT_KEY_KIND__PSUM_FP64:
+                case KSORT_KEY_KIND__PAVG_INT64:
+                case KSORT_KEY_KIND__PAVG_FP64:
+                case KSORT_KEY_KIND__PSUM_NUMERIC:
+                    if (cm.attlen != -1)
+                        FAIL("GPU-Sort aggregate key is not a partial state column");
+                    sk.nwords = d.kind == KSORT_KEY_KIND__PSUM_NUMERIC ? 3 : 2;
+                    break;
+                default:
+                    FAIL("GPU-Sort key kind is not supported by this build");
+            }
+            nw += sk.nwords;
+            out->sort_keys.push_back(sk);
+        }
+        out->sort_nwords = nw;
+        out->sort_limit = s->gpusort_limit_count > 0 ? s->gpusort_limit_count : 0;
     }
     /* parallelCurrentXids, for the visibility of KDS_FORMAT_ROW sources */
     if (s->session_xact_state)

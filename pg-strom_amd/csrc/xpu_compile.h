@@ -8,6 +8,7 @@
 
 #include "pgstrom_xpu.h"
 #include "xpu_program.h"
+#include "xpu_kernels.h"
 
 /* a kds_src column the programs load: XH_FIXED (width bytes), XH_BOOL or
  * XH_VAR (xpu_kernels.h); kind 0: not loaded */
@@ -35,6 +36,10 @@ struct xpu_compiled
     std::vector<xpu_src_col> src_cols;          /* by kds_src column index */
     bool                has_xact_state = false;
     std::vector<uint32_t> xids;                 /* parallelCurrentXids */
+    /* GPU-Sort of the final buffer (FuncOpCode__SortKeys) */
+    std::vector<xsort_key> sort_keys;
+    uint32_t            sort_nwords = 0;
+    int64_t             sort_limit = 0;         /* gpusort_limit_count, if positive */
 };
 
 /* Validates and translates the session blob; returns 0 or an ERRCODE_*

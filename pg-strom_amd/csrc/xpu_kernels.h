@@ -267,6 +267,49 @@ int xpu_launch_hash_append(kern_data_store *hk, const kern_data_store *rk, uint6
                            uint32_t nrows, void *stream);
 int xpu_launch_hash_rechain(kern_data_store *hk, uint32_t nitems, void *stream);
 /* heap / row sources: count pass, unit scan (total into st->total), deform pass */
+/* ---- GPU-Sort of a final buffer (xpu_sort.hip) ----
+ * Each sort key becomes one to three 64-bit words whose unsigned order is
+ * the key's order (NULL placement, ASC/DESC folded in); items are sorted by
+ * their word tuples with a stable LSD radix sort of 8-bit digits. */
+#define XSORT_MAX_KEYS      8
+#define XSORT_MAX_WORDS     (3 * XSORT_MAX_KEYS)
+#define XSORT_BLOCK         256
+#define XSORT_ROUNDS        16                      /* items per thread of a tile */
+#define XSORT_TILE          (XSORT_BLOCK * XSORT_ROUNDS)
+
+typedef struct
+{
+    uint16_t    kind;           /* KSORT_KEY_KIND__* */
+    uint8_t     nulls_first;
+    uint8_t     order_asc;
+    uint16_t    anum;           /* 1-based attribute of the kds_final tuple */
+    uint8_t     word;           /* first key word */
+    uint8_t     nwords;         /* 1 (<= 32-bit value), 2 (64-bit), 3 (128-bit) */
+    int32_t     type_code;      /* TypeOpCode of a VREF key */
+} xsort_key;
+
+typedef struct
+{
+    const kern_data_store *kds;     /* the final buffer (ROW or HASH) */
+    const uint64_t *rowindex;       /* its row index */
+    uint32_t    n;                  /* items */
+    uint32_t    nkeys;
+    uint32_t    nwords;
+    uint32_t    __pad;
+    xsort_key   keys[XSORT_MAX_KEYS];
+    uint64_t   *words;              /* [nwords][n] */
+    uint32_t   *idx;                /* [n] item numbers */
+    uint64_t   *wor;                /* [nwords] OR over the items */
+    uint64_t   *wand;               /* [nwords] AND over the items */
+} xsort_params;
+
+int xpu_launch_sort_keys(const xsort_params *sp, void *stream);
+int xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
+                         uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
+                         uint32_t shift, uint32_t *counts, void *stream);
+int xpu_launch_sort_permute(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out,
+                            uint32_t n, void *stream);
+
 int xpu_launch_heap_count(const xheap_params *hp, void *stream);
 int xpu_launch_heap_deform(const xheap_params *hp, int ncu, void *stream);
 int xpu_launch_copy(void *dst, const void *src, size_t nbytes, void *stream);

@@ -1,0 +1,446 @@
+/*
+ * xpu_sort.hip - GPU-Sort of a final buffer (the group-by results) by the
+ * session's FuncOpCode__SortKeys descriptors, ahead of the write-back.
+ *
+ * The reference sorts the final buffer's row index with a bitonic network,
+ * one launch per (scale, step) pair, each comparison re-reading both tuples
+ * through kern_fetch_minimal_tuple_attr and the key type's comparator
+ * (cuda_gpusort.cu:15-399, gpu_service.c:5199-5345).  Here every tuple is
+ * read once:
+ *   key pass      one lane per item walks its tuple to each key attribute
+ *                 and writes the key as 64-bit words whose unsigned order is
+ *                 the comparator's order (NULLs first or last, ASC or DESC,
+ *                 and an empty aggregate state sorting as NULL), plus the
+ *                 OR and AND of every word over all items
+ *   radix passes  stable LSD radix sort of (words, item number) by 8-bit
+ *                 digits, last word first; a digit that is the same in every
+ *                 item (OR and AND agree there) is skipped, so an int4 key
+ *                 sorts in 4 passes and a column without NULLs pays no pass
+ *                 for its NULL class.  Each pass is a digit histogram per
+ *                 tile of 4,096 items, one exclusive scan over (digit, tile),
+ *                 and a scatter that ranks equal digits with wave ballots
+ *   permute       the sorted item numbers pick the row-index entries
+ *
+ * Key words (each key in turn, most significant first):
+ *   32-bit and narrower values   one word: class << 32 | value bits
+ *   64-bit values                class, then the 64-bit value bits
+ *   numeric sums (int128)        class (with -inf / finite / +inf / NaN),
+ *                                then the high and low halves
+ * class is 1 for a value, 0 or 2 for NULL (NULLS FIRST / LAST).  Integers
+ * have their sign bit flipped; floats have the sign bit set when positive
+ * and every bit inverted when negative, -0 read as +0 (the reference's
+ * comparators use < and >, which hold -0 and +0 equal) and NaN above every
+ * value.  DESC inverts the value bits (and reverses the numeric classes),
+ * never the NULL class.
+ */
+#include <hip/hip_runtime.h>
+#include "pgstrom_xpu.h"
+#include "xpu_kernels.h"
+
+/* the attribute anum (1-based) of a minimal tuple, or nullptr when it is
+ * NULL (kern_fetch_minimal_tuple_attr, xpu_common.cu) */
+__device__ static const char *
+sort_tuple_attr(const kern_data_store *kds, const kern_tupitem *t, int anum)
+{
+    const int natts = t->t_infomask2 & HEAP_NATTS_MASK;
+    if (anum < 1 || anum > natts)
+        return nullptr;
+    const bool hasnull = (t->t_infomask & HEAP_HASNULL) != 0;
+    const uint8_t *bits = (const uint8_t *)t + KERN_TUPITEM_T_BITS_OFFSET;
+    const char *base = (const char *)t;
+    uint32_t off = (uint32_t)t->t_hoff - MINIMAL_TUPLE_OFFSET;
+    for (int j = 0; j < anum; j++)
+    {
+        if (hasnull && !(bits[j >> 3] & (1 << (j & 7))))
+        {
+            if (j == anum - 1)
+                return nullptr;
+            continue;
+        }
+        const kern_colmeta *cm = &kds->colmeta[j];
+        const uint32_t al = cm->attalign > 0 ? (uint32_t)cm->attalign : 1U;
+        if (cm->attlen > 0 || base[off] == 0)
+            off = (off + al - 1) & ~(al - 1);
+        if (j == anum - 1)
+            return base + off;
+        if (cm->attlen > 0)
+            off += cm->attlen;
+        else
+        {
+            const uint8_t b = (uint8_t)base[off];
+            if (b & 0x01)
+                off += (b >> 1) & 0x7f;             /* 1-byte header */
+            else
+            {
+                uint32_t h;
+                __builtin_memcpy(&h, base + off, 4);
+                off += (h >> 2) & 0x3fffffffU;      /* 4-byte header */
+            }
+        }
+    }
+    return nullptr;
+}
+
+__device__ static inline uint64_t
+sort_load64(const char *p)
+{
+    uint64_t v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+
+__device__ static inline uint32_t
+sort_enc_f16(uint32_t b)
+{
+    b &= 0xffffU;
+    if ((b & 0x7fffU) > 0x7c00U)
+        return 0xffffU;                             /* NaN */
+    if ((b & 0x7fffU) == 0)
+        b = 0;
+    return (b & 0x8000U) ? (~b & 0xffffU) : (b | 0x8000U);
+}
+
+__device__ static inline uint32_t
+sort_enc_f32(uint32_t b)
+{
+    if ((b & 0x7fffffffU) > 0x7f800000U)
+        return 0xffffffffU;
+    if ((b & 0x7fffffffU) == 0)
+        b = 0;
+    return (b & 0x80000000U) ? ~b : (b | 0x80000000U);
+}
+
+__device__ static inline uint64_t
+sort_enc_f64(uint64_t b)
+{
+    if ((b & 0x7fffffffffffffffULL) > 0x7ff0000000000000ULL)
+        return ~0ULL;
+    if ((b & 0x7fffffffffffffffULL) == 0)
+        b = 0;
+    return (b & 0x8000000000000000ULL) ? ~b : (b | 0x8000000000000000ULL);
+}
+
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_keys_kernel(xsort_params sp)
+{
+    __shared__ unsigned long long s_or[XSORT_MAX_WORDS];
+    __shared__ unsigned long long s_and[XSORT_MAX_WORDS];
+    const uint32_t nw = sp.nwords;
+    if (threadIdx.x < nw)
+    {
+        s_or[threadIdx.x] = 0;
+        s_and[threadIdx.x] = ~0ULL;
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * XSORT_BLOCK + threadIdx.x;
+    const bool valid = i < sp.n;
+    uint64_t w[XSORT_MAX_WORDS];
+#pragma unroll
+    for (int k = 0; k < XSORT_MAX_WORDS; k++)
+        w[k] = 0;
+    if (valid)
+    {
+        const kern_tupitem *t = (const kern_tupitem *)
+            ((const char *)sp.kds + sp.kds->length - sp.rowindex[i]);
+        for (uint32_t k = 0; k < sp.nkeys; k++)
+        {
+            const xsort_key K = sp.keys[k];
+            const char *p = sort_tuple_attr(sp.kds, t, K.anum);
+            const uint64_t null_class = K.nulls_first ? 0 : 2;
+            uint64_t cls = 1, v = 0, v2 = 0;
+            switch (K.kind)
+            {
+                case KSORT_KEY_KIND__VREF:
+                    if (!p)
+                        break;
+                    switch (K.type_code)
+                    {
+                        case TypeOpCode__bool: v = *(const uint8_t *)p ? 1 : 0; break;
+                        case TypeOpCode__int1: v = (uint32_t)(int32_t)*(const int8_t *)p ^ 0x80000000U; break;
+                        case TypeOpCode__int2: v = (uint32_t)(int32_t)*(const int16_t *)p ^ 0x80000000U; break;
+                        case TypeOpCode__int4:
+                        case TypeOpCode__date: v = (uint32_t)*(const int32_t *)p ^ 0x80000000U; break;
+                        case TypeOpCode__float2: v = sort_enc_f16(*(const uint16_t *)p); break;
+                        case TypeOpCode__float4: v = sort_enc_f32(*(const uint32_t *)p); break;
+                        case TypeOpCode__int8: v = sort_load64(p) ^ 0x8000000000000000ULL; break;
+                        case TypeOpCode__float8: v = sort_enc_f64(sort_load64(p)); break;
+                        default: break;
+                    }
+                    break;
+                case KSORT_KEY_KIND__PMINMAX_INT64:
+                    /* kagg_state__pminmax_int64_packed (cuda_gpusort.cu:66-93) */
+                    if (p && (*(const uint32_t *)(p + 4) & __PAGG_MINMAX_ATTRS__VALID))
+                        v = sort_load64(p + 8) ^ 0x8000000000000000ULL;
+                    else
+                        p = nullptr;
+                    break;
+                case KSORT_KEY_KIND__PMINMAX_FP64:
+                    if (p && (*(const uint32_t *)(p + 4) & __PAGG_MINMAX_ATTRS__VALID))
+                        v = sort_enc_f64(sort_load64(p + 8));
+                    else
+                        p = nullptr;
+                    break;
+                case KSORT_KEY_KIND__PSUM_INT64:
+                    /* kagg_state__psum_int_packed: nitems > 0 (:124-151) */
+                    if (p && (int64_t)sort_load64(p + 8) > 0)
+                        v = sort_load64(p + 16) ^ 0x8000000000000000ULL;
+                    else
+                        p = nullptr;
+                    break;
+                case KSORT_KEY_KIND__PSUM_FP64:
+                    if (p && (int64_t)sort_load64(p + 8) > 0)
+                        v = sort_enc_f64(sort_load64(p + 16));
+                    else
+                        p = nullptr;
+                    break;
+                case KSORT_KEY_KIND__PAVG_INT64:
+                case KSORT_KEY_KIND__PAVG_FP64:
+                {
+                    /* sum / nitems in float8 (__gpusort_prep_pavg_*, :401-459) */
+                    const int64_t nitems = p ? (int64_t)sort_load64(p + 8) : 0;
+                    if (nitems == 0)
+                    {
+                        p = nullptr;
+                        break;
+                    }
+                    const uint64_t raw = sort_load64(p + 16);
+                    double sum;
+                    if (K.kind == KSORT_KEY_KIND__PAVG_INT64)
+                        sum = (double)(int64_t)raw;
+                    else
+                        __builtin_memcpy(&sum, &raw, 8);
+                    const double avg = sum / (double)nitems;
+                    uint64_t bits;
+                    __builtin_memcpy(&bits, &avg, 8);
+                    v = sort_enc_f64(bits);
+                    break;
+                }
+                case KSORT_KEY_KIND__PSUM_NUMERIC:
+                {
+                    /* kagg_state__psum_numeric_packed (:183-243): NaN sorts
+                     * above +inf, -inf below every value */
+                    if (!p || sort_load64(p + 8) == 0)
+                    {
+                        p = nullptr;
+                        break;
+                    }
+                    const uint32_t attrs = *(const uint32_t *)(p + 4);
+                    const uint32_t special = attrs & __PAGG_NUMERIC_ATTRS__MASK;
+                    if (special == 0)
+                    {
+                        cls = 2;
+                        v = sort_load64(p + 24) ^ 0x8000000000000000ULL;    /* high half */
+                        v2 = sort_load64(p + 16);                           /* low half */
+                    }
+                    else
+                        cls = (special == __PAGG_NUMERIC_ATTRS__NINF) ? 1
+                            : (special == __PAGG_NUMERIC_ATTRS__PINF) ? 3 : 4;
+                    if (!K.order_asc)
+                        cls = 5 - cls;
+                    break;
+                }
+                default:
+                    break;
+            }
+            const bool numeric = (K.kind == KSORT_KEY_KIND__PSUM_NUMERIC);
+            if (!p)
+            {
+                cls = numeric ? (K.nulls_first ? 0 : 5) : null_class;
+                v = v2 = 0;
+            }
+            else if (!K.order_asc)
+            {
+                v = ~v;
+                v2 = ~v2;
+            }
+            if (K.nwords == 1)
+                w[K.word] = (cls << 32) | (v & 0xffffffffULL);
+            else
+            {
+                w[K.word] = cls;
+                w[K.word + 1] = v;
+                if (K.nwords == 3)
+                    w[K.word + 2] = v2;
+            }
+        }
+        for (uint32_t k = 0; k < nw; k++)
+            sp.words[(uint64_t)k * sp.n + i] = w[k];
+        sp.idx[i] = i;
+    }
+    for (uint32_t k = 0; k < nw; k++)
+    {
+        unsigned long long o = valid ? w[k] : 0ULL;
+        unsigned long long a = valid ? w[k] : ~0ULL;
+        for (int d = 32; d >= 1; d >>= 1)
+        {
+            o |= __shfl_xor(o, d);
+            a &= __shfl_xor(a, d);
+        }
+        if (__lane_id() == 0)
+        {
+            atomicOr(&s_or[k], o);
+            atomicAnd(&s_and[k], a);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nw)
+    {
+        atomicOr((unsigned long long *)&sp.wor[threadIdx.x], s_or[threadIdx.x]);
+        atomicAnd((unsigned long long *)&sp.wand[threadIdx.x], s_and[threadIdx.x]);
+    }
+}
+
+/* digit counts of one tile: counts[digit * ntiles + tile] */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_hist_kernel(const uint64_t *words, uint32_t n, uint32_t word, uint32_t shift,
+                     uint32_t *counts)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t *key = words + (uint64_t)word * n;
+    const uint32_t base = blockIdx.x * XSORT_TILE;
+    for (int r = 0; r < XSORT_ROUNDS; r++)
+    {
+        const uint32_t j = base + r * XSORT_BLOCK + threadIdx.x;
+        if (j < n)
+            atomicAdd(&h[(uint32_t)(key[j] >> shift) & 0xffU], 1U);
+    }
+    __syncthreads();
+    counts[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+/* exclusive prefix over m counts, one block: each thread owns a contiguous
+ * segment */
+__global__ void __launch_bounds__(1024)
+xpu_sort_scan_kernel(uint32_t *counts, uint32_t m)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (m + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = lo + per < m ? lo + per : m;
+    uint32_t sum = 0;
+    for (uint32_t j = lo; j < hi; j++)
+        sum += counts[j];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1)
+    {
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t j = lo; j < hi; j++)
+    {
+        const uint32_t c = counts[j];
+        counts[j] = run;
+        run += c;
+    }
+}
+
+/* stable scatter of one tile: items go out in (digit, tile, position) order.
+ * A round takes 256 consecutive items, one per lane; lanes of a wave with
+ * the same digit find each other with 8 ballots (one per digit bit), so an
+ * item's place is the digit's running offset, the counts of the earlier
+ * waves of the round, and its rank among its wave's equal digits. */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_scatter_kernel(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
+                        uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
+                        uint32_t shift, const uint32_t *counts)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[XSORT_BLOCK / 64][256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid / 64;
+    const uint32_t lane = __lane_id();
+    run[tid] = counts[(uint64_t)tid * gridDim.x + blockIdx.x];
+    const uint64_t *key = words_in + (uint64_t)word * n;
+    const uint32_t base = blockIdx.x * XSORT_TILE;
+    for (int r = 0; r < XSORT_ROUNDS; r++)
+    {
+        for (uint32_t q = tid; q < (XSORT_BLOCK / 64) * 256; q += XSORT_BLOCK)
+            wc[q / 256][q % 256] = 0;
+        __syncthreads();
+        const uint32_t j = base + r * XSORT_BLOCK + tid;
+        const bool valid = j < n;
+        const uint32_t d = valid ? (uint32_t)(key[j] >> shift) & 0xffU : 0U;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+        {
+            const bool bit = (d >> b) & 1U;
+            const uint64_t bb = __ballot(valid && bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & ((1ULL << lane) - 1ULL));
+        if (valid && rank == 0)
+            wc[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid)
+        {
+            uint32_t pos = run[d] + rank;
+            for (uint32_t w2 = 0; w2 < wave; w2++)
+                pos += wc[w2][d];
+            for (uint32_t k = 0; k < nwords; k++)
+                words_out[(uint64_t)k * n + pos] = words_in[(uint64_t)k * n + j];
+            idx_out[pos] = idx_in[j];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (uint32_t w2 = 0; w2 < XSORT_BLOCK / 64; w2++)
+            add += wc[w2][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+__global__ void
+xpu_sort_permute_kernel(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = rowindex[idx[i]];
+}
+
+extern "C" int
+xpu_launch_sort_keys(const xsort_params *sp, void *stream)
+{
+    if (sp->n == 0)
+        return 0;
+    hipLaunchKernelGGL(xpu_sort_keys_kernel, dim3((sp->n + XSORT_BLOCK - 1) / XSORT_BLOCK),
+                       dim3(XSORT_BLOCK), 0, (hipStream_t)stream, *sp);
+    return (int)hipGetLastError();
+}
+
+/* one stable pass by the digit (words[word] >> shift) & 0xff; counts holds
+ * 256 * ceil(n / XSORT_TILE) entries */
+extern "C" int
+xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
+                     uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
+                     uint32_t shift, uint32_t *counts, void *stream)
+{
+    if (n == 0)
+        return 0;
+    const uint32_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
+    hipLaunchKernelGGL(xpu_sort_hist_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0,
+                       (hipStream_t)stream, words_in, n, word, shift, counts);
+    hipLaunchKernelGGL(xpu_sort_scan_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream,
+                       counts, 256U * ntiles);
+    hipLaunchKernelGGL(xpu_sort_scatter_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0,
+                       (hipStream_t)stream, words_in, idx_in, words_out, idx_out, n, nwords,
+                       word, shift, (const uint32_t *)counts);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_sort_permute(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out,
+                        uint32_t n, void *stream)
+{
+    if (n == 0)
+        return 0;
+    hipLaunchKernelGGL(xpu_sort_permute_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, rowindex, idx, out, n);
+    return (int)hipGetLastError();
+}

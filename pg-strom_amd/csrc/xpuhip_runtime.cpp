@@ -213,6 +213,8 @@ struct xpuhip_session
     char           *d_inner_own[XP_MAX_DEPTH + 1] = {};  /* inner buffers rebuilt from pinned
                                                             results (owned) */
     bool            inner_ready = false;
+    char           *d_sort = nullptr;       /* GPU-Sort work space of task_final */
+    size_t          sort_cap = 0;
     xpuhip_tuning   tuning;
     std::mutex      lock;
     std::vector<Scratch *> pool;
@@ -1824,6 +1826,112 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
     return rc;
 }
 
+/* GPU-Sort of the final buffer's n items: the first nout entries of the
+ * sorted row index go to *d_out (in the session's work space) */
+static int
+sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
+           const uint64_t **d_out, kern_errorbuf *eb)
+{
+    const xpu_compiled &c = s->comp;
+    const uint32_t nw = c.sort_nwords;
+    const uint64_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
+    const size_t words_bytes = 8ULL * nw * n;
+    const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * ntiles + 16ULL * nw +
+                        8ULL * nout + 8 * 256;     /* 7 slices, each rounded up to 256 B */
+    if (need > s->sort_cap)
+    {
+        if (s->d_sort)
+            (void)hipFree(s->d_sort);
+        s->d_sort = nullptr;
+        s->sort_cap = 0;
+        if (!hip_ok(hipMalloc((void **)&s->d_sort, need), eb, "hipMalloc(sort)"))
+            return -1;
+        s->sort_cap = need;
+    }
+    char *p = s->d_sort;
+    auto take = [&](size_t bytes) { char *r = p; p += (bytes + 255) & ~(size_t)255; return r; };
+    uint64_t *W[2] = {(uint64_t *)take(words_bytes), (uint64_t *)take(words_bytes)};
+    uint32_t *I[2] = {(uint32_t *)take(4ULL * n), (uint32_t *)take(4ULL * n)};
+    uint32_t *counts = (uint32_t *)take(4ULL * 256 * ntiles);
+    uint64_t *wor = (uint64_t *)take(8ULL * nw);
+    uint64_t *wand = (uint64_t *)take(8ULL * nw);
+    uint64_t *ridx_out = (uint64_t *)take(8ULL * nout);
+    if ((size_t)(p - s->d_sort) > s->sort_cap)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort work space layout exceeds its allocation");
+        return -1;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, nullptr);
+    xsort_params sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.kds = s->d_kds_final;
+    sp.rowindex = d_ridx;
+    sp.n = n;
+    sp.nkeys = (uint32_t)c.sort_keys.size();
+    sp.nwords = nw;
+    for (size_t k = 0; k < c.sort_keys.size(); k++)
+        sp.keys[k] = c.sort_keys[k];
+    sp.words = W[0];
+    sp.idx = I[0];
+    sp.wor = wor;
+    sp.wand = wand;
+    int rc = xpu_launch_fill(wor, 0, nw, nullptr);
+    if (rc == 0)
+        rc = xpu_launch_fill(wand, ~0ULL, nw, nullptr);
+    if (rc == 0)
+        rc = xpu_launch_sort_keys(&sp, nullptr);
+    std::vector<uint64_t> h_or(nw), h_and(nw);
+    if (rc != 0 ||
+        !hip_ok(hipMemcpy(h_or.data(), wor, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+        !hip_ok(hipMemcpy(h_and.data(), wand, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+    {
+        if (rc != 0)
+            XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort key pass launch failed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return -1;
+    }
+    /* least significant digit first: the last word's low byte */
+    int cur = 0;
+    uint32_t passes = 0;
+    for (int w = (int)nw - 1; w >= 0 && rc == 0; w--)
+    {
+        const uint64_t diff = h_or[w] ^ h_and[w];
+        for (uint32_t shift = 0; shift < 64 && rc == 0; shift += 8)
+        {
+            if (((diff >> shift) & 0xffU) == 0)
+                continue;
+            rc = xpu_launch_sort_pass(W[cur], I[cur], W[cur ^ 1], I[cur ^ 1], n, nw, (uint32_t)w,
+                                      shift, counts, nullptr);
+            cur ^= 1;
+            passes++;
+        }
+    }
+    if (rc == 0)
+        rc = xpu_launch_sort_permute(d_ridx, I[cur], ridx_out, nout, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    if (rc != 0 || !hip_ok(hipEventSynchronize(e1), eb, "hipEventSynchronize"))
+    {
+        if (rc != 0)
+            XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort pass launch failed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return -1;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    t_last_stats.sort_ms = ms;
+    t_last_stats.sort_items = n;
+    t_last_stats.sort_passes = passes;
+    *d_out = ridx_out;
+    return 0;
+}
+
 extern "C" int64_t
 xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
                   kern_exec_results *res, kern_errorbuf *eb)
@@ -1841,12 +1949,17 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
         return -1;
     kern_data_store *h = (kern_data_store *)hbuf.data();
     uint64_t nitems = h->nitems, usage = h->usage, nslots = h->hash_nslots;
-    uint64_t need = head_len + 8 * nitems + usage;
+    /* GPU-Sort and its LIMIT (gpuservSortingFinalBuffer) */
+    const bool sorted = !s->comp.sort_keys.empty();
+    uint64_t nout = nitems;
+    if (sorted && s->comp.sort_limit > 0 && (uint64_t)s->comp.sort_limit < nitems)
+        nout = (uint64_t)s->comp.sort_limit;
+    uint64_t need = head_len + 8 * nout + usage;
     if (res)
     {
         memset(res, 0, sizeof(kern_exec_results));
         res->final_plan_task = true;
-        res->final_nitems = (uint32_t)nitems;
+        res->final_nitems = (uint32_t)nout;
         res->final_usage = usage;
         res->final_total = need;
     }
@@ -1857,13 +1970,29 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
         XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "kds_final buffer is too small");
         return -1;
     }
+    const uint64_t *d_ridx = (const uint64_t *)((char *)s->d_kds_final + head_len + 8 * nslots);
+    t_last_stats.sort_ms = 0;
+    t_last_stats.sort_items = 0;
+    t_last_stats.sort_passes = 0;
+    if (sorted && nitems > 1)
+    {
+        if (nitems > UINT32_MAX)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort of more than 2^32 - 1 items");
+            return -1;
+        }
+        std::lock_guard<std::mutex> g(s->lock);
+        if (sort_final(s, (uint32_t)nitems, (uint32_t)nout, d_ridx, &d_ridx, eb) != 0)
+            return -1;
+    }
     /* HASH -> ROW on the wire (gpu_service.c:2780-2818) */
     h->format = KDS_FORMAT_ROW;
     h->hash_nslots = 0;
     h->length = need;
+    h->nitems = (uint32_t)nout;
     memcpy(out, hbuf.data(), head_len);
-    if (!hip_ok(hipMemcpy((char *)out + head_len, (char *)s->d_kds_final + head_len + 8 * nslots,
-                          8 * nitems, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+    if (!hip_ok(hipMemcpy((char *)out + head_len, d_ridx,
+                          8 * nout, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
         (usage > 0 &&
          !hip_ok(hipMemcpy((char *)out + need - usage, (char *)s->d_kds_final + s->final_length - usage,
                            usage, hipMemcpyDeviceToHost), eb, "hipMemcpy")))
@@ -1902,6 +2031,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_bloom) (void)hipFree(s->d_bloom);
     if (s->d_params) (void)hipFree(s->d_params);
     if (s->d_xids) (void)hipFree(s->d_xids);
+    if (s->d_sort) (void)hipFree(s->d_sort);
     for (int d = 0; d <= XP_MAX_DEPTH; d++)
         if (s->d_inner_own[d]) (void)hipFree(s->d_inner_own[d]);
     delete s;

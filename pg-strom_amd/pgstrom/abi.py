@@ -73,6 +73,22 @@ XpuCommandTag__XpuTaskExec = 110
 XpuCommandTag__XpuTaskFinal = 119
 XpuCommandMagicNumber = 0xdeadbeaf
 
+# GPU-Sort key descriptors (xpu_common.h:2474-2520)
+KSORT_KEY_ATTR__NULLS_FIRST = 0x0400
+KSORT_KEY_ATTR__ORDER_ASC = 0x8000
+KSORT_KEY_KIND__VREF = 0
+KSORT_KEY_KIND__PMINMAX_INT64 = 1
+KSORT_KEY_KIND__PMINMAX_FP64 = 2
+KSORT_KEY_KIND__PSUM_INT64 = 3
+KSORT_KEY_KIND__PSUM_FP64 = 4
+KSORT_KEY_KIND__PSUM_NUMERIC = 5
+KSORT_KEY_KIND__PAVG_INT64 = 6
+KSORT_KEY_KIND__PAVG_FP64 = 7
+KSORT_KEY_KIND__PAVG_NUMERIC = 8
+KSORT_KEY_KIND__PVARIANCE_SAMP = 9
+KSORT_WINDOW_FUNC__ROW_NUMBER = ord("n")
+KSORT_WINDOW_FUNC__RANK = ord("r")
+
 KAGG_ACTION__VREF = 101
 KAGG_ACTION__VREF_NOKEY = 102
 KAGG_ACTION__NROWS_ANY = 201

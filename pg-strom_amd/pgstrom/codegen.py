@@ -339,7 +339,8 @@ class Plan:
         return slots
 
     def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None,
-              pinned_hash=None, pinned_nslots=137, pinned_length=512 << 20):
+              pinned_hash=None, pinned_nslots=137, pinned_length=512 << 20, sort_keys=None,
+              sort_limit=0, window_rank=None):
         """The session blob.  xids: parallelCurrentXids of the transaction
         (SerializedTransactionState at session_xact_state, xpu_common.h:3156-3165),
         which decide the visibility of KDS_FORMAT_ROW sources.  pinned_hash:
@@ -347,7 +348,11 @@ class Plan:
         as a pinned inner buffer (DEVTASK__PINNED_HASH_RESULTS): the
         projection carries their HashValue (codegen.c:3933-3941) and
         groupby_kds_final the buffer's KDS_FORMAT_HASH header
-        (executor.c:655-672)."""
+        (executor.c:655-672).  sort_keys: GPU-Sort of the group-by results,
+        [(KSORT_KEY_KIND__*, attribute number in kds_final, typname of a VREF
+        key or None, asc, nulls_first)], with sort_limit rows kept
+        (gpusort_limit_count); window_rank: (func, limit, partby, orderby)
+        of a window rank function."""
         nrels = len(self.inner_coldefs)
         codes = {}
         # scan quals
@@ -510,6 +515,12 @@ class Plan:
             sess.groupby_kds_final = _append_aligned(buf, bytes(head))
             sess.groupby_ngroups_estimation = float(final_nrows)
             flags |= abi.DEVTASK__PINNED_HASH_RESULTS
+        if sort_keys:
+            sess.xpucode_gpusort_keydesc = _append_aligned(buf, _sortkeys_kexp(sort_keys,
+                                                                             window_rank))
+            sess.gpusort_limit_count = int(sort_limit)
+            finalized = A_KSORT_PAVG_KINDS
+            sess.gpusort_htup_margin = 9 * sum(1 for k in sort_keys if k[0] in finalized)
         if xids is not None:
             xs = struct.pack("<i?3xQQIi", 1, False, 0, 0, 0, len(xids)) + \
                 struct.pack("<%dI" % len(xids), *xids)
@@ -523,6 +534,45 @@ class Plan:
         for i, o in enumerate(poffsets):
             struct.pack_into("<I", buf, abi.SESSION_POFFSET_OFFSET + 4 * i, o)
         return bytes(buf)
+
+
+A_KSORT_PAVG_KINDS = (abi.KSORT_KEY_KIND__PAVG_INT64, abi.KSORT_KEY_KIND__PAVG_FP64,
+                      abi.KSORT_KEY_KIND__PAVG_NUMERIC)
+A_KSORT_RAW_KINDS = (abi.KSORT_KEY_KIND__VREF, abi.KSORT_KEY_KIND__PMINMAX_INT64,
+                     abi.KSORT_KEY_KIND__PMINMAX_FP64, abi.KSORT_KEY_KIND__PSUM_INT64,
+                     abi.KSORT_KEY_KIND__PSUM_FP64, abi.KSORT_KEY_KIND__PSUM_NUMERIC)
+
+
+def _sortkeys_kexp(keys, window_rank=None):
+    """FuncOpCode__SortKeys (codegen.c:4360-4525): u.sort header, then one
+    kern_sortkey_desc per key; finalized keys (averages) get 9 bytes of the
+    tuple margin each (bool + float8)."""
+    b = bytearray(_kexp_head(TYPE_CODE["int4"], SPECIAL["SortKeys"]))
+    needs_fin = any(k[0] not in A_KSORT_RAW_KINDS for k in keys)
+    func, wlimit, partby, orderby = window_rank or (0, 0, 0, 0)
+    b += struct.pack("<i?BxxIHH", len(keys), needs_fin, func, wlimit, partby, orderby)
+    margin = 0
+    for kind, anum, typname, asc, nulls_first in keys:
+        if kind == abi.KSORT_KEY_KIND__VREF:
+            tcode = TYPE_CODE[typname]
+            boff = 0
+        elif kind in A_KSORT_PAVG_KINDS:
+            tcode = TYPE_CODE["float8"]
+            boff = margin
+            margin += 9
+        elif kind == abi.KSORT_KEY_KIND__PSUM_NUMERIC:
+            tcode = TYPE_CODE["numeric"]
+            boff = 0
+        elif kind in (abi.KSORT_KEY_KIND__PMINMAX_FP64, abi.KSORT_KEY_KIND__PSUM_FP64):
+            tcode = TYPE_CODE["float8"]
+            boff = 0
+        else:
+            tcode = TYPE_CODE["int8"]
+            boff = 0
+        b += struct.pack("<HbbHHi4xQ", kind, 1 if nulls_first else 0, 1 if asc else 0, anum,
+                         boff, tcode, 0)
+    _finish(b, 0)
+    return bytes(b)
 
 
 def xpu_command(tag, payload):

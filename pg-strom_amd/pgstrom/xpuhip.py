@@ -73,7 +73,8 @@ class xpuhip_task_stats(C.Structure):
                 ("ntiles", C.c_uint32), ("lds_nents", C.c_uint32), ("lds_bytes", C.c_uint64),
                 ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64),
                 ("spill_rows", C.c_uint64), ("jit", C.c_uint32), ("_pad", C.c_uint32),
-                ("heap_ms", C.c_double), ("heap_rows", C.c_uint64)]
+                ("heap_ms", C.c_double), ("heap_rows", C.c_uint64),
+                ("sort_ms", C.c_double), ("sort_items", C.c_uint32), ("sort_passes", C.c_uint32)]
 
 
 _lib = None

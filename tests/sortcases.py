@@ -1,0 +1,84 @@
+"""GPU-Sort cases shared by the CPU (oracle) and GPU tests: the group-by of
+count(*), sum(v), avg(w), min(x), max(x) GROUP BY key with sort keys over its
+final buffer (kds_final attributes: 1 count, 2 sum(v) numeric state, 3 avg(w)
+state, 4 min(x), 5 max(x), 6 key)."""
+import numpy as np
+
+from pgstrom import abi, codegen as cg, kds, workloads as W
+from pgstrom.kds import Column
+
+A = abi
+NROWS, PSUM, PAVG, PMIN, PMAX, KEY = 1, 2, 3, 4, 5, 6
+
+
+def key_vref(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__VREF, KEY, "int4", asc, nulls_first)
+
+
+def count_vref(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__VREF, NROWS, "int8", asc, nulls_first)
+
+
+def sum_key(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__PSUM_NUMERIC, PSUM, None, asc, nulls_first)
+
+
+def avg_key(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__PAVG_FP64, PAVG, None, asc, nulls_first)
+
+
+def min_key(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__PMINMAX_INT64, PMIN, None, asc, nulls_first)
+
+
+def max_key(asc=True, nulls_first=False):
+    return (A.KSORT_KEY_KIND__PMINMAX_INT64, PMAX, None, asc, nulls_first)
+
+
+def sorted_query(cols, sort_keys, limit=0, final_nrows=10000, final_length=1 << 30):
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    key, v, w, x = (plan.scan_var(n) for n in ("key", "v", "w", "x"))
+    plan.set_groupby(W.preagg_tlist(plan, key, v, w, x))
+    return plan.build(final_nrows=final_nrows, final_length=final_length, sort_keys=sort_keys,
+                      sort_limit=limit)
+
+
+def tiny_table():
+    """Nine rows, five groups (one of them the NULL key):
+         key   count  sum(v)  avg(w)  min(x)  max(x)
+         1     2      10      4.0     -3      8
+         2     2      8       2.0     2       4
+         3     2      -10     3.0     1       7
+         5     1      100     0.5     0       0
+         NULL  2      3       6.0     -5      9"""
+    key = np.array([3, 1, 2, 0, 1, 3, 2, 5, 0], dtype=np.int32)
+    knull = np.array([0, 0, 0, 1, 0, 0, 0, 0, 1], dtype=bool)
+    v = np.array([10, 5, 7, 1, 5, -20, 1, 100, 2], dtype=np.int64)
+    w = np.array([1.0, 2.0, 3.0, 4.0, 6.0, 5.0, 1.0, 0.5, 8.0])
+    x = np.array([7, -3, 4, 9, 8, 1, 2, 0, -5], dtype=np.int32)
+    return [Column("key", "int4", key, knull), Column("v", "int8", v),
+            Column("w", "float8", w), Column("x", "int4", x)]
+
+
+# (sort keys, limit, the group keys in the expected order)
+KNOWN = [
+    ([key_vref(True, False)], 0, [1, 2, 3, 5, None]),
+    ([key_vref(False, True)], 0, [None, 5, 3, 2, 1]),
+    ([key_vref(True, True)], 0, [None, 1, 2, 3, 5]),
+    ([key_vref(False, False)], 0, [5, 3, 2, 1, None]),
+    ([sum_key(False), key_vref()], 0, [5, 1, 2, None, 3]),
+    ([sum_key(True), key_vref()], 0, [3, None, 2, 1, 5]),
+    ([avg_key(True), key_vref()], 0, [5, 2, 3, 1, None]),
+    ([min_key(True), key_vref()], 0, [None, 1, 5, 3, 2]),
+    ([max_key(False), key_vref()], 0, [None, 1, 3, 2, 5]),
+    ([count_vref(False), key_vref(True, False)], 0, [1, 2, 3, None, 5]),
+    ([count_vref(False), key_vref(False, True)], 0, [None, 3, 2, 1, 5]),
+    ([sum_key(False), key_vref()], 2, [5, 1]),
+    ([count_vref(True), key_vref()], 3, [5, 1, 2]),
+]
+
+
+def group_keys_in_order(kds_final):
+    """The group key of each row of a written-back final buffer, in row-index
+    order."""
+    return [vals[KEY - 1] for vals, _ in kds.parse_row_kds(kds_final)]

@@ -1,0 +1,87 @@
+"""GPU-Sort of the group-by results on the GPU: the final buffer's row index
+sorted by the session's SortKeys on the device (key words + LSD radix sort)
+gives the oracle's order (cuda_gpusort.cu:273-342), with LIMIT keeping the
+leading rows; every sort ends in the group key, so the order is total and
+the row sequences must agree exactly."""
+import numpy as np
+import pytest
+
+from pgstrom import abi, datagen, kds, xpuhip
+from pgstrom.kds import Column
+from helpers import assert_groups_equal, run_oracle_groupby
+import sortcases as S
+
+pytestmark = pytest.mark.gpu
+
+C1_ACTIONS = [abi.KAGG_ACTION__NROWS_ANY, abi.KAGG_ACTION__PSUM_INT64, abi.KAGG_ACTION__PAVG_FP,
+              abi.KAGG_ACTION__PMIN_INT32, abi.KAGG_ACTION__PMAX_INT32, abi.KAGG_ACTION__VREF]
+
+
+def _gpu_sorted(blob, chunks, tuning=None):
+    s = xpuhip.Session(blob)
+    if tuning:
+        s.set_tuning(**tuning)
+    for src in chunks:
+        rc, _, _, _ = s.exec(src)
+        assert rc == 0
+    fin = s.final()
+    st = xpuhip.last_task_stats()
+    again = s.final()
+    s.close()
+    return fin, st, again
+
+
+@pytest.mark.parametrize("case", range(len(S.KNOWN)))
+def test_gpu_sort_known_orders(gpu, case):
+    keys, limit, expected = S.KNOWN[case]
+    cols = S.tiny_table()
+    fin, st, again = _gpu_sorted(S.sorted_query(cols, keys, limit), [kds.build_arrow_kds(cols)])
+    assert S.group_keys_in_order(fin) == expected
+    assert S.group_keys_in_order(again) == expected
+    assert st["sort_items"] == 5
+
+
+SORTS = [
+    ([S.sum_key(False), S.key_vref()], 0),
+    ([S.sum_key(False), S.key_vref()], 100),
+    ([S.avg_key(True), S.key_vref()], 0),
+    ([S.min_key(True, True), S.max_key(False), S.key_vref(False)], 0),
+    ([S.count_vref(False), S.key_vref()], 25),
+    ([S.key_vref(False)], 0),
+]
+
+
+@pytest.mark.parametrize("which", range(len(SORTS)))
+def test_gpu_sort_matches_oracle(gpu, jit, which):
+    """C1's table (400K rows, Zipf keys): the sorted groups, in order."""
+    keys, limit = SORTS[which]
+    cols = datagen.preagg_table(400_000, fragment=21)
+    src = kds.build_arrow_kds(cols)
+    blob = S.sorted_query(cols, keys, limit)
+    fin, st, _ = _gpu_sorted(blob, [src], dict(jit=jit))
+    exp = run_oracle_groupby(blob, [src])
+    got_order = S.group_keys_in_order(fin)
+    assert got_order == S.group_keys_in_order(exp)
+    assert len(got_order) == (limit or len(got_order))
+    assert st["sort_items"] > limit and st["sort_passes"] >= 1
+    assert_groups_equal(fin, exp, C1_ACTIONS)
+
+
+def test_gpu_sort_million_groups(gpu):
+    """About a million groups: sum(v) DESC, key; the first rows and the whole
+    order against the oracle."""
+    r = np.random.default_rng(77)
+    n = 3_000_000
+    key = r.integers(0, 1_000_000, size=n, dtype=np.int32)
+    cols = [Column("key", "int4", key),
+            Column("v", "int8", r.integers(-1000, 1_000_000, size=n, dtype=np.int64)),
+            Column("w", "float8", r.random(n) * 1000.0),
+            Column("x", "int4", r.integers(-2**31, 2**31 - 1, size=n, dtype=np.int32))]
+    src = kds.build_arrow_kds(cols)
+    blob = S.sorted_query(cols, [S.sum_key(False), S.key_vref()], 0,
+                          final_nrows=1_000_000, final_length=1 << 30)
+    fin, st, _ = _gpu_sorted(blob, [src])
+    ngroups = len(np.unique(key))
+    assert st["sort_items"] == ngroups
+    exp = run_oracle_groupby(blob, [src])
+    assert S.group_keys_in_order(fin) == S.group_keys_in_order(exp)

@@ -1,0 +1,66 @@
+"""GPU-Sort on the CPU: the oracle's restatement of the reference's
+comparator (cuda_gpusort.cu:15-343) and LIMIT (gpu_service.c:5099-5197)
+against hand-computed orders, and the compiler's checks of SortKeys."""
+import pytest
+
+from pgstrom import abi, kds, xpuhip
+from helpers import run_oracle_groupby
+import sortcases as S
+
+
+@pytest.mark.parametrize("case", range(len(S.KNOWN)))
+def test_oracle_sort_known_orders(case):
+    keys, limit, expected = S.KNOWN[case]
+    cols = S.tiny_table()
+    blob = S.sorted_query(cols, keys, limit)
+    fin = run_oracle_groupby(blob, [kds.build_arrow_kds(cols)])
+    assert S.group_keys_in_order(fin) == expected
+
+
+def test_oracle_sort_limit_keeps_leading_rows():
+    """LIMIT n keeps the first n rows of the full order."""
+    cols = S.tiny_table()
+    full = S.group_keys_in_order(run_oracle_groupby(
+        S.sorted_query(cols, [S.avg_key(False), S.key_vref()]), [kds.build_arrow_kds(cols)]))
+    for n in range(1, 6):
+        got = S.group_keys_in_order(run_oracle_groupby(
+            S.sorted_query(cols, [S.avg_key(False), S.key_vref()], limit=n),
+            [kds.build_arrow_kds(cols)]))
+        assert got == full[:n]
+
+
+def _compile_error(blob):
+    rc, err = xpuhip.session_check(blob)
+    return None if rc == 0 else err.message.decode()
+
+
+@pytest.mark.parametrize("bad, message", [
+    ("window", "window rank functions are not supported"),
+    ("anum", "key attribute out of range"),
+    ("type", "does not match the final buffer column"),
+    ("kind", "kind is not supported"),
+])
+def test_sort_keys_compile_checks(bad, message):
+    cols = S.tiny_table()
+    if bad == "window":
+        from pgstrom import codegen as cg, workloads as W
+        plan = cg.Plan([(c.name, c.typname) for c in cols])
+        key, v, w, x = (plan.scan_var(n) for n in ("key", "v", "w", "x"))
+        plan.set_groupby(W.preagg_tlist(plan, key, v, w, x))
+        blob = plan.build(sort_keys=[S.key_vref()],
+                          window_rank=(abi.KSORT_WINDOW_FUNC__RANK, 3, 0, 1))
+    elif bad == "anum":
+        blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__VREF, 9, "int4", True, False)])
+    elif bad == "type":
+        blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__VREF, S.KEY, "int8", True, False)])
+    else:
+        blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__PVARIANCE_SAMP, S.PAVG, None, True,
+                                      False)])
+    err = _compile_error(blob)
+    assert err is not None and message in err
+
+
+def test_sort_keys_compile():
+    cols = S.tiny_table()
+    for keys, limit, _ in S.KNOWN:
+        assert _compile_error(S.sorted_query(cols, keys, limit)) is None
