@@ -187,6 +187,79 @@ def bench_star(n, warmup, steps):
                 host_build_s=host_build_s)
 
 
+def sort_query(cols, sort_keys, limit=0):
+    from pgstrom import codegen as cg, workloads as W
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    key, v, w, x = (plan.scan_var(nm) for nm in ("key", "v", "w", "x"))
+    plan.set_groupby(W.preagg_tlist(plan, key, v, w, x))
+    return plan.build(final_nrows=1_000_000, final_length=1 << 30, sort_keys=sort_keys,
+                      sort_limit=limit)
+
+
+def bench_sort(n, warmup, steps):
+    """GPU-Sort downstream of the group-by: n rows with an int4 key uniform
+    over 1M values (about a million groups) aggregated as C1's query, then
+    ORDER BY sum(v) DESC, key over the final buffer.  A step is one
+    xpuhip_task_final: key pass, radix passes, permute and the write-back;
+    sort_ms is the device time of the sort passes alone (HIP events).  The
+    oracle's qsort of the same final buffer is timed beside it (the final
+    with the sort keys minus the final without them)."""
+    from pgstrom import abi, kds, xpuhip
+    from pgstrom.kds import Column
+    r = np.random.default_rng(77)
+    cols = [Column("key", "int4", r.integers(0, 1_000_000, size=n, dtype=np.int32)),
+            Column("v", "int8", r.integers(-1000, 1_000_000, size=n, dtype=np.int64)),
+            Column("w", "float8", r.random(n) * 1000.0),
+            Column("x", "int4", r.integers(-2**31, 2**31 - 1, size=n, dtype=np.int32))]
+    src = kds.build_arrow_kds(cols)
+    keys = [(abi.KSORT_KEY_KIND__PSUM_NUMERIC, 2, None, False, False),
+            (abi.KSORT_KEY_KIND__VREF, 6, "int4", True, False)]
+    blob = sort_query(cols, keys)
+    s = xpuhip.Session(blob)
+    rc, _, _, _ = s.exec(src)
+    assert rc == 0
+    sort_ms, walls = [], []
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        fin = s.final()
+        dt = time.perf_counter() - t0
+        st = xpuhip.last_task_stats()
+        if i >= warmup:
+            sort_ms.append(st["sort_ms"])
+            walls.append(dt)
+    s.close()
+    items, passes = st["sort_items"], st["sort_passes"]
+    nwords = 4      # numeric sum: class + two halves; int4 key: one word
+    # algorithmic bytes: key pass (row index in, words and item numbers out),
+    # per radix pass the digit word read by the histogram plus the words and
+    # item numbers read and written by the scatter, the permute's reads and
+    # writes
+    algo = items * ((8 + 8 * nwords + 4) + passes * (8 + 2 * (8 * nwords + 4)) + (4 + 8 + 8))
+    ms = float(np.mean(sort_ms))
+    # the oracle's qsort of the same groups
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cpu = {}
+    for name, b in (("sorted", blob), ("plain", sort_query(cols, None))):
+        o = oracle.OracleSession(b)
+        rc, _, _ = o.exec(src, nthreads=min(os.cpu_count() or 1, 16))
+        assert rc == 0
+        t0 = time.perf_counter()
+        o.final()
+        cpu[name] = time.perf_counter() - t0
+        o.close()
+    cpu_sort_s = max(cpu["sorted"] - cpu["plain"], 1e-9)
+    return dict(workload="GPU-Sort of %d groups (C1's aggregates over %d rows, key uniform "
+                         "over 1M): ORDER BY sum(v) DESC, key" % (items, n),
+                sort_ms=ms, final_wall_ms=float(np.mean(walls)) * 1e3, items=items,
+                radix_passes=passes, items_per_s=items / (ms / 1e3),
+                achieved_gbs=algo / (ms / 1e3) / 1e9, hbm_frac=algo / (ms / 1e3) / HBM_PEAK,
+                algo_bytes=algo,
+                cpu_baseline={"value": items / cpu_sort_s, "unit": "items/s", "cores": 1,
+                              "kind": "port", "sample": "the same %d groups, oracle qsort "
+                              "(final with sort keys minus final without)" % items})
+
+
 def bench_pinned(warmup, steps):
     """C2's three dimensions (2M / 200K / 2,556 rows, resident in HBM as Arrow
     chunks) built on the GPU: one scan session per dimension with its filter
@@ -440,7 +513,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--only", default="all",
-                    choices=["all", "preagg", "batch", "served", "join", "scan", "heap", "pinned"])
+                    choices=["all", "preagg", "batch", "served", "join", "scan", "heap", "pinned",
+                             "sort"])
     ap.add_argument("--served-requests", type=int, default=300)
     ap.add_argument("--served-table-rows", type=int, default=50_000_000)
     ap.add_argument("--served-slots", type=int, default=8)
@@ -476,6 +550,9 @@ def main():
     if args.only == "pinned":
         print(json.dumps(bench_pinned(args.warmup, args.steps)))
         return
+    if args.only == "sort":
+        print(json.dumps(bench_sort(3_000_000, args.warmup, args.steps)))
+        return
     if args.only == "heap":
         print(json.dumps(heap_line(bench_heap(1_000_000, args.warmup, args.steps))))
         return
@@ -498,6 +575,7 @@ def main():
                                         args.served_table_rows)
         others["star_join_200M"] = join_line(bench_star(200_000_000, 1, 5))
         others["pinned_inner_build"] = bench_pinned(2, 10)
+        others["gpu_sort_1M_groups"] = bench_sort(3_000_000, 2, 10)
     cpu = None
     if args.cpu_sample > 0:
         log("cpu baseline")

@@ -299,14 +299,15 @@ typedef struct
     xsort_key   keys[XSORT_MAX_KEYS];
     uint64_t   *words;              /* [nwords][n] */
     uint32_t   *idx;                /* [n] item numbers */
-    uint64_t   *wor;                /* [nwords] OR over the items */
-    uint64_t   *wand;               /* [nwords] AND over the items */
+    uint64_t   *wmin;               /* [nwords] least value over the items */
+    uint64_t   *wmax;               /* [nwords] greatest value over the items */
 } xsort_params;
 
 int xpu_launch_sort_keys(const xsort_params *sp, void *stream);
 int xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
                          uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
-                         uint32_t shift, uint32_t *counts, void *stream);
+                         uint64_t base, uint32_t shift, uint32_t *counts,
+                         void *stream);          /* counts: 256 * (ntiles + 1) */
 int xpu_launch_sort_permute(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out,
                             uint32_t n, void *stream);
 

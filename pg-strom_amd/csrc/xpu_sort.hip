@@ -11,21 +11,25 @@
  *                 and writes the key as 64-bit words whose unsigned order is
  *                 the comparator's order (NULLs first or last, ASC or DESC,
  *                 and an empty aggregate state sorting as NULL), plus the
- *                 OR and AND of every word over all items
+ *                 least and greatest value of every word over all items
  *   radix passes  stable LSD radix sort of (words, item number) by 8-bit
- *                 digits, last word first; a digit that is the same in every
- *                 item (OR and AND agree there) is skipped, so an int4 key
- *                 sorts in 4 passes and a column without NULLs pays no pass
- *                 for its NULL class.  Each pass is a digit histogram per
- *                 tile of 4,096 items, one exclusive scan over (digit, tile),
- *                 and a scatter that ranks equal digits with wave ballots
+ *                 digits of (word - least), last word first: subtracting a
+ *                 word's least value keeps its order, and only the bytes of
+ *                 (greatest - least) can differ, so a million int4 keys sort
+ *                 in 3 passes, values of both signs near zero pay for their
+ *                 range only, and a column without NULLs pays no pass for its
+ *                 NULL class.  Each pass is a digit histogram per tile of
+ *                 4,096 items, one exclusive scan over (digit, tile), and a
+ *                 scatter that ranks equal digits with wave ballots
  *   permute       the sorted item numbers pick the row-index entries
  *
  * Key words (each key in turn, most significant first):
  *   32-bit and narrower values   one word: class << 32 | value bits
  *   64-bit values                class, then the 64-bit value bits
  *   numeric sums (int128)        class (with -inf / finite / +inf / NaN),
- *                                then the high and low halves
+ *                                then the high and low halves of the sum
+ *                                plus 2^63 (sums within int64 keep a
+ *                                constant high half)
  * class is 1 for a value, 0 or 2 for NULL (NULLS FIRST / LAST).  Integers
  * have their sign bit flipped; floats have the sign bit set when positive
  * and every bit inverted when negative, -0 read as +0 (the reference's
@@ -123,28 +127,27 @@ sort_enc_f64(uint64_t b)
 __global__ void __launch_bounds__(XSORT_BLOCK)
 xpu_sort_keys_kernel(xsort_params sp)
 {
-    __shared__ unsigned long long s_or[XSORT_MAX_WORDS];
-    __shared__ unsigned long long s_and[XSORT_MAX_WORDS];
+    __shared__ unsigned long long s_min[XSORT_MAX_WORDS];
+    __shared__ unsigned long long s_max[XSORT_MAX_WORDS];
     const uint32_t nw = sp.nwords;
     if (threadIdx.x < nw)
     {
-        s_or[threadIdx.x] = 0;
-        s_and[threadIdx.x] = ~0ULL;
+        s_min[threadIdx.x] = ~0ULL;
+        s_max[threadIdx.x] = 0;
     }
     __syncthreads();
     const uint32_t i = blockIdx.x * XSORT_BLOCK + threadIdx.x;
     const bool valid = i < sp.n;
-    uint64_t w[XSORT_MAX_WORDS];
-#pragma unroll
-    for (int k = 0; k < XSORT_MAX_WORDS; k++)
-        w[k] = 0;
-    if (valid)
+    const kern_tupitem *t = valid ? (const kern_tupitem *)
+        ((const char *)sp.kds + sp.kds->length - sp.rowindex[i]) : nullptr;
+    /* keys are wave-uniform: each key's words are stored and folded into
+     * the least / greatest values as they are made */
+    for (uint32_t k = 0; k < sp.nkeys; k++)
     {
-        const kern_tupitem *t = (const kern_tupitem *)
-            ((const char *)sp.kds + sp.kds->length - sp.rowindex[i]);
-        for (uint32_t k = 0; k < sp.nkeys; k++)
+        const xsort_key K = sp.keys[k];
+        uint64_t kw[3] = {0, 0, 0};
+        if (valid)
         {
-            const xsort_key K = sp.keys[k];
             const char *p = sort_tuple_attr(sp.kds, t, K.anum);
             const uint64_t null_class = K.nulls_first ? 0 : 2;
             uint64_t cls = 1, v = 0, v2 = 0;
@@ -228,9 +231,12 @@ xpu_sort_keys_kernel(xsort_params sp)
                     const uint32_t special = attrs & __PAGG_NUMERIC_ATTRS__MASK;
                     if (special == 0)
                     {
+                        const uint64_t lo = sort_load64(p + 16);
+                        const uint64_t lo_t = lo + 0x8000000000000000ULL;
+                        const uint64_t hi_t = sort_load64(p + 24) + (lo_t < lo ? 1 : 0);
                         cls = 2;
-                        v = sort_load64(p + 24) ^ 0x8000000000000000ULL;    /* high half */
-                        v2 = sort_load64(p + 16);                           /* low half */
+                        v = hi_t ^ 0x8000000000000000ULL;       /* high half */
+                        v2 = lo_t;                              /* low half */
                     }
                     else
                         cls = (special == __PAGG_NUMERIC_ATTRS__NINF) ? 1
@@ -254,46 +260,50 @@ xpu_sort_keys_kernel(xsort_params sp)
                 v2 = ~v2;
             }
             if (K.nwords == 1)
-                w[K.word] = (cls << 32) | (v & 0xffffffffULL);
+                kw[0] = (cls << 32) | (v & 0xffffffffULL);
             else
             {
-                w[K.word] = cls;
-                w[K.word + 1] = v;
-                if (K.nwords == 3)
-                    w[K.word + 2] = v2;
+                kw[0] = cls;
+                kw[1] = v;
+                kw[2] = v2;
+            }
+            for (uint32_t q = 0; q < K.nwords; q++)
+                sp.words[(uint64_t)(K.word + q) * sp.n + i] = kw[q];
+        }
+        for (uint32_t q = 0; q < K.nwords; q++)
+        {
+            const uint32_t wd = K.word + q;
+            const uint64_t x = q == 0 ? kw[0] : q == 1 ? kw[1] : kw[2];
+            unsigned long long lo = valid ? x : ~0ULL;
+            unsigned long long hi = valid ? x : 0ULL;
+            for (int d = 32; d >= 1; d >>= 1)
+            {
+                const unsigned long long a = __shfl_xor(lo, d);
+                const unsigned long long b = __shfl_xor(hi, d);
+                lo = a < lo ? a : lo;
+                hi = b > hi ? b : hi;
+            }
+            if (__lane_id() == 0)
+            {
+                atomicMin(&s_min[wd], lo);
+                atomicMax(&s_max[wd], hi);
             }
         }
-        for (uint32_t k = 0; k < nw; k++)
-            sp.words[(uint64_t)k * sp.n + i] = w[k];
+    }
+    if (valid)
         sp.idx[i] = i;
-    }
-    for (uint32_t k = 0; k < nw; k++)
-    {
-        unsigned long long o = valid ? w[k] : 0ULL;
-        unsigned long long a = valid ? w[k] : ~0ULL;
-        for (int d = 32; d >= 1; d >>= 1)
-        {
-            o |= __shfl_xor(o, d);
-            a &= __shfl_xor(a, d);
-        }
-        if (__lane_id() == 0)
-        {
-            atomicOr(&s_or[k], o);
-            atomicAnd(&s_and[k], a);
-        }
-    }
     __syncthreads();
     if (threadIdx.x < nw)
     {
-        atomicOr((unsigned long long *)&sp.wor[threadIdx.x], s_or[threadIdx.x]);
-        atomicAnd((unsigned long long *)&sp.wand[threadIdx.x], s_and[threadIdx.x]);
+        atomicMin((unsigned long long *)&sp.wmin[threadIdx.x], s_min[threadIdx.x]);
+        atomicMax((unsigned long long *)&sp.wmax[threadIdx.x], s_max[threadIdx.x]);
     }
 }
 
 /* digit counts of one tile: counts[digit * ntiles + tile] */
 __global__ void __launch_bounds__(XSORT_BLOCK)
-xpu_sort_hist_kernel(const uint64_t *words, uint32_t n, uint32_t word, uint32_t shift,
-                     uint32_t *counts)
+xpu_sort_hist_kernel(const uint64_t *words, uint32_t n, uint32_t word, uint64_t wbase,
+                     uint32_t shift, uint32_t *counts)
 {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
@@ -304,40 +314,60 @@ xpu_sort_hist_kernel(const uint64_t *words, uint32_t n, uint32_t word, uint32_t 
     {
         const uint32_t j = base + r * XSORT_BLOCK + threadIdx.x;
         if (j < n)
-            atomicAdd(&h[(uint32_t)(key[j] >> shift) & 0xffU], 1U);
+            atomicAdd(&h[(uint32_t)((key[j] - wbase) >> shift) & 0xffU], 1U);
     }
     __syncthreads();
     counts[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-/* exclusive prefix over m counts, one block: each thread owns a contiguous
- * segment */
-__global__ void __launch_bounds__(1024)
-xpu_sort_scan_kernel(uint32_t *counts, uint32_t m)
+/* exclusive prefix of a 256-thread block's values (wave scans, then the
+ * wave totals through LDS); *total gets the block's sum */
+__device__ static inline uint32_t
+sort_block_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    const uint32_t hi = lo + per < m ? lo + per : m;
-    uint32_t sum = 0;
-    for (uint32_t j = lo; j < hi; j++)
-        sum += counts[j];
-    part[threadIdx.x] = sum;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[wave] = x;
     __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1)
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < XSORT_BLOCK / 64; w++)
     {
-        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
+        before += w < wave ? wsum[w] : 0;
+        all += wsum[w];
     }
-    uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t j = lo; j < hi; j++)
+    __syncthreads();
+    *total = all;
+    return x - v + before;
+}
+
+/* exclusive prefix over the tiles of one digit (block = digit): counts
+ * [digit * ntiles + tile] become offsets within the digit, and
+ * totals[digit] the digit's count */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_scan_kernel(uint32_t *counts, uint32_t ntiles, uint32_t *totals)
+{
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
+    uint32_t *row = counts + (uint64_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += XSORT_BLOCK)
     {
-        const uint32_t c = counts[j];
-        counts[j] = run;
-        run += c;
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t v = j < ntiles ? row[j] : 0;
+        uint32_t total;
+        const uint32_t ex = sort_block_scan(v, wsum, &total);
+        if (j < ntiles)
+            row[j] = carry + ex;
+        carry += total;
     }
+    if (threadIdx.x == 0)
+        totals[blockIdx.x] = carry;
 }
 
 /* stable scatter of one tile: items go out in (digit, tile, position) order.
@@ -348,14 +378,18 @@ xpu_sort_scan_kernel(uint32_t *counts, uint32_t m)
 __global__ void __launch_bounds__(XSORT_BLOCK)
 xpu_sort_scatter_kernel(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
                         uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
-                        uint32_t shift, const uint32_t *counts)
+                        uint64_t wbase, uint32_t shift, const uint32_t *counts,
+                        const uint32_t *totals)
 {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wc[XSORT_BLOCK / 64][256];
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid / 64;
     const uint32_t lane = __lane_id();
-    run[tid] = counts[(uint64_t)tid * gridDim.x + blockIdx.x];
+    uint32_t all;
+    const uint32_t digit_base = sort_block_scan(totals[tid], wsum, &all);
+    run[tid] = digit_base + counts[(uint64_t)tid * gridDim.x + blockIdx.x];
     const uint64_t *key = words_in + (uint64_t)word * n;
     const uint32_t base = blockIdx.x * XSORT_TILE;
     for (int r = 0; r < XSORT_ROUNDS; r++)
@@ -365,7 +399,7 @@ xpu_sort_scatter_kernel(const uint64_t *words_in, const uint32_t *idx_in, uint64
         __syncthreads();
         const uint32_t j = base + r * XSORT_BLOCK + tid;
         const bool valid = j < n;
-        const uint32_t d = valid ? (uint32_t)(key[j] >> shift) & 0xffU : 0U;
+        const uint32_t d = valid ? (uint32_t)((key[j] - wbase) >> shift) & 0xffU : 0U;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++)
@@ -414,23 +448,24 @@ xpu_launch_sort_keys(const xsort_params *sp, void *stream)
     return (int)hipGetLastError();
 }
 
-/* one stable pass by the digit (words[word] >> shift) & 0xff; counts holds
- * 256 * ceil(n / XSORT_TILE) entries */
+/* one stable pass by the digit ((words[word] - base) >> shift) & 0xff;
+ * counts holds 256 * (ceil(n / XSORT_TILE) + 1) entries */
 extern "C" int
 xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint64_t *words_out,
                      uint32_t *idx_out, uint32_t n, uint32_t nwords, uint32_t word,
-                     uint32_t shift, uint32_t *counts, void *stream)
+                     uint64_t base, uint32_t shift, uint32_t *counts, void *stream)
 {
     if (n == 0)
         return 0;
     const uint32_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
     hipLaunchKernelGGL(xpu_sort_hist_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0,
-                       (hipStream_t)stream, words_in, n, word, shift, counts);
-    hipLaunchKernelGGL(xpu_sort_scan_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream,
-                       counts, 256U * ntiles);
+                       (hipStream_t)stream, words_in, n, word, base, shift, counts);
+    uint32_t *totals = counts + 256ULL * ntiles;
+    hipLaunchKernelGGL(xpu_sort_scan_kernel, dim3(256), dim3(XSORT_BLOCK), 0, (hipStream_t)stream,
+                       counts, ntiles, totals);
     hipLaunchKernelGGL(xpu_sort_scatter_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0,
                        (hipStream_t)stream, words_in, idx_in, words_out, idx_out, n, nwords,
-                       word, shift, (const uint32_t *)counts);
+                       word, base, shift, (const uint32_t *)counts, (const uint32_t *)totals);
     return (int)hipGetLastError();
 }
 

@@ -1836,7 +1836,7 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     const uint32_t nw = c.sort_nwords;
     const uint64_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
     const size_t words_bytes = 8ULL * nw * n;
-    const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * ntiles + 16ULL * nw +
+    const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * (ntiles + 1) + 16ULL * nw +
                         8ULL * nout + 8 * 256;     /* 7 slices, each rounded up to 256 B */
     if (need > s->sort_cap)
     {
@@ -1852,9 +1852,9 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     auto take = [&](size_t bytes) { char *r = p; p += (bytes + 255) & ~(size_t)255; return r; };
     uint64_t *W[2] = {(uint64_t *)take(words_bytes), (uint64_t *)take(words_bytes)};
     uint32_t *I[2] = {(uint32_t *)take(4ULL * n), (uint32_t *)take(4ULL * n)};
-    uint32_t *counts = (uint32_t *)take(4ULL * 256 * ntiles);
-    uint64_t *wor = (uint64_t *)take(8ULL * nw);
-    uint64_t *wand = (uint64_t *)take(8ULL * nw);
+    uint32_t *counts = (uint32_t *)take(4ULL * 256 * (ntiles + 1));
+    uint64_t *wmin = (uint64_t *)take(8ULL * nw);
+    uint64_t *wmax = (uint64_t *)take(8ULL * nw);
     uint64_t *ridx_out = (uint64_t *)take(8ULL * nout);
     if ((size_t)(p - s->d_sort) > s->sort_cap)
     {
@@ -1876,17 +1876,17 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
         sp.keys[k] = c.sort_keys[k];
     sp.words = W[0];
     sp.idx = I[0];
-    sp.wor = wor;
-    sp.wand = wand;
-    int rc = xpu_launch_fill(wor, 0, nw, nullptr);
+    sp.wmin = wmin;
+    sp.wmax = wmax;
+    int rc = xpu_launch_fill(wmin, ~0ULL, nw, nullptr);
     if (rc == 0)
-        rc = xpu_launch_fill(wand, ~0ULL, nw, nullptr);
+        rc = xpu_launch_fill(wmax, 0, nw, nullptr);
     if (rc == 0)
         rc = xpu_launch_sort_keys(&sp, nullptr);
-    std::vector<uint64_t> h_or(nw), h_and(nw);
+    std::vector<uint64_t> h_min(nw), h_max(nw);
     if (rc != 0 ||
-        !hip_ok(hipMemcpy(h_or.data(), wor, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
-        !hip_ok(hipMemcpy(h_and.data(), wand, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        !hip_ok(hipMemcpy(h_min.data(), wmin, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+        !hip_ok(hipMemcpy(h_max.data(), wmax, 8ULL * nw, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
     {
         if (rc != 0)
             XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort key pass launch failed");
@@ -1894,18 +1894,17 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
         (void)hipEventDestroy(e1);
         return -1;
     }
-    /* least significant digit first: the last word's low byte */
+    /* least significant digit first: the last word's low byte; a word
+     * sorts by the bytes of (greatest - least) only */
     int cur = 0;
     uint32_t passes = 0;
     for (int w = (int)nw - 1; w >= 0 && rc == 0; w--)
     {
-        const uint64_t diff = h_or[w] ^ h_and[w];
-        for (uint32_t shift = 0; shift < 64 && rc == 0; shift += 8)
+        const uint64_t range = h_max[w] - h_min[w];
+        for (uint32_t shift = 0; shift < 64 && (range >> shift) != 0 && rc == 0; shift += 8)
         {
-            if (((diff >> shift) & 0xffU) == 0)
-                continue;
             rc = xpu_launch_sort_pass(W[cur], I[cur], W[cur ^ 1], I[cur ^ 1], n, nw, (uint32_t)w,
-                                      shift, counts, nullptr);
+                                      h_min[w], shift, counts, nullptr);
             cur ^= 1;
             passes++;
         }
