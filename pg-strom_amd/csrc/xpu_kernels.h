@@ -295,7 +295,7 @@ typedef struct
     uint32_t    n;                  /* items */
     uint32_t    nkeys;
     uint32_t    nwords;
-    uint32_t    __pad;
+    uint32_t    max_anum;           /* last attribute any key reads */
     xsort_key   keys[XSORT_MAX_KEYS];
     uint64_t   *words;              /* [nwords][n] */
     uint32_t   *idx;                /* [n] item numbers */

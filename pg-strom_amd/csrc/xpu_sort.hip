@@ -7,7 +7,7 @@
  * through kern_fetch_minimal_tuple_attr and the key type's comparator
  * (cuda_gpusort.cu:15-399, gpu_service.c:5199-5345).  Here every tuple is
  * read once:
- *   key pass      one lane per item walks its tuple to each key attribute
+ *   key pass      one lane per item walks its tuple once to the key attributes
  *                 and writes the key as 64-bit words whose unsigned order is
  *                 the comparator's order (NULLs first or last, ASC or DESC,
  *                 and an empty aggregate state sorting as NULL), plus the
@@ -40,50 +40,6 @@
 #include <hip/hip_runtime.h>
 #include "pgstrom_xpu.h"
 #include "xpu_kernels.h"
-
-/* the attribute anum (1-based) of a minimal tuple, or nullptr when it is
- * NULL (kern_fetch_minimal_tuple_attr, xpu_common.cu) */
-__device__ static const char *
-sort_tuple_attr(const kern_data_store *kds, const kern_tupitem *t, int anum)
-{
-    const int natts = t->t_infomask2 & HEAP_NATTS_MASK;
-    if (anum < 1 || anum > natts)
-        return nullptr;
-    const bool hasnull = (t->t_infomask & HEAP_HASNULL) != 0;
-    const uint8_t *bits = (const uint8_t *)t + KERN_TUPITEM_T_BITS_OFFSET;
-    const char *base = (const char *)t;
-    uint32_t off = (uint32_t)t->t_hoff - MINIMAL_TUPLE_OFFSET;
-    for (int j = 0; j < anum; j++)
-    {
-        if (hasnull && !(bits[j >> 3] & (1 << (j & 7))))
-        {
-            if (j == anum - 1)
-                return nullptr;
-            continue;
-        }
-        const kern_colmeta *cm = &kds->colmeta[j];
-        const uint32_t al = cm->attalign > 0 ? (uint32_t)cm->attalign : 1U;
-        if (cm->attlen > 0 || base[off] == 0)
-            off = (off + al - 1) & ~(al - 1);
-        if (j == anum - 1)
-            return base + off;
-        if (cm->attlen > 0)
-            off += cm->attlen;
-        else
-        {
-            const uint8_t b = (uint8_t)base[off];
-            if (b & 0x01)
-                off += (b >> 1) & 0x7f;             /* 1-byte header */
-            else
-            {
-                uint32_t h;
-                __builtin_memcpy(&h, base + off, 4);
-                off += (h >> 2) & 0x3fffffffU;      /* 4-byte header */
-            }
-        }
-    }
-    return nullptr;
-}
 
 __device__ static inline uint64_t
 sort_load64(const char *p)
@@ -138,17 +94,63 @@ xpu_sort_keys_kernel(xsort_params sp)
     __syncthreads();
     const uint32_t i = blockIdx.x * XSORT_BLOCK + threadIdx.x;
     const bool valid = i < sp.n;
-    const kern_tupitem *t = valid ? (const kern_tupitem *)
-        ((const char *)sp.kds + sp.kds->length - sp.rowindex[i]) : nullptr;
+    /* one walk of the tuple finds every key attribute (the attribute walk
+     * of kern_fetch_minimal_tuple_attr: NULL bitmap, alignment of fixed
+     * values and of 4-byte varlena headers, 1- and 4-byte header lengths) */
+    const char *kptr[XSORT_MAX_KEYS];
+#pragma unroll
+    for (int k = 0; k < XSORT_MAX_KEYS; k++)
+        kptr[k] = nullptr;
+    if (valid)
+    {
+        const kern_tupitem *t = (const kern_tupitem *)
+            ((const char *)sp.kds + sp.kds->length - sp.rowindex[i]);
+        const int natts = t->t_infomask2 & HEAP_NATTS_MASK;
+        const int last = (int)sp.max_anum < natts ? (int)sp.max_anum : natts;
+        const bool hasnull = (t->t_infomask & HEAP_HASNULL) != 0;
+        const uint8_t *bits = (const uint8_t *)t + KERN_TUPITEM_T_BITS_OFFSET;
+        const char *base = (const char *)t;
+        uint32_t off = (uint32_t)t->t_hoff - MINIMAL_TUPLE_OFFSET;
+        for (int j = 0; j < last; j++)
+        {
+            if (hasnull && !(bits[j >> 3] & (1 << (j & 7))))
+                continue;                           /* NULL: kptr stays null */
+            const kern_colmeta *cm = &sp.kds->colmeta[j];
+            const uint32_t al = cm->attalign > 0 ? (uint32_t)cm->attalign : 1U;
+            if (cm->attlen > 0 || base[off] == 0)
+                off = (off + al - 1) & ~(al - 1);
+#pragma unroll
+            for (int k = 0; k < XSORT_MAX_KEYS; k++)
+                if (k < (int)sp.nkeys && sp.keys[k].anum == j + 1)
+                    kptr[k] = base + off;
+            if (cm->attlen > 0)
+                off += cm->attlen;
+            else
+            {
+                const uint8_t b = (uint8_t)base[off];
+                if (b & 0x01)
+                    off += (b >> 1) & 0x7f;
+                else
+                {
+                    uint32_t h;
+                    __builtin_memcpy(&h, base + off, 4);
+                    off += (h >> 2) & 0x3fffffffU;
+                }
+            }
+        }
+    }
     /* keys are wave-uniform: each key's words are stored and folded into
      * the least / greatest values as they are made */
-    for (uint32_t k = 0; k < sp.nkeys; k++)
+#pragma unroll
+    for (int k = 0; k < XSORT_MAX_KEYS; k++)
     {
+        if (k >= (int)sp.nkeys)
+            break;
         const xsort_key K = sp.keys[k];
         uint64_t kw[3] = {0, 0, 0};
         if (valid)
         {
-            const char *p = sort_tuple_attr(sp.kds, t, K.anum);
+            const char *p = kptr[k];
             const uint64_t null_class = K.nulls_first ? 0 : 2;
             uint64_t cls = 1, v = 0, v2 = 0;
             switch (K.kind)

@@ -1873,7 +1873,10 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     sp.nkeys = (uint32_t)c.sort_keys.size();
     sp.nwords = nw;
     for (size_t k = 0; k < c.sort_keys.size(); k++)
+    {
         sp.keys[k] = c.sort_keys[k];
+        sp.max_anum = c.sort_keys[k].anum > sp.max_anum ? c.sort_keys[k].anum : sp.max_anum;
+    }
     sp.words = W[0];
     sp.idx = I[0];
     sp.wmin = wmin;
