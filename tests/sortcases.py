@@ -82,3 +82,36 @@ def group_keys_in_order(kds_final):
     """The group key of each row of a written-back final buffer, in row-index
     order."""
     return [vals[KEY - 1] for vals, _ in kds.parse_row_kds(kds_final)]
+
+
+# group keys of other types: (typname, numpy dtype, values) with the ascending
+# order of the distinct non-NULL values written out; row 0's key is NULL
+TYPED = [
+    ("int2", np.int16, [7, -300, 12, 7, 32767, -32768, 0]),
+    ("int8", np.int64, [2**40, -2**62, 5, -1, 2**40, 0, -(2**40)]),
+    ("float8", np.float64, [1.5, -2.25, 1e300, -1e-300, 0.0, -7.0, 1.5]),
+    ("float4", np.float32, [0.25, -0.5, 3.0e38, -3.0e38, 0.25, 1.0, -0.0]),
+    ("bool", np.bool_, [True, False, True, False, True, False, True]),
+]
+
+
+def typed_table(typname, dtype, values):
+    """One row per value plus a NULL-key row in front; v = position."""
+    n = len(values) + 1
+    key = np.zeros(n, dtype=dtype)
+    key[1:] = np.array(values, dtype=dtype)
+    knull = np.zeros(n, dtype=bool)
+    knull[0] = True
+    return [Column("key", typname, key, knull), Column("v", "int8", np.arange(n, dtype=np.int64)),
+            Column("w", "float8", np.ones(n)), Column("x", "int4", np.arange(n, dtype=np.int32))]
+
+
+def typed_expected(dtype, values, asc, nulls_first):
+    """Distinct keys (as stored in the column's type) in the comparator's
+    order (-0.0 and 0.0 compare equal)."""
+    distinct = []
+    for v in np.array(values, dtype=dtype).tolist():
+        if not any(v == d for d in distinct):
+            distinct.append(v)
+    distinct.sort(reverse=not asc)
+    return [None] + distinct if nulls_first else distinct + [None]

@@ -85,3 +85,40 @@ def test_gpu_sort_million_groups(gpu):
     assert st["sort_items"] == ngroups
     exp = run_oracle_groupby(blob, [src])
     assert S.group_keys_in_order(fin) == S.group_keys_in_order(exp)
+
+
+@pytest.mark.parametrize("typed", range(len(S.TYPED)))
+@pytest.mark.parametrize("asc, nulls_first", [(True, False), (False, True), (False, False)])
+def test_gpu_sort_key_types(gpu, typed, asc, nulls_first):
+    """Group keys of int2 / int8 / float8 / float4 / bool (negative values,
+    extremes, -0.0 beside 0.0), ASC and DESC, NULLs first and last: the
+    distinct keys in order, as the oracle."""
+    typname, dtype, values = S.TYPED[typed]
+    cols = S.typed_table(typname, dtype, values)
+    src = kds.build_arrow_kds(cols)
+    blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__VREF, S.KEY, typname, asc, nulls_first)])
+    fin, _, _ = _gpu_sorted(blob, [src])
+    got = [None if g is None else float(g) for g in S.group_keys_in_order(fin)]
+    exp = [None if e is None else float(e) for e in S.typed_expected(dtype, values, asc, nulls_first)]
+    assert got == exp
+    assert S.group_keys_in_order(run_oracle_groupby(blob, [src])) == S.group_keys_in_order(fin)
+
+
+@pytest.mark.parametrize("nulls_first", [False, True])
+def test_gpu_sort_null_states(gpu, nulls_first):
+    """Groups whose min(x) saw only NULL arguments carry an invalid state,
+    which sorts as NULL (cuda_gpusort.cu:66-93)."""
+    n = 40
+    key = np.arange(n, dtype=np.int32) % 8
+    xnull = (key % 3 == 0)
+    cols = [Column("key", "int4", key), Column("v", "int8", np.arange(n, dtype=np.int64)),
+            Column("w", "float8", np.ones(n)),
+            Column("x", "int4", (np.arange(n, dtype=np.int32) * 7) % 11 - 5, xnull)]
+    src = kds.build_arrow_kds(cols)
+    blob = S.sorted_query(cols, [S.min_key(True, nulls_first), S.key_vref()])
+    fin, _, _ = _gpu_sorted(blob, [src])
+    exp = run_oracle_groupby(blob, [src])
+    order = S.group_keys_in_order(fin)
+    assert order == S.group_keys_in_order(exp)
+    null_groups = sorted(int(k) for k in set(key[xnull]))
+    assert (order[:len(null_groups)] if nulls_first else order[-len(null_groups):]) == null_groups

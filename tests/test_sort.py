@@ -64,3 +64,17 @@ def test_sort_keys_compile():
     cols = S.tiny_table()
     for keys, limit, _ in S.KNOWN:
         assert _compile_error(S.sorted_query(cols, keys, limit)) is None
+
+
+@pytest.mark.parametrize("typed", range(len(S.TYPED)))
+@pytest.mark.parametrize("asc, nulls_first", [(True, False), (False, True), (False, False)])
+def test_oracle_sort_key_types(typed, asc, nulls_first):
+    """Group keys of int2 / int8 / float8 / float4 / bool, ASC and
+    DESC, NULLs first and last: the distinct keys in order."""
+    typname, dtype, values = S.TYPED[typed]
+    cols = S.typed_table(typname, dtype, values)
+    blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__VREF, S.KEY, typname, asc, nulls_first)])
+    got = S.group_keys_in_order(run_oracle_groupby(blob, [kds.build_arrow_kds(cols)]))
+    exp = S.typed_expected(dtype, values, asc, nulls_first)
+    assert [None if g is None else float(g) for g in got] == \
+        [None if e is None else float(e) for e in exp]
