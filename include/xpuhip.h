@@ -142,8 +142,9 @@ int     xpuhip_task_exec_device_tuples(xpuhip_session *sess,
  * A session with FuncOpCode__SortKeys (xpucode_gpusort_keydesc) gets its
  * row index sorted by those keys on the device first, and with a positive
  * gpusort_limit_count only that many leading rows (gpuservSortingFinalBuffer,
- * gpu_service.c:5199-5345); the tuples of the rows past the limit stay in
- * the written buffer's tail, unreferenced.  The device buffer itself is not
+ * gpu_service.c:5199-5345), their tuples packed as kern_buffer_simple_limit
+ * packs them (the size query returns the bound before packing; the written
+ * buffer's header gives its length).  The device buffer itself is not
  * reordered. */
 int64_t xpuhip_task_final(xpuhip_session *sess, kern_data_store *kds_final, size_t length,
                           kern_exec_results *results, kern_errorbuf *errbuf);

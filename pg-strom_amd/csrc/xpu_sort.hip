@@ -440,6 +440,52 @@ xpu_sort_permute_kernel(const uint64_t *rowindex, const uint32_t *idx, uint64_t 
         out[i] = rowindex[idx[i]];
 }
 
+/* LIMIT: the kept rows' tuples packed at the tail of stage (stage_len
+ * bytes), in sorted order, each MAXALIGNed (kern_buffer_simple_limit,
+ * cuda_gpusort.cu:1401-1454); ridx_out[j] is the tuple's distance from the
+ * stage's end and *used the bytes packed.  One block: LIMIT keeps few rows. */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_limit_kernel(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                      char *stage, uint64_t stage_len, uint64_t *ridx_out, uint64_t *used)
+{
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
+    const char *tail = (const char *)kds + kds->length;
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < nout; base += XSORT_BLOCK)
+    {
+        const uint32_t j = base + threadIdx.x;
+        const kern_tupitem *t = j < nout ? (const kern_tupitem *)(tail - ridx[j]) : nullptr;
+        const uint32_t sz = t ? ((t->t_len + 7U) & ~7U) : 0U;
+        uint32_t total;
+        const uint32_t ex = sort_block_scan(sz, wsum, &total);
+        if (t)
+        {
+            const uint64_t end_off = carry + ex + sz;
+            if (end_off <= stage_len)
+            {
+                uint64_t *dst = (uint64_t *)(stage + stage_len - end_off);
+                const uint64_t *src = (const uint64_t *)t;
+                for (uint32_t q = 0; q < sz / 8; q++)
+                    dst[q] = src[q];
+                ridx_out[j] = end_off;
+            }
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        *used = carry;
+}
+
+extern "C" int
+xpu_launch_sort_limit(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                      char *stage, uint64_t stage_len, uint64_t *ridx_out, uint64_t *used,
+                      void *stream)
+{
+    hipLaunchKernelGGL(xpu_sort_limit_kernel, dim3(1), dim3(XSORT_BLOCK), 0, (hipStream_t)stream,
+                       kds, ridx, nout, stage, stage_len, ridx_out, used);
+    return (int)hipGetLastError();
+}
+
 extern "C" int
 xpu_launch_sort_keys(const xsort_params *sp, void *stream)
 {

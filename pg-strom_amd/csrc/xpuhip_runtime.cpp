@@ -1830,14 +1830,20 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
  * sorted row index go to *d_out (in the session's work space) */
 static int
 sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
-           const uint64_t **d_out, kern_errorbuf *eb)
+           const uint64_t **d_out, uint64_t usage, const char **d_tail, uint64_t *used,
+           kern_errorbuf *eb)
 {
+    /* with a LIMIT the kept tuples are packed into a stage of at most the
+     * buffer's usage bytes (kern_buffer_simple_limit) */
+    const bool limited = nout < n;
+    const uint64_t stage_len = limited ? ((usage + 7) & ~7ULL) : 0;
     const xpu_compiled &c = s->comp;
     const uint32_t nw = c.sort_nwords;
     const uint64_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
     const size_t words_bytes = 8ULL * nw * n;
     const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * (ntiles + 1) + 16ULL * nw +
-                        8ULL * nout + 8 * 256;     /* 7 slices, each rounded up to 256 B */
+                        8ULL * nout + 8 * 256 +     /* 7 slices, each rounded up to 256 B */
+                        (limited ? stage_len + 8ULL * nout + 8 + 3 * 256 : 0);
     if (need > s->sort_cap)
     {
         if (s->d_sort)
@@ -1856,6 +1862,9 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     uint64_t *wmin = (uint64_t *)take(8ULL * nw);
     uint64_t *wmax = (uint64_t *)take(8ULL * nw);
     uint64_t *ridx_out = (uint64_t *)take(8ULL * nout);
+    char *stage = limited ? take(stage_len) : nullptr;
+    uint64_t *ridx_packed = limited ? (uint64_t *)take(8ULL * nout) : nullptr;
+    uint64_t *d_used = limited ? (uint64_t *)take(8) : nullptr;
     if ((size_t)(p - s->d_sort) > s->sort_cap)
     {
         XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort work space layout exceeds its allocation");
@@ -1914,6 +1923,9 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     }
     if (rc == 0)
         rc = xpu_launch_sort_permute(d_ridx, I[cur], ridx_out, nout, nullptr);
+    if (rc == 0 && limited)
+        rc = xpu_launch_sort_limit(s->d_kds_final, ridx_out, nout, stage, stage_len, ridx_packed,
+                                   d_used, nullptr);
     (void)hipEventRecord(e1, nullptr);
     if (rc != 0 || !hip_ok(hipEventSynchronize(e1), eb, "hipEventSynchronize"))
     {
@@ -1931,6 +1943,18 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     t_last_stats.sort_items = n;
     t_last_stats.sort_passes = passes;
     *d_out = ridx_out;
+    if (limited)
+    {
+        if (!hip_ok(hipMemcpy(used, d_used, 8, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+            return -1;
+        if (*used > stage_len)
+        {
+            XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort LIMIT: the kept tuples exceed the stage");
+            return -1;
+        }
+        *d_out = ridx_packed;
+        *d_tail = stage + stage_len - *used;
+    }
     return 0;
 }
 
@@ -1973,6 +1997,7 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
         return -1;
     }
     const uint64_t *d_ridx = (const uint64_t *)((char *)s->d_kds_final + head_len + 8 * nslots);
+    const char *d_tail = (const char *)s->d_kds_final + s->final_length - usage;
     t_last_stats.sort_ms = 0;
     t_last_stats.sort_items = 0;
     t_last_stats.sort_passes = 0;
@@ -1984,20 +2009,23 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
             return -1;
         }
         std::lock_guard<std::mutex> g(s->lock);
-        if (sort_final(s, (uint32_t)nitems, (uint32_t)nout, d_ridx, &d_ridx, eb) != 0)
+        if (sort_final(s, (uint32_t)nitems, (uint32_t)nout, d_ridx, &d_ridx, usage, &d_tail,
+                       &usage, eb) != 0)
             return -1;
+        need = head_len + 8 * nout + usage;     /* the packed kept tuples */
     }
     /* HASH -> ROW on the wire (gpu_service.c:2780-2818) */
     h->format = KDS_FORMAT_ROW;
     h->hash_nslots = 0;
     h->length = need;
     h->nitems = (uint32_t)nout;
+    h->usage = usage;
     memcpy(out, hbuf.data(), head_len);
     if (!hip_ok(hipMemcpy((char *)out + head_len, d_ridx,
                           8 * nout, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
         (usage > 0 &&
-         !hip_ok(hipMemcpy((char *)out + need - usage, (char *)s->d_kds_final + s->final_length - usage,
-                           usage, hipMemcpyDeviceToHost), eb, "hipMemcpy")))
+         !hip_ok(hipMemcpy((char *)out + need - usage, d_tail, usage, hipMemcpyDeviceToHost), eb,
+                 "hipMemcpy")))
         return -1;
     return (int64_t)need;
 }

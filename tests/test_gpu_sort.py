@@ -65,6 +65,11 @@ def test_gpu_sort_matches_oracle(gpu, jit, which):
     assert len(got_order) == (limit or len(got_order))
     assert st["sort_items"] > limit and st["sort_passes"] >= 1
     assert_groups_equal(fin, exp, C1_ACTIONS)
+    if limit:
+        # the kept rows' tuples are packed (kern_buffer_simple_limit)
+        hd, _ = kds.kds_colmeta(memoryview(bytes(fin)))
+        assert hd.length == abi.kds_head_length(hd.nr_colmeta) + 8 * hd.nitems + hd.usage
+        assert hd.usage <= 512 * limit
 
 
 def test_gpu_sort_million_groups(gpu):
