@@ -2398,7 +2398,32 @@ oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_l
             qsort(rowidx, kds_final->nitems, sizeof(uint64_t), sort_comp_rows);
         if (os->session->gpusort_limit_count > 0 &&
             (uint64_t)os->session->gpusort_limit_count < kds_final->nitems)
-            kds_final->nitems = (uint32_t)os->session->gpusort_limit_count;
+        {
+            /* kern_buffer_simple_limit (cuda_gpusort.cu:1401-1454): the kept
+             * tuples are packed at the buffer's new tail, each with its row
+             * id set to its sorted position (:1446) */
+            uint32_t nout = (uint32_t)os->session->gpusort_limit_count;
+            uint64_t kept = 0, pos = 0;
+            for (uint32_t j = 0; j < nout; j++)
+                kept += XPU_MAXALIGN(((kern_tupitem *)((char *)kds_final + need - rowidx[j]))->t_len);
+            char *stage = calloc(1, kept ? kept : 1);
+            for (uint32_t j = 0; j < nout; j++)
+            {
+                const kern_tupitem *t = (const kern_tupitem *)((char *)kds_final + need - rowidx[j]);
+                uint32_t sz = XPU_MAXALIGN(t->t_len);
+                pos += sz;
+                memcpy(stage + kept - pos, t, t->t_len);
+                memcpy(stage + kept - pos + t->t_len - ROWID_SIZE, &j, ROWID_SIZE);
+                rowidx[j] = pos;
+            }
+            uint64_t new_len = head_sz + 8ULL * nout + kept;
+            memcpy((char *)kds_final + new_len - kept, stage, kept);
+            free(stage);
+            kds_final->nitems = nout;
+            kds_final->usage = kept;
+            kds_final->length = new_len;
+            return (int64_t)new_len;
+        }
     }
     return (int64_t)need;
 }

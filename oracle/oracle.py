@@ -66,7 +66,7 @@ class OracleSession:
         got = lib().oracle_task_final(self.h, buf.ctypes.data, need, C.byref(self.err))
         if got < 0:
             raise RuntimeError(self.err.describe())
-        return buf
+        return buf[:got]        # a LIMIT packs the kept tuples into fewer bytes
 
     def close(self):
         if self.h:

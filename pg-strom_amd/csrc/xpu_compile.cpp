@@ -1322,6 +1322,14 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
                 case KSORT_KEY_KIND__PSUM_NUMERIC:
                     if (cm.attlen != -1)
                         FAIL("GPU-Sort aggregate key is not a partial state column");
+                    /* the numeric key's words are the raw int128 sum, which
+                     * orders states only at one scale: those of sum(int8)
+                     * (PSUM_INT64, weight 0); the reference rescales by the
+                     * weight first (cuda_gpusort.cu:205-222) */
+                    if (d.kind == KSORT_KEY_KIND__PSUM_NUMERIC &&
+                        (d.src_anum > (int)P->gb_nattrs ||
+                         P->aggs[d.src_anum - 1].action != KAGG_ACTION__PSUM_INT64))
+                        FAIL("GPU-Sort numeric key must be a sum(int8) state in this build");
                     sk.nwords = d.kind == KSORT_KEY_KIND__PSUM_NUMERIC ? 3 : 2;
                     break;
                 default:

@@ -310,9 +310,11 @@ int xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint6
                          void *stream);          /* counts: 256 * (ntiles + 1) */
 int xpu_launch_sort_permute(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out,
                             uint32_t n, void *stream);
-int xpu_launch_sort_limit(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
-                          char *stage, uint64_t stage_len, uint64_t *ridx_out, uint64_t *used,
-                          void *stream);
+int xpu_launch_sort_limit_size(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                               uint64_t *offs, uint64_t *used, void *stream);
+int xpu_launch_sort_limit_copy(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                               const uint64_t *offs, char *stage, uint64_t stage_len,
+                               uint64_t *ridx_out, void *stream);
 
 int xpu_launch_heap_count(const xheap_params *hp, void *stream);
 int xpu_launch_heap_deform(const xheap_params *hp, int ncu, void *stream);

@@ -440,13 +440,13 @@ xpu_sort_permute_kernel(const uint64_t *rowindex, const uint32_t *idx, uint64_t 
         out[i] = rowindex[idx[i]];
 }
 
-/* LIMIT: the kept rows' tuples packed at the tail of stage (stage_len
- * bytes), in sorted order, each MAXALIGNed (kern_buffer_simple_limit,
- * cuda_gpusort.cu:1401-1454); ridx_out[j] is the tuple's distance from the
- * stage's end and *used the bytes packed.  One block: LIMIT keeps few rows. */
+/* LIMIT, first step: the kept rows' tuple sizes (MAXALIGNed) in sorted
+ * order, as exclusive offsets offs[j] and their total *used, so the caller
+ * sizes the stage to exactly the kept bytes.  One block: a LIMIT keeps few
+ * rows, and the scan carries a 64-bit running total across rounds. */
 __global__ void __launch_bounds__(XSORT_BLOCK)
-xpu_sort_limit_kernel(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
-                      char *stage, uint64_t stage_len, uint64_t *ridx_out, uint64_t *used)
+xpu_sort_limit_size_kernel(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                           uint64_t *offs, uint64_t *used)
 {
     __shared__ uint32_t wsum[XSORT_BLOCK / 64];
     const char *tail = (const char *)kds + kds->length;
@@ -459,30 +459,62 @@ xpu_sort_limit_kernel(const kern_data_store *kds, const uint64_t *ridx, uint32_t
         uint32_t total;
         const uint32_t ex = sort_block_scan(sz, wsum, &total);
         if (t)
-        {
-            const uint64_t end_off = carry + ex + sz;
-            if (end_off <= stage_len)
-            {
-                uint64_t *dst = (uint64_t *)(stage + stage_len - end_off);
-                const uint64_t *src = (const uint64_t *)t;
-                for (uint32_t q = 0; q < sz / 8; q++)
-                    dst[q] = src[q];
-                ridx_out[j] = end_off;
-            }
-        }
+            offs[j] = carry + ex;
         carry += total;
     }
     if (threadIdx.x == 0)
         *used = carry;
 }
 
-extern "C" int
-xpu_launch_sort_limit(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
-                      char *stage, uint64_t stage_len, uint64_t *ridx_out, uint64_t *used,
-                      void *stream)
+/* LIMIT, second step: each kept tuple copied to the tail of the stage
+ * (stage_len bytes, the total of the first step) in sorted order, with its
+ * row id set to its sorted position, as kern_buffer_simple_limit packs them
+ * (cuda_gpusort.cu:1401-1454, KERN_TUPITEM_SET_ROWID at :1446);
+ * ridx_out[j] becomes the tuple's distance from the stage's end.  One lane
+ * per kept row, any number of blocks. */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_sort_limit_copy_kernel(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                           const uint64_t *offs, char *stage, uint64_t stage_len,
+                           uint64_t *ridx_out)
 {
-    hipLaunchKernelGGL(xpu_sort_limit_kernel, dim3(1), dim3(XSORT_BLOCK), 0, (hipStream_t)stream,
-                       kds, ridx, nout, stage, stage_len, ridx_out, used);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nout)
+        return;
+    const char *tail = (const char *)kds + kds->length;
+    const kern_tupitem *t = (const kern_tupitem *)(tail - ridx[j]);
+    const uint32_t t_len = t->t_len;
+    const uint32_t sz = (t_len + 7U) & ~7U;
+    const uint64_t end_off = offs[j] + sz;
+    if (end_off > stage_len || t_len < offsetof(kern_tupitem, t_bits) + ROWID_SIZE)
+        return;
+    uint64_t *dst = (uint64_t *)(stage + stage_len - end_off);
+    const uint64_t *src = (const uint64_t *)t;
+    for (uint32_t q = 0; q < sz / 8; q++)
+        dst[q] = src[q];
+    const uint32_t rowid = j;
+    __builtin_memcpy((char *)dst + t_len - ROWID_SIZE, &rowid, ROWID_SIZE);
+    ridx_out[j] = end_off;
+}
+
+extern "C" int
+xpu_launch_sort_limit_size(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                           uint64_t *offs, uint64_t *used, void *stream)
+{
+    hipLaunchKernelGGL(xpu_sort_limit_size_kernel, dim3(1), dim3(XSORT_BLOCK), 0,
+                       (hipStream_t)stream, kds, ridx, nout, offs, used);
+    return (int)hipGetLastError();
+}
+
+extern "C" int
+xpu_launch_sort_limit_copy(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
+                           const uint64_t *offs, char *stage, uint64_t stage_len,
+                           uint64_t *ridx_out, void *stream)
+{
+    if (nout == 0)
+        return 0;
+    hipLaunchKernelGGL(xpu_sort_limit_copy_kernel, dim3((nout + XSORT_BLOCK - 1) / XSORT_BLOCK),
+                       dim3(XSORT_BLOCK), 0, (hipStream_t)stream, kds, ridx, nout, offs, stage,
+                       stage_len, ridx_out);
     return (int)hipGetLastError();
 }
 

@@ -215,6 +215,8 @@ struct xpuhip_session
     bool            inner_ready = false;
     char           *d_sort = nullptr;       /* GPU-Sort work space of task_final */
     size_t          sort_cap = 0;
+    char           *d_sort_stage = nullptr; /* GPU-Sort LIMIT: the kept tuples, packed */
+    size_t          sort_stage_cap = 0;
     xpuhip_tuning   tuning;
     std::mutex      lock;
     std::vector<Scratch *> pool;
@@ -1833,17 +1835,17 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
            const uint64_t **d_out, uint64_t usage, const char **d_tail, uint64_t *used,
            kern_errorbuf *eb)
 {
-    /* with a LIMIT the kept tuples are packed into a stage of at most the
-     * buffer's usage bytes (kern_buffer_simple_limit) */
+    /* with a LIMIT the kept tuples are packed (kern_buffer_simple_limit)
+     * into a stage sized to their total, which a size pass computes first */
     const bool limited = nout < n;
-    const uint64_t stage_len = limited ? ((usage + 7) & ~7ULL) : 0;
+    (void)usage;
     const xpu_compiled &c = s->comp;
     const uint32_t nw = c.sort_nwords;
     const uint64_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
     const size_t words_bytes = 8ULL * nw * n;
     const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * (ntiles + 1) + 16ULL * nw +
                         8ULL * nout + 8 * 256 +     /* 7 slices, each rounded up to 256 B */
-                        (limited ? stage_len + 8ULL * nout + 8 + 3 * 256 : 0);
+                        (limited ? 16ULL * nout + 8 + 3 * 256 : 0);
     if (need > s->sort_cap)
     {
         if (s->d_sort)
@@ -1862,7 +1864,7 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     uint64_t *wmin = (uint64_t *)take(8ULL * nw);
     uint64_t *wmax = (uint64_t *)take(8ULL * nw);
     uint64_t *ridx_out = (uint64_t *)take(8ULL * nout);
-    char *stage = limited ? take(stage_len) : nullptr;
+    uint64_t *offs = limited ? (uint64_t *)take(8ULL * nout) : nullptr;
     uint64_t *ridx_packed = limited ? (uint64_t *)take(8ULL * nout) : nullptr;
     uint64_t *d_used = limited ? (uint64_t *)take(8) : nullptr;
     if ((size_t)(p - s->d_sort) > s->sort_cap)
@@ -1923,9 +1925,22 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     }
     if (rc == 0)
         rc = xpu_launch_sort_permute(d_ridx, I[cur], ridx_out, nout, nullptr);
+    uint64_t stage_len = 0;
     if (rc == 0 && limited)
-        rc = xpu_launch_sort_limit(s->d_kds_final, ridx_out, nout, stage, stage_len, ridx_packed,
-                                   d_used, nullptr);
+    {
+        rc = xpu_launch_sort_limit_size(s->d_kds_final, ridx_out, nout, offs, d_used, nullptr);
+        if (rc == 0 &&
+            (!hip_ok(hipMemcpy(&stage_len, d_used, 8, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+             !grow(&s->d_sort_stage, &s->sort_stage_cap, stage_len > 0 ? stage_len : 8, eb)))
+        {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return -1;
+        }
+        if (rc == 0)
+            rc = xpu_launch_sort_limit_copy(s->d_kds_final, ridx_out, nout, offs, s->d_sort_stage,
+                                            stage_len, ridx_packed, nullptr);
+    }
     (void)hipEventRecord(e1, nullptr);
     if (rc != 0 || !hip_ok(hipEventSynchronize(e1), eb, "hipEventSynchronize"))
     {
@@ -1945,15 +1960,9 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     *d_out = ridx_out;
     if (limited)
     {
-        if (!hip_ok(hipMemcpy(used, d_used, 8, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
-            return -1;
-        if (*used > stage_len)
-        {
-            XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort LIMIT: the kept tuples exceed the stage");
-            return -1;
-        }
+        *used = stage_len;
         *d_out = ridx_packed;
-        *d_tail = stage + stage_len - *used;
+        *d_tail = s->d_sort_stage;
     }
     return 0;
 }
@@ -1990,7 +1999,7 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
         res->final_total = need;
     }
     if (!out)
-        return (int64_t)need;
+        return (int64_t)need;       /* a bound: a LIMIT packs at most this */
     if (length < need)
     {
         XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "kds_final buffer is too small");
@@ -2013,6 +2022,13 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
                        &usage, eb) != 0)
             return -1;
         need = head_len + 8 * nout + usage;     /* the packed kept tuples */
+        if (res)
+        {
+            /* what the written buffer holds (gpu_service.c:5530-5540) */
+            res->final_usage = usage;
+            res->final_total = need;
+            res->final_sorting_msec = (uint32_t)t_last_stats.sort_ms;   /* whole ms, as the reference */
+        }
     }
     /* HASH -> ROW on the wire (gpu_service.c:2780-2818) */
     h->format = KDS_FORMAT_ROW;
@@ -2062,6 +2078,7 @@ xpuhip_session_close(xpuhip_session *s)
     if (s->d_params) (void)hipFree(s->d_params);
     if (s->d_xids) (void)hipFree(s->d_xids);
     if (s->d_sort) (void)hipFree(s->d_sort);
+    if (s->d_sort_stage) (void)hipFree(s->d_sort_stage);
     for (int d = 0; d <= XP_MAX_DEPTH; d++)
         if (s->d_inner_own[d]) (void)hipFree(s->d_inner_own[d]);
     delete s;

@@ -289,7 +289,7 @@ class Session:
         got = lib().xpuhip_task_final(self.h, buf.ctypes.data, need, None, C.byref(self.err))
         if got < 0:
             raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
-        return buf
+        return buf[:got]        # a LIMIT packs the kept tuples into fewer bytes
 
     def close(self):
         if getattr(self, "h", None):

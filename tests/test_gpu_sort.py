@@ -70,6 +70,10 @@ def test_gpu_sort_matches_oracle(gpu, jit, which):
         hd, _ = kds.kds_colmeta(memoryview(bytes(fin)))
         assert hd.length == abi.kds_head_length(hd.nr_colmeta) + 8 * hd.nitems + hd.usage
         assert hd.usage <= 512 * limit
+        # each packed tuple's row id is its sorted position (KERN_TUPITEM_SET_ROWID,
+        # cuda_gpusort.cu:1446)
+        rows = kds.parse_row_kds(fin)
+        assert [rowid for _, rowid in rows] == list(range(len(rows)))
 
 
 def test_gpu_sort_million_groups(gpu):

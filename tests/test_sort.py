@@ -29,6 +29,28 @@ def test_oracle_sort_limit_keeps_leading_rows():
         assert got == full[:n]
 
 
+def test_oracle_sort_limit_packs_rows():
+    """LIMIT packs the kept tuples (kern_buffer_simple_limit,
+    cuda_gpusort.cu:1401-1454): the buffer holds only them, and each row id
+    is its sorted position (:1446)."""
+    cols = S.tiny_table()
+    fin = run_oracle_groupby(S.sorted_query(cols, [S.avg_key(False), S.key_vref()], limit=3),
+                             [kds.build_arrow_kds(cols)])
+    hd, _ = kds.kds_colmeta(memoryview(bytes(fin)))
+    assert hd.nitems == 3 and len(fin) == hd.length
+    assert hd.length == abi.kds_head_length(hd.nr_colmeta) + 8 * 3 + hd.usage
+    assert [rowid for _, rowid in kds.parse_row_kds(fin)] == [0, 1, 2]
+
+
+def test_sort_numeric_key_needs_int8_sum():
+    """A numeric sort key is accepted only over a sum(int8) state (weight
+    0), whose raw int128 sums order correctly."""
+    cols = S.tiny_table()
+    blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__PSUM_NUMERIC, S.PAVG, None, True, False)])
+    err = _compile_error(blob)
+    assert err is not None and "numeric key must be a sum(int8) state" in err
+
+
 def _compile_error(blob):
     rc, err = xpuhip.session_check(blob)
     return None if rc == 0 else err.message.decode()
