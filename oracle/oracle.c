@@ -33,7 +33,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "../pg-strom_amd/csrc/xpu_opcodes_table.h"
+#include "oracle_funcs.h"
 
 #define OMAX_SLOTS  256
 #define OMAX_DEPTH  16
@@ -118,7 +118,7 @@ typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, res;
 #define T_text      TypeOpCode__text
 #define OFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) \
     { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##RES },
-static const ofunc ofunc_table[] = { XPU_FUNCTION_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0 } };
+static const ofunc ofunc_table[] = { OFUNC_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0 } };
 
 static const ofunc *
 lookup_func(int opcode)
@@ -386,8 +386,9 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                         v = x / y;
                         break;
                     case O_mod:
+                        /* "<type>mod : division by zero" (xpu_basetype.cu:761) */
                         if (y == 0)
-                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s: division by zero", f->name);
+                            return oerror(c, ERRCODE_DEVICE_ERROR, "%s : division by zero", f->name);
                         v = (y == -1) ? 0 : x % y;
                         break;
                 }
@@ -472,8 +473,17 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                 case O_and: v = x & y; break;
                 case O_or:  v = x | y; break;
                 case O_xor: v = x ^ y; break;
-                case O_shl: v = (int64_t)((uint64_t)x << (y & (bits - 1))); break;
-                case O_shr: v = x >> (y & (bits - 1)); break;
+                /* C's x << y / x >> y of xpu_basetype.cu:858-866 as the host
+                 * CPU runs it (PostgreSQL's int2shl etc. on x86-64): int1 /
+                 * int2 / int4 are shifted as 32-bit ints by y mod 32, int8 by
+                 * y mod 64, then cut to the result width */
+                case O_shl:
+                    v = bits == 64 ? (int64_t)((uint64_t)x << (y & 63))
+                                   : (int64_t)(int32_t)((uint32_t)x << (y & 31));
+                    break;
+                case O_shr:
+                    v = bits == 64 ? x >> (y & 63) : (int64_t)((int32_t)x >> (y & 31));
+                    break;
             }
             /* wrap into the result width */
             if (bits == 8) v = (int8_t)v;
@@ -485,6 +495,14 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
         case K_cast:
         {
             int s = f->t0, d = f->res;
+            /* "<src> to <dst>: out of range" (PG_SIMPLE_TYPECAST_TEMPLATE,
+             * xpu_basetype.cu:343) */
+            char cname[64];
+            const char *sep = strstr(f->name, "_to_");
+            if (sep)
+                snprintf(cname, sizeof(cname), "%.*s to %s", (int)(sep - f->name), f->name, sep + 4);
+            else
+                snprintf(cname, sizeof(cname), "%s", f->name);
             if (d == T_bool)
             {
                 r->i = a[0].i != 0;
@@ -497,7 +515,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                 if (is_int_type(s))
                 {
                     if (a[0].i < lo || a[0].i > hi)
-                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", cname);
                     r->i = a[0].i;
                     return true;
                 }
@@ -506,7 +524,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                     float v = rintf(a[0].f);
                     if (isnan(a[0].f) || v < (float)lo || v > (float)hi ||
                         (d == T_int8 && v >= 9223372036854775808.0f))
-                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", cname);
                     r->i = (int64_t)v;
                     return true;
                 }
@@ -514,7 +532,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                     double v = rint(a[0].d);
                     if (isnan(a[0].d) || v < (double)lo || v > (double)hi ||
                         (d == T_int8 && v >= 9223372036854775808.0))
-                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                        return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", cname);
                     r->i = (int64_t)v;
                     return true;
                 }
@@ -525,7 +543,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
                 double x = a[0].d;
                 if (isinf(x) || isnan(x) || x < -3.40282346638528859812e+38 ||
                     x > 3.40282346638528859812e+38)
-                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", f->name);
+                    return oerror(c, ERRCODE_DEVICE_ERROR, "%s: out of range", cname);
                 r->f = (float)x;
             }
             else if (d == T_float4)
@@ -1250,6 +1268,67 @@ oracle_session_close(oracle_session *os)
     free(os->gt.slots);
     free(os->session);
     free(os);
+}
+
+/* ------------------------------------------------------------------
+ * One expression over variable slots, without a session: the checker of
+ * each device function against the reference's golden vectors.  Slot i has
+ * type code slot_types[i] and the PostgreSQL datum image at slot_data +
+ * slot_offs[i] (SQL NULL if negative).  Returns 0 with the result's image
+ * (*res_len = -1 for NULL), or 1 with the error code and message.
+ * ------------------------------------------------------------------ */
+int
+oracle_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
+                 int nslots, const int32_t *slot_types, const uint8_t *slot_data,
+                 const int64_t *slot_offs, int32_t res_type, uint8_t *res_image, int32_t res_cap,
+                 int32_t *res_len, uint32_t *errcode, char *errmsg, int errlen)
+{
+    static const kern_session_info empty_session;
+    uint64_t *ebuf = calloc(kexp_len / 8 + 2, 8);
+    octx *c = calloc(1, sizeof(octx));
+    od r;
+    int rc = 0;
+
+    memcpy(ebuf, kexp_blob, kexp_len);
+    c->session = &empty_session;
+    c->compare_nulls = compare_nulls != 0;
+    c->nslots = nslots;
+    *errcode = 0;
+    for (int i = 0; i < nslots && i < OMAX_SLOTS; i++)
+    {
+        if (slot_offs[i] < 0)
+            c->slots[i].isnull = true;
+        else if (!datum_from_image(c, slot_types[i], (const char *)slot_data + slot_offs[i],
+                                   &c->slots[i]))
+        {
+            rc = -1;
+            goto out;
+        }
+    }
+    if (!eval(c, (const kern_expression *)ebuf, &r))
+    {
+        *errcode = c->errcode;
+        snprintf(errmsg, errlen, "%s", c->errmsg);
+        rc = 1;
+        goto out;
+    }
+    if (r.isnull)
+        *res_len = -1;
+    else
+    {
+        int n = datum_write(res_type, &r, NULL);
+        if (n < 0 || n > res_cap)
+        {
+            rc = -1;
+            goto out;
+        }
+        datum_write(res_type, &r, (char *)res_image);
+        *res_len = n;
+    }
+out:
+    free(c);
+    free(ebuf);
+    return rc;
 }
 
 static void

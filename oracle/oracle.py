@@ -26,6 +26,10 @@ def lib():
         L.oracle_session_close.argtypes = [C.c_void_p]
         L.oracle_hash_any.restype = C.c_uint32
         L.oracle_hash_any.argtypes = [C.c_void_p, C.c_int]
+        vp = C.c_void_p
+        L.oracle_eval_expr.restype = C.c_int
+        L.oracle_eval_expr.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp, vp, C.c_int32, vp,
+                                       C.c_int32, vp, vp, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -80,3 +84,35 @@ class OracleSession:
 def hash_any(data):
     b = bytes(data)
     return lib().oracle_hash_any(C.c_char_p(b), len(b))
+
+
+def eval_expr(blob, slot_type_codes, images, res_type_code, compare_nulls=0):
+    """One expression over variable slots (oracle_eval_expr): returns the
+    result's datum image, None for NULL, or ("E", errcode, message)."""
+    n = len(slot_type_codes)
+    types = (C.c_int32 * max(n, 1))(*slot_type_codes)
+    data = bytearray()
+    offs = []
+    for img in images:
+        if img is None:
+            offs.append(-1)
+            continue
+        data += b"\0" * ((-len(data)) % 8)
+        offs.append(len(data))
+        data += img
+    data += b"\0" * 16
+    offs = (C.c_int64 * max(n, 1))(*offs)
+    dbuf = C.create_string_buffer(bytes(data), len(data))
+    rl, ec = C.c_int32(), C.c_uint32()
+    out = C.create_string_buffer(8192)
+    msg = C.create_string_buffer(256)
+    b = bytes(blob)
+    rc = lib().oracle_eval_expr(b, len(b), compare_nulls, n, types, dbuf, offs, res_type_code, out,
+                                8192, C.byref(rl), C.byref(ec), msg, 256)
+    if rc < 0:
+        raise RuntimeError("oracle cannot evaluate this expression")
+    if rc == 1:
+        return ("E", ec.value, msg.value.decode())
+    if rl.value < 0:
+        return None
+    return out.raw[:rl.value]

@@ -321,3 +321,162 @@ ref_heap_deform(const void *kds, const void *htup, int nload, const int16_t *res
     }
     return 0;
 }
+
+/*
+ * The reference's device-function catalog as the reference's own opcode list
+ * declares it (xpu_opcodes.h): device name -> opcode number, SQL name and
+ * argument types (FUNC_OPCODE's FUNC_ARGS, e.g. "int4/int8").
+ */
+static std::string g_catalog;
+
+extern "C" const char *
+ref_func_catalog_json(void)
+{
+    g_catalog = "{";
+    char buf[512];
+#define FUNC_OPCODE(SQL,ARGS,FLAGS,DEV,COST,EXT)                               \
+    snprintf(buf, sizeof(buf), "%s\"%s\": {\"opcode\": %d, \"sql\": \"%s\", \"args\": \"%s\"}", \
+             g_catalog.size() > 1 ? ",\n " : "", #DEV, (int)FuncOpCode__##DEV, #SQL, #ARGS); \
+    g_catalog += buf;
+#define DEVONLY_FUNC_OPCODE(RET,DEV,ARGS,FLAGS,COST)                            \
+    snprintf(buf, sizeof(buf), "%s\"%s\": {\"opcode\": %d, \"sql\": \"\", \"args\": \"%s\", \"ret\": \"%s\"}", \
+             g_catalog.size() > 1 ? ",\n " : "", #DEV, (int)FuncOpCode__##DEV, #ARGS, #RET); \
+    g_catalog += buf;
+#include "xpu_opcodes.h"
+#undef FUNC_OPCODE
+#undef DEVONLY_FUNC_OPCODE
+    g_catalog += "}\n";
+    return g_catalog.c_str();
+}
+
+static const xpu_datum_operators *
+ref_type_ops(int type_code)
+{
+    for (int i = 0; builtin_xpu_types_catalog[i].type_opcode != TypeOpCode__Invalid; i++)
+        if ((int)builtin_xpu_types_catalog[i].type_opcode == type_code)
+            return builtin_xpu_types_catalog[i].type_ops;
+    return NULL;
+}
+
+static xpu_function_t
+ref_func_dptr(int opcode)
+{
+    for (int i = 0; builtin_xpu_functions_catalog[i].func_opcode != FuncOpCode__Invalid; i++)
+        if ((int)builtin_xpu_functions_catalog[i].func_opcode == opcode)
+            return builtin_xpu_functions_catalog[i].func_dptr;
+    return NULL;
+}
+
+/* the service's pointer resolution (gpu_service.c:3156-3255) on one tree */
+static bool
+ref_resolve(kern_expression *kexp)
+{
+    kexp->fn_dptr = ref_func_dptr(kexp->opcode);
+    if (!kexp->fn_dptr)
+        return false;
+    kexp->expr_ops = ref_type_ops(kexp->exptype);
+    if (kexp->opcode == FuncOpCode__CaseWhenExpr)
+    {
+        if (kexp->u.casewhen.case_comp &&
+            !ref_resolve((kern_expression *)((char *)kexp + kexp->u.casewhen.case_comp)))
+            return false;
+        if (kexp->u.casewhen.case_else &&
+            !ref_resolve((kern_expression *)((char *)kexp + kexp->u.casewhen.case_else)))
+            return false;
+    }
+    if (kexp->opcode == FuncOpCode__HashedArrayOp)
+        kexp->u.haop.elem_ops = ref_type_ops(kexp->u.haop.elem_type);
+    kern_expression *karg = KEXP_FIRST_ARG(kexp);
+    for (int i = 0; i < kexp->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+        if (!ref_resolve(karg))
+            return false;
+    return true;
+}
+
+/*
+ * Evaluates one kern_expression with the reference's own device functions
+ * (EXEC_KERN_EXPRESSION, xpu_common.h:2694) over nslots variable slots.
+ * Slot i has type code slot_types[i]; its value is the PostgreSQL datum
+ * image at slot_data + slot_offs[i] (read with the type's
+ * xpu_datum_heap_read), or SQL NULL if slot_offs[i] < 0.  On success
+ * returns 0 with *res_type the result's type code and the result's datum
+ * image (xpu_datum_write) in res_image, *res_len its length or -1 for NULL.
+ * If the reference raises an error, returns 1 with errcode and message.
+ * Returns -1 for a tree it cannot resolve.
+ */
+extern "C" int
+ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
+              int nslots, const int32_t *slot_types, const uint8_t *slot_data,
+              const int64_t *slot_offs,
+              int32_t *res_type, uint8_t *res_image, int32_t res_cap, int32_t *res_len,
+              uint32_t *errcode, char *errmsg, int errlen)
+{
+    std::vector<uint64_t> ebuf((kexp_len + 7) / 8 + 1, 0);
+    memcpy(ebuf.data(), kexp_blob, kexp_len);
+    kern_expression *kexp = (kern_expression *)ebuf.data();
+    if (!ref_resolve(kexp))
+        return -1;
+    std::vector<uint64_t> sbuf(4096 / 8, 0);      /* a session with no params / encoding */
+    const size_t vl = 1 << 16;
+    std::vector<uint64_t> kbuf((offsetof(kern_context, vlbuf) + vl) / 8 + 1, 0);
+    kern_context *kcxt = (kern_context *)kbuf.data();
+    kcxt->session = (kern_session_info *)sbuf.data();
+    kcxt->kmode_compare_nulls = compare_nulls != 0;
+    kcxt->vlpos = kcxt->vlbuf;
+    kcxt->vlend = kcxt->vlbuf + vl;
+    std::vector<std::vector<uint64_t>> store(nslots > 0 ? nslots : 1);
+    std::vector<xpu_datum_t *> slots(nslots > 0 ? nslots : 1);
+    std::vector<kern_varslot_desc> desc(nslots > 0 ? nslots : 1);
+    for (int i = 0; i < nslots; i++)
+    {
+        const xpu_datum_operators *ops = ref_type_ops(slot_types[i]);
+        if (!ops)
+            return -1;
+        store[i].assign(16, 0);
+        slots[i] = (xpu_datum_t *)store[i].data();
+        memset(&desc[i], 0, sizeof(kern_varslot_desc));
+        desc[i].vs_type_code = (TypeOpCode)slot_types[i];
+        desc[i].vs_typbyval = ops->xpu_type_byval;
+        desc[i].vs_typalign = ops->xpu_type_align;
+        desc[i].vs_typlen = ops->xpu_type_length;
+        desc[i].vs_offset = -1;
+        desc[i].vs_ops = ops;
+        if (slot_offs[i] < 0)
+            slots[i]->expr_ops = NULL;
+        else if (!ops->xpu_datum_heap_read(kcxt, slot_data + slot_offs[i], slots[i]))
+            return -1;
+    }
+    kcxt->kvars_slot = slots.data();
+    kcxt->kvars_desc = desc.data();
+    kcxt->kvars_nslots = nslots;
+    kcxt->kvars_nrooms = nslots;
+    std::vector<uint64_t> rbuf(16, 0);
+    xpu_datum_t *r = (xpu_datum_t *)rbuf.data();
+    *errcode = 0;
+    if (!EXEC_KERN_EXPRESSION(kcxt, kexp, r))
+    {
+        *errcode = kcxt->errcode;
+        snprintf(errmsg, errlen, "%s", kcxt->error_message ? kcxt->error_message : "");
+        return 1;
+    }
+    if (XPU_DATUM_ISNULL(r))
+    {
+        *res_type = kexp->exptype;
+        *res_len = -1;
+        return 0;
+    }
+    const xpu_datum_operators *rops = r->expr_ops;
+    *res_type = rops->xpu_type_code;
+    kern_colmeta cm;
+    memset(&cm, 0, sizeof(cm));
+    cm.attbyval = rops->xpu_type_byval;
+    cm.attalign = rops->xpu_type_align;
+    cm.attlen = rops->xpu_type_length;
+    cm.atttypmod = -1;
+    int n = rops->xpu_datum_write(kcxt, NULL, &cm, r);
+    if (n < 0 || n > res_cap)
+        return -1;
+    rops->xpu_datum_write(kcxt, (char *)res_image, &cm, r);
+    *res_len = n;
+    return 0;
+}

@@ -593,7 +593,9 @@ struct Compiler
                     static const int iops[] = {0, 0, 0, 0, 0, 0, 0, XO_ADD_I, XO_SUB_I, XO_MUL_I,
                                                XO_DIV_I, XO_MOD_I};
                     bool divlike = (f->op == O_div || f->op == O_mod);
-                    int m = msg(fname + (divlike ? ": division by zero" : ": value out of range"));
+                    /* message texts of xpu_basetype.cu:670, :761 */
+                    int m = msg(fname + (f->op == O_mod ? " : division by zero" :
+                                         divlike ? ": division by zero" : ": value out of range"));
                     int m2 = msg(fname + ": value out of range");
                     emit(iops[f->op], r, a[0].r, a[1].r, guard, int_class(f->res),
                          (int64_t)m | ((int64_t)m2 << 16));
@@ -641,7 +643,12 @@ struct Compiler
             case K_cast:
             {
                 int sd = a[0].dom, dd = type_domain(f->res);
-                int m = msg(fname + ": out of range");
+                /* "<src> to <dst>: out of range" (xpu_basetype.cu:343) */
+                std::string cname = fname;
+                size_t sep = cname.find("_to_");
+                if (sep != std::string::npos)
+                    cname = cname.substr(0, sep) + " to " + cname.substr(sep + 4);
+                int m = msg(cname + ": out of range");
                 if (f->res == TypeOpCode__bool)
                     emit(XO_I_BOOL, r, a[0].r);
                 else if (sd == XD_INT && dd == XD_INT)

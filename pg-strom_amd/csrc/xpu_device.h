@@ -649,8 +649,16 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     case XO_AND_I: v = x & y; break;
                     case XO_OR_I:  v = x | y; break;
                     case XO_XOR_I: v = x ^ y; break;
-                    case XO_SHL_I: v = (int64_t)((uint64_t)x << (y & (bits - 1))); break;
-                    default:       v = x >> (y & (bits - 1)); break;
+                    /* C's shifts of xpu_basetype.cu:858-866 as PostgreSQL's
+                     * executor runs them: int1/int2/int4 promoted to 32-bit
+                     * and shifted by y mod 32, int8 by y mod 64 */
+                    case XO_SHL_I:
+                        v = bits == 64 ? (int64_t)((uint64_t)x << (y & 63))
+                                       : (int64_t)(int32_t)((uint32_t)x << (y & 31));
+                        break;
+                    default:
+                        v = bits == 64 ? x >> (y & 63) : (int64_t)((int32_t)x >> (y & 31));
+                        break;
                 }
                 L.R[d] = wrap_class(o.aux, v);
                 SETNULL(L, d, n);

@@ -121,6 +121,14 @@ class Coalesce(Expr):
         self.args = list(args)
 
 
+class HashValue(Expr):
+    """HashValue over its arguments (pgfn_HashValue, xpu_common.cu:1363-1398)."""
+
+    def __init__(self, *args):
+        self.args = list(args)
+        self.typname = "int4"
+
+
 def And(*a):
     return BoolOp("and", *a)
 
@@ -178,6 +186,8 @@ def emit_expr(buf, e, flags=0):
         return _emit_with_args(buf, tc, SPECIAL["BoolTestExpr_" + e.kind], [e.arg], flags)
     if isinstance(e, Coalesce):
         return _emit_with_args(buf, tc, SPECIAL["CoalesceExpr"], e.args, flags)
+    if isinstance(e, HashValue):
+        return _emit_with_args(buf, tc, SPECIAL["HashValue"], e.args, flags)
     if isinstance(e, SaveExpr):
         pos = _append_aligned(buf, _kexp_head(tc, SPECIAL["SaveExpr"], 1, 48, flags) +
                               struct.pack("<H6x", e.slot))

@@ -1,0 +1,110 @@
+"""Inputs of the device-function golden vectors (shared by the generator,
+gen_expr_vectors.py, and the tests that replay the vectors): per type a
+table of edge-case values as PostgreSQL datum images, and the expression
+templates over variable slots that exercise the constructs beyond single
+functions (SQL three-valued AND / OR / NOT, NullTest, BoolTest, COALESCE,
+HashValue, short-circuit guards)."""
+import struct
+
+NAN = float("nan")
+INF = float("inf")
+
+
+def _text(s):
+    b = s.encode() if isinstance(s, str) else bytes(s)
+    return struct.pack("<I", (len(b) + 4) << 2) + b
+
+
+VALUES = {
+    "bool": [struct.pack("<?", v) for v in (False, True)],
+    "int1": [struct.pack("<b", v) for v in (0, 1, -1, 5, -7, 11, 127, -128, 100, -100, 64)],
+    "int2": [struct.pack("<h", v) for v in (0, 1, -1, 7, -7, 32767, -32768, 181, 182, 300,
+                                            -300, 255, 128)],
+    "int4": [struct.pack("<i", v) for v in (0, 1, -1, 7, -7, 2**31 - 1, -2**31, 46340, 46341,
+                                            123456, -654321, 32767, 128, 500000)],
+    "int8": [struct.pack("<q", v) for v in (0, 1, -1, 7, -7, 2**63 - 1, -2**63, 3037000499,
+                                            3037000500, 10**12, -10**12, 2**31, 2**40 + 3)],
+    "float4": [struct.pack("<f", v) for v in (0.0, -0.0, 1.0, -1.0, 2.5, 0.1, 3.0e38, -3.0e38,
+                                              1e-38, INF, -INF, NAN, 1.5e10, 127.5, -32768.5,
+                                              2147483520.0, 9.2233715e18)],
+    "float8": [struct.pack("<d", v) for v in (0.0, -0.0, 1.0, -1.0, 2.5, 3.5, -2.5, 0.1, 1e308,
+                                              -1e308, 5e-324, INF, -INF, NAN, 2147483647.5,
+                                              -2147483648.5, 2147483647.4, 9.3e18,
+                                              -9.2233720368547758e18, 1e39, 127.5, 3.4028235e38)],
+    "text": [_text(s) for s in ("", "a", "abc", "abd", "ab", "ABC", "abc ", "b", "été",
+                                "zzzzzzzzzzzzzzzzzzzzzzzzzzzzzz", "a%c", "_b")],
+}
+
+# binary functions take the cross product of the first BINARY_N values of
+# each argument type (the full tables for unary functions)
+BINARY_N = {"bool": 2, "int1": 9, "int2": 10, "int4": 11, "int8": 11, "float4": 14,
+            "float8": 16, "text": 12}
+
+
+_INT_FMT = {"int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q"}
+
+
+def host_traps(name, types, imgs):
+    """True for the one input the reference's device code cannot run on a
+    host CPU: TYPEmod(MIN, -1) computes x % y directly (xpu_basetype.cu:
+    748-769), which traps on x86.  PostgreSQL's int4mod defines it as 0,
+    which this build and the oracle return (pinned by the closed form
+    x % -1 = 0 in the tests instead)."""
+    if not name.endswith("mod") or None in imgs:
+        return False
+    fx, fy = _INT_FMT[types[0]], _INT_FMT[types[1]]
+    x = struct.unpack(fx, imgs[0])[0]
+    y = struct.unpack(fy, imgs[1])[0]
+    bits = 8 * struct.calcsize(fx)
+    return y == -1 and x == -(1 << (bits - 1))
+
+
+def arg_tuples(types):
+    if len(types) == 1:
+        return [(i,) for i in range(len(VALUES[types[0]]))] + [(None,)]
+    a, b = types
+    out = [(i, j) for i in range(BINARY_N[a]) for j in range(BINARY_N[b])]
+    return out + [(None, 0), (0, None), (None, None)]
+
+
+def templates(cg):
+    """(name, expression over slots, slot types, compare_nulls) of the
+    composite constructs; each is evaluated over every combination of its
+    slots' values (bool: false / true / NULL; int4: a few values and NULL)."""
+    B = lambda i: cg.Var(i, "bool")
+    I = lambda i: cg.Var(i, "int4")
+    T = lambda i: cg.Var(i, "text")
+    out = [
+        ("and2", cg.And(B(0), B(1)), ["bool", "bool"], 0),
+        ("and3", cg.And(B(0), B(1), B(2)), ["bool", "bool", "bool"], 0),
+        ("or2", cg.Or(B(0), B(1)), ["bool", "bool"], 0),
+        ("or3", cg.Or(B(0), B(1), B(2)), ["bool", "bool", "bool"], 0),
+        ("not", cg.Not(B(0)), ["bool"], 0),
+        ("not_and_or", cg.Not(cg.Or(cg.And(B(0), B(1)), B(2))), ["bool", "bool", "bool"], 0),
+        ("isnull_int4", cg.NullTest(I(0), True), ["int4"], 0),
+        ("isnotnull_text", cg.NullTest(T(0), False), ["text"], 0),
+        ("coalesce3", cg.Coalesce("int4", I(0), I(1), I(2)), ["int4", "int4", "int4"], 0),
+        ("coalesce_text", cg.Coalesce("text", T(0), T(1)), ["text", "text"], 0),
+        ("hash_int4_text", cg.HashValue(I(0), T(1)), ["int4", "text"], 0),
+        ("eq_nulls", cg.Func("int4eq", I(0), I(1)), ["int4", "int4"], 1),
+        ("ne_nulls", cg.Func("int4ne", I(0), I(1)), ["int4", "int4"], 1),
+        ("text_eq_nulls", cg.Func("text_eq", T(0), T(1)), ["text", "text"], 1),
+        # AND stops at the first false: the division is evaluated only for
+        # rows whose first argument is not false
+        ("and_guard", cg.And(B(0), cg.Func("int4eq", cg.Func("int4div", I(1), I(2)), I(1))),
+         ["bool", "int4", "int4"], 0),
+        ("or_guard", cg.Or(B(0), cg.Func("int4eq", cg.Func("int4div", I(1), I(2)), I(1))),
+         ["bool", "int4", "int4"], 0),
+        ("nested_arith", cg.Func("int8mul", cg.Func("int4_to_int8", cg.Func("int4pl", I(0), I(1))),
+                                 cg.Func("int4_to_int8", I(2))), ["int4", "int4", "int4"], 0),
+    ]
+    for kind in cg.BoolTest.KINDS:
+        out.append(("booltest_" + kind, cg.BoolTest(kind, B(0)), ["bool"], 0))
+    return out
+
+
+TEMPLATE_VALUES = {
+    "bool": [struct.pack("<?", False), struct.pack("<?", True), None],
+    "int4": [struct.pack("<i", v) for v in (0, 1, -2, 7, 2**31 - 1)] + [None],
+    "text": [_text(s) for s in ("", "abc", "abd")] + [None],
+}
