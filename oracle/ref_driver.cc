@@ -480,3 +480,112 @@ ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
     *res_len = n;
     return 0;
 }
+
+/* kcxt with nslots variable slots loaded from datum images (see
+ * ref_eval_expr); false if a type is unknown */
+struct RefSlots
+{
+    std::vector<uint64_t> kbuf, sbuf;
+    std::vector<std::vector<uint64_t>> store;
+    std::vector<xpu_datum_t *> slots;
+    std::vector<kern_varslot_desc> desc;
+    kern_context *kcxt = nullptr;
+
+    bool load(int nslots, const int32_t *slot_types, const uint8_t *slot_data,
+              const int64_t *slot_offs)
+    {
+        const size_t vl = 1 << 16;
+        kbuf.assign((offsetof(kern_context, vlbuf) + vl) / 8 + 1, 0);
+        sbuf.assign(4096 / 8, 0);
+        kcxt = (kern_context *)kbuf.data();
+        kcxt->session = (kern_session_info *)sbuf.data();
+        kcxt->vlpos = kcxt->vlbuf;
+        kcxt->vlend = kcxt->vlbuf + vl;
+        int n = nslots > 0 ? nslots : 1;
+        store.assign(n, std::vector<uint64_t>(16, 0));
+        slots.assign(n, nullptr);
+        desc.assign(n, kern_varslot_desc());
+        for (int i = 0; i < nslots; i++)
+        {
+            const xpu_datum_operators *ops = ref_type_ops(slot_types[i]);
+            if (!ops)
+                return false;
+            slots[i] = (xpu_datum_t *)store[i].data();
+            memset(&desc[i], 0, sizeof(kern_varslot_desc));
+            desc[i].vs_type_code = (TypeOpCode)slot_types[i];
+            desc[i].vs_ops = ops;
+            if (slot_offs[i] < 0)
+                slots[i]->expr_ops = NULL;
+            else if (!ops->xpu_datum_heap_read(kcxt, slot_data + slot_offs[i], slots[i]))
+                return false;
+        }
+        kcxt->kvars_slot = slots.data();
+        kcxt->kvars_desc = desc.data();
+        kcxt->kvars_nslots = nslots;
+        kcxt->kvars_nrooms = nslots;
+        return true;
+    }
+};
+
+/*
+ * The reference's projection tuple former, kern_form_minimal_tuple
+ * (xpu_common.cu:1096-1209), over nattrs values: attribute j is slot j, its
+ * datum image at slot_data + slot_offs[j] (NULL if negative); kds_dst is the
+ * destination's KDS_FORMAT_ROW header with its colmeta.  Writes the
+ * kern_tupitem into titem (cap bytes) and returns its t_len, or -1.
+ */
+extern "C" int
+ref_form_tuple(const void *kds_dst, int nattrs, const int32_t *slot_types,
+               const uint8_t *slot_data, const int64_t *slot_offs, uint32_t rowid,
+               void *titem, int cap)
+{
+    RefSlots rs;
+    if (!rs.load(nattrs, slot_types, slot_data, slot_offs))
+        return -1;
+    size_t klen = offsetof(kern_expression, u.proj.slot_id) + 2 * nattrs + 8;
+    std::vector<uint64_t> ebuf(klen / 8 + 2, 0);
+    kern_expression *kexp = (kern_expression *)ebuf.data();
+    kexp->len = klen;
+    kexp->opcode = FuncOpCode__Projection;
+    kexp->u.proj.nattrs = nattrs;
+    for (int j = 0; j < nattrs; j++)
+        kexp->u.proj.slot_id[j] = j;
+    int sz = kern_form_minimal_tuple(rs.kcxt, kexp, (const kern_data_store *)kds_dst, NULL, rowid);
+    if (sz < 0 || sz > cap)
+        return -1;
+    memset(titem, 0, cap);
+    return kern_form_minimal_tuple(rs.kcxt, kexp, (const kern_data_store *)kds_dst,
+                                   (kern_tupitem *)titem, rowid);
+}
+
+/*
+ * Walks a KDS_FORMAT_HASH buffer the way the reference's probe does
+ * (KDS_HASH_FIRST_ITEM / KDS_HASH_NEXT_ITEM, xpu_common.h:1424-1451): for
+ * each hash value hashes[i], writes into rowids the row ids of the items on
+ * its chain whose t.hash equals it, in chain order, at most cap in total;
+ * counts[i] receives how many.  Returns the total, or -1 past cap.
+ */
+extern "C" int64_t
+ref_hash_walk(const void *kds_hash, int nhashes, const uint32_t *hashes, uint32_t *counts,
+              uint32_t *rowids, int64_t cap)
+{
+    const kern_data_store *kds = (const kern_data_store *)kds_hash;
+    int64_t total = 0;
+    for (int i = 0; i < nhashes; i++)
+    {
+        uint32_t c = 0;
+        for (kern_hashitem *hitem = KDS_HASH_FIRST_ITEM(kds, hashes[i]);
+             hitem != NULL;
+             hitem = KDS_HASH_NEXT_ITEM(kds, hitem->next))
+        {
+            if (hitem->t.hash != hashes[i])
+                continue;
+            if (total >= cap)
+                return -1;
+            rowids[total++] = KERN_TUPITEM_GET_ROWID(&hitem->t);
+            c++;
+        }
+        counts[i] = c;
+    }
+    return total;
+}
