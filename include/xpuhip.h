@@ -118,6 +118,24 @@ int     xpuhip_task_exec_device(xpuhip_session *sess,
                                 uint64_t *resume_pos,
                                 kern_exec_results *results, kern_errorbuf *errbuf);
 
+/* The same as xpuhip_task_exec_device for scan + projection and scan +
+ * group-by sessions (no joins), allocating nothing: no hipMalloc, hipFree or
+ * allocating library call.  Has no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid it).  The task scratch is one
+ * device array of 32 MiB held by each GPU for the life of the process: a
+ * projection chunk needs 16 bytes per 256 rows of it (larger chunks fail
+ * with 'E'); a group-by's spill and dump records take what fits, the rest
+ * merge into kds_final directly.  Runs the ahead-of-time kernels on the null
+ * stream and returns when they are done (xpuhip_last_kernel_ms reads 0).
+ * Calls from threads of one process serialise on a lock, since they share
+ * the array; calls from several processes each use their own copy. */
+int     xpuhip_task_exec_device_noalloc(xpuhip_session *sess,
+                                        const kern_data_store *d_kds_src, uint32_t src_nitems,
+                                        kern_data_store *d_kds_dst, size_t dst_length,
+                                        uint64_t *resume_pos,
+                                        kern_exec_results *results, kern_errorbuf *errbuf);
+
 /* The same on a heap-page chunk (KDS_FORMAT_BLOCK: kds_head->nitems pages of
  * BLCKSZ bytes with their line pointers, as relscan.c:413-742 builds it) or a
  * row chunk (KDS_FORMAT_ROW / KDS_FORMAT_HASH: kds_head->nitems row-index

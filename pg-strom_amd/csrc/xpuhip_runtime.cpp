@@ -36,6 +36,15 @@
 #define XERR(eb, code, msg) \
     xpu_set_error((eb), (code), "xpuhip_runtime.cpp", __LINE__, __func__, (msg))
 
+/* Task scratch of xpuhip_task_exec_device_noalloc: the task control block,
+ * then the projection's tile totals or the group-by's dump counters, dump
+ * records and spill records.  Each device holds its own copy for the life of
+ * the process.  Host code never passes this name to a kernel or a copy: it
+ * is a host-side stand-in there; the device address comes from
+ * hipGetSymbolAddress (g_scratch_ptr). */
+#define XPU_NOALLOC_SCRATCH_BYTES   (32ULL << 20)
+static __device__ __align__(256) char scratch[XPU_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 struct Device
@@ -49,9 +58,17 @@ struct Device
 
 Device      g_dev;
 std::mutex  g_dev_lock;
+
+/* device address of the no-alloc entry point's scratch array on each
+ * device (set once per device with the device current) */
+#define XPU_MAX_DEVICES         64
+char       *g_scratch_ptr[XPU_MAX_DEVICES] = {};
 thread_local double t_last_kernel_ms = 0.0;
 thread_local xpuhip_task_stats t_last_stats;
 
+
+bool hip_ok(hipError_t rc, kern_errorbuf *eb, const char *what);
+bool scratch_symbol_address(char **p, kern_errorbuf *eb);
 
 bool
 hip_ok(hipError_t rc, kern_errorbuf *eb, const char *what)
@@ -62,6 +79,23 @@ hip_ok(hipError_t rc, kern_errorbuf *eb, const char *what)
     snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(rc));
     XERR(eb, ERRCODE_DEVICE_ERROR, buf);
     return false;
+}
+
+/* the device address of this device's scratch array; the device must be
+ * current */
+bool
+scratch_symbol_address(char **p, kern_errorbuf *eb)
+{
+    void *ptr = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), eb, "hipGetSymbolAddress(scratch)"))
+        return false;
+    if (!ptr)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "hipGetSymbolAddress(scratch) returned no address");
+        return false;
+    }
+    *p = (char *)ptr;
+    return true;
 }
 
 int
@@ -87,6 +121,13 @@ ensure_device(kern_errorbuf *eb)
     g_dev.ncu = prop.multiProcessorCount;
     g_dev.lds_per_block = prop.sharedMemPerBlock;
     g_dev.name = prop.gcnArchName;
+    if (g_dev.id < 0 || g_dev.id >= XPU_MAX_DEVICES)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "device number out of range");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!g_scratch_ptr[g_dev.id] && !scratch_symbol_address(&g_scratch_ptr[g_dev.id], eb))
+        return ERRCODE_DEVICE_ERROR;
     g_dev.ready = true;
     return 0;
 }
@@ -1752,6 +1793,224 @@ xpuhip_task_exec_device(xpuhip_session *s, const kern_data_store *d_kds_src, uin
         rc = task_exec_common(s, x, d_kds_src, src_nitems, d_kds_dst, dst_length, resume_pos, res, eb);
     scratch_put(s, x);
     return rc;
+}
+
+/* XpuTaskExec on a device-resident Arrow chunk that allocates nothing: for
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops).  Scan + projection and scan + group-by sessions
+ * without joins; the ahead-of-time kernels, on the null stream, with the
+ * task scratch in the file-scope array above.  Calls from threads of one
+ * process serialise on a lock (they share the device's array, which keeps
+ * the last call's contents; everything a call reads is cleared first). */
+namespace {
+std::mutex g_noalloc_lock;
+}
+
+extern "C" int
+xpuhip_task_exec_device_noalloc(xpuhip_session *s, const kern_data_store *d_src, uint32_t nitems,
+                                kern_data_store *d_dst, size_t dst_length, uint64_t *resume_pos,
+                                kern_exec_results *res, kern_errorbuf *eb)
+{
+    if (!s)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no session");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (ensure_device(eb) != 0)
+        return ERRCODE_DEVICE_ERROR;
+    char *base = g_scratch_ptr[g_dev.id];
+    const xplan *P = &s->comp.plan;
+    const bool preagg = s->comp.has_groupby;
+    if (!base)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no-alloc scratch has no device address");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (P->nrels > 0 || s->comp.pinned_hash || (preagg && !uses_scan_agg(s)))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR,
+             "the no-alloc entry point runs scan + projection and scan + group-by sessions only");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg && (!d_dst || dst_length < s->comp.kds_dst_head.size() + 64 ||
+                    dst_length % XPU_MAXIMUM_ALIGNOF != 0 || dst_length >= (32ULL << 30)))
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "projection needs a MAXALIGN'ed kds_dst buffer below 32GB");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    uint64_t row_begin = resume_pos ? *resume_pos : 0;
+    if (res)
+    {
+        memset(res, 0, sizeof(kern_exec_results));
+        res->num_rels = 0;
+    }
+    xpuhip_tuning t = s->tuning;
+    uint32_t block = t.block_size ? t.block_size : 256;
+    if (block > XP_BLOCK_MAX)
+        block = XP_BLOCK_MAX;
+    uint32_t rpt = preagg ? (t.rows_per_thread ? t.rows_per_thread : 1) : 1;
+    rpt = rpt >= 4 ? 4 : rpt >= 2 ? 2 : 1;
+    const uint64_t nrows = row_begin < nitems ? nitems - row_begin : 0;
+    const uint32_t rows_per_tile = block * rpt;
+    const uint64_t ntiles = (nrows + rows_per_tile - 1) / rows_per_tile;
+    /* scratch layout: control block, then per task kind */
+    const size_t ctl_bytes = (sizeof(xpu_task_ctl) + 255) & ~(size_t)255;
+    const size_t avail = XPU_NOALLOC_SCRATCH_BYTES - ctl_bytes;
+    if (!preagg && 16ULL * ntiles > avail)
+    {
+        char msg[160];
+        snprintf(msg, sizeof(msg), "chunk of %llu rows exceeds the no-alloc scratch (at most %llu rows)",
+                 (unsigned long long)nrows, (unsigned long long)(avail / 16 * rows_per_tile));
+        XERR(eb, ERRCODE_DEVICE_ERROR, msg);
+        return ERRCODE_DEVICE_ERROR;
+    }
+    std::lock_guard<std::mutex> g(g_noalloc_lock);
+    xpu_task_ctl *d_ctl = (xpu_task_ctl *)base;
+    char *area = base + ctl_bytes;
+    if (nrows == 0)
+    {
+        if (resume_pos)
+            *resume_pos = 0;
+        if (!preagg)
+        {
+            xpu_launch_fill(d_ctl, 0, sizeof(xpu_task_ctl) / 8, nullptr);
+            xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                    dst_length, d_ctl, nullptr);
+            if (!hip_ok(hipStreamSynchronize(nullptr), eb, "hipStreamSynchronize"))
+                return ERRCODE_DEVICE_ERROR;
+        }
+        return 0;
+    }
+    const bool scan_agg = preagg;
+    uint32_t bpc = t.blocks_per_cu ? t.blocks_per_cu : 2;
+    uint64_t grid = (uint64_t)g_dev.ncu * bpc;
+    if (scan_agg)
+    {
+        uint64_t by_rows = (nrows + XPU_MIN_ROWS_PER_BLOCK - 1) / XPU_MIN_ROWS_PER_BLOCK;
+        grid = grid < by_rows ? grid : by_rows;
+    }
+    grid = grid < ntiles ? grid : ntiles;
+    uint32_t lds_nents = (preagg && P->lds_fast) ? t.lds_nents : 0;
+    size_t lds = lds_bytes_for(P, lds_nents);
+    if (lds + (scan_agg ? 0 : XP_PIPELINE_STATIC_LDS) > g_dev.lds_per_block)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "group-by LDS table exceeds the LDS of a block");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    xpu_kparams kp;
+    memset(&kp, 0, sizeof(kp));
+    kp.plan = s->d_plan;
+    kp.kds_src = d_src;
+    kp.kds_out = preagg ? s->d_kds_final : d_dst;
+    kp.ctl = d_ctl;
+    kp.const_pool = s->d_const_pool;
+    kp.row_begin = (uint32_t)row_begin;
+    kp.row_end = nitems;
+    kp.rows_per_tile = rows_per_tile;
+    kp.ntiles = (uint32_t)ntiles;
+    kp.lds_nents = lds_nents;
+    kp.params = s->d_params;
+    kp.param_nulls = s->comp.param_nulls;
+    if (!preagg)
+        kp.tile_status = (uint64_t *)area;
+    /* group-by: dump counters, then the dump records and the spill records
+     * share the rest; rows or entries beyond either merge into kds_final */
+    const bool spill = scan_agg && P->spill_ok && lds_nents > 0;
+    if (spill)
+    {
+        const uint32_t nparts = (uint32_t)g_dev.ncu < XP_DUMP_MAXPARTS ? (uint32_t)g_dev.ncu
+                                                                      : XP_DUMP_MAXPARTS;
+        const uint64_t rw = 1 + P->lds_n64 + (1 + P->lds_n32 + 1) / 2;
+        const uint64_t rec = 8ULL * (1 + P->spill_nregs);
+        const size_t next_bytes = (8ULL * XP_DUMP_MAXPARTS + 255) & ~(size_t)255;
+        const size_t rest = avail - next_bytes;
+        const uint64_t dump_cap = (rest / 2) / (nparts * rw * 8);
+        const size_t dump_bytes = ((size_t)dump_cap * nparts * rw * 8 + 255) & ~(size_t)255;
+        kp.dump_next = (uint64_t *)area;
+        kp.dump = (uint64_t *)(area + next_bytes);
+        kp.dump_cap = dump_cap;
+        kp.dump_nparts = nparts;
+        kp.spill = (uint64_t *)(area + next_bytes + dump_bytes);
+        kp.spill_cap = (rest - dump_bytes) / rec;
+    }
+    int rc = xpu_launch_fill(d_ctl, 0, sizeof(xpu_task_ctl) / 8, nullptr);
+    if (rc == 0 && kp.dump)
+        rc = xpu_launch_fill(kp.dump_next, 0, kp.dump_nparts, nullptr);
+    if (rc == 0 && !preagg)
+    {
+        rc = xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                     dst_length, d_ctl, nullptr);
+        if (rc == 0)
+            rc = xpu_launch_fill(kp.tile_status, 0, 2ULL * ntiles, nullptr);
+        if (rc == 0)
+            rc = xpu_launch_projection(&kp, P->nregs, (int)grid, (int)block, nullptr);
+    }
+    else if (rc == 0)
+    {
+        rc = xpu_launch_scan_agg(&kp, P->nregs, (int)rpt, (int)grid, (int)block, lds, nullptr);
+        if (rc == 0 && spill)
+        {
+            const uint32_t sgrid = (uint32_t)g_dev.ncu * (t.blocks_per_cu ? t.blocks_per_cu : 1);
+            rc = xpu_launch_spill_agg(&kp, P->nregs, (int)sgrid, (int)block, lds, nullptr);
+            if (rc == 0)
+                rc = xpu_launch_dump_merge(&kp, P->nregs, (int)kp.dump_nparts, (int)block, lds,
+                                           nullptr);
+        }
+    }
+    if (rc != 0)
+    {
+        hip_ok((hipError_t)rc, eb, "no-alloc kernel launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg &&
+        xpu_launch_finalize_dst(d_dst, s->d_kds_dst_head, (uint32_t)s->comp.kds_dst_head.size(),
+                                dst_length, d_ctl, nullptr) != 0)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no-alloc kernel launch");
+        return ERRCODE_DEVICE_ERROR;
+    }
+    xpu_task_ctl c;
+    if (!hip_ok(hipStreamSynchronize(nullptr), eb, "no-alloc pipeline kernel") ||
+        !hip_ok(hipMemcpy(&c, d_ctl, sizeof(c), hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return ERRCODE_DEVICE_ERROR;
+    t_last_kernel_ms = 0.0;     /* no events: nothing is timed */
+    memset(&t_last_stats, 0, sizeof(t_last_stats));
+    t_last_stats.grid = (uint32_t)grid;
+    t_last_stats.block = block;
+    t_last_stats.ntiles = (uint32_t)ntiles;
+    t_last_stats.lds_nents = lds_nents;
+    t_last_stats.lds_bytes = lds;
+    t_last_stats.rows = nrows;
+    t_last_stats.lds_miss = c.lds_miss;
+    t_last_stats.global_merges = c.global_merges;
+    t_last_stats.spill_rows = c.spill_rows;
+    if (res)
+    {
+        res->nitems_raw = (uint32_t)nrows;
+        res->nitems_in = c.nitems_in;
+        res->nitems_out = c.nitems_out;
+    }
+    if (c.errcode && c.errcode != ERRCODE_SUSPEND_NO_SPACE)
+    {
+        XERR(eb, c.errcode, message_of(s, c.errmsg));
+        return (int)c.errcode;
+    }
+    if (c.errcode == ERRCODE_SUSPEND_NO_SPACE)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, message_of(s, c.errmsg));
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!preagg && c.resume_tile != 0)
+    {
+        uint64_t next = row_begin + (uint64_t)(0xffffffffU - c.resume_tile) * rows_per_tile;
+        if (resume_pos)
+            *resume_pos = next;
+        XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "GpuProjection - no space to write");
+        return ERRCODE_SUSPEND_NO_SPACE;
+    }
+    if (resume_pos)
+        *resume_pos = 0;
+    return 0;
 }
 
 extern "C" int

@@ -15,7 +15,7 @@ LIBPATH = os.path.join(os.path.dirname(HERE), "libxpuhip.so")
 EXPORTS = [
     "xpuhip_init", "xpuhip_shutdown", "xpuhip_session_open", "xpuhip_session_check",
     "xpuhip_session_attach_inner", "xpuhip_task_exec", "xpuhip_task_exec_device",
-    "xpuhip_task_exec_device_tuples",
+    "xpuhip_task_exec_device_tuples", "xpuhip_task_exec_device_noalloc",
     "xpuhip_task_final", "xpuhip_session_close", "xpuhip_session_set_tuning",
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_memcpy_dtod", "xpuhip_device_synchronize",
@@ -96,6 +96,8 @@ def lib():
                                               C.POINTER(C.c_uint64), vp, vp]
         L.xpuhip_task_exec_device_tuples.argtypes = [vp, vp, vp, vp, sz, C.POINTER(C.c_uint64),
                                                      vp, vp]
+        L.xpuhip_task_exec_device_noalloc.argtypes = [vp, vp, C.c_uint32, vp, sz,
+                                                      C.POINTER(C.c_uint64), vp, vp]
         L.xpuhip_task_final.restype = C.c_int64
         L.xpuhip_task_final.argtypes = [vp, vp, sz, vp, vp]
         L.xpuhip_session_close.argtypes = [vp]
@@ -249,6 +251,17 @@ class Session:
         rc = lib().xpuhip_task_exec_device(self.h, d_src.ptr if d_src else None, nitems,
                                            d_dst.ptr if d_dst else None, dst_length,
                                            C.byref(pos), C.byref(res), C.byref(self.err))
+        if rc and rc != abi.ERRCODE_SUSPEND_NO_SPACE:
+            raise XpuError(rc, self.err)
+        return rc, res, pos.value
+
+    def exec_device_noalloc(self, d_src, nitems, d_dst=None, dst_length=0, resume_pos=0):
+        """exec_device through the entry point that allocates nothing."""
+        res = abi.kern_exec_results()
+        pos = C.c_uint64(resume_pos)
+        rc = lib().xpuhip_task_exec_device_noalloc(self.h, d_src.ptr if d_src else None, nitems,
+                                                   d_dst.ptr if d_dst else None, dst_length,
+                                                   C.byref(pos), C.byref(res), C.byref(self.err))
         if rc and rc != abi.ERRCODE_SUSPEND_NO_SPACE:
             raise XpuError(rc, self.err)
         return rc, res, pos.value
