@@ -324,6 +324,7 @@ KDS_HEAD_LENGTH(const kern_data_store *kds)
 #define TypeOpCode__text        11
 #define TypeOpCode__bpchar      12
 #define TypeOpCode__date        13
+#define TypeOpCode__array       26      /* after the WITH_POSTGIS=0 types and composite */
 
 #define FuncOpCode__ConstExpr               1
 #define FuncOpCode__ParamExpr               2
@@ -344,6 +345,9 @@ KDS_HEAD_LENGTH(const kern_data_store *kds)
 #define FuncOpCode__LeastExpr               17
 #define FuncOpCode__GreatestExpr            18
 #define FuncOpCode__CaseWhenExpr            19
+#define FuncOpCode__ScalarArrayOpAny        20
+#define FuncOpCode__ScalarArrayOpAll        21
+#define FuncOpCode__HashedArrayOp           22
 #define FuncOpCode__LoadVars                9999
 #define FuncOpCode__MoveVars                10000
 #define FuncOpCode__JoinQuals               10001
@@ -483,6 +487,14 @@ typedef struct kern_expression
             char        data[1] XPU_MAXALIGNED;
         } saop;
         struct {
+            const void *elem_ops;           /* device pointer (unused here) */
+            int32_t     elem_type;          /* TypeOpCode of the keys */
+            uint32_t    nslots;
+            uint32_t    default_offset;     /* offset of the default value; 0: NULL */
+            char        __padding[4];
+            uint32_t    hslots[1] XPU_MAXALIGNED;
+        } haop;
+        struct {
             int32_t     depth;
             int32_t     nitems;
             kern_varload_desc desc[1];
@@ -522,6 +534,27 @@ typedef struct kern_expression
 } kern_expression;
 
 #define SizeOfKernExpr(PAYLOAD_SZ)  (offsetof(kern_expression, u.data) + (PAYLOAD_SZ))
+
+/* items of a HashedArrayOp's hash table (xpu_common.h:2576-2594): 8-byte
+ * aligned from the expression head; next carries flags in its low bits */
+typedef struct
+{
+    uint32_t    hash;
+    uint32_t    next;
+    char        data[1] XPU_MAXALIGNED;
+} kern_hashed_array_op_item;
+#define KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY       0x01U
+#define KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE     0x02U
+#define KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK     0x07U
+#define KERN_HASHED_ARRAY_OP_NITEMS_THRESHOLD   4
+
+/* the session's database encoding (xpu_textlib.h:28-32) */
+typedef struct
+{
+    char        encname[16];
+    int32_t     enc_maxlen;
+    void       *enc_mblen;          /* device pointer (unused here) */
+} xpu_encode_info;
 #define SizeOfKernExprVar           (offsetof(kern_expression, u.v.__data))
 
 /* ---------------------------------------------------------------

@@ -101,11 +101,12 @@ oracle_hash_any(const void *ptr, int sz)
 /* ------------------------------------------------------------------
  * function table
  * ------------------------------------------------------------------ */
-enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen };
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr };
 enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
-       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len };
+       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len,
+       O_like, O_nlike, O_ilike, O_nilike, O_substr };
 
-typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, res; } ofunc;
+typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, t2, res; } ofunc;
 
 #define T_Invalid   TypeOpCode__Invalid
 #define T_bool      TypeOpCode__bool
@@ -116,9 +117,10 @@ typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, res;
 #define T_float4    TypeOpCode__float4
 #define T_float8    TypeOpCode__float8
 #define T_text      TypeOpCode__text
-#define OFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) \
-    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##RES },
-static const ofunc ofunc_table[] = { OFUNC_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0 } };
+#define T_bpchar    TypeOpCode__bpchar
+#define OFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, A2, RES) \
+    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##A2, T_##RES },
+static const ofunc ofunc_table[] = { OFUNC_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0, 0 } };
 
 static const ofunc *
 lookup_func(int opcode)
@@ -171,9 +173,173 @@ typedef struct
     int         nslots;
     od          slots[OMAX_SLOTS];
     bool        compare_nulls;
+    int         enc;        /* database encoding: ENC_NONE / ENC_SINGLE / ENC_UTF8 / ENC_OTHER */
     uint32_t    errcode;
     char        errmsg[KERN_ERRORBUF_MESSAGE_LEN + 1];
 } octx;
+
+enum { ENC_NONE = 0, ENC_SINGLE, ENC_UTF8, ENC_OTHER };
+
+/* the session's database encoding by name (xpu_encode_catalog,
+ * xpu_textlib.cu:853-900): single-byte encodings, UTF8, or another
+ * multi-byte one */
+static int
+encoding_kind(const char *name)
+{
+    static const char *mb[] = { "EUC_JP", "EUC_CN", "EUC_KR", "EUC_TW", "EUC_JIS_2004",
+                                "MULE_INTERNAL", "SJIS", "BIG5", "GBK", "UHC", "GB18030",
+                                "JOHAB", "SHIFT_JIS_2004", NULL };
+    if (!name || !*name)
+        return ENC_NONE;
+    if (strcmp(name, "UTF8") == 0)
+        return ENC_UTF8;
+    for (int i = 0; mb[i]; i++)
+        if (strcmp(name, mb[i]) == 0)
+            return ENC_OTHER;
+    return ENC_SINGLE;
+}
+
+static int
+session_encoding(const kern_session_info *s)
+{
+    if (!s || !s->session_encode)
+        return ENC_NONE;
+    char name[17];
+    memcpy(name, (const char *)s + s->session_encode, 16);
+    name[16] = 0;
+    return encoding_kind(name);
+}
+
+/* bytes of the character at s (pg_utf8_mblen, xpu_textlib.cu:774-787) */
+static int
+utf8_mblen(const char *s)
+{
+    unsigned char c = (unsigned char)*s;
+    if ((c & 0x80) == 0) return 1;
+    if ((c & 0xe0) == 0xc0) return 2;
+    if ((c & 0xf0) == 0xe0) return 3;
+    if ((c & 0xf8) == 0xf0) return 4;
+    return 1;
+}
+
+static int
+mblen_of(int enc, const char *s)
+{
+    return enc == ENC_UTF8 ? utf8_mblen(s) : 1;
+}
+
+/* length without the trailing spaces of a bpchar (bpchar_truelen,
+ * xpu_textlib.cu:59-69) */
+static int
+bpchar_truelen(const char *p, int len)
+{
+    while (len > 0 && p[len - 1] == ' ')
+        len--;
+    return len;
+}
+
+/* LIKE / ILIKE (GenericMatchText, xpu_textlib.cu:915-1085) without
+ * recursion: the newest % is the only point a failed match resumes from,
+ * since the reference's recursive search returns "abort" rather than
+ * "false" once a deeper % has scanned the rest of the text.  Returns 1 /
+ * 0, or -1 for an invalid escape. */
+#define LIKE_UPPER(c, ic) (((ic) && (c) >= 'a' && (c) <= 'z') ? (c) + ('A' - 'a') : (c))
+static int
+like_match(int enc, bool ic, const char *t, int tlen, const char *p, int plen)
+{
+    int ti = 0, pi = 0;
+    int bt_t = -1, bt_p = -1;
+    char firstpat = 0;
+
+    if (plen == 1 && p[0] == '%')
+        return 1;
+    for (;;)
+    {
+        bool mismatch = false;
+        while (ti < tlen && pi < plen && !mismatch)
+        {
+            char c = p[pi];
+            if (c == '\\')
+            {
+                pi++;
+                if (pi >= plen)
+                    return -1;
+                if (LIKE_UPPER(p[pi], ic) != LIKE_UPPER(t[ti], ic))
+                    mismatch = true;
+                else
+                {
+                    ti++;
+                    pi++;
+                }
+            }
+            else if (c == '%')
+            {
+                pi++;
+                while (pi < plen)
+                {
+                    if (p[pi] == '%')
+                        pi++;
+                    else if (p[pi] == '_')
+                    {
+                        if (ti >= tlen)
+                            return 0;
+                        ti += mblen_of(enc, t + ti);
+                        pi++;
+                    }
+                    else
+                        break;
+                }
+                if (pi >= plen)
+                    return 1;
+                if (p[pi] == '\\')
+                {
+                    if (plen - pi < 2)
+                        return -1;
+                    firstpat = LIKE_UPPER(p[pi + 1], ic);
+                }
+                else
+                    firstpat = LIKE_UPPER(p[pi], ic);
+                bt_p = pi;
+                bt_t = ti;
+                goto scan;
+            }
+            else if (c == '_')
+            {
+                ti += mblen_of(enc, t + ti);
+                pi++;
+            }
+            else if (LIKE_UPPER(c, ic) != LIKE_UPPER(t[ti], ic))
+                mismatch = true;
+            else
+            {
+                ti++;
+                pi++;
+            }
+        }
+        if (!mismatch)
+        {
+            if (ti < tlen)
+                mismatch = true;        /* end of pattern, not of text */
+            else
+            {
+                while (pi < plen && p[pi] == '%')
+                    pi++;
+                return pi >= plen ? 1 : 0;
+            }
+        }
+        /* a mismatch: resume the newest %'s scan one character on */
+        if (bt_p < 0)
+            return 0;
+        bt_t += mblen_of(enc, t + bt_t);
+    scan:
+        while (bt_t < tlen && LIKE_UPPER(t[bt_t], ic) != firstpat)
+            bt_t += mblen_of(enc, t + bt_t);
+        if (bt_t >= tlen)
+            return 0;
+        ti = bt_t;
+        pi = bt_p;
+    }
+}
 
 static bool
 oerror(octx *c, uint32_t code, const char *fmt, const char *arg)
@@ -321,7 +487,7 @@ cmp_result(int op, int comp)
 static bool
 eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
 {
-    od a[2];
+    od a[3];
     const kern_expression *karg = KEXP_FIRST_ARG(k);
 
     memset(r, 0, sizeof(od));
@@ -330,7 +496,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
     for (int i = 0; i < f->nargs; i++, karg = KEXP_NEXT_ARG(karg))
         if (!eval(c, karg, &a[i]))
             return false;
-    if (a[0].isnull || (f->nargs > 1 && a[1].isnull))
+    if (a[0].isnull || (f->nargs > 1 && a[1].isnull) || (f->nargs > 2 && a[2].isnull))
     {
         /* NULL input: comparisons honour kmode_compare_nulls for eq/ne
          * (xpu_basetype.h:30-60); everything else yields NULL */
@@ -359,6 +525,12 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
             return true;
         case K_textcmp:
         {
+            if (f->t0 == T_bpchar)
+            {
+                /* bpchar compares without trailing spaces (xpu_textlib.cu:59-69) */
+                a[0].len = bpchar_truelen(a[0].p, a[0].len);
+                a[1].len = bpchar_truelen(a[1].p, a[1].len);
+            }
             int n = a[0].len < a[1].len ? a[0].len : a[1].len;
             int comp = memcmp(a[0].p, a[1].p, n);
             if (comp == 0)
@@ -367,8 +539,111 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
             return true;
         }
         case K_textlen:
-            r->i = a[0].len;
+        {
+            /* pgfn_textlen / pgfn_bpcharlen (xpu_textlib.cu:583-720):
+             * characters by the database encoding, up to a NUL byte */
+            int len = f->t0 == T_bpchar ? bpchar_truelen(a[0].p, a[0].len) : a[0].len;
+            if (c->enc == ENC_NONE)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "No encoding info was supplied");
+            if (c->enc == ENC_OTHER)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "encoding not supported");
+            if (c->enc == ENC_UTF8)
+            {
+                const char *s = a[0].p;
+                int n = 0;
+                while (len > 0 && *s)
+                {
+                    int w = utf8_mblen(s);
+                    len -= w;
+                    s += w;
+                    n++;
+                }
+                len = n;
+            }
+            r->i = len;
             return true;
+        }
+        case K_like:
+        {
+            /* PG_TEXTLIKE_TEMPLATE (xpu_textlib.cu:1087-1155) */
+            int tlen = f->t0 == T_bpchar ? bpchar_truelen(a[0].p, a[0].len) : a[0].len;
+            if (c->enc == ENC_NONE || c->enc == ENC_OTHER)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", c->enc == ENC_NONE
+                              ? "No encoding info was supplied" : "encoding not supported");
+            int m = like_match(c->enc, f->op == O_ilike || f->op == O_nilike,
+                               a[0].p, tlen, a[1].p, a[1].len);
+            if (m < 0)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "invalid escape in LIKE pattern");
+            r->i = (f->op == O_like || f->op == O_ilike) ? m : !m;
+            return true;
+        }
+        case K_substr:
+        {
+            /* __substring_common (xpu_textlib.cu:1159-1259), 1-origin start */
+            int32_t start = (int32_t)a[1].i - 1;
+            int32_t count = f->nargs > 2 ? (int32_t)a[2].i : -1;
+            const char *str = a[0].p;
+            int32_t length = a[0].len;
+            if (f->nargs > 2 && count < 0)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "negative substring length not allowed");
+            if (c->enc == ENC_NONE || c->enc == ENC_OTHER)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", c->enc == ENC_NONE
+                              ? "No encoding info was supplied" : "encoding not supported");
+            r->p = str;
+            if (c->enc == ENC_SINGLE)
+            {
+                if (start >= length || count == 0 || (count > 0 && start + count <= 0))
+                    r->len = 0;
+                else if (start < 0)
+                    r->len = (count < 0 || start + count >= length) ? length
+                             : (start + count < length ? start + count : length);
+                else
+                {
+                    int64_t end = (int64_t)start + count;
+                    r->len = (int32_t)((count < 0 || end > length ? length : end) - start);
+                    r->p = str + start;
+                }
+                return true;
+            }
+            if (start < 0)
+            {
+                if (count < 0)
+                {
+                    r->len = length;
+                    return true;
+                }
+                count = start + count;
+                if (count < 0)
+                {
+                    r->len = 0;
+                    return true;
+                }
+                start = 0;
+            }
+            const char *pos = str, *end = str + length, *val = NULL;
+            if (pos < end)
+            {
+                for (int i = 0;; i++)
+                {
+                    int w = utf8_mblen(pos);
+                    if (pos + w > end)
+                        break;
+                    if (i == start)
+                        val = pos;
+                    if (count >= 0 && i == start + count)
+                        break;
+                    pos += w;
+                }
+            }
+            if (val)
+            {
+                r->p = val;
+                r->len = (int32_t)(pos - val);
+            }
+            else
+                r->len = 0;
+            return true;
+        }
         case K_arith:
             if (is_int_type(f->res))
             {
@@ -556,6 +831,209 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
     return oerror(c, ERRCODE_DEVICE_ERROR, "%s: unsupported function", f->name);
 }
 
+/* the type's three-way comparison (xpu_*_datum_comp): integers by value,
+ * floats by < and > (NaN compares equal to everything, xpu_basetype.cu:
+ * 224-238), text by bytes then length, bpchar without trailing spaces */
+static int
+datum_comp(int type, const od *x, const od *y)
+{
+    switch (type)
+    {
+        case T_float4:
+            return x->f > y->f ? 1 : x->f < y->f ? -1 : 0;
+        case T_float8:
+            return x->d > y->d ? 1 : x->d < y->d ? -1 : 0;
+        case T_text: case TypeOpCode__bytea: case T_bpchar:
+        {
+            int xl = x->len, yl = y->len;
+            if (type == T_bpchar)
+            {
+                xl = bpchar_truelen(x->p, xl);
+                yl = bpchar_truelen(y->p, yl);
+            }
+            int comp = memcmp(x->p, y->p, xl < yl ? xl : yl);
+            if (comp == 0)
+                comp = xl < yl ? -1 : xl > yl ? 1 : 0;
+            return comp;
+        }
+        default:
+            return x->i > y->i ? 1 : x->i < y->i ? -1 : 0;
+    }
+}
+
+/* value equality of two non-NULL datums as the type's '=' operator (floats
+ * with NaN equal to NaN only, as float8eq) */
+static bool
+datum_equal(int type, const od *x, const od *y)
+{
+    if (type == T_float4)
+        return float_cmp(O_eq, (double)x->f, (double)y->f);
+    if (type == T_float8)
+        return float_cmp(O_eq, x->d, y->d);
+    return datum_comp(type, x, y) == 0;
+}
+
+static bool eval(octx *c, const kern_expression *k, od *r);
+
+/* scalar op ANY / ALL (array constant), as PostgreSQL's executor defines it
+ * (ExecEvalScalarArrayOp): ANY is true at the first true comparison, else
+ * NULL if a comparison was NULL, else false; ALL is false at the first false
+ * one, else NULL if one was NULL, else true; a NULL array is NULL.  The
+ * reference's pgfn_ScalarArrayOp (xpu_common.cu:815-1011) walks the
+ * elements the same way (element into the temporary slot, then the
+ * comparator), but its ALL returns true where a comparison is false. */
+static bool
+eval_saop(octx *c, const kern_expression *k, od *r)
+{
+    bool use_any = (k->opcode == FuncOpCode__ScalarArrayOpAny);
+    const kern_expression *karr = KEXP_FIRST_ARG(k);
+    const kern_expression *kcmp = karr ? KEXP_NEXT_ARG(karr) : NULL;
+    int slot = k->u.saop.elem_slot_id;
+    if (!karr || !kcmp || k->nr_args != 2 || slot >= c->nslots ||
+        karr->opcode != FuncOpCode__ConstExpr)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "ScalarArrayOp needs a constant array");
+    if (karr->u.c.const_isnull)
+    {
+        r->isnull = true;
+        return true;
+    }
+    const kern_expression *kelem = KEXP_NEXT_ARG(KEXP_FIRST_ARG(kcmp));
+    int etype = kelem->exptype;
+    const char *img = karr->u.c.const_value;
+    uint32_t hdr;
+    memcpy(&hdr, img, 4);
+    int32_t ndim, dataoffset, nitems = 1;
+    memcpy(&ndim, img + 4, 4);
+    memcpy(&dataoffset, img + 8, 4);
+    for (int d = 0; d < ndim; d++)
+    {
+        int32_t dim;
+        memcpy(&dim, img + 16 + 4 * d, 4);
+        nitems *= dim;
+    }
+    if (ndim == 0)
+        nitems = 0;
+    const uint8_t *nullmap = dataoffset ? (const uint8_t *)img + 16 + 8 * ndim : NULL;
+    const char *data = img + (dataoffset ? dataoffset : XPU_MAXALIGN(16 + 8 * ndim));
+    int typlen = type_width(etype), align = typlen > 0 ? typlen : 4;
+    if (etype == T_bool || etype == T_int1)
+        align = 1;
+    uint32_t off = 0;
+    bool meet_null = false;
+    for (int32_t i = 0; i < nitems; i++)
+    {
+        od *e = &c->slots[slot];
+        memset(e, 0, sizeof(od));
+        if (nullmap && !(nullmap[i >> 3] & (1 << (i & 7))))
+            e->isnull = true;
+        else
+        {
+            const char *addr;
+            off = XPU_TYPEALIGN(align, off);
+            addr = data + off;
+            if (!datum_from_image(c, etype, addr, e))
+                return false;
+            off += typlen > 0 ? (uint32_t)typlen : (uint32_t)(e->len + (e->p - addr));
+        }
+        od st;
+        if (!eval(c, kcmp, &st))
+            return false;
+        if (st.isnull)
+            meet_null = true;
+        else if (use_any ? st.i : !st.i)
+        {
+            r->i = use_any ? 1 : 0;
+            return true;
+        }
+    }
+    r->isnull = meet_null;
+    r->i = meet_null ? 0 : !use_any;
+    return true;
+}
+
+/* the hash table of constant keys of HashedArrayOp (codegen.c:2820-2906,
+ * 2458-2660): the value of the first item, in its slot's chain order, whose
+ * key equals the scalar; else the default (at default_offset, or NULL).
+ * NULL scalars give NULL.  As PostgreSQL's hashed "= ANY", an IN list (its
+ * default is the header's zero padding) with a NULL element gives NULL
+ * rather than false when nothing matches; the reference's
+ * pgfn_HashedArrayOp (xpu_common.cu:1013-1088) gives false there.  A
+ * matching item without a value gives NULL. */
+static bool
+eval_haop(octx *c, const kern_expression *k, od *r)
+{
+    const kern_expression *karg = KEXP_FIRST_ARG(k);
+    od sv;
+    if (!karg || k->nr_args != 1)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "malformed HashedArrayOp");
+    if (!eval(c, karg, &sv))
+        return false;
+    if (sv.isnull)
+    {
+        r->isnull = true;
+        return true;
+    }
+    int etype = k->u.haop.elem_type;
+    uint32_t nslots = k->u.haop.nslots;
+    int ewidth = type_width(etype);
+    int ralign = type_width(k->exptype) > 0 ? type_width(k->exptype) : 4;
+    if (k->exptype == T_bool || k->exptype == T_int1)
+        ralign = 1;
+    bool in_list = (k->u.haop.default_offset == offsetof(kern_expression, u.haop.__padding));
+    bool has_null_key = false;
+    for (uint32_t sl = 0; sl < nslots; sl++)
+    {
+        uint32_t next = k->u.haop.hslots[sl];
+        while (next != 0)
+        {
+            const char *item = (const char *)k + next;
+            uint32_t inext;
+            memcpy(&inext, item + 4, 4);
+            if (!(inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY))
+                has_null_key = true;
+            next = inext & ~KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK;
+        }
+    }
+    /* the chain of the scalar's hash, in chain order */
+    uint32_t h = datum_hash(etype, &sv);
+    uint32_t next = k->u.haop.hslots[h % nslots];
+    while (next != 0)
+    {
+        const char *item = (const char *)k + next;
+        uint32_t ihash, inext;
+        memcpy(&ihash, item, 4);
+        memcpy(&inext, item + 4, 4);
+        if (ihash == h && (inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY))
+        {
+            od ev;
+            if (!datum_from_image(c, etype, item + 8, &ev))
+                return false;
+            if (datum_equal(etype, &sv, &ev))
+            {
+                if (!(inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE))
+                {
+                    r->isnull = true;
+                    return true;
+                }
+                uint32_t klen = ewidth > 0 ? (uint32_t)ewidth : (uint32_t)(ev.len + (ev.p - (item + 8)));
+                return datum_from_image(c, k->exptype, item + 8 + XPU_TYPEALIGN(ralign, klen), r);
+            }
+        }
+        next = inext & ~KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK;
+    }
+    if (in_list && has_null_key)
+    {
+        r->isnull = true;
+        return true;
+    }
+    if (!k->u.haop.default_offset)
+    {
+        r->isnull = true;
+        return true;
+    }
+    return datum_from_image(c, k->exptype, (const char *)k + k->u.haop.default_offset, r);
+}
+
 static const char *
 session_param(const kern_session_info *s, uint32_t id)
 {
@@ -688,6 +1166,80 @@ eval(octx *c, const kern_expression *k, od *r)
             }
             r->isnull = true;
             return true;
+        case FuncOpCode__DistinctFrom:
+        {
+            /* pgfn_DistinctFrom (xpu_common.cu:613-660): the arguments of
+             * the '=' function, then the function itself */
+            const kern_expression *kcmp = KEXP_FIRST_ARG(k);
+            od x, y;
+            if (!kcmp || kcmp->nr_args != 2)
+                return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "malformed DistinctFrom");
+            karg = KEXP_FIRST_ARG(kcmp);
+            if (!eval(c, karg, &x) || !eval(c, KEXP_NEXT_ARG(karg), &y))
+                return false;
+            if (x.isnull || y.isnull)
+            {
+                r->i = !(x.isnull && y.isnull);
+                return true;
+            }
+            if (!eval(c, kcmp, &tmp))
+                return false;
+            r->i = !tmp.i;
+            return true;
+        }
+        case FuncOpCode__LeastExpr:
+        case FuncOpCode__GreatestExpr:
+        {
+            /* pgfn_LeastExpr / pgfn_GreatestExpr (xpu_common.cu:683-757):
+             * the least / greatest non-NULL argument by the type's
+             * comparison; an equal later argument does not replace it */
+            bool least = (k->opcode == FuncOpCode__LeastExpr);
+            r->isnull = true;
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                if (tmp.isnull)
+                    continue;
+                if (r->isnull)
+                {
+                    *r = tmp;
+                    continue;
+                }
+                int comp = datum_comp(k->exptype, r, &tmp);
+                if ((least && comp > 0) || (!least && comp < 0))
+                    *r = tmp;
+            }
+            return true;
+        }
+        case FuncOpCode__CaseWhenExpr:
+        {
+            /* pgfn_CaseWhenExpr (xpu_common.cu:759-811) */
+            if (k->u.casewhen.case_comp &&
+                !eval(c, (const kern_expression *)((const char *)k + k->u.casewhen.case_comp), &tmp))
+                return false;
+            karg = KEXP_FIRST_ARG(k);
+            for (uint32_t i = 0; i + 1 < k->nr_args; i += 2, karg = KEXP_NEXT_ARG(karg))
+            {
+                if (!eval(c, karg, &tmp))
+                    return false;
+                karg = KEXP_NEXT_ARG(karg);
+                if (!tmp.isnull && tmp.i)
+                    return eval(c, karg, r);
+            }
+            if (!k->u.casewhen.case_else)
+            {
+                r->isnull = true;
+                return true;
+            }
+            return eval(c, (const kern_expression *)((const char *)k + k->u.casewhen.case_else), r);
+        }
+        case FuncOpCode__ScalarArrayOpAny:
+        case FuncOpCode__ScalarArrayOpAll:
+            return eval_saop(c, k, r);
+        case FuncOpCode__HashedArrayOp:
+            return eval_haop(c, k, r);
         case FuncOpCode__HashValue:
         {
             uint32_t h = 0xffffffffU;
@@ -739,7 +1291,7 @@ load_arrow(octx *c, const kern_data_store *kds, int resno, uint32_t row, int typ
             return true;
         }
     }
-    if (type == T_text || type == TypeOpCode__bytea)
+    if (type == T_text || type == TypeOpCode__bytea || type == T_bpchar)
     {
         /* __xpu_varlena_datum_arrow_ref (xpu_textlib.cu:19-54): by the
          * column's Arrow type; an offset out of range reads as NULL */
@@ -1153,6 +1705,7 @@ struct oracle_session
 {
     kern_session_info *session;
     size_t      length;
+    int         enc;            /* the session's database encoding (ENC_*) */
     const kern_varslot_desc *vdesc;
     int         nslots;
     int         nrels;
@@ -1207,6 +1760,7 @@ oracle_session_open(const kern_session_info *session, size_t length, kern_errorb
 
     memcpy(s, session, length);
     os->session = s;
+    os->enc = session_encoding(s);
     os->length = length;
     os->nslots = s->kcxt_kvars_nslots;
     if (os->nslots > OMAX_SLOTS)
@@ -1278,7 +1832,7 @@ oracle_session_close(oracle_session *os)
  * (*res_len = -1 for NULL), or 1 with the error code and message.
  * ------------------------------------------------------------------ */
 int
-oracle_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
+oracle_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls, const char *encname,
                  int nslots, const int32_t *slot_types, const uint8_t *slot_data,
                  const int64_t *slot_offs, int32_t res_type, uint8_t *res_image, int32_t res_cap,
                  int32_t *res_len, uint32_t *errcode, char *errmsg, int errlen)
@@ -1292,6 +1846,7 @@ oracle_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
     memcpy(ebuf, kexp_blob, kexp_len);
     c->session = &empty_session;
     c->compare_nulls = compare_nulls != 0;
+    c->enc = encoding_kind(encname);
     c->nslots = nslots;
     *errcode = 0;
     for (int i = 0; i < nslots && i < OMAX_SLOTS; i++)
@@ -1338,6 +1893,7 @@ octx_init(octx *c, const oracle_session *os)
     c->session = os->session;
     c->vdesc = os->vdesc;
     c->nslots = os->nslots;
+    c->enc = os->enc;
 }
 
 /* ------------------------------------------------------------------

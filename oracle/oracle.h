@@ -36,7 +36,7 @@ uint32_t oracle_hash_any(const void *ptr, int len);
 
 /* one expression over variable slots (the per-function checker) */
 int     oracle_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
-                         int nslots, const int32_t *slot_types, const uint8_t *slot_data,
+                         const char *encname, int nslots, const int32_t *slot_types, const uint8_t *slot_data,
                          const int64_t *slot_offs, int32_t res_type, uint8_t *res_image,
                          int32_t res_cap, int32_t *res_len, uint32_t *errcode, char *errmsg,
                          int errlen);

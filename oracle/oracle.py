@@ -28,8 +28,8 @@ def lib():
         L.oracle_hash_any.argtypes = [C.c_void_p, C.c_int]
         vp = C.c_void_p
         L.oracle_eval_expr.restype = C.c_int
-        L.oracle_eval_expr.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp, vp, C.c_int32, vp,
-                                       C.c_int32, vp, vp, C.c_char_p, C.c_int]
+        L.oracle_eval_expr.argtypes = [vp, C.c_size_t, C.c_int, C.c_char_p, C.c_int, vp, vp, vp,
+                                       C.c_int32, vp, C.c_int32, vp, vp, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -86,7 +86,7 @@ def hash_any(data):
     return lib().oracle_hash_any(C.c_char_p(b), len(b))
 
 
-def eval_expr(blob, slot_type_codes, images, res_type_code, compare_nulls=0):
+def eval_expr(blob, slot_type_codes, images, res_type_code, compare_nulls=0, encoding="UTF8"):
     """One expression over variable slots (oracle_eval_expr): returns the
     result's datum image, None for NULL, or ("E", errcode, message)."""
     n = len(slot_type_codes)
@@ -107,7 +107,8 @@ def eval_expr(blob, slot_type_codes, images, res_type_code, compare_nulls=0):
     out = C.create_string_buffer(8192)
     msg = C.create_string_buffer(256)
     b = bytes(blob)
-    rc = lib().oracle_eval_expr(b, len(b), compare_nulls, n, types, dbuf, offs, res_type_code, out,
+    rc = lib().oracle_eval_expr(b, len(b), compare_nulls, (encoding or "").encode(), n, types,
+                                dbuf, offs, res_type_code, out,
                                 8192, C.byref(rl), C.byref(ec), msg, 256)
     if rc < 0:
         raise RuntimeError("oracle cannot evaluate this expression")

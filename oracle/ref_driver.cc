@@ -103,6 +103,19 @@ ref_abi_json(void)
     OFF(kern_expression, u.proj.hash); OFF(kern_expression, u.proj.nattrs);
     OFF(kern_expression, u.proj.slot_id);
     OFF(kern_expression, u.pack.npacked); OFF(kern_expression, u.pack.offset);
+    OFF(kern_expression, u.saop.elem_slot_id); OFF(kern_expression, u.saop.data);
+    OFF(kern_expression, u.haop.elem_ops); OFF(kern_expression, u.haop.elem_type);
+    OFF(kern_expression, u.haop.nslots); OFF(kern_expression, u.haop.default_offset);
+    OFF(kern_expression, u.haop.hslots);
+    SZ(kern_hashed_array_op_item); OFF(kern_hashed_array_op_item, next);
+    OFF(kern_hashed_array_op_item, data);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY", KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE", KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK", KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK);
+    emit("const:KERN_HASHED_ARRAY_OP_NITEMS_THRESHOLD", KERN_HASHED_ARRAY_OP_NITEMS_THRESHOLD);
+    emit("type:composite", TypeOpCode__composite);
+    emit("type:array", TypeOpCode__array);
+    emit("type:internal", TypeOpCode__internal);
 
     SZ(kagg_state__pminmax_int64_packed); OFF(kagg_state__pminmax_int64_packed, value);
     SZ(kagg_state__pminmax_fp64_packed); OFF(kagg_state__pminmax_fp64_packed, value);
@@ -191,6 +204,9 @@ ref_abi_json(void)
     emit("func:LeastExpr", FuncOpCode__LeastExpr);
     emit("func:GreatestExpr", FuncOpCode__GreatestExpr);
     emit("func:CaseWhenExpr", FuncOpCode__CaseWhenExpr);
+    emit("func:ScalarArrayOpAny", FuncOpCode__ScalarArrayOpAny);
+    emit("func:ScalarArrayOpAll", FuncOpCode__ScalarArrayOpAll);
+    emit("func:HashedArrayOp", FuncOpCode__HashedArrayOp);
     emit("func:LoadVars", FuncOpCode__LoadVars);
     emit("func:MoveVars", FuncOpCode__MoveVars);
     emit("func:JoinQuals", FuncOpCode__JoinQuals);
@@ -404,8 +420,32 @@ ref_resolve(kern_expression *kexp)
  * If the reference raises an error, returns 1 with errcode and message.
  * Returns -1 for a tree it cannot resolve.
  */
+/* the session's xpu_encode_info for encname, resolved from the
+ * reference's catalog as the service does (gpu_service.c:3284-3305);
+ * encname NULL or empty leaves the session without one */
+static bool
+ref_session_encode(kern_session_info *session, size_t cap, const char *encname)
+{
+    if (!encname || !*encname)
+        return true;
+    size_t off = 2048;
+    if (off + sizeof(xpu_encode_info) > cap)
+        return false;
+    xpu_encode_info *enc = (xpu_encode_info *)((char *)session + off);
+    for (int i = 0; xpu_encode_catalog[i].enc_maxlen > 0; i++)
+    {
+        if (strcmp(xpu_encode_catalog[i].encname, encname) == 0)
+        {
+            *enc = xpu_encode_catalog[i];
+            session->session_encode = off;
+            return true;
+        }
+    }
+    return false;
+}
+
 extern "C" int
-ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
+ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls, const char *encname,
               int nslots, const int32_t *slot_types, const uint8_t *slot_data,
               const int64_t *slot_offs,
               int32_t *res_type, uint8_t *res_image, int32_t res_cap, int32_t *res_len,
@@ -416,7 +456,9 @@ ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls,
     kern_expression *kexp = (kern_expression *)ebuf.data();
     if (!ref_resolve(kexp))
         return -1;
-    std::vector<uint64_t> sbuf(4096 / 8, 0);      /* a session with no params / encoding */
+    std::vector<uint64_t> sbuf(4096 / 8, 0);      /* a session with no params */
+    if (!ref_session_encode((kern_session_info *)sbuf.data(), 4096, encname))
+        return -1;
     const size_t vl = 1 << 16;
     std::vector<uint64_t> kbuf((offsetof(kern_context, vlbuf) + vl) / 8 + 1, 0);
     kern_context *kcxt = (kern_context *)kbuf.data();

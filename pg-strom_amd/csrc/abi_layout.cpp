@@ -86,6 +86,16 @@ xpuhip_abi_layout_json(void)
     OFF(kern_expression, u.v.__data);
     OFF(kern_expression, u.casewhen.case_comp); OFF(kern_expression, u.casewhen.case_else);
     OFF(kern_expression, u.casewhen.data);
+    OFF(kern_expression, u.saop.elem_slot_id); OFF(kern_expression, u.saop.data);
+    OFF(kern_expression, u.haop.elem_ops); OFF(kern_expression, u.haop.elem_type);
+    OFF(kern_expression, u.haop.nslots); OFF(kern_expression, u.haop.default_offset);
+    OFF(kern_expression, u.haop.hslots);
+    SZ(kern_hashed_array_op_item); OFF(kern_hashed_array_op_item, next);
+    OFF(kern_hashed_array_op_item, data);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY", KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE", KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE);
+    emit("const:KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK", KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK);
+    emit("const:KERN_HASHED_ARRAY_OP_NITEMS_THRESHOLD", KERN_HASHED_ARRAY_OP_NITEMS_THRESHOLD);
     OFF(kern_expression, u.load.depth); OFF(kern_expression, u.load.nitems);
     OFF(kern_expression, u.load.desc);
     OFF(kern_expression, u.move.depth); OFF(kern_expression, u.move.nitems);
@@ -161,7 +171,9 @@ xpuhip_abi_layout_json(void)
     XF(DistinctFrom); XF(CoalesceExpr); XF(LeastExpr); XF(GreatestExpr); XF(CaseWhenExpr);
     XF(LoadVars); XF(MoveVars); XF(JoinQuals); XF(HashValue); XF(GiSTEval); XF(SaveExpr);
     XF(AggFuncs); XF(Projection); XF(SortKeys); XF(Packed);
-#define XFUNC_EMIT(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) emit("func:" #NAME, OPC);
+    XF(ScalarArrayOpAny); XF(ScalarArrayOpAll); XF(HashedArrayOp);
+    XT(array);
+#define XFUNC_EMIT(OPC, NAME, KIND, OP, NARGS, A0, A1, A2, RES) emit("func:" #NAME, OPC);
     XPU_FUNCTION_TABLE(XFUNC_EMIT)
     emit("const:KERN_EXPRESSION_MAGIC", KERN_EXPRESSION_MAGIC);
     g_json += "}\n";

@@ -19,11 +19,12 @@
 
 namespace {
 
-enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen };
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr };
 enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
-       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len };
+       O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len,
+       O_like, O_nlike, O_ilike, O_nilike, O_substr };
 
-struct devfunc { int opcode; const char *name; int kind, op, nargs, t0, t1, res; };
+struct devfunc { int opcode; const char *name; int kind, op, nargs, t0, t1, t2, res; };
 
 #define T_Invalid   TypeOpCode__Invalid
 #define T_bool      TypeOpCode__bool
@@ -34,8 +35,9 @@ struct devfunc { int opcode; const char *name; int kind, op, nargs, t0, t1, res;
 #define T_float4    TypeOpCode__float4
 #define T_float8    TypeOpCode__float8
 #define T_text      TypeOpCode__text
-#define DEVFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, RES) \
-    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##RES },
+#define T_bpchar    TypeOpCode__bpchar
+#define DEVFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, A2, RES) \
+    { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##A2, T_##RES },
 const devfunc devfunc_table[] = { XPU_FUNCTION_TABLE(DEVFUNC) };
 
 const devfunc *
@@ -91,6 +93,46 @@ int_class(int t)
 
 struct Reg { int r; int dom; int type; };
 
+/* the session's database encoding (xpu_encode_info, xpu_textlib.h:28-32)
+ * by name: single-byte encodings count one character per byte; UTF8 by
+ * pg_utf8_mblen; other multi-byte encodings are not executed here */
+enum { ENC_NONE = 0, ENC_SINGLE, ENC_UTF8, ENC_OTHER };
+
+int
+encoding_kind(const kern_session_info *s, size_t length)
+{
+    static const char *mb[] = { "EUC_JP", "EUC_CN", "EUC_KR", "EUC_TW", "EUC_JIS_2004",
+                                "MULE_INTERNAL", "SJIS", "BIG5", "GBK", "UHC", "GB18030",
+                                "JOHAB", "SHIFT_JIS_2004", nullptr };
+    if (!s->session_encode || s->session_encode + 16 > length)
+        return ENC_NONE;
+    char name[17];
+    memcpy(name, (const char *)s + s->session_encode, 16);
+    name[16] = 0;
+    if (!name[0])
+        return ENC_NONE;
+    if (strcmp(name, "UTF8") == 0)
+        return ENC_UTF8;
+    for (int i = 0; mb[i]; i++)
+        if (strcmp(name, mb[i]) == 0)
+            return ENC_OTHER;
+    return ENC_SINGLE;
+}
+
+/* kind and byte width of a HashedArrayOp key or result type (XHA_KIND_*) */
+bool
+haop_kind(int t, int *kind, int *width)
+{
+    switch (type_domain(t))
+    {
+        case XD_INT: *kind = XHA_KIND_INT; *width = type_width(t); return *width > 0;
+        case XD_F32: *kind = XHA_KIND_F32; *width = 4; return true;
+        case XD_F64: *kind = XHA_KIND_F64; *width = 8; return true;
+        case XD_TEXT: *kind = XHA_KIND_TEXT; *width = 0; return true;
+    }
+    return false;
+}
+
 struct Compiler
 {
     const kern_session_info *s;
@@ -106,6 +148,7 @@ struct Compiler
     int         temp_next = XP_NREGS;   /* temporaries grow down per segment */
     int         temp_min = XP_NREGS;
     bool        compare_nulls = false;
+    int         enc = 0;            /* ENC_*: the session's database encoding */
     std::string err;
 
     bool fail(const std::string &m)
@@ -517,9 +560,425 @@ struct Compiler
                 emit(XO_HASH_FIN, h);
                 return Reg{h, XD_INT, TypeOpCode__int4};
             }
+            case FuncOpCode__DistinctFrom:
+                return distinct_from(k, guard);
+            case FuncOpCode__LeastExpr:
+            case FuncOpCode__GreatestExpr:
+                return minmax(k, guard);
+            case FuncOpCode__CaseWhenExpr:
+                return casewhen(k, guard);
+            case FuncOpCode__ScalarArrayOpAny:
+            case FuncOpCode__ScalarArrayOpAll:
+                return scalar_array_op(k, guard);
+            case FuncOpCode__HashedArrayOp:
+                return hashed_array_op(k, guard);
             default:
                 return func(k, guard);
         }
+    }
+
+    /* text functions count characters by the session's encoding; the
+     * reference raises this message when it has none (xpu_textlib.cu:600) */
+    bool need_encoding(const std::string &fname)
+    {
+        if (enc == ENC_NONE)
+            return fail("No encoding info was supplied");
+        if (enc == ENC_OTHER)
+            return fail(fname + ": this database encoding is not supported by this build");
+        return true;
+    }
+
+    bool in_node(const kern_expression *k, uint32_t off, size_t need)
+    {
+        return off >= offsetof(kern_expression, u) && off + need <= k->len;
+    }
+
+    /* IS DISTINCT FROM (pgfn_DistinctFrom, xpu_common.cu:613-660): the
+     * arguments of its '=' function, the function, then the NULL rules */
+    Reg distinct_from(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        const kern_expression *kcmp = first_arg(k);
+        if (k->nr_args != 1 || !kcmp || !check_node(k, kcmp) || kcmp->nr_args != 2)
+        {
+            fail("malformed DistinctFrom");
+            return bad;
+        }
+        const devfunc *f = lookup_devfunc(kcmp->opcode);
+        if (!f || f->nargs != 2 || f->res != T_bool)
+        {
+            fail("DistinctFrom needs a supported comparison function");
+            return bad;
+        }
+        int r = alloc_reg();
+        int mark = temp_next;
+        Reg av[3];
+        const kern_expression *ka = first_arg(kcmp);
+        av[0] = expr(kcmp, ka, guard);
+        av[1] = expr(kcmp, next_arg(ka), guard);
+        if (!err.empty())
+            return bad;
+        func_op(f, av, r, guard);
+        emit(XO_DISTINCT, r, av[0].r, av[1].r);
+        temp_next = mark;
+        return Reg{r, XD_INT, TypeOpCode__bool};
+    }
+
+    /* LEAST / GREATEST (xpu_common.cu:683-757) */
+    Reg minmax(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        int dom = type_domain(k->exptype);
+        if (dom < 0)
+        {
+            fail("LEAST/GREATEST of unsupported type");
+            return bad;
+        }
+        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        int mark = temp_next;
+        int flags = dom | (k->opcode == FuncOpCode__GreatestExpr ? XMM_GREATEST : 0) |
+                    (k->exptype == T_bpchar ? XMM_BPCHAR : 0);
+        emit(XO_SETNULL, r);
+        const kern_expression *a = first_arg(k);
+        for (uint32_t i = 0; i < k->nr_args; i++, a = next_arg(a))
+        {
+            Reg v = expr(k, a, guard);
+            if (!err.empty())
+                return bad;
+            if (v.dom != dom)
+            {
+                fail("LEAST/GREATEST argument type mismatch");
+                return bad;
+            }
+            emit(XO_MINMAX, r, v.r, 0, XP_REG_TRUE, flags);
+            temp_next = mark;
+        }
+        return Reg{r, dom, (int)k->exptype};
+    }
+
+    /* CASE [key] WHEN ... THEN ... [ELSE] END (pgfn_CaseWhenExpr,
+     * xpu_common.cu:759-811): an arm's result is evaluated, and may raise
+     * errors, only on the rows no earlier arm took */
+    Reg casewhen(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        int dom = type_domain(k->exptype);
+        if (dom < 0 || (k->nr_args % 2) != 0)
+        {
+            fail("CASE of unsupported type or malformed CaseWhenExpr");
+            return bad;
+        }
+        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        int gopen = alloc_reg();
+        int gthen = alloc_reg();
+        int mark = temp_next;
+        emit(XO_SETNULL, r);
+        emit(XO_MOV, gopen, guard);
+        if (k->u.casewhen.case_comp)
+        {
+            if (!in_node(k, k->u.casewhen.case_comp, offsetof(kern_expression, u)))
+            {
+                fail("CASE key out of the expression");
+                return bad;
+            }
+            const kern_expression *kc =
+                (const kern_expression *)((const char *)k + k->u.casewhen.case_comp);
+            if (!check_node(k, kc) || kc->opcode != FuncOpCode__SaveExpr)
+            {
+                fail("CASE key must be a SaveExpr");
+                return bad;
+            }
+            expr(k, kc, gopen);
+            temp_next = mark;
+        }
+        const kern_expression *a = first_arg(k);
+        for (uint32_t i = 0; i + 1 < k->nr_args && err.empty(); i += 2)
+        {
+            Reg c = expr(k, a, gopen);
+            emit(XO_G_TRUE, gthen, c.r, gopen);
+            emit(XO_G_NOTTRUE, gopen, c.r, gopen);
+            temp_next = mark;
+            a = next_arg(a);
+            Reg v = expr(k, a, gthen);
+            if (err.empty() && v.dom != dom)
+            {
+                fail("CASE result type mismatch");
+                return bad;
+            }
+            emit(XO_SELECT, r, v.r, 0, gthen, dom == XD_TEXT ? 1 : 0);
+            temp_next = mark;
+            a = next_arg(a);
+        }
+        if (k->u.casewhen.case_else && err.empty())
+        {
+            if (!in_node(k, k->u.casewhen.case_else, offsetof(kern_expression, u)))
+            {
+                fail("CASE ELSE out of the expression");
+                return bad;
+            }
+            const kern_expression *ke =
+                (const kern_expression *)((const char *)k + k->u.casewhen.case_else);
+            if (!check_node(k, ke))
+                return bad;
+            Reg v = expr(k, ke, gopen);
+            if (err.empty() && v.dom != dom)
+            {
+                fail("CASE ELSE type mismatch");
+                return bad;
+            }
+            emit(XO_SELECT, r, v.r, 0, gopen, dom == XD_TEXT ? 1 : 0);
+        }
+        temp_next = r;
+        return Reg{r, dom, (int)k->exptype};
+    }
+
+    /* scalar op ANY / ALL over a constant array (pgfn_ScalarArrayOp,
+     * xpu_common.cu:815-1011), unrolled: per element the element slot takes
+     * the element and the comparator runs; ANY accumulates with OR and
+     * ALL with AND in SQL three-valued logic, and later elements run only
+     * where the result is still open.  That is PostgreSQL's ANY / ALL
+     * (the reference's ALL returns true where a comparison is false). */
+#define XP_MAX_SAOP_ITEMS   128
+    Reg scalar_array_op(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        const bool use_any = (k->opcode == FuncOpCode__ScalarArrayOpAny);
+        const kern_expression *karr = first_arg(k);
+        const kern_expression *kcmp = karr ? next_arg(karr) : nullptr;
+        if (k->nr_args != 2 || !karr || !check_node(k, karr) || !check_node(k, kcmp) ||
+            kcmp->nr_args != 2)
+        {
+            fail("malformed ScalarArrayOp");
+            return bad;
+        }
+        if (karr->opcode != FuncOpCode__ConstExpr || karr->exptype != TypeOpCode__array)
+        {
+            fail("ScalarArrayOp over a non-constant array is not supported by this build");
+            return bad;
+        }
+        int slot = k->u.saop.elem_slot_id;
+        const kern_expression *kelem = first_arg(kcmp) ? next_arg(first_arg(kcmp)) : nullptr;
+        if (slot >= nslots || !kelem || !check_node(kcmp, kelem) ||
+            kelem->opcode != FuncOpCode__VarExpr || kelem->u.v.var_slot_id != slot)
+        {
+            fail("ScalarArrayOp comparator must read the element slot");
+            return bad;
+        }
+        int etype = slot_type[slot], edom = type_domain(etype);
+        if (edom < 0)
+        {
+            fail("ScalarArrayOp element of unsupported type");
+            return bad;
+        }
+        int acc = alloc_reg();
+        int gr = alloc_reg();
+        int mark = temp_next;
+        if (karr->u.c.const_isnull)
+        {
+            emit(XO_CONST, acc, 0, 0, XP_REG_TRUE, 1, 0);
+            temp_next = mark;
+            return Reg{acc, XD_INT, TypeOpCode__bool};
+        }
+        /* the array image (ArrayType, utils/array.h) */
+        const char *img = karr->u.c.const_value;
+        const char *kend = (const char *)karr + karr->len - 4;
+        if (img + 16 > kend)
+        {
+            fail("ScalarArrayOp array constant out of its expression");
+            return bad;
+        }
+        uint32_t hdr;
+        int32_t ndim, dataoffset;
+        memcpy(&hdr, img, 4);
+        memcpy(&ndim, img + 4, 4);
+        memcpy(&dataoffset, img + 8, 4);
+        const char *aend = img + ((hdr >> 2) & 0x3fffffff);
+        if ((hdr & 3) != 0 || aend > kend || ndim < 0 || ndim > 6 || img + 16 + 8 * ndim > aend)
+        {
+            fail("ScalarArrayOp array constant is malformed");
+            return bad;
+        }
+        int64_t nitems = ndim > 0 ? 1 : 0;
+        for (int d = 0; d < ndim; d++)
+        {
+            int32_t dim;
+            memcpy(&dim, img + 16 + 4 * d, 4);
+            nitems *= dim < 0 ? 0 : dim;
+        }
+        if (nitems > XP_MAX_SAOP_ITEMS)
+        {
+            fail("ScalarArrayOp over more than 128 elements is not supported by this build");
+            return bad;
+        }
+        const uint8_t *nullmap = dataoffset ? (const uint8_t *)img + 16 + 8 * ndim : nullptr;
+        const char *data = img + (dataoffset ? dataoffset : XPU_MAXALIGN(16 + 8 * ndim));
+        if ((nullmap && (const char *)nullmap + (nitems + 7) / 8 > aend) || data > aend)
+        {
+            fail("ScalarArrayOp array constant is malformed");
+            return bad;
+        }
+        const int typlen = type_width(etype);
+        const int align = (etype == T_bool || etype == TypeOpCode__int1) ? 1
+                        : typlen > 0 ? typlen : 4;
+        int er = slot_register(slot);
+        emit(XO_CONST, acc, 0, 0, XP_REG_TRUE, 0, use_any ? 0 : 1);
+        int g = guard;
+        size_t off = 0;
+        for (int64_t i = 0; i < nitems && err.empty(); i++)
+        {
+            if (nullmap && !(nullmap[i >> 3] & (1 << (i & 7))))
+                emit(XO_CONST, er, 0, 0, XP_REG_TRUE, 1, 0);
+            else
+            {
+                /* the element, aligned unless a varlena starts here without a pad byte */
+                if (typlen > 0 || data[off] == 0)
+                    off = XPU_TYPEALIGN(align, off);
+                const char *ep = data + off;
+                size_t esz;
+                if (typlen > 0)
+                    esz = typlen;
+                else
+                {
+                    uint8_t b0 = (uint8_t)ep[0];
+                    if (b0 & 1)
+                        esz = (b0 >> 1) & 0x7f;
+                    else
+                    {
+                        uint32_t h;
+                        memcpy(&h, ep, 4);
+                        esz = (h >> 2) & 0x3fffffff;
+                    }
+                }
+                if (ep + esz > aend || esz == 0)
+                {
+                    fail("ScalarArrayOp array element out of the array");
+                    return bad;
+                }
+                temp_next = mark;
+                Reg cv = const_value(etype, false, ep);
+                emit(edom == XD_TEXT ? XO_MOV2 : XO_MOV, er, cv.r);
+                off += esz;
+            }
+            temp_next = mark;
+            Reg v = expr(k, kcmp, g);
+            emit(use_any ? XO_OR3 : XO_AND3, acc, acc, v.r);
+            temp_next = mark;
+            if (i + 1 < nitems)
+            {
+                emit(use_any ? XO_G_NOTTRUE : XO_G_NOTFALSE, gr, acc, g);
+                g = gr;
+            }
+        }
+        temp_next = mark;
+        return Reg{acc, XD_INT, TypeOpCode__bool};
+    }
+
+    /* HashedArrayOp (codegen.c:2820-2906 and 2458-2660): the expression's
+     * hash table of constant keys, copied into the constant pool and walked
+     * on the device by XO_HAOP.  The table is checked here: every chain stays
+     * inside the expression and the walk of all chains takes at most as many
+     * steps as there are items. */
+    Reg hashed_array_op(const kern_expression *k, int guard)
+    {
+        Reg bad{0, 0, 0};
+        const uint32_t nslots = k->u.haop.nslots;
+        const uint32_t head = (uint32_t)offsetof(kern_expression, u.haop.hslots);
+        const uint32_t aoff = k->args_offset;
+        int kkind, kwidth, rkind, rwidth;
+        if (k->nr_args != 1 || nslots == 0 || nslots > (1U << 20) ||
+            (uint64_t)head + 4ULL * nslots > aoff || aoff > k->len)
+        {
+            fail("malformed HashedArrayOp");
+            return bad;
+        }
+        if (!haop_kind(k->u.haop.elem_type, &kkind, &kwidth) ||
+            !haop_kind(k->exptype, &rkind, &rwidth))
+        {
+            fail("HashedArrayOp of unsupported type");
+            return bad;
+        }
+        const char *base = (const char *)k;
+        const uint32_t dflt = k->u.haop.default_offset;
+        const bool in_list = (dflt == (uint32_t)offsetof(kern_expression, u.haop.__padding));
+        if (dflt && !in_list && (dflt < head + 4 * nslots || dflt + 8 > aoff))
+        {
+            fail("HashedArrayOp default value out of the expression");
+            return bad;
+        }
+        uint32_t nitems = 0;
+        bool null_key = false;
+        const uint32_t max_items = aoff / 8;
+        for (uint32_t sl = 0; sl < nslots; sl++)
+        {
+            uint32_t next = k->u.haop.hslots[sl];
+            while (next != 0)
+            {
+                if ((next & KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK) || next < head + 4 * nslots ||
+                    next + 8 > aoff || ++nitems > max_items)
+                {
+                    fail("HashedArrayOp hash chain out of the expression");
+                    return bad;
+                }
+                uint32_t inext;
+                memcpy(&inext, base + next + 4, 4);
+                uint32_t pos = next + 8;
+                if (inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY)
+                {
+                    uint32_t klen = (uint32_t)kwidth;
+                    if (kkind == XHA_KIND_TEXT)
+                    {
+                        uint8_t b0 = (uint8_t)base[pos];
+                        uint32_t h;
+                        memcpy(&h, base + pos, 4);
+                        klen = (b0 & 1) ? ((b0 >> 1) & 0x7f) : ((h >> 2) & 0x3fffffff);
+                        if (b0 == 0x01 || (b0 & 3) == 2)
+                            klen = 0;
+                    }
+                    if (klen == 0 || pos + klen > aoff)
+                    {
+                        fail("HashedArrayOp key out of the expression");
+                        return bad;
+                    }
+                    pos += klen;
+                }
+                else
+                    null_key = true;
+                if (inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE)
+                {
+                    uint32_t ralign = rkind == XHA_KIND_TEXT ? 4 : (uint32_t)rwidth;
+                    uint32_t koff = pos - (next + 8);
+                    pos = next + 8 + (koff + ralign - 1) / ralign * ralign;
+                    if (pos + (rkind == XHA_KIND_TEXT ? 4 : (uint32_t)rwidth) > aoff)
+                    {
+                        fail("HashedArrayOp value out of the expression");
+                        return bad;
+                    }
+                }
+                next = inext & ~KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK;
+            }
+        }
+        /* the header and items into the constant pool, with the item count
+         * in the (device-unused) elem_ops field */
+        uint64_t poff = out->const_pool.size();
+        out->const_pool.insert(out->const_pool.end(), base, base + aoff);
+        memcpy(&out->const_pool[poff + 40], &nitems, 4);
+        while (out->const_pool.size() % 8)
+            out->const_pool.push_back(0);
+        int r = alloc_reg(rkind == XHA_KIND_TEXT ? 2 : 1);
+        int mark = temp_next;
+        Reg sv = expr(k, first_arg(k), guard);
+        if (!err.empty())
+            return bad;
+        if (sv.dom != type_domain(k->u.haop.elem_type))
+        {
+            fail("HashedArrayOp scalar type mismatch");
+            return bad;
+        }
+        int aux = XHA_KEY(kkind, kwidth) | XHA_RES(rkind, rwidth) |
+                  (in_list && null_key && k->exptype == T_bool ? XHA_NULL_MISS : 0);
+        emit(XO_HAOP, r, sv.r, 0, XP_REG_TRUE, aux, (int64_t)poff);
+        temp_next = mark;
+        return Reg{r, rkind == XHA_KIND_TEXT ? XD_TEXT : type_domain(k->exptype), (int)k->exptype};
     }
 
     Reg func(const kern_expression *k, int guard)
@@ -540,9 +999,9 @@ struct Compiler
         }
         /* the result register comes first, so the arguments' temporaries
          * are released once the operator has consumed them */
-        int r = alloc_reg();
+        int r = alloc_reg(type_domain(f->res) == XD_TEXT ? 2 : 1);
         int mark = temp_next;
-        Reg a[2];
+        Reg a[3];
         const kern_expression *ka = first_arg(k);
         for (int i = 0; i < f->nargs; i++, ka = next_arg(ka))
             a[i] = expr(k, ka, guard);
@@ -582,11 +1041,43 @@ struct Compiler
                 }
                 return Reg{r, XD_INT, TypeOpCode__bool};
             case K_textcmp:
-                emit(XO_CMP_TEXT, r, a[0].r, a[1].r, XP_REG_TRUE, cmpop | cn);
+                /* bpchar compares without trailing spaces (xpu_textlib.cu:59-69) */
+                emit(XO_CMP_TEXT, r, a[0].r, a[1].r, XP_REG_TRUE,
+                     cmpop | cn | (f->t0 == T_bpchar ? XCMP_BPCHAR : 0));
                 return Reg{r, XD_INT, TypeOpCode__bool};
             case K_textlen:
-                emit(XO_TEXTLEN, r, a[0].r);
+            {
+                /* pgfn_textlen / pgfn_bpcharlen (xpu_textlib.cu:583-720) */
+                if (!need_encoding(fname))
+                    return bad;
+                emit(XO_TEXTLEN, r, a[0].r, 0, XP_REG_TRUE,
+                     (enc == ENC_UTF8 ? XENC_UTF8 : 0) | (f->t0 == T_bpchar ? XTXT_BPCHAR : 0));
                 return Reg{r, XD_INT, TypeOpCode__int4};
+            }
+            case K_like:
+            {
+                /* PG_TEXTLIKE_TEMPLATE / PG_BPCHARLIKE_TEMPLATE (xpu_textlib.cu:1087-1155) */
+                if (!need_encoding(fname))
+                    return bad;
+                int flags = (enc == ENC_UTF8 ? XENC_UTF8 : 0) |
+                            (f->t0 == T_bpchar ? XTXT_BPCHAR : 0) |
+                            ((f->op == O_ilike || f->op == O_nilike) ? XLIKE_ICASE : 0) |
+                            ((f->op == O_nlike || f->op == O_nilike) ? XLIKE_NEGATE : 0);
+                emit(XO_LIKE, r, a[0].r, a[1].r, guard, flags, msg("invalid escape in LIKE pattern"));
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            }
+            case K_substr:
+            {
+                /* pgfn_substring / pgfn_substring_nolen (xpu_textlib.cu:1261-1322) */
+                if (!need_encoding(fname))
+                    return bad;
+                int64_t imm = f->nargs > 2
+                    ? ((int64_t)a[2].r | XSUB_HAS_COUNT |
+                       ((int64_t)msg("negative substring length not allowed") << 16))
+                    : 0;
+                emit(XO_SUBSTR, r, a[0].r, a[1].r, guard, enc == ENC_UTF8 ? XENC_UTF8 : 0, imm);
+                return Reg{r, XD_TEXT, TypeOpCode__text};
+            }
             case K_arith:
                 if (type_domain(f->res) == XD_INT)
                 {
@@ -835,6 +1326,7 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     C.length = length;
     C.out = out;
     C.P = P;
+    C.enc = encoding_kind(s, length);
 #define FAIL(msg) do { xpu_set_error(eb, ERRCODE_DEVICE_ERROR, "xpu_compile.cpp", __LINE__, \
                                      __func__, (msg)); return ERRCODE_DEVICE_ERROR; } while (0)
     if (length < offsetof(kern_session_info, poffset) ||
@@ -1379,7 +1871,8 @@ xpu_plan_dump(const xpu_compiled *c)
         "NEG_F32", "ABS_F32", "NEG_F64", "ABS_F64", "I2I", "F32_I", "F64_I", "I_F32", "I_F64",
         "F32_F64", "F64_F32", "I_BOOL", "TEXTLEN", "AND3", "OR3", "NOT3", "ISNULL", "ISNOTNULL",
         "BOOLTEST", "COALESCE", "G_NOTFALSE", "G_NOTTRUE", "G_ISNULL", "G_POSITIVE", "G_NONZERO",
-        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL", "PARAM"};
+        "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL", "PARAM", "G_TRUE", "SELECT",
+        "DISTINCT", "MINMAX", "LIKE", "SUBSTR", "HAOP"};
     const xplan *P = &c->plan;
     std::string s;
     char buf[256];

@@ -331,6 +331,238 @@ ldtup(Lane &L, int d, const kern_tupitem *t, const kern_data_store *kds, int res
 }
 
 /* ------------------------------------------------------------------
+ * text helpers of the string operators
+ * ------------------------------------------------------------------ */
+/* bytes of the character at s: pg_utf8_mblen (xpu_textlib.cu:774-787), or 1
+ * for a single-byte database encoding */
+__device__ __forceinline__ int
+txt_mblen(const uint8_t *s, bool utf8)
+{
+    if (!utf8)
+        return 1;
+    const uint8_t c = *s;
+    if ((c & 0x80) == 0) return 1;
+    if ((c & 0xe0) == 0xc0) return 2;
+    if ((c & 0xf0) == 0xe0) return 3;
+    if ((c & 0xf8) == 0xf0) return 4;
+    return 1;
+}
+
+/* length without trailing spaces (bpchar_truelen, xpu_textlib.cu:59-69) */
+__device__ __forceinline__ int64_t
+txt_truelen(const uint8_t *s, int64_t len)
+{
+    while (len > 0 && s[len - 1] == ' ')
+        len--;
+    return len;
+}
+
+__device__ __forceinline__ int
+txt_bytecmp(const uint8_t *s1, int64_t l1, const uint8_t *s2, int64_t l2)
+{
+    const int64_t n = l1 < l2 ? l1 : l2;
+    int c = 0;
+    for (int64_t i = 0; i < n && c == 0; i++)
+        c = (int)s1[i] - (int)s2[i];
+    if (c == 0)
+        c = l1 < l2 ? -1 : (l1 > l2 ? 1 : 0);
+    return c;
+}
+
+__device__ __forceinline__ uint8_t
+like_fold(uint8_t c, bool ic)
+{
+    return (ic && c >= 'a' && c <= 'z') ? (uint8_t)(c - ('a' - 'A')) : c;
+}
+
+/* LIKE / ILIKE with the result of GenericMatchText (xpu_textlib.cu:915-1085)
+ * and its error cases, without recursion: the newest % is the only point a
+ * mismatch resumes from, since the reference's recursive search answers
+ * "abort" rather than "false" once a deeper % has scanned to the end of the
+ * text.  Returns 1 / 0, or -1 for an invalid escape.  Every loop advances t
+ * or p, so the walk ends. */
+__device__ int
+like_match(const uint8_t *t, int64_t tlen, const uint8_t *p, int64_t plen, bool ic, bool utf8)
+{
+    int64_t ti = 0, pi = 0, bt_t = -1, bt_p = -1;
+    uint8_t firstpat = 0;
+    if (plen == 1 && p[0] == '%')
+        return 1;
+    for (;;)
+    {
+        bool mismatch = false, scan = false;
+        while (ti < tlen && pi < plen && !mismatch && !scan)
+        {
+            const uint8_t c = p[pi];
+            if (c == '\\')
+            {
+                if (++pi >= plen)
+                    return -1;
+                if (like_fold(p[pi], ic) != like_fold(t[ti], ic))
+                    mismatch = true;
+                else
+                {
+                    ti++;
+                    pi++;
+                }
+            }
+            else if (c == '%')
+            {
+                pi++;
+                while (pi < plen)
+                {
+                    if (p[pi] == '%')
+                        pi++;
+                    else if (p[pi] == '_')
+                    {
+                        if (ti >= tlen)
+                            return 0;
+                        ti += txt_mblen(t + ti, utf8);
+                        pi++;
+                    }
+                    else
+                        break;
+                }
+                if (pi >= plen)
+                    return 1;
+                if (p[pi] == '\\')
+                {
+                    if (plen - pi < 2)
+                        return -1;
+                    firstpat = like_fold(p[pi + 1], ic);
+                }
+                else
+                    firstpat = like_fold(p[pi], ic);
+                bt_p = pi;
+                bt_t = ti;
+                scan = true;
+            }
+            else if (c == '_')
+            {
+                ti += txt_mblen(t + ti, utf8);
+                pi++;
+            }
+            else if (like_fold(c, ic) != like_fold(t[ti], ic))
+                mismatch = true;
+            else
+            {
+                ti++;
+                pi++;
+            }
+        }
+        if (!mismatch && !scan)
+        {
+            if (ti < tlen)
+                mismatch = true;
+            else
+            {
+                while (pi < plen && p[pi] == '%')
+                    pi++;
+                return pi >= plen ? 1 : 0;
+            }
+        }
+        if (mismatch)
+        {
+            if (bt_p < 0)
+                return 0;
+            bt_t += txt_mblen(t + bt_t, utf8);
+        }
+        while (bt_t < tlen && like_fold(t[bt_t], ic) != firstpat)
+            bt_t += txt_mblen(t + bt_t, utf8);
+        if (bt_t >= tlen)
+            return 0;
+        ti = bt_t;
+        pi = bt_p;
+    }
+}
+
+/* __substring_common (xpu_textlib.cu:1159-1259) on 0-based start and count
+ * (-1: to the end): the byte range (*off, *len) of str */
+__device__ __forceinline__ void
+substr_range(const uint8_t *str, int64_t length, int32_t start, int32_t count, bool utf8,
+             int64_t *off, int64_t *len)
+{
+    *off = 0;
+    if (!utf8)
+    {
+        if (start >= length || count == 0 || (count > 0 && (int64_t)start + count <= 0))
+            *len = 0;
+        else if (start < 0)
+            *len = (count < 0 || (int64_t)start + count >= length) ? length : (int64_t)start + count;
+        else
+        {
+            const int64_t end = (count < 0 || (int64_t)start + count > length) ? length
+                                                                                : (int64_t)start + count;
+            *off = start;
+            *len = end - start;
+        }
+        return;
+    }
+    if (start < 0)
+    {
+        if (count < 0)
+        {
+            *len = length;
+            return;
+        }
+        count = start + count;
+        if (count < 0)
+        {
+            *len = 0;
+            return;
+        }
+        start = 0;
+    }
+    int64_t pos = 0, val = -1;
+    for (int64_t i = 0; pos < length; i++)
+    {
+        const int w = txt_mblen(str + pos, true);
+        if (pos + w > length)
+            break;
+        if (i == start)
+            val = pos;
+        if (count >= 0 && i == (int64_t)start + count)
+            break;
+        pos += w;
+    }
+    if (val >= 0)
+    {
+        *off = val;
+        *len = pos - val;
+    }
+    else
+        *len = 0;
+}
+
+/* a varlena image's payload (1- or 4-byte header) */
+__device__ __forceinline__ const uint8_t *
+varlena_payload(const uint8_t *v, int64_t *len)
+{
+    if (v[0] & 0x01)
+    {
+        *len = ((v[0] >> 1) & 0x7f) - 1;
+        return v + 1;
+    }
+    uint32_t h;
+    __builtin_memcpy(&h, v, 4);
+    *len = (int64_t)((h >> 2) & 0x3fffffff) - 4;
+    return v + 4;
+}
+
+/* a fixed-width image of 1/2/4/8 bytes, sign-extended (float bits kept) */
+__device__ __forceinline__ int64_t
+load_fixed(const uint8_t *v, int width)
+{
+    switch (width)
+    {
+        case 1: return (int8_t)v[0];
+        case 2: { int16_t x; __builtin_memcpy(&x, v, 2); return x; }
+        case 4: { int32_t x; __builtin_memcpy(&x, v, 4); return x; }
+        default: { int64_t x; __builtin_memcpy(&x, v, 8); return x; }
+    }
+}
+
+/* ------------------------------------------------------------------
  * the register-program interpreter (one inlined copy per kernel)
  * ------------------------------------------------------------------ */
 #define RAISE(g, m) do { if (!RNULL(L, g) && L.R[g] != 0 && !L.err && !L.dead) \
@@ -565,13 +797,12 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     const uint8_t *s1 = (const uint8_t *)(uintptr_t)L.R[a];
                     const uint8_t *s2 = (const uint8_t *)(uintptr_t)L.R[b];
                     int64_t l1 = L.R[a + 1], l2 = L.R[b + 1];
-                    int64_t n = l1 < l2 ? l1 : l2;
-                    int c = 0;
-                    for (int64_t i = 0; i < n && c == 0; i++)
-                        c = (int)s1[i] - (int)s2[i];
-                    if (c == 0)
-                        c = l1 < l2 ? -1 : (l1 > l2 ? 1 : 0);
-                    r = cmp_by(op, c);
+                    if (o.aux & XCMP_BPCHAR)
+                    {
+                        l1 = txt_truelen(s1, l1);
+                        l2 = txt_truelen(s2, l2);
+                    }
+                    r = cmp_by(op, txt_bytecmp(s1, l1, s2, l2));
                 }
                 L.R[d] = r;
                 SETNULL(L, d, 0);
@@ -778,7 +1009,70 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 break;
             }
             case XO_I_BOOL: L.R[d] = (L.R[a] != 0); SETNULL(L, d, RNULL(L, a)); break;
-            case XO_TEXTLEN: L.R[d] = L.R[a + 1]; SETNULL(L, d, RNULL(L, a)); break;
+            case XO_TEXTLEN:
+            {
+                /* pgfn_textlen / pgfn_bpcharlen (xpu_textlib.cu:583-720):
+                 * characters by the database encoding, up to a NUL byte */
+                const uint8_t *s = (const uint8_t *)(uintptr_t)L.R[a];
+                int64_t len = L.R[a + 1];
+                const bool n = RNULL(L, a);
+                if (!n && (o.aux & XTXT_BPCHAR))
+                    len = txt_truelen(s, len);
+                if (!n && (o.aux & XENC_UTF8))
+                {
+                    int64_t cnt = 0, i = 0;
+                    while (i < len && s[i] != 0)
+                    {
+                        i += txt_mblen(s + i, true);
+                        cnt++;
+                    }
+                    len = cnt;
+                }
+                L.R[d] = n ? 0 : len;
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_LIKE:
+            {
+                const bool n = RNULL(L, a) | RNULL(L, b);
+                int64_t v = 0;
+                if (!n)
+                {
+                    const uint8_t *t = (const uint8_t *)(uintptr_t)L.R[a];
+                    int64_t tlen = L.R[a + 1];
+                    if (o.aux & XTXT_BPCHAR)
+                        tlen = txt_truelen(t, tlen);
+                    const int m = like_match(t, tlen, (const uint8_t *)(uintptr_t)L.R[b], L.R[b + 1],
+                                             (o.aux & XLIKE_ICASE) != 0, (o.aux & XENC_UTF8) != 0);
+                    if (m < 0)
+                        RAISE(g, o.imm);
+                    v = (m > 0) != ((o.aux & XLIKE_NEGATE) != 0);
+                }
+                L.R[d] = v;
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_SUBSTR:
+            {
+                const bool has_count = (o.imm & XSUB_HAS_COUNT) != 0;
+                const int rc = (int)(o.imm & 0xff);
+                const bool n = RNULL(L, a) | RNULL(L, b) | (has_count && RNULL(L, rc));
+                int64_t off = 0, len = 0;
+                const uint8_t *str = (const uint8_t *)(uintptr_t)L.R[a];
+                if (!n)
+                {
+                    const int32_t count = has_count ? (int32_t)L.R[rc] : -1;
+                    if (has_count && count < 0)
+                        RAISE(g, (o.imm >> 16) & 0xffff);
+                    else
+                        substr_range(str, L.R[a + 1], (int32_t)L.R[b] - 1, count,
+                                     (o.aux & XENC_UTF8) != 0, &off, &len);
+                }
+                L.R[d] = (int64_t)(uintptr_t)(str + off);
+                L.R[d + 1] = len;
+                SETNULL(L, d, n);
+                break;
+            }
             case XO_AND3:
             {
                 bool na = RNULL(L, a), nb = RNULL(L, b);
@@ -897,6 +1191,165 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 L.R[d] = kp.params[o.imm];
                 SETNULL(L, d, (kp.param_nulls >> o.imm) & 1);
                 break;
+            case XO_G_TRUE:
+                L.R[d] = (L.R[b] != 0) && !RNULL(L, a) && L.R[a] != 0;
+                SETNULL(L, d, 0);
+                break;
+            case XO_SELECT:
+                if (L.R[g] != 0)
+                {
+                    L.R[d] = L.R[a];
+                    if (o.aux & 1)
+                        L.R[d + 1] = L.R[a + 1];
+                    SETNULL(L, d, RNULL(L, a));
+                }
+                break;
+            case XO_DISTINCT:
+            {
+                /* pgfn_DistinctFrom (xpu_common.cu:613-660) */
+                const bool na = RNULL(L, a), nb = RNULL(L, b);
+                L.R[d] = (na || nb) ? (int64_t)!(na && nb) : (int64_t)(L.R[d] == 0);
+                SETNULL(L, d, 0);
+                break;
+            }
+            case XO_MINMAX:
+            {
+                /* pgfn_LeastExpr / pgfn_GreatestExpr (xpu_common.cu:683-757):
+                 * the type's comparison; floats by < and > only */
+                if (RNULL(L, a))
+                    break;
+                bool take = RNULL(L, d);
+                if (!take)
+                {
+                    int c;
+                    const int dom = o.aux & 3;
+                    if (dom == XD_INT)
+                        c = L.R[d] > L.R[a] ? 1 : (L.R[d] < L.R[a] ? -1 : 0);
+                    else if (dom == XD_F32)
+                    {
+                        const float x = as_f32(L.R[d]), y = as_f32(L.R[a]);
+                        c = x > y ? 1 : (x < y ? -1 : 0);
+                    }
+                    else if (dom == XD_F64)
+                    {
+                        const double x = as_f64(L.R[d]), y = as_f64(L.R[a]);
+                        c = x > y ? 1 : (x < y ? -1 : 0);
+                    }
+                    else
+                    {
+                        const uint8_t *s1 = (const uint8_t *)(uintptr_t)L.R[d];
+                        const uint8_t *s2 = (const uint8_t *)(uintptr_t)L.R[a];
+                        int64_t l1 = L.R[d + 1], l2 = L.R[a + 1];
+                        if (o.aux & XMM_BPCHAR)
+                        {
+                            l1 = txt_truelen(s1, l1);
+                            l2 = txt_truelen(s2, l2);
+                        }
+                        c = txt_bytecmp(s1, l1, s2, l2);
+                    }
+                    take = (o.aux & XMM_GREATEST) ? (c < 0) : (c > 0);
+                }
+                if (take)
+                {
+                    L.R[d] = L.R[a];
+                    if ((o.aux & 3) == XD_TEXT)
+                        L.R[d + 1] = L.R[a + 1];
+                    SETNULL(L, d, 0);
+                }
+                break;
+            }
+            case XO_HAOP:
+            {
+                /* pgfn_HashedArrayOp (xpu_common.cu:1013-1088) over the
+                 * expression's bytes in the constant pool; equality as the
+                 * type's '=' (floats: NaN equals NaN only) */
+                const uint8_t *kx = (const uint8_t *)const_pool + o.imm;
+                const int kkind = o.aux & 7, kwidth = (o.aux >> 3) & 15;
+                const int rkind = (o.aux >> 7) & 7, rwidth = (o.aux >> 10) & 15;
+                const bool text_res = (rkind == XHA_KIND_TEXT);
+                bool n = RNULL(L, a);
+                const uint8_t *val = nullptr;
+                if (!n)
+                {
+                    uint32_t nslots, default_offset, nitems;
+                    __builtin_memcpy(&nslots, kx + 52, 4);
+                    __builtin_memcpy(&default_offset, kx + 56, 4);
+                    /* the item count: the compiler stores it in the
+                     * copy's elem_ops field, the bound of a chain walk */
+                    __builtin_memcpy(&nitems, kx + 40, 4);
+                    const uint32_t h = (kkind == XHA_KIND_TEXT)
+                        ? hash_bytes((const uint8_t *)(uintptr_t)L.R[a], (uint32_t)L.R[a + 1])
+                        : hash_fixed((uint64_t)L.R[a], kwidth);
+                    uint32_t next;
+                    __builtin_memcpy(&next, kx + 64 + 4 * (h % nslots), 4);
+                    bool found = false;
+                    for (uint32_t step = 0; next != 0 && step < nitems && !found; step++)
+                    {
+                        const uint8_t *item = kx + next;
+                        uint32_t ihash, inext;
+                        __builtin_memcpy(&ihash, item, 4);
+                        __builtin_memcpy(&inext, item + 4, 4);
+                        if (ihash == h && (inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY))
+                        {
+                            const uint8_t *key = item + 8;
+                            int64_t klen = kwidth;
+                            bool eq;
+                            if (kkind == XHA_KIND_TEXT)
+                            {
+                                const uint8_t *kp8 = varlena_payload(key, &klen);
+                                eq = txt_bytecmp((const uint8_t *)(uintptr_t)L.R[a], L.R[a + 1],
+                                                 kp8, klen) == 0;
+                                klen += (int64_t)(kp8 - key);
+                            }
+                            else if (kkind == XHA_KIND_F32)
+                                eq = fcmp(XC_EQ, (double)as_f32(L.R[a]),
+                                          (double)as_f32((uint32_t)load_fixed(key, 4)));
+                            else if (kkind == XHA_KIND_F64)
+                                eq = fcmp(XC_EQ, as_f64(L.R[a]), as_f64(load_fixed(key, 8)));
+                            else
+                                eq = L.R[a] == load_fixed(key, kwidth);
+                            if (eq)
+                            {
+                                found = true;
+                                if (inext & KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE)
+                                {
+                                    const int ralign = text_res ? 4 : rwidth;
+                                    val = key + ((klen + ralign - 1) / ralign) * ralign;
+                                }
+                                else
+                                    n = true;
+                            }
+                        }
+                        next = inext & ~KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK;
+                    }
+                    if (!found)
+                    {
+                        if (o.aux & XHA_NULL_MISS)
+                            n = true;
+                        else if (default_offset)
+                            val = kx + default_offset;
+                        else
+                            n = true;
+                    }
+                }
+                if (!n && val)
+                {
+                    if (text_res)
+                    {
+                        int64_t len;
+                        const uint8_t *pl = varlena_payload(val, &len);
+                        L.R[d] = (int64_t)(uintptr_t)pl;
+                        L.R[d + 1] = len;
+                    }
+                    else
+                        L.R[d] = (rkind == XHA_KIND_F32) ? (int64_t)(uint32_t)load_fixed(val, 4)
+                                                         : load_fixed(val, rwidth);
+                }
+                else
+                    L.R[d] = 0;
+                SETNULL(L, d, n || !val);
+                break;
+            }
             default:
                 break;
         }

@@ -49,6 +49,24 @@ enum {
 /* comparison operators (aux low byte of XO_CMP*) */
 enum { XC_EQ = 0, XC_NE, XC_LT, XC_LE, XC_GT, XC_GE };
 
+/* text operator flags (aux of XO_TEXTLEN, XO_CMP_TEXT, XO_LIKE, XO_SUBSTR) */
+#define XENC_UTF8           0x01    /* characters by UTF-8 byte patterns (else 1 byte) */
+#define XTXT_BPCHAR         0x02    /* bpchar: trailing spaces do not count */
+#define XCMP_BPCHAR         0x200   /* XO_CMP_TEXT: bpchar operands */
+#define XLIKE_ICASE         0x04    /* ILIKE: ASCII letters compare upper-cased */
+#define XLIKE_NEGATE        0x08    /* NOT LIKE */
+#define XSUB_HAS_COUNT      0x100
+#define XMM_GREATEST        0x10    /* XO_MINMAX: keep the greatest */
+#define XMM_BPCHAR          0x20
+/* XO_HAOP aux: key kind and width, result kind and width, flags */
+#define XHA_KIND_INT        1
+#define XHA_KIND_F32        2
+#define XHA_KIND_F64        3
+#define XHA_KIND_TEXT       4
+#define XHA_KEY(kind, width)    ((kind) | ((width) << 3))
+#define XHA_RES(kind, width)    (((kind) | ((width) << 3)) << 7)
+#define XHA_NULL_MISS       0x4000  /* no match gives NULL (IN list with a NULL item) */
+
 /* integer result classes for range checks */
 enum { XT_BOOL = 0, XT_INT1, XT_INT2, XT_INT4, XT_INT8 };
 
@@ -105,6 +123,28 @@ enum xop_code {
     /* dst <- query parameter imm of the launch (fixed-width parameters are
        run-time values, so plans differing only in them share a kernel) */
     XO_PARAM,
+    /* guard: dst <- R[b] && R[a] is true (not NULL and not zero) */
+    XO_G_TRUE,
+    /* conditional move: where R[guard] is true, dst <- R[a] with its null
+       flag (aux & 1: text, two registers) */
+    XO_SELECT,
+    /* IS DISTINCT FROM: dst holds the '=' result of R[a], R[b]; dst <- false
+       if both are NULL, true if one is, else NOT dst (never NULL) */
+    XO_DISTINCT,
+    /* LEAST / GREATEST step: if R[a] is not NULL and dst is NULL or R[a]
+       compares below (above) it, dst <- R[a]; aux = XD_* domain | XMM_* */
+    XO_MINMAX,
+    /* dst <- LIKE of text R[a],R[a+1] and pattern R[b],R[b+1]; aux = XLIKE_*;
+       an invalid escape raises message imm where R[guard] */
+    XO_LIKE,
+    /* dst,dst+1 <- substring of text R[a],R[a+1] from 1-based start R[b];
+       imm = count register | XSUB_HAS_COUNT | message << 16 (a negative
+       count raises the message where R[guard]); aux = XENC_* */
+    XO_SUBSTR,
+    /* dst <- value of the HashedArrayOp table at const pool offset imm for
+       the key R[a] (hash-slot walk of the reference's kern_expression
+       layout); aux = XHA_* key / result description */
+    XO_HAOP,
     XO_NUM_OPS
 };
 

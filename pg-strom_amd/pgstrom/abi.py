@@ -57,6 +57,9 @@ T_BITS_OFFSET = 15
 
 KERN_EXPRESSION_MAGIC = 0x4b657870
 KEXP_FLAG__IS_PUSHED_DOWN = 0x0001
+KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY = 0x01
+KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE = 0x02
+KERN_HASHED_ARRAY_OP_ITEM_NEXT_MASK = 0x07
 SPECIAL_DEPTH__PREAGG_FINAL = -2
 
 DEVKIND__NVIDIA_GPU = 0x00000001

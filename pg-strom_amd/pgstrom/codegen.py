@@ -121,6 +121,105 @@ class Coalesce(Expr):
         self.args = list(args)
 
 
+class DistinctFrom(Expr):
+    """a IS DISTINCT FROM b over a comparator function (codegen.c:1929-1949):
+    DistinctFrom with one argument, the '=' function of a and b."""
+
+    def __init__(self, cmp):
+        self.cmp = cmp
+        self.typname = "bool"
+
+
+class MinMax(Expr):
+    """LEAST / GREATEST (codegen.c:2373-2420)."""
+
+    def __init__(self, kind, typname, *args):
+        assert kind in ("least", "greatest")
+        self.kind = kind
+        self.typname = typname
+        self.args = list(args)
+
+
+class CaseWhen(Expr):
+    """CASE [key] WHEN cond THEN result ... [ELSE default] END
+    (codegen.c:2701-2810).  whens: [(cond, result)].  key: (slot, expr) of
+    CASE <key>: a SaveExpr of the key into the slot, which the conditions
+    read as Var(slot)."""
+
+    def __init__(self, typname, whens, default=None, key=None):
+        self.typname = typname
+        self.whens = list(whens)
+        self.default = default
+        self.key = key
+
+
+ARRAY_OIDS = {"bool": 1000, "int2": 1005, "int4": 1007, "int8": 1016, "float4": 1021,
+              "float8": 1022, "text": 1009, "bpchar": 1014}
+
+
+def array_image(elem_typname, values):
+    """A one-dimensional PostgreSQL array (ArrayType, utils/array.h) of
+    elem_typname holding values (None = NULL element)."""
+    oid, typlen, byval, align, _ = kdsmod.PG_TYPES[elem_typname]
+    n = len(values)
+    hasnull = any(v is None for v in values)
+    head = 16 + 8          # vl_len_, ndim, dataoffset, elemtype; dims[1], lbound[1]
+    bitmap = b""
+    if hasnull:
+        bits = bytearray((n + 7) // 8)
+        for i, v in enumerate(values):
+            if v is not None:
+                bits[i >> 3] |= 1 << (i & 7)
+        bitmap = bytes(bits)
+    dataoffset = abi.maxalign(head + len(bitmap)) if hasnull else 0
+    data = bytearray()
+    for v in values:
+        if v is None:
+            continue
+        d = encode_const_value(elem_typname, v)
+        if typlen > 0 or True:
+            data += b"\0" * ((-len(data)) % align)
+        data += d
+    start = abi.maxalign(head + len(bitmap))
+    total = start + len(data)
+    img = struct.pack("<IiiI", total << 2, 1 if n else 0, dataoffset, oid)
+    img += struct.pack("<ii", n, 1) + bitmap
+    img += b"\0" * (start - len(img))
+    return img + bytes(data)
+
+
+class ScalarArrayOp(Expr):
+    """scalar <op> ANY / ALL (ARRAY[...]) over a constant array
+    (codegen.c:2909-3060): arg 1 the array constant, arg 2 the comparator
+    function of the scalar and Var(elem_slot), the array element."""
+
+    def __init__(self, use_any, fname, scalar, elem_slot, elem_typname, values):
+        self.use_any = use_any
+        self.fname = fname
+        self.scalar = scalar
+        self.elem_slot = elem_slot
+        self.elem_typname = elem_typname
+        self.values = values        # list, or None for a NULL array
+        self.typname = "bool"
+
+
+class HashedArrayOp(Expr):
+    """scalar = ANY (constant array), or CASE scalar WHEN const THEN const ...
+    ELSE const END, as the reference's hash table of constant keys
+    (codegen.c:2820-2906 and 2458-2660).  items: [(key or None, value or
+    None)] in order; default: the value of no match (None = NULL)."""
+
+    def __init__(self, typname, elem_typname, scalar, items, default=None, in_list=False):
+        self.typname = typname
+        self.elem_typname = elem_typname
+        self.scalar = scalar
+        self.items = list(items)
+        self.default = default
+        # scalar = ANY (array): the default is the zero byte of the header's
+        # padding, i.e. false (codegen.c:2855)
+        self.in_list = in_list
+
+
 class HashValue(Expr):
     """HashValue over its arguments (pgfn_HashValue, xpu_common.cu:1363-1398)."""
 
@@ -142,7 +241,7 @@ def Not(a):
 
 
 def encode_const_value(typname, value):
-    if typname in ("text", "bytea"):
+    if typname in ("text", "bytea", "bpchar"):
         v = bytes(value)
         return struct.pack("<I", (len(v) + 4) << 2) + v
     return kdsmod.encode_datum(typname, value)
@@ -188,6 +287,17 @@ def emit_expr(buf, e, flags=0):
         return _emit_with_args(buf, tc, SPECIAL["CoalesceExpr"], e.args, flags)
     if isinstance(e, HashValue):
         return _emit_with_args(buf, tc, SPECIAL["HashValue"], e.args, flags)
+    if isinstance(e, DistinctFrom):
+        return _emit_with_args(buf, tc, SPECIAL["DistinctFrom"], [e.cmp], flags)
+    if isinstance(e, MinMax):
+        return _emit_with_args(buf, tc, SPECIAL["LeastExpr" if e.kind == "least"
+                                                else "GreatestExpr"], e.args, flags)
+    if isinstance(e, CaseWhen):
+        return _emit_casewhen(buf, tc, e, flags)
+    if isinstance(e, ScalarArrayOp):
+        return _emit_saop(buf, tc, e, flags)
+    if isinstance(e, HashedArrayOp):
+        return _emit_haop(buf, tc, e, flags)
     if isinstance(e, SaveExpr):
         pos = _append_aligned(buf, _kexp_head(tc, SPECIAL["SaveExpr"], 1, 48, flags) +
                               struct.pack("<H6x", e.slot))
@@ -203,6 +313,116 @@ def _emit_with_args(buf, tc, opcode, args, flags):
         emit_expr(buf, a, flags & ~abi.KEXP_FLAG__IS_PUSHED_DOWN)
     _finish(buf, pos)
     return pos
+
+
+def _emit_casewhen(buf, tc, e, flags):
+    sub = flags & ~abi.KEXP_FLAG__IS_PUSHED_DOWN
+    pos = _append_aligned(buf, _kexp_head(tc, SPECIAL["CaseWhenExpr"], 2 * len(e.whens), 48, flags)
+                          + struct.pack("<II", 0, 0))
+    case_comp = case_else = 0
+    args_offset = 48
+    if e.key is not None:
+        slot, kexpr = e.key
+        _pad_to(buf, 8)
+        case_comp = len(buf) - pos
+        emit_expr(buf, SaveExpr(slot, kexpr), sub)
+        _pad_to(buf, 8)
+        args_offset = len(buf) - pos
+    for cond, res in e.whens:
+        emit_expr(buf, cond, sub)
+        emit_expr(buf, res, sub)
+    if e.default is not None:
+        _pad_to(buf, 8)
+        case_else = len(buf) - pos
+        emit_expr(buf, e.default, sub)
+    struct.pack_into("<I", buf, pos + 36, args_offset)
+    struct.pack_into("<II", buf, pos + 40, case_comp, case_else)
+    _finish(buf, pos)
+    return pos
+
+
+def _emit_array_const(buf, elem_typname, values, flags):
+    """ConstExpr of an array type (exptype TypeOpCode__array)."""
+    head = _kexp_head(TYPE_CODE["array"], SPECIAL["ConstExpr"], flags=flags) + \
+        struct.pack("<I?3x", ARRAY_OIDS[elem_typname], values is None)
+    pos = _append_aligned(buf, head)
+    if values is not None:
+        buf += array_image(elem_typname, values)
+    _finish(buf, pos)
+    return pos
+
+
+def _emit_saop(buf, tc, e, flags):
+    sub = flags & ~abi.KEXP_FLAG__IS_PUSHED_DOWN
+    opcode = SPECIAL["ScalarArrayOpAny" if e.use_any else "ScalarArrayOpAll"]
+    pos = _append_aligned(buf, _kexp_head(tc, opcode, 2, 48, flags) + struct.pack("<H6x", e.elem_slot))
+    _emit_array_const(buf, e.elem_typname, e.values, sub)
+    emit_expr(buf, Func(e.fname, e.scalar, Var(e.elem_slot, e.elem_typname)), sub)
+    _finish(buf, pos)
+    return pos
+
+
+def _datum_image(typname, value):
+    return encode_const_value(typname, value)
+
+
+def _emit_haop(buf, tc, e, flags):
+    """HashedArrayOp: header with the hash slots, then the items (hash, next
+    with HAS_KEY / HAS_VALUE bits, key image, value image aligned to the
+    result type), the default value, then the scalar argument."""
+    from . import hashing
+    sub = flags & ~abi.KEXP_FLAG__IS_PUSHED_DOWN
+    nitems = len(e.items)
+    nslots = nitems * 5 // 4 + 1
+    head_sz = abi.maxalign(64 + 4 * nslots)
+    tmp = bytearray(head_sz)
+    hslots = [0] * nslots
+    r_align = kdsmod.PG_TYPES[e.typname][3]
+    for key, value in e.items:
+        h = 0 if key is None else hashing.hash_any(_hash_image(e.elem_typname, key))
+        idx = h % nslots
+        nxt = hslots[idx]
+        if key is not None:
+            nxt |= abi.KERN_HASHED_ARRAY_OP_ITEM_HAS_KEY
+        if value is not None:
+            nxt |= abi.KERN_HASHED_ARRAY_OP_ITEM_HAS_VALUE
+        _pad_to(tmp, 8)
+        hslots[idx] = len(tmp)
+        tmp += struct.pack("<II", h, nxt)
+        klen = 0
+        if key is not None:
+            kimg = _datum_image(e.elem_typname, key)
+            tmp += kimg
+            klen = len(kimg)
+        if value is not None:
+            gap = (-klen) % r_align
+            tmp += b"\0" * gap
+            tmp += _datum_image(e.typname, value)
+    default_offset = 0
+    if e.in_list:
+        default_offset = 60     # offsetof(kern_expression, u.haop.__padding)
+    elif e.default is not None:
+        _pad_to(tmp, 8)
+        default_offset = len(tmp)
+        tmp += _datum_image(e.typname, e.default)
+    _pad_to(tmp, 8)
+    args_offset = len(tmp)
+    tmp[0:40] = _kexp_head(tc, SPECIAL["HashedArrayOp"], 1, args_offset, flags)
+    struct.pack_into("<QiII", tmp, 40, 0, TYPE_CODE[e.elem_typname], nslots, default_offset)
+    for i, h in enumerate(hslots):
+        struct.pack_into("<I", tmp, 64 + 4 * i, h)
+    pos = _append_aligned(buf, bytes(tmp))
+    emit_expr(buf, e.scalar, sub)
+    _finish(buf, pos)
+    return pos
+
+
+def _hash_image(typname, value):
+    """The bytes PostgreSQL's type hash function hashes (hashint4 etc. hash
+    the value's own width; text its payload)."""
+    if typname in ("text", "bytea", "bpchar"):
+        return bytes(value)
+    return kdsmod.encode_datum(typname, value)
 
 
 class SaveExpr(Expr):
@@ -350,7 +570,7 @@ class Plan:
 
     def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None,
               pinned_hash=None, pinned_nslots=137, pinned_length=512 << 20, sort_keys=None,
-              sort_limit=0, window_rank=None):
+              sort_limit=0, window_rank=None, encoding="UTF8"):
         """The session blob.  xids: parallelCurrentXids of the transaction
         (SerializedTransactionState at session_xact_state, xpu_common.h:3156-3165),
         which decide the visibility of KDS_FORMAT_ROW sources.  pinned_hash:
@@ -531,6 +751,12 @@ class Plan:
             sess.gpusort_limit_count = int(sort_limit)
             finalized = A_KSORT_PAVG_KINDS
             sess.gpusort_htup_margin = 9 * sum(1 for k in sort_keys if k[0] in finalized)
+        if encoding:
+            # xpu_encode_info (__build_session_encode, executor.c:494-506):
+            # the database encoding's name and longest character
+            maxlen = {"UTF8": 4, "SQL_ASCII": 1}.get(encoding, 1)
+            sess.session_encode = _append_aligned(
+                buf, struct.pack("<16si4xQ", encoding.encode(), maxlen, 0))
         if xids is not None:
             xs = struct.pack("<i?3xQQIi", 1, False, 0, 0, 0, len(xids)) + \
                 struct.pack("<%dI" % len(xids), *xids)

@@ -112,7 +112,7 @@ def take_rows(cols, keep):
     out = []
     for c in cols:
         nulls = None if c.nulls is None else c.nulls[keep]
-        if c.typname in ("text", "bytea"):
+        if c.typname in ("text", "bytea", "bpchar"):
             vals = [bytes(c.values[i]) for i in idx]
         else:
             vals = c.values[keep]

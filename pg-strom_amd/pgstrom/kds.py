@@ -28,6 +28,7 @@ PG_TYPES = {
     "numeric": (1700, -1, False, 4, 9),
     "bytea": (17, -1, False, 4, 10),
     "text": (25, -1, False, 4, 11),
+    "bpchar": (1042, -1, False, 4, 12),
     "date": (1082, 4, True, 4, 13),
 }
 
@@ -66,7 +67,7 @@ class Column:
     def __init__(self, name, typname, values, nulls=None):
         self.name = name
         self.typname = typname
-        if typname in ("text", "bytea"):
+        if typname in ("text", "bytea", "bpchar"):
             if isinstance(values, ArrowVarlena):
                 self.values = values
             else:
@@ -160,7 +161,7 @@ def build_arrow_kds(columns, text_layout="utf8"):
             valid = np.packbits(~c.nulls, bitorder="little")
             ent["nullmap"] = (pos, valid.tobytes())
             pos = abi.typealign(64, pos + len(valid))
-        if c.typname in ("text", "bytea"):
+        if c.typname in ("text", "bytea", "bpchar"):
             if isinstance(c.values, ArrowVarlena):
                 offs = c.values.offsets
                 data = c.values.data
@@ -225,15 +226,15 @@ def build_arrow_kds(columns, text_layout="utf8"):
             cm.attopts.unitsz = PG_TYPES[t][1]
             cm.attopts.u.floating_point.precision = (
                 abi.ArrowPrecision__Single if t == "float4" else abi.ArrowPrecision__Double)
-        elif t in ("text", "bytea") and "fixed_width" in ent:
+        elif t in ("text", "bytea", "bpchar") and "fixed_width" in ent:
             cm.attopts.tag = ARROW_FIXED_SIZE_BINARY
             cm.attopts.unitsz = ent["fixed_width"]
             cm.attopts.u.fixed_size_binary.byteWidth = ent["fixed_width"]
-        elif t in ("text", "bytea") and text_layout == "large":
-            cm.attopts.tag = ARROW_LARGE_UTF8 if t == "text" else ARROW_LARGE_BINARY
+        elif t in ("text", "bytea", "bpchar") and text_layout == "large":
+            cm.attopts.tag = ARROW_LARGE_UTF8 if t != "bytea" else ARROW_LARGE_BINARY
             cm.attopts.unitsz = 8
-        elif t in ("text", "bytea"):
-            cm.attopts.tag = abi.ArrowType__Utf8 if t == "text" else 4
+        elif t in ("text", "bytea", "bpchar"):
+            cm.attopts.tag = abi.ArrowType__Utf8 if t != "bytea" else 4
             cm.attopts.unitsz = 4
         elif t == "bool":
             cm.attopts.tag = abi.ArrowType__Bool
@@ -254,7 +255,7 @@ _FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
 
 
 def encode_datum(typname, value):
-    if typname in ("text", "bytea"):
+    if typname in ("text", "bytea", "bpchar"):
         v = bytes(value)
         return struct.pack("<I", (len(v) + 4) << 2) + v
     return struct.pack(_FMT[typname], value)

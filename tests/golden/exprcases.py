@@ -33,12 +33,44 @@ VALUES = {
                                               -9.2233720368547758e18, 1e39, 127.5, 3.4028235e38)],
     "text": [_text(s) for s in ("", "a", "abc", "abd", "ab", "ABC", "abc ", "b", "été",
                                 "zzzzzzzzzzzzzzzzzzzzzzzzzzzzzz", "a%c", "_b")],
+    # bpchar values as a char(n) column stores them: padded with spaces
+    "bpchar": [_text(s) for s in ("", "a   ", "abc", "abc  ", "abd ", "ab", "ABC", " a", "été ",
+                                  "b")],
 }
+
+# the session's database encoding of the golden vectors (text functions
+# count characters by it)
+ENCODING = "UTF8"
+
+# LIKE patterns (the second argument of the LIKE family)
+PATTERNS = [_text(s) for s in ("%", "", "a%", "%c", "a_c", "%b%", "abc", "ABC", "a%c%",
+                               "_", "__", "%_", "\\%", "a\\", "%\\", "a\\_c", "é%", "_t_",
+                               "%%c", "b%", "%d")]
+LIKE_TEXTS = [_text(s) for s in ("", "a", "abc", "abd", "ab", "ABC", "aXc", "a%c", "a_c", "%",
+                                 "été", "xbx", "abcabc", "a\\")]
+
+# substr / substring arguments: (text index, start, count)
+SUBSTR_STARTS = [-3, -1, 0, 1, 2, 3, 5, 100]
+SUBSTR_COUNTS = [0, 1, 2, 3, 100, -1]
+
+
+def func_arg_sets(kind, types):
+    """Explicit argument tuples for function kinds whose inputs are not a
+    plain cross product: (value lists per argument)."""
+    if kind == "like":
+        return [VALUES[types[0]] if types[0] == "bpchar" else LIKE_TEXTS, PATTERNS]
+    if kind == "substr":
+        texts = [VALUES["text"][i] for i in (0, 2, 8, 9)]
+        sets = [texts, [struct.pack("<i", v) for v in SUBSTR_STARTS]]
+        if len(types) == 3:
+            sets.append([struct.pack("<i", v) for v in SUBSTR_COUNTS])
+        return sets
+    return None
 
 # binary functions take the cross product of the first BINARY_N values of
 # each argument type (the full tables for unary functions)
 BINARY_N = {"bool": 2, "int1": 9, "int2": 10, "int4": 11, "int8": 11, "float4": 14,
-            "float8": 16, "text": 12}
+            "float8": 16, "text": 12, "bpchar": 10}
 
 
 _INT_FMT = {"int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q"}
@@ -59,7 +91,15 @@ def host_traps(name, types, imgs):
     return y == -1 and x == -(1 << (bits - 1))
 
 
-def arg_tuples(types):
+def arg_tuples(types, kind=None):
+    """Index tuples into the argument value lists (func_arg_lists)."""
+    sets = func_arg_sets(kind, types)
+    if sets is not None:
+        out = [[]]
+        for vals in sets:
+            out = [o + [i] for o in out for i in range(len(vals))]
+        return [tuple(o) for o in out] + [tuple(None if j == k else 0 for j in range(len(types)))
+                                          for k in range(len(types))]
     if len(types) == 1:
         return [(i,) for i in range(len(VALUES[types[0]]))] + [(None,)]
     a, b = types
@@ -100,6 +140,49 @@ def templates(cg):
     ]
     for kind in cg.BoolTest.KINDS:
         out.append(("booltest_" + kind, cg.BoolTest(kind, B(0)), ["bool"], 0))
+    C = lambda t, v: cg.Const(t, v)
+    out += [
+        ("distinct_int4", cg.DistinctFrom(cg.Func("int4eq", I(0), I(1))), ["int4", "int4"], 0),
+        ("distinct_text", cg.DistinctFrom(cg.Func("text_eq", T(0), T(1))), ["text", "text"], 0),
+        ("least3", cg.MinMax("least", "int4", I(0), I(1), I(2)), ["int4", "int4", "int4"], 0),
+        ("greatest3", cg.MinMax("greatest", "int4", I(0), I(1), I(2)),
+         ["int4", "int4", "int4"], 0),
+        ("least_text", cg.MinMax("least", "text", T(0), T(1)), ["text", "text"], 0),
+        ("greatest_text", cg.MinMax("greatest", "text", T(0), T(1), C("text", b"abc")),
+         ["text", "text"], 0),
+        ("case_searched", cg.CaseWhen("int4", [(cg.Func("int4gt", I(0), C("int4", 0)), I(1)),
+                                               (cg.Func("int4lt", I(0), C("int4", 0)), I(2))],
+                                      C("int4", -99)), ["int4", "int4", "int4"], 0),
+        ("case_no_else", cg.CaseWhen("text", [(B(0), T(1)), (cg.Not(B(0)), C("text", b"no"))]),
+         ["bool", "text"], 0),
+        # later WHEN / THEN arms are evaluated only where no earlier arm matched
+        ("case_guard", cg.CaseWhen("int4", [(cg.Func("int4eq", I(1), C("int4", 0)), C("int4", 0)),
+                                            (B(0), cg.Func("int4div", I(2), I(1)))],
+                                   cg.Func("int4div", I(1), I(2))), ["bool", "int4", "int4"], 0),
+        ("case_key", cg.CaseWhen("int4", [(cg.Func("int4eq", I(3), C("int4", 1)), C("int4", 10)),
+                                          (cg.Func("int4eq", I(3), C("int4", 7)), I(1))],
+                                 I(2), key=(3, cg.Func("int4pl", I(0), C("int4", 0)))),
+         ["int4", "int4", "int4", "int4"], 0),
+        ("any_int4", cg.ScalarArrayOp(True, "int4eq", I(0), 1, "int4", [1, 7, 2**31 - 1]),
+         ["int4", "int4"], 0),
+        ("any_int4_nulls", cg.ScalarArrayOp(True, "int4eq", I(0), 1, "int4", [7, None, 1]),
+         ["int4", "int4"], 0),
+        ("any_lt", cg.ScalarArrayOp(True, "int4lt", I(0), 1, "int4", [-5, 0]), ["int4", "int4"], 0),
+        ("any_empty", cg.ScalarArrayOp(True, "int4eq", I(0), 1, "int4", []), ["int4", "int4"], 0),
+        ("any_text", cg.ScalarArrayOp(True, "text_eq", T(0), 1, "text", [b"abd", b"", b"zz"]),
+         ["text", "text"], 0),
+        ("in_hashed", cg.HashedArrayOp("bool", "int4", I(0), [(v, True) for v in (1, 7, 9, 11, 0)],
+                                       in_list=True), ["int4"], 0),
+        ("in_hashed_text", cg.HashedArrayOp("bool", "text", T(0),
+                                            [(v, True) for v in (b"abc", b"x", b"y", b"", b"q")],
+                                            in_list=True), ["text"], 0),
+        ("case_hashed", cg.HashedArrayOp("int4", "int4", I(0),
+                                         [(1, 100), (7, 700), (2**31 - 1, -1), (-2, 20)], -5),
+         ["int4"], 0),
+        ("case_hashed_text", cg.HashedArrayOp("text", "int4", I(0),
+                                              [(0, b"zero"), (1, b"one"), (7, b"seven"),
+                                               (-2, b"")], b"other"), ["int4"], 0),
+    ]
     return out
 
 
@@ -108,3 +191,21 @@ TEMPLATE_VALUES = {
     "int4": [struct.pack("<i", v) for v in (0, 1, -2, 7, 2**31 - 1)] + [None],
     "text": [_text(s) for s in ("", "abc", "abd")] + [None],
 }
+
+
+def func_arg_lists(kind, types):
+    """The value list each argument's index refers to."""
+    sets = func_arg_sets(kind, types)
+    if sets is not None:
+        return sets
+    return [VALUES[t] for t in types]
+
+
+def template_traps(name, imgs):
+    """Inputs of a template the reference's host build cannot evaluate:
+    HashedArrayOp with a non-bool result dereferences a NULL address where
+    the result is NULL (no matching key and no ELSE, or a NULL THEN value;
+    pgfn_HashedArrayOp, xpu_common.cu:1084, reads it with a heap reader that
+    does not check for NULL, xpu_basetype.cu:150-159).  This build returns
+    NULL there, as PostgreSQL does."""
+    return False

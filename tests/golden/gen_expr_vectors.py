@@ -36,8 +36,8 @@ class RefLib:
         L = C.CDLL(os.path.join(ROOT, "oracle", "_ref", "libpgstrom_ref.so"))
         vp = C.c_void_p
         L.ref_eval_expr.restype = C.c_int
-        L.ref_eval_expr.argtypes = [vp, C.c_size_t, C.c_int, C.c_int, vp, vp, vp, vp, vp,
-                                    C.c_int32, vp, vp, C.c_char_p, C.c_int]
+        L.ref_eval_expr.argtypes = [vp, C.c_size_t, C.c_int, C.c_char_p, C.c_int, vp, vp, vp, vp,
+                                    vp, C.c_int32, vp, vp, C.c_char_p, C.c_int]
         L.ref_func_catalog_json.restype = C.c_char_p
         self.L = L
 
@@ -59,7 +59,8 @@ class RefLib:
         rt, rl, ec = C.c_int32(), C.c_int32(), C.c_uint32()
         img = C.create_string_buffer(8192)
         msg = C.create_string_buffer(256)
-        rc = self.L.ref_eval_expr(blob, len(blob), compare_nulls, n, types, dbuf, offs,
+        rc = self.L.ref_eval_expr(blob, len(blob), compare_nulls, X.ENCODING.encode(), n, types,
+                                  dbuf, offs,
                                   C.byref(rt), img, 8192, C.byref(rl), C.byref(ec), msg, 256)
         if rc < 0:
             raise RuntimeError("reference cannot evaluate this expression")
@@ -89,8 +90,9 @@ def main():
         expr = cg.Func(name, *[cg.Var(i, t) for i, t in enumerate(args)])
         blob = blob_of(expr)
         cases, rtype = [], None
-        for tup in X.arg_tuples(list(args)):
-            imgs = [None if i is None else X.VALUES[t][i] for i, t in zip(tup, args)]
+        lists = X.func_arg_lists(kind, list(args))
+        for tup in X.arg_tuples(list(args), kind):
+            imgs = [None if i is None else lists[j][i] for j, i in enumerate(tup)]
             if X.host_traps(name, args, imgs):
                 continue
             out, t = ref.eval(blob, 0, list(args), imgs)
@@ -106,6 +108,8 @@ def main():
             combos = [c + [i] for c in combos for i in range(len(X.TEMPLATE_VALUES[t]))]
         for combo in combos:
             imgs = [X.TEMPLATE_VALUES[t][i] for t, i in zip(slots, combo)]
+            if X.template_traps(name, imgs):
+                continue
             out, t = ref.eval(blob, cmpn, slots, imgs)
             rtype = rtype or t
             cases.append(combo + [out])

@@ -57,11 +57,12 @@ def test_oracle_function_matches_reference(name):
     assert opcodes.FUNCS[name][0] == g["opcode"]
     blob = _blob(cg.Func(name, *[cg.Var(i, t) for i, t in enumerate(args)]))
     codes = [opcodes.TYPES[t] for t in args]
+    lists = X.func_arg_lists(opcodes.FUNCS[name][1], list(args))
     bad = []
     for case in g["cases"]:
         idx, exp = case[:-1], _norm(case[-1])
-        imgs = [None if i is None else X.VALUES[t][i] for i, t in zip(idx, args)]
-        got = oracle.eval_expr(blob, codes, imgs, opcodes.TYPES[rtype])
+        imgs = [None if i is None else lists[j][i] for j, i in enumerate(idx)]
+        got = oracle.eval_expr(blob, codes, imgs, opcodes.TYPES[rtype], encoding=X.ENCODING)
         if _canon(rtype, got) != _canon(rtype, exp):
             bad.append((idx, got, exp))
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])
@@ -78,7 +79,8 @@ def test_oracle_construct_matches_reference(name):
     for case in g["cases"]:
         idx, exp = case[:-1], _norm(case[-1])
         imgs = [X.TEMPLATE_VALUES[t][i] for t, i in zip(slots, idx)]
-        got = oracle.eval_expr(blob, codes, imgs, opcodes.TYPES[rtype], g["compare_nulls"])
+        got = oracle.eval_expr(blob, codes, imgs, opcodes.TYPES[rtype], g["compare_nulls"],
+                               encoding=X.ENCODING)
         if _canon(rtype, got) != _canon(rtype, exp):
             bad.append((idx, got, exp))
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])
@@ -110,4 +112,16 @@ def test_library_compiles_every_golden_function(name):
     e = cg.Func(name, *vs)
     plan.set_projection([e], [("r", e.typname)])
     rc, err = xpuhip.session_check(plan.build())
+    assert rc == 0, err.describe()
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN["templates"]))
+def test_library_compiles_every_golden_construct(name):
+    """The session compiler accepts a projection of each construct (CASE,
+    DISTINCT FROM, LEAST/GREATEST, ANY/ALL, hashed IN lists ...)."""
+    from pgstrom import xpuhip
+    from test_gpu_expr_golden import _plan_blob
+    g = GOLDEN["templates"][name]
+    expr = {n: e for n, e, _, _ in X.templates(cg)}[name]
+    rc, err = xpuhip.session_check(_plan_blob(expr, g["slots"]))
     assert rc == 0, err.describe()

@@ -19,7 +19,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 
-from pgstrom import abi, codegen as cg, kds, xpuhip  # noqa: E402
+from pgstrom import abi, codegen as cg, kds, opcodes, xpuhip  # noqa: E402
 from pgstrom.kds import Column  # noqa: E402
 import exprcases as X  # noqa: E402
 
@@ -33,7 +33,7 @@ FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
 
 
 def _value(t, img):
-    if t in ("text", "bytea"):
+    if t in ("text", "bytea", "bpchar"):
         if img[0] & 1:
             return bytes(img[1:])
         return bytes(img[4:(struct.unpack_from("<I", img)[0] >> 2)])
@@ -42,30 +42,84 @@ def _value(t, img):
 
 def _column(name, t, imgs):
     nulls = np.array([i is None for i in imgs], dtype=bool)
-    if t in ("text", "bytea"):
+    if t in ("text", "bytea", "bpchar"):
         vals = [b"" if i is None else _value(t, i) for i in imgs]
     else:
         vals = [0 if i is None else _value(t, i) for i in imgs]
     return Column(name, t, vals, nulls)
 
 
-def _session(expr, slot_types):
+def _plan_blob(expr, slot_types):
+    """A scan + projection of expr over columns c0.. of slot_types; slots
+    beyond the columns the expression reads (CASE keys, array elements)
+    become temporaries."""
     plan = cg.Plan([("c%d" % i, t) for i, t in enumerate(slot_types)])
-    vars_ = [plan.scan_var("c%d" % i) for i in range(len(slot_types))]
+    used = _slots_read(expr)
+    vars_ = {}
+    for i, t in enumerate(slot_types):
+        if i in used:
+            vars_[i] = plan.scan_var("c%d" % i)
+    temps = _slots_written(expr)
+    for i in sorted(temps):
+        while len(plan.slots) <= i:
+            plan.temp_slot(temps[i], 0)
     e = _rebind(expr, vars_)
     plan.set_projection([e], [("r", e.typname)])
-    return xpuhip.Session(plan.build())
+    return plan.build()
+
+
+def _session(expr, slot_types):
+    return xpuhip.Session(_plan_blob(expr, slot_types))
+
+
+def _walk(e, fn):
+    fn(e)
+    for attr in ("args", "whens"):
+        for a in getattr(e, attr, []) or []:
+            for x in (a if isinstance(a, tuple) else (a,)):
+                _walk(x, fn)
+    for attr in ("arg", "cmp", "scalar", "default"):
+        x = getattr(e, attr, None)
+        if isinstance(x, cg.Expr):
+            _walk(x, fn)
+    if getattr(e, "key", None):
+        _walk(e.key[1], fn)
+
+
+def _slots_written(expr):
+    """Slots an expression defines itself: CASE keys and array elements."""
+    out = {}
+
+    def fn(e):
+        if isinstance(e, cg.CaseWhen) and e.key:
+            out[e.key[0]] = e.key[1].typname
+        if isinstance(e, cg.ScalarArrayOp):
+            out[e.elem_slot] = e.elem_typname
+    _walk(expr, fn)
+    return out
+
+
+def _slots_read(expr):
+    out = set()
+    _walk(expr, lambda e: out.add(e.slot) if isinstance(e, cg.Var) else None)
+    return out - set(_slots_written(expr))
 
 
 def _rebind(e, vars_):
-    """The expression with its slot Vars replaced by the plan's column Vars."""
+    """The expression with its column-slot Vars replaced by the plan's
+    column Vars (temporary slots keep their numbers)."""
     if isinstance(e, cg.Var):
-        return vars_[e.slot]
-    for attr in ("args",):
-        if hasattr(e, attr):
-            setattr(e, attr, [_rebind(a, vars_) for a in getattr(e, attr)])
-    if hasattr(e, "arg"):
-        e.arg = _rebind(e.arg, vars_)
+        return vars_.get(e.slot, e)
+    if hasattr(e, "args"):
+        e.args = [_rebind(a, vars_) for a in e.args]
+    if hasattr(e, "whens"):
+        e.whens = [(_rebind(c, vars_), _rebind(r, vars_)) for c, r in e.whens]
+    for attr in ("arg", "cmp", "scalar", "default"):
+        x = getattr(e, attr, None)
+        if isinstance(x, cg.Expr):
+            setattr(e, attr, _rebind(x, vars_))
+    if getattr(e, "key", None):
+        e.key = (e.key[0], _rebind(e.key[1], vars_))
     return e
 
 
@@ -79,7 +133,7 @@ def _run(sess, cols, n):
 def _image(t, v):
     if v is None:
         return None
-    if t in ("text", "bytea"):
+    if t in ("text", "bytea", "bpchar"):
         return v
     return struct.pack(FMT[t], v)
 
@@ -88,16 +142,16 @@ def _expected(t, out):
     if out is None:
         return None
     img = bytes.fromhex(out)
-    return _value(t, img) if t in ("text", "bytea") else img
+    return _value(t, img) if t in ("text", "bytea", "bpchar") else img
 
 
 def _replay(expr_fn, slot_types, rtype, cases, pick):
-    """cases: [idx..., out]; pick(type, index) -> datum image or None."""
+    """cases: [idx..., out]; pick(argument number, index) -> datum image or None."""
     ok = [c for c in cases if not isinstance(c[-1], list)]
     err = [c for c in cases if isinstance(c[-1], list)]
     bad = []
     if ok:
-        cols = [_column("c%d" % i, t, [pick(t, c[i]) for c in ok])
+        cols = [_column("c%d" % i, t, [pick(i, c[i]) for c in ok])
                 for i, t in enumerate(slot_types)]
         s = _session(expr_fn(), slot_types)
         got = _run(s, cols, len(ok))
@@ -108,7 +162,7 @@ def _replay(expr_fn, slot_types, rtype, cases, pick):
                 bad.append((c[:-1], g, c[-1]))
     s = _session(expr_fn(), slot_types) if err else None
     for c in err:
-        cols = [_column("c%d" % i, t, [pick(t, c[i])]) for i, t in enumerate(slot_types)]
+        cols = [_column("c%d" % i, t, [pick(i, c[i])]) for i, t in enumerate(slot_types)]
         src = kds.build_arrow_kds(cols)
         with pytest.raises(xpuhip.XpuError) as e:
             s.exec(src, dst_length=1 << 16)
@@ -125,8 +179,9 @@ def test_gpu_function_matches_reference(gpu, name):
     g = GOLDEN["funcs"][name]
     args = g["args"]
     expr_fn = lambda: cg.Func(name, *[cg.Var(i, t) for i, t in enumerate(args)])
+    lists = X.func_arg_lists(opcodes.FUNCS[name][1], list(args))
     bad = _replay(expr_fn, args, g["result"], g["cases"],
-                  lambda t, i: None if i is None else X.VALUES[t][i])
+                  lambda j, i: None if i is None else lists[j][i])
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])
 
 
@@ -137,6 +192,7 @@ def test_gpu_construct_matches_reference(gpu, name):
         pytest.skip("kmode_compare_nulls applies to group-key comparison only, "
                     "covered by the group-by tests")
     expr_fn = lambda: {n: e for n, e, _, _ in X.templates(cg)}[name]
-    bad = _replay(expr_fn, g["slots"], g["result"], g["cases"],
-                  lambda t, i: X.TEMPLATE_VALUES[t][i])
+    slots = g["slots"]
+    bad = _replay(expr_fn, slots, g["result"], g["cases"],
+                  lambda j, i: X.TEMPLATE_VALUES[slots[j]][i])
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])
