@@ -304,6 +304,8 @@ typedef struct
                                        key, radix and permute passes */
     uint32_t    sort_items;         /* items it sorted (0: no sort) */
     uint32_t    sort_passes;        /* radix passes run (constant digits skipped) */
+    uint32_t    join_slices;        /* slices of rows a multi-pass join ran in (0: none) */
+    uint32_t    __pad2;
 } xpuhip_task_stats;
 
 int     xpuhip_last_task_stats(xpuhip_task_stats *stats);

@@ -949,6 +949,22 @@ xpuhip_session_attach_inner(xpuhip_session *s, const kern_multirels *kmrels, siz
 /* device memory for the records of multi-pass joins, per task */
 #define XPU_JOIN_REC_BUDGET     (4ULL << 30)
 
+/* the record budget: XPU_JOIN_REC_BUDGET, or the environment's
+ * XPUHIP_JOIN_REC_BUDGET (bytes, at least 1 MiB) - smaller budgets split a
+ * chunk into more slices without changing results */
+static uint64_t
+join_rec_budget(void)
+{
+    const char *e = getenv("XPUHIP_JOIN_REC_BUDGET");
+    if (e && *e)
+    {
+        unsigned long long v = strtoull(e, nullptr, 10);
+        if (v >= (1ULL << 20))
+            return v;
+    }
+    return XPU_JOIN_REC_BUDGET;
+}
+
 namespace {
 
 /* Sizes of multi-pass join records: the rows per slice (*slice), the
@@ -973,7 +989,7 @@ join_pass_sizes(const xplan *P, const uint32_t *maxdup, uint64_t nrows, uint32_t
         if (mult > (1ULL << 20))
             return false;
     }
-    uint64_t n = XPU_JOIN_REC_BUDGET / (4 * (words[0] + words[1]));
+    uint64_t n = join_rec_budget() / (4 * (words[0] + words[1]));
     n = n / round * round;
     if (n == 0)
         return false;
@@ -1255,6 +1271,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     }
 
     int rc = 0;
+    uint32_t nslices = 0;
     xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
     if (kp.dump)
         xpu_launch_fill(kp.dump_next, 0, kp.dump_nparts, x->stream);
@@ -1303,6 +1320,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
          * the pass of each depth from level d (buffer (d-1)&1) to level d+1 */
         for (uint64_t sb = row_begin; rc == 0 && sb < nitems; sb += slice)
         {
+            nslices++;
             xpu_kparams q = kp;
             void *args[] = {&q};
             q.row_begin = (uint32_t)sb;
@@ -1402,6 +1420,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_stats.global_merges = c->global_merges;
     t_last_stats.spill_rows = c->spill_rows;
     t_last_stats.jit = jf ? 1 : 0;
+    t_last_stats.join_slices = nslices;
     t_last_stats.heap_ms = 0.0;
     t_last_stats.heap_rows = 0;
     if (res)
@@ -2771,6 +2790,7 @@ xpuhip_batch_exec(const xpuhip_batch_task *tasks, uint32_t ntasks, uint32_t flag
     t_last_stats.lds_bytes = lds;
     t_last_stats.rows = total;
     t_last_stats.jit = jf ? 1 : 0;
+    t_last_stats.join_slices = 0;
     t_last_stats.heap_ms = 0.0;
     t_last_stats.heap_rows = 0;
 

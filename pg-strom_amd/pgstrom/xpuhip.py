@@ -74,7 +74,8 @@ class xpuhip_task_stats(C.Structure):
                 ("rows", C.c_uint64), ("lds_miss", C.c_uint64), ("global_merges", C.c_uint64),
                 ("spill_rows", C.c_uint64), ("jit", C.c_uint32), ("_pad", C.c_uint32),
                 ("heap_ms", C.c_double), ("heap_rows", C.c_uint64),
-                ("sort_ms", C.c_double), ("sort_items", C.c_uint32), ("sort_passes", C.c_uint32)]
+                ("sort_ms", C.c_double), ("sort_items", C.c_uint32), ("sort_passes", C.c_uint32),
+                ("join_slices", C.c_uint32), ("_pad2", C.c_uint32)]
 
 
 _lib = None
