@@ -167,6 +167,22 @@ int     xpuhip_task_exec_device_tuples(xpuhip_session *sess,
 int64_t xpuhip_task_final(xpuhip_session *sess, kern_data_store *kds_final, size_t length,
                           kern_exec_results *results, kern_errorbuf *errbuf);
 
+/* The CPU-fallback rows of a session whose blob carries fallback_kds_head
+ * (pg_strom.cpu_fallback on): the rows its tasks sent to the CPU since the
+ * last call, as the KDS_FORMAT_FALLBACK chunk that gpuClientWriteBackPartial
+ * returns to the backend (gpu_service.c:2853-2889; the backend re-runs them,
+ * execCpuFallbackOneChunk, fallback.c:497-542).  A row falls back when its
+ * depth-0 work (scan quals, the depth-1 join key hash, and without joins its
+ * projection or group-by arguments) needs the bytes of a compressed or
+ * external datum (HandleErrorIfCpuFallback, xpu_common.cu:1898-1955); it is
+ * written with depth 0 and its depth-0 variables, and produces nothing on
+ * the GPU.  The device buffer is emptied.  Returns the bytes written, 0 when
+ * there are no rows, the bytes needed with kds_fallback == NULL, or -1 with
+ * errbuf filled.  A fallback that does not fit the buffer (its length is the
+ * session header's) fails the task with ERRCODE_DEVICE_ERROR. */
+int64_t xpuhip_session_fallback(xpuhip_session *sess, kern_data_store *kds_fallback,
+                                size_t length, kern_errorbuf *errbuf);
+
 /* Releases the session and its device memory (gpuservCloseSession). */
 void    xpuhip_session_close(xpuhip_session *sess);
 
@@ -305,7 +321,7 @@ typedef struct
     uint32_t    sort_items;         /* items it sorted (0: no sort) */
     uint32_t    sort_passes;        /* radix passes run (constant digits skipped) */
     uint32_t    join_slices;        /* slices of rows a multi-pass join ran in (0: none) */
-    uint32_t    __pad2;
+    uint32_t    final_grows;        /* times the group-by kds_final grew during the task */
 } xpuhip_task_stats;
 
 int     xpuhip_last_task_stats(xpuhip_task_stats *stats);

@@ -108,6 +108,7 @@ enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
 
 typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, t2, res; } ofunc;
 
+#define OMAX_FB_ATTRS   256
 #define T_Invalid   TypeOpCode__Invalid
 #define T_bool      TypeOpCode__bool
 #define T_int1      TypeOpCode__int1
@@ -176,6 +177,8 @@ typedef struct
     int         enc;        /* database encoding: ENC_NONE / ENC_SINGLE / ENC_UTF8 / ENC_OTHER */
     uint32_t    errcode;
     char        errmsg[KERN_ERRORBUF_MESSAGE_LEN + 1];
+    bool        fb_stage;   /* the row's depth-0 work runs (a fallback error
+                               then sends it to the CPU fallback) */
 } octx;
 
 enum { ENC_NONE = 0, ENC_SINGLE, ENC_UTF8, ENC_OTHER };
@@ -378,8 +381,13 @@ datum_from_image(octx *c, int type, const char *addr, od *r)
         {
             uint8_t b0 = (uint8_t)addr[0];
             if (b0 == 0x01 || (b0 & 0x03) == 0x02)
-                return oerror(c, ERRCODE_SUSPEND_FALLBACK,
-                              "%s: text datum is compressed or external", "oracle");
+            {
+                /* compressed or external: kept as it is, length -1
+                 * (xpu_text_datum_heap_read); its bytes are never read */
+                r->len = -1;
+                r->p = addr;
+                break;
+            }
             if (b0 & 0x01)
             {
                 r->len = ((b0 >> 1) & 0x7f) - 1;
@@ -399,7 +407,10 @@ datum_from_image(octx *c, int type, const char *addr, od *r)
     return true;
 }
 
-/* size and bytes of a datum written into a tuple (xpu_*_datum_write) */
+static uint32_t varsize_any(const uint8_t *p);
+
+/* size and bytes of a datum written into a tuple (xpu_*_datum_write); a
+ * compressed or external varlena is copied as it is (xpu_textlib.cu:319-325) */
 static int
 datum_write(int type, const od *v, char *buf)
 {
@@ -414,6 +425,13 @@ datum_write(int type, const od *v, char *buf)
         case T_float4: if (buf) memcpy(buf, &v->f, 4); return 4;
         case T_float8: if (buf) memcpy(buf, &v->d, 8); return 8;
         case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
+            if (v->len < 0)
+            {
+                uint32_t sz = varsize_any((const uint8_t *)v->p);
+                if (buf)
+                    memcpy(buf, v->p, sz);
+                return (int)sz;
+            }
             if (buf)
             {
                 uint32_t h = (uint32_t)(v->len + 4) << 2;
@@ -484,6 +502,23 @@ cmp_result(int op, int comp)
     return false;
 }
 
+static bool is_text_type(int t)
+{
+    return t == T_text || t == TypeOpCode__bytea || t == T_bpchar;
+}
+
+/* A compressed or external text/bpchar/bytea needs its bytes: the CPU
+ * fallback takes it (xpu_text_is_valid, xpu_textlib.h:40-70).  True, with
+ * the error set, for such a datum. */
+static bool
+raw_text(octx *c, int type, const od *v)
+{
+    if (!is_text_type(type) || v->isnull || v->len >= 0)
+        return false;
+    oerror(c, ERRCODE_SUSPEND_FALLBACK, "%s", "text datum is compressed or external");
+    return true;
+}
+
 static bool
 eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
 {
@@ -509,6 +544,26 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
         }
         r->isnull = true;
         return true;
+    }
+    if (f->kind == K_textlen && f->t0 == T_text && a[0].len < 0 && c->enc == ENC_SINGLE)
+    {
+        /* pgfn_textlen under a single-byte encoding counts bytes, which
+         * the header of a compressed or external datum tells
+         * (xpu_textlib.cu:676-703): the external pointer's va_extinfo, or
+         * for a compressed datum the word that the reference's memcpy from
+         * VARDATA puts into toast_compress_header's tcinfo (the one after
+         * the compression info) */
+        const uint8_t *h = (const uint8_t *)a[0].p;
+        uint32_t w;
+        memcpy(&w, h + (h[0] == 0x01 ? 6 : 8), 4);
+        r->i = (int64_t)(w & 0x3fffffffU);
+        return true;
+    }
+    {
+        const int ts[3] = { f->t0, f->t1, f->t2 };
+        for (int i = 0; i < f->nargs && i < 3; i++)
+            if (raw_text(c, ts[i], &a[i]))
+                return false;
     }
     switch (f->kind)
     {
@@ -995,6 +1050,8 @@ eval_haop(octx *c, const kern_expression *k, od *r)
         }
     }
     /* the chain of the scalar's hash, in chain order */
+    if (raw_text(c, etype, &sv))
+        return false;
     uint32_t h = datum_hash(etype, &sv);
     uint32_t next = k->u.haop.hslots[h % nslots];
     while (next != 0)
@@ -1207,6 +1264,8 @@ eval(octx *c, const kern_expression *k, od *r)
                     *r = tmp;
                     continue;
                 }
+                if (raw_text(c, k->exptype, r) || raw_text(c, k->exptype, &tmp))
+                    return false;
                 int comp = datum_comp(k->exptype, r, &tmp);
                 if ((least && comp > 0) || (!least && comp < 0))
                     *r = tmp;
@@ -1247,6 +1306,8 @@ eval(octx *c, const kern_expression *k, od *r)
             for (uint32_t i = 0; i < k->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
             {
                 if (!eval(c, karg, &tmp))
+                    return false;
+                if (raw_text(c, karg->exptype, &tmp))
                     return false;
                 if (!tmp.isnull)
                     h = pg_hash_merge(h, datum_hash(karg->exptype, &tmp));
@@ -1723,6 +1784,14 @@ struct oracle_session
     const kern_expression *gb_actions;
     int         key_types[OMAX_ATTRS];
     ogtable     gt;
+    /* CPU fallback (executor.c:705-724): the scan slot's header and the
+     * slot of each of its attributes at depth 0 (-1: NULL) */
+    const kern_data_store *kds_fb_head;
+    int         fb_slot[OMAX_FB_ATTRS];
+    /* fallback items of the tasks so far: kern_fallbackitem images */
+    char       *fb_buf;
+    size_t      fb_len, fb_cap;
+    uint32_t    fb_nrows;
 };
 
 static const kern_expression *
@@ -1791,6 +1860,24 @@ oracle_session_open(const kern_session_info *session, size_t length, kern_errorb
     os->gb_keyload = session_kexp(s, s->xpucode_groupby_keyload);
     os->gb_keycomp = session_kexp(s, s->xpucode_groupby_keycomp);
     os->gb_actions = session_kexp(s, s->xpucode_groupby_actions);
+    if (s->fallback_kds_head)
+    {
+        const kern_data_store *h = (const kern_data_store *)((const char *)s + s->fallback_kds_head);
+        const kern_fallback_desc *fd = (const kern_fallback_desc *)((const char *)s + s->fallback_desc_defs);
+        if (h->ncols > OMAX_FB_ATTRS || h->format != KDS_FORMAT_FALLBACK)
+        {
+            set_error(eb, ERRCODE_DEVICE_ERROR, __func__, "unsupported fallback buffer header");
+            free(s); free(os);
+            return NULL;
+        }
+        os->kds_fb_head = h;
+        for (uint32_t j = 0; j < h->ncols; j++)
+            os->fb_slot[j] = -1;
+        for (uint32_t i = 0; i < s->fallback_desc_nitems; i++)
+            if (fd[i].fb_src_depth == 0 && fd[i].fb_dst_resno >= 1 &&
+                (uint32_t)fd[i].fb_dst_resno <= h->ncols && fd[i].fb_slot_id < os->nslots)
+                os->fb_slot[fd[i].fb_dst_resno - 1] = fd[i].fb_slot_id;
+    }
     if (os->gb_actions)
     {
         if (os->gb_actions->u.pagg.nattrs > OMAX_ATTRS)
@@ -1808,6 +1895,55 @@ oracle_session_open(const kern_session_info *session, size_t length, kern_errorb
     return os;
 }
 
+/* the CPU-fallback rows of the tasks so far as a KDS_FORMAT_FALLBACK chunk
+ * (header, row index, items at the tail in input order, each tuple's row id
+ * its index), as gpuClientWriteBackPartial sends them
+ * (gpu_service.c:2853-2889); the store is emptied.  Returns the bytes
+ * written, 0 with no rows, the bytes needed with out == NULL, or -1. */
+int64_t
+oracle_session_fallback(oracle_session *os, kern_data_store *out, size_t length)
+{
+    if (!os->kds_fb_head || os->fb_nrows == 0)
+        return 0;
+    const kern_data_store *h = os->kds_fb_head;
+    size_t head_len = KDS_HEAD_LENGTH(h);
+    uint64_t usage = 0;
+    for (size_t pos = 0; pos < os->fb_len; )
+    {
+        uint32_t ilen;
+        memcpy(&ilen, os->fb_buf + pos, 4);
+        usage += ilen;
+        pos += 4 + ilen;
+    }
+    uint64_t need = head_len + 8ULL * os->fb_nrows + usage;
+    if (!out)
+        return (int64_t)need;
+    if (length < need)
+        return -1;
+    memcpy(out, h, head_len);
+    out->length = need;
+    out->nitems = os->fb_nrows;
+    out->usage = usage;
+    uint64_t *rowidx = (uint64_t *)((char *)out + head_len);
+    uint64_t u = 0;
+    uint32_t i = 0;
+    for (size_t pos = 0; pos < os->fb_len; i++)
+    {
+        uint32_t ilen;
+        memcpy(&ilen, os->fb_buf + pos, 4);
+        u += ilen;
+        char *item = (char *)out + need - u;
+        memcpy(item, os->fb_buf + pos + 4, ilen);
+        kern_tupitem *t = (kern_tupitem *)(item + 16);
+        memcpy((char *)t + t->t_len - ROWID_SIZE, &i, ROWID_SIZE);
+        rowidx[i] = u - 16;
+        pos += 4 + ilen;
+    }
+    os->fb_len = 0;
+    os->fb_nrows = 0;
+    return (int64_t)need;
+}
+
 void
 oracle_session_close(oracle_session *os)
 {
@@ -1820,6 +1956,7 @@ oracle_session_close(oracle_session *os)
     }
     free(os->gt.groups);
     free(os->gt.slots);
+    free(os->fb_buf);
     free(os->session);
     free(os);
 }
@@ -1945,7 +2082,77 @@ form_tuple(const oracle_session *os, octx *c, char *buf, uint32_t rowid)
         if (sz < 0)
             return -1;
         if (cm->attlen == -1)
+        {
+            if (buf && (uint8_t)buf[nxt] == 0x01)
+                infomask |= HEAP_HASEXTERNAL;       /* VARATT_IS_EXTERNAL */
             infomask |= HEAP_HASVARWIDTH;
+        }
+        if (buf && hasnull)
+            t->t_bits[j >> 3] |= (1 << (j & 7));
+        t_hoff = nxt + sz;
+    }
+    t_hoff += ROWID_SIZE;
+    if (buf)
+    {
+        t->t_len = t_hoff;
+        t->t_infomask = infomask;
+        memcpy(buf + t_hoff - ROWID_SIZE, &rowid, ROWID_SIZE);
+    }
+    return t_hoff;
+}
+
+/* the CPU-fallback tuple of a row: the scan slot's attributes, those from
+ * depth-0 variables with their values, the rest NULL
+ * (HandleErrorIfCpuFallback at depth 0, xpu_common.cu:1898-1955, forming
+ * it with __kern_form_minimal_tuple, :1096-1190) */
+static int
+form_fallback_tuple(const oracle_session *os, octx *c, char *buf, uint32_t rowid)
+{
+    const kern_data_store *h = os->kds_fb_head;
+    int nattrs = (int)h->ncols;
+    bool hasnull = false;
+    uint32_t t_hoff;
+    uint16_t infomask = 0;
+    kern_tupitem *t = (kern_tupitem *)buf;
+
+    for (int j = 0; j < nattrs; j++)
+    {
+        int s = os->fb_slot[j];
+        if (s < 0 || s >= c->nslots || c->slots[s].isnull)
+        {
+            hasnull = true;
+            infomask |= HEAP_HASNULL;
+            break;
+        }
+    }
+    t_hoff = KERN_TUPITEM_T_BITS_OFFSET;
+    if (hasnull)
+        t_hoff += XPU_BITMAPLEN(nattrs);
+    t_hoff = XPU_MAXALIGN(t_hoff);
+    if (buf)
+    {
+        memset(buf, 0, t_hoff);
+        t->t_infomask2 = nattrs & HEAP_NATTS_MASK;
+        t->t_hoff = t_hoff + MINIMAL_TUPLE_OFFSET;
+    }
+    for (int j = 0; j < nattrs; j++)
+    {
+        const kern_colmeta *cm = &h->colmeta[j];
+        int s = os->fb_slot[j];
+        if (s < 0 || s >= c->nslots || c->slots[s].isnull)
+            continue;
+        uint32_t nxt = XPU_TYPEALIGN(cm->attalign, t_hoff);
+        if (buf && nxt > t_hoff)
+            memset(buf + t_hoff, 0, nxt - t_hoff);
+        int sz = datum_write(c->vdesc[s].vs_type_code, &c->slots[s], buf ? buf + nxt : NULL);
+        if (sz < 0)
+            return -1;
+        if (cm->attlen == -1)
+        {
+            if (buf && (uint8_t)buf[nxt] == 0x01)
+                infomask |= HEAP_HASEXTERNAL;
+            infomask |= HEAP_HASVARWIDTH;
+        }
         if (buf && hasnull)
             t->t_bits[j >> 3] |= (1 << (j & 7));
         t_hoff = nxt + sz;
@@ -2327,6 +2534,8 @@ join_depth(orow *w, int depth)
 
     if (!kh || !eval(c, kh, &hv))
         return kh ? false : oerror(c, ERRCODE_DEVICE_ERROR, "%s: no hash value", "join");
+    if (depth == 1)
+        c->fb_stage = false;    /* past the row's depth-0 work */
     uint32_t hash = (uint32_t)hv.i;
     const uint64_t *hslots = (const uint64_t *)((const char *)kds + KDS_HEAD_LENGTH(kds));
     uint64_t off = hslots[hash % kds->hash_nslots];
@@ -2453,6 +2662,7 @@ emit_row(orow *w)
 typedef struct
 {
     obuf        out;
+    obuf        fb;         /* CPU-fallback items ([u32 length][item]) */
     ogtable     gt;
     uint32_t    nitems_in;
     uint32_t    nitems_out;
@@ -2483,19 +2693,46 @@ run_rows(const oracle_session *os, const kern_data_store *kds_src, const otuple 
             memset(&c->slots[s], 0, sizeof(od));
             c->slots[s].isnull = true;
         }
-        if (tups ? !exec_loadvars_heap(c, os->load[0], kds_src, &tups[row])
-                 : !exec_loadvars_arrow(c, os->load[0], kds_src, row))
-            break;
-        if (os->scan_quals)
+        c->fb_stage = true;
+        bool ok = tups ? exec_loadvars_heap(c, os->load[0], kds_src, &tups[row])
+                       : exec_loadvars_arrow(c, os->load[0], kds_src, row);
+        if (ok && os->scan_quals)
         {
-            if (!eval(c, os->scan_quals, &q))
-                break;
-            if (q.isnull || !q.i)
+            ok = eval(c, os->scan_quals, &q);
+            if (ok && (q.isnull || !q.i))
                 continue;
         }
-        th->nitems_in++;
-        if (!join_depth(&w, 1))
+        if (ok)
+        {
+            th->nitems_in++;
+            ok = join_depth(&w, 1);
+        }
+        if (!ok)
+        {
+            /* HandleErrorIfCpuFallback (xpu_common.cu:1898-1955): a
+             * fallback error of the row's depth-0 work sends the row to
+             * the CPU with depth 0; anything else fails the task */
+            if (c->errcode == ERRCODE_SUSPEND_FALLBACK && c->fb_stage && os->kds_fb_head)
+            {
+                int sz = form_fallback_tuple(os, c, NULL, 0);
+                if (sz < 0)
+                {
+                    oerror(c, ERRCODE_DEVICE_ERROR, "%s", "unable to form the fallback tuple");
+                    break;
+                }
+                uint32_t ilen = (uint32_t)XPU_MAXALIGN(16 + sz);
+                char item[ilen];
+                memset(item, 0, ilen);      /* l_state 0, depth 0, not matched */
+                form_fallback_tuple(os, c, item + 16, 0);
+                obuf_append(&th->fb, &ilen, 4);
+                obuf_append(&th->fb, item, ilen);
+                th->fb.nrows++;
+                c->errcode = 0;
+                c->errmsg[0] = '\0';
+                continue;
+            }
             break;
+        }
     }
     th->nitems_out = w.nitems_out;
     memcpy(th->depth_out, w.depth_out, sizeof(w.depth_out));
@@ -2562,6 +2799,21 @@ oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
             results->nitems_out += th[t].nitems_out;
             for (int d = 1; d <= os->nrels && d <= 8; d++)
                 results->stats[d - 1].nitems_out += th[t].depth_out[d];
+        }
+    }
+    /* CPU-fallback items, in input order, to the session's store */
+    if (!rc && os->kds_fb_head)
+    {
+        for (int t = 0; t < nthreads; t++)
+        {
+            if (os->fb_len + th[t].fb.len > os->fb_cap)
+            {
+                os->fb_cap = (os->fb_len + th[t].fb.len) * 2 + 4096;
+                os->fb_buf = realloc(os->fb_buf, os->fb_cap);
+            }
+            memcpy(os->fb_buf + os->fb_len, th[t].fb.buf, th[t].fb.len);
+            os->fb_len += th[t].fb.len;
+            os->fb_nrows += th[t].fb.nrows;
         }
     }
     /* projection: write kds_dst in input order */
@@ -2631,6 +2883,7 @@ oracle_task_exec(oracle_session *os, const kern_data_store *kds_src,
     for (int t = 0; t < nthreads; t++)
     {
         free(th[t].out.buf);
+        free(th[t].fb.buf);
         for (size_t i = 0; i < th[t].gt.ngroups; i++)
         {
             free(th[t].gt.groups[i]->keybuf);

@@ -30,6 +30,8 @@ int     oracle_task_exec(oracle_session *os,
                          kern_errorbuf *errbuf);
 int64_t oracle_task_final(oracle_session *os, kern_data_store *kds_final,
                           size_t final_length, kern_errorbuf *errbuf);
+/* the CPU-fallback rows of the tasks so far (KDS_FORMAT_FALLBACK) */
+int64_t oracle_session_fallback(oracle_session *os, kern_data_store *out, size_t length);
 void    oracle_session_close(oracle_session *os);
 
 uint32_t oracle_hash_any(const void *ptr, int len);

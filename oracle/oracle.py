@@ -23,6 +23,8 @@ def lib():
                                        C.c_size_t, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_task_final.restype = C.c_int64
         L.oracle_task_final.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.oracle_session_fallback.restype = C.c_int64
+        L.oracle_session_fallback.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.oracle_session_close.argtypes = [C.c_void_p]
         L.oracle_hash_any.restype = C.c_uint32
         L.oracle_hash_any.argtypes = [C.c_void_p, C.c_int]
@@ -71,6 +73,18 @@ class OracleSession:
         if got < 0:
             raise RuntimeError(self.err.describe())
         return buf[:got]        # a LIMIT packs the kept tuples into fewer bytes
+
+    def fallback(self):
+        """The CPU-fallback rows of the tasks so far (KDS_FORMAT_FALLBACK
+        bytes), or None."""
+        need = lib().oracle_session_fallback(self.h, None, 0)
+        if need <= 0:
+            return None
+        buf = np.zeros(need, dtype=np.uint8)
+        got = lib().oracle_session_fallback(self.h, buf.ctypes.data, need)
+        if got != need:
+            raise RuntimeError("oracle_session_fallback")
+        return buf
 
     def close(self):
         if self.h:

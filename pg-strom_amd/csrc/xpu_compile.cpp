@@ -1415,6 +1415,61 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
     if (!C.err.empty())
         FAIL(C.err.c_str());
 
+    /* CPU fallback (executor.c:705-724, xpu_common.cu:1898-1955): a row
+     * that falls back at depth 0 becomes a tuple of the scan slot whose
+     * attributes from depth-0 variables hold their values; the rest are
+     * NULL */
+    if (s->fallback_kds_head)
+    {
+        if ((uint64_t)s->fallback_kds_head + sizeof(kern_data_store) > length)
+            FAIL("fallback buffer header out of the session blob");
+        const kern_data_store *h = (const kern_data_store *)((const char *)s + s->fallback_kds_head);
+        const size_t hl = KDS_HEAD_LENGTH(h);
+        if ((uint64_t)s->fallback_kds_head + hl > length || h->nr_colmeta < h->ncols)
+            FAIL("fallback buffer header out of the session blob");
+        if (h->format != KDS_FORMAT_FALLBACK)
+            FAIL("fallback buffer header is not KDS_FORMAT_FALLBACK");
+        if (h->ncols > XP_MAX_PROJ)
+            FAIL("too many attributes in the CPU-fallback tuple");
+        if (h->length < hl + 4096)
+            FAIL("fallback buffer is too small");
+        out->kds_fallback_head.assign((const char *)h, (const char *)h + hl);
+        P->fb_nattrs = h->ncols;
+        for (uint32_t j = 0; j < h->ncols; j++)
+        {
+            xattr *fa = &P->fb[j];
+            fa->reg = 0xff;
+            fa->domain = 0;
+            fa->attalign = h->colmeta[j].attalign;
+            fa->width = h->colmeta[j].attlen > 0 ? (int8_t)h->colmeta[j].attlen : (int8_t)-1;
+        }
+        const uint32_t nfb = s->fallback_desc_nitems;
+        if (nfb > 0 && (!s->fallback_desc_defs ||
+                        (uint64_t)s->fallback_desc_defs + (uint64_t)nfb * sizeof(kern_fallback_desc) > length))
+            FAIL("fallback descriptors out of the session blob");
+        const kern_fallback_desc *fd = (const kern_fallback_desc *)((const char *)s + s->fallback_desc_defs);
+        for (uint32_t i = 0; i < nfb; i++)
+        {
+            if (fd[i].fb_dst_resno < 1 || (uint32_t)fd[i].fb_dst_resno > h->ncols)
+                FAIL("fallback descriptor out of the scan slot");
+            if (fd[i].fb_src_depth != 0)
+                continue;
+            const int slot = fd[i].fb_slot_id;
+            if (slot >= (int)C.nslots)
+                FAIL("fallback descriptor slot out of range");
+            if (C.slot_reg[slot] < 0)
+                continue;           /* not loaded at depth 0: NULL */
+            Reg r = C.reg_of_slot(slot);
+            xattr *fa = &P->fb[fd[i].fb_dst_resno - 1];
+            const int attlen = h->colmeta[fd[i].fb_dst_resno - 1].attlen;
+            if ((r.dom == XD_TEXT) != (attlen == -1) ||
+                (r.dom != XD_TEXT && attlen != type_width(r.type)))
+                FAIL("CPU-fallback attribute does not match its variable's type");
+            fa->reg = (uint8_t)r.r;
+            fa->domain = (uint8_t)r.dom;
+        }
+    }
+
     /* joins */
     for (int d = 1; d <= nrels; d++)
     {
@@ -1501,6 +1556,19 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             const kern_expression *a = C.first_arg(kactions);
             for (uint32_t i = 0; i < kactions->nr_args; i++, a = C.next_arg(a))
                 C.expr(kactions, a, XP_REG_TRUE);
+        }
+        /* a text grouping key is hashed and compared by its bytes: one held
+         * compressed or external falls back to the CPU before that */
+        for (int j = 0; j < nattrs; j++)
+        {
+            const kern_aggregate_desc *d = &kactions->u.pagg.desc[j];
+            if (d->action == KAGG_ACTION__VREF && d->arg0_slot_id >= 0 &&
+                d->arg0_slot_id < C.nslots)
+            {
+                Reg r = C.reg_of_slot(d->arg0_slot_id);
+                if (r.dom == XD_TEXT)
+                    C.emit(XO_TXTCHK, r.r, r.r, 0, XP_REG_TRUE);
+            }
         }
         C.end_seg(P->seg_final);
         if (!C.err.empty())
@@ -1872,7 +1940,7 @@ xpu_plan_dump(const xpu_compiled *c)
         "F32_F64", "F64_F32", "I_BOOL", "TEXTLEN", "AND3", "OR3", "NOT3", "ISNULL", "ISNOTNULL",
         "BOOLTEST", "COALESCE", "G_NOTFALSE", "G_NOTTRUE", "G_ISNULL", "G_POSITIVE", "G_NONZERO",
         "JQ_STATUS", "HASH_INIT", "HASH_ADD", "HASH_FIN", "KILL", "PARAM", "G_TRUE", "SELECT",
-        "DISTINCT", "MINMAX", "LIKE", "SUBSTR", "HAOP"};
+        "DISTINCT", "MINMAX", "LIKE", "SUBSTR", "HAOP", "TXTCHK"};
     const xplan *P = &c->plan;
     std::string s;
     char buf[256];

@@ -25,6 +25,8 @@ struct xpu_compiled
     std::vector<std::string> messages;      /* error texts, indexed by xop.imm */
     std::vector<char>   kds_dst_head;       /* KDS_FORMAT_ROW header of kds_dst */
     std::vector<char>   kds_final_head;     /* header of kds_final (HASH or ROW) */
+    std::vector<char>   kds_fallback_head;  /* KDS_FORMAT_FALLBACK header (empty: no
+                                               CPU fallback) */
     bool                has_projection = false;
     bool                has_groupby = false;
     bool                pinned_hash = false;    /* projection rows go to a device-resident

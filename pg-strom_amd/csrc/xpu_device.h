@@ -375,105 +375,128 @@ like_fold(uint8_t c, bool ic)
     return (ic && c >= 'a' && c <= 'z') ? (uint8_t)(c - ('a' - 'A')) : c;
 }
 
+/* states of like_match's walk */
+enum { XLK_UNDECIDED = -2, XLK_STEP = 0, XLK_PERCENT, XLK_MISMATCH, XLK_SCAN };
+
 /* LIKE / ILIKE with the result of GenericMatchText (xpu_textlib.cu:915-1085)
  * and its error cases, without recursion: the newest % is the only point a
  * mismatch resumes from, since the reference's recursive search answers
  * "abort" rather than "false" once a deeper % has scanned to the end of the
- * text.  Returns 1 / 0, or -1 for an invalid escape.  Every loop advances t
- * or p, so the walk ends. */
+ * text.  Returns 1 / 0, or -1 for an invalid escape.  One flat loop over the
+ * walk's states: each step advances the text or pattern position or the
+ * resume point, or moves to the next state, and a match attempt starts at
+ * most once per text position, so (tlen + 2)(plen + 2) times four bounds
+ * the steps.  (A nesting of loops with early returns for the same walk gave
+ * wrong answers on gfx950 for lanes whose wave mates took other paths.) */
 __device__ int
 like_match(const uint8_t *t, int64_t tlen, const uint8_t *p, int64_t plen, bool ic, bool utf8)
 {
-    int64_t ti = 0, pi = 0, bt_t = -1, bt_p = -1;
-    uint8_t firstpat = 0;
     if (plen == 1 && p[0] == '%')
         return 1;
-    for (;;)
+    int64_t ti = 0, pi = 0, bt_t = -1, bt_p = -1;
+    uint8_t firstpat = 0;
+    int state = XLK_STEP, res = XLK_UNDECIDED;
+    const int64_t limit = 4 * (tlen + 2) * (plen + 2);
+    for (int64_t iter = 0; res == XLK_UNDECIDED; iter++)
     {
-        bool mismatch = false, scan = false;
-        while (ti < tlen && pi < plen && !mismatch && !scan)
+        if (iter > limit)
+            res = 0;
+        else if (state == XLK_STEP)
         {
-            const uint8_t c = p[pi];
-            if (c == '\\')
+            if (ti < tlen && pi < plen)
             {
-                if (++pi >= plen)
-                    return -1;
-                if (like_fold(p[pi], ic) != like_fold(t[ti], ic))
-                    mismatch = true;
+                const uint8_t c = p[pi];
+                if (c == '\\')
+                {
+                    if (pi + 1 >= plen)
+                        res = -1;
+                    else if (like_fold(p[pi + 1], ic) != like_fold(t[ti], ic))
+                        state = XLK_MISMATCH;
+                    else
+                    {
+                        ti++;
+                        pi += 2;
+                    }
+                }
+                else if (c == '%')
+                {
+                    pi++;
+                    state = XLK_PERCENT;
+                }
+                else if (c == '_')
+                {
+                    ti += txt_mblen(t + ti, utf8);
+                    pi++;
+                }
+                else if (like_fold(c, ic) != like_fold(t[ti], ic))
+                    state = XLK_MISMATCH;
                 else
                 {
                     ti++;
                     pi++;
                 }
             }
-            else if (c == '%')
-            {
+            else if (ti < tlen)
+                state = XLK_MISMATCH;
+            else if (pi < plen && p[pi] == '%')
                 pi++;
-                while (pi < plen)
-                {
-                    if (p[pi] == '%')
-                        pi++;
-                    else if (p[pi] == '_')
-                    {
-                        if (ti >= tlen)
-                            return 0;
-                        ti += txt_mblen(t + ti, utf8);
-                        pi++;
-                    }
-                    else
-                        break;
-                }
-                if (pi >= plen)
-                    return 1;
-                if (p[pi] == '\\')
-                {
-                    if (plen - pi < 2)
-                        return -1;
-                    firstpat = like_fold(p[pi + 1], ic);
-                }
+            else
+                res = pi >= plen ? 1 : 0;
+        }
+        else if (state == XLK_PERCENT)
+        {
+            /* after a %: further % and _ (each _ takes a character), then
+             * the first pattern character a match may start with */
+            if (pi >= plen)
+                res = 1;
+            else if (p[pi] == '%')
+                pi++;
+            else if (p[pi] == '_')
+            {
+                if (ti >= tlen)
+                    res = 0;
                 else
-                    firstpat = like_fold(p[pi], ic);
+                {
+                    ti += txt_mblen(t + ti, utf8);
+                    pi++;
+                }
+            }
+            else if (p[pi] == '\\' && plen - pi < 2)
+                res = -1;
+            else
+            {
+                firstpat = like_fold(p[pi] == '\\' ? p[pi + 1] : p[pi], ic);
                 bt_p = pi;
                 bt_t = ti;
-                scan = true;
-            }
-            else if (c == '_')
-            {
-                ti += txt_mblen(t + ti, utf8);
-                pi++;
-            }
-            else if (like_fold(c, ic) != like_fold(t[ti], ic))
-                mismatch = true;
-            else
-            {
-                ti++;
-                pi++;
+                state = XLK_SCAN;
             }
         }
-        if (!mismatch && !scan)
-        {
-            if (ti < tlen)
-                mismatch = true;
-            else
-            {
-                while (pi < plen && p[pi] == '%')
-                    pi++;
-                return pi >= plen ? 1 : 0;
-            }
-        }
-        if (mismatch)
+        else if (state == XLK_MISMATCH)
         {
             if (bt_p < 0)
-                return 0;
-            bt_t += txt_mblen(t + bt_t, utf8);
+                res = 0;
+            else
+            {
+                bt_t += txt_mblen(t + bt_t, utf8);
+                state = XLK_SCAN;
+            }
         }
-        while (bt_t < tlen && like_fold(t[bt_t], ic) != firstpat)
-            bt_t += txt_mblen(t + bt_t, utf8);
-        if (bt_t >= tlen)
-            return 0;
-        ti = bt_t;
-        pi = bt_p;
+        else
+        {
+            /* XLK_SCAN: the next text position the newest % can resume at */
+            if (bt_t >= tlen)
+                res = 0;
+            else if (like_fold(t[bt_t], ic) != firstpat)
+                bt_t += txt_mblen(t + bt_t, utf8);
+            else
+            {
+                ti = bt_t;
+                pi = bt_p;
+                state = XLK_STEP;
+            }
+        }
     }
+    return res;
 }
 
 /* __substring_common (xpu_textlib.cu:1159-1259) on 0-based start and count
@@ -568,6 +591,59 @@ load_fixed(const uint8_t *v, int width)
 #define RAISE(g, m) do { if (!RNULL(L, g) && L.R[g] != 0 && !L.err && !L.dead) \
                                 L.err = (uint32_t)(m) + 1; } while (0)
 
+/* The NaN that x86 SSE arithmetic gives, where the reference's float
+ * results are computed (its pgfn_float* code on the backend's CPU): the
+ * first NaN operand, quieted, else the default NaN of an invalid operation,
+ * whose sign is set.  CDNA's own NaN results can differ in sign: it
+ * subtracts by adding the negated operand, and its default NaN is
+ * positive. */
+__device__ __forceinline__ double
+nan_as_x86_f64(double x, double y, double v)
+{
+    if (!__builtin_isnan(v))
+        return v;
+    uint64_t b = 0xfff8000000000000ULL;
+    if (__builtin_isnan(x))
+        b = (uint64_t)__double_as_longlong(x) | 0x0008000000000000ULL;
+    else if (__builtin_isnan(y))
+        b = (uint64_t)__double_as_longlong(y) | 0x0008000000000000ULL;
+    return __longlong_as_double((long long)b);
+}
+
+__device__ __forceinline__ float
+nan_as_x86_f32(float x, float y, float v)
+{
+    if (!__builtin_isnan(v))
+        return v;
+    uint32_t b = 0xffc00000U;
+    if (__builtin_isnan(x))
+        b = (uint32_t)__float_as_int(x) | 0x00400000U;
+    else if (__builtin_isnan(y))
+        b = (uint32_t)__float_as_int(y) | 0x00400000U;
+    return __int_as_float((int)b);
+}
+
+/* A text register pair (R[r], R[r + 1]) with a negative length holds a
+ * compressed or external varlena as it is, R[r] addressing its header (the
+ * reference's xpu_text_t with length -1).  Projection copies it unchanged;
+ * an op that needs its bytes raises "text datum is compressed or external",
+ * which the session's CPU fallback takes (xpu_textlib.h:52-58). */
+#define XTEXT_RAW           (-1)
+#define TXT_RAW(r)          (L.R[(r) + 1] < 0)
+
+/* VARSIZE_ANY of a varlena header (1B_E: the TOAST pointer's size by tag) */
+__device__ __forceinline__ uint32_t
+varsize_any(const uint8_t *p)
+{
+    const uint8_t b0 = p[0];
+    if (b0 == 0x01)
+        return XPU_VARSIZE_EXTERNAL(p[1]);
+    if (b0 & 0x01)
+        return b0 >> 1;
+    return (((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
+             ((uint32_t)p[3] << 24)) >> 2) & 0x3fffffff;
+}
+
 /* STAGED: the caller has loaded the columns of the plan's stage[] list into
  * their registers, so the program's XLD_STAGED loads are skipped */
 template <bool STAGED>
@@ -657,9 +733,16 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                         uint2 pr = ((const uint2 *)(kbase + cm->values_offset))[row];
                         if (pr.y == XHEAP_LEN_EXTERNAL)
                         {
-                            if (!L.err)
-                                L.err = XMSG_TEXT_EXTERNAL + 1;
-                            isnull = true;
+                            /* kept as it is: the offset addresses its header */
+                            const uint8_t *vp = (const uint8_t *)kbase + cm->extra_offset + pr.x;
+                            if ((uint64_t)pr.x + 2 > cm->extra_length ||
+                                (uint64_t)pr.x + varsize_any(vp) > cm->extra_length)
+                                isnull = true;
+                            else
+                            {
+                                ptr = (int64_t)(uintptr_t)vp;
+                                len = XTEXT_RAW;
+                            }
                         }
                         else if ((uint64_t)pr.x + pr.y > cm->extra_length)
                             isnull = true;
@@ -794,6 +877,13 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     r = fcmp(op, as_f64(L.R[a]), as_f64(L.R[b]));
                 else
                 {
+                    if (TXT_RAW(a) || TXT_RAW(b))
+                    {
+                        RAISE(g, XMSG_TEXT_EXTERNAL);
+                        L.R[d] = 0;
+                        SETNULL(L, d, 1);
+                        break;
+                    }
                     const uint8_t *s1 = (const uint8_t *)(uintptr_t)L.R[a];
                     const uint8_t *s2 = (const uint8_t *)(uintptr_t)L.R[b];
                     int64_t l1 = L.R[a + 1], l2 = L.R[b + 1];
@@ -916,6 +1006,7 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     }
                     if (isinf(v) && !isinf(x) && !isinf(y))
                         RAISE(g, o.imm & 0xffff);
+                    v = nan_as_x86_f32(x, y, v);
                 }
                 L.R[d] = f32_bits(v);
                 SETNULL(L, d, n);
@@ -942,6 +1033,7 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     }
                     if (isinf(v) && !isinf(x) && !isinf(y))
                         RAISE(g, o.imm & 0xffff);
+                    v = nan_as_x86_f64(x, y, v);
                 }
                 L.R[d] = f64_bits(v);
                 SETNULL(L, d, n);
@@ -1015,7 +1107,27 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                  * characters by the database encoding, up to a NUL byte */
                 const uint8_t *s = (const uint8_t *)(uintptr_t)L.R[a];
                 int64_t len = L.R[a + 1];
-                const bool n = RNULL(L, a);
+                bool n = RNULL(L, a);
+                if (!n && len < 0)
+                {
+                    if ((o.aux & (XTXT_BPCHAR | XENC_UTF8)) == 0)
+                    {
+                        /* a single-byte encoding counts bytes, which the
+                         * header tells: the external pointer's va_extinfo,
+                         * or for a compressed datum the word the reference
+                         * copies from VARDATA into toast_compress_header's
+                         * tcinfo (the one after the compression info) */
+                        const uint8_t *w = s + ((s[0] == 0x01) ? 6 : 8);
+                        len = (int64_t)((((uint32_t)w[0] | ((uint32_t)w[1] << 8) |
+                                          ((uint32_t)w[2] << 16) | ((uint32_t)w[3] << 24))) &
+                                        0x3fffffffU);
+                        L.R[d] = len;
+                        SETNULL(L, d, 0);
+                        break;
+                    }
+                    RAISE(g, XMSG_TEXT_EXTERNAL);
+                    n = true;
+                }
                 if (!n && (o.aux & XTXT_BPCHAR))
                     len = txt_truelen(s, len);
                 if (!n && (o.aux & XENC_UTF8))
@@ -1034,8 +1146,13 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
             }
             case XO_LIKE:
             {
-                const bool n = RNULL(L, a) | RNULL(L, b);
+                bool n = RNULL(L, a) | RNULL(L, b);
                 int64_t v = 0;
+                if (!n && (TXT_RAW(a) || TXT_RAW(b)))
+                {
+                    RAISE(g, XMSG_TEXT_EXTERNAL);
+                    n = true;
+                }
                 if (!n)
                 {
                     const uint8_t *t = (const uint8_t *)(uintptr_t)L.R[a];
@@ -1056,9 +1173,14 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
             {
                 const bool has_count = (o.imm & XSUB_HAS_COUNT) != 0;
                 const int rc = (int)(o.imm & 0xff);
-                const bool n = RNULL(L, a) | RNULL(L, b) | (has_count && RNULL(L, rc));
+                bool n = RNULL(L, a) | RNULL(L, b) | (has_count && RNULL(L, rc));
                 int64_t off = 0, len = 0;
                 const uint8_t *str = (const uint8_t *)(uintptr_t)L.R[a];
+                if (!n && TXT_RAW(a))
+                {
+                    RAISE(g, XMSG_TEXT_EXTERNAL);
+                    n = true;
+                }
                 if (!n)
                 {
                     const int32_t count = has_count ? (int32_t)L.R[rc] : -1;
@@ -1174,6 +1296,11 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 if (!RNULL(L, a))
                 {
                     uint32_t h;
+                    if (o.aux == 0 && TXT_RAW(a))
+                    {
+                        RAISE(g, XMSG_TEXT_EXTERNAL);
+                        break;
+                    }
                     if (o.aux == 0)
                         h = hash_bytes((const uint8_t *)(uintptr_t)L.R[a], (uint32_t)L.R[a + 1]);
                     else
@@ -1237,6 +1364,11 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     }
                     else
                     {
+                        if (TXT_RAW(d) || TXT_RAW(a))
+                        {
+                            RAISE(g, XMSG_TEXT_EXTERNAL);
+                            break;
+                        }
                         const uint8_t *s1 = (const uint8_t *)(uintptr_t)L.R[d];
                         const uint8_t *s2 = (const uint8_t *)(uintptr_t)L.R[a];
                         int64_t l1 = L.R[d + 1], l2 = L.R[a + 1];
@@ -1249,13 +1381,21 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     }
                     take = (o.aux & XMM_GREATEST) ? (c < 0) : (c > 0);
                 }
-                if (take)
+                /* the choice is made opaque before both words of a text
+                 * value move: with the branches of the comparison above,
+                 * gfx950 code generation otherwise lost the pointer word's
+                 * update for lanes that took a uniform (constant) argument */
+                int32_t tk = take ? 1 : 0;
+                asm volatile("" : "+v"(tk));
+                const int64_t w0 = tk ? L.R[a] : L.R[d];
+                L.R[d] = w0;
+                if ((o.aux & 3) == XD_TEXT)
                 {
-                    L.R[d] = L.R[a];
-                    if ((o.aux & 3) == XD_TEXT)
-                        L.R[d + 1] = L.R[a + 1];
-                    SETNULL(L, d, 0);
+                    const int64_t w1 = tk ? L.R[a + 1] : L.R[d + 1];
+                    L.R[d + 1] = w1;
                 }
+                if (tk)
+                    SETNULL(L, d, 0);
                 break;
             }
             case XO_HAOP:
@@ -1269,6 +1409,11 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 const bool text_res = (rkind == XHA_KIND_TEXT);
                 bool n = RNULL(L, a);
                 const uint8_t *val = nullptr;
+                if (!n && kkind == XHA_KIND_TEXT && TXT_RAW(a))
+                {
+                    RAISE(g, XMSG_TEXT_EXTERNAL);
+                    n = true;
+                }
                 if (!n)
                 {
                     uint32_t nslots, default_offset, nitems;
@@ -1350,6 +1495,10 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 SETNULL(L, d, n || !val);
                 break;
             }
+            case XO_TXTCHK:
+                if (!RNULL(L, a) && TXT_RAW(a))
+                    RAISE(g, XMSG_TEXT_EXTERNAL);
+                break;
             default:
                 break;
         }
@@ -1398,6 +1547,11 @@ wave_incl_scan_u64(uint64_t v)
 __device__ __forceinline__ void
 record_error(xpu_task_ctl *ctl, uint32_t msgid, uint32_t code)
 {
+    /* a datum the GPU cannot read is the CPU fallback's error
+     * (SUSPEND_FALLBACK, xpu_textlib.h:52-58), also where the session
+     * has no fallback buffer to take the row */
+    if (msgid == XMSG_TEXT_EXTERNAL)
+        code = ERRCODE_SUSPEND_FALLBACK;
     if (atomicCAS(&ctl->errcode, 0U, code) == 0U)
         __hip_atomic_store(&ctl->errmsg, msgid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1406,13 +1560,12 @@ record_error(xpu_task_ctl *ctl, uint32_t msgid, uint32_t code)
  * projection: kern_form_minimal_tuple (xpu_common.cu:1096-1190)
  * ------------------------------------------------------------------ */
 __device__ __forceinline__ uint32_t
-proj_tuple_size(cplan *P, const Lane &L)
+form_tuple_size(const xattr *attrs, int nattrs, const Lane &L)
 {
-    int nattrs = P->proj_nattrs;
     bool hasnull = false;
     for (int j = 0; j < nattrs; j++)
     {
-        int r = P->proj[j].reg;
+        int r = attrs[j].reg;
         if (r == 0xff || RNULL(L, r))
             hasnull = true;
     }
@@ -1420,23 +1573,28 @@ proj_tuple_size(cplan *P, const Lane &L)
     t_hoff = (uint32_t)xalign(8, t_hoff);
     for (int j = 0; j < nattrs; j++)
     {
-        xattr pa = P->proj[j];
+        xattr pa = attrs[j];
         if (pa.reg == 0xff || RNULL(L, pa.reg))
             continue;
         t_hoff = (uint32_t)xalign(pa.attalign, t_hoff);
-        t_hoff += (pa.width > 0) ? pa.width : 4 + (uint32_t)L.R[pa.reg + 1];
+        if (pa.width > 0)
+            t_hoff += pa.width;
+        else if (TXT_RAW(pa.reg))
+            t_hoff += varsize_any((const uint8_t *)(uintptr_t)L.R[pa.reg]);
+        else
+            t_hoff += 4 + (uint32_t)L.R[pa.reg + 1];
     }
     return t_hoff + ROWID_SIZE;
 }
 
 __device__ __forceinline__ void
-proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t rowid)
+form_tuple_write(const xattr *attrs, int nattrs, const Lane &L, char *buf, uint32_t unit,
+                 uint32_t rowid, uint32_t hash)
 {
-    int nattrs = P->proj_nattrs;
     bool hasnull = false;
     for (int j = 0; j < nattrs; j++)
     {
-        int r = P->proj[j].reg;
+        int r = attrs[j].reg;
         if (r == 0xff || RNULL(L, r))
             hasnull = true;
     }
@@ -1449,7 +1607,7 @@ proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t row
     uint8_t *bits = (uint8_t *)buf + KERN_TUPITEM_T_BITS_OFFSET;
     for (int j = 0; j < nattrs; j++)
     {
-        xattr pa = P->proj[j];
+        xattr pa = attrs[j];
         if (pa.reg == 0xff || RNULL(L, pa.reg))
             continue;
         t_hoff = (uint32_t)xalign(pa.attalign, t_hoff);
@@ -1463,8 +1621,21 @@ proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t row
             case 8: *(int64_t *)p = v; t_hoff += 8; break;
             default:
             {
-                uint32_t len = (uint32_t)L.R[pa.reg + 1];
                 const uint8_t *s = (const uint8_t *)(uintptr_t)v;
+                if (TXT_RAW(pa.reg))
+                {
+                    /* a compressed or external datum, as it is
+                     * (xpu_text_datum_write, xpu_textlib.cu:319-325) */
+                    const uint32_t sz = varsize_any(s);
+                    for (uint32_t i = 0; i < sz; i++)
+                        p[i] = s[i];
+                    t_hoff += sz;
+                    if (s[0] == 0x01)
+                        infomask |= HEAP_HASEXTERNAL;
+                    infomask |= HEAP_HASVARWIDTH;
+                    break;
+                }
+                uint32_t len = (uint32_t)L.R[pa.reg + 1];
                 *(uint32_t *)p = (len + 4) << 2;
                 for (uint32_t i = 0; i < len; i++)
                     p[4 + i] = s[i];
@@ -1479,14 +1650,78 @@ proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t row
     t_hoff += ROWID_SIZE;
     kern_tupitem *t = (kern_tupitem *)buf;
     t->t_len = t_hoff;
-    /* rows of a pinned inner buffer carry their join-key hash */
-    t->hash = P->r_projhash ? (uint32_t)L.R[P->r_projhash] : 0U;
+    t->hash = hash;
     t->t_infomask2 = (uint16_t)(nattrs & HEAP_NATTS_MASK);
     t->t_infomask = infomask;
     t->t_hoff = (uint8_t)((uint32_t)xalign(8, KERN_TUPITEM_T_BITS_OFFSET +
                                           (hasnull ? XPU_BITMAPLEN(nattrs) : 0)) +
                           MINIMAL_TUPLE_OFFSET);
     *(uint32_t *)(buf + t_hoff - ROWID_SIZE) = rowid;
+}
+
+__device__ __forceinline__ uint32_t
+proj_tuple_size(cplan *P, const Lane &L)
+{
+    return form_tuple_size(P->proj, (int)P->proj_nattrs, L);
+}
+
+__device__ __forceinline__ void
+proj_tuple_write(cplan *P, const Lane &L, char *buf, uint32_t unit, uint32_t rowid)
+{
+    /* rows of a pinned inner buffer carry their join-key hash */
+    form_tuple_write(P->proj, (int)P->proj_nattrs, L, buf, unit, rowid,
+                     P->r_projhash ? (uint32_t)L.R[P->r_projhash] : 0U);
+}
+
+/* ------------------------------------------------------------------
+ * CPU fallback: a row whose depth-0 work (scan quals, the depth-1 join key
+ * hash, and without joins its projection or group-by arguments) raised
+ * "compressed or external" becomes a kern_fallbackitem of the session's
+ * KDS_FORMAT_FALLBACK buffer: depth 0, l_state 0, not matched, and a
+ * minimal tuple of the scan slot with the row's depth-0 variables
+ * (HandleErrorIfCpuFallback and __writeOutCpuFallbackTuple,
+ * xpu_common.cu:1832-1965).  The backend re-runs it from its WHERE clause
+ * (execCpuFallbackOneTuple, fallback.c:441-470), which the reference also
+ * does for the depth it records; such a row produces nothing on the GPU.
+ * ------------------------------------------------------------------ */
+#define XFB_ITEM_HEAD   16      /* offsetof(kern_fallbackitem, t) */
+
+__device__ __forceinline__ bool
+fallback_write(cplan *P, const Lane &L, kern_data_store *kfb)
+{
+    const int nattrs = (int)P->fb_nattrs;
+    const uint32_t req = (uint32_t)xalign(8, XFB_ITEM_HEAD + form_tuple_size(P->fb, nattrs, L));
+    const unsigned long long usage =
+        atomicAdd((unsigned long long *)&kfb->usage, (unsigned long long)req) + req;
+    const uint64_t head = KDS_HEAD_LENGTH(kfb);
+    uint32_t cur = __hip_atomic_load(&kfb->nitems, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t old;
+    do
+    {
+        old = cur;
+        if (head + 8ULL * ((uint64_t)old + 1) + usage > kfb->length)
+            return false;
+        cur = atomicCAS(&kfb->nitems, old, old + 1);
+    } while (cur != old);
+    char *item = (char *)kfb + kfb->length - usage;
+    *(uint64_t *)item = 0;                  /* l_state */
+    *(uint64_t *)(item + 8) = 0;            /* depth 0, not matched */
+    form_tuple_write(P->fb, nattrs, L, item + XFB_ITEM_HEAD, req - XFB_ITEM_HEAD, old, 0U);
+    ((uint64_t *)((char *)kfb + head))[old] = usage - XFB_ITEM_HEAD;
+    return true;
+}
+
+/* true if the lane's error sent its row to the CPU-fallback buffer (the
+ * error is then cleared); a full buffer fails the task */
+__device__ __forceinline__ bool
+row_fallback(cplan *P, Lane &L, const xpu_kparams &kp, bool write)
+{
+    if (L.err != XMSG_TEXT_EXTERNAL + 1 || kp.kds_fallback == nullptr || P->fb_nattrs == 0)
+        return false;
+    if (write && !fallback_write(P, L, kp.kds_fallback))
+        record_error(kp.ctl, XMSG_FALLBACK_FULL, ERRCODE_DEVICE_ERROR);
+    L.err = 0;
+    return true;
 }
 
 /* ------------------------------------------------------------------
@@ -1965,18 +2200,185 @@ bloom_build_range(kern_data_store *ik, uint64_t *words, uint32_t mask, uint64_t 
     }
 }
 
+/* Reserves a group tuple of `unit` bytes (a multiple of 128) in a
+ * KDS_FORMAT_HASH buffer, whose tuples grow from the tail and row index
+ * from the head.  Every unit is at least 128 bytes, so tuples ending `end`
+ * bytes from the tail number at most end/128, and the test keeps room for
+ * their row-index entries too.  Space is taken with one atomic add: the
+ * reservations that pass the test are those ending below a bound, a prefix
+ * of the atomic order, so a failed one leaves no hole among the tuples; only
+ * kds->usage runs past them (the host recomputes it from the row index
+ * before it grows the buffer).  The row id is taken after the space, so the
+ * row index has no hole either (the reference instead locks nitems around
+ * both, cuda_gpupreagg.cu:1141-1168). */
+__device__ __forceinline__ kern_hashitem *
+gb_alloc(kern_data_store *kds, uint32_t unit, uint32_t *p_rowid)
+{
+    const uint64_t fixed = KDS_HEAD_LENGTH(kds) + 8ULL * kds->hash_nslots;
+    const unsigned long long end =
+        atomicAdd((unsigned long long *)&kds->usage, (unsigned long long)unit) + unit;
+    if (fixed + end + 8ULL * (end / XPU_L1_CACHELINE_SZ) > kds->length)
+        return nullptr;
+    *p_rowid = atomicAdd(&kds->nitems, 1U);
+    return hashitem_at(kds, end);
+}
+
+/* merges the partial states of group tuple src into group tuple dst (both
+ * of this plan's layout, xpu_common.h:2400-2471): what the final merge of
+ * partial results does per group (gpu_service.c:5429-5569) */
+__device__ __forceinline__ void
+gb_merge_tuple(cplan *P, kern_tupitem *dst, const kern_tupitem *src)
+{
+    char *d = (char *)dst + P->gb_t_hoff;
+    const char *s = (const char *)src + P->gb_t_hoff;
+    for (int j = 0; j < (int)P->gb_nattrs; j++)
+    {
+        xagg ag = P->aggs[j];
+        if (ag.action == KAGG_ACTION__VREF)
+            continue;
+        char *ds = d + ag.state_off;
+        const char *ss = s + ag.state_off;
+        switch (ag.action)
+        {
+            case KAGG_ACTION__NROWS_ANY:
+            case KAGG_ACTION__NROWS_COND:
+            {
+                const unsigned long long c = *(const unsigned long long *)ss;
+                if (c)
+                    atomicAdd((unsigned long long *)ds, c);
+                break;
+            }
+            case KAGG_ACTION__PMIN_INT32: case KAGG_ACTION__PMIN_INT64:
+            case KAGG_ACTION__PMAX_INT32: case KAGG_ACTION__PMAX_INT64:
+            {
+                if (!(*(const uint32_t *)(ss + 4) & __PAGG_MINMAX_ATTRS__VALID))
+                    break;
+                const long long v = *(const long long *)(ss + 8);
+                atomicOr((unsigned int *)(ds + 4), __PAGG_MINMAX_ATTRS__VALID);
+                if (ag.action == KAGG_ACTION__PMIN_INT32 || ag.action == KAGG_ACTION__PMIN_INT64)
+                    atomicMin((long long *)(ds + 8), v);
+                else
+                    atomicMax((long long *)(ds + 8), v);
+                break;
+            }
+            case KAGG_ACTION__PMIN_FP64: case KAGG_ACTION__PMAX_FP64:
+            {
+                if (!(*(const uint32_t *)(ss + 4) & __PAGG_MINMAX_ATTRS__VALID))
+                    break;
+                const double v = *(const double *)(ss + 8);
+                atomicOr((unsigned int *)(ds + 4), __PAGG_MINMAX_ATTRS__VALID);
+                if (ag.action == KAGG_ACTION__PMIN_FP64)
+                    atomicMin((double *)(ds + 8), v);
+                else
+                    atomicMax((double *)(ds + 8), v);
+                break;
+            }
+            case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
+            {
+                const unsigned long long n = *(const unsigned long long *)(ss + 8);
+                if (!n)
+                    break;
+                atomicAdd((unsigned long long *)(ds + 8), n);
+                atomicAdd((unsigned long long *)(ds + 16), *(const unsigned long long *)(ss + 16));
+                break;
+            }
+            case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            {
+                const unsigned long long n = *(const unsigned long long *)(ss + 8);
+                if (!n)
+                    break;
+                const unsigned long long lo = *(const unsigned long long *)(ss + 16);
+                unsigned long long hi = *(const unsigned long long *)(ss + 24);
+                atomicAdd((unsigned long long *)(ds + 8), n);
+                const unsigned long long old = atomicAdd((unsigned long long *)(ds + 16), lo);
+                hi += (old + lo < old) ? 1ULL : 0ULL;
+                if (hi)
+                    atomicAdd((unsigned long long *)(ds + 24), hi);
+                break;
+            }
+            case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
+            {
+                const unsigned long long n = *(const unsigned long long *)(ss + 8);
+                if (!n)
+                    break;
+                atomicAdd((unsigned long long *)(ds + 8), n);
+                atomicAdd((double *)(ds + 16), *(const double *)(ss + 16));
+                break;
+            }
+            case KAGG_ACTION__STDDEV:
+            {
+                const unsigned long long n = *(const unsigned long long *)(ss + 8);
+                if (!n)
+                    break;
+                atomicAdd((unsigned long long *)(ds + 8), n);
+                atomicAdd((double *)(ds + 16), *(const double *)(ss + 16));
+                atomicAdd((double *)(ds + 24), *(const double *)(ss + 24));
+                break;
+            }
+        }
+    }
+}
+
+/* loads the grouping keys of group tuple t into their registers: the
+ * inverse of the key part of gb_tuple_write (a text key's register
+ * addresses its bytes inside t) */
+__device__ __forceinline__ void
+gb_tuple_keys(cplan *P, Lane &L, const kern_tupitem *t)
+{
+    const char *base = (const char *)t;
+    uint32_t pos = P->gb_t_hoff + P->gb_states_end;
+    for (int j = 0; j < (int)P->gb_nattrs; j++)
+    {
+        xagg ag = P->aggs[j];
+        if (ag.action != KAGG_ACTION__VREF)
+            continue;
+        xattr ka = P->keys[ag.key_index];
+        pos = (uint32_t)xalign(ka.attalign, pos);
+        const bool tnull = !((t->t_bits[j >> 3] >> (j & 7)) & 1);
+        SETNULL(L, ka.reg, tnull);
+        if (tnull)
+            continue;
+        const char *p = base + pos;
+        switch (ka.width)
+        {
+            case 1: L.R[ka.reg] = *(const int8_t *)p; pos += 1; break;
+            case 2: L.R[ka.reg] = *(const int16_t *)p; pos += 2; break;
+            case 4:
+                L.R[ka.reg] = ka.domain == XD_F32 ? (int64_t)*(const uint32_t *)p
+                                                  : (int64_t)*(const int32_t *)p;
+                pos += 4;
+                break;
+            case 8: L.R[ka.reg] = *(const int64_t *)p; pos += 8; break;
+            default:
+            {
+                const uint32_t len = ((*(const uint32_t *)p >> 2) & 0x3fffffff) - 4;
+                L.R[ka.reg] = (int64_t)(uintptr_t)(p + 4);
+                L.R[ka.reg + 1] = len;
+                pos += 4 + len;
+                break;
+            }
+        }
+    }
+}
+
 /* Find-or-insert the group of the key registers of every pending lane and
- * merge its contribution; a wave-level retry loop in which a lane either
- * finds the group, or takes the slot lock once and inserts it, so lanes of
- * one wave never wait on each other's locks (the reference does the same at
- * block level, cuda_gpupreagg.cu:1607-1731). */
+ * merge its contribution: the row in the registers, LDS entry lds_slot of
+ * T, or the partial states of group tuple src.  A wave-level retry loop in
+ * which a lane either finds the group, or takes the slot lock once and
+ * inserts it, so lanes of one wave never wait on each other's locks (the
+ * reference does the same at block level, cuda_gpupreagg.cu:1607-1731).
+ * A group that finds no room in kds_final goes to the overflow table
+ * kds_ovf, when there is one: once a reservation in kds_final has failed,
+ * every later one fails too, so a group lives in exactly one of the two
+ * tables.  Without room in either, the contribution is dropped and the task
+ * fails with ERRCODE_SUSPEND_NO_SPACE. */
 __device__ __forceinline__ void
 gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pending,
-          const LdsTable *T, int lds_slot, xpu_task_ctl *ctl)
+          const LdsTable *T, int lds_slot, xpu_task_ctl *ctl,
+          kern_data_store *kds_ovf = nullptr, const kern_tupitem *src = nullptr)
 {
     if (!wave_any(pending))
         return;
-    uint32_t nslots = kds_final->hash_nslots;
     uint32_t iter = 0;
 
     if (P->nkeys == 0)
@@ -1986,12 +2388,15 @@ gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pen
         {
             uint64_t *rowidx = (uint64_t *)((char *)kds_final + KDS_HEAD_LENGTH(kds_final));
             kern_tupitem *t = (kern_tupitem *)((char *)kds_final + kds_final->length - rowidx[0]);
-            gb_merge_global(P, L, t, T, lds_slot);
+            if (src)
+                gb_merge_tuple(P, t, src);
+            else
+                gb_merge_global(P, L, t, T, lds_slot);
         }
         return;
     }
-    uint64_t *slots = (uint64_t *)((char *)kds_final + KDS_HEAD_LENGTH(kds_final));
-    uint64_t *slot = slots + (hash % nslots);
+    kern_data_store *kds = kds_final;
+    uint64_t *slot = (uint64_t *)((char *)kds + KDS_HEAD_LENGTH(kds)) + (hash % kds->hash_nslots);
     while (wave_any(pending))
     {
         if (pending)
@@ -2007,7 +2412,7 @@ gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pen
             {
                 for (uint64_t off = head; off != 0; )
                 {
-                    kern_hashitem *hi = hashitem_at(kds_final, off);
+                    kern_hashitem *hi = hashitem_at(kds, off);
                     uint32_t h2 = __hip_atomic_load(&hi->t.hash, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
                     if (h2 == hash && gb_keys_match(P, L, &hi->t))
@@ -2025,33 +2430,43 @@ gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pen
                     uint32_t tsz = gb_tuple_size(P, L);
                     uint32_t unit = (uint32_t)xalign(XPU_L1_CACHELINE_SZ,
                                                      offsetof(kern_hashitem, t) + tsz);
-                    uint32_t rowid = atomicAdd(&kds_final->nitems, 1U);
-                    unsigned long long usage =
-                        atomicAdd((unsigned long long *)&kds_final->usage, (unsigned long long)unit) + unit;
-                    uint64_t need = KDS_HEAD_LENGTH(kds_final) +
-                                    8ULL * ((uint64_t)nslots + rowid + 1) + usage;
-                    if (need > kds_final->length)
+                    uint32_t rowid = 0;
+                    kern_hashitem *hi = gb_alloc(kds, unit, &rowid);
+                    if (!hi)
                     {
-                        record_error(ctl, XMSG_FINAL_NO_SPACE, ERRCODE_SUSPEND_NO_SPACE);
                         __hip_atomic_store(slot, head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        pending = false;
+                        if (kds_ovf && kds != kds_ovf)
+                        {
+                            kds = kds_ovf;
+                            slot = (uint64_t *)((char *)kds + KDS_HEAD_LENGTH(kds)) +
+                                   (hash % kds->hash_nslots);
+                        }
+                        else
+                        {
+                            record_error(ctl, XMSG_FINAL_NO_SPACE, ERRCODE_SUSPEND_NO_SPACE);
+                            pending = false;
+                        }
                     }
                     else
                     {
-                        kern_hashitem *hi = hashitem_at(kds_final, usage);
                         gb_tuple_write(P, L, hi, unit, hash, rowid, head);
-                        uint64_t *rowidx = slots + nslots;
-                        rowidx[rowid] = (uint64_t)((char *)kds_final + kds_final->length -
-                                                   (char *)&hi->t);
-                        __hip_atomic_store(slot, (uint64_t)usage, __ATOMIC_RELEASE,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+                        uint64_t *rowidx = (uint64_t *)((char *)kds + KDS_HEAD_LENGTH(kds)) +
+                                           kds->hash_nslots;
+                        rowidx[rowid] = (uint64_t)((char *)kds + kds->length - (char *)&hi->t);
+                        __hip_atomic_store(slot, (uint64_t)((char *)kds + kds->length - (char *)hi),
+                                           __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                        if (kds != kds_final)
+                            ctl->final_ovf = 1U;
                         found = hi;
                     }
                 }
             }
             if (found)
             {
-                gb_merge_global(P, L, &found->t, T, lds_slot);
+                if (src)
+                    gb_merge_tuple(P, &found->t, src);
+                else
+                    gb_merge_global(P, L, &found->t, T, lds_slot);
                 pending = false;
             }
         }
@@ -2061,6 +2476,43 @@ gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pen
                 record_error(ctl, XMSG_GROUP_RETRY, ERRCODE_DEVICE_ERROR);
             break;
         }
+    }
+}
+
+/* the groups of the overflow table kds_ovf, merged into kds_final (grown by
+ * the host to hold them): one lane per overflow group */
+template <int NR>
+__device__ __forceinline__ void
+final_merge_ovf_body(cplan *P, kern_data_store *kds_final, const kern_data_store *kds_ovf,
+                     xpu_task_ctl *ctl)
+{
+    int64_t Rfile[NR];
+    Lane L;
+    L.R = Rfile;
+    L.nmask = ~1U;
+    L.err = 0;
+    L.dead = 0;
+#pragma unroll
+    for (int i = 0; i < NR; i++)
+        L.R[i] = 0;
+    L.R[XP_REG_TRUE] = 1;
+    const uint32_t n = kds_ovf->nitems;
+    const uint64_t *rowidx = (const uint64_t *)((const char *)kds_ovf + KDS_HEAD_LENGTH(kds_ovf)) +
+                             kds_ovf->hash_nslots;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride)
+    {
+        const uint32_t i = base + threadIdx.x;
+        const bool pending = i < n;
+        const kern_tupitem *t = nullptr;
+        uint32_t hash = 0;
+        if (pending)
+        {
+            t = (const kern_tupitem *)((const char *)kds_ovf + kds_ovf->length - rowidx[i]);
+            gb_tuple_keys(P, L, t);
+            hash = t->hash;
+        }
+        gb_global(P, L, kds_final, hash, pending, nullptr, -1, ctl, nullptr, t);
     }
 }
 
@@ -2264,7 +2716,8 @@ lds_init(cplan *P, LdsTable &T)
  * barrier that ends all updates of the table) */
 template <int NR>
 __device__ __forceinline__ void
-lds_flush(cplan *P, LdsTable &T, kern_data_store *kds_final, xpu_task_ctl *ctl)
+lds_flush(cplan *P, LdsTable &T, kern_data_store *kds_final, xpu_task_ctl *ctl,
+          kern_data_store *kds_ovf = nullptr)
 {
     int64_t Rfile[NR];
     Lane L;
@@ -2293,7 +2746,7 @@ lds_flush(cplan *P, LdsTable &T, kern_data_store *kds_final, xpu_task_ctl *ctl)
                 nflush++;
             }
         }
-        gb_global(P, L, kds_final, hash, pending, &T, pending ? (int)h : -1, ctl);
+        gb_global(P, L, kds_final, hash, pending, &T, pending ? (int)h : -1, ctl, kds_ovf);
     }
     if (nflush)
         atomicAdd((unsigned long long *)&ctl->global_merges, (unsigned long long)nflush);
@@ -2389,7 +2842,7 @@ lds_dump(cplan *P, LdsTable &T, const xpu_kparams &kp)
                 }
             }
         }
-        gb_global(P, L, kp.kds_out, hash, pending, &T, pending ? (int)h : -1, kp.ctl);
+        gb_global(P, L, kp.kds_out, hash, pending, &T, pending ? (int)h : -1, kp.ctl, kp.kds_ovf);
     }
     if (ndirect)
         atomicAdd((unsigned long long *)&kp.ctl->global_merges, (unsigned long long)ndirect);
@@ -2485,7 +2938,7 @@ dump_merge_body(cplan *P, const xpu_kparams &kp)
                 /* the table is full: merge it, start over, then fold the
                  * records that found no room (each round places at least
                  * one of them in the emptied table) */
-                lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+                lds_flush<NR>(P, T, kp.kds_out, kp.ctl, kp.kds_ovf);
                 __syncthreads();
                 lds_init(P, T);
                 __syncthreads();
@@ -2501,7 +2954,7 @@ dump_merge_body(cplan *P, const xpu_kparams &kp)
             }
         }
         __syncthreads();
-        lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+        lds_flush<NR>(P, T, kp.kds_out, kp.ctl, kp.kds_ovf);
         __syncthreads();
     }
 }
@@ -2764,7 +3217,8 @@ screen_row(cplan *P, const xpu_kparams &kp, Lane &L, uint32_t row, bool hash, ui
     PROG::template run<STAGED>(P, P->seg_scan.begin, P->seg_scan.end, L, kp.kds_src, row, kp);
     if (L.err)
     {
-        record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+        if (!row_fallback(P, L, kp, true))
+            record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
         return false;
     }
     if (RNULL(L, P->r_scan_ok) || L.R[P->r_scan_ok] == 0)
@@ -2775,7 +3229,8 @@ screen_row(cplan *P, const xpu_kparams &kp, Lane &L, uint32_t row, bool hash, ui
                                    kp.kds_src, row, kp);
         if (L.err)
         {
-            record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+            if (!row_fallback(P, L, kp, true))
+                record_error(kp.ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
             return false;
         }
         *h = (uint32_t)L.R[P->r_hash];
@@ -2863,14 +3318,17 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
         for (int i = 0; i < XP_MAX_DEPTH + 2; i++)
             it[i] = 0;
 
-        /* an error ends the lane's row and is reported for the task */
-        auto run_seg = [&](const xseg seg, const bool act) -> bool {
+        /* an error ends the lane's row and is reported for the task, or in
+         * the row's depth-0 work (fb) sends it to the CPU fallback: written
+         * by the pass that keeps the row's output, dropped by a count pass */
+        auto run_seg = [&](const xseg seg, const bool act, const bool fb = false) -> bool {
             if (!act)
                 return false;
             PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
             if (L.err)
             {
-                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                if (!(fb && row_fallback(P, L, kp, PM != PM_COUNT && pass == 0)))
+                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
                 L.err = 0;
                 return true;
             }
@@ -2902,7 +3360,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
             /* bucket walk to the first candidate */
             {
                 const bool act = (cd == d && it[d] == 0);
-                const bool err = run_seg(P->depth[d].seg_hash, act);
+                const bool err = run_seg(P->depth[d].seg_hash, act, d == 1);
                 if (act && !err)
                 {
                     const uint32_t h = (uint32_t)L.R[P->r_hash];
@@ -2975,7 +3433,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
 
         /* scan quals */
         {
-            const bool err = run_seg(P->seg_scan, valid);
+            const bool err = run_seg(P->seg_scan, valid, true);
             const bool ok = valid && !err && !RNULL(L, P->r_scan_ok) && L.R[P->r_scan_ok] != 0;
             if (ok && pass == 0 && COUNTS && !queued)
                 cnt_in++;
@@ -2997,7 +3455,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
             }
             /* the final stage */
             const bool act = (cd == nrels + 1);
-            const bool err = run_seg(P->seg_final, act);
+            const bool err = run_seg(P->seg_final, act, nrels == 0);
             if (act && !err)
             {
                 if (pass == 0 && COUNTS)
@@ -3017,7 +3475,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
                     if (direct)
                         lds_miss++;
                     uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl, kp.kds_ovf);
                 }
                 else
                 {
@@ -3290,7 +3748,7 @@ pipeline_body(cplan *P, const xpu_kparams &kp)
     if (PREAGG && T.n > 0)
     {
         __syncthreads();
-        lds_flush<NR>(P, T, kp.kds_out, ctl);
+        lds_flush<NR>(P, T, kp.kds_out, ctl, kp.kds_ovf);
     }
     if (cnt_in)
         atomicAdd(&ctl->nitems_in, cnt_in);
@@ -3496,14 +3954,17 @@ join_probe_body(cplan *P, const xpu_kparams &kp)
         lane_reset<NR>(L);
         uint32_t row = 0;
         bool alive = valid;
-        /* an error ends the record and is reported for the task */
-        auto run_seg = [&](const xseg seg, const bool act) -> bool {
+        /* an error ends the record and is reported for the task; the
+         * depth-1 key hash of a level-1 record (the row alone) may send the
+         * row to the CPU fallback instead */
+        auto run_seg = [&](const xseg seg, const bool act, const bool fb = false) -> bool {
             if (!act)
                 return false;
             PROG::template run<false>(P, seg.begin, seg.end, L, kds, row, kp);
             if (L.err)
             {
-                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                if (!(fb && row_fallback(P, L, kp, true)))
+                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
                 L.err = 0;
                 return true;
             }
@@ -3529,7 +3990,7 @@ join_probe_body(cplan *P, const xpu_kparams &kp)
         }
         /* bucket walk to the first candidate */
         uint32_t h = 0;
-        alive = alive && !run_seg(P->depth[d].seg_hash, alive);
+        alive = alive && !run_seg(P->depth[d].seg_hash, alive, d == 1);
         if (alive)
             h = (uint32_t)L.R[P->r_hash];
         uint64_t off = 0;
@@ -3611,7 +4072,7 @@ join_probe_body(cplan *P, const xpu_kparams &kp)
                     if (direct)
                         lds_miss++;
                     uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+                    gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl, kp.kds_ovf);
                 }
             }
             else
@@ -3656,7 +4117,7 @@ join_probe_body(cplan *P, const xpu_kparams &kp)
     if (T.n > 0)
     {
         __syncthreads();
-        lds_flush<NR>(P, T, kp.kds_out, ctl);
+        lds_flush<NR>(P, T, kp.kds_out, ctl, kp.kds_ovf);
     }
     if (cnt_out)
         atomicAdd(&ctl->nitems_out, cnt_out);
@@ -3765,7 +4226,9 @@ scan_proj_body(cplan *P, const xpu_kparams &kp)
             bool ok = valid && !L.dead;
             if (valid && L.err)
             {
-                if (PM == PM_COUNT)
+                /* the write pass writes a CPU-fallback row once (the
+                 * count pass drops it) */
+                if (!row_fallback(P, L, kp, PM == PM_WRITE) && PM == PM_COUNT)
                     record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
                 ok = false;
             }
@@ -4026,7 +4489,8 @@ scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
             bool ok = valid && !L.dead;
             if (valid && L.err)
             {
-                record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
+                if (!row_fallback(P, L, kp, true))
+                    record_error(ctl, L.err - 1, ERRCODE_DEVICE_ERROR);
                 ok = false;
             }
             if (ok)
@@ -4088,7 +4552,7 @@ scan_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t wt_first,
             if (wave_any(direct))
             {
                 uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-                gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+                gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl, kp.kds_ovf);
             }
         };
         if constexpr (PROG::kUnrollRows)
@@ -4148,7 +4612,7 @@ scan_agg_body(cplan *P, const xpu_kparams &kp)
         if (kp.dump)
             lds_dump<NR>(P, T, kp);
         else
-            lds_flush<NR>(P, T, kp.kds_out, kp.ctl);
+            lds_flush<NR>(P, T, kp.kds_out, kp.ctl, kp.kds_ovf);
     }
     scan_agg_report(kp.ctl, W);
 }
@@ -4230,7 +4694,7 @@ spill_agg_range(cplan *P, const xpu_kparams &kp, LdsTable &T, uint64_t lo, uint6
         if (wave_any(direct))
         {
             uint32_t hash = direct ? gb_hash_regs(P, L) : 0;
-            gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl);
+            gb_global(P, L, kp.kds_out, hash, direct, &T, -1, ctl, kp.kds_ovf);
         }
     }
     return lds_miss;
@@ -4336,7 +4800,7 @@ spill_agg_body(cplan *P, const xpu_kparams &kp)
     const uint64_t hi = nrec * (kblk + 1) / nblk;
     uint32_t lds_miss = spill_agg_range<NR>(P, kp, T, lo, hi, part, nparts);
     __syncthreads();
-    lds_flush<NR>(P, T, kp.kds_out, ctl);
+    lds_flush<NR>(P, T, kp.kds_out, ctl, kp.kds_ovf);
     if (lds_miss)
         atomicAdd((unsigned long long *)&ctl->lds_miss, (unsigned long long)lds_miss);
 }

@@ -288,7 +288,10 @@ deform_tuple(const xheap_params &p, const xheap_col *cols, const uint8_t *h, uin
                 }
                 else
                 {
-                    voff = (uint32_t)(h_src + pos + hdr);
+                    /* a compressed or external datum is addressed with its
+                     * header: it is copied as it is (projection) or falls
+                     * back to the CPU (anything that reads its bytes) */
+                    voff = (uint32_t)(h_src + pos + (ext ? 0 : hdr));
                     vlen = ext ? XHEAP_LEN_EXTERNAL : sz - hdr;
                     pos += sz;
                 }

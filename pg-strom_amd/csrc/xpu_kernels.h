@@ -48,7 +48,7 @@ typedef struct
     uint32_t    nitems_in;          /* rows that passed the scan quals */
     uint32_t    nitems_out;         /* rows reaching the final stage */
     uint32_t    depth_out[XP_MAX_DEPTH + 1];
-    uint32_t    __pad;
+    uint32_t    final_ovf;          /* a group went to the overflow table */
     uint64_t    out_nitems;         /* projection rows written */
     uint64_t    out_usage;          /* projection bytes written */
     uint64_t    lds_miss;           /* group-by rows sent straight to kds_final */
@@ -67,6 +67,7 @@ typedef struct
 #define XMSG_HEAP_ITEM          0xfff6
 #define XMSG_HEAP_TUPLE         0xfff7
 #define XMSG_ARROW_TYPE         0xfff8
+#define XMSG_FALLBACK_FULL      0xfff9
 /* the longest varlena an Arrow column may hold (VARATT_MAX, xpu_common.h:1575) */
 #define XARROW_VARATT_MAX       0x4ffffff8U
 
@@ -81,7 +82,8 @@ typedef struct
  * the datum's bytes inside the source chunk (no bytes are copied): such a
  * column carries attopts.tag XARROW_TAG_HEAPREF and its extra_offset is the
  * source chunk's address minus the tile buffer's, modulo 2^64.  A length of
- * XHEAP_LEN_EXTERNAL marks a compressed or external datum.
+ * XHEAP_LEN_EXTERNAL marks a compressed or external datum, whose offset
+ * addresses its varlena header.
  * ------------------------------------------------------------------ */
 #define XARROW_TAG_HEAPREF      0x4845      /* not an ArrowTypeTag */
 #define XHEAP_LEN_EXTERNAL      0xffffffffU
@@ -221,6 +223,12 @@ typedef struct
     uint32_t    screened;               /* level-1 records passed the depth-1 filter */
     /* scan + projection: rows and bytes of each count-pass block */
     uint64_t   *block_status;
+    /* group-by: the session's overflow table (KDS_FORMAT_HASH, nullptr:
+     * none), which takes the groups that find no room in kds_final; the
+     * host grows kds_final and merges it in after the launch */
+    kern_data_store *kds_ovf;
+    /* the session's CPU-fallback buffer (KDS_FORMAT_FALLBACK, nullptr: none) */
+    kern_data_store *kds_fallback;
 } xpu_kparams;
 
 #ifdef __cplusplus
@@ -238,6 +246,11 @@ int xpu_launch_tile_scan_blocks(const xpu_kparams *kp, int grid, void *stream);
 /* folds the partial-state dumps of a scan + group-by task into kds_final */
 int xpu_launch_dump_merge(const xpu_kparams *kp, int nregs, int grid, int block,
                           size_t lds_bytes, void *stream);
+/* merges the groups of a session's overflow table into kds_final (grown
+ * to hold them); the grid covers ovf_nitems groups */
+int xpu_launch_final_merge_ovf(const xplan *plan, int nregs, kern_data_store *kds_final,
+                               const kern_data_store *kds_ovf, uint32_t ovf_nitems,
+                               xpu_task_ctl *ctl, void *stream);
 /* the spill setup kernel of a batch alone (for specialized pass kernels) */
 int xpu_launch_batch_setup(const xpu_kparams *kp, void *stream);
 /* batched fragments: pass 1, spill setup, pass 2 (all stream-ordered) */

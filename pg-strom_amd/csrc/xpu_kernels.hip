@@ -221,6 +221,35 @@ xpu_launch_spill_agg(const xpu_kparams *kp, int nregs, int grid, int block, size
     return (int)hipGetLastError();
 }
 
+template <int NR>
+__global__ void __launch_bounds__(256)
+xpu_final_merge_ovf_kernel(const xplan *plan, kern_data_store *kds_final,
+                           const kern_data_store *kds_ovf, xpu_task_ctl *ctl)
+{
+    final_merge_ovf_body<NR>((cplan *)(uintptr_t)plan, kds_final, kds_ovf, ctl);
+}
+
+extern "C" int
+xpu_launch_final_merge_ovf(const xplan *plan, int nregs, kern_data_store *kds_final,
+                           const kern_data_store *kds_ovf, uint32_t ovf_nitems, xpu_task_ctl *ctl,
+                           void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t block = 256;
+    uint32_t grid = (ovf_nitems + block - 1) / block;
+    if (grid == 0)
+        return 0;
+    if (grid > 4096)
+        grid = 4096;
+    if (nregs <= XP_NREGS_SMALL)
+        hipLaunchKernelGGL((xpu_final_merge_ovf_kernel<XP_NREGS_SMALL>), dim3(grid), dim3(block),
+                           0, s, plan, kds_final, kds_ovf, ctl);
+    else
+        hipLaunchKernelGGL((xpu_final_merge_ovf_kernel<XP_NREGS>), dim3(grid), dim3(block),
+                           0, s, plan, kds_final, kds_ovf, ctl);
+    return (int)hipGetLastError();
+}
+
 /* device-to-device copy in 16-byte words (plus a byte tail) */
 __global__ void
 xpu_copy_kernel(uint4 *dst, const uint4 *src, size_t nwords, char *dtail, const char *stail,

@@ -145,6 +145,10 @@ enum xop_code {
        the key R[a] (hash-slot walk of the reference's kern_expression
        layout); aux = XHA_* key / result description */
     XO_HAOP,
+    /* R[a] (text) holding a compressed or external datum raises "text datum
+       is compressed or external" where R[guard]: a grouping key needs its
+       bytes */
+    XO_TXTCHK,
     XO_NUM_OPS
 };
 
@@ -254,6 +258,11 @@ typedef struct
     xattr       keys[XP_MAX_KEYS];
     uint8_t     key_bits[XP_MAX_KEYS];  /* packed width of each key in bits */
     xagg        aggs[XP_MAX_PROJ];
+    /* CPU fallback: the attributes of the session's KDS_FORMAT_FALLBACK
+     * tuple (the scan slot); reg 0xff for those a depth-0 row leaves NULL */
+    uint32_t    fb_nattrs;          /* 0: the session has no CPU fallback */
+    uint32_t    __pad4;
+    xattr       fb[XP_MAX_PROJ];
     uint64_t    const_pool;         /* device address of text constants */
     xop         ops[XP_MAX_OPS];
 } xplan;

@@ -25,6 +25,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -241,6 +242,16 @@ struct xpuhip_session
     kern_data_store *d_kds_dst_head = nullptr;
     kern_data_store *d_kds_final = nullptr;
     uint64_t        final_length = 0;
+    /* group-by overflow table: the groups a launch finds no room for in
+     * kds_final, merged in after the launch once kds_final has grown */
+    kern_data_store *d_kds_ovf = nullptr;
+    uint64_t        ovf_length = 0;
+    /* CPU-fallback rows of the session's tasks (KDS_FORMAT_FALLBACK), until
+     * xpuhip_session_fallback takes them */
+    kern_data_store *d_kds_fallback = nullptr;
+    /* launches on kds_final hold it shared; growing kds_final holds it
+     * exclusively (the reference's m_kds_rwlock, gpu_service.c:2423) */
+    std::shared_mutex final_rw;     /* also held around the fallback buffer */
     char           *d_kmrels = nullptr;
     uint64_t       *d_bloom = nullptr;      /* Bloom filters of the inner relations */
     uint64_t        bloom_off[XP_MAX_DEPTH + 1] = {};   /* words from d_bloom */
@@ -539,6 +550,7 @@ message_of(const xpuhip_session *s, uint32_t msgid)
         case XMSG_HEAP_ITEM: return "heap page line pointer or row-index entry out of its chunk";
         case XMSG_HEAP_TUPLE: return "heap tuple attribute beyond its item";
         case XMSG_ARROW_TYPE: return "not a mappable Arrow data type";
+        case XMSG_FALLBACK_FULL: return "CPU fallback buffer is full";
     }
     if (msgid < s->comp.messages.size())
         return s->comp.messages[msgid].c_str();
@@ -703,6 +715,23 @@ xpuhip_session_open(const kern_session_info *session, size_t length,
         {
             xpuhip_session_close(s);
             return rc ? rc : ERRCODE_DEVICE_ERROR;
+        }
+    }
+    if (!s->comp.kds_fallback_head.empty())
+    {
+        /* the CPU-fallback buffer, as long as the session's header asks
+         * (__allocateGpuFallbackBuffer, gpu_service.c:2556-2585) */
+        std::vector<char> img = s->comp.kds_fallback_head;
+        kern_data_store *fh = (kern_data_store *)img.data();
+        fh->nitems = 0;
+        fh->usage = 0;
+        fh->length &= ~(uint64_t)(XPU_MAXIMUM_ALIGNOF - 1);
+        if (!hip_ok(hipMalloc((void **)&s->d_kds_fallback, fh->length), eb, "hipMalloc(kds_fallback)") ||
+            !hip_ok(hipMemcpy(s->d_kds_fallback, img.data(), img.size(), hipMemcpyHostToDevice), eb,
+                    "hipMemcpy"))
+        {
+            xpuhip_session_close(s);
+            return ERRCODE_DEVICE_ERROR;
         }
     }
     if (s->comp.pinned_hash)
@@ -1091,6 +1120,180 @@ jit_function(xpuhip_session *s, int kind, int param, kern_errorbuf *eb)
 
 }   /* namespace */
 
+/* ------------------------------------------------------------------
+ * Group-by kds_final growth (gpu_service.c:3868-3884 and
+ * __expandGpuQueryGroupByBuffer, :2418-2496).  The reference's kernel
+ * suspends when a new group finds no room, the service grows the buffer and
+ * the kernel resumes.  Here a launch never stops: a group that finds no room
+ * in kds_final goes to the session's overflow table (same layout), and after
+ * the launch kds_final grows as the reference grows it (by its length, at
+ * least 1 GiB, and at least by what the overflow table holds) and the
+ * overflow groups are merged into it.  The task then completes.
+ * ------------------------------------------------------------------ */
+#define XPU_OVF_MIN_BYTES       (256ULL << 20)
+#define XPU_OVF_ROWS_BASE       (64ULL << 20)
+#define XPU_OVF_BYTES_PER_ROW   256ULL
+
+/* the overflow table's header image: kds_final's with its own length and
+ * hash slots, empty */
+static std::vector<char>
+ovf_head(const xpuhip_session *s, uint64_t length, uint32_t nslots)
+{
+    std::vector<char> img(s->comp.kds_final_head.begin(), s->comp.kds_final_head.end());
+    kern_data_store *h = (kern_data_store *)img.data();
+    h->length = length;
+    h->usage = 0;
+    h->nitems = 0;
+    h->hash_nslots = nslots;
+    return img;
+}
+
+/* Sizes the overflow table for a launch of nrows rows: a quarter of
+ * kds_final's length, at least 256 MiB, and no more than 64 MiB plus 256 B
+ * per row.  One launch can place that much of new groups beyond kds_final's
+ * free space; kds_final then grows by at least 1 GiB before the next one.
+ * The table is empty between launches, so it is replaced under the
+ * exclusive lock when a longer one is due. */
+static int
+ensure_ovf(xpuhip_session *s, uint64_t nrows, kern_errorbuf *eb)
+{
+    uint64_t want = std::max<uint64_t>(s->final_length / 4, XPU_OVF_MIN_BYTES);
+    const uint64_t by_rows = XPU_OVF_ROWS_BASE + nrows * XPU_OVF_BYTES_PER_ROW;
+    if (want > by_rows)
+        want = by_rows;
+    want = (want + XPU_L1_CACHELINE_SZ - 1) & ~(uint64_t)(XPU_L1_CACHELINE_SZ - 1);
+    {
+        std::shared_lock<std::shared_mutex> g(s->final_rw);
+        if (s->d_kds_ovf && s->ovf_length >= want)
+            return 0;
+    }
+    std::unique_lock<std::shared_mutex> g(s->final_rw);
+    if (s->d_kds_ovf && s->ovf_length >= want)
+        return 0;
+    if (s->d_kds_ovf)
+    {
+        (void)hipFree(s->d_kds_ovf);
+        s->d_kds_ovf = nullptr;
+        s->ovf_length = 0;
+    }
+    /* chains of about four groups when the table is full */
+    const uint32_t nslots = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want / 512, 1024),
+                                                         UINT32_MAX / 2);
+    std::vector<char> img = ovf_head(s, want, nslots);
+    if (!hip_ok(hipMalloc((void **)&s->d_kds_ovf, want), eb, "hipMalloc(group-by overflow table)"))
+    {
+        s->d_kds_ovf = nullptr;
+        return ERRCODE_DEVICE_ERROR;
+    }
+    if (!hip_ok(hipMemcpy(s->d_kds_ovf, img.data(), img.size(), hipMemcpyHostToDevice), eb, "hipMemcpy") ||
+        xpu_launch_fill((char *)s->d_kds_ovf + img.size(), 0, nslots, nullptr) != 0 ||
+        !hip_ok(hipDeviceSynchronize(), eb, "overflow table init"))
+    {
+        (void)hipFree(s->d_kds_ovf);
+        s->d_kds_ovf = nullptr;
+        return ERRCODE_DEVICE_ERROR;
+    }
+    s->ovf_length = want;
+    return 0;
+}
+
+/* bytes of items a KDS_FORMAT_HASH (or FALLBACK) buffer really holds:
+ * kds->usage may run past them after a failed reservation, the row index
+ * (offsets of the items' tuples, item_head bytes into each item) does not */
+static bool
+kds_tuple_bytes(const kern_data_store *d_kds, const kern_data_store &h, uint64_t head_len,
+                uint64_t *bytes, kern_errorbuf *eb, uint64_t item_head = offsetof(kern_hashitem, t))
+{
+    *bytes = 0;
+    if (h.nitems == 0)
+        return true;
+    std::vector<uint64_t> ridx(h.nitems);
+    if (!hip_ok(hipMemcpy(ridx.data(), (const char *)d_kds + head_len + 8ULL * h.hash_nslots,
+                          8ULL * h.nitems, hipMemcpyDeviceToHost), eb, "hipMemcpy(row index)"))
+        return false;
+    uint64_t most = 0;
+    for (uint64_t r : ridx)
+        most = r > most ? r : most;
+    *bytes = most + item_head;
+    return true;
+}
+
+/* Grows kds_final and merges the overflow table into it (once, whichever
+ * task gets the exclusive lock first; the others find the table empty). */
+static int
+final_grow_merge(xpuhip_session *s, Scratch *x, kern_errorbuf *eb)
+{
+    std::unique_lock<std::shared_mutex> g(s->final_rw);
+    const uint64_t head_len = s->comp.kds_final_head.size();
+    kern_data_store fh, oh;
+    if (!hip_ok(hipMemcpy(&fh, s->d_kds_final, sizeof(fh), hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+        !hip_ok(hipMemcpy(&oh, s->d_kds_ovf, sizeof(oh), hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return ERRCODE_DEVICE_ERROR;
+    if (oh.nitems == 0)
+        return 0;
+    uint64_t uf = 0, uo = 0;
+    if (!kds_tuple_bytes(s->d_kds_final, fh, head_len, &uf, eb) ||
+        !kds_tuple_bytes(s->d_kds_ovf, oh, head_len, &uo, eb))
+        return ERRCODE_DEVICE_ERROR;
+    const uint64_t old_len = s->final_length;
+    uint64_t new_len = old_len + std::max<uint64_t>(old_len, 1ULL << 30);
+    const uint64_t fixed = head_len + 8ULL * fh.hash_nslots;
+    const uint64_t tuples = uf + uo;
+    const uint64_t need = fixed + tuples + 8 * (tuples / XPU_L1_CACHELINE_SZ) + 4096;
+    if (new_len < need)
+        new_len = need;
+    new_len = (new_len + XPU_L1_CACHELINE_SZ - 1) & ~(uint64_t)(XPU_L1_CACHELINE_SZ - 1);
+    kern_data_store *d_new = nullptr;
+    if (!hip_ok(hipMalloc((void **)&d_new, new_len), eb, "hipMalloc(kds_final)"))
+        return ERRCODE_DEVICE_ERROR;
+    /* head, slots and row index stay; the tuples keep their distance from
+     * the tail, so chains and row index stay valid */
+    fh.length = new_len;
+    fh.usage = uf;
+    int rc = 0;
+    if (!hip_ok(hipMemcpyAsync(d_new, s->d_kds_final, fixed + 8ULL * fh.nitems,
+                               hipMemcpyDeviceToDevice, x->stream), eb, "hipMemcpyAsync") ||
+        (uf > 0 && !hip_ok(hipMemcpyAsync((char *)d_new + new_len - uf,
+                                          (const char *)s->d_kds_final + old_len - uf, uf,
+                                          hipMemcpyDeviceToDevice, x->stream), eb, "hipMemcpyAsync")) ||
+        !hip_ok(hipMemcpyAsync(d_new, &fh, sizeof(fh), hipMemcpyHostToDevice, x->stream), eb,
+                "hipMemcpyAsync"))
+        rc = ERRCODE_DEVICE_ERROR;
+    if (rc == 0)
+    {
+        xpu_launch_fill(x->d_ctl, 0, sizeof(xpu_task_ctl) / 8, x->stream);
+        if (xpu_launch_final_merge_ovf(s->d_plan, s->comp.plan.nregs, d_new, s->d_kds_ovf,
+                                       oh.nitems, x->d_ctl, x->stream) != 0)
+            rc = ERRCODE_DEVICE_ERROR;
+    }
+    /* the overflow table, empty again */
+    std::vector<char> img = ovf_head(s, s->ovf_length, oh.hash_nslots);
+    if (rc == 0 &&
+        (!hip_ok(hipMemcpyAsync(s->d_kds_ovf, img.data(), img.size(), hipMemcpyHostToDevice,
+                                x->stream), eb, "hipMemcpyAsync") ||
+         xpu_launch_fill((char *)s->d_kds_ovf + img.size(), 0, oh.hash_nslots, x->stream) != 0 ||
+         !hip_ok(hipMemcpyAsync(x->h_ctl, x->d_ctl, sizeof(xpu_task_ctl), hipMemcpyDeviceToHost,
+                                x->stream), eb, "hipMemcpyAsync") ||
+         !hip_ok(hipStreamSynchronize(x->stream), eb, "kds_final merge")))
+        rc = ERRCODE_DEVICE_ERROR;
+    if (rc == 0 && x->h_ctl->errcode)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, message_of(s, x->h_ctl->errmsg));
+        rc = ERRCODE_DEVICE_ERROR;
+    }
+    if (rc != 0)
+    {
+        (void)hipStreamSynchronize(x->stream);
+        (void)hipFree(d_new);
+        return rc;
+    }
+    (void)hipFree(s->d_kds_final);
+    s->d_kds_final = d_new;
+    s->final_length = new_len;
+    t_last_stats.final_grows++;
+    return 0;
+}
+
 static int
 task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, uint32_t nitems,
                  kern_data_store *d_dst, size_t dst_length, uint64_t *resume_pos,
@@ -1135,6 +1338,16 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     uint32_t block = t.block_size ? t.block_size : 256;
     uint32_t rpt = preagg ? (t.rows_per_thread ? t.rows_per_thread : 1) : 1;
     uint64_t nrows = nitems - row_begin;
+
+    /* group-by: a launch holds kds_final shared (growing it takes it
+     * exclusively); the groups that find no room in it go to the overflow
+     * table */
+    const bool use_ovf = preagg && P->nkeys > 0;
+    if (use_ovf && ensure_ovf(s, nrows, eb) != 0)
+        return ERRCODE_DEVICE_ERROR;
+    std::shared_lock<std::shared_mutex> final_hold;
+    if (preagg || s->d_kds_fallback)
+        final_hold = std::shared_lock<std::shared_mutex>(s->final_rw);
 
     /* the kernels specialized for the plan, or the interpreter */
     const bool use_jit = t.jit == XPUHIP_JIT_ON ||
@@ -1206,6 +1419,8 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     kp.params = s->d_params;
     kp.param_nulls = s->comp.param_nulls;
     kp.block_status = sproj ? x->d_block_status : nullptr;
+    kp.kds_ovf = use_ovf ? s->d_kds_ovf : nullptr;
+    kp.kds_fallback = s->d_kds_fallback;
     for (uint32_t d = 1; d <= P->nrels && d <= XP_MAX_DEPTH; d++)
         kp.inner_kds[d] = P->depth[d].inner_kds;
     if (t.join_filter != XPUHIP_JOIN_FILTER_OFF && s->d_bloom)
@@ -1421,6 +1636,7 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     t_last_stats.spill_rows = c->spill_rows;
     t_last_stats.jit = jf ? 1 : 0;
     t_last_stats.join_slices = nslices;
+    t_last_stats.final_grows = 0;
     t_last_stats.heap_ms = 0.0;
     t_last_stats.heap_rows = 0;
     if (res)
@@ -1438,9 +1654,18 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     }
     if (c->errcode == ERRCODE_SUSPEND_NO_SPACE)
     {
-        /* kds_final is full: not resumable in this build */
+        /* neither kds_final nor the overflow table had room for a group:
+         * its contribution is lost */
         XERR(eb, ERRCODE_DEVICE_ERROR, message_of(s, c->errmsg));
         return ERRCODE_DEVICE_ERROR;
+    }
+    if (preagg && c->final_ovf)
+    {
+        /* kds_final grows and takes the overflow groups: the task completes */
+        final_hold.unlock();
+        int grc = final_grow_merge(s, x, eb);
+        if (grc != 0)
+            return grc;
     }
     if (!preagg && c->resume_tile != 0)
     {
@@ -1884,6 +2109,9 @@ xpuhip_task_exec_device_noalloc(xpuhip_session *s, const kern_data_store *d_src,
         return ERRCODE_DEVICE_ERROR;
     }
     std::lock_guard<std::mutex> g(g_noalloc_lock);
+    /* kds_final cannot grow here (that allocates): a group-by whose groups
+     * do not fit fails with 'E' */
+    std::shared_lock<std::shared_mutex> final_hold(s->final_rw);
     xpu_task_ctl *d_ctl = (xpu_task_ctl *)base;
     char *area = base + ctl_bytes;
     if (nrows == 0)
@@ -2324,6 +2552,63 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
     return (int64_t)need;
 }
 
+/* The CPU-fallback rows the session's tasks have written since the last
+ * call, as the KDS_FORMAT_FALLBACK chunk gpuClientWriteBackPartial sends
+ * the backend (gpu_service.c:2853-2889): header, row index, then the
+ * kern_fallbackitems at the tail; the device buffer is emptied.  Returns the
+ * bytes written, 0 when there are none, with out == NULL the bytes needed,
+ * or -1 with eb filled. */
+extern "C" int64_t
+xpuhip_session_fallback(xpuhip_session *s, kern_data_store *out, size_t length, kern_errorbuf *eb)
+{
+    if (!s)
+    {
+        XERR(eb, ERRCODE_DEVICE_ERROR, "no session");
+        return -1;
+    }
+    if (!s->d_kds_fallback)
+        return 0;
+    if (ensure_device(eb) != 0)
+        return -1;
+    std::unique_lock<std::shared_mutex> g(s->final_rw);
+    const size_t head_len = s->comp.kds_fallback_head.size();
+    std::vector<char> hbuf(head_len);
+    if (!hip_ok(hipMemcpy(hbuf.data(), s->d_kds_fallback, head_len, hipMemcpyDeviceToHost), eb,
+                "hipMemcpy"))
+        return -1;
+    kern_data_store *h = (kern_data_store *)hbuf.data();
+    if (h->nitems == 0)
+        return 0;
+    uint64_t usage = 0;
+    if (!kds_tuple_bytes(s->d_kds_fallback, *h, head_len, &usage, eb, 16))
+        return -1;
+    const uint64_t dev_len = h->length;
+    const uint64_t need = head_len + 8ULL * h->nitems + usage;
+    if (!out)
+        return (int64_t)need;
+    if (length < need)
+    {
+        XERR(eb, ERRCODE_SUSPEND_NO_SPACE, "fallback buffer of the caller is too small");
+        return -1;
+    }
+    h->length = need;
+    h->usage = usage;
+    memcpy(out, hbuf.data(), head_len);
+    if (!hip_ok(hipMemcpy((char *)out + head_len, (const char *)s->d_kds_fallback + head_len,
+                          8ULL * h->nitems, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
+        !hip_ok(hipMemcpy((char *)out + need - usage, (const char *)s->d_kds_fallback + dev_len - usage,
+                          usage, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        return -1;
+    /* the device buffer, empty again (gpuClientWriteBackPartial's reset) */
+    h->length = dev_len;
+    h->nitems = 0;
+    h->usage = 0;
+    if (!hip_ok(hipMemcpy(s->d_kds_fallback, hbuf.data(), head_len, hipMemcpyHostToDevice), eb,
+                "hipMemcpy"))
+        return -1;
+    return (int64_t)need;
+}
+
 extern "C" void
 xpuhip_session_close(xpuhip_session *s)
 {
@@ -2351,6 +2636,8 @@ xpuhip_session_close(xpuhip_session *s)
     }
     else if (s->d_kds_final)
         (void)hipFree(s->d_kds_final);
+    if (s->d_kds_ovf) (void)hipFree(s->d_kds_ovf);
+    if (s->d_kds_fallback) (void)hipFree(s->d_kds_fallback);
     if (s->d_kmrels) (void)hipFree(s->d_kmrels);
     if (s->d_bloom) (void)hipFree(s->d_bloom);
     if (s->d_params) (void)hipFree(s->d_params);
@@ -2466,6 +2753,7 @@ xpuhip_session_reset(xpuhip_session *s, kern_errorbuf *eb)
     }
     if (!s->comp.has_groupby && !s->comp.pinned_hash)
         return 0;
+    std::unique_lock<std::shared_mutex> g(s->final_rw);
     return init_kds_final(s, eb);
 }
 

@@ -50,6 +50,7 @@ ArrowPrecision__Double = 2
 
 HEAP_HASNULL = 0x0001
 HEAP_HASVARWIDTH = 0x0002
+HEAP_HASEXTERNAL = 0x0004
 HEAP_NATTS_MASK = 0x07FF
 MINIMAL_TUPLE_OFFSET = 8
 ROWID_SIZE = 4

@@ -570,7 +570,8 @@ class Plan:
 
     def build(self, kds_dst_length=0, final_length=1 << 30, final_nrows=10000, xids=None,
               pinned_hash=None, pinned_nslots=137, pinned_length=512 << 20, sort_keys=None,
-              sort_limit=0, window_rank=None, encoding="UTF8"):
+              sort_limit=0, window_rank=None, encoding="UTF8", fallback=False,
+              fallback_length=64 << 20):
         """The session blob.  xids: parallelCurrentXids of the transaction
         (SerializedTransactionState at session_xact_state, xpu_common.h:3156-3165),
         which decide the visibility of KDS_FORMAT_ROW sources.  pinned_hash:
@@ -582,7 +583,12 @@ class Plan:
         [(KSORT_KEY_KIND__*, attribute number in kds_final, typname of a VREF
         key or None, asc, nulls_first)], with sort_limit rows kept
         (gpusort_limit_count); window_rank: (func, limit, partby, orderby)
-        of a window rank function."""
+        of a window rank function.  fallback: the session carries the CPU
+        fallback (pg_strom.cpu_fallback on, executor.c:705-724): a
+        KDS_FORMAT_FALLBACK header over the scan slot (the table's columns,
+        then each depth's inner columns) of fallback_length bytes, and one
+        kern_fallback_desc per loaded variable mapping it into that slot
+        (__execInitTaskStateCpuFallback, executor.c:1280-1380)."""
         nrels = len(self.inner_coldefs)
         codes = {}
         # scan quals
@@ -751,6 +757,28 @@ class Plan:
             sess.gpusort_limit_count = int(sort_limit)
             finalized = A_KSORT_PAVG_KINDS
             sess.gpusort_htup_margin = 9 * sum(1 for k in sort_keys if k[0] in finalized)
+        if fallback:
+            scan_slot = list(self.scan_coldefs)
+            base = {0: 0}
+            for d in sorted(self.inner_coldefs):
+                base[d] = len(scan_slot)
+                scan_slot += list(self.inner_coldefs[d])
+            head = kdsmod.kds_header(abi.KDS_FORMAT_FALLBACK, scan_slot, fallback_length)
+            sess.fallback_kds_head = _append_aligned(buf, bytes(head))
+            descs = bytearray()
+            nfb = 0
+            for d in sorted(self.loads):
+                if d < 0 or d not in base:
+                    continue
+                for resno, slot in self.loads[d]:
+                    # kern_fallback_desc: src depth / resno, dst resno, max
+                    # depth, slot id, kvec offset (none: no kvec buffers)
+                    descs += struct.pack("<hhhhHxxi", d, resno, base[d] + resno, nrels + 1,
+                                         slot, -1)
+                    nfb += 1
+            if nfb:
+                sess.fallback_desc_defs = _append_aligned(buf, bytes(descs))
+            sess.fallback_desc_nitems = nfb
         if encoding:
             # xpu_encode_info (__build_session_encode, executor.c:494-506):
             # the database encoding's name and longest character

@@ -53,6 +53,13 @@ class ArrowVarlena:
         return len(self.offsets) - 1
 
     def __getitem__(self, i):
+        if isinstance(i, slice):
+            start, stop, step = i.indices(len(self))
+            if step != 1:
+                raise ValueError("ArrowVarlena slices take step 1")
+            stop = max(stop, start)
+            offs = self.offsets[start:stop + 1]
+            return ArrowVarlena(offs - offs[0], self.data[int(offs[0]):int(offs[-1])])
         return self.data[self.offsets[i]:self.offsets[i + 1]]
 
     def __iter__(self):
@@ -255,6 +262,8 @@ _FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
 
 
 def encode_datum(typname, value):
+    if isinstance(value, (ExternalDatum, CompressedDatum)):
+        return value.encode()
     if typname in ("text", "bytea", "bpchar"):
         v = bytes(value)
         return struct.pack("<I", (len(v) + 4) << 2) + v
@@ -282,6 +291,8 @@ def form_tuple_payload(coldefs, values):
         d = encode_datum(t, v)
         body += d
         if PG_TYPES[t][1] == -1:
+            if isinstance(v, ExternalDatum):
+                infomask |= abi.HEAP_HASEXTERNAL
             infomask |= abi.HEAP_HASVARWIDTH
         bits[j >> 3] |= 1 << (j & 7)
         pos = nxt + len(d)
@@ -453,6 +464,25 @@ def parse_row_kds(buf):
     return out
 
 
+def parse_fallback_kds(buf):
+    """The items of a KDS_FORMAT_FALLBACK buffer, in row-index order:
+    (depth, l_state, matched, tuple bytes without the row id, row id)
+    (kern_fallbackitem, xpu_common.h:1296-1305; KDS_GET_FALLBACK_ITEM)."""
+    raw = memoryview(bytes(buf))
+    hd, cms = kds_colmeta(raw)
+    assert hd.format == abi.KDS_FORMAT_FALLBACK
+    head_sz = abi.kds_head_length(hd.nr_colmeta)
+    ridx = np.frombuffer(raw, dtype=np.uint64, count=hd.nitems, offset=head_sz)
+    out = []
+    for i in range(hd.nitems):
+        t = hd.length - int(ridx[i])
+        l_state, depth, matched = struct.unpack_from("<QHB", raw, t - 16)
+        t_len = struct.unpack_from("<I", raw, t)[0]
+        rowid = struct.unpack_from("<I", raw, t + t_len - 4)[0]
+        out.append((depth, l_state, matched, bytes(raw[t:t + t_len - 4]), rowid))
+    return out
+
+
 def window_headers(base, windows, base_offset, stride):
     """KDS_FORMAT_ARROW headers of row windows over one columnar chunk.
 
@@ -524,12 +554,28 @@ class ExternalDatum:
         return bytes([0x01, self.tag]) + bytes(range(1, body + 1))
 
 
+class CompressedDatum:
+    """An inline compressed datum in a text/bytea column: a 4-byte header
+    with the compressed bit (VARATT_IS_4B_C), va_tcinfo (raw size, method in
+    the top two bits), then the compressed bytes."""
+
+    def __init__(self, rawsize, payload=b"\x00pglz"):
+        self.rawsize = rawsize
+        self.payload = bytes(payload)
+
+    def encode(self):
+        total = 8 + len(self.payload)
+        return struct.pack("<II", (total << 2) | 0x02, self.rawsize & 0x3fffffff) + self.payload
+
+
 def _varlena(v):
     """PostgreSQL's on-page varlena form: a 1-byte header for data of up to
     126 bytes (VARATT_CAN_MAKE_SHORT), else a 4-byte header.  Returns
     (bytes, needs_alignment)."""
     if isinstance(v, ExternalDatum):
         return v.encode(), False
+    if isinstance(v, CompressedDatum):
+        return v.encode(), True
     v = bytes(v)
     if len(v) + 1 <= 0x7f:
         return bytes([((len(v) + 1) << 1) | 0x01]) + v, False

@@ -16,7 +16,8 @@ EXPORTS = [
     "xpuhip_init", "xpuhip_shutdown", "xpuhip_session_open", "xpuhip_session_check",
     "xpuhip_session_attach_inner", "xpuhip_task_exec", "xpuhip_task_exec_device",
     "xpuhip_task_exec_device_tuples", "xpuhip_task_exec_device_noalloc",
-    "xpuhip_task_final", "xpuhip_session_close", "xpuhip_session_set_tuning",
+    "xpuhip_task_final", "xpuhip_session_fallback", "xpuhip_session_close",
+    "xpuhip_session_set_tuning",
     "xpuhip_session_get_tuning", "xpuhip_device_alloc", "xpuhip_device_free",
     "xpuhip_memcpy_htod", "xpuhip_memcpy_dtoh", "xpuhip_memcpy_dtod", "xpuhip_device_synchronize",
     "xpuhip_last_kernel_ms", "xpuhip_abi_layout_json", "xpuhip_session_explain",
@@ -75,7 +76,7 @@ class xpuhip_task_stats(C.Structure):
                 ("spill_rows", C.c_uint64), ("jit", C.c_uint32), ("_pad", C.c_uint32),
                 ("heap_ms", C.c_double), ("heap_rows", C.c_uint64),
                 ("sort_ms", C.c_double), ("sort_items", C.c_uint32), ("sort_passes", C.c_uint32),
-                ("join_slices", C.c_uint32), ("_pad2", C.c_uint32)]
+                ("join_slices", C.c_uint32), ("final_grows", C.c_uint32)]
 
 
 _lib = None
@@ -101,6 +102,8 @@ def lib():
                                                       C.POINTER(C.c_uint64), vp, vp]
         L.xpuhip_task_final.restype = C.c_int64
         L.xpuhip_task_final.argtypes = [vp, vp, sz, vp, vp]
+        L.xpuhip_session_fallback.restype = C.c_int64
+        L.xpuhip_session_fallback.argtypes = [vp, vp, sz, vp]
         L.xpuhip_session_close.argtypes = [vp]
         L.xpuhip_session_set_tuning.argtypes = [vp, vp]
         L.xpuhip_session_get_tuning.argtypes = [vp, vp]
@@ -304,6 +307,20 @@ class Session:
         if got < 0:
             raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
         return buf[:got]        # a LIMIT packs the kept tuples into fewer bytes
+
+    def fallback(self):
+        """The CPU-fallback rows the session's tasks wrote since the last
+        call (KDS_FORMAT_FALLBACK bytes), or None."""
+        need = lib().xpuhip_session_fallback(self.h, None, 0, C.byref(self.err))
+        if need < 0:
+            raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
+        if need == 0:
+            return None
+        buf = np.zeros(need, dtype=np.uint8)
+        got = lib().xpuhip_session_fallback(self.h, buf.ctypes.data, need, C.byref(self.err))
+        if got < 0:
+            raise XpuError(abi.ERRCODE_DEVICE_ERROR, self.err)
+        return buf[:got]
 
     def close(self):
         if getattr(self, "h", None):

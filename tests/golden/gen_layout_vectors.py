@@ -4,6 +4,9 @@ reference's own code (oracle/_ref/libpgstrom_ref.so, `make -C oracle ref`):
   tuples   kern_form_minimal_tuple (xpu_common.cu:1096-1209) over rows of
            mixed types with NULL patterns and texts of several lengths, into
            a KDS_FORMAT_ROW kds_dst of those columns: the kern_tupitem bytes
+  tuples_external  the same over rows whose texts are TOAST pointers or
+           inline compressed datums (read by xpu_text_datum_heap_read as
+           length -1, written as they are)
   hash     a KDS_FORMAT_HASH inner buffer built by this repository's builder
            (pgstrom.kds.build_hash_kds, keys with duplicates), walked with the
            reference's KDS_HASH_FIRST_ITEM / KDS_HASH_NEXT_ITEM
@@ -42,8 +45,11 @@ def main():
     head = bytes(kds.kds_header(abi.KDS_FORMAT_ROW, LC.TUPLE_COLDEFS, 1 << 20))
     hbuf = C.create_string_buffer(head, len(head))
     types = (C.c_int32 * len(LC.TUPLE_COLDEFS))(*[opcodes.TYPES[t] for _, t in LC.TUPLE_COLDEFS])
-    tuples = []
-    for rowid, row in enumerate(LC.tuple_rows()):
+
+    def form(rows):
+        return [form_one(rowid, row) for rowid, row in enumerate(rows)]
+
+    def form_one(rowid, row):
         data = bytearray()
         offs = []
         for (_, t), v in zip(LC.TUPLE_COLDEFS, row):
@@ -59,7 +65,10 @@ def main():
         out = C.create_string_buffer(4096)
         n = L.ref_form_tuple(hbuf, len(row), types, dbuf, ob, rowid, out, 4096)
         assert n > 0
-        tuples.append(out.raw[:n].hex())
+        return out.raw[:n].hex()
+
+    tuples = form(LC.tuple_rows())
+    tuples_ext = form(LC.tuple_rows_external())
 
     coldefs, rows, hashes = LC.hash_inner()
     buf = kds.build_hash_kds(coldefs, rows, hashes)
@@ -75,7 +84,7 @@ def main():
         walk.append(list(rowids[pos:pos + c]))
         pos += c
     with open(os.path.join(HERE, "layouts.json"), "w") as f:
-        json.dump({"tuples": tuples,
+        json.dump({"tuples": tuples, "tuples_external": tuples_ext,
                    "hash": {"sha256": hashlib.sha256(bytes(buf)).hexdigest(),
                             "probe_keys": list(range(48)), "walk": walk}}, f, indent=0)
         f.write("\n")

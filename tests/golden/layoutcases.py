@@ -3,6 +3,7 @@ the tests that replay it)."""
 import numpy as np
 
 from pgstrom import hashing
+from pgstrom.kds import CompressedDatum, ExternalDatum
 
 TUPLE_COLDEFS = [("a", "int2"), ("b", "int4"), ("c", "text"), ("d", "float8"), ("e", "bool"),
                  ("f", "int8"), ("g", "text"), ("h", "float4")]
@@ -28,6 +29,27 @@ def tuple_rows():
             if r.random() < 0.3:
                 row[j] = None
         rows.append(row)
+    return rows
+
+
+def tuple_rows_external():
+    """Rows of TUPLE_COLDEFS whose texts include TOAST pointers (on-disk and
+    indirect tags) and inline compressed datums, which a tuple carries as
+    they are (xpu_text_datum_write with length -1)."""
+    base = [-7, 123456, b"abc", -2.5, True, -(2 ** 40), b"", 0.75]
+    kinds = [ExternalDatum(18), ExternalDatum(1), CompressedDatum(5000),
+             CompressedDatum(70000, b"\x01" * 37)]
+    rows = []
+    for i, x in enumerate(kinds):
+        for col in (2, 6):
+            row = list(base)
+            row[col] = x
+            if i % 2:
+                row[0] = None
+            rows.append(row)
+    row = list(base)
+    row[2], row[6] = kinds[0], kinds[2]
+    rows.append(row)
     return rows
 
 

@@ -128,25 +128,31 @@ def test_block_device_entry(gpu):
     assert kds.parse_row_kds(dev) == kds.parse_row_kds(host)
 
 
-def test_block_external_datum(gpu):
-    """A TOAST pointer in a text column: loading that column fails the task
-    (the oracle fails it too); the columns after it load as the reference
-    loads them."""
-    import json
+def test_block_external_datum(gpu, jit):
+    """A TOAST pointer in a text column: the columns after it load as the
+    reference loads them, and a projection of the column copies the pointer
+    as it is (kds_dst bytes equal the oracle's); an operator that needs its
+    bytes fails the task with the reference's fallback error when the
+    session has no CPU fallback."""
     import os
     import sys
     here = os.path.dirname(os.path.abspath(__file__))
     sys.path.insert(0, os.path.join(here, "golden"))
     import gen_heap_vectors as G
     cols, src = G.external_chunk()
-    blob = project_all_query(cols, names=["a", "c", "d"])
-    got = run_gpu_projection(blob, src, 8 << 20)
-    exp, _ = run_oracle_projection(blob, src, 8 << 20)
-    assert np.array_equal(got[0], exp)
-    s = xpuhip.Session(project_all_query(cols))
+    for names in (["a", "c", "d"], None):
+        blob = project_all_query(cols, names=names)
+        got = run_gpu_projection(blob, src, 8 << 20, tuning=dict(jit=jit))
+        exp, _ = run_oracle_projection(blob, src, 8 << 20)
+        assert len(got) == 1 and np.array_equal(got[0], exp)
+    blob = project_all_query(cols, quals=lambda plan, v: cg.Func("text_ne", v["g"],
+                                                                 cg.Const("text", b"q")))
+    s = xpuhip.Session(blob)
+    s.set_tuning(jit=jit)
     with pytest.raises(xpuhip.XpuError) as ei:
         s.exec(src, dst_length=8 << 20)
     s.close()
+    assert ei.value.code == abi.ERRCODE_SUSPEND_FALLBACK
     assert "compressed or external" in str(ei.value)
 
 

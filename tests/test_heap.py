@@ -64,8 +64,8 @@ def test_oracle_deform_matches_reference_mixed():
 
 def test_oracle_walks_past_external_datums():
     """Attributes after a TOAST pointer load as the reference loads them; the
-    pointer itself is not loadable (the reference keeps it packed, length -1,
-    and its functions fall back to the CPU)."""
+    pointer itself stays packed (length -1, xpu_text_datum_heap_read) and a
+    projection copies it as it is (xpu_text_datum_write)."""
     cols, buf = G.external_chunk()
     assert hashlib.sha256(bytes(buf)).hexdigest() == GOLDEN["external"]["sha256"]
     rc, dst, _ = oracle_rows(project_all_query(cols, names=["a", "c", "d"]), buf)
@@ -74,8 +74,13 @@ def test_oracle_walks_past_external_datums():
     exp = [[r[0], r[2], r[3]] for r in GOLDEN["external"]["rows"]]
     assert got == exp
     assert any(r[1] == "external" for r in GOLDEN["external"]["rows"])
-    rc, _, _ = oracle_rows(project_all_query(cols), buf)
-    assert rc != 0
+    rc, dst, _ = oracle_rows(project_all_query(cols), buf)
+    assert rc == 0
+    raw = bytes(dst)
+    g = cols[1]
+    for i, v in enumerate(g.values):
+        if isinstance(v, kds.ExternalDatum) and not g.nulls[i]:
+            assert v.encode() in raw
 
 
 @pytest.mark.parametrize("case", ["empty", "one", "plain", "holes", "sparse"])

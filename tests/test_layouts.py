@@ -22,8 +22,8 @@ with open(os.path.join(HERE, "golden", "layouts.json")) as _f:
     GOLDEN = json.load(_f)
 
 
-def tuple_chunk():
-    rows = LC.tuple_rows()
+def tuple_chunk(rows=None):
+    rows = LC.tuple_rows() if rows is None else rows
     cols = []
     for j, (name, t) in enumerate(LC.TUPLE_COLDEFS):
         vals = [r[j] for r in rows]
@@ -47,10 +47,28 @@ def dst_tuples(dst):
     return out
 
 
+def external_tuple_chunk():
+    """Heap pages (KDS_FORMAT_BLOCK) of the rows whose texts are TOAST
+    pointers or compressed datums, which only a heap page can hold."""
+    cols = tuple_chunk(LC.tuple_rows_external())
+    return cols, kds.build_block_kds(cols, seed=8)
+
+
+def test_oracle_projection_external_tuples_match_reference():
+    """A compressed or external text is copied into the projected tuple as it
+    is, and a TOAST pointer sets HEAP_HASEXTERNAL (xpu_text_datum_write,
+    xpu_textlib.cu:312-335; kern_form_minimal_tuple)."""
+    cols, src = external_tuple_chunk()
+    dst, res = run_oracle_projection(project_all_query(cols), src, 1 << 20)
+    assert [t.hex() for t in dst_tuples(dst)] == GOLDEN["tuples_external"]
+
+
 def test_builder_tuples_match_reference():
     """The host builder of ROW / HASH inputs (kds.form_tupitem) writes the
-    reference's bytes."""
+    reference's bytes, TOAST pointers and compressed datums included."""
     for rowid, (row, exp) in enumerate(zip(LC.tuple_rows(), GOLDEN["tuples"])):
+        assert kds.form_tupitem(LC.TUPLE_COLDEFS, row, rowid).hex() == exp, rowid
+    for rowid, (row, exp) in enumerate(zip(LC.tuple_rows_external(), GOLDEN["tuples_external"])):
         assert kds.form_tupitem(LC.TUPLE_COLDEFS, row, rowid).hex() == exp, rowid
 
 
