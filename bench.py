@@ -165,9 +165,9 @@ def bench_star(n, warmup, steps):
     from pgstrom import datagen, kds, workloads as W, xpuhip
     log("generating %d-row star-join fact table and dimensions" % n)
     fact, dims = datagen.star_tables(n)
-    t0 = time.perf_counter()
-    km = W.star_inner(dims)
-    host_build_s = time.perf_counter() - t0
+    tm = {}
+    km = W.star_inner(dims, tm)
+    host_build_s = tm["hash_build_s"]
     src = kds.build_arrow_kds(fact)
     blob = W.star_query(fact)
     d_src = xpuhip.DeviceBuffer(len(src), src)

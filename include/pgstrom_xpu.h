@@ -92,6 +92,7 @@ typedef struct
 #define ArrowType__Bool             6
 #define ArrowType__Decimal          7
 #define ArrowType__Date             8
+#define ArrowType__Decimal          7
 #define ArrowType__Time             9
 #define ArrowType__Timestamp        10
 #define ArrowType__FixedSizeBinary  15

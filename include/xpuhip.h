@@ -97,6 +97,23 @@ int64_t xpuhip_session_explain(const kern_session_info *session, size_t length,
 int     xpuhip_session_attach_inner(xpuhip_session *sess, const kern_multirels *kmrels,
                                     size_t length, kern_errorbuf *errbuf);
 
+/* Host-side build of one inner relation's KDS_FORMAT_HASH buffer, as the
+ * backend's inner preload does it: hashes every MinimalTuple's join keys
+ * (get_tuple_hashvalue, gpu_join.c:1548-1584: the device types' hash of
+ * each key attribute, by its colmeta atttypid, merged by pg_hash_merge) and
+ * packs the rows as kern_hashitems from the buffer's tail, chained from the
+ * hash slots (__innerPreloadSetupHashBuffer, gpu_join.c:2128-2164).
+ * kds: header and colmeta set up by the caller, length the buffer size,
+ * hash_nslots > 0 (KDS_GET_HASHSLOT_WIDTH), nitems / usage what earlier
+ * calls appended (0 at first).  mtups: nitems MinimalTuples (t_len without
+ * a row id).  key_attnums: 1-based attribute numbers of the join keys.
+ * nthreads <= 0 uses every core.  Needs no GPU.  Returns 0, or -1 with
+ * errbuf set (buffer too small, unsupported key type, key that is still
+ * compressed or external). */
+int     xpuhip_inner_hash_setup(kern_data_store *kds, const void *const *mtups, uint32_t nitems,
+                                const int16_t *key_attnums, uint32_t nkeys, int nthreads,
+                                kern_errorbuf *errbuf);
+
 /* XpuTaskExec on a host-resident chunk (KDS_FORMAT_ARROW, or a heap-page /
  * row chunk as xpuhip_task_exec_device_tuples describes): copies it to the
  * device, runs the pipeline, and for a scan/join projection copies the rows

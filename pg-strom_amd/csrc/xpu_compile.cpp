@@ -19,7 +19,8 @@
 
 namespace {
 
-enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr };
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr,
+       K_numcmp, K_numarith, K_numunary, K_numcast };
 enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
        O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len,
        O_like, O_nlike, O_ilike, O_nilike, O_substr };
@@ -36,6 +37,7 @@ struct devfunc { int opcode; const char *name; int kind, op, nargs, t0, t1, t2, 
 #define T_float8    TypeOpCode__float8
 #define T_text      TypeOpCode__text
 #define T_bpchar    TypeOpCode__bpchar
+#define T_numeric   TypeOpCode__numeric
 #define DEVFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, A2, RES) \
     { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##A2, T_##RES },
 const devfunc devfunc_table[] = { XPU_FUNCTION_TABLE(DEVFUNC) };

@@ -55,6 +55,14 @@ typedef const XCONST kern_colmeta ccolmeta;
  * ------------------------------------------------------------------ */
 #define RNULL(L, r)         (((L).nmask >> (r)) & 1U)
 #define SETNULL(L, r, n)    ((L).nmask = ((L).nmask & ~(1U << (r))) | ((uint32_t)(n) << (r)))
+/* length word of a text pair holding a compressed or external datum */
+#define XTEXT_RAW           (-1)
+/* kinds of a numeric register triple (the reference's XPU_NUMERIC_KIND__*) */
+#define XNUM_VALID          0
+#define XNUM_NAN            1
+#define XNUM_PINF           2
+#define XNUM_NINF           3
+#define XNUM_VARLENA        0xff
 
 /* The register file is a separate local array (a homogeneous alloca that
  * the backend keeps in VGPRs, indexed with uniform register numbers); a
@@ -280,6 +288,16 @@ ldtup(Lane &L, int d, const kern_tupitem *t, const kern_data_store *kds, int res
                 case TypeOpCode__float4: v = (int64_t)*(const uint32_t *)p; break;
                 case TypeOpCode__int8:
                 case TypeOpCode__float8: v = *(const int64_t *)p; break;
+                case TypeOpCode__numeric:
+                {
+                    /* kept in its varlena form, addressed by its header */
+                    const uint8_t b0 = (uint8_t)p[0];
+                    L.R[d] = (int64_t)(uintptr_t)p;
+                    L.R[d + 1] = (b0 == 0x01 || (b0 & 0x03) == 0x02) ? XTEXT_RAW : 0;
+                    L.R[d + 2] = XNUM_VARLENA;
+                    SETNULL(L, d, 0);
+                    return;
+                }
                 default:
                 {
                     uint8_t b0 = (uint8_t)p[0];
@@ -628,7 +646,6 @@ nan_as_x86_f32(float x, float y, float v)
  * reference's xpu_text_t with length -1).  Projection copies it unchanged;
  * an op that needs its bytes raises "text datum is compressed or external",
  * which the session's CPU fallback takes (xpu_textlib.h:52-58). */
-#define XTEXT_RAW           (-1)
 #define TXT_RAW(r)          (L.R[(r) + 1] < 0)
 
 /* VARSIZE_ANY of a varlena header (1B_E: the TOAST pointer's size by tag) */
@@ -642,6 +659,532 @@ varsize_any(const uint8_t *p)
         return b0 >> 1;
     return (((uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) |
              ((uint32_t)p[3] << 24)) >> 2) & 0x3fffffff;
+}
+
+/* ------------------------------------------------------------------
+ * numeric (xpu_numeric.h, xpu_numeric.cu).  A numeric value occupies a
+ * register triple:
+ *   R[r], R[r + 1]  a finite value's 128-bit integer (low, high word), or
+ *                   for a datum still in its varlena form the address of
+ *                   its header in R[r] and R[r + 1] < 0 when that datum is
+ *                   compressed or external;
+ *   R[r + 2]        the kind (low byte, XNUM_*) and, for a finite value,
+ *                   its 16-bit decimal scale in bits 16..31: the value is
+ *                   R[r] * 10^-scale (the reference's "weight").
+ * Datums are parsed where an operator needs their value, as the reference
+ * validates its xpu_numeric_t; one that is only moved keeps its varlena
+ * form and is written back byte for byte (xpu_numeric_datum_write).
+ * ------------------------------------------------------------------ */
+typedef __int128 xi128;
+typedef unsigned __int128 xu128;
+
+struct xnum
+{
+    xi128   v;
+    int     kind;
+    int16_t scale;
+};
+
+/* NUMERIC_NORMALIZE (xpu_numeric.h:165-207): trailing zeros leave the
+ * value while the scale stays non-negative; zero has scale 0 */
+__device__ __forceinline__ void
+num_normalize(xnum &x)
+{
+    if (x.v == 0)
+    {
+        x.scale = 0;
+        return;
+    }
+    while (x.scale > 0 && x.v % 10 == 0)
+    {
+        x.v /= 10;
+        x.scale = (int16_t)(x.scale - 1);
+    }
+}
+
+__device__ __forceinline__ xi128 num_mul10(xi128 v) { return (xi128)((xu128)v * 10); }
+
+/* __decimal_from_varlena (xpu_numeric.h:210-273); returns a message id + 1
+ * on error.  A compressed or external datum goes to the CPU fallback. */
+__device__ uint32_t
+num_parse(const uint8_t *hdr, xnum *x)
+{
+    int64_t len;
+    if (hdr[0] == 0x01 || (hdr[0] & 0x03) == 0x02)
+        return XMSG_TEXT_EXTERNAL + 1;
+    const uint8_t *p = varlena_payload(hdr, &len);
+    if (len < 2)
+        return XMSG_NUM_HEADER + 1;
+    const uint16_t n_head = (uint16_t)(p[0] | (p[1] << 8));
+    x->v = 0;
+    x->scale = 0;
+    if ((n_head & 0xC000) == 0xC000)
+    {
+        if (n_head == 0xC000)
+            x->kind = XNUM_NAN;
+        else if (n_head == 0xD000)
+            x->kind = XNUM_PINF;
+        else if (n_head == 0xF000)
+            x->kind = XNUM_NINF;
+        else
+            return XMSG_NUM_SPECIAL + 1;
+        return 0;
+    }
+    const bool is_short = (n_head & 0xC000) == 0x8000;
+    const int hsz = is_short ? 2 : 4;
+    if (len < hsz)
+        return XMSG_NUM_HEADER + 1;
+    int weight;
+    bool neg;
+    if (is_short)
+    {
+        weight = n_head & 0x003F;
+        if (n_head & 0x0040)
+            weight |= ~0x003F;
+        neg = (n_head & 0x2000) != 0;
+    }
+    else
+    {
+        weight = (int16_t)(p[2] | (p[3] << 8));
+        neg = (n_head & 0xC000) == 0x4000;
+    }
+    const int ndigits = (int)((len - hsz) / 2);
+    xi128 v = 0;
+    for (int i = 0; i < ndigits; i++)
+    {
+        if ((v >> 114) != 0)
+            return XMSG_NUM_RANGE + 1;
+        const int16_t dig = (int16_t)(p[hsz + 2 * i] | (p[hsz + 2 * i + 1] << 8));
+        v = v * 10000 + dig;
+    }
+    if (neg)
+        v = -v;
+    x->kind = XNUM_VALID;
+    x->v = v;
+    x->scale = (int16_t)(4 * (ndigits - (int16_t)(weight + 1)));
+    num_normalize(*x);
+    return 0;
+}
+
+/* the value of numeric register triple r; a message id + 1 on error */
+__device__ __forceinline__ uint32_t
+num_read(const Lane &L, int r, xnum *x)
+{
+    const int kind = (int)(L.R[r + 2] & 0xff);
+    if (kind == XNUM_VARLENA)
+    {
+        if (L.R[r + 1] < 0)
+            return XMSG_TEXT_EXTERNAL + 1;
+        return num_parse((const uint8_t *)(uintptr_t)L.R[r], x);
+    }
+    x->kind = kind;
+    x->scale = (int16_t)(L.R[r + 2] >> 16);
+    x->v = (xi128)(((xu128)(uint64_t)L.R[r + 1] << 64) | (uint64_t)L.R[r]);
+    return 0;
+}
+
+__device__ __forceinline__ void
+num_store(Lane &L, int d, const xnum &x)
+{
+    const xi128 v = x.kind == XNUM_VALID ? x.v : 0;
+    L.R[d] = (int64_t)(uint64_t)(xu128)v;
+    L.R[d + 1] = (int64_t)(uint64_t)((xu128)v >> 64);
+    L.R[d + 2] = (int64_t)((uint32_t)x.kind | ((uint32_t)(uint16_t)(x.kind == XNUM_VALID ? x.scale : 0) << 16));
+}
+
+/* a numeric register triple holding the varlena at hdr (raw: compressed or
+ * external) */
+__device__ __forceinline__ void
+num_store_varlena(Lane &L, int d, const uint8_t *hdr, bool raw)
+{
+    L.R[d] = (int64_t)(uintptr_t)hdr;
+    L.R[d + 1] = raw ? XTEXT_RAW : 0;
+    L.R[d + 2] = XNUM_VARLENA;
+}
+
+/* align two finite values to the larger scale (the loops of
+ * __decimal_add / __numeric_compare) */
+__device__ __forceinline__ void
+num_align(xnum &a, xnum &b)
+{
+    while (a.scale > b.scale)
+    {
+        b.v = num_mul10(b.v);
+        b.scale = (int16_t)(b.scale + 1);
+    }
+    while (a.scale < b.scale)
+    {
+        a.v = num_mul10(a.v);
+        a.scale = (int16_t)(a.scale + 1);
+    }
+}
+
+/* __numeric_compare (xpu_numeric.cu:478-539) */
+__device__ int
+num_compare(xnum a, xnum b)
+{
+    if (a.kind != XNUM_VALID)
+    {
+        if (a.kind == XNUM_NAN)
+            return b.kind == XNUM_NAN ? 0 : 1;
+        if (a.kind == XNUM_PINF)
+            return b.kind == XNUM_NAN ? -1 : (b.kind == XNUM_PINF ? 0 : 1);
+        return b.kind == XNUM_NINF ? 0 : -1;
+    }
+    if (b.kind != XNUM_VALID)
+        return b.kind == XNUM_NINF ? 1 : -1;
+    if ((a.v > 0 && b.v <= 0) || (a.v == 0 && b.v < 0))
+        return 1;
+    if ((b.v > 0 && a.v <= 0) || (b.v == 0 && a.v < 0))
+        return -1;
+    num_align(a, b);
+    return a.v > b.v ? 1 : (a.v < b.v ? -1 : 0);
+}
+
+__device__ __forceinline__ bool num_is_neg(const xnum &x) { return x.kind == XNUM_VALID && x.v < 0; }
+__device__ __forceinline__ bool num_is_pos(const xnum &x) { return x.kind == XNUM_VALID && x.v > 0; }
+
+/* __decimal_add / __decimal_sub / __decimal_mul / __decimal_div
+ * (xpu_numeric.h:443-776).  As there, add, sub and mul take the finite
+ * path when either operand is finite (a NaN or infinity then counts as 0),
+ * and numeric_mod runs the division (xpu_numeric.cu:681-702). */
+__device__ uint32_t
+num_arith(int op, xnum a, xnum b, xnum *r)
+{
+    r->scale = 0;
+    r->v = 0;
+    if (op == XO_NUM_ADD || op == XO_NUM_SUB)
+    {
+        if (a.kind == XNUM_VALID || b.kind == XNUM_VALID)
+        {
+            num_align(a, b);
+            r->kind = XNUM_VALID;
+            r->scale = a.scale;
+            r->v = (xi128)(op == XO_NUM_ADD ? (xu128)a.v + (xu128)b.v : (xu128)a.v - (xu128)b.v);
+            num_normalize(*r);
+            return 0;
+        }
+        const bool sub = (op == XO_NUM_SUB);
+        if (a.kind == XNUM_NAN || b.kind == XNUM_NAN)
+            r->kind = XNUM_NAN;
+        else if (a.kind == XNUM_PINF)
+            r->kind = (b.kind == (sub ? XNUM_PINF : XNUM_NINF)) ? XNUM_NAN : XNUM_PINF;
+        else if (a.kind == XNUM_NINF)
+            r->kind = (b.kind == (sub ? XNUM_NINF : XNUM_PINF)) ? XNUM_NAN : XNUM_NINF;
+        else if (b.kind == XNUM_PINF)
+            r->kind = sub ? XNUM_NINF : XNUM_PINF;
+        else
+            r->kind = sub ? XNUM_PINF : XNUM_NINF;
+        return 0;
+    }
+    if (op == XO_NUM_MUL)
+    {
+        if (a.kind == XNUM_VALID || b.kind == XNUM_VALID)
+        {
+            r->kind = XNUM_VALID;
+            r->scale = (int16_t)(a.scale + b.scale);
+            r->v = (xi128)((xu128)a.v * (xu128)b.v);
+            num_normalize(*r);
+            return 0;
+        }
+        const xnum &x = (a.kind == XNUM_PINF || a.kind == XNUM_NINF) && b.kind != XNUM_NAN ? b : a;
+        const bool a_inf = (a.kind == XNUM_PINF || a.kind == XNUM_NINF) && b.kind != XNUM_NAN;
+        const bool b_inf = !a_inf && a.kind != XNUM_NAN && (b.kind == XNUM_PINF || b.kind == XNUM_NINF);
+        if (a_inf || b_inf)
+        {
+            const int inf = a_inf ? a.kind : b.kind;
+            const xnum &o = a_inf ? b : a;
+            (void)x;
+            const bool oneg = o.kind == XNUM_NINF || num_is_neg(o);
+            const bool opos = o.kind == XNUM_PINF || num_is_pos(o);
+            if (oneg)
+                r->kind = (inf == XNUM_PINF) ? XNUM_NINF : XNUM_PINF;
+            else if (opos)
+                r->kind = inf;
+            else
+                r->kind = XNUM_NAN;
+        }
+        else
+            r->kind = XNUM_NAN;
+        return 0;
+    }
+    /* division, and numeric_mod */
+    if (a.kind == XNUM_VALID && b.kind == XNUM_VALID)
+    {
+        xi128 rem = a.v, div = b.v, ival = 0;
+        int16_t weight = (int16_t)(a.scale - b.scale);
+        bool negative = false;
+        if (div == 0)
+            return XMSG_NUM_DIV0 + 1;
+        if (rem < 0)
+        {
+            rem = -rem;
+            if (div < 0)
+                div = -div;
+            else
+                negative = true;
+        }
+        else if (div < 0)
+        {
+            negative = true;
+            div = -div;
+        }
+        for (;;)
+        {
+            const xi128 q = rem / div;
+            ival = (xi128)((xu128)ival * 10 + (xu128)q);
+            rem -= q * div;
+            if (rem == 0 || (ival >> 50) != 0)
+                break;
+            rem = num_mul10(rem);
+            weight = (int16_t)(weight + 1);
+        }
+        if (negative)
+            ival = -ival;
+        r->kind = XNUM_VALID;
+        r->scale = weight;
+        r->v = ival;
+        num_normalize(*r);
+        return 0;
+    }
+    if (a.kind == XNUM_NAN || b.kind == XNUM_NAN)
+        r->kind = XNUM_NAN;
+    else if (a.kind == XNUM_PINF || a.kind == XNUM_NINF)
+    {
+        const bool apos = (a.kind == XNUM_PINF);
+        if (b.kind == XNUM_PINF)
+            r->kind = apos ? XNUM_PINF : XNUM_NINF;
+        else if (b.kind == XNUM_NINF)
+            r->kind = apos ? XNUM_NINF : XNUM_PINF;
+        else if (b.v > 0)
+            r->kind = apos ? XNUM_PINF : XNUM_NINF;
+        else if (b.v < 0)
+            r->kind = apos ? XNUM_NINF : XNUM_PINF;
+        else
+            return XMSG_NUM_DIV0 + 1;
+    }
+    else
+        r->kind = XNUM_VALID;   /* finite / infinite: 0 */
+    return 0;
+}
+
+/* __xpu_numeric_to_int64 (xpu_numeric.cu:250-295): rounded at the first
+ * fractional digit, range-checked against [lo, hi] */
+__device__ uint32_t
+num_to_int(const xnum &x, int64_t lo, int64_t hi, int64_t *out)
+{
+    if (x.kind != XNUM_VALID)
+        return XMSG_NUM_NANINF + 1;
+    xi128 ival = x.v;
+    int weight = x.scale;
+    if (ival != 0)
+    {
+        while (weight > 0)
+        {
+            if (weight == 1)
+                ival += (ival > 0 ? 5 : -5);
+            ival /= 10;
+            weight--;
+        }
+        while (weight < 0)
+        {
+            ival = num_mul10(ival);
+            weight++;
+            if (ival < lo || ival > hi)
+                break;
+        }
+        if (ival < lo || ival > hi)
+            return XMSG_NUM_INT_RANGE + 1;
+    }
+    *out = (int64_t)ival;
+    return 0;
+}
+
+/* __xpu_numeric_to_fp64 (xpu_numeric.cu:341-380) */
+__device__ uint32_t
+num_to_f64(const xnum &x, double *out)
+{
+    if (x.kind == XNUM_VALID)
+    {
+        double f = (double)x.v;
+        int weight = x.scale;
+        if (f != 0.0)
+        {
+            for (; weight > 0; weight--)
+                f /= 10.0;
+            for (; weight < 0; weight++)
+                f *= 10.0;
+            if (__builtin_isnan(f) || __builtin_isinf(f))
+                return XMSG_NUM_FLOAT_RANGE + 1;
+        }
+        *out = f;
+    }
+    else if (x.kind == XNUM_PINF)
+        *out = __builtin_inf();
+    else if (x.kind == XNUM_NINF)
+        *out = -__builtin_inf();
+    else
+        *out = __builtin_nan("");
+    return 0;
+}
+
+/* __decimal_from_float8 (xpu_numeric.h:778-846): the 52-bit fraction
+ * rescaled by powers of 2 and 10 to the binary exponent of 1, then cut to
+ * 15 decimal digits.  As there, any float whose exponent bits are all ones
+ * becomes NaN, and zero takes the path of the normal numbers. */
+__device__ xnum
+num_from_f64(double dv)
+{
+    const uint64_t bits = (uint64_t)__double_as_longlong(dv);
+    uint64_t frac = bits & ((1ULL << 52) - 1);
+    int32_t expo = (int32_t)((bits >> 52) & 0x7ff);
+    const bool sign = (bits >> 63) != 0;
+    int weight = 0;
+    xnum x;
+    x.v = 0;
+    x.scale = 0;
+    if (expo == 0x7ff)
+    {
+        x.kind = XNUM_NAN;
+        return x;
+    }
+    frac |= (1ULL << 52);
+    while (expo > 1023 + 52)
+    {
+        if (frac <= 0x1800000000000000ULL)
+        {
+            frac *= 2;
+            expo--;
+        }
+        else
+        {
+            frac /= 10;
+            weight--;
+        }
+    }
+    while (expo < 1023 + 52)
+    {
+        if (frac >= 0x1800000000000000ULL)
+        {
+            frac /= 2;
+            expo++;
+        }
+        else
+        {
+            frac *= 10;
+            weight++;
+        }
+    }
+    while (frac >= 1000000000000000ULL)
+    {
+        frac /= 10;
+        weight--;
+    }
+    x.kind = XNUM_VALID;
+    x.scale = (int16_t)weight;
+    x.v = sign ? -(xi128)frac : (xi128)frac;
+    return x;
+}
+
+/* xpu_numeric_datum_hash (xpu_numeric.cu:180-196) */
+__device__ __forceinline__ uint32_t
+num_hash(const xnum &x)
+{
+    if (x.kind != XNUM_VALID)
+    {
+        const uint8_t k = (uint8_t)x.kind;
+        return hash_bytes(&k, 1);
+    }
+    uint8_t w[2], v[16];
+    __builtin_memcpy(w, &x.scale, 2);
+    __builtin_memcpy(v, &x.v, 16);
+    return hash_bytes(w, 2) ^ hash_bytes(v, 16);
+}
+
+/* __decimal_to_varlena (xpu_numeric.h:275-372): the long form with a 4-byte
+ * header; the last base-10000 digit is split off so the scale becomes a
+ * multiple of 4, which is also the display scale written.  Returns the
+ * size; writes only if buf.  NaN and the infinities are the 6-byte
+ * PostgreSQL encodings (NUMERIC_NAN / PINF / NINF). */
+__device__ uint32_t
+num_to_varlena(const xnum &x, uint8_t *buf)
+{
+    if (x.kind != XNUM_VALID)
+    {
+        if (buf)
+        {
+            const uint16_t h = x.kind == XNUM_PINF ? 0xD000 : (x.kind == XNUM_NINF ? 0xF000 : 0xC000);
+            const uint32_t vl = 6u << 2;
+            __builtin_memcpy(buf, &vl, 4);
+            __builtin_memcpy(buf + 4, &h, 2);
+        }
+        return 6;
+    }
+    xnum n = x;
+    const bool negative = n.v < 0;
+    if (negative)
+        n.v = -n.v;
+    num_normalize(n);
+    int weight = n.scale;
+    xi128 value = n.v;
+    uint16_t digs[12];
+    int ndigits = 0;
+    if (value != 0)
+    {
+        int mod = -1;
+        switch (weight % 4)
+        {
+            case -1: case 3:
+                mod = (int)(value % 1000) * 10; value /= 1000; weight += 1; break;
+            case -2: case 2:
+                mod = (int)(value % 100) * 100; value /= 100; weight += 2; break;
+            case -3: case 1:
+                mod = (int)(value % 10) * 1000; value /= 10; weight += 3; break;
+        }
+        if (mod >= 0)
+            digs[ndigits++] = (uint16_t)mod;
+    }
+    else
+        weight = 0;
+    while (value != 0 && ndigits < 12)
+    {
+        digs[ndigits++] = (uint16_t)(value % 10000);
+        value /= 10000;
+    }
+    const int nzero = weight < 0 ? -weight / 4 : 0;
+    const uint32_t len = 8 + 2 * (uint32_t)(ndigits + nzero);
+    if (buf)
+    {
+        const uint32_t vl = len << 2;
+        __builtin_memcpy(buf, &vl, 4);
+        uint16_t sign_dscale = (uint16_t)(weight > 0 ? weight : 0);
+        if (negative)
+            sign_dscale |= 0x4000;
+        const int16_t n_weight = (int16_t)(ndigits - (weight / 4) - 1);
+        __builtin_memcpy(buf + 4, &sign_dscale, 2);
+        __builtin_memcpy(buf + 6, &n_weight, 2);
+        /* digits most significant first: digs[] holds them least first */
+        for (int i = 0; i < ndigits; i++)
+            __builtin_memcpy(buf + 8 + 2 * i, &digs[ndigits - 1 - i], 2);
+        for (int i = 0; i < nzero; i++)
+        {
+            buf[8 + 2 * (ndigits + i)] = 0;
+            buf[9 + 2 * (ndigits + i)] = 0;
+        }
+    }
+    return len;
+}
+
+/* __normalize_numeric_int128 (xpu_numeric.cu:1080-1126): a finite value
+ * rescaled to the scale of a numeric sum (truncated when digits go) */
+__device__ __forceinline__ xi128
+num_rescale(xi128 v, int scale_d, int scale_s)
+{
+    for (; scale_d > scale_s; scale_s++)
+        v = num_mul10(v);
+    for (; scale_d < scale_s; scale_s--)
+        v /= 10;
+    return v;
 }
 
 /* STAGED: the caller has loaded the columns of the plan's stage[] list into
@@ -1261,6 +1804,8 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                     L.R[d] = L.R[a];
                     if (b)
                         L.R[d + 1] = L.R[a + 1];
+                    if (b > 1)
+                        L.R[d + 2] = L.R[a + 2];
                     SETNULL(L, d, RNULL(L, a));
                 }
                 break;
@@ -1301,7 +1846,18 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                         RAISE(g, XMSG_TEXT_EXTERNAL);
                         break;
                     }
-                    if (o.aux == 0)
+                    if (o.aux == XHASH_NUMERIC)
+                    {
+                        xnum x;
+                        const uint32_t e = num_read(L, a, &x);
+                        if (e)
+                        {
+                            RAISE(g, e - 1);
+                            break;
+                        }
+                        h = num_hash(x);
+                    }
+                    else if (o.aux == 0)
                         h = hash_bytes((const uint8_t *)(uintptr_t)L.R[a], (uint32_t)L.R[a + 1]);
                     else
                         h = hash_fixed((uint64_t)L.R[a], o.aux);
@@ -1326,8 +1882,10 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 if (L.R[g] != 0)
                 {
                     L.R[d] = L.R[a];
-                    if (o.aux & 1)
+                    if (o.aux & 3)
                         L.R[d + 1] = L.R[a + 1];
+                    if (o.aux & 2)
+                        L.R[d + 2] = L.R[a + 2];
                     SETNULL(L, d, RNULL(L, a));
                 }
                 break;
@@ -1345,11 +1903,38 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                  * the type's comparison; floats by < and > only */
                 if (RNULL(L, a))
                     break;
+                if ((o.aux & XMM_DOMAIN) == XD_NUM)
+                {
+                    /* the comparison parses both datums in place, so the
+                     * kept value is in its parsed form */
+                    if (RNULL(L, d))
+                    {
+                        L.R[d] = L.R[a];
+                        L.R[d + 1] = L.R[a + 1];
+                        L.R[d + 2] = L.R[a + 2];
+                        SETNULL(L, d, 0);
+                        break;
+                    }
+                    xnum x, y;
+                    uint32_t e = num_read(L, d, &x);
+                    if (!e)
+                        e = num_read(L, a, &y);
+                    if (e)
+                    {
+                        RAISE(g, e - 1);
+                        break;
+                    }
+                    const int c = num_compare(x, y);
+                    int32_t tk = ((o.aux & XMM_GREATEST) ? (c < 0) : (c > 0)) ? 1 : 0;
+                    asm volatile("" : "+v"(tk));
+                    num_store(L, d, tk ? y : x);
+                    break;
+                }
                 bool take = RNULL(L, d);
                 if (!take)
                 {
                     int c;
-                    const int dom = o.aux & 3;
+                    const int dom = o.aux & XMM_DOMAIN;
                     if (dom == XD_INT)
                         c = L.R[d] > L.R[a] ? 1 : (L.R[d] < L.R[a] ? -1 : 0);
                     else if (dom == XD_F32)
@@ -1389,7 +1974,7 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 asm volatile("" : "+v"(tk));
                 const int64_t w0 = tk ? L.R[a] : L.R[d];
                 L.R[d] = w0;
-                if ((o.aux & 3) == XD_TEXT)
+                if ((o.aux & XMM_DOMAIN) == XD_TEXT)
                 {
                     const int64_t w1 = tk ? L.R[a + 1] : L.R[d + 1];
                     L.R[d + 1] = w1;
@@ -1499,6 +2084,250 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 if (!RNULL(L, a) && TXT_RAW(a))
                     RAISE(g, XMSG_TEXT_EXTERNAL);
                 break;
+            case XO_NUM_VL:
+            {
+                /* a varlena addressed by its payload (heap tiles, tuples): a
+                 * 1-byte header is odd, the top byte of a 4-byte header of
+                 * a datum below 4 MB even; a raw datum is addressed by its
+                 * header already */
+                const uint8_t *pl = (const uint8_t *)(uintptr_t)L.R[a];
+                const bool n = RNULL(L, a);
+                if (n)
+                    num_store_varlena(L, d, nullptr, false);
+                else if (L.R[a + 1] < 0)
+                    num_store_varlena(L, d, pl, true);
+                else
+                    num_store_varlena(L, d, (pl[-1] & 1) ? pl - 1 : pl - 4, false);
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_CONST_NUM:
+                num_store_varlena(L, d, (const uint8_t *)const_pool + o.imm, false);
+                SETNULL(L, d, 0);
+                break;
+            case XO_LDCOL_DEC:
+            {
+                /* xpu_numeric_datum_arrow_read (xpu_numeric.cu:46-79):
+                 * Decimal128 at the column's scale, normalized; heap tiles
+                 * keep the varlena */
+                ccolmeta *cm = &kmeta->colmeta[o.aux];
+                bool isnull = false;
+                if (cm->nullmap_offset)
+                {
+                    const uint32_t idx = row >> 3;
+                    isnull = (idx >= cm->nullmap_length) ||
+                             !((((const uint8_t *)kbase)[cm->nullmap_offset + idx] >> (row & 7)) & 1);
+                }
+                if (cm->attopts.tag == XARROW_TAG_HEAPREF)
+                {
+                    if (8ULL * ((uint64_t)row + 1) > cm->values_length)
+                        isnull = true;
+                    const uint8_t *vp = nullptr;
+                    bool raw = false;
+                    if (!isnull)
+                    {
+                        const uint2 pr = ((const uint2 *)(kbase + cm->values_offset))[row];
+                        vp = (const uint8_t *)kbase + cm->extra_offset + pr.x;
+                        raw = (pr.y == XHEAP_LEN_EXTERNAL);
+                        if (raw ? ((uint64_t)pr.x + 2 > cm->extra_length ||
+                                   (uint64_t)pr.x + varsize_any(vp) > cm->extra_length)
+                                : ((uint64_t)pr.x + pr.y > cm->extra_length || pr.x < 1))
+                            isnull = true;
+                        else if (!raw)
+                            vp = (vp[-1] & 1) ? vp - 1 : vp - 4;
+                    }
+                    num_store_varlena(L, d, isnull ? nullptr : vp, raw);
+                    SETNULL(L, d, isnull);
+                    break;
+                }
+                if (cm->attopts.tag != ArrowType__Decimal || cm->attopts.u.decimal.bitWidth != 128)
+                {
+                    if (!L.err)
+                        L.err = XMSG_ARROW_TYPE + 1;
+                    isnull = true;
+                }
+                if (16ULL * ((uint64_t)row + 1) > cm->values_length)
+                    isnull = true;
+                xnum x;
+                x.kind = XNUM_VALID;
+                x.v = 0;
+                x.scale = (int16_t)cm->attopts.u.decimal.scale;
+                if (!isnull)
+                {
+                    const uint64_t *q = (const uint64_t *)(kbase + cm->values_offset + 16ULL * row);
+                    x.v = (xi128)(((xu128)q[1] << 64) | q[0]);
+                    num_normalize(x);
+                }
+                num_store(L, d, x);
+                SETNULL(L, d, isnull);
+                break;
+            }
+            case XO_MOV3:
+                L.R[d] = L.R[a];
+                L.R[d + 1] = L.R[a + 1];
+                L.R[d + 2] = L.R[a + 2];
+                SETNULL(L, d, RNULL(L, a));
+                break;
+            case XO_NUM_CMP:
+            {
+                const int op = o.aux & 0xff;
+                const bool na = RNULL(L, a), nb = RNULL(L, b);
+                if (na || nb)
+                {
+                    if ((o.aux & 0x100) && (op == XC_EQ || op == XC_NE))
+                    {
+                        const bool both = na && nb;
+                        L.R[d] = (op == XC_EQ) ? both : !both;
+                        SETNULL(L, d, 0);
+                    }
+                    else
+                    {
+                        L.R[d] = 0;
+                        SETNULL(L, d, 1);
+                    }
+                    break;
+                }
+                xnum x, y;
+                uint32_t e = num_read(L, a, &x);
+                if (!e)
+                    e = num_read(L, b, &y);
+                if (e)
+                {
+                    RAISE(g, e - 1);
+                    L.R[d] = 0;
+                    SETNULL(L, d, 1);
+                    break;
+                }
+                const int c = num_compare(x, y);
+                bool v;
+                switch (op)
+                {
+                    case XC_EQ: v = (c == 0); break;
+                    case XC_NE: v = (c != 0); break;
+                    case XC_LT: v = (c < 0); break;
+                    case XC_LE: v = (c <= 0); break;
+                    case XC_GT: v = (c > 0); break;
+                    default: v = (c >= 0); break;
+                }
+                L.R[d] = v;
+                SETNULL(L, d, 0);
+                break;
+            }
+            case XO_NUM_ADD: case XO_NUM_SUB: case XO_NUM_MUL: case XO_NUM_DIV:
+            {
+                const bool n = RNULL(L, a) || RNULL(L, b);
+                xnum r;
+                r.kind = XNUM_VALID;
+                r.v = 0;
+                r.scale = 0;
+                if (!n)
+                {
+                    xnum x, y;
+                    uint32_t e = num_read(L, a, &x);
+                    if (!e)
+                        e = num_read(L, b, &y);
+                    if (!e)
+                        e = num_arith(o.op, x, y, &r);
+                    if (e)
+                        RAISE(g, e - 1);
+                }
+                num_store(L, d, r);
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_NUM_NEG: case XO_NUM_ABS:
+            {
+                /* pgfn_numeric_uminus / pgfn_numeric_abs (xpu_numeric.cu:
+                 * 715-757); NaN stays NaN */
+                const bool n = RNULL(L, a);
+                xnum x;
+                x.kind = XNUM_VALID;
+                x.v = 0;
+                x.scale = 0;
+                if (!n)
+                {
+                    const uint32_t e = num_read(L, a, &x);
+                    if (e)
+                        RAISE(g, e - 1);
+                    else if (o.op == XO_NUM_NEG)
+                    {
+                        if (x.kind == XNUM_VALID)
+                            x.v = -x.v;
+                        else if (x.kind == XNUM_PINF)
+                            x.kind = XNUM_NINF;
+                        else if (x.kind == XNUM_NINF)
+                            x.kind = XNUM_PINF;
+                    }
+                    else
+                    {
+                        if (x.kind == XNUM_VALID && x.v < 0)
+                            x.v = -x.v;
+                        else if (x.kind == XNUM_NINF)
+                            x.kind = XNUM_PINF;
+                    }
+                }
+                num_store(L, d, x);
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_NUM_INT:
+            {
+                const bool n = RNULL(L, a);
+                int64_t v = 0;
+                if (!n)
+                {
+                    xnum x;
+                    uint32_t e = num_read(L, a, &x);
+                    if (!e)
+                        e = num_to_int(x, int_class_min(o.aux),
+                                       o.aux == XT_INT8 ? INT64_MAX : -(int_class_min(o.aux) + 1), &v);
+                    if (e)
+                    {
+                        RAISE(g, e - 1);
+                        v = 0;
+                    }
+                }
+                L.R[d] = v;
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_NUM_F64: case XO_NUM_F32:
+            {
+                const bool n = RNULL(L, a);
+                double f = 0.0;
+                if (!n)
+                {
+                    xnum x;
+                    uint32_t e = num_read(L, a, &x);
+                    if (!e)
+                        e = num_to_f64(x, &f);
+                    if (e)
+                    {
+                        RAISE(g, e - 1);
+                        f = 0.0;
+                    }
+                }
+                L.R[d] = (o.op == XO_NUM_F64) ? f64_bits(f) : f32_bits((float)f);
+                SETNULL(L, d, n);
+                break;
+            }
+            case XO_INT_NUM:
+            {
+                xnum x;
+                x.kind = XNUM_VALID;
+                x.v = (xi128)L.R[a];
+                x.scale = 0;
+                num_store(L, d, x);
+                SETNULL(L, d, RNULL(L, a));
+                break;
+            }
+            case XO_F64_NUM: case XO_F32_NUM:
+            {
+                const double f = (o.op == XO_F64_NUM) ? as_f64(L.R[a]) : (double)as_f32(L.R[a]);
+                num_store(L, d, num_from_f64(f));
+                SETNULL(L, d, RNULL(L, a));
+                break;
+            }
             default:
                 break;
         }

@@ -68,6 +68,14 @@ typedef struct
 #define XMSG_HEAP_TUPLE         0xfff7
 #define XMSG_ARROW_TYPE         0xfff8
 #define XMSG_FALLBACK_FULL      0xfff9
+/* numeric errors, with the reference's texts (xpu_numeric.h/.cu) */
+#define XMSG_NUM_RANGE          0xffe0  /* numeric value is out of range */
+#define XMSG_NUM_HEADER         0xffe1  /* corrupted numeric header */
+#define XMSG_NUM_SPECIAL        0xffe2  /* unknown special numeric value */
+#define XMSG_NUM_DIV0           0xffe3  /* division by zero */
+#define XMSG_NUM_INT_RANGE      0xffe4  /* integer out of range */
+#define XMSG_NUM_NANINF         0xffe5  /* cannot convert NaN/Inf to integer */
+#define XMSG_NUM_FLOAT_RANGE    0xffe6  /* float out of range */
 /* the longest varlena an Arrow column may hold (VARATT_MAX, xpu_common.h:1575) */
 #define XARROW_VARATT_MAX       0x4ffffff8U
 

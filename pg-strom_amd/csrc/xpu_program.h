@@ -44,6 +44,7 @@ enum {
     XD_F32 = 1,     /* float4 bits in the low word */
     XD_F64 = 2,     /* float8 bits */
     XD_TEXT = 3,    /* pointer in R[r], length in R[r+1] */
+    XD_NUM = 4,     /* numeric: R[r], R[r+1], R[r+2] (value or varlena, kind | scale) */
 };
 
 /* comparison operators (aux low byte of XO_CMP*) */
@@ -56,6 +57,7 @@ enum { XC_EQ = 0, XC_NE, XC_LT, XC_LE, XC_GT, XC_GE };
 #define XLIKE_ICASE         0x04    /* ILIKE: ASCII letters compare upper-cased */
 #define XLIKE_NEGATE        0x08    /* NOT LIKE */
 #define XSUB_HAS_COUNT      0x100
+#define XMM_DOMAIN          0x07    /* XO_MINMAX: XD_* of the values */
 #define XMM_GREATEST        0x10    /* XO_MINMAX: keep the greatest */
 #define XMM_BPCHAR          0x20
 /* XO_HAOP aux: key kind and width, result kind and width, flags */
@@ -66,6 +68,8 @@ enum { XC_EQ = 0, XC_NE, XC_LT, XC_LE, XC_GT, XC_GE };
 #define XHA_KEY(kind, width)    ((kind) | ((width) << 3))
 #define XHA_RES(kind, width)    (((kind) | ((width) << 3)) << 7)
 #define XHA_NULL_MISS       0x4000  /* no match gives NULL (IN list with a NULL item) */
+
+#define XHASH_NUMERIC       0x100   /* XO_HASH_ADD of a numeric triple */
 
 /* integer result classes for range checks */
 enum { XT_BOOL = 0, XT_INT1, XT_INT2, XT_INT4, XT_INT8 };
@@ -115,7 +119,7 @@ enum xop_code {
     /* join-qual status: if R[b] && (R[a] NULL or false): dst <- imm */
     XO_JQ_STATUS,
     /* hashing: dst <- 0xffffffff; dst <- merge(dst, hash(R[a])) unless NULL;
-       aux = hash width in bytes (0 = text); dst <- dst ^ 0xffffffff */
+       aux = hash width in bytes (0 = text, XHASH_NUMERIC); dst <- dst ^ 0xffffffff */
     XO_HASH_INIT, XO_HASH_ADD, XO_HASH_FIN,
     /* end of the scan quals when the final stage follows in the same pass:
        a lane whose R[a] is not true raises no further errors */
@@ -149,6 +153,20 @@ enum xop_code {
        is compressed or external" where R[guard]: a grouping key needs its
        bytes */
     XO_TXTCHK,
+    /* numeric (register triples, XD_NUM); ops that parse a datum raise its
+       error where R[guard] */
+    XO_NUM_VL,          /* dst triple <- the varlena of text-form pair R[a],R[a+1]
+                           (payload address and length, or raw) */
+    XO_CONST_NUM,       /* dst triple <- varlena at const pool offset imm */
+    XO_LDCOL_DEC,       /* dst triple <- Arrow Decimal128 column aux at its scale */
+    XO_MOV3,            /* dst,dst+1,dst+2 <- R[a],R[a+1],R[a+2] with null flag */
+    XO_NUM_CMP,         /* dst(bool) <- R[a] op R[b]; aux as XO_CMP_* */
+    XO_NUM_ADD, XO_NUM_SUB, XO_NUM_MUL, XO_NUM_DIV,     /* dst triple; MOD is DIV */
+    XO_NUM_NEG, XO_NUM_ABS,
+    XO_NUM_INT,         /* dst <- integer of R[a]; aux = integer class */
+    XO_NUM_F64, XO_NUM_F32,
+    XO_INT_NUM,         /* dst triple <- integer R[a] */
+    XO_F64_NUM, XO_F32_NUM,
     XO_NUM_OPS
 };
 

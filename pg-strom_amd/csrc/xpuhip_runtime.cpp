@@ -551,6 +551,13 @@ message_of(const xpuhip_session *s, uint32_t msgid)
         case XMSG_HEAP_TUPLE: return "heap tuple attribute beyond its item";
         case XMSG_ARROW_TYPE: return "not a mappable Arrow data type";
         case XMSG_FALLBACK_FULL: return "CPU fallback buffer is full";
+        case XMSG_NUM_RANGE: return "numeric value is out of range";
+        case XMSG_NUM_HEADER: return "corrupted numeric header";
+        case XMSG_NUM_SPECIAL: return "unknown special numeric value";
+        case XMSG_NUM_DIV0: return "division by zero";
+        case XMSG_NUM_INT_RANGE: return "integer out of range";
+        case XMSG_NUM_NANINF: return "cannot convert NaN/Inf to integer";
+        case XMSG_NUM_FLOAT_RANGE: return "float out of range";
     }
     if (msgid < s->comp.messages.size())
         return s->comp.messages[msgid].c_str();

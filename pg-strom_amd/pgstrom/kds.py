@@ -313,6 +313,14 @@ def form_tupitem(coldefs, values, rowid, hash_value=0):
     return bytes(buf)
 
 
+def form_minimal_tuple(coldefs, values):
+    """A MinimalTuple of PostgreSQL (heap_form_minimal_tuple): the
+    kern_tupitem image without its row id, t_len counting the tuple only."""
+    t = bytearray(form_tupitem(coldefs, values, 0)[:-abi.ROWID_SIZE])
+    struct.pack_into("<I", t, 0, len(t))
+    return bytes(t)
+
+
 def build_hash_kds(coldefs, rows, hashes):
     """Inner relation as KDS_FORMAT_HASH, laid out like
     __innerPreloadSetupHashBuffer (gpu_join.c:2128-2164): items are packed

@@ -9,6 +9,8 @@ through exactly the XpuCommand payloads of the reference.
                   GROUP BY (dim1.category, dim3.category), sum(quantity*price)
   fragment_query  WHERE a < K GROUP BY key % m with the pre-agg aggregates
 """
+import time
+
 import numpy as np
 
 from . import abi, codegen as cg, datagen, hashing, kds
@@ -75,19 +77,27 @@ def dim_filter_mod(dim_index):
     return 10, dim_index
 
 
-def star_inner(dims):
+def star_inner(dims, timings=None):
     """KDS_FORMAT_HASH inner buffers of the three dimensions after their
     filters (pk % 10 = i), keyed by pk, and the kern_multirels holding them,
-    built as the backend's inner preload does (gpu_join.c:2098-2164)."""
+    built as the backend's inner preload does (gpu_join.c:2098-2164): the
+    filtered rows as MinimalTuples, hashed and laid out by the library's
+    native xpuhip_inner_hash_setup (its seconds go to timings["hash_build_s"])."""
+    from . import xpuhip
     inners = []
     coldefs = [("pk", "int4"), ("category", "int2"), ("txt", "text")]
+    spent = 0.0
     for i, dim in enumerate(dims):
         pk, cat, txt = dim
         mod, rem = dim_filter_mod(i)
         sel = np.nonzero(pk.values % mod == rem)[0]
-        hashes = hashing.hash_keys_np([(pk.values[sel], 4)])
-        rows = [(int(pk.values[j]), int(cat.values[j]), bytes(txt.values[j])) for j in sel]
-        inners.append((kds.build_hash_kds(coldefs, rows, [int(h) for h in hashes]), False))
+        mtups = [kds.form_minimal_tuple(coldefs, (int(pk.values[j]), int(cat.values[j]),
+                                                  bytes(txt.values[j]))) for j in sel]
+        t0 = time.perf_counter()
+        inners.append((xpuhip.inner_hash_setup(coldefs, mtups, [1]), False))
+        spent += time.perf_counter() - t0
+    if timings is not None:
+        timings["hash_build_s"] = spent
     return kds.build_multirels(inners)
 
 

@@ -24,7 +24,7 @@ EXPORTS = [
     "xpuhip_last_task_stats", "xpuhip_jit_prepare", "xpuhip_batch_exec", "xpuhip_session_reset",
     "xpuhip_session_autotune", "xpuhip_batch_autotune",
     "xpuhip_server_create", "xpuhip_server_submit", "xpuhip_server_wait", "xpuhip_server_stats",
-    "xpuhip_server_destroy",
+    "xpuhip_server_destroy", "xpuhip_inner_hash_setup",
 ]
 REQ_BATCHABLE = 1
 BATCH_GRAPH = 1
@@ -92,6 +92,7 @@ def lib():
         L.xpuhip_init.argtypes = [C.c_int, vp]
         L.xpuhip_session_open.argtypes = [vp, sz, C.POINTER(vp), vp]
         L.xpuhip_session_check.argtypes = [vp, sz, vp]
+        L.xpuhip_inner_hash_setup.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_int, vp]
         L.xpuhip_session_attach_inner.argtypes = [vp, vp, sz, vp]
         L.xpuhip_task_exec.argtypes = [vp, vp, sz, vp, sz, C.POINTER(C.c_uint64), vp, vp]
         L.xpuhip_task_exec_device.argtypes = [vp, vp, C.c_uint32, vp, sz,
@@ -172,6 +173,38 @@ def session_check(blob):
     b = _buf(blob)
     rc = lib().xpuhip_session_check(b.ctypes.data, len(b), C.byref(err))
     return rc, err
+
+
+def inner_hash_setup(coldefs, mtups, key_attnums, nthreads=0, nslots=None):
+    """KDS_FORMAT_HASH buffer of an inner relation built by the library's
+    xpuhip_inner_hash_setup (needs no GPU): coldefs are the relation's
+    (name, type) columns, mtups its rows as MinimalTuple bytes (t_len
+    without a row id), key_attnums the 1-based join key attributes.  The
+    buffer is sized exactly, with KDS_GET_HASHSLOT_WIDTH slots unless nslots
+    is given."""
+    from . import kds as K
+    n = len(mtups)
+    if nslots is None:
+        nslots = abi.kds_get_hashslot_width(n)
+    head = K.kds_header(abi.KDS_FORMAT_HASH, coldefs, hash_nslots=nslots)
+    lens = np.fromiter((len(t) for t in mtups), dtype=np.uint64, count=n)
+    usage = int(((lens + 8 + abi.ROWID_SIZE + 7) & ~np.uint64(7)).sum())
+    length = abi.maxalign(len(head) + 8 * (nslots + n) + usage)
+    out = np.zeros(length, dtype=np.uint8)
+    out[:len(head)] = np.frombuffer(bytes(head), dtype=np.uint8)
+    abi.kern_data_store_head.from_buffer(out).length = length
+    packed = np.frombuffer(b"".join(bytes(t) for t in mtups) or b"\0", dtype=np.uint8)
+    ptrs = np.zeros(max(1, n), dtype=np.uint64)
+    if n:
+        ptrs[1:] = np.cumsum(lens[:-1])
+        ptrs += np.uint64(packed.ctypes.data)
+    keys = np.asarray(key_attnums, dtype=np.int16)
+    err = abi.kern_errorbuf()
+    rc = lib().xpuhip_inner_hash_setup(out.ctypes.data, ptrs.ctypes.data, n, keys.ctypes.data,
+                                       len(keys), int(nthreads), C.byref(err))
+    if rc != 0:
+        raise XpuError(err.errcode, err)
+    return out
 
 
 class DeviceBuffer:

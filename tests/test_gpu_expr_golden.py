@@ -68,8 +68,11 @@ def _plan_blob(expr, slot_types):
     return plan.build()
 
 
-def _session(expr, slot_types):
-    return xpuhip.Session(_plan_blob(expr, slot_types))
+def _session(expr, slot_types, jit=None):
+    s = xpuhip.Session(_plan_blob(expr, slot_types))
+    if jit is not None:
+        s.set_tuning(jit=jit)
+    return s
 
 
 def _walk(e, fn):
@@ -145,7 +148,7 @@ def _expected(t, out):
     return _value(t, img) if t in ("text", "bytea", "bpchar") else img
 
 
-def _replay(expr_fn, slot_types, rtype, cases, pick):
+def _replay(expr_fn, slot_types, rtype, cases, pick, jit=None):
     """cases: [idx..., out]; pick(argument number, index) -> datum image or None."""
     ok = [c for c in cases if not isinstance(c[-1], list)]
     err = [c for c in cases if isinstance(c[-1], list)]
@@ -153,14 +156,14 @@ def _replay(expr_fn, slot_types, rtype, cases, pick):
     if ok:
         cols = [_column("c%d" % i, t, [pick(i, c[i]) for c in ok])
                 for i, t in enumerate(slot_types)]
-        s = _session(expr_fn(), slot_types)
+        s = _session(expr_fn(), slot_types, jit)
         got = _run(s, cols, len(ok))
         s.close()
         assert len(got) == len(ok)
         for c, g in zip(ok, got):
             if _image(rtype, g) != _expected(rtype, c[-1]):
                 bad.append((c[:-1], g, c[-1]))
-    s = _session(expr_fn(), slot_types) if err else None
+    s = _session(expr_fn(), slot_types, jit) if err else None
     for c in err:
         cols = [_column("c%d" % i, t, [pick(i, c[i])]) for i, t in enumerate(slot_types)]
         src = kds.build_arrow_kds(cols)
@@ -174,19 +177,38 @@ def _replay(expr_fn, slot_types, rtype, cases, pick):
     return bad
 
 
-@pytest.mark.parametrize("name", sorted(GOLDEN["funcs"]))
-def test_gpu_function_matches_reference(gpu, name):
+def _check_function(name, jit=None):
     g = GOLDEN["funcs"][name]
     args = g["args"]
     expr_fn = lambda: cg.Func(name, *[cg.Var(i, t) for i, t in enumerate(args)])
     lists = X.func_arg_lists(opcodes.FUNCS[name][1], list(args))
     bad = _replay(expr_fn, args, g["result"], g["cases"],
-                  lambda j, i: None if i is None else lists[j][i])
+                  lambda j, i: None if i is None else lists[j][i], jit)
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])
 
 
+@pytest.mark.parametrize("name", sorted(GOLDEN["funcs"]))
+def test_gpu_function_matches_reference(gpu, name):
+    """Every function, on the interpreter (small chunks run it)."""
+    _check_function(name)
+
+
+# functions with data-dependent control flow over texts, and the float
+# operators with their NaN and overflow paths
+JIT_FUNCS = sorted(n for n, f in GOLDEN["funcs"].items()
+                   if any(a in ("text", "bpchar", "float4", "float8") for a in f["args"]) or
+                   f["result"] in ("text", "bpchar", "float4", "float8"))
+
+
+@pytest.mark.parametrize("name", JIT_FUNCS)
+def test_gpu_function_matches_reference_specialised(gpu, name):
+    """The same on the kernel specialised for the plan (hiprtc), where the
+    compiler sees constant operands: text and float functions."""
+    _check_function(name, xpuhip.JIT_ON)
+
+
 @pytest.mark.parametrize("name", sorted(GOLDEN["templates"]))
-def test_gpu_construct_matches_reference(gpu, name):
+def test_gpu_construct_matches_reference(gpu, jit, name):
     g = GOLDEN["templates"][name]
     if g["compare_nulls"]:
         pytest.skip("kmode_compare_nulls applies to group-key comparison only, "
@@ -194,5 +216,5 @@ def test_gpu_construct_matches_reference(gpu, name):
     expr_fn = lambda: {n: e for n, e, _, _ in X.templates(cg)}[name]
     slots = g["slots"]
     bad = _replay(expr_fn, slots, g["result"], g["cases"],
-                  lambda j, i: X.TEMPLATE_VALUES[slots[j]][i])
+                  lambda j, i: X.TEMPLATE_VALUES[slots[j]][i], jit)
     assert not bad, "%d of %d cases differ, first: %r" % (len(bad), len(g["cases"]), bad[:3])

@@ -110,3 +110,41 @@ def test_final_fits_does_not_grow(gpu):
     assert grows == 0
     exp = run_oracle_groupby(_query(cols, SMALL_FINAL), [src])
     assert_groups_equal(got, exp, C1_ACTIONS)
+
+
+@pytest.mark.parametrize("final_length", [SMALL_FINAL, 1 << 30])
+def test_one_session_from_several_threads(gpu, final_length):
+    """Four threads drive one group-by session at once, as the service's
+    worker threads do (each call takes its own stream and scratch; launches
+    share kds_final through device atomics and hold the session's
+    final-buffer lock shared, growth holds it exclusively): the groups equal
+    the oracle's over all chunks."""
+    import threading
+    chunks = []
+    for i in range(8):
+        cols = _table(120_000, 30_000, seed=500 + i)
+        chunks.append(kds.build_arrow_kds(cols))
+    blob = _query(cols, final_length)
+    s = xpuhip.Session(blob)
+    s.set_tuning(jit=xpuhip.JIT_ON)
+    errors = []
+
+    def work(mine):
+        try:
+            for src in mine:
+                rc, _, _, _ = s.exec(src)
+                assert rc == 0
+        except Exception as e:      # surfaced after join
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(chunks[i::4],)) for i in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    got = s.final()
+    s.close()
+    exp = run_oracle_groupby(_query(cols, 1 << 30), chunks)
+    groups = assert_groups_equal(got, exp, C1_ACTIONS)
+    assert sum(g[0] for g in groups.values()) == 8 * 120_000
