@@ -114,6 +114,28 @@ int     xpuhip_inner_hash_setup(kern_data_store *kds, const void *const *mtups, 
                                 const int16_t *key_attnums, uint32_t nkeys, int nthreads,
                                 kern_errorbuf *errbuf);
 
+/* Arrow IPC file chunk producer (arrowFdwLoadRecordBatch /
+ * arrowFdwFillupRecordBatch, arrow_fdw.c:4018-4100): a file image is opened
+ * once (its footer: schema and record batch blocks), then each record batch
+ * becomes a KDS_FORMAT_ARROW chunk.  kds_head is the chunk's head with the
+ * relation's colmeta (PostgreSQL types); field_index[j] names the Arrow
+ * field of column j, or -1 for a column the plan does not reference.  The
+ * attribute options and buffer offsets are set as
+ * __arrowKdsAssignAttrOptions sets them, and only the referenced buffers
+ * are copied.  xpuhip_arrow_load_batch returns the chunk size; with a null
+ * kds or a too small length it only returns that size.  Uncompressed files
+ * with flat fields (Int, FloatingPoint, Bool, Decimal128, Utf8 / Binary,
+ * their Large forms, FixedSizeBinary); -1 with errbuf set otherwise. */
+typedef struct xpuhip_arrow_file xpuhip_arrow_file;
+int     xpuhip_arrow_open(const void *image, size_t length, xpuhip_arrow_file **p_file,
+                          uint32_t *p_nbatches, uint32_t *p_nfields, kern_errorbuf *errbuf);
+int     xpuhip_arrow_field(xpuhip_arrow_file *file, uint32_t index, char *name, size_t name_len,
+                           int32_t *p_type, int32_t *p_bit_width, int32_t *p_scale);
+int64_t xpuhip_arrow_load_batch(xpuhip_arrow_file *file, uint32_t rb_index,
+                                const kern_data_store *kds_head, const int32_t *field_index,
+                                kern_data_store *kds, size_t length, kern_errorbuf *errbuf);
+void    xpuhip_arrow_close(xpuhip_arrow_file *file);
+
 /* XpuTaskExec on a host-resident chunk (KDS_FORMAT_ARROW, or a heap-page /
  * row chunk as xpuhip_task_exec_device_tuples describes): copies it to the
  * device, runs the pipeline, and for a scan/join projection copies the rows

@@ -36,6 +36,14 @@
 #include "oracle_funcs.h"
 
 #define OMAX_SLOTS  256
+/* kinds of a numeric datum (XPU_NUMERIC_KIND__*, xpu_numeric.h:36-40) */
+#define ON_VALID    0
+#define ON_NAN      1
+#define ON_PINF     2
+#define ON_NINF     3
+#define ON_VARLENA  0xff
+typedef unsigned __int128 ouint128_t;
+static uint32_t varsize_any(const uint8_t *p);
 #define OMAX_DEPTH  16
 #define OMAX_ATTRS  64
 
@@ -101,7 +109,8 @@ oracle_hash_any(const void *ptr, int sz)
 /* ------------------------------------------------------------------
  * function table
  * ------------------------------------------------------------------ */
-enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr };
+enum { K_cmp = 1, K_arith, K_unary, K_bitop, K_cast, K_textcmp, K_textlen, K_like, K_substr,
+       K_numcmp, K_numarith, K_numunary, K_numcast };
 enum { O_eq = 1, O_ne, O_lt, O_le, O_gt, O_ge, O_pl, O_mi, O_mul, O_div, O_mod,
        O_up, O_um, O_abs, O_not, O_and, O_or, O_xor, O_shl, O_shr, O_cast, O_len,
        O_like, O_nlike, O_ilike, O_nilike, O_substr };
@@ -119,6 +128,7 @@ typedef struct { int opcode; const char *name; int kind, op, nargs, t0, t1, t2, 
 #define T_float8    TypeOpCode__float8
 #define T_text      TypeOpCode__text
 #define T_bpchar    TypeOpCode__bpchar
+#define T_numeric   TypeOpCode__numeric
 #define OFUNC(OPC, NAME, KIND, OP, NARGS, A0, A1, A2, RES) \
     { OPC, #NAME, K_##KIND, O_##OP, NARGS, T_##A0, T_##A1, T_##A2, T_##RES },
 static const ofunc ofunc_table[] = { OFUNC_TABLE(OFUNC) { 0, NULL, 0, 0, 0, 0, 0, 0, 0 } };
@@ -163,9 +173,17 @@ typedef struct
     int64_t     i;          /* bool / ints / date */
     double      d;          /* float8 */
     float       f;          /* float4 */
-    const char *p;          /* text payload (no header) */
-    int32_t     len;
+    const char *p;          /* text payload (no header); numeric: its varlena */
+    int32_t     len;        /* numeric varlena: its size, -1 when raw */
+    /* numeric (xpu_numeric_t): kind, scale ("weight") and value; kind
+     * ON_VARLENA until an operator parses the varlena at p */
+    uint8_t     nk;
+    int16_t     nw;
+    int128_t    nv;
 } od;
+
+static int on_encode(const od *v, char *buf);
+static uint32_t on_hash(const od *v);
 
 typedef struct
 {
@@ -377,6 +395,18 @@ datum_from_image(octx *c, int type, const char *addr, od *r)
         case T_int8: { int64_t v; memcpy(&v, addr, 8); r->i = v; } break;
         case T_float4: memcpy(&r->f, addr, 4); break;
         case T_float8: memcpy(&r->d, addr, 8); break;
+        case TypeOpCode__numeric:
+        {
+            /* xpu_numeric_datum_heap_read (xpu_numeric.cu:32-44): the
+             * varlena as it is, parsed by the first operator that needs
+             * its value */
+            uint8_t b0 = (uint8_t)addr[0];
+            r->nk = ON_VARLENA;
+            r->p = addr;
+            r->len = (b0 == 0x01 || (b0 & 0x03) == 0x02) ? (int32_t)-1
+                                                         : (int32_t)varsize_any((const uint8_t *)addr);
+            break;
+        }
         case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
         {
             uint8_t b0 = (uint8_t)addr[0];
@@ -407,8 +437,6 @@ datum_from_image(octx *c, int type, const char *addr, od *r)
     return true;
 }
 
-static uint32_t varsize_any(const uint8_t *p);
-
 /* size and bytes of a datum written into a tuple (xpu_*_datum_write); a
  * compressed or external varlena is copied as it is (xpu_textlib.cu:319-325) */
 static int
@@ -424,6 +452,17 @@ datum_write(int type, const od *v, char *buf)
         case T_int8: if (buf) memcpy(buf, &v->i, 8); return 8;
         case T_float4: if (buf) memcpy(buf, &v->f, 4); return 4;
         case T_float8: if (buf) memcpy(buf, &v->d, 8); return 8;
+        case TypeOpCode__numeric:
+            /* xpu_numeric_datum_write (xpu_numeric.cu:163-178): an unparsed
+             * datum is copied as it is */
+            if (v->nk == ON_VARLENA)
+            {
+                uint32_t sz = varsize_any((const uint8_t *)v->p);
+                if (buf)
+                    memcpy(buf, v->p, sz);
+                return (int)sz;
+            }
+            return on_encode(v, buf);
         case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
             if (v->len < 0)
             {
@@ -457,6 +496,8 @@ datum_hash(int type, const od *v)
         case T_float8: return oracle_hash_any(&v->d, 8);
         case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
             return oracle_hash_any(v->p, v->len);
+        case TypeOpCode__numeric:
+            return on_hash(v);      /* parsed by the caller */
     }
     return 0;
 }
@@ -519,6 +560,424 @@ raw_text(octx *c, int type, const od *v)
     return true;
 }
 
+
+/* ------------------------------------------------------------------
+ * numeric (xpu_numeric.h, xpu_numeric.cu), restated over the datum's
+ * 128-bit integer nv and decimal scale nw (the value is nv * 10^-nw, the
+ * reference's "weight"), with the reference's int128 wraparound
+ * ------------------------------------------------------------------ */
+static int128_t on_x10(int128_t v) { return (int128_t)((ouint128_t)v * 10); }
+
+/* NUMERIC_NORMALIZE (xpu_numeric.h:165-207): as many trailing zeros leave
+ * the value as the scale has positive digits; zero gets scale 0 */
+static void
+on_trim(od *v)
+{
+    if (v->nv == 0)
+    {
+        v->nw = 0;
+        return;
+    }
+    while (v->nw > 0 && v->nv % 10 == 0)
+    {
+        v->nv /= 10;
+        v->nw--;
+    }
+}
+
+/* the datum's value, parsed from its varlena where needed
+ * (__decimal_from_varlena, xpu_numeric.h:210-273): NumericShort or
+ * NumericLong, base-10000 digits with the leading digit at `weight`,
+ * overflow when 14 high bits are in use before a digit is added */
+static bool
+on_parse(octx *c, od *v)
+{
+    if (v->nk != ON_VARLENA)
+        return true;
+    if (v->len < 0)
+        return oerror(c, ERRCODE_SUSPEND_FALLBACK, "%s", "text datum is compressed or external");
+    const uint8_t *h = (const uint8_t *)v->p;
+    const uint8_t *body;
+    uint32_t blen;
+    if (h[0] & 1)
+    {
+        blen = ((h[0] >> 1) & 0x7f) - 1;
+        body = h + 1;
+    }
+    else
+    {
+        uint32_t w;
+        memcpy(&w, h, 4);
+        blen = ((w >> 2) & 0x3fffffff) - 4;
+        body = h + 4;
+    }
+    if (blen < 2)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "corrupted numeric header");
+    uint16_t head = (uint16_t)(body[0] | (body[1] << 8));
+    v->nv = 0;
+    v->nw = 0;
+    switch (head & 0xC000)
+    {
+        case 0xC000:
+            /* NaN, +Inf, -Inf (NUMERIC_NAN / PINF / NINF) */
+            if (head == 0xC000) v->nk = ON_NAN;
+            else if (head == 0xD000) v->nk = ON_PINF;
+            else if (head == 0xF000) v->nk = ON_NINF;
+            else return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "unknown special numeric value");
+            return true;
+        default:
+            break;
+    }
+    int digit_at, weight;
+    bool minus;
+    if ((head & 0xC000) == 0x8000)
+    {
+        /* NumericShort: sign, display scale and a 7-bit weight in n_header */
+        digit_at = 2;
+        weight = (head & 0x0040) ? (int)(head & 0x003F) - 64 : (int)(head & 0x003F);
+        minus = (head & 0x2000) != 0;
+    }
+    else
+    {
+        if (blen < 4)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "corrupted numeric header");
+        digit_at = 4;
+        weight = (int16_t)(body[2] | (body[3] << 8));
+        minus = (head & 0xC000) == 0x4000;
+    }
+    int ndigits = (int)(blen - digit_at) / 2;
+    int128_t acc = 0;
+    for (int i = 0; i < ndigits; i++)
+    {
+        int16_t d = (int16_t)(body[digit_at + 2 * i] | (body[digit_at + 2 * i + 1] << 8));
+        if ((acc >> 114) != 0)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "numeric value is out of range");
+        acc = acc * 10000 + d;
+    }
+    v->nk = ON_VALID;
+    v->nv = minus ? -acc : acc;
+    /* digits after the decimal point: those right of digit `weight` */
+    v->nw = (int16_t)(4 * (ndigits - (int16_t)(weight + 1)));
+    on_trim(v);
+    return true;
+}
+
+/* a and b at the larger of their scales */
+static void
+on_same_scale(od *a, od *b)
+{
+    for (; a->nw > b->nw; b->nw++)
+        b->nv = on_x10(b->nv);
+    for (; b->nw > a->nw; a->nw++)
+        a->nv = on_x10(a->nv);
+}
+
+/* __numeric_compare (xpu_numeric.cu:478-539): NaN above +Inf above every
+ * finite value above -Inf; finite values by sign first */
+static int
+on_cmp(od a, od b)
+{
+    if (a.nk == ON_NAN || b.nk == ON_NAN)
+        return (a.nk == ON_NAN) - (b.nk == ON_NAN);
+    if (a.nk != ON_VALID || b.nk != ON_VALID)
+    {
+        int ra = a.nk == ON_PINF ? 1 : a.nk == ON_NINF ? -1 : 0;
+        int rb = b.nk == ON_PINF ? 1 : b.nk == ON_NINF ? -1 : 0;
+        return ra > rb ? 1 : ra < rb ? -1 : 0;
+    }
+    int sa = a.nv > 0 ? 1 : a.nv < 0 ? -1 : 0;
+    int sb = b.nv > 0 ? 1 : b.nv < 0 ? -1 : 0;
+    if (sa != sb)
+        return sa > sb ? 1 : -1;
+    on_same_scale(&a, &b);
+    return a.nv > b.nv ? 1 : a.nv < b.nv ? -1 : 0;
+}
+
+static int on_sign(const od *v) { return v->nk == ON_PINF ? 1 : v->nk == ON_NINF ? -1 : v->nk == ON_VALID ? (v->nv > 0) - (v->nv < 0) : 0; }
+
+/* __decimal_add / _sub / _mul (xpu_numeric.h:443-605): the finite rule
+ * applies as soon as ONE operand is finite (a NaN or infinity then counts
+ * as zero); otherwise IEEE-like rules over NaN and the infinities */
+static bool
+on_addsub(bool sub, od a, od b, od *r)
+{
+    r->nk = ON_VALID;
+    if (a.nk == ON_VALID || b.nk == ON_VALID)
+    {
+        on_same_scale(&a, &b);
+        r->nw = a.nw;
+        r->nv = sub ? (int128_t)((ouint128_t)a.nv - (ouint128_t)b.nv)
+                    : (int128_t)((ouint128_t)a.nv + (ouint128_t)b.nv);
+        on_trim(r);
+        return true;
+    }
+    int ia = on_sign(&a), ib = sub ? -on_sign(&b) : on_sign(&b);
+    if (a.nk == ON_NAN || b.nk == ON_NAN || ia + ib == 0)
+        r->nk = ON_NAN;
+    else
+        r->nk = (ia + ib > 0) ? ON_PINF : ON_NINF;
+    return true;
+}
+
+static bool
+on_mul(od a, od b, od *r)
+{
+    r->nk = ON_VALID;
+    if (a.nk == ON_VALID || b.nk == ON_VALID)
+    {
+        r->nw = (int16_t)(a.nw + b.nw);
+        r->nv = (int128_t)((ouint128_t)a.nv * (ouint128_t)b.nv);
+        on_trim(r);
+        return true;
+    }
+    int prod = on_sign(&a) * on_sign(&b);
+    r->nk = (a.nk == ON_NAN || b.nk == ON_NAN || prod == 0) ? ON_NAN
+          : prod > 0 ? ON_PINF : ON_NINF;
+    return true;
+}
+
+/* __decimal_div (xpu_numeric.h:607-700): long division of the magnitudes,
+ * one decimal digit per step, until the remainder is zero or the quotient
+ * reaches 2^50 */
+static bool
+on_div(octx *c, od a, od b, od *r)
+{
+    r->nk = ON_VALID;
+    if (a.nk == ON_VALID && b.nk == ON_VALID)
+    {
+        if (b.nv == 0)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "division by zero");
+        bool minus = (a.nv < 0) != (b.nv < 0);
+        int128_t num = a.nv < 0 ? -a.nv : a.nv;
+        int128_t den = b.nv < 0 ? -b.nv : b.nv;
+        int128_t q = 0;
+        int16_t w = (int16_t)(a.nw - b.nw);
+        for (;;)
+        {
+            int128_t digit = num / den;
+            q = (int128_t)((ouint128_t)q * 10 + (ouint128_t)digit);
+            num -= digit * den;
+            if (num == 0 || (q >> 50) != 0)
+                break;
+            num = on_x10(num);
+            w++;
+        }
+        r->nv = minus ? -q : q;
+        r->nw = w;
+        on_trim(r);
+        return true;
+    }
+    if (a.nk == ON_NAN || b.nk == ON_NAN)
+        r->nk = ON_NAN;
+    else if (a.nk == ON_VALID)
+    {
+        r->nv = 0;          /* finite / infinite */
+        r->nw = 0;
+    }
+    else
+    {
+        int sb = on_sign(&b);
+        if (b.nk == ON_VALID && sb == 0)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "division by zero");
+        int q = on_sign(&a) * sb;
+        r->nk = q > 0 ? ON_PINF : ON_NINF;
+    }
+    return true;
+}
+
+/* __xpu_numeric_to_int64 (xpu_numeric.cu:250-295): rounded half away from
+ * zero at the first fractional digit, then range-checked */
+static bool
+on_to_int(octx *c, const od *v, int64_t lo, int64_t hi, int64_t *out)
+{
+    if (v->nk != ON_VALID)
+        return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "cannot convert NaN/Inf to integer");
+    int128_t x = v->nv;
+    int w = v->nw;
+    if (x != 0)
+    {
+        for (; w > 1; w--)
+            x /= 10;
+        if (w == 1)
+            x = (x + (x > 0 ? 5 : -5)) / 10;
+        for (; w < 0; w++)
+        {
+            x = on_x10(x);
+            if (x < lo || x > hi)
+                break;
+        }
+        if (x < lo || x > hi)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "integer out of range");
+    }
+    *out = (int64_t)x;
+    return true;
+}
+
+/* __xpu_numeric_to_fp64 (xpu_numeric.cu:341-380) */
+static bool
+on_to_f64(octx *c, const od *v, double *out)
+{
+    switch (v->nk)
+    {
+        case ON_PINF: *out = INFINITY; return true;
+        case ON_NINF: *out = -INFINITY; return true;
+        case ON_NAN: *out = NAN; return true;
+    }
+    double f = (double)v->nv;
+    if (f != 0.0)
+    {
+        for (int w = v->nw; w > 0; w--)
+            f /= 10.0;
+        for (int w = v->nw; w < 0; w++)
+            f *= 10.0;
+        if (isnan(f) || isinf(f))
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "float out of range");
+    }
+    *out = f;
+    return true;
+}
+
+/* __decimal_from_float8 (xpu_numeric.h:778-846): the 53-bit significand is
+ * brought to binary exponent 0 by halving / doubling, trading factors of
+ * 10 into the decimal scale to keep it in range, then cut to 15 digits.
+ * A float with all exponent bits set (NaN and the infinities) is NaN, and
+ * zero and subnormals go through the same steps with the implicit bit. */
+static void
+on_from_f64(double d, od *r)
+{
+    uint64_t bits;
+    memcpy(&bits, &d, 8);
+    int e = (int)((bits >> 52) & 0x7ff);
+    memset(r, 0, sizeof(od));
+    if (e == 0x7ff)
+    {
+        r->nk = ON_NAN;
+        return;
+    }
+    uint64_t m = (bits & 0xfffffffffffffULL) | (1ULL << 52);
+    int scale = 0;
+    const uint64_t limit = 0x1800000000000000ULL;
+    while (e > 1075)
+    {
+        if (m <= limit) { m <<= 1; e--; }
+        else { m /= 10; scale--; }
+    }
+    while (e < 1075)
+    {
+        if (m >= limit) { m >>= 1; e++; }
+        else { m *= 10; scale++; }
+    }
+    while (m >= 1000000000000000ULL)
+    {
+        m /= 10;
+        scale--;
+    }
+    r->nk = ON_VALID;
+    r->nw = (int16_t)scale;
+    r->nv = (bits >> 63) ? -(int128_t)m : (int128_t)m;
+}
+
+/* __decimal_to_varlena (xpu_numeric.h:275-372), restated on the decimal
+ * digits: the magnitude's digits, padded with zeros so that the scale is a
+ * multiple of 4 (which becomes the display scale), grouped in fours from
+ * the right into NumericLong base-10000 digits, then a zero digit for
+ * every 4 places of a negative scale.  NaN and the infinities take the
+ * 6-byte PostgreSQL encodings. */
+static int
+on_encode(const od *v, char *buf)
+{
+    if (v->nk != ON_VALID)
+    {
+        if (buf)
+        {
+            uint32_t vl = 6 << 2;
+            uint16_t h = v->nk == ON_PINF ? 0xD000 : v->nk == ON_NINF ? 0xF000 : 0xC000;
+            memcpy(buf, &vl, 4);
+            memcpy(buf + 4, &h, 2);
+        }
+        return 6;
+    }
+    od n = *v;
+    bool minus = n.nv < 0;
+    if (minus)
+        n.nv = -n.nv;
+    on_trim(&n);
+    char dig[64];
+    int nd = 0, w = n.nw, w4 = 0;
+    if (n.nv != 0)
+    {
+        char rev[48];
+        int k = 0;
+        for (ouint128_t x = (ouint128_t)n.nv; x != 0; x /= 10)
+            rev[k++] = (char)('0' + (int)(x % 10));
+        /* smallest multiple of 4 not below the scale */
+        w4 = w >= 0 ? (w + 3) / 4 * 4 : -((-w) / 4 * 4);
+        int pad_right = w4 - w;
+        int len = k + pad_right;
+        int pad_left = (4 - len % 4) % 4;
+        for (int i = 0; i < pad_left; i++)
+            dig[nd++] = '0';
+        while (k > 0)
+            dig[nd++] = rev[--k];
+        for (int i = 0; i < pad_right; i++)
+            dig[nd++] = '0';
+    }
+    int ngroups = nd / 4;
+    int nzero = w4 < 0 ? -w4 / 4 : 0;
+    int len = 8 + 2 * (ngroups + nzero);
+    if (buf)
+    {
+        uint32_t vl = (uint32_t)len << 2;
+        uint16_t sign_dscale = (uint16_t)((w4 > 0 ? w4 : 0) | (minus ? 0x4000 : 0));
+        int16_t weight = (int16_t)(ngroups - w4 / 4 - 1);
+        memcpy(buf, &vl, 4);
+        memcpy(buf + 4, &sign_dscale, 2);
+        memcpy(buf + 6, &weight, 2);
+        for (int g = 0; g < ngroups; g++)
+        {
+            uint16_t x = (uint16_t)((dig[4 * g] - '0') * 1000 + (dig[4 * g + 1] - '0') * 100 +
+                                    (dig[4 * g + 2] - '0') * 10 + (dig[4 * g + 3] - '0'));
+            memcpy(buf + 8 + 2 * g, &x, 2);
+        }
+        memset(buf + 8 + 2 * ngroups, 0, 2 * nzero);
+    }
+    return len;
+}
+
+/* xpu_numeric_datum_hash (xpu_numeric.cu:180-196) of a parsed datum */
+static uint32_t
+on_hash(const od *v)
+{
+    if (v->nk != ON_VALID)
+    {
+        uint8_t k = v->nk;
+        return oracle_hash_any(&k, 1);
+    }
+    return oracle_hash_any(&v->nw, 2) ^ oracle_hash_any(&v->nv, 16);
+}
+
+/* __normalize_numeric_int128 (xpu_numeric.cu:1080-1126): the value at
+ * scale `to`, digits cut (toward zero) when the scale shrinks */
+static int128_t
+on_rescale(int128_t v, int from, int to)
+{
+    for (; from < to; from++)
+        v = on_x10(v);
+    for (; from > to; from--)
+        v /= 10;
+    return v;
+}
+
+/* __numeric_typmod_weight (xpu_numeric.h:416-430) */
+static int
+on_typmod_scale(int32_t typmod)
+{
+    if (typmod < 0)
+        return 15;
+    int w = (((typmod - 4) & 0x7ff) ^ 1024) - 1024;
+    return w < 0 ? 0 : w > 15 ? 15 : w;
+}
+
 static bool
 eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
 {
@@ -535,7 +994,7 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
     {
         /* NULL input: comparisons honour kmode_compare_nulls for eq/ne
          * (xpu_basetype.h:30-60); everything else yields NULL */
-        if ((f->kind == K_cmp || f->kind == K_textcmp) && c->compare_nulls &&
+        if ((f->kind == K_cmp || f->kind == K_textcmp || f->kind == K_numcmp) && c->compare_nulls &&
             (f->op == O_eq || f->op == O_ne))
         {
             bool both = a[0].isnull && a[1].isnull;
@@ -822,6 +1281,82 @@ eval_func(octx *c, const kern_expression *k, const ofunc *f, od *r)
             r->i = v;
             return true;
         }
+        case K_numcmp:
+        {
+            /* PG_NUMERIC_COMPARE_TEMPLATE (xpu_numeric.cu:541-568) */
+            if (!on_parse(c, &a[0]) || !on_parse(c, &a[1]))
+                return false;
+            r->i = cmp_result(f->op, on_cmp(a[0], a[1]));
+            return true;
+        }
+        case K_numarith:
+        {
+            /* pgfn_numeric_add .. _mod (xpu_numeric.cu:570-703); numeric_mod
+             * runs the reference's __decimal_div, as its pgfn does */
+            if (!on_parse(c, &a[0]) || !on_parse(c, &a[1]))
+                return false;
+            switch (f->op)
+            {
+                case O_pl: return on_addsub(false, a[0], a[1], r);
+                case O_mi: return on_addsub(true, a[0], a[1], r);
+                case O_mul: return on_mul(a[0], a[1], r);
+                default: return on_div(c, a[0], a[1], r);
+            }
+        }
+        case K_numunary:
+            /* numeric_uplus returns its argument itself; uminus and abs
+             * parse it (xpu_numeric.cu:705-757) */
+            *r = a[0];
+            if (f->op == O_up)
+                return true;
+            if (!on_parse(c, r))
+                return false;
+            if (f->op == O_um)
+            {
+                if (r->nk == ON_VALID)
+                    r->nv = -r->nv;
+                else if (r->nk == ON_PINF)
+                    r->nk = ON_NINF;
+                else if (r->nk == ON_NINF)
+                    r->nk = ON_PINF;
+            }
+            else if (r->nk == ON_VALID && r->nv < 0)
+                r->nv = -r->nv;
+            else if (r->nk == ON_NINF)
+                r->nk = ON_PINF;
+            return true;
+        case K_numcast:
+        {
+            if (f->res == T_numeric)
+            {
+                /* PG_INT_TO_NUMERIC / PG_FLOAT_TO_NUMERIC (xpu_numeric.cu:398-476) */
+                if (f->t0 == T_float8 || f->t0 == T_float4)
+                    on_from_f64(f->t0 == T_float8 ? a[0].d : (double)a[0].f, r);
+                else
+                {
+                    r->nk = ON_VALID;
+                    r->nv = a[0].i;
+                    r->nw = 0;
+                }
+                return true;
+            }
+            if (!on_parse(c, &a[0]))
+                return false;
+            if (is_int_type(f->res))
+            {
+                int64_t lo, hi;
+                int_range(f->res, &lo, &hi);
+                return on_to_int(c, &a[0], lo, hi, &r->i);
+            }
+            double dv;
+            if (!on_to_f64(c, &a[0], &dv))
+                return false;
+            if (f->res == T_float4)
+                r->f = (float)dv;
+            else
+                r->d = dv;
+            return true;
+        }
         case K_cast:
         {
             int s = f->t0, d = f->res;
@@ -898,6 +1433,8 @@ datum_comp(int type, const od *x, const od *y)
             return x->f > y->f ? 1 : x->f < y->f ? -1 : 0;
         case T_float8:
             return x->d > y->d ? 1 : x->d < y->d ? -1 : 0;
+        case TypeOpCode__numeric:
+            return on_cmp(*x, *y);  /* parsed by the caller */
         case T_text: case TypeOpCode__bytea: case T_bpchar:
         {
             int xl = x->len, yl = y->len;
@@ -1266,6 +1803,9 @@ eval(octx *c, const kern_expression *k, od *r)
                 }
                 if (raw_text(c, k->exptype, r) || raw_text(c, k->exptype, &tmp))
                     return false;
+                /* the type's comparison parses numeric datums in place */
+                if (k->exptype == T_numeric && (!on_parse(c, r) || !on_parse(c, &tmp)))
+                    return false;
                 int comp = datum_comp(k->exptype, r, &tmp);
                 if ((least && comp > 0) || (!least && comp < 0))
                     *r = tmp;
@@ -1308,6 +1848,8 @@ eval(octx *c, const kern_expression *k, od *r)
                 if (!eval(c, karg, &tmp))
                     return false;
                 if (raw_text(c, karg->exptype, &tmp))
+                    return false;
+                if (!tmp.isnull && karg->exptype == T_numeric && !on_parse(c, &tmp))
                     return false;
                 if (!tmp.isnull)
                     h = pg_hash_merge(h, datum_hash(karg->exptype, &tmp));
@@ -1414,6 +1956,23 @@ load_arrow(octx *c, const kern_data_store *kds, int resno, uint32_t row, int typ
     {
         const uint8_t *bits = (const uint8_t *)base + cm->values_offset;
         r->i = (bits[row >> 3] >> (row & 7)) & 1;
+        return true;
+    }
+    if (type == T_numeric)
+    {
+        /* xpu_numeric_datum_arrow_read (xpu_numeric.cu:46-79): a
+         * Decimal128 at the column's scale */
+        if (cm->attopts.tag != ArrowType__Decimal || cm->attopts.u.decimal.bitWidth != 128)
+            return oerror(c, ERRCODE_DEVICE_ERROR, "%s", "not a mappable Arrow data type");
+        if (16 * ((uint64_t)row + 1) > cm->values_length)
+        {
+            r->isnull = true;
+            return true;
+        }
+        memcpy(&r->nv, base + cm->values_offset + 16 * (uint64_t)row, 16);
+        r->nk = ON_VALID;
+        r->nw = (int16_t)cm->attopts.u.decimal.scale;
+        on_trim(r);
         return true;
     }
     {
@@ -2192,6 +2751,16 @@ run_actions_saves(const oracle_session *os, octx *c)
     for (uint32_t i = 0; i < ka->nr_args; i++, karg = KEXP_NEXT_ARG(karg))
         if (!eval(c, karg, &tmp))
             return false;
+    /* a numeric sum's argument is parsed in its slot, raising its error
+     * (__update_groupby__psum_numeric's xpu_numeric_validate) */
+    for (int j = 0; j < ka->u.pagg.nattrs; j++)
+    {
+        const kern_aggregate_desc *d = &ka->u.pagg.desc[j];
+        if ((d->action == KAGG_ACTION__PSUM_NUMERIC || d->action == KAGG_ACTION__PAVG_NUMERIC) &&
+            d->arg0_slot_id >= 0 && !c->slots[d->arg0_slot_id].isnull &&
+            !on_parse(c, &c->slots[d->arg0_slot_id]))
+            return false;
+    }
     return true;
 }
 
@@ -2221,6 +2790,14 @@ keys_equal(const oracle_session *os, const ogroup *g, octx *c)
             case T_text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
                 if (a->len != b->len || memcmp(a->p, b->p, a->len) != 0) return false;
                 break;
+            case T_numeric:
+            {
+                /* by value (xpu_numeric_datum_comp) */
+                od x = *a, y = *b;
+                if (!on_parse(c, &x) || !on_parse(c, &y) || on_cmp(x, y) != 0)
+                    return false;
+                break;
+            }
             default: if (a->i != b->i) return false; break;
         }
     }
@@ -2241,6 +2818,11 @@ group_init(const oracle_session *os, ogroup *g)
             case KAGG_ACTION__PMAX_INT64: g->i0[j] = INT64_MIN; break;
             case KAGG_ACTION__PMIN_FP64: g->d0[j] = 1.7976931348623157e308; break;
             case KAGG_ACTION__PMAX_FP64: g->d0[j] = -1.7976931348623157e308; break;
+            case KAGG_ACTION__PSUM_NUMERIC:
+            case KAGG_ACTION__PAVG_NUMERIC:
+                /* the sum's scale, from the argument's typmod */
+                g->attrs[j] = (uint32_t)on_typmod_scale(ka->u.pagg.desc[j].typmod);
+                break;
             default: break;
         }
     }
@@ -2278,7 +2860,8 @@ group_lookup(const oracle_session *os, ogtable *gt, octx *c, uint32_t hash)
     g->hash = hash;
     group_init(os, g);
     for (int j = 0; j < ka->u.pagg.nattrs; j++)
-        if (ka->u.pagg.desc[j].action == KAGG_ACTION__VREF)
+        if (ka->u.pagg.desc[j].action == KAGG_ACTION__VREF &&
+            c->slots[ka->u.pagg.desc[j].arg0_slot_id].len > 0)
             klen += c->slots[ka->u.pagg.desc[j].arg0_slot_id].len;
     g->keybuf = malloc(klen + 1);
     for (int j = 0; j < ka->u.pagg.nattrs; j++)
@@ -2287,7 +2870,7 @@ group_lookup(const oracle_session *os, ogtable *gt, octx *c, uint32_t hash)
         if (d->action != KAGG_ACTION__VREF)
             continue;
         g->keys[j] = c->slots[d->arg0_slot_id];
-        if (g->keys[j].p && !g->keys[j].isnull)
+        if (g->keys[j].p && !g->keys[j].isnull && g->keys[j].len > 0)
         {
             memcpy(g->keybuf + kpos, g->keys[j].p, g->keys[j].len);
             g->keys[j].p = g->keybuf + kpos;
@@ -2403,6 +2986,23 @@ group_update(const oracle_session *os, ogroup *g, octx *c)
                     g->d0[j] += fv;
                 }
                 break;
+            case KAGG_ACTION__PSUM_NUMERIC:
+            case KAGG_ACTION__PAVG_NUMERIC:
+                /* __update_groupby__psum_numeric (cuda_gpupreagg.cu:1408-1446):
+                 * finite values at the sum's scale; NaN and the infinities
+                 * flag the state (the reference ORs the kind 1 of NaN into
+                 * the scale bits instead of __PAGG_NUMERIC_ATTRS__NAN) */
+                if (!v->isnull && v->nk == ON_VALID)
+                {
+                    g->i0[j]++;
+                    g->s128[j] = (int128_t)((ouint128_t)g->s128[j] +
+                        (ouint128_t)on_rescale(v->nv, v->nw, (int)(g->attrs[j] & __PAGG_NUMERIC_ATTRS__WEIGHT)));
+                }
+                else if (!v->isnull && v->nk != ON_VARLENA)
+                    g->attrs[j] |= v->nk == ON_PINF ? __PAGG_NUMERIC_ATTRS__PINF
+                                 : v->nk == ON_NINF ? __PAGG_NUMERIC_ATTRS__NINF
+                                 : __PAGG_NUMERIC_ATTRS__NAN;
+                break;
             case KAGG_ACTION__STDDEV:
                 if (fetch_fp64(t, v, &fv))
                 {
@@ -2456,6 +3056,12 @@ group_merge(const oracle_session *os, ogroup *dst, const ogroup *src)
             case KAGG_ACTION__PAVG_INT64:
                 dst->i0[j] += src->i0[j];
                 dst->s128[j] += src->s128[j];
+                break;
+            case KAGG_ACTION__PSUM_NUMERIC:
+            case KAGG_ACTION__PAVG_NUMERIC:
+                dst->i0[j] += src->i0[j];
+                dst->s128[j] = (int128_t)((ouint128_t)dst->s128[j] + (ouint128_t)src->s128[j]);
+                dst->attrs[j] |= src->attrs[j] & __PAGG_NUMERIC_ATTRS__MASK;
                 break;
             case KAGG_ACTION__PSUM_FP:
             case KAGG_ACTION__PAVG_FP:
@@ -2979,10 +3585,12 @@ write_group_tuple(const oracle_session *os, const ogroup *g, char *buf)
                 break;
             }
             case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+            case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
             {
                 kagg_state__psum_numeric_packed r;
                 memset(&r, 0, sizeof(r));
                 r.vl_len_ = (int32_t)(sizeof(r) << 2);
+                r.attrs = g->attrs[j];
                 r.nitems = (uint64_t)g->i0[j];
                 r.sum.u64_lo = (uint64_t)g->s128[j];
                 r.sum.u64_hi = (uint64_t)(g->s128[j] >> 64);

@@ -509,6 +509,25 @@ ref_eval_expr(const void *kexp_blob, size_t kexp_len, int compare_nulls, const c
     }
     const xpu_datum_operators *rops = r->expr_ops;
     *res_type = rops->xpu_type_code;
+    if (rops->xpu_type_code == TypeOpCode__numeric)
+    {
+        /* NaN and the infinities: __decimal_to_varlena stores their header
+         * and then overwrites it with SET_VARSIZE (xpu_numeric.h:289-300),
+         * so the kind the reference computed is recorded in PostgreSQL's
+         * own 6-byte encoding instead of the bytes it leaves behind */
+        const xpu_numeric_t *nr = (const xpu_numeric_t *)r;
+        if (nr->kind == XPU_NUMERIC_KIND__NAN || nr->kind == XPU_NUMERIC_KIND__POS_INF ||
+            nr->kind == XPU_NUMERIC_KIND__NEG_INF)
+        {
+            uint32_t vl = 6 << 2;
+            uint16_t h = nr->kind == XPU_NUMERIC_KIND__POS_INF ? 0xD000
+                       : nr->kind == XPU_NUMERIC_KIND__NEG_INF ? 0xF000 : 0xC000;
+            memcpy(res_image, &vl, 4);
+            memcpy(res_image + 4, &h, 2);
+            *res_len = 6;
+            return 0;
+        }
+    }
     kern_colmeta cm;
     memset(&cm, 0, sizeof(cm));
     cm.attbyval = rops->xpu_type_byval;

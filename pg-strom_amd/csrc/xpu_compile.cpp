@@ -63,8 +63,38 @@ type_domain(int t)
         case TypeOpCode__float8: return XD_F64;
         case TypeOpCode__text: case TypeOpCode__bytea: case TypeOpCode__bpchar:
             return XD_TEXT;
+        case TypeOpCode__numeric:
+            return XD_NUM;
     }
     return -1;
+}
+
+/* registers a value of a domain occupies */
+int
+dom_regs(int dom)
+{
+    return dom == XD_TEXT ? 2 : dom == XD_NUM ? 3 : 1;
+}
+
+/* the move op of a domain */
+int
+mov_op(int dom)
+{
+    return dom == XD_TEXT ? XO_MOV2 : dom == XD_NUM ? XO_MOV3 : XO_MOV;
+}
+
+/* __numeric_typmod_weight (xpu_numeric.h:416-430): the scale of a numeric
+ * sum, from the argument's typmod (15 digits when it has none) */
+int
+numeric_typmod_scale(int32_t typmod)
+{
+    int w = 15;
+    if (typmod >= 0)
+    {
+        w = (((typmod - 4) & 0x7ff) ^ 1024) - 1024;
+        w = w < 0 ? 0 : (w > 15 ? 15 : w);
+    }
+    return w;
 }
 
 int
@@ -182,7 +212,7 @@ struct Compiler
         {
             int dom = type_domain(slot_type[slot]);
             slot_reg[slot] = slot_top;
-            slot_top += (dom == XD_TEXT) ? 2 : 1;
+            slot_top += dom_regs(dom);
             if (slot_top > temp_next)
                 fail("expression needs more than the per-lane register file");
         }
@@ -303,10 +333,44 @@ struct Compiler
             fail("constant of unsupported type");
             return Reg{0, 0, 0};
         }
-        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        int r = alloc_reg(dom_regs(dom));
         if (isnull)
         {
             emit(XO_CONST, r, 0, 0, XP_REG_TRUE, 1, 0);
+            return Reg{r, dom, type};
+        }
+        if (dom == XD_NUM)
+        {
+            /* the varlena as it is, header included: it is moved and
+             * written back unparsed (pgfn_ConstExpr reads it like a heap
+             * datum) */
+            uint8_t b0 = (uint8_t)val[0];
+            if (b0 == 0x01 || (b0 & 0x03) == 0x02)
+            {
+                fail("compressed or external constant");
+                return Reg{r, dom, type};
+            }
+            uint32_t sz;
+            if (b0 & 0x01)
+                sz = (b0 >> 1) & 0x7f;
+            else
+            {
+                uint32_t h;
+                memcpy(&h, val, 4);
+                sz = (h >> 2) & 0x3fffffff;
+            }
+            if (sz < 1 || (uint64_t)(val - (const char *)s) + sz > length)
+            {
+                fail("numeric constant out of the session blob");
+                return Reg{r, dom, type};
+            }
+            while (out->const_pool.size() % 8)
+                out->const_pool.push_back(0);
+            uint64_t off = out->const_pool.size();
+            out->const_pool.insert(out->const_pool.end(), val, val + sz);
+            while (out->const_pool.size() % 8)
+                out->const_pool.push_back(0);
+            emit(XO_CONST_NUM, r, 0, 0, XP_REG_TRUE, 0, (int64_t)off);
             return Reg{r, dom, type};
         }
         int64_t imm = 0;
@@ -388,7 +452,7 @@ struct Compiler
                     return bad;
                 }
                 int dom = type_domain(k->exptype);
-                if (dom < 0 || dom == XD_TEXT || id >= XP_MAX_PARAMS)
+                if (dom < 0 || dom == XD_TEXT || dom == XD_NUM || id >= XP_MAX_PARAMS)
                     return const_value(k->exptype, off == 0, (const char *)s + off);
                 /* fixed width: a run-time value of the launch */
                 if (param_index[id] < 0)
@@ -446,7 +510,7 @@ struct Compiler
                         fail("SaveExpr type mismatch");
                         return bad;
                     }
-                    emit(sr.dom == XD_TEXT ? XO_MOV2 : XO_MOV, sr.r, v.r);
+                    emit(mov_op(sr.dom), sr.r, v.r);
                 }
                 return sr;
             }
@@ -517,7 +581,7 @@ struct Compiler
                     fail("COALESCE of unsupported type");
                     return bad;
                 }
-                int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+                int r = alloc_reg(dom_regs(dom));
                 int gr = (k->nr_args > 1) ? alloc_reg() : XP_REG_TRUE;
                 int mark = temp_next;
                 int g = guard;
@@ -531,7 +595,7 @@ struct Compiler
                         fail("COALESCE argument type mismatch");
                         return bad;
                     }
-                    emit(XO_COALESCE, r, v.r, dom == XD_TEXT ? 1 : 0);
+                    emit(XO_COALESCE, r, v.r, dom_regs(dom) - 1);
                     temp_next = mark;
                     if (i + 1 < k->nr_args)
                     {
@@ -551,7 +615,8 @@ struct Compiler
                 for (uint32_t i = 0; i < k->nr_args; i++, a = next_arg(a), temp_next = mark)
                 {
                     Reg v = expr(k, a, guard);
-                    int w = (v.dom == XD_TEXT) ? 0 : type_width(a->exptype);
+                    int w = (v.dom == XD_TEXT) ? 0 : (v.dom == XD_NUM) ? XHASH_NUMERIC
+                                                                        : type_width(a->exptype);
                     if (v.dom != XD_TEXT && w < 0)
                     {
                         fail("hash key of unsupported type");
@@ -636,7 +701,7 @@ struct Compiler
             fail("LEAST/GREATEST of unsupported type");
             return bad;
         }
-        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        int r = alloc_reg(dom_regs(dom));
         int mark = temp_next;
         int flags = dom | (k->opcode == FuncOpCode__GreatestExpr ? XMM_GREATEST : 0) |
                     (k->exptype == T_bpchar ? XMM_BPCHAR : 0);
@@ -652,7 +717,7 @@ struct Compiler
                 fail("LEAST/GREATEST argument type mismatch");
                 return bad;
             }
-            emit(XO_MINMAX, r, v.r, 0, XP_REG_TRUE, flags);
+            emit(XO_MINMAX, r, v.r, 0, guard, flags);
             temp_next = mark;
         }
         return Reg{r, dom, (int)k->exptype};
@@ -670,7 +735,7 @@ struct Compiler
             fail("CASE of unsupported type or malformed CaseWhenExpr");
             return bad;
         }
-        int r = alloc_reg(dom == XD_TEXT ? 2 : 1);
+        int r = alloc_reg(dom_regs(dom));
         int gopen = alloc_reg();
         int gthen = alloc_reg();
         int mark = temp_next;
@@ -707,7 +772,7 @@ struct Compiler
                 fail("CASE result type mismatch");
                 return bad;
             }
-            emit(XO_SELECT, r, v.r, 0, gthen, dom == XD_TEXT ? 1 : 0);
+            emit(XO_SELECT, r, v.r, 0, gthen, dom_regs(dom) - 1);
             temp_next = mark;
             a = next_arg(a);
         }
@@ -728,7 +793,7 @@ struct Compiler
                 fail("CASE ELSE type mismatch");
                 return bad;
             }
-            emit(XO_SELECT, r, v.r, 0, gopen, dom == XD_TEXT ? 1 : 0);
+            emit(XO_SELECT, r, v.r, 0, gopen, dom_regs(dom) - 1);
         }
         temp_next = r;
         return Reg{r, dom, (int)k->exptype};
@@ -858,7 +923,7 @@ struct Compiler
                 }
                 temp_next = mark;
                 Reg cv = const_value(etype, false, ep);
-                emit(edom == XD_TEXT ? XO_MOV2 : XO_MOV, er, cv.r);
+                emit(mov_op(edom), er, cv.r);
                 off += esz;
             }
             temp_next = mark;
@@ -1001,7 +1066,7 @@ struct Compiler
         }
         /* the result register comes first, so the arguments' temporaries
          * are released once the operator has consumed them */
-        int r = alloc_reg(type_domain(f->res) == XD_TEXT ? 2 : 1);
+        int r = alloc_reg(dom_regs(type_domain(f->res)));
         int mark = temp_next;
         Reg a[3];
         const kern_expression *ka = first_arg(k);
@@ -1079,6 +1144,43 @@ struct Compiler
                     : 0;
                 emit(XO_SUBSTR, r, a[0].r, a[1].r, guard, enc == ENC_UTF8 ? XENC_UTF8 : 0, imm);
                 return Reg{r, XD_TEXT, TypeOpCode__text};
+            }
+            case K_numcmp:
+                /* PG_NUMERIC_COMPARE_TEMPLATE (xpu_numeric.cu:541-568) */
+                emit(XO_NUM_CMP, r, a[0].r, a[1].r, guard, cmpop | cn);
+                return Reg{r, XD_INT, TypeOpCode__bool};
+            case K_numarith:
+            {
+                /* pgfn_numeric_add .. _mod (xpu_numeric.cu:570-703); the
+                 * reference's numeric_mod computes the division */
+                int op = f->op == O_pl ? XO_NUM_ADD : f->op == O_mi ? XO_NUM_SUB :
+                         f->op == O_mul ? XO_NUM_MUL : XO_NUM_DIV;
+                emit(op, r, a[0].r, a[1].r, guard);
+                return Reg{r, XD_NUM, TypeOpCode__numeric};
+            }
+            case K_numunary:
+                /* numeric_uplus returns its argument as it is */
+                if (f->op == O_up)
+                    emit(XO_MOV3, r, a[0].r);
+                else
+                    emit(f->op == O_um ? XO_NUM_NEG : XO_NUM_ABS, r, a[0].r, 0, guard);
+                return Reg{r, XD_NUM, TypeOpCode__numeric};
+            case K_numcast:
+            {
+                int dd = type_domain(f->res);
+                if (f->res == TypeOpCode__numeric)
+                {
+                    if (a[0].dom == XD_INT)
+                        emit(XO_INT_NUM, r, a[0].r);
+                    else
+                        emit(a[0].dom == XD_F64 ? XO_F64_NUM : XO_F32_NUM, r, a[0].r);
+                    return Reg{r, XD_NUM, TypeOpCode__numeric};
+                }
+                if (dd == XD_INT)
+                    emit(XO_NUM_INT, r, a[0].r, 0, guard, int_class(f->res));
+                else
+                    emit(dd == XD_F64 ? XO_NUM_F64 : XO_NUM_F32, r, a[0].r, 0, guard);
+                return Reg{r, dd, f->res};
             }
             case K_arith:
                 if (type_domain(f->res) == XD_INT)
@@ -1195,6 +1297,8 @@ struct Compiler
             {
                 if (sr.dom == XD_TEXT)
                     emit(XO_LDCOL_VAR, sr.r, 0, 0, XP_REG_TRUE, resno - 1);
+                else if (sr.dom == XD_NUM)
+                    emit(XO_LDCOL_DEC, sr.r, 0, 0, XP_REG_TRUE, resno - 1);
                 else if (sr.type == TypeOpCode__bool)
                     emit(XO_LDCOL_BOOL, sr.r, 0, 0, XP_REG_TRUE, resno - 1);
                 else
@@ -1262,6 +1366,7 @@ layout_state_size(int action)
         case KAGG_ACTION__PSUM_INT: case KAGG_ACTION__PAVG_INT:
             return sizeof(kagg_state__psum_int_packed);
         case KAGG_ACTION__PSUM_INT64: case KAGG_ACTION__PAVG_INT64:
+        case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
             return sizeof(kagg_state__psum_numeric_packed);
         case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
             return sizeof(kagg_state__psum_fp_packed);
@@ -1464,8 +1569,8 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             Reg r = C.reg_of_slot(slot);
             xattr *fa = &P->fb[fd[i].fb_dst_resno - 1];
             const int attlen = h->colmeta[fd[i].fb_dst_resno - 1].attlen;
-            if ((r.dom == XD_TEXT) != (attlen == -1) ||
-                (r.dom != XD_TEXT && attlen != type_width(r.type)))
+            const bool varlena = (r.dom == XD_TEXT || r.dom == XD_NUM);
+            if (varlena != (attlen == -1) || (!varlena && attlen != type_width(r.type)))
                 FAIL("CPU-fallback attribute does not match its variable's type");
             fa->reg = (uint8_t)r.r;
             fa->domain = (uint8_t)r.dom;
@@ -1571,6 +1676,15 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
                 if (r.dom == XD_TEXT)
                     C.emit(XO_TXTCHK, r.r, r.r, 0, XP_REG_TRUE);
             }
+            /* a numeric sum's argument is parsed where it is loaded, and
+             * raises its error there (xpu_numeric_validate) */
+            if ((d->action == KAGG_ACTION__PSUM_NUMERIC || d->action == KAGG_ACTION__PAVG_NUMERIC) &&
+                d->arg0_slot_id >= 0 && d->arg0_slot_id < C.nslots)
+            {
+                Reg r = C.reg_of_slot(d->arg0_slot_id);
+                if (r.dom == XD_NUM)
+                    C.emit(XO_NUM_PARSE, r.r, r.r, 0, XP_REG_TRUE);
+            }
         }
         C.end_seg(P->seg_final);
         if (!C.err.empty())
@@ -1620,10 +1734,10 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
                 ka->reg = ag->arg_reg;
                 ka->domain = ag->arg_domain;
                 ka->attalign = cm->attalign;
-                ka->width = (ka->domain == XD_TEXT) ? -1 : type_width(t);
+                ka->width = (ka->domain == XD_TEXT || ka->domain == XD_NUM) ? -1 : type_width(t);
                 if (ka->width > 0 && cm->attlen != ka->width)
                     FAIL("grouping key width differs from the kds_final column");
-                if (ka->domain == XD_TEXT)
+                if (ka->width < 0)
                     lds_ok = false;
                 else
                 {
@@ -1647,6 +1761,12 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             ag->state_off = off;
             ag->state_size = sz;
             off += sz;
+            if (d->action == KAGG_ACTION__PSUM_NUMERIC || d->action == KAGG_ACTION__PAVG_NUMERIC)
+            {
+                if (ag->arg_reg == 0xff || ag->arg_domain != XD_NUM)
+                    FAIL("numeric sum of a non-numeric argument");
+                ag->num_scale = (uint16_t)numeric_typmod_scale(d->typmod);
+            }
             int f64, f32;
             if (!lds_fields(d->action, &f64, &f32))
                 lds_ok = false;
@@ -1746,7 +1866,7 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             pa->reg = r.r;
             pa->domain = r.dom;
             pa->attalign = cm->attalign;
-            pa->width = (r.dom == XD_TEXT) ? -1 : type_width(r.type);
+            pa->width = (r.dom == XD_TEXT || r.dom == XD_NUM) ? -1 : type_width(r.type);
             if (pa->width > 0 && cm->attlen != pa->width)
                 FAIL("projected value width differs from the kds_dst column");
         }
@@ -1816,8 +1936,8 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             kind = XH_BOOL;
             width = 1;
         }
-        else if (o.op == XO_LDCOL_VAR)
-            kind = XH_VAR;
+        else if (o.op == XO_LDCOL_VAR || o.op == XO_LDCOL_DEC)
+            kind = XH_VAR;      /* numeric on heap pages: a varlena */
         else
             continue;
         if (o.aux >= out->src_cols.size())

@@ -382,6 +382,12 @@ txt_bytecmp(const uint8_t *s1, int64_t l1, const uint8_t *s2, int64_t l2)
     int c = 0;
     for (int64_t i = 0; i < n && c == 0; i++)
         c = (int)s1[i] - (int)s2[i];
+    /* the loop's exit value stays a per-lane integer: where the caller only
+     * tests c != 0 (bpcharne specialised for a plan), gfx950 code
+     * generation otherwise carried "bytes differ" out of the loop in a
+     * lane mask written by the last iteration, in which lanes that had
+     * already left the loop read 0 */
+    asm volatile("" : "+v"(c));
     if (c == 0)
         c = l1 < l2 ? -1 : (l1 > l2 ? 1 : 0);
     return c;
@@ -706,7 +712,7 @@ __device__ __forceinline__ xi128 num_mul10(xi128 v) { return (xi128)((xu128)v * 
 
 /* __decimal_from_varlena (xpu_numeric.h:210-273); returns a message id + 1
  * on error.  A compressed or external datum goes to the CPU fallback. */
-__device__ uint32_t
+static __device__ __noinline__ uint32_t
 num_parse(const uint8_t *hdr, xnum *x)
 {
     int64_t len;
@@ -820,7 +826,7 @@ num_align(xnum &a, xnum &b)
 }
 
 /* __numeric_compare (xpu_numeric.cu:478-539) */
-__device__ int
+static __device__ __noinline__ int
 num_compare(xnum a, xnum b)
 {
     if (a.kind != XNUM_VALID)
@@ -848,7 +854,7 @@ __device__ __forceinline__ bool num_is_pos(const xnum &x) { return x.kind == XNU
  * (xpu_numeric.h:443-776).  As there, add, sub and mul take the finite
  * path when either operand is finite (a NaN or infinity then counts as 0),
  * and numeric_mod runs the division (xpu_numeric.cu:681-702). */
-__device__ uint32_t
+static __device__ __noinline__ uint32_t
 num_arith(int op, xnum a, xnum b, xnum *r)
 {
     r->scale = 0;
@@ -970,8 +976,8 @@ num_arith(int op, xnum a, xnum b, xnum *r)
 
 /* __xpu_numeric_to_int64 (xpu_numeric.cu:250-295): rounded at the first
  * fractional digit, range-checked against [lo, hi] */
-__device__ uint32_t
-num_to_int(const xnum &x, int64_t lo, int64_t hi, int64_t *out)
+static __device__ __noinline__ uint32_t
+num_to_int(const xnum x, int64_t lo, int64_t hi, int64_t *out)
 {
     if (x.kind != XNUM_VALID)
         return XMSG_NUM_NANINF + 1;
@@ -1001,8 +1007,8 @@ num_to_int(const xnum &x, int64_t lo, int64_t hi, int64_t *out)
 }
 
 /* __xpu_numeric_to_fp64 (xpu_numeric.cu:341-380) */
-__device__ uint32_t
-num_to_f64(const xnum &x, double *out)
+static __device__ __noinline__ uint32_t
+num_to_f64(const xnum x, double *out)
 {
     if (x.kind == XNUM_VALID)
     {
@@ -1032,7 +1038,7 @@ num_to_f64(const xnum &x, double *out)
  * rescaled by powers of 2 and 10 to the binary exponent of 1, then cut to
  * 15 decimal digits.  As there, any float whose exponent bits are all ones
  * becomes NaN, and zero takes the path of the normal numbers. */
-__device__ xnum
+static __device__ __noinline__ xnum
 num_from_f64(double dv)
 {
     const uint64_t bits = (uint64_t)__double_as_longlong(dv);
@@ -1087,8 +1093,8 @@ num_from_f64(double dv)
 }
 
 /* xpu_numeric_datum_hash (xpu_numeric.cu:180-196) */
-__device__ __forceinline__ uint32_t
-num_hash(const xnum &x)
+static __device__ __noinline__ uint32_t
+num_hash(const xnum x)
 {
     if (x.kind != XNUM_VALID)
     {
@@ -1106,8 +1112,8 @@ num_hash(const xnum &x)
  * multiple of 4, which is also the display scale written.  Returns the
  * size; writes only if buf.  NaN and the infinities are the 6-byte
  * PostgreSQL encodings (NUMERIC_NAN / PINF / NINF). */
-__device__ uint32_t
-num_to_varlena(const xnum &x, uint8_t *buf)
+static __device__ __noinline__ uint32_t
+num_to_varlena(const xnum x, uint8_t *buf)
 {
     if (x.kind != XNUM_VALID)
     {
@@ -1175,9 +1181,47 @@ num_to_varlena(const xnum &x, uint8_t *buf)
     return len;
 }
 
+/* xpu_numeric_datum_write (xpu_numeric.cu:163-178): a datum still in its
+ * varlena form is copied as it is (compressed and external ones too),
+ * anything else is encoded */
+__device__ __forceinline__ uint32_t
+num_reg_varlena_size(const Lane &L, int r)
+{
+    if ((L.R[r + 2] & 0xff) == XNUM_VARLENA)
+        return varsize_any((const uint8_t *)(uintptr_t)L.R[r]);
+    xnum x;
+    num_read(L, r, &x);
+    return num_to_varlena(x, nullptr);
+}
+
+__device__ __forceinline__ uint32_t
+num_reg_varlena_write(const Lane &L, int r, uint8_t *buf)
+{
+    if ((L.R[r + 2] & 0xff) == XNUM_VARLENA)
+    {
+        const uint8_t *s = (const uint8_t *)(uintptr_t)L.R[r];
+        const uint32_t sz = varsize_any(s);
+        for (uint32_t i = 0; i < sz; i++)
+            buf[i] = s[i];
+        return sz;
+    }
+    xnum x;
+    num_read(L, r, &x);
+    return num_to_varlena(x, buf);
+}
+
+/* the attrs flag of a NaN or infinite value in a numeric sum
+ * (__PAGG_NUMERIC_ATTRS__*) */
+__device__ __forceinline__ uint32_t
+num_special_flag(int kind)
+{
+    return kind == XNUM_PINF ? __PAGG_NUMERIC_ATTRS__PINF
+         : kind == XNUM_NINF ? __PAGG_NUMERIC_ATTRS__NINF : __PAGG_NUMERIC_ATTRS__NAN;
+}
+
 /* __normalize_numeric_int128 (xpu_numeric.cu:1080-1126): a finite value
  * rescaled to the scale of a numeric sum (truncated when digits go) */
-__device__ __forceinline__ xi128
+static __device__ __noinline__ xi128
 num_rescale(xi128 v, int scale_d, int scale_s)
 {
     for (; scale_d > scale_s; scale_s++)
@@ -2162,6 +2206,22 @@ xop_exec(const xop o, Lane &L, const kern_data_store *__restrict__ kds, uint32_t
                 SETNULL(L, d, isnull);
                 break;
             }
+            case XO_NUM_PARSE:
+            {
+                /* the datum parsed in place, as xpu_numeric_validate does
+                 * to a numeric sum's argument */
+                if (RNULL(L, a))
+                    break;
+                xnum x;
+                const uint32_t e = num_read(L, a, &x);
+                if (e)
+                {
+                    RAISE(g, e - 1);
+                    break;
+                }
+                num_store(L, d, x);
+                break;
+            }
             case XO_MOV3:
                 L.R[d] = L.R[a];
                 L.R[d + 1] = L.R[a + 1];
@@ -2408,6 +2468,8 @@ form_tuple_size(const xattr *attrs, int nattrs, const Lane &L)
         t_hoff = (uint32_t)xalign(pa.attalign, t_hoff);
         if (pa.width > 0)
             t_hoff += pa.width;
+        else if (pa.domain == XD_NUM)
+            t_hoff += num_reg_varlena_size(L, pa.reg);
         else if (TXT_RAW(pa.reg))
             t_hoff += varsize_any((const uint8_t *)(uintptr_t)L.R[pa.reg]);
         else
@@ -2451,6 +2513,17 @@ form_tuple_write(const xattr *attrs, int nattrs, const Lane &L, char *buf, uint3
             default:
             {
                 const uint8_t *s = (const uint8_t *)(uintptr_t)v;
+                if (pa.domain == XD_NUM)
+                {
+                    /* only a datum still in its varlena form addresses
+                     * bytes; a parsed value holds its integer there */
+                    if ((L.R[pa.reg + 2] & 0xff) == XNUM_VARLENA && L.R[pa.reg + 1] < 0 &&
+                        s[0] == 0x01)
+                        infomask |= HEAP_HASEXTERNAL;
+                    t_hoff += num_reg_varlena_write(L, pa.reg, (uint8_t *)p);
+                    infomask |= HEAP_HASVARWIDTH;
+                    break;
+                }
                 if (TXT_RAW(pa.reg))
                 {
                     /* a compressed or external datum, as it is
@@ -2587,7 +2660,9 @@ gb_tuple_size(cplan *P, const Lane &L)
         xattr ka = P->keys[ag.key_index];
         t_hoff = (uint32_t)xalign(ka.attalign, t_hoff);
         if (!RNULL(L, ka.reg))
-            t_hoff += (ka.width > 0) ? ka.width : 4 + (uint32_t)L.R[ka.reg + 1];
+            t_hoff += (ka.width > 0) ? ka.width
+                    : (ka.domain == XD_NUM) ? num_reg_varlena_size(L, ka.reg)
+                    : 4 + (uint32_t)L.R[ka.reg + 1];
     }
     return t_hoff + ROWID_SIZE;
 }
@@ -2640,6 +2715,16 @@ gb_keys_match(cplan *P, const Lane &L, const kern_tupitem *t)
                 break;
             default:
             {
+                if (ka.domain == XD_NUM)
+                {
+                    /* numeric keys group by value (xpu_numeric_datum_comp) */
+                    xnum x, y;
+                    if (num_parse((const uint8_t *)p, &x) || num_read(L, ka.reg, &y) ||
+                        num_compare(x, y) != 0)
+                        return false;
+                    pos += varsize_any((const uint8_t *)p);
+                    break;
+                }
                 uint32_t len = (uint32_t)L.R[ka.reg + 1];
                 uint32_t hdr = *(const uint32_t *)p;
                 if (((hdr >> 2) & 0x3fffffff) != len + 4)
@@ -2703,6 +2788,12 @@ gb_tuple_write(cplan *P, const Lane &L, kern_hashitem *hi, uint32_t unit,
                 *(double *)(s + 8) = -1.7976931348623157e308;
                 infomask |= HEAP_HASVARWIDTH;
                 break;
+            case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
+                /* attrs: the sum's scale (cuda_gpupreagg.cu:173-183) */
+                *(int32_t *)s = ag.state_size << 2;
+                *(uint32_t *)(s + 4) = ag.num_scale;
+                infomask |= HEAP_HASVARWIDTH;
+                break;
             default:
                 *(int32_t *)s = ag.state_size << 2;
                 infomask |= HEAP_HASVARWIDTH;
@@ -2731,6 +2822,12 @@ gb_tuple_write(cplan *P, const Lane &L, kern_hashitem *hi, uint32_t unit,
             case 8: *(int64_t *)p = v; pos += 8; break;
             default:
             {
+                if (ka.domain == XD_NUM)
+                {
+                    pos += num_reg_varlena_write(L, ka.reg, (uint8_t *)p);
+                    infomask |= HEAP_HASVARWIDTH;
+                    break;
+                }
                 uint32_t len = (uint32_t)L.R[ka.reg + 1];
                 const uint8_t *s = (const uint8_t *)(uintptr_t)v;
                 *(uint32_t *)p = (len + 4) << 2;
@@ -2909,6 +3006,32 @@ gb_merge_global(cplan *P, const Lane &L, kern_tupitem *t, const LdsTable *T,
                 hi += (old + lo < old) ? 1ULL : 0ULL;
                 if (hi)
                     atomicAdd((unsigned long long *)(s + 24), hi);
+                break;
+            }
+            case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
+            {
+                /* __update_groupby__psum_numeric (cuda_gpupreagg.cu:1408-1446):
+                 * a finite value rescaled to the sum's scale; NaN and the
+                 * infinities set their flag (the argument was parsed by
+                 * XO_NUM_PARSE, so no error is left here) */
+                if (!has)
+                    break;
+                xnum x;
+                if (num_read(L, ag.arg_reg, &x))
+                    break;
+                if (x.kind == XNUM_VALID)
+                {
+                    const xi128 v = num_rescale(x.v, ag.num_scale, x.scale);
+                    const unsigned long long lo = (unsigned long long)(xu128)v;
+                    unsigned long long hi = (unsigned long long)((xu128)v >> 64);
+                    atomicAdd((unsigned long long *)(s + 8), 1ULL);
+                    const unsigned long long old = atomicAdd((unsigned long long *)(s + 16), lo);
+                    hi += (old + lo < old) ? 1ULL : 0ULL;
+                    if (hi)
+                        atomicAdd((unsigned long long *)(s + 24), hi);
+                }
+                else
+                    atomicOr((unsigned int *)(s + 4), num_special_flag(x.kind));
                 break;
             }
             case KAGG_ACTION__PSUM_FP: case KAGG_ACTION__PAVG_FP:
@@ -3134,6 +3257,23 @@ gb_merge_tuple(cplan *P, kern_tupitem *dst, const kern_tupitem *src)
                 atomicAdd((double *)(ds + 16), *(const double *)(ss + 16));
                 break;
             }
+            case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
+            {
+                const uint32_t special = *(const uint32_t *)(ss + 4) & __PAGG_NUMERIC_ATTRS__MASK;
+                if (special)
+                    atomicOr((unsigned int *)(ds + 4), special);
+                const unsigned long long n = *(const unsigned long long *)(ss + 8);
+                if (!n)
+                    break;
+                const unsigned long long lo = *(const unsigned long long *)(ss + 16);
+                unsigned long long hi = *(const unsigned long long *)(ss + 24);
+                atomicAdd((unsigned long long *)(ds + 8), n);
+                const unsigned long long old = atomicAdd((unsigned long long *)(ds + 16), lo);
+                hi += (old + lo < old) ? 1ULL : 0ULL;
+                if (hi)
+                    atomicAdd((unsigned long long *)(ds + 24), hi);
+                break;
+            }
             case KAGG_ACTION__STDDEV:
             {
                 const unsigned long long n = *(const unsigned long long *)(ss + 8);
@@ -3180,6 +3320,12 @@ gb_tuple_keys(cplan *P, Lane &L, const kern_tupitem *t)
             case 8: L.R[ka.reg] = *(const int64_t *)p; pos += 8; break;
             default:
             {
+                if (ka.domain == XD_NUM)
+                {
+                    num_store_varlena(L, ka.reg, (const uint8_t *)p, false);
+                    pos += varsize_any((const uint8_t *)p);
+                    break;
+                }
                 const uint32_t len = ((*(const uint32_t *)p >> 2) & 0x3fffffff) - 4;
                 L.R[ka.reg] = (int64_t)(uintptr_t)(p + 4);
                 L.R[ka.reg + 1] = len;
@@ -3188,6 +3334,21 @@ gb_tuple_keys(cplan *P, Lane &L, const kern_tupitem *t)
             }
         }
     }
+}
+
+/* the group tuple of the key registers on the chain from head, or null */
+__device__ __forceinline__ kern_hashitem *
+gb_chain_find(cplan *P, const Lane &L, kern_data_store *kds, uint64_t head, uint32_t hash)
+{
+    for (uint64_t off = head; off != 0; )
+    {
+        kern_hashitem *hi = hashitem_at(kds, off);
+        const uint32_t h2 = __hip_atomic_load(&hi->t.hash, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h2 == hash && gb_keys_match(P, L, &hi->t))
+            return hi;
+        off = __hip_atomic_load(&hi->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return nullptr;
 }
 
 /* Find-or-insert the group of the key registers of every pending lane and
@@ -3239,17 +3400,19 @@ gb_global(cplan *P, Lane &L, kern_data_store *kds_final, uint32_t hash, bool pen
             kern_hashitem *found = nullptr;
             if (head != XGB_LOCKED_SLOT)
             {
-                for (uint64_t off = head; off != 0; )
+                found = gb_chain_find(P, L, kds, head, hash);
+                if (!found)
                 {
-                    kern_hashitem *hi = hashitem_at(kds, off);
-                    uint32_t h2 = __hip_atomic_load(&hi->t.hash, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-                    if (h2 == hash && gb_keys_match(P, L, &hi->t))
-                    {
-                        found = hi;
-                        break;
-                    }
-                    off = __hip_atomic_load(&hi->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    /* A miss is confirmed after an acquire: the relaxed
+                     * walk reads a group's key bytes with plain loads, which
+                     * may hit a stale copy of its unit in this XCD's L2
+                     * (the unit was zero-filled before the group was
+                     * inserted from another XCD), and a false miss would
+                     * insert the group twice.  The acquire pairs with the
+                     * release that published `head`; misses are rare next
+                     * to hits, so lookups stay fence-free. */
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    found = gb_chain_find(P, L, kds, head, hash);
                 }
                 if (!found &&
                     atomicCAS((unsigned long long *)slot, (unsigned long long)head,
@@ -3396,9 +3559,19 @@ gb_hash_regs(cplan *P, const Lane &L)
         xattr ka = P->keys[k];
         if (RNULL(L, ka.reg))
             continue;
-        uint32_t x = (ka.domain == XD_TEXT)
-            ? hash_bytes((const uint8_t *)(uintptr_t)L.R[ka.reg], (uint32_t)L.R[ka.reg + 1])
-            : hash_fixed((uint64_t)L.R[ka.reg], ka.width);
+        uint32_t x;
+        if (ka.domain == XD_NUM)
+        {
+            /* by value, as HashValue hashes a numeric: equal values of
+             * different images share a slot.  A datum that does not parse
+             * has sent its row to the CPU fallback at its load. */
+            xnum v;
+            x = num_read(L, ka.reg, &v) ? 0 : num_hash(v);
+        }
+        else if (ka.domain == XD_TEXT)
+            x = hash_bytes((const uint8_t *)(uintptr_t)L.R[ka.reg], (uint32_t)L.R[ka.reg + 1]);
+        else
+            x = hash_fixed((uint64_t)L.R[ka.reg], ka.width);
         h = hash_merge(h, x);
     }
     return h ^ 0xffffffffU;

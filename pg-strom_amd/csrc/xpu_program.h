@@ -167,6 +167,7 @@ enum xop_code {
     XO_NUM_F64, XO_NUM_F32,
     XO_INT_NUM,         /* dst triple <- integer R[a] */
     XO_F64_NUM, XO_F32_NUM,
+    XO_NUM_PARSE,       /* dst triple <- R[a] parsed (raises its error where R[guard]) */
     XO_NUM_OPS
 };
 
@@ -209,7 +210,7 @@ typedef struct
     uint32_t    state_off;  /* offset of the state from the payload start */
     int8_t      lds64;      /* 8-byte LDS field array of this action (-1: none) */
     int8_t      lds32;      /* 4-byte LDS field array of this action (-1: none) */
-    uint16_t    __pad;
+    uint16_t    num_scale;  /* PSUM/PAVG_NUMERIC: decimal scale of the sum */
 } xagg;
 
 /* a depth-0 fixed-width column loaded by the scan kernel's tile prologue */

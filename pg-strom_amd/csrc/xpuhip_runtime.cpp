@@ -482,6 +482,14 @@ nogroup_tuple(const xplan *P, uint32_t *p_unit)
                 d = -1.7976931348623157e308; memcpy(s + 8, &d, 8);
                 infomask |= HEAP_HASVARWIDTH;
                 break;
+            case KAGG_ACTION__PSUM_NUMERIC: case KAGG_ACTION__PAVG_NUMERIC:
+            {
+                *(int32_t *)s = ag->state_size << 2;
+                const uint32_t attrs = ag->num_scale;
+                memcpy(s + 4, &attrs, 4);
+                infomask |= HEAP_HASVARWIDTH;
+                break;
+            }
             default:
                 *(int32_t *)s = ag->state_size << 2;
                 infomask |= HEAP_HASVARWIDTH;

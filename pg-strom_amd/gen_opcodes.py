@@ -84,6 +84,24 @@ def classify(name):
         op = ("nilike" if neg else "ilike") if "ic" in name else ("nlike" if neg else "like")
         a0 = "bpchar" if name.startswith("bpchar") else "text"
         return ("like", op, (a0, "text"), "bool")
+    # numeric (xpu_numeric.cu): compares, arithmetic, unary, casts
+    m = re.fullmatch(r"numeric_(eq|ne|lt|le|gt|ge)", name)
+    if m:
+        return ("numcmp", m.group(1), ("numeric", "numeric"), "bool")
+    m = re.fullmatch(r"numeric_(add|sub|mul|div|mod)", name)
+    if m:
+        op = {"add": "pl", "sub": "mi", "mul": "mul", "div": "div", "mod": "mod"}[m.group(1)]
+        return ("numarith", op, ("numeric", "numeric"), "numeric")
+    m = re.fullmatch(r"numeric_(uplus|uminus|abs)", name)
+    if m:
+        op = {"uplus": "up", "uminus": "um", "abs": "abs"}[m.group(1)]
+        return ("numunary", op, ("numeric",), "numeric")
+    m = re.fullmatch(r"numeric_to_(int[1248]|float[48])", name)
+    if m:
+        return ("numcast", "cast", ("numeric",), m.group(1))
+    m = re.fullmatch(r"(int[1248]|float[48])_to_numeric", name)
+    if m:
+        return ("numcast", "cast", (m.group(1),), "numeric")
     if name in ("substr", "substring"):
         return ("substr", "substr", ("text", "int4", "int4"), "text")
     if name in ("substr_nolen", "substring_nolen"):

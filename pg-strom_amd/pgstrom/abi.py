@@ -45,6 +45,7 @@ ArrowType__Int = 2
 ArrowType__FloatingPoint = 3
 ArrowType__Utf8 = 5
 ArrowType__Bool = 6
+ArrowType__Decimal = 7
 ArrowPrecision__Single = 1
 ArrowPrecision__Double = 2
 
@@ -106,13 +107,19 @@ KAGG_ACTION__PMAX_FP64 = 404
 KAGG_ACTION__PSUM_INT = 501
 KAGG_ACTION__PSUM_INT64 = 502
 KAGG_ACTION__PSUM_FP = 503
+KAGG_ACTION__PSUM_NUMERIC = 504
 KAGG_ACTION__PAVG_INT = 601
 KAGG_ACTION__PAVG_INT64 = 602
 KAGG_ACTION__PAVG_FP = 603
+KAGG_ACTION__PAVG_NUMERIC = 604
 KAGG_ACTION__STDDEV = 701
 KAGG_ACTION__COVAR = 801
 
 __PAGG_MINMAX_ATTRS__VALID = 0x0001
+__PAGG_NUMERIC_ATTRS__WEIGHT = 0x00ffff
+__PAGG_NUMERIC_ATTRS__NAN = 0x010000
+__PAGG_NUMERIC_ATTRS__PINF = 0x020000
+__PAGG_NUMERIC_ATTRS__NINF = 0x040000
 
 
 class kern_errorbuf(C.Structure):
@@ -140,9 +147,14 @@ class _ArrowFixedBinary(C.Structure):
     _fields_ = [("byteWidth", C.c_uint32)]
 
 
+class _ArrowDecimal(C.Structure):
+    _fields_ = [("precision", C.c_uint16), ("scale", C.c_uint16), ("bitWidth", C.c_uint16)]
+
+
 class _ArrowOptsUnion(C.Union):
     _fields_ = [("integer", _ArrowInteger), ("floating_point", _ArrowFloat),
-                ("fixed_size_binary", _ArrowFixedBinary), ("_pad", C.c_uint16 * 4)]
+                ("fixed_size_binary", _ArrowFixedBinary), ("decimal", _ArrowDecimal),
+                ("_pad", C.c_uint16 * 4)]
 
 
 class ArrowTypeOptions(C.Structure):

@@ -640,8 +640,8 @@ class Plan:
         # group-by
         final_coldefs = None
         if self.groupby is not None:
-            final_coldefs = [cd for _, _, cd in self.groupby]
-            keys = [(e, cd) for a, e, cd in self.groupby if a == abi.KAGG_ACTION__VREF]
+            final_coldefs = [tuple(cd[:2]) for _, _, cd in self.groupby]
+            keys = [(e, tuple(cd[:2])) for a, e, cd in self.groupby if a == abi.KAGG_ACTION__VREF]
             if keys:
                 # keyhash: HashValue over SaveExpr(key)  (codegen.c:3963-4010)
                 b = bytearray(40)
@@ -663,7 +663,8 @@ class Plan:
                 for (e, _), si, sf in zip(keys, key_slots, final_slots):
                     eqf = {"int1": "int1eq", "int2": "int2eq", "int4": "int4eq",
                            "int8": "int8eq", "float4": "float4eq", "float8": "float8eq",
-                           "text": "text_eq", "date": None, "bool": "booleq"}[e.typname]
+                           "text": "text_eq", "date": None, "bool": "booleq",
+                           "numeric": "numeric_eq", "bpchar": "bpchareq"}[e.typname]
                     cmps.append(Func(eqf, Var(si, e.typname), Var(sf, e.typname)))
                 b = bytearray()
                 emit_expr(b, cmps[0] if len(cmps) == 1 else And(*cmps))
@@ -683,11 +684,12 @@ class Plan:
                         emit_expr(b, SaveExpr(a0, e))
                         emitted.add(a0)
                         nr_args += 1
-                descs.append((action, a0))
+                # the argument's typmod: (name, typname, typmod) in the tlist
+                descs.append((action, a0, cd[2] if len(cd) > 2 else -1))
             b[0:40] = _kexp_head(TYPE_CODE["int4"], SPECIAL["AggFuncs"], nr_args, head_sz)
             struct.pack_into("<i", b, 40, n)
-            for i, (action, a0) in enumerate(descs):
-                struct.pack_into("<Hhhhi", b, 44 + 12 * i, action, a0, -1, -1, -1)
+            for i, (action, a0, typmod) in enumerate(descs):
+                struct.pack_into("<Hhhhi", b, 44 + 12 * i, action, a0, -1, -1, typmod)
             _finish(b, 0)
             codes["groupby_actions"] = bytes(b)
         # load vars (packed by depth; codegen.c:3203-3248)

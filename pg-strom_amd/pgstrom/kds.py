@@ -67,9 +67,28 @@ class ArrowVarlena:
             yield self[i]
 
 
+class DecimalValues:
+    """A numeric column as an Arrow Decimal128 holds it: 128-bit integers at
+    one decimal scale (value = integer * 10^-scale)."""
+
+    def __init__(self, ints, scale, precision=38):
+        self.ints = [int(v) for v in ints]
+        self.scale = int(scale)
+        self.precision = int(precision)
+
+    def __len__(self):
+        return len(self.ints)
+
+    def __getitem__(self, i):
+        import decimal
+        return decimal.Decimal(self.ints[i]).scaleb(-self.scale)
+
+
 class Column:
     """One column of a table: typed numpy values plus an optional NULL mask
-    (True = NULL).  Text columns hold a list/array of bytes objects."""
+    (True = NULL).  Text columns hold a list/array of bytes objects; numeric
+    columns a list of varlena images / decimal values, or DecimalValues
+    (an Arrow Decimal128 column in build_arrow_kds)."""
 
     def __init__(self, name, typname, values, nulls=None):
         self.name = name
@@ -79,6 +98,8 @@ class Column:
                 self.values = values
             else:
                 self.values = list(values)
+        elif typname == "numeric":
+            self.values = values if isinstance(values, DecimalValues) else list(values)
         else:
             self.values = np.ascontiguousarray(values, dtype=NP_DTYPES[typname])
         n = len(self.values)
@@ -207,6 +228,13 @@ def build_arrow_kds(columns, text_layout="utf8"):
             bits = np.packbits(c.values, bitorder="little")
             ent["values"] = (pos, bits.tobytes())
             pos = abi.typealign(64, pos + len(bits))
+        elif c.typname == "numeric":
+            if not isinstance(c.values, DecimalValues):
+                raise ValueError("an Arrow numeric column is a Decimal128 (DecimalValues)")
+            raw = b"".join((v & ((1 << 128) - 1)).to_bytes(16, "little") for v in c.values.ints)
+            ent["values"] = (pos, raw)
+            ent["decimal"] = c.values
+            pos = abi.typealign(64, pos + len(raw))
         else:
             raw = c.values.tobytes()
             ent["values"] = (pos, raw)
@@ -245,6 +273,12 @@ def build_arrow_kds(columns, text_layout="utf8"):
             cm.attopts.unitsz = 4
         elif t == "bool":
             cm.attopts.tag = abi.ArrowType__Bool
+        elif t == "numeric":
+            cm.attopts.tag = abi.ArrowType__Decimal
+            cm.attopts.unitsz = 16
+            cm.attopts.u.decimal.precision = ent["decimal"].precision
+            cm.attopts.u.decimal.scale = ent["decimal"].scale
+            cm.attopts.u.decimal.bitWidth = 128
         for key in ("nullmap", "values", "extra"):
             if key in ent:
                 off, raw = ent[key]
@@ -261,9 +295,85 @@ _FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
         "float4": "<f", "float8": "<d", "date": "<i"}
 
 
+def numeric_image(value, short_varlena=False, long_form=False):
+    """PostgreSQL numeric varlena of a decimal value ('NaN', 'Infinity' and
+    '-Infinity' included), as numeric_in / make_result build it
+    (utils/adt/numeric.c): base-10000 digits without leading or trailing
+    zero digits, the NumericShort header when the display scale and weight
+    fit it (or NumericLong if long_form), under a 4-byte varlena header, or
+    the 1-byte one a short datum has on disk (short_varlena)."""
+    import decimal
+    d = decimal.Decimal(value) if not isinstance(value, decimal.Decimal) else value
+    if d.is_nan():
+        body = struct.pack("<H", 0xC000)
+    elif d.is_infinite():
+        body = struct.pack("<H", 0xF000 if d < 0 else 0xD000)
+    else:
+        sign, digits, exp = d.as_tuple()
+        dscale = max(0, -exp)
+        # decimal digits aligned so that the point falls on a group border
+        ds = "".join(map(str, digits)) or "0"
+        frac = max(0, -exp)
+        ds = ds + "0" * max(0, exp)
+        ds = ds.rjust(frac + 1, "0")
+        ip = ds[:len(ds) - frac] if frac else ds
+        fp = ds[len(ds) - frac:] if frac else ""
+        ip = ip.lstrip("0")
+        ip = "0" * ((-len(ip)) % 4) + ip
+        fp = fp + "0" * ((-len(fp)) % 4)
+        groups = [int(ip[i:i + 4]) for i in range(0, len(ip), 4)] + \
+                 [int(fp[i:i + 4]) for i in range(0, len(fp), 4)]
+        weight = len(ip) // 4 - 1
+        while groups and groups[0] == 0:
+            groups.pop(0)
+            weight -= 1
+        while groups and groups[-1] == 0:
+            groups.pop()
+        if not groups:
+            weight, sign = 0, 0
+        if not long_form and dscale <= 63 and -64 <= weight <= 63:
+            head = 0x8000 | (0x2000 if sign else 0) | (dscale << 7) | (weight & 0x7F)
+            body = struct.pack("<H", head)
+        else:
+            body = struct.pack("<Hh", (0x4000 if sign else 0) | dscale, weight)
+        body += b"".join(struct.pack("<h", g) for g in groups)
+    if short_varlena and len(body) + 1 <= 127:
+        return bytes([((len(body) + 1) << 1) | 1]) + body
+    return struct.pack("<I", (len(body) + 4) << 2) + body
+
+
+def numeric_value(image):
+    """The value of a numeric varlena image as a decimal.Decimal (its display
+    scale ignored)."""
+    import decimal
+    b0 = image[0]
+    body = image[1:(b0 >> 1)] if b0 & 1 else image[4:(struct.unpack_from("<I", image)[0] >> 2)]
+    head = struct.unpack_from("<H", body)[0]
+    if head & 0xC000 == 0xC000:
+        return decimal.Decimal({0xC000: "NaN", 0xD000: "Infinity", 0xF000: "-Infinity"}[head])
+    if head & 0xC000 == 0x8000:
+        weight = (head & 0x3F) - (64 if head & 0x40 else 0)
+        neg = bool(head & 0x2000)
+        digs = body[2:]
+    else:
+        weight = struct.unpack_from("<h", body, 2)[0]
+        neg = head & 0xC000 == 0x4000
+        digs = body[4:]
+    n = len(digs) // 2
+    v = decimal.Decimal(0)
+    with decimal.localcontext() as ctx:
+        ctx.prec = 200
+        for i in range(n):
+            g = struct.unpack_from("<h", digs, 2 * i)[0]
+            v += decimal.Decimal(g).scaleb(4 * (weight - i))
+        return -v if neg else v
+
+
 def encode_datum(typname, value):
     if isinstance(value, (ExternalDatum, CompressedDatum)):
         return value.encode()
+    if typname == "numeric":
+        return bytes(value) if isinstance(value, (bytes, bytearray)) else numeric_image(value)
     if typname in ("text", "bytea", "bpchar"):
         v = bytes(value)
         return struct.pack("<I", (len(v) + 4) << 2) + v
@@ -419,8 +529,12 @@ def _decode_datum(cm, raw, pos):
         hdr = raw[pos]
         if hdr & 0x01:      # 1-byte header
             ln = (hdr >> 1) & 0x7f
+            if cm.atttypid == 1700:     # numeric: the varlena image itself
+                return bytes(raw[pos:pos + ln]), pos + ln
             return bytes(raw[pos + 1:pos + ln]), pos + ln
         ln = (struct.unpack_from("<I", raw, pos)[0] >> 2) & 0x3fffffff
+        if cm.atttypid == 1700:
+            return bytes(raw[pos:pos + ln]), pos + ln
         return bytes(raw[pos + 4:pos + ln]), pos + ln
     fmt = {1: "<b", 2: "<h", 4: "<i", 8: "<q"}[cm.attlen]
     if cm.atttypid == 700:
@@ -608,7 +722,17 @@ def heap_payload(coldefs, values, start=0):
         if hasnull:
             bits[j >> 3] |= 1 << (j & 7)
         _, typlen, _, align, _ = PG_TYPES[t]
-        if typlen == -1:
+        if t == "numeric" and not isinstance(v, (ExternalDatum, CompressedDatum)):
+            # a numeric image goes on the page as given; a decimal value in
+            # its on-page form (a 1-byte header where it fits)
+            d = bytes(v) if isinstance(v, (bytes, bytearray)) else numeric_image(v, short_varlena=True)
+            aligned = not (d[0] & 0x01)
+            infomask |= abi.HEAP_HASVARWIDTH
+            if aligned:
+                nxt = abi.typealign(align, pos)
+                body += b"\0" * (nxt - pos)
+                pos = nxt
+        elif typlen == -1:
             d, aligned = _varlena(v)
             infomask |= abi.HEAP_HASVARWIDTH
             if aligned:

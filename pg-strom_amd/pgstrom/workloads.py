@@ -177,6 +177,9 @@ def decode_state(action, raw):
     if action in (A.KAGG_ACTION__PSUM_FP, A.KAGG_ACTION__PAVG_FP):
         _, n, s = struct.unpack_from("<Iqd", b)
         return (n, s)
+    if action in (A.KAGG_ACTION__PSUM_NUMERIC, A.KAGG_ACTION__PAVG_NUMERIC):
+        attrs, n, lo, hi = struct.unpack_from("<IqQq", b)
+        return (attrs, n, (hi << 64) | lo)
     if action == A.KAGG_ACTION__STDDEV:
         _, n, sx, sxx = struct.unpack_from("<Iqdd", b)
         return (n, sx, sxx)

@@ -25,6 +25,7 @@ EXPORTS = [
     "xpuhip_session_autotune", "xpuhip_batch_autotune",
     "xpuhip_server_create", "xpuhip_server_submit", "xpuhip_server_wait", "xpuhip_server_stats",
     "xpuhip_server_destroy", "xpuhip_inner_hash_setup",
+    "xpuhip_arrow_open", "xpuhip_arrow_field", "xpuhip_arrow_load_batch", "xpuhip_arrow_close",
 ]
 REQ_BATCHABLE = 1
 BATCH_GRAPH = 1
@@ -93,6 +94,13 @@ def lib():
         L.xpuhip_session_open.argtypes = [vp, sz, C.POINTER(vp), vp]
         L.xpuhip_session_check.argtypes = [vp, sz, vp]
         L.xpuhip_inner_hash_setup.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, C.c_int, vp]
+        L.xpuhip_arrow_open.argtypes = [vp, sz, C.POINTER(vp), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32), vp]
+        L.xpuhip_arrow_field.argtypes = [vp, C.c_uint32, C.c_char_p, sz, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.xpuhip_arrow_load_batch.restype = C.c_int64
+        L.xpuhip_arrow_load_batch.argtypes = [vp, C.c_uint32, vp, vp, vp, sz, vp]
+        L.xpuhip_arrow_close.argtypes = [vp]
         L.xpuhip_session_attach_inner.argtypes = [vp, vp, sz, vp]
         L.xpuhip_task_exec.argtypes = [vp, vp, sz, vp, sz, C.POINTER(C.c_uint64), vp, vp]
         L.xpuhip_task_exec_device.argtypes = [vp, vp, C.c_uint32, vp, sz,
@@ -205,6 +213,55 @@ def inner_hash_setup(coldefs, mtups, key_attnums, nthreads=0, nslots=None):
     if rc != 0:
         raise XpuError(err.errcode, err)
     return out
+
+
+class ArrowFile:
+    """An Arrow IPC file opened by the library's chunk producer
+    (xpuhip_arrow_open); batch(i, coldefs, field_names) returns record batch
+    i as a KDS_FORMAT_ARROW chunk with the relation's columns coldefs
+    [(name, typname)], each read from the Arrow field of the same name
+    (or of field_names[j]; None = not referenced).  Needs no GPU."""
+
+    def __init__(self, image):
+        self._img = np.frombuffer(bytes(image), dtype=np.uint8)
+        self.h = C.c_void_p()
+        nb, nf = C.c_uint32(), C.c_uint32()
+        err = abi.kern_errorbuf()
+        rc = lib().xpuhip_arrow_open(self._img.ctypes.data, len(self._img), C.byref(self.h),
+                                     C.byref(nb), C.byref(nf), C.byref(err))
+        if rc != 0:
+            raise XpuError(err.errcode, err)
+        self.nbatches = nb.value
+        self.fields = []
+        for i in range(nf.value):
+            name = C.create_string_buffer(256)
+            t, w, sc = C.c_int32(), C.c_int32(), C.c_int32()
+            lib().xpuhip_arrow_field(self.h, i, name, 256, C.byref(t), C.byref(w), C.byref(sc))
+            self.fields.append((name.value.decode(), t.value, w.value, sc.value))
+
+    def batch(self, rb_index, coldefs, field_names=None):
+        from . import kds as K
+        names = [f[0] for f in self.fields]
+        field_names = field_names or [n for n, _ in coldefs]
+        index = np.array([-1 if n is None else names.index(n) for n in field_names],
+                         dtype=np.int32)
+        head = np.frombuffer(bytes(K.kds_header(abi.KDS_FORMAT_ARROW, coldefs)), dtype=np.uint8)
+        err = abi.kern_errorbuf()
+        n = lib().xpuhip_arrow_load_batch(self.h, rb_index, head.ctypes.data, index.ctypes.data,
+                                          None, 0, C.byref(err))
+        if n < 0:
+            raise XpuError(err.errcode, err)
+        out = np.zeros(n, dtype=np.uint8)
+        n = lib().xpuhip_arrow_load_batch(self.h, rb_index, head.ctypes.data, index.ctypes.data,
+                                          out.ctypes.data, len(out), C.byref(err))
+        if n < 0:
+            raise XpuError(err.errcode, err)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().xpuhip_arrow_close(self.h)
+            self.h = C.c_void_p()
 
 
 class DeviceBuffer:

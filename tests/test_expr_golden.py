@@ -39,6 +39,8 @@ def _norm(out):
 
 def _canon(typname, img):
     """Datum images compare by value: a text's header length form aside."""
+    if typname == "numeric":
+        return X.numeric_canon(img)
     if img is None or isinstance(img, tuple) or typname not in ("text", "bytea", "bpchar"):
         return img
     if img[0] & 1:

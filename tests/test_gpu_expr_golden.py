@@ -33,6 +33,8 @@ FMT = {"bool": "<?", "int1": "<b", "int2": "<h", "int4": "<i", "int8": "<q",
 
 
 def _value(t, img):
+    if t == "numeric":
+        return bytes(img)
     if t in ("text", "bytea", "bpchar"):
         if img[0] & 1:
             return bytes(img[1:])
@@ -42,7 +44,9 @@ def _value(t, img):
 
 def _column(name, t, imgs):
     nulls = np.array([i is None for i in imgs], dtype=bool)
-    if t in ("text", "bytea", "bpchar"):
+    if t == "numeric":
+        vals = [kds.numeric_image("0") if i is None else bytes(i) for i in imgs]
+    elif t in ("text", "bytea", "bpchar"):
         vals = [b"" if i is None else _value(t, i) for i in imgs]
     else:
         vals = [0 if i is None else _value(t, i) for i in imgs]
@@ -126,8 +130,17 @@ def _rebind(e, vars_):
     return e
 
 
+def _chunk(cols):
+    """Arrow columns; a chunk with numeric columns is heap pages instead,
+    which keep each datum's varlena image as the test gives it (an Arrow
+    numeric is a Decimal128)."""
+    if any(c.typname == "numeric" for c in cols):
+        return kds.build_block_kds(cols)
+    return kds.build_arrow_kds(cols)
+
+
 def _run(sess, cols, n):
-    src = kds.build_arrow_kds(cols)
+    src = _chunk(cols)
     rc, dst, res, _ = sess.exec(src, dst_length=max(1 << 16, 256 * n + (1 << 14)))
     assert rc == 0
     return [r[0][0] for r in kds.parse_row_kds(dst)]
@@ -136,6 +149,8 @@ def _run(sess, cols, n):
 def _image(t, v):
     if v is None:
         return None
+    if t == "numeric":
+        return X.numeric_canon(v)
     if t in ("text", "bytea", "bpchar"):
         return v
     return struct.pack(FMT[t], v)
@@ -145,6 +160,8 @@ def _expected(t, out):
     if out is None:
         return None
     img = bytes.fromhex(out)
+    if t == "numeric":
+        return X.numeric_canon(img)
     return _value(t, img) if t in ("text", "bytea", "bpchar") else img
 
 
@@ -166,7 +183,7 @@ def _replay(expr_fn, slot_types, rtype, cases, pick, jit=None):
     s = _session(expr_fn(), slot_types, jit) if err else None
     for c in err:
         cols = [_column("c%d" % i, t, [pick(i, c[i])]) for i, t in enumerate(slot_types)]
-        src = kds.build_arrow_kds(cols)
+        src = _chunk(cols)
         with pytest.raises(xpuhip.XpuError) as e:
             s.exec(src, dst_length=1 << 16)
         got = (e.value.errbuf.errcode, e.value.errbuf.message.decode())
@@ -193,17 +210,17 @@ def test_gpu_function_matches_reference(gpu, name):
     _check_function(name)
 
 
-# functions with data-dependent control flow over texts, and the float
-# operators with their NaN and overflow paths
+# functions with data-dependent control flow over texts and numerics, and
+# the float operators with their NaN and overflow paths
+JIT_TYPES = ("text", "bpchar", "float4", "float8", "numeric")
 JIT_FUNCS = sorted(n for n, f in GOLDEN["funcs"].items()
-                   if any(a in ("text", "bpchar", "float4", "float8") for a in f["args"]) or
-                   f["result"] in ("text", "bpchar", "float4", "float8"))
+                   if any(a in JIT_TYPES for a in f["args"]) or f["result"] in JIT_TYPES)
 
 
 @pytest.mark.parametrize("name", JIT_FUNCS)
 def test_gpu_function_matches_reference_specialised(gpu, name):
     """The same on the kernel specialised for the plan (hiprtc), where the
-    compiler sees constant operands: text and float functions."""
+    compiler sees constant operands: text, numeric and float functions."""
     _check_function(name, xpuhip.JIT_ON)
 
 

@@ -141,3 +141,31 @@ def test_errors():
     assert xpuhip.lib().xpuhip_inner_hash_setup(small.ctypes.data, ptrs, 1, keys.ctypes.data, 1,
                                                 1, C.byref(err)) == -1
     assert b"too small" in bytes(err.message)
+
+
+def test_numeric_key_hash_is_the_device_types():
+    """numeric join keys hash by value (devtype_numeric_hash: the parsed
+    scale and 128-bit value), as the reference's HashValue over the same
+    datum does: every item's hash equals the oracle's HashValue, and
+    1.5 / 1.50 / a short-header 1.5 hash alike."""
+    import oracle
+    from pgstrom import codegen as cg, opcodes
+    from pgstrom.kds import numeric_image
+    imgs = [numeric_image(v) for v in ("1.5", "-2", "0", "1e20", "0.001", "NaN", "Infinity")] + \
+        [numeric_image("1.50"), numeric_image("1.5", short_varlena=True),
+         numeric_image("1.5", long_form=True)]
+    coldefs = [("n", "numeric"), ("i", "int4")]
+    mt = [kds.form_minimal_tuple(coldefs, (im, j)) for j, im in enumerate(imgs)]
+    buf = xpuhip.inner_hash_setup(coldefs, mt, [1])
+    b = bytearray()
+    cg.emit_expr(b, cg.HashValue(cg.Var(0, "numeric")))
+    raw = memoryview(bytes(buf))
+    hd, cms = kds.kds_colmeta(raw)
+    ridx = np.frombuffer(raw, dtype=np.uint64, count=hd.nitems,
+                         offset=abi.kds_head_length(hd.nr_colmeta) + 8 * hd.hash_nslots)
+    for j, im in enumerate(imgs):
+        _, _, h = kds.decode_tupitem(cms, raw, hd.length - int(ridx[j]))
+        exp = oracle.eval_expr(bytes(b), [opcodes.TYPES["numeric"]], [im], opcodes.TYPES["int4"])
+        assert h == struct.unpack("<I", exp)[0], (j, im.hex())
+    hs = [kds.decode_tupitem(cms, raw, hd.length - int(ridx[j]))[2] for j in (0, 7, 8, 9)]
+    assert len(set(hs)) == 1
