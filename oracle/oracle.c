@@ -3664,6 +3664,7 @@ oracle_half_to_float(uint16_t h)
 
 static const kern_data_store *sort_kds;
 static const kern_expression *sort_kexp;
+static int sort_ncmp;           /* keys sort_comp_rows compares (a prefix for window rank) */
 
 /* kern_fetch_minimal_tuple_attr: attribute anum of a kds_final tuple */
 static const char *
@@ -3746,7 +3747,7 @@ sort_comp_rows(const void *a, const void *b)
     const kern_tupitem *tx = (const kern_tupitem *)(tail - ra);
     const kern_tupitem *ty = (const kern_tupitem *)(tail - rb);
 
-    for (int k = 0; k < sort_kexp->u.sort.nkeys; k++)
+    for (int k = 0; k < sort_ncmp; k++)
     {
         const kern_sortkey_desc *d = &sort_kexp->u.sort.desc[k];
         const char *x = sort_fetch_attr(sort_kds, tx, d->src_anum);
@@ -3840,6 +3841,93 @@ sort_comp_rows(const void *a, const void *b)
     return 0;
 }
 
+/* rows ridx[x] and ridx[y] equal on the sort's first m keys */
+static bool
+sort_prefix_equal(const uint64_t *ridx, uint32_t x, uint32_t y, int m)
+{
+    const int save = sort_ncmp;
+    sort_ncmp = m;
+    const bool eq = sort_comp_rows(&ridx[x], &ridx[y]) == 0;
+    sort_ncmp = save;
+    return eq;
+}
+
+/* Window-rank filter of the sorted row index (kern_windowrank_exec_*,
+ * cuda_gpusort.cu:814-987): the first window_partby_nkeys sort keys make
+ * the partition, the next window_orderby_nkeys the peers; a row is kept
+ * when its 0-based row_number / rank / dense_rank within its partition is
+ * below window_rank_limit - 1 (unsigned, as there), i.e. rank() <
+ * window_rank_limit.  The reference finds partition and peer boundaries
+ * where the 32-bit hash of those keys changes (kern_windowrank_prep_hash,
+ * :1170-1311); here they are where the keys change, which is the same
+ * unless two adjacent distinct keys share a hash.  The kept entries move to
+ * the front in sorted order; returns their number. */
+static uint32_t
+window_rank_filter(uint64_t *ridx, uint32_t n)
+{
+    const int np = sort_kexp->u.sort.window_partby_nkeys;
+    const int no = np + sort_kexp->u.sort.window_orderby_nkeys;
+    const char func = sort_kexp->u.sort.window_rank_func;
+    const uint32_t lim = sort_kexp->u.sort.window_rank_limit - 1U;
+    uint32_t part = 0, peer = 0, dense = 0, nkept = 0;
+    uint64_t *kept = malloc(sizeof(uint64_t) * (n ? n : 1));
+    for (uint32_t i = 0; i < n; i++)
+    {
+        if (i == 0 || !sort_prefix_equal(ridx, i - 1, i, np))
+        {
+            part = peer = i;
+            dense = 0;
+        }
+        else if (!sort_prefix_equal(ridx, i - 1, i, no))
+        {
+            peer = i;
+            dense++;
+        }
+        uint32_t r;
+        if (func == KSORT_WINDOW_FUNC__ROW_NUMBER)
+            r = i - part;
+        else if (func == KSORT_WINDOW_FUNC__RANK)
+            r = peer - part;
+        else
+            r = dense;
+        if (r < lim)
+            kept[nkept++] = ridx[i];
+    }
+    memcpy(ridx, kept, sizeof(uint64_t) * nkept);
+    free(kept);
+    return nkept;
+}
+
+/* The kept rows ridx[0..nout) packed at the buffer's new tail, each with
+ * its row id set to its sorted position, as kern_buffer_simple_limit
+ * (cuda_gpusort.cu:1401-1454, rowid at :1446) and kern_windowrank_finalize
+ * (:1314-1356) pack them; returns the new length. */
+static int64_t
+pack_kept_rows(kern_data_store *kds_final, uint64_t need, size_t head_sz, uint64_t *rowidx,
+               uint32_t nout)
+{
+    uint64_t kept = 0, pos = 0;
+    for (uint32_t j = 0; j < nout; j++)
+        kept += XPU_MAXALIGN(((kern_tupitem *)((char *)kds_final + need - rowidx[j]))->t_len);
+    char *stage = calloc(1, kept ? kept : 1);
+    for (uint32_t j = 0; j < nout; j++)
+    {
+        const kern_tupitem *t = (const kern_tupitem *)((char *)kds_final + need - rowidx[j]);
+        uint32_t sz = XPU_MAXALIGN(t->t_len);
+        pos += sz;
+        memcpy(stage + kept - pos, t, t->t_len);
+        memcpy(stage + kept - pos + t->t_len - ROWID_SIZE, &j, ROWID_SIZE);
+        rowidx[j] = pos;
+    }
+    uint64_t new_len = head_sz + 8ULL * nout + kept;
+    memcpy((char *)kds_final + new_len - kept, stage, kept);
+    free(stage);
+    kds_final->nitems = nout;
+    kds_final->usage = kept;
+    kds_final->length = new_len;
+    return (int64_t)new_len;
+}
+
 int64_t
 oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_length,
                   kern_errorbuf *eb)
@@ -3890,36 +3978,17 @@ oracle_task_final(oracle_session *os, kern_data_store *kds_final, size_t final_l
         sort_kds = kds_final;
         sort_kexp = (const kern_expression *)((const char *)os->session +
                                               os->session->xpucode_gpusort_keydesc);
+        sort_ncmp = sort_kexp->u.sort.nkeys;
         if (kds_final->nitems > 1)
             qsort(rowidx, kds_final->nitems, sizeof(uint64_t), sort_comp_rows);
-        if (os->session->gpusort_limit_count > 0 &&
-            (uint64_t)os->session->gpusort_limit_count < kds_final->nitems)
-        {
-            /* kern_buffer_simple_limit (cuda_gpusort.cu:1401-1454): the kept
-             * tuples are packed at the buffer's new tail, each with its row
-             * id set to its sorted position (:1446) */
-            uint32_t nout = (uint32_t)os->session->gpusort_limit_count;
-            uint64_t kept = 0, pos = 0;
-            for (uint32_t j = 0; j < nout; j++)
-                kept += XPU_MAXALIGN(((kern_tupitem *)((char *)kds_final + need - rowidx[j]))->t_len);
-            char *stage = calloc(1, kept ? kept : 1);
-            for (uint32_t j = 0; j < nout; j++)
-            {
-                const kern_tupitem *t = (const kern_tupitem *)((char *)kds_final + need - rowidx[j]);
-                uint32_t sz = XPU_MAXALIGN(t->t_len);
-                pos += sz;
-                memcpy(stage + kept - pos, t, t->t_len);
-                memcpy(stage + kept - pos + t->t_len - ROWID_SIZE, &j, ROWID_SIZE);
-                rowidx[j] = pos;
-            }
-            uint64_t new_len = head_sz + 8ULL * nout + kept;
-            memcpy((char *)kds_final + new_len - kept, stage, kept);
-            free(stage);
-            kds_final->nitems = nout;
-            kds_final->usage = kept;
-            kds_final->length = new_len;
-            return (int64_t)new_len;
-        }
+        uint32_t nout = kds_final->nitems;
+        if (sort_kexp->u.sort.window_rank_func)
+            nout = window_rank_filter(rowidx, kds_final->nitems);
+        else if (os->session->gpusort_limit_count > 0 &&
+                 (uint64_t)os->session->gpusort_limit_count < kds_final->nitems)
+            nout = (uint32_t)os->session->gpusort_limit_count;
+        if (nout < kds_final->nitems || (sort_kexp->u.sort.window_rank_func && nout > 0))
+            return pack_kept_rows(kds_final, need, head_sz, rowidx, nout);
     }
     return (int64_t)need;
 }

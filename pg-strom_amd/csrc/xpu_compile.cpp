@@ -1959,9 +1959,17 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             FAIL("malformed SortKeys");
         if (!out->has_groupby)
             FAIL("GPU-Sort of scan or join results is not supported by this build");
-        if (ks->u.sort.window_rank_func != 0)
-            FAIL("GPU-Sort window rank functions are not supported by this build");
+        const char wf = ks->u.sort.window_rank_func;
+        if (wf != 0 && wf != KSORT_WINDOW_FUNC__ROW_NUMBER && wf != KSORT_WINDOW_FUNC__RANK &&
+            wf != KSORT_WINDOW_FUNC__DENSE_RANK)
+            FAIL("unknown GPU-Sort window rank function");
+        /* codegen.c:4505-4512: a window rank and a LIMIT never come together */
+        if (wf != 0 && s->gpusort_limit_count > 0)
+            FAIL("GPU-Sort window rank with a LIMIT");
         const int nk = ks->u.sort.nkeys;
+        if (wf != 0 && (ks->u.sort.window_partby_nkeys < 1 || ks->u.sort.window_orderby_nkeys < 1 ||
+                        ks->u.sort.window_partby_nkeys + ks->u.sort.window_orderby_nkeys > nk))
+            FAIL("GPU-Sort window rank key counts out of range");
         if (nk < 1 || nk > XSORT_MAX_KEYS)
             FAIL("GPU-Sort key count out of range");
         if (offsetof(kern_expression, u.sort.desc) + (size_t)nk * sizeof(kern_sortkey_desc) > ks->len)
@@ -2026,8 +2034,14 @@ xpu_compile_session(const kern_session_info *s, size_t length, xpu_compiled *out
             }
             nw += sk.nwords;
             out->sort_keys.push_back(sk);
+            if (wf != 0 && k + 1 == ks->u.sort.window_partby_nkeys)
+                out->sort_wr_pwords = nw;
+            if (wf != 0 && k + 1 == ks->u.sort.window_partby_nkeys + ks->u.sort.window_orderby_nkeys)
+                out->sort_wr_owords = nw;
         }
         out->sort_nwords = nw;
+        out->sort_wr_func = wf;
+        out->sort_wr_limit = ks->u.sort.window_rank_limit;
         out->sort_limit = s->gpusort_limit_count > 0 ? s->gpusort_limit_count : 0;
     }
     /* parallelCurrentXids, for the visibility of KDS_FORMAT_ROW sources */

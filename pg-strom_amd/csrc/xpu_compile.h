@@ -41,6 +41,12 @@ struct xpu_compiled
     /* GPU-Sort of the final buffer (FuncOpCode__SortKeys) */
     std::vector<xsort_key> sort_keys;
     uint32_t            sort_nwords = 0;
+    /* GPU-Sort window rank (KSORT_WINDOW_FUNC__*, 0: none): the limit and
+     * the key words of the partition keys and of partition + order keys */
+    char        sort_wr_func = 0;
+    uint32_t    sort_wr_limit = 0;
+    uint32_t    sort_wr_pwords = 0;
+    uint32_t    sort_wr_owords = 0;
     int64_t             sort_limit = 0;         /* gpusort_limit_count, if positive */
 };
 

@@ -331,6 +331,10 @@ int xpu_launch_sort_pass(const uint64_t *words_in, const uint32_t *idx_in, uint6
                          void *stream);          /* counts: 256 * (ntiles + 1) */
 int xpu_launch_sort_permute(const uint64_t *rowindex, const uint32_t *idx, uint64_t *out,
                             uint32_t n, void *stream);
+int xpu_launch_window_rank(const uint64_t *words, const uint64_t *ridx, uint32_t n, uint32_t pw,
+                           uint32_t ow, int func, uint32_t limit, uint32_t *flag, uint32_t *cpeer,
+                           uint32_t *keep, uint32_t *pos, uint32_t *tile_sum, uint32_t *nkept,
+                           uint64_t *ridx_out, void *stream);
 int xpu_launch_sort_limit_size(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
                                uint64_t *offs, uint64_t *used, void *stream);
 int xpu_launch_sort_limit_copy(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,

@@ -496,6 +496,186 @@ xpu_sort_limit_copy_kernel(const kern_data_store *kds, const uint64_t *ridx, uin
     ridx_out[j] = end_off;
 }
 
+/* ------------------------------------------------------------------
+ * Window rank over the sorted key words (kern_windowrank_exec_*,
+ * cuda_gpusort.cu:814-987).  words is the sort's output, [nwords][n] in
+ * sorted order; the partition keys are its first pw words and partition +
+ * order keys its first ow words.  Rows of one partition are contiguous
+ * (the partition keys lead the sort), so a row's partition start is the
+ * first index on [0, i] whose partition words equal its own: a binary
+ * search, as the reference's over its partition hashes.  Boundaries are
+ * key changes; the reference's are changes of a 32-bit hash of the keys.
+ * ------------------------------------------------------------------ */
+__device__ static inline bool
+wrank_equal(const uint64_t *words, uint32_t n, uint32_t nw, uint32_t a, uint32_t b)
+{
+    for (uint32_t w = 0; w < nw; w++)
+        if (words[(uint64_t)w * n + a] != words[(uint64_t)w * n + b])
+            return false;
+    return true;
+}
+
+/* first index on [lo, i] equal to i on the first nw words */
+__device__ static inline uint32_t
+wrank_run_start(const uint64_t *words, uint32_t n, uint32_t nw, uint32_t lo, uint32_t i)
+{
+    uint32_t hi = i;
+    while (lo < hi)
+    {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (wrank_equal(words, n, nw, mid, i))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+/* dense rank, first step: 1 where a peer group starts (a partition start
+ * is one too: the order words include the partition words) */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_wrank_peer_kernel(const uint64_t *words, uint32_t n, uint32_t ow, uint32_t *flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flag[i] = (i == 0 || !wrank_equal(words, n, ow, i - 1, i)) ? 1U : 0U;
+}
+
+/* keep[i] = 1 when row i's 0-based row_number / rank / dense_rank within
+ * its partition is below lim (window_rank_limit - 1, unsigned as in the
+ * reference); cpeer is the inclusive count of peer starts (dense rank) */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_wrank_keep_kernel(const uint64_t *words, uint32_t n, uint32_t pw, uint32_t ow, int func,
+                      uint32_t lim, const uint32_t *cpeer, uint32_t *keep)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t part = wrank_run_start(words, n, pw, 0, i);
+    uint32_t r;
+    if (func == KSORT_WINDOW_FUNC__ROW_NUMBER)
+        r = i - part;
+    else if (func == KSORT_WINDOW_FUNC__RANK)
+        r = wrank_run_start(words, n, ow, part, i) - part;
+    else
+        r = cpeer[i] - cpeer[part];
+    keep[i] = r < lim ? 1U : 0U;
+}
+
+/* the kept entries of the sorted row index, in order: pos is the inclusive
+ * prefix of keep */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_wrank_scatter_kernel(const uint64_t *ridx, const uint32_t *keep, const uint32_t *pos,
+                         uint32_t n, uint64_t *ridx_out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && keep[i])
+        ridx_out[pos[i] - 1] = ridx[i];
+}
+
+/* inclusive prefix of n counters, three steps: the sum of each tile of
+ * XSORT_TILE, an exclusive scan of the tile sums in one block (its total
+ * to *total), then each tile's scan from its base */
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_scan_u32_tiles_kernel(const uint32_t *in, uint32_t n, uint32_t *tile_sum)
+{
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
+    const uint32_t base = blockIdx.x * XSORT_TILE;
+    uint32_t v = 0;
+    for (int r = 0; r < XSORT_ROUNDS; r++)
+    {
+        const uint32_t j = base + r * XSORT_BLOCK + threadIdx.x;
+        v += j < n ? in[j] : 0U;
+    }
+    uint32_t total;
+    (void)sort_block_scan(v, wsum, &total);
+    if (threadIdx.x == 0)
+        tile_sum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_scan_u32_top_kernel(uint32_t *tile_sum, uint32_t ntiles, uint32_t *total)
+{
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += XSORT_BLOCK)
+    {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t v = j < ntiles ? tile_sum[j] : 0U;
+        uint32_t all;
+        const uint32_t ex = sort_block_scan(v, wsum, &all);
+        if (j < ntiles)
+            tile_sum[j] = carry + ex;
+        carry += all;
+    }
+    if (threadIdx.x == 0)
+        *total = carry;
+}
+
+__global__ void __launch_bounds__(XSORT_BLOCK)
+xpu_scan_u32_apply_kernel(const uint32_t *in, uint32_t n, const uint32_t *tile_base,
+                          uint32_t *out)
+{
+    __shared__ uint32_t wsum[XSORT_BLOCK / 64];
+    const uint32_t base = blockIdx.x * XSORT_TILE;
+    uint32_t carry = tile_base[blockIdx.x];
+    for (int r = 0; r < XSORT_ROUNDS; r++)
+    {
+        const uint32_t j = base + r * XSORT_BLOCK + threadIdx.x;
+        const uint32_t v = j < n ? in[j] : 0U;
+        uint32_t all;
+        const uint32_t ex = sort_block_scan(v, wsum, &all);
+        if (j < n)
+            out[j] = carry + ex + v;
+        carry += all;
+    }
+}
+
+static int
+scan_u32(const uint32_t *in, uint32_t n, uint32_t *tile_sum, uint32_t *total, uint32_t *out,
+         hipStream_t st)
+{
+    const uint32_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
+    hipLaunchKernelGGL(xpu_scan_u32_tiles_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0, st,
+                       in, n, tile_sum);
+    hipLaunchKernelGGL(xpu_scan_u32_top_kernel, dim3(1), dim3(XSORT_BLOCK), 0, st,
+                       tile_sum, ntiles, total);
+    hipLaunchKernelGGL(xpu_scan_u32_apply_kernel, dim3(ntiles), dim3(XSORT_BLOCK), 0, st,
+                       in, n, (const uint32_t *)tile_sum, out);
+    return (int)hipGetLastError();
+}
+
+/* Window rank of the n sorted rows: ridx_out gets the kept entries of ridx
+ * (sorted order) and *nkept (device) their count.  Work arrays: flag, cpeer,
+ * keep and pos of n counters each, tile_sum of ceil(n / XSORT_TILE). */
+extern "C" int
+xpu_launch_window_rank(const uint64_t *words, const uint64_t *ridx, uint32_t n, uint32_t pw,
+                       uint32_t ow, int func, uint32_t limit, uint32_t *flag, uint32_t *cpeer,
+                       uint32_t *keep, uint32_t *pos, uint32_t *tile_sum, uint32_t *nkept,
+                       uint64_t *ridx_out, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (n == 0)
+        return 0;
+    const dim3 grid((n + XSORT_BLOCK - 1) / XSORT_BLOCK);
+    int rc = 0;
+    if (func == KSORT_WINDOW_FUNC__DENSE_RANK)
+    {
+        hipLaunchKernelGGL(xpu_wrank_peer_kernel, grid, dim3(XSORT_BLOCK), 0, st, words, n, ow, flag);
+        rc = scan_u32(flag, n, tile_sum, nkept, cpeer, st);
+    }
+    if (rc == 0)
+    {
+        hipLaunchKernelGGL(xpu_wrank_keep_kernel, grid, dim3(XSORT_BLOCK), 0, st, words, n, pw, ow,
+                           func, limit - 1U, (const uint32_t *)cpeer, keep);
+        rc = scan_u32(keep, n, tile_sum, nkept, pos, st);
+    }
+    if (rc == 0)
+        hipLaunchKernelGGL(xpu_wrank_scatter_kernel, grid, dim3(XSORT_BLOCK), 0, st, ridx,
+                           (const uint32_t *)keep, (const uint32_t *)pos, n, ridx_out);
+    return rc == 0 ? (int)hipGetLastError() : rc;
+}
+
 extern "C" int
 xpu_launch_sort_limit_size(const kern_data_store *kds, const uint64_t *ridx, uint32_t nout,
                            uint64_t *offs, uint64_t *used, void *stream)

@@ -2354,19 +2354,22 @@ xpuhip_task_exec(xpuhip_session *s, const kern_data_store *kds_src, size_t src_l
 static int
 sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
            const uint64_t **d_out, uint64_t usage, const char **d_tail, uint64_t *used,
-           kern_errorbuf *eb)
+           uint32_t *p_nout, kern_errorbuf *eb)
 {
-    /* with a LIMIT the kept tuples are packed (kern_buffer_simple_limit)
-     * into a stage sized to their total, which a size pass computes first */
-    const bool limited = nout < n;
-    (void)usage;
+    /* with a LIMIT or a window rank the kept tuples are packed
+     * (kern_buffer_simple_limit, kern_windowrank_finalize) into a stage
+     * sized to their total, which a size pass computes first */
     const xpu_compiled &c = s->comp;
+    const bool wr = c.sort_wr_func != 0;
+    const bool limited = nout < n || wr;
+    (void)usage;
     const uint32_t nw = c.sort_nwords;
     const uint64_t ntiles = (n + XSORT_TILE - 1) / XSORT_TILE;
     const size_t words_bytes = 8ULL * nw * n;
     const size_t need = 2 * words_bytes + 2 * 4ULL * n + 4ULL * 256 * (ntiles + 1) + 16ULL * nw +
                         8ULL * nout + 8 * 256 +     /* 7 slices, each rounded up to 256 B */
-                        (limited ? 16ULL * nout + 8 + 3 * 256 : 0);
+                        (limited ? 16ULL * nout + 8 + 3 * 256 : 0) +
+                        (wr ? 16ULL * n + 4 * ntiles + 4 + 8ULL * n + 7 * 256 : 0);
     if (need > s->sort_cap)
     {
         if (s->d_sort)
@@ -2388,6 +2391,15 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     uint64_t *offs = limited ? (uint64_t *)take(8ULL * nout) : nullptr;
     uint64_t *ridx_packed = limited ? (uint64_t *)take(8ULL * nout) : nullptr;
     uint64_t *d_used = limited ? (uint64_t *)take(8) : nullptr;
+    /* window rank: flags, peer counts, keep flags and their prefix, tile
+     * sums, the kept count and the kept row index */
+    uint32_t *wr_flag = wr ? (uint32_t *)take(4ULL * n) : nullptr;
+    uint32_t *wr_cpeer = wr ? (uint32_t *)take(4ULL * n) : nullptr;
+    uint32_t *wr_keep = wr ? (uint32_t *)take(4ULL * n) : nullptr;
+    uint32_t *wr_pos = wr ? (uint32_t *)take(4ULL * n) : nullptr;
+    uint32_t *wr_tiles = wr ? (uint32_t *)take(4ULL * ntiles) : nullptr;
+    uint32_t *wr_nkept = wr ? (uint32_t *)take(4) : nullptr;
+    uint64_t *wr_ridx = wr ? (uint64_t *)take(8ULL * n) : nullptr;
     if ((size_t)(p - s->d_sort) > s->sort_cap)
     {
         XERR(eb, ERRCODE_DEVICE_ERROR, "GPU-Sort work space layout exceeds its allocation");
@@ -2446,10 +2458,25 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     }
     if (rc == 0)
         rc = xpu_launch_sort_permute(d_ridx, I[cur], ridx_out, nout, nullptr);
+    /* window rank over the sorted words (W[cur], the last pass's output) */
+    const uint64_t *kept_ridx = ridx_out;
+    if (rc == 0 && wr)
+    {
+        rc = xpu_launch_window_rank(W[cur], ridx_out, n, c.sort_wr_pwords, c.sort_wr_owords,
+                                    c.sort_wr_func, c.sort_wr_limit, wr_flag, wr_cpeer, wr_keep,
+                                    wr_pos, wr_tiles, wr_nkept, wr_ridx, nullptr);
+        if (rc == 0 && !hip_ok(hipMemcpy(&nout, wr_nkept, 4, hipMemcpyDeviceToHost), eb, "hipMemcpy"))
+        {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return -1;
+        }
+        kept_ridx = wr_ridx;
+    }
     uint64_t stage_len = 0;
     if (rc == 0 && limited)
     {
-        rc = xpu_launch_sort_limit_size(s->d_kds_final, ridx_out, nout, offs, d_used, nullptr);
+        rc = xpu_launch_sort_limit_size(s->d_kds_final, kept_ridx, nout, offs, d_used, nullptr);
         if (rc == 0 &&
             (!hip_ok(hipMemcpy(&stage_len, d_used, 8, hipMemcpyDeviceToHost), eb, "hipMemcpy") ||
              !grow(&s->d_sort_stage, &s->sort_stage_cap, stage_len > 0 ? stage_len : 8, eb)))
@@ -2459,7 +2486,7 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
             return -1;
         }
         if (rc == 0)
-            rc = xpu_launch_sort_limit_copy(s->d_kds_final, ridx_out, nout, offs, s->d_sort_stage,
+            rc = xpu_launch_sort_limit_copy(s->d_kds_final, kept_ridx, nout, offs, s->d_sort_stage,
                                             stage_len, ridx_packed, nullptr);
     }
     (void)hipEventRecord(e1, nullptr);
@@ -2479,6 +2506,7 @@ sort_final(xpuhip_session *s, uint32_t n, uint32_t nout, const uint64_t *d_ridx,
     t_last_stats.sort_items = n;
     t_last_stats.sort_passes = passes;
     *d_out = ridx_out;
+    *p_nout = nout;
     if (limited)
     {
         *used = stage_len;
@@ -2531,7 +2559,8 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
     t_last_stats.sort_ms = 0;
     t_last_stats.sort_items = 0;
     t_last_stats.sort_passes = 0;
-    if (sorted && nitems > 1)
+    /* a window rank re-packs even one row (kern_windowrank_finalize) */
+    if (sorted && (nitems > 1 || (s->comp.sort_wr_func != 0 && nitems > 0)))
     {
         if (nitems > UINT32_MAX)
         {
@@ -2539,12 +2568,15 @@ xpuhip_task_final(xpuhip_session *s, kern_data_store *out, size_t length,
             return -1;
         }
         std::lock_guard<std::mutex> g(s->lock);
+        uint32_t kept = (uint32_t)nout;
         if (sort_final(s, (uint32_t)nitems, (uint32_t)nout, d_ridx, &d_ridx, usage, &d_tail,
-                       &usage, eb) != 0)
+                       &usage, &kept, eb) != 0)
             return -1;
+        nout = kept;                            /* fewer after a window rank */
         need = head_len + 8 * nout + usage;     /* the packed kept tuples */
         if (res)
         {
+            res->final_nitems = (uint32_t)nout;
             /* what the written buffer holds (gpu_service.c:5530-5540) */
             res->final_usage = usage;
             res->final_total = need;

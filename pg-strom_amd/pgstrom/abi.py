@@ -93,6 +93,7 @@ KSORT_KEY_KIND__PAVG_NUMERIC = 8
 KSORT_KEY_KIND__PVARIANCE_SAMP = 9
 KSORT_WINDOW_FUNC__ROW_NUMBER = ord("n")
 KSORT_WINDOW_FUNC__RANK = ord("r")
+KSORT_WINDOW_FUNC__DENSE_RANK = ord("d")
 
 KAGG_ACTION__VREF = 101
 KAGG_ACTION__VREF_NOKEY = 102

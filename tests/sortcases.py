@@ -115,3 +115,65 @@ def typed_expected(dtype, values, asc, nulls_first):
             distinct.append(v)
     distinct.sort(reverse=not asc)
     return [None] + distinct if nulls_first else distinct + [None]
+
+
+# ---- window rank: rank() over (PARTITION BY count(*) ORDER BY sum(v) DESC)
+# with the group key as a trailing sort key, so the sorted order is total
+WINDOW_KEYS = [count_vref(True), sum_key(False), key_vref(True, False)]
+WINDOW_ACTIONS = [A.KAGG_ACTION__NROWS_ANY, A.KAGG_ACTION__PSUM_INT64, A.KAGG_ACTION__PAVG_FP,
+                  A.KAGG_ACTION__PMIN_INT32, A.KAGG_ACTION__PMAX_INT32, A.KAGG_ACTION__VREF]
+WINDOW_FUNCS = {"row_number": A.KSORT_WINDOW_FUNC__ROW_NUMBER, "rank": A.KSORT_WINDOW_FUNC__RANK,
+                "dense_rank": A.KSORT_WINDOW_FUNC__DENSE_RANK}
+
+
+def window_table(n, nkeys, seed=4):
+    """key int4 uniform over nkeys values (about one row in 500 NULL), v int8
+    in 0..3 so that sums tie within a partition, w float8, x int4."""
+    r = np.random.default_rng(seed)
+    key = r.integers(0, nkeys, size=n, dtype=np.int32)
+    knull = r.random(n) < 0.002
+    v = r.integers(0, 4, size=n, dtype=np.int64)
+    w = r.random(n)
+    x = r.integers(-1000, 1000, size=n, dtype=np.int32)
+    return [Column("key", "int4", key, knull), Column("v", "int8", v),
+            Column("w", "float8", w), Column("x", "int4", x)]
+
+
+def window_query(cols, func, limit, final_nrows=10000):
+    """The group-by with GPU-Sort by WINDOW_KEYS and the window rank
+    (func, limit) over partition key 1 (count) and order key 2 (sum)."""
+    plan = cg.Plan([(c.name, c.typname) for c in cols])
+    key, v, w, x = (plan.scan_var(n) for n in ("key", "v", "w", "x"))
+    plan.set_groupby(W.preagg_tlist(plan, key, v, w, x))
+    return plan.build(final_nrows=final_nrows, final_length=1 << 30, sort_keys=WINDOW_KEYS,
+                      window_rank=(WINDOW_FUNCS[func], limit, 1, 1))
+
+
+def window_expected(groups, func, limit):
+    """SQL's row_number / rank / dense_rank over the groups ({(key,): states}
+    of decode_groups), written from their definitions: the group keys kept
+    by `rank() < limit`, in the sort's order (count asc, sum desc, key asc
+    NULLS LAST)."""
+    rows = []
+    for (k,), st in groups.items():
+        cnt, (_, total) = st[0], st[1]
+        rows.append((cnt, total, k))
+    rows.sort(key=lambda t: (t[0], -t[1], t[2] is None, t[2] if t[2] is not None else 0))
+    out = []
+    i = 0
+    while i < len(rows):
+        j = i
+        while j < len(rows) and rows[j][0] == rows[i][0]:
+            j += 1
+        part = rows[i:j]
+        for pos, row in enumerate(part):
+            if func == "row_number":
+                rank = pos + 1
+            elif func == "rank":
+                rank = 1 + sum(1 for q in part if q[1] > row[1])
+            else:
+                rank = len({q[1] for q in part if q[1] >= row[1]})
+            if rank < limit:
+                out.append(row[2])
+        i = j
+    return out

@@ -131,3 +131,37 @@ def test_gpu_sort_null_states(gpu, nulls_first):
     assert order == S.group_keys_in_order(exp)
     null_groups = sorted(int(k) for k in set(key[xnull]))
     assert (order[:len(null_groups)] if nulls_first else order[-len(null_groups):]) == null_groups
+
+
+@pytest.mark.parametrize("func", sorted(S.WINDOW_FUNCS))
+@pytest.mark.parametrize("nrows, nkeys, limit", [(20000, 3000, 3), (2_000_000, 200_000, 5)])
+def test_gpu_window_rank_matches_oracle(gpu, func, nrows, nkeys, limit):
+    """rank() < limit over (PARTITION BY count ORDER BY sum DESC), the group
+    key last: the kept groups in the oracle's order with their states, row
+    ids renumbered; 200K groups span many scan tiles and long partitions."""
+    cols = S.window_table(nrows, nkeys)
+    blob = S.window_query(cols, func, limit, final_nrows=nkeys + 10)
+    src = kds.build_arrow_kds(cols)
+    fin, st, again = _gpu_sorted(blob, [src])
+    exp = run_oracle_groupby(blob, [src])
+    keys = S.group_keys_in_order(fin)
+    assert keys == S.group_keys_in_order(exp)
+    assert S.group_keys_in_order(again) == keys
+    assert 0 < len(keys) < nkeys
+    assert [rid for _, rid in kds.parse_row_kds(fin)] == list(range(len(keys)))
+    assert_groups_equal(fin, exp, S.WINDOW_ACTIONS)
+    assert st["sort_items"] > len(keys)
+
+
+def test_gpu_window_rank_single_group(gpu):
+    """One group: kept (its rank is 1) and re-packed, as the reference's
+    finalize re-packs every kept row."""
+    cols = S.window_table(50, 1)
+    cols[0] = Column("key", "int4", cols[0].values)
+    blob = S.window_query(cols, "rank", 2)
+    src = kds.build_arrow_kds(cols)
+    fin, _, _ = _gpu_sorted(blob, [src])
+    exp = run_oracle_groupby(blob, [src])
+    assert S.group_keys_in_order(fin) == S.group_keys_in_order(exp) == [0]
+    assert np.array_equal(np.frombuffer(bytes(fin), np.uint8)[:16],
+                          np.frombuffer(bytes(exp), np.uint8)[:16])

@@ -3,7 +3,7 @@ comparator (cuda_gpusort.cu:15-343) and LIMIT (gpu_service.c:5099-5197)
 against hand-computed orders, and the compiler's checks of SortKeys."""
 import pytest
 
-from pgstrom import abi, kds, xpuhip
+from pgstrom import abi, kds, workloads as W, xpuhip
 from helpers import run_oracle_groupby
 import sortcases as S
 
@@ -57,20 +57,25 @@ def _compile_error(blob):
 
 
 @pytest.mark.parametrize("bad, message", [
-    ("window", "window rank functions are not supported"),
+    ("window_limit", "window rank with a LIMIT"),
+    ("window_keys", "window rank key counts out of range"),
     ("anum", "key attribute out of range"),
     ("type", "does not match the final buffer column"),
     ("kind", "kind is not supported"),
 ])
 def test_sort_keys_compile_checks(bad, message):
     cols = S.tiny_table()
-    if bad == "window":
+    if bad.startswith("window"):
         from pgstrom import codegen as cg, workloads as W
         plan = cg.Plan([(c.name, c.typname) for c in cols])
         key, v, w, x = (plan.scan_var(n) for n in ("key", "v", "w", "x"))
         plan.set_groupby(W.preagg_tlist(plan, key, v, w, x))
-        blob = plan.build(sort_keys=[S.key_vref()],
-                          window_rank=(abi.KSORT_WINDOW_FUNC__RANK, 3, 0, 1))
+        if bad == "window_limit":
+            blob = plan.build(sort_keys=S.WINDOW_KEYS, sort_limit=5,
+                              window_rank=(abi.KSORT_WINDOW_FUNC__RANK, 3, 1, 1))
+        else:
+            blob = plan.build(sort_keys=[S.key_vref()],
+                              window_rank=(abi.KSORT_WINDOW_FUNC__RANK, 3, 0, 1))
     elif bad == "anum":
         blob = S.sorted_query(cols, [(abi.KSORT_KEY_KIND__VREF, 9, "int4", True, False)])
     elif bad == "type":
@@ -100,3 +105,42 @@ def test_oracle_sort_key_types(typed, asc, nulls_first):
     exp = S.typed_expected(dtype, values, asc, nulls_first)
     assert [None if g is None else float(g) for g in got] == \
         [None if e is None else float(e) for e in exp]
+
+
+def _oracle_final(blob, cols):
+    return run_oracle_groupby(blob, [kds.build_arrow_kds(cols)])
+
+
+@pytest.mark.parametrize("func", sorted(S.WINDOW_FUNCS))
+@pytest.mark.parametrize("limit", [2, 4])
+def test_oracle_window_rank_matches_sql(func, limit):
+    """rank() < limit over (PARTITION BY count ORDER BY sum DESC): the
+    oracle's kept rows, in order, equal SQL's definitions applied to the
+    groups; ties in sum make the three functions keep different rows."""
+    cols = S.window_table(20000, 3000)
+    groups = W.decode_groups(_oracle_final(S.sorted_query(cols, [S.key_vref()]), cols),
+                             S.WINDOW_ACTIONS)
+    fin = _oracle_final(S.window_query(cols, func, limit), cols)
+    got = S.group_keys_in_order(fin)
+    assert got == S.window_expected(groups, func, limit)
+    assert 0 < len(got) < len(groups)
+    # packed as kern_windowrank_finalize packs: row ids are the new positions
+    assert [rid for _, rid in kds.parse_row_kds(fin)] == list(range(len(got)))
+
+
+def test_window_rank_functions_differ():
+    """The test table separates the three functions (ties in sum exist)."""
+    cols = S.window_table(20000, 3000)
+    groups = W.decode_groups(_oracle_final(S.sorted_query(cols, [S.key_vref()]), cols),
+                             S.WINDOW_ACTIONS)
+    kept = {f: S.window_expected(groups, f, 3) for f in S.WINDOW_FUNCS}
+    assert len(kept["row_number"]) < len(kept["rank"]) < len(kept["dense_rank"])
+
+
+def test_oracle_window_rank_known():
+    """tiny_table: partition count 1 = {5}; count 2 by sum desc = 1 (10),
+    2 (8), NULL (3), 3 (-10); rank() < 3 keeps 5, 1, 2."""
+    cols = S.tiny_table()
+    for func in S.WINDOW_FUNCS:
+        fin = _oracle_final(S.window_query(cols, func, 3), cols)
+        assert S.group_keys_in_order(fin) == [5, 1, 2]
