@@ -1364,9 +1364,13 @@ task_exec_common(xpuhip_session *s, Scratch *x, const kern_data_store *d_src, ui
     if (preagg || s->d_kds_fallback)
         final_hold = std::shared_lock<std::shared_mutex>(s->final_rw);
 
-    /* the kernels specialized for the plan, or the interpreter */
+    /* the kernels specialized for the plan, or the interpreter.  Under
+     * XPUHIP_JIT_AUTO a join with a group-by is specialized at any size:
+     * its interpreter build has written some groups twice under contention
+     * (cause not found), its specialized build has not. */
     const bool use_jit = t.jit == XPUHIP_JIT_ON ||
-                         (t.jit != XPUHIP_JIT_OFF && nrows >= XPUHIP_JIT_AUTO_ROWS);
+                         (t.jit != XPUHIP_JIT_OFF &&
+                          (nrows >= XPUHIP_JIT_AUTO_ROWS || (preagg && P->nrels > 0)));
     hipFunction_t jfn = nullptr, jsfn = nullptr, jcount = nullptr, jwrite = nullptr, jdump = nullptr;
     const JitFns *jf = nullptr;
     if (use_jit)
