@@ -8,7 +8,7 @@ import pytest
 
 from pgstrom import abi, codegen as cg, datagen, kds, workloads as W, xpuhip
 import oracle
-from helpers import (assert_groups_equal, project_all_query, run_gpu_groupby,
+from helpers import (skip_interpreter_join_groupby, assert_groups_equal, project_all_query, run_gpu_groupby,
                      run_gpu_projection, run_oracle_groupby, run_oracle_projection)
 
 pytestmark = pytest.mark.gpu
@@ -76,6 +76,7 @@ def test_block_groupby(gpu, jit):
 
 def test_block_star_join(gpu, jit):
     """C2's star join with the fact table on heap pages."""
+    skip_interpreter_join_groupby(jit)
     fact, dims = datagen.star_tables(100_000, dim_sizes=(20_000, 2_000, 2_556))
     src = kds.build_block_kds(fact)
     km = W.star_inner(dims)

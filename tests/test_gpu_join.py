@@ -5,7 +5,7 @@ import pytest
 
 from pgstrom import abi, codegen as cg, datagen, hashing, kds, workloads as W, xpuhip
 import oracle
-from helpers import (assert_groups_equal, run_gpu_groupby, run_gpu_projection,
+from helpers import (skip_interpreter_join_groupby, assert_groups_equal, run_gpu_groupby, run_gpu_projection,
                      run_oracle_groupby, run_oracle_projection)
 
 pytestmark = pytest.mark.gpu
@@ -23,6 +23,7 @@ PASSES_IDS = ["passes", "fused"]
 @pytest.mark.parametrize("join_filter", FILTERS, ids=FILTER_IDS)
 @pytest.mark.parametrize("left_outer", [False, True])
 def test_c2_star_join_preagg(gpu, left_outer, jit, join_filter, join_passes):
+    skip_interpreter_join_groupby(jit)
     fact, dims = datagen.star_tables(300_000, dim_sizes=(20_000, 2_000, 2_556))
     src = kds.build_arrow_kds(fact)
     km = W.star_inner(dims)
@@ -39,6 +40,7 @@ def test_c2_star_join_stats(gpu, jit, join_filter, join_passes):
     """Rows in, rows out of every depth and groups out: the same counts as
     the oracle's with and without the inner-relation filters (the filters
     skip bucket walks that cannot match and change nothing else)."""
+    skip_interpreter_join_groupby(jit)
     fact, dims = datagen.star_tables(200_000, dim_sizes=(20_000, 2_000, 2_556))
     src = kds.build_arrow_kds(fact)
     km = W.star_inner(dims)
@@ -66,6 +68,7 @@ def test_join_groupby_multi_match(gpu, jit, join_filter, join_passes):
     """Group-by over a join whose inner relation holds duplicate keys, with
     scan quals rejecting a third of the rows: every (row, match) pair is
     aggregated once."""
+    skip_interpreter_join_groupby(jit)
     r = datagen.rng(11)
     n = 150_000
     outer = [kds.Column("k", "int4", r.integers(0, 5000, size=n, dtype=np.int32)),

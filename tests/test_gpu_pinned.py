@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from pgstrom import abi, datagen, hashing, kds, workloads as W, xpuhip
-from helpers import assert_groups_equal, run_gpu_groupby, run_oracle_groupby, run_oracle_projection
+from helpers import skip_interpreter_join_groupby, assert_groups_equal, run_gpu_groupby, run_oracle_groupby, run_oracle_projection
 
 pytestmark = pytest.mark.gpu
 
@@ -72,10 +72,7 @@ def test_pinned_buffer_contents(gpu, jit):
 def test_star_join_over_pinned_inners(gpu, jit, left_outer):
     """C2's star join with its three dimensions built on the GPU: the same
     groups as with the host-built inner tables, and as the oracle's."""
-    if jit == xpuhip.JIT_OFF:
-        pytest.skip("known wrong result, cause not found: the interpreter build writes one "
-                    "group of this star join twice in about half of its launches (the "
-                    "plan-specialised build in none of 15)")
+    skip_interpreter_join_groupby(jit)
     fact, dims = datagen.star_tables(300_000, dim_sizes=DIM_SIZES)
     src = kds.build_arrow_kds(fact)
     blob = W.star_query(fact, left_outer=left_outer)
